@@ -1,0 +1,251 @@
+"""artemis_amd — MI355X-native ray-traced rendering path of the Kobra renderer.
+
+Python mirror of the reference's ArmadaRTX / AttachmentRTX / Accelerator API
+(include/amadeus/armada.cuh:107-336, include/amadeus/accelerator.cuh:133-277)
+over the C ABI of include/krt.h (libkrt.so, HIP for gfx950).  Every method is a
+thin call into the library; there is no CPU fallback: a missing or failing
+library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .camera import CameraFrame, look_at
+from .scenes import SceneData
+
+__all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "CameraFrame", "look_at", "SceneData",
+           "load_attachment"]
+
+
+def _f(a):
+    return a.ctypes.data_as(L.fp)
+
+
+class Context:
+    """krt_create / krt_destroy: one HIP device, one stream."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        L.check(self._lib.krt_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._lib.krt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True) -> "Scene":
+        return Scene(self, scene, leaf_max=leaf_max, build=build)
+
+
+class Scene:
+    """A resident scene and its acceleration structure (the "System" layer)."""
+
+    def __init__(self, ctx: Context, data: SceneData, leaf_max: int = 4, build: bool = True):
+        self.ctx = ctx
+        self.data = data
+        lib = ctx._lib
+        self._keep = [np.ascontiguousarray(data.positions, np.float32),
+                      np.ascontiguousarray(data.normals, np.float32),
+                      np.ascontiguousarray(data.indices, np.uint32),
+                      np.ascontiguousarray(data.tri_material, np.uint32),
+                      np.ascontiguousarray(data.materials, np.float32)]
+        p, n, i, m, mats = self._keep
+        d = L.SceneDesc(_f(p), _f(n), len(p), i.ctypes.data_as(L.u32p), m.ctypes.data_as(L.u32p), len(i),
+                        _f(mats), len(mats))
+        h = L.handle()
+        L.check(lib.krt_scene_upload(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h.value
+        if build:
+            self.build(leaf_max)
+
+    def build(self, leaf_max: int = 4) -> None:
+        """Accelerator::update + build_tlas: GPU build of the bvh.cpp tree."""
+        L.check(self.ctx._lib.krt_accel_build(self.ctx.h, self.h, leaf_max))
+
+    def stats(self) -> dict:
+        s = L.BvhStats()
+        L.check(self.ctx._lib.krt_accel_stats(self.ctx.h, self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.BvhStats._fields_}
+
+    def serialized(self) -> np.ndarray:
+        st = self.stats()
+        out = np.empty((st["serialized_nodes"], 12), np.float32)
+        L.check(self.ctx._lib.krt_accel_serialized(self.ctx.h, self.h, _f(out), out.size))
+        return out
+
+    def leaf_order(self) -> np.ndarray:
+        out = np.empty(len(self.data.indices), np.int32)
+        L.check(self.ctx._lib.krt_accel_leaf_order(self.ctx.h, self.h, out.ctypes.data_as(L.i32p), out.size))
+        return out
+
+    def lights(self) -> np.ndarray:
+        n = C.c_uint32()
+        L.check(self.ctx._lib.krt_scene_light_count(self.ctx.h, self.h, C.byref(n)))
+        out = np.empty((n.value, 12), np.float32)
+        if n.value:
+            L.check(self.ctx._lib.krt_scene_lights(self.ctx.h, self.h, _f(out), out.size))
+        return out
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.empty((len(rays), 4), np.float32)
+        L.check(self.ctx._lib.krt_trace_closest(self.ctx.h, self.h, _f(rays), len(rays), _f(hits)))
+        return hits
+
+    def trace_occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty(len(rays), np.int32)
+        L.check(self.ctx._lib.krt_trace_occluded(self.ctx.h, self.h, _f(rays), len(rays),
+                                                 out.ctypes.data_as(L.i32p)))
+        return out
+
+    def free(self):
+        if self.h:
+            L.check(self.ctx._lib.krt_scene_free(self.ctx.h, self.h))
+            self.h = 0
+
+
+class _Attachment:
+    kind = ""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(ctx._lib.krt_attachment_create(ctx.h, self.kind.encode(), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx._lib.krt_attachment_destroy(self.h)
+        except Exception:
+            pass
+
+
+class PathTracer(_Attachment):
+    """include/amadeus/path_tracer.cuh:17-167"""
+    kind = "path_tracer"
+
+
+class ReSTIR(_Attachment):
+    """include/amadeus/restir.cuh:35-305"""
+    kind = "restir"
+
+
+def load_attachment(ctx: Context, kind: str):
+    """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
+    return {"path_tracer": PathTracer, "restir": ReSTIR}[kind](ctx)
+
+
+class ArmadaRTX:
+    """include/amadeus/armada.cuh:142-336 over krt_armada_*."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        h = C.c_void_p()
+        L.check(ctx._lib.krt_armada_create(ctx.h, width, height, C.byref(h)))
+        self.h = h
+        self._attached = {}
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx._lib.krt_armada_destroy(self.h)
+        except Exception:
+            pass
+
+    def size(self) -> int:
+        return self.width * self.height
+
+    def extent(self):
+        return (self.width, self.height)
+
+    def attach(self, name: str, attachment: _Attachment) -> None:
+        L.check(self.ctx._lib.krt_armada_attach(self.h, name.encode(), attachment.h))
+        self._attached[name] = attachment
+
+    def attachments(self):
+        return sorted(self._attached)
+
+    def activate(self, name: str) -> None:
+        L.check(self.ctx._lib.krt_armada_activate(self.h, name.encode()))
+
+    def set_option(self, field: str, value) -> None:
+        o = L.Option()
+        if isinstance(value, bool):
+            o.kind, o.b = L.OPT_BOOL, int(value)
+        elif isinstance(value, int):
+            o.kind, o.i = L.OPT_INT, value
+        elif isinstance(value, float):
+            o.kind, o.f = L.OPT_FLOAT, value
+        else:
+            o.kind, o.s = L.OPT_STRING, str(value).encode()
+        L.check(self.ctx._lib.krt_armada_set_option(self.h, field.encode(), C.byref(o)))
+
+    def get_option(self, field: str):
+        o = L.Option()
+        L.check(self.ctx._lib.krt_armada_get_option(self.h, field.encode(), C.byref(o)))
+        return {L.OPT_INT: o.i, L.OPT_FLOAT: o.f, L.OPT_BOOL: bool(o.b)}.get(o.kind)
+
+    def set_depth(self, depth: int) -> None:
+        L.check(self.ctx._lib.krt_armada_set_depth(self.h, depth))
+
+    def render(self, scene: Scene, camera: CameraFrame, accumulate: bool = True) -> None:
+        cam = camera.to_ctypes()
+        L.check(self.ctx._lib.krt_armada_render(self.h, scene.h, C.byref(cam), int(accumulate)))
+
+    def _read(self, which: int) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.float32)
+        L.check(self.ctx._lib.krt_armada_readback(self.h, which, _f(out)))
+        return out
+
+    def color_buffer(self):
+        return self._read(0)
+
+    def normal_buffer(self):
+        return self._read(1)
+
+    def albedo_buffer(self):
+        return self._read(2)
+
+    def position_buffer(self):
+        return self._read(3)
+
+    def upload_buffer(self, which: int, data: np.ndarray) -> None:
+        data = np.ascontiguousarray(data, np.float32)
+        assert data.size == self.width * self.height * 4
+        L.check(self.ctx._lib.krt_armada_upload(self.h, which, _f(data)))
+
+    def device_buffers(self) -> dict:
+        b = L.Buffers()
+        L.check(self.ctx._lib.krt_armada_buffers(self.h, C.byref(b)))
+        return {"color": b.color, "normal": b.normal, "albedo": b.albedo, "position": b.position}
+
+    def stats(self) -> dict:
+        s = L.FrameStats()
+        L.check(self.ctx._lib.krt_armada_stats(self.h, C.byref(s)))
+        return {"closest_rays": s.closest_rays, "shadow_rays": s.shadow_rays, "samples": s.samples,
+                "gpu_ms": s.gpu_ms, "kernel_ms": list(s.kernel_ms), "pass_rays": list(s.pass_rays),
+                "node_visits": s.node_visits, "tri_tests": s.tri_tests}
+
+    def sync(self) -> None:
+        L.check(self.ctx._lib.krt_armada_sync(self.h))

@@ -1,0 +1,123 @@
+"""ctypes declarations for libkrt.so (include/krt.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libkrt.so"
+
+KRT_OK = 0
+KRT_ERR_INVALID = 1
+KRT_ERR_HIP = 2
+KRT_ERR_BUDGET = 3
+KRT_ERR_STATE = 4
+
+OPT_NONE, OPT_INT, OPT_FLOAT, OPT_BOOL, OPT_STRING = 0, 1, 2, 3, 4
+
+fp = C.POINTER(C.c_float)
+u32p = C.POINTER(C.c_uint32)
+i32p = C.POINTER(C.c_int32)
+vp = C.c_void_p
+handle = C.c_uint64
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("positions", fp), ("normals", fp), ("nv", C.c_uint32),
+        ("indices", u32p), ("tri_material", u32p), ("nt", C.c_uint32),
+        ("materials", fp), ("nm", C.c_uint32),
+    ]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_uint32), ("serialized_nodes", C.c_uint32), ("traversal_nodes", C.c_uint32),
+        ("depth_bound", C.c_uint32), ("build_levels", C.c_uint32), ("build_ms", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3), ("W", C.c_float * 3),
+                ("pv", C.c_float * 16)]
+
+
+class Option(C.Structure):
+    _fields_ = [("kind", C.c_int), ("i", C.c_int), ("f", C.c_float), ("b", C.c_int), ("s", C.c_char_p)]
+
+
+class Buffers(C.Structure):
+    _fields_ = [("color", vp), ("normal", vp), ("albedo", vp), ("position", vp)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
+                ("gpu_ms", C.c_float), ("kernel_ms", C.c_float * 8), ("pass_rays", C.c_uint64 * 4),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
+
+
+class AttachmentRet(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("ptr", vp)]
+
+
+_SIGS = {
+    "krt_last_error": (C.c_char_p, []),
+    "krt_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "krt_destroy": (None, [vp]),
+    "krt_scene_upload": (C.c_int, [vp, C.POINTER(SceneDesc), C.POINTER(handle)]),
+    "krt_scene_free": (C.c_int, [vp, handle]),
+    "krt_scene_light_count": (C.c_int, [vp, handle, u32p]),
+    "krt_scene_lights": (C.c_int, [vp, handle, fp, C.c_size_t]),
+    "krt_accel_build": (C.c_int, [vp, handle, C.c_int]),
+    "krt_accel_stats": (C.c_int, [vp, handle, C.POINTER(BvhStats)]),
+    "krt_accel_serialized": (C.c_int, [vp, handle, fp, C.c_size_t]),
+    "krt_accel_leaf_order": (C.c_int, [vp, handle, i32p, C.c_size_t]),
+    "krt_trace_closest": (C.c_int, [vp, handle, fp, C.c_uint32, fp]),
+    "krt_trace_occluded": (C.c_int, [vp, handle, fp, C.c_uint32, i32p]),
+    "krt_trace_closest_device": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
+    "krt_armada_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+    "krt_armada_destroy": (None, [vp]),
+    "krt_attachment_create": (C.c_int, [vp, C.c_char_p, C.POINTER(vp)]),
+    "krt_attachment_destroy": (None, [vp]),
+    "krt_armada_attach": (C.c_int, [vp, C.c_char_p, vp]),
+    "krt_armada_activate": (C.c_int, [vp, C.c_char_p]),
+    "krt_armada_set_option": (C.c_int, [vp, C.c_char_p, C.POINTER(Option)]),
+    "krt_armada_get_option": (C.c_int, [vp, C.c_char_p, C.POINTER(Option)]),
+    "krt_armada_set_depth": (C.c_int, [vp, C.c_int]),
+    "krt_armada_render": (C.c_int, [vp, handle, C.POINTER(Camera), C.c_int]),
+    "krt_armada_buffers": (C.c_int, [vp, C.POINTER(Buffers)]),
+    "krt_armada_readback": (C.c_int, [vp, C.c_int, fp]),
+    "krt_armada_upload": (C.c_int, [vp, C.c_int, fp]),
+    "krt_armada_stats": (C.c_int, [vp, C.POINTER(FrameStats)]),
+    "krt_armada_sync": (C.c_int, [vp]),
+    "krt_load_attachment": (AttachmentRet, [vp, C.c_char_p]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class KrtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"krt error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libkrt.so; raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m artemis_amd.build` first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != KRT_OK:
+        raise KrtError(code, lib().krt_last_error().decode())

@@ -1,0 +1,558 @@
+// C ABI of include/krt.h and the ArmadaRTX / AttachmentRTX state behind it.
+//
+// ArmadaRTX::render (source/amadeus/armada.cu:733-780): switch attachment
+// (unload previous, load new), reset the sample count when not accumulating or
+// when the acceleration structure changed, render, samples++.
+// Attachment options follow AttachmentRTX::set_option (armada.cuh:122-126):
+// unknown names are ignored, get_option of an unknown name is empty.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "krt_internal.h"
+
+using namespace krt;
+
+namespace krt {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+SceneView make_view(const Scene &s) {
+    return SceneView{s.pos.as<float>(),   s.nrm.as<float>(),    s.idx.as<uint32_t>(),
+                     s.tri_mat.as<uint32_t>(), s.mats.as<float4>(), s.lights.as<float4>(),
+                     s.nl,                s.accel.tri.as<float4>(), s.accel.nodes.as<float4>()};
+}
+
+}  // namespace krt
+
+// ---------------------------------------------------------------------------
+// Attachment and armada state
+// ---------------------------------------------------------------------------
+struct krt_attachment {
+    Context *ctx = nullptr;
+    std::string kind;
+    bool loaded = false;
+    uint32_t user_seed = 0;
+    RestirOptions ropt;
+    RestirState rst;
+    DevBuf history;
+    uint32_t w = 0, h = 0;
+
+    void load(uint32_t width, uint32_t height) {  // restir.cuh:203-211
+        w = width;
+        h = height;
+        if (kind == "restir") restir_alloc(*ctx, rst, width, height, user_seed);
+        loaded = true;
+    }
+    void unload() {  // restir.cuh:213-220
+        rst = RestirState();
+        history.release();
+        loaded = false;
+    }
+    bool set_option(const std::string &f, const krt_option &v) {
+        auto as_int = [&](int &dst) {
+            if (v.kind == KRT_OPT_INT) dst = v.i;
+            else if (v.kind == KRT_OPT_FLOAT) dst = (int)v.f;
+            else if (v.kind == KRT_OPT_BOOL) dst = v.b ? 1 : 0;
+            else return false;
+            return true;
+        };
+        int tmp;
+        if (f == "seed") {
+            if (!as_int(tmp)) return false;
+            user_seed = (uint32_t)tmp;
+            return true;
+        }
+        if (kind != "restir") return true;  // unknown option names are ignored
+        if (f == "candidates") return as_int(ropt.candidates);
+        if (f == "m_cap") return as_int(ropt.m_cap);
+        if (f == "spatial_passes") return as_int(ropt.spatial_passes);
+        if (f == "neighbours") return as_int(ropt.neighbours);
+        if (f == "radius") return as_int(ropt.radius);
+        return true;
+    }
+    krt_option get_option(const std::string &f) const {
+        krt_option o{};
+        o.kind = KRT_OPT_INT;
+        if (f == "seed") o.i = (int)user_seed;
+        else if (kind == "restir" && f == "candidates") o.i = ropt.candidates;
+        else if (kind == "restir" && f == "m_cap") o.i = ropt.m_cap;
+        else if (kind == "restir" && f == "spatial_passes") o.i = ropt.spatial_passes;
+        else if (kind == "restir" && f == "neighbours") o.i = ropt.neighbours;
+        else if (kind == "restir" && f == "radius") o.i = ropt.radius;
+        else o.kind = KRT_OPT_NONE;
+        return o;
+    }
+};
+
+struct krt_armada {
+    Context *ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    DevBuf color, normal, albedo, position;
+    std::map<std::string, krt_attachment *> attachments;
+    std::string active, previous;
+    int max_depth = 2;  // armada.cu:56
+    int samples = 0;
+    krt_handle last_scene = 0;
+    uint64_t last_version = 0;
+    DevBuf ctr;  // unsigned long long x 8: closest, shadow x3, nodes, tris
+    krt_frame_stats stats{};
+};
+
+struct krt_context {
+    Context impl;
+};
+
+// ---------------------------------------------------------------------------
+// Exception boundary
+// ---------------------------------------------------------------------------
+template <class F>
+static int guard(F &&f) {
+    try {
+        f();
+        set_last_error("");
+        return KRT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return KRT_ERR_BUDGET;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return KRT_ERR_INVALID;
+    }
+}
+
+extern "C" {
+
+const char *krt_last_error(void) { return g_last_error.c_str(); }
+
+int krt_create(int device, krt_context **out) {
+    return guard([&] {
+        KRT_REQUIRE(out != nullptr, KRT_ERR_INVALID, "out is null");
+        int n = 0;
+        KRT_HIP(hipGetDeviceCount(&n));
+        KRT_REQUIRE(device >= 0 && device < n, KRT_ERR_INVALID, "no such HIP device");
+        KRT_HIP(hipSetDevice(device));
+        auto *c = new krt_context();
+        c->impl.device = device;
+        KRT_HIP(hipStreamCreate(&c->impl.stream));
+        c->impl.counters.alloc(sizeof(unsigned long long) * 16);
+        *out = c;
+    });
+}
+
+void krt_destroy(krt_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->impl.device);
+    (void)hipStreamSynchronize(ctx->impl.stream);
+    ctx->impl.scenes.clear();
+    ctx->impl.counters.release();
+    ctx->impl.spill.release();
+    (void)hipStreamDestroy(ctx->impl.stream);
+    delete ctx;
+}
+
+static void upload(Context &C, DevBuf &b, const void *src, size_t bytes) {
+    b.alloc(bytes);
+    if (bytes) KRT_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, C.stream));
+}
+
+int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scene) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && d && scene, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(d->nt > 0 && d->nv > 0 && d->nm > 0, KRT_ERR_INVALID, "empty scene");
+        KRT_REQUIRE(d->positions && d->normals && d->indices && d->tri_material && d->materials, KRT_ERR_INVALID,
+                    "null scene array");
+        Context &C = ctx->impl;
+        KRT_HIP(hipSetDevice(C.device));
+        for (uint32_t i = 0; i < 3 * d->nt; i++)
+            KRT_REQUIRE(d->indices[i] < d->nv, KRT_ERR_INVALID, "vertex index out of range");
+        for (uint32_t t = 0; t < d->nt; t++)
+            KRT_REQUIRE(d->tri_material[t] < d->nm, KRT_ERR_INVALID, "material index out of range");
+        auto s = std::make_unique<Scene>();
+        s->nv = d->nv;
+        s->nt = d->nt;
+        s->nm = d->nm;
+        upload(C, s->pos, d->positions, sizeof(float) * 3 * d->nv);
+        upload(C, s->nrm, d->normals, sizeof(float) * 3 * d->nv);
+        upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * d->nt);
+        upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * d->nt);
+        // material rows: {diffuse, rough}, {specular, refr}, {emission, type}, {0}
+        std::vector<float> mt(16 * (size_t)d->nm, 0.0f);
+        for (uint32_t m = 0; m < d->nm; m++) {
+            const float *q = d->materials + 12 * m;
+            float *o = &mt[16 * m];
+            o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[9];
+            o[4] = q[3]; o[5] = q[4]; o[6] = q[5]; o[7] = q[10];
+            o[8] = q[6]; o[9] = q[7]; o[10] = q[8];
+            int type = (int)q[11];
+            std::memcpy(&o[11], &type, 4);
+        }
+        upload(C, s->mats, mt.data(), sizeof(float) * mt.size());
+        // triangle lights (armada.cu:117-140, emissive test :240), area and
+        // normal precomputed with the per-sample expressions of lighting.cuh:102-110
+        std::vector<float> lt;
+        for (uint32_t t = 0; t < d->nt; t++) {
+            const float *e = d->materials + 12 * d->tri_material[t] + 6;
+            if (!(std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.0f)) continue;
+            const float *a = d->positions + 3 * d->indices[3 * t];
+            const float *b = d->positions + 3 * d->indices[3 * t + 1];
+            const float *c = d->positions + 3 * d->indices[3 * t + 2];
+            float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+            float dd = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+            float area = std::sqrt(dd) * 0.5f;
+            float inv = 1.0f / std::sqrt(dd);
+            float nrm[3] = {cr[0] * inv, cr[1] * inv, cr[2] * inv};
+            float row[16] = {a[0], a[1], a[2], area, ab[0], ab[1], ab[2], nrm[0],
+                             ac[0], ac[1], ac[2], nrm[1], e[0], e[1], e[2], nrm[2]};
+            lt.insert(lt.end(), row, row + 16);
+        }
+        s->nl = (uint32_t)(lt.size() / 16);
+        upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
+        KRT_HIP(hipStreamSynchronize(C.stream));
+        krt_handle h = C.next_handle++;
+        C.scenes[h] = std::move(s);
+        *scene = h;
+    });
+}
+
+int krt_scene_free(krt_context *ctx, krt_handle scene) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
+        KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+        KRT_REQUIRE(ctx->impl.scenes.erase(scene) == 1, KRT_ERR_INVALID, "unknown scene handle");
+    });
+}
+
+int krt_scene_light_count(krt_context *ctx, krt_handle scene, uint32_t *out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        *out = ctx->impl.scene(scene).nl;
+    });
+}
+
+int krt_scene_lights(krt_context *ctx, krt_handle scene, float *out, size_t nfloats) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(nfloats >= 12 * (size_t)s.nl, KRT_ERR_INVALID, "output too small");
+        std::vector<float> rows(16 * (size_t)s.nl);
+        if (s.nl)
+            KRT_HIP(hipMemcpy(rows.data(), s.lights.p, sizeof(float) * rows.size(), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < s.nl; i++) {
+            const float *r = &rows[16 * i];
+            float v[12] = {r[0], r[1], r[2], r[4], r[5], r[6], r[8], r[9], r[10], r[12], r[13], r[14]};
+            std::memcpy(out + 12 * i, v, sizeof(v));
+        }
+    });
+}
+
+int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        accel_build(ctx->impl, ctx->impl.scene(scene), leaf_max);
+    });
+}
+
+int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        const Accel &A = ctx->impl.scene(scene).accel;
+        KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
+        out->triangles = A.nt;
+        out->serialized_nodes = A.nser;
+        out->traversal_nodes = A.nnodes;
+        out->depth_bound = A.depth_bound;
+        out->build_levels = A.levels;
+        out->build_ms = A.build_ms;
+    });
+}
+
+int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t nfloats) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        const Accel &A = ctx->impl.scene(scene).accel;
+        KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
+        KRT_REQUIRE(nfloats >= 12 * (size_t)A.nser, KRT_ERR_INVALID, "output too small");
+        KRT_HIP(hipMemcpy(out, A.ser.p, sizeof(float) * 12 * (size_t)A.nser, hipMemcpyDeviceToHost));
+    });
+}
+
+int krt_accel_leaf_order(krt_context *ctx, krt_handle scene, int32_t *out, size_t n) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        const Accel &A = ctx->impl.scene(scene).accel;
+        KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
+        KRT_REQUIRE(n >= A.nt, KRT_ERR_INVALID, "output too small");
+        KRT_HIP(hipMemcpy(out, A.leaf_order.p, sizeof(int32_t) * A.nt, hipMemcpyDeviceToHost));
+    });
+}
+
+int krt_trace_closest(krt_context *ctx, krt_handle scene, const float *rays, uint32_t n, float *hits) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && rays && hits, KRT_ERR_INVALID, "null argument");
+        Context &C = ctx->impl;
+        Scene &s = C.scene(scene);
+        DevBuf r(sizeof(float) * 8 * (size_t)n), h(sizeof(float) * 4 * (size_t)n);
+        KRT_HIP(hipMemcpyAsync(r.p, rays, r.bytes, hipMemcpyHostToDevice, C.stream));
+        trace_closest_dev(C, s, r.as<float>(), n, h.as<float>());
+        KRT_HIP(hipMemcpyAsync(hits, h.p, sizeof(float) * 4 * (size_t)n, hipMemcpyDeviceToHost, C.stream));
+        KRT_HIP(hipStreamSynchronize(C.stream));
+    });
+}
+
+int krt_trace_occluded(krt_context *ctx, krt_handle scene, const float *rays, uint32_t n, int32_t *occ) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && rays && occ, KRT_ERR_INVALID, "null argument");
+        Context &C = ctx->impl;
+        Scene &s = C.scene(scene);
+        DevBuf r(sizeof(float) * 8 * (size_t)n), o(sizeof(int32_t) * (size_t)n);
+        KRT_HIP(hipMemcpyAsync(r.p, rays, r.bytes, hipMemcpyHostToDevice, C.stream));
+        trace_occluded_dev(C, s, r.as<float>(), n, o.as<int32_t>());
+        KRT_HIP(hipMemcpyAsync(occ, o.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, C.stream));
+        KRT_HIP(hipStreamSynchronize(C.stream));
+    });
+}
+
+int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n, void *hits_dev) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && rays_dev && hits_dev, KRT_ERR_INVALID, "null argument");
+        trace_closest_dev(ctx->impl, ctx->impl.scene(scene), (const float *)rays_dev, n, (float *)hits_dev);
+    });
+}
+
+// ------------------------------------------------------------------- armada
+int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(width > 0 && height > 0, KRT_ERR_INVALID, "zero extent");
+        KRT_REQUIRE((uint64_t)width * height < (1ull << 31), KRT_ERR_INVALID, "extent too large");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        auto *a = new krt_armada();
+        a->ctx = &ctx->impl;
+        a->w = width;
+        a->h = height;
+        size_t bytes = sizeof(float) * 4 * (size_t)width * height;
+        try {
+            a->color.alloc(bytes);
+            a->normal.alloc(bytes);
+            a->albedo.alloc(bytes);
+            a->position.alloc(bytes);
+            a->ctr.alloc(sizeof(unsigned long long) * 8);
+            for (DevBuf *b : {&a->color, &a->normal, &a->albedo, &a->position})
+                KRT_HIP(hipMemsetAsync(b->p, 0, bytes, ctx->impl.stream));
+            KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+        } catch (...) {
+            delete a;
+            throw;
+        }
+        *out = a;
+    });
+}
+
+void krt_armada_destroy(krt_armada *a) {
+    if (!a) return;
+    (void)hipStreamSynchronize(a->ctx->stream);
+    for (auto &kv : a->attachments)
+        if (kv.second && kv.second->loaded) kv.second->unload();
+    delete a;
+}
+
+int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && kind && out, KRT_ERR_INVALID, "null argument");
+        std::string k(kind);
+        KRT_REQUIRE(k == "path_tracer" || k == "restir", KRT_ERR_INVALID, "unknown attachment kind: " + k);
+        auto *a = new krt_attachment();
+        a->ctx = &ctx->impl;
+        a->kind = k;
+        *out = a;
+    });
+}
+
+void krt_attachment_destroy(krt_attachment *att) {
+    if (!att) return;
+    if (att->loaded) {
+        (void)hipStreamSynchronize(att->ctx->stream);
+        att->unload();
+    }
+    delete att;
+}
+
+int krt_armada_attach(krt_armada *a, const char *name, krt_attachment *att) {
+    return guard([&] {
+        KRT_REQUIRE(a && name && att, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(att->ctx == a->ctx, KRT_ERR_INVALID, "attachment belongs to another context");
+        a->attachments[name] = att;  // armada.cuh:304-311: attach makes it active
+        a->active = name;
+    });
+}
+
+int krt_armada_activate(krt_armada *a, const char *name) {
+    return guard([&] {
+        KRT_REQUIRE(a && name, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(a->attachments.count(name), KRT_ERR_INVALID, std::string("no attachment named ") + name);
+        a->active = name;
+    });
+}
+
+int krt_armada_set_option(krt_armada *a, const char *field, const krt_option *value) {
+    return guard([&] {
+        KRT_REQUIRE(a && field && value, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(!a->active.empty(), KRT_ERR_STATE, "no active attachment");
+        KRT_REQUIRE(a->attachments.at(a->active)->set_option(field, *value), KRT_ERR_INVALID,
+                    std::string("bad value type for option ") + field);
+    });
+}
+
+int krt_armada_get_option(krt_armada *a, const char *field, krt_option *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && field && out, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(!a->active.empty(), KRT_ERR_STATE, "no active attachment");
+        *out = a->attachments.at(a->active)->get_option(field);
+    });
+}
+
+int krt_armada_set_depth(krt_armada *a, int depth) {
+    return guard([&] {
+        KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(depth >= 0 && depth <= 64, KRT_ERR_INVALID, "depth must be in [0, 64]");
+        a->max_depth = depth;
+    });
+}
+
+int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, int accumulate) {
+    return guard([&] {
+        KRT_REQUIRE(a && cam, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(!a->active.empty(), KRT_ERR_STATE, "no active attachment");
+        for (int k = 0; k < 3; k++)
+            KRT_REQUIRE(std::isfinite(cam->center[k]) && std::isfinite(cam->U[k]) && std::isfinite(cam->V[k]) &&
+                            std::isfinite(cam->W[k]),
+                        KRT_ERR_INVALID, "camera has non-finite components");
+        Context &C = *a->ctx;
+        KRT_HIP(hipSetDevice(C.device));
+        Scene &s = C.scene(scene);
+        KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+        // attachment switch (armada.cu:747-753)
+        if (a->active != a->previous) {
+            if (!a->previous.empty() && a->attachments.count(a->previous)) {
+                KRT_HIP(hipStreamSynchronize(C.stream));
+                a->attachments[a->previous]->unload();
+            }
+            a->previous = a->active;
+        }
+        krt_attachment *att = a->attachments.at(a->previous);
+        if (!att->loaded || att->w != a->w || att->h != a->h) att->load(a->w, a->h);
+        // armada.cu:771-772: restart accumulation on a new acceleration structure
+        if (!accumulate || scene != a->last_scene || s.version != a->last_version) a->samples = 0;
+        a->last_scene = scene;
+        a->last_version = s.version;
+        LaunchInfo li{};
+        li.width = a->w;
+        li.height = a->h;
+        std::memcpy(li.center, cam->center, sizeof(li.center));
+        std::memcpy(li.U, cam->U, sizeof(li.U));
+        std::memcpy(li.V, cam->V, sizeof(li.V));
+        std::memcpy(li.W, cam->W, sizeof(li.W));
+        std::memcpy(li.pv, cam->pv, sizeof(li.pv));
+        li.max_depth = a->max_depth;
+        li.samples = a->samples;
+        li.accumulate = 1;  // ArmadaRTX launches always accumulate (armada.cu:58); samples==0 restarts
+        li.user_seed = att->user_seed;
+        li.color = a->color.as<float4>();
+        li.normal = a->normal.as<float4>();
+        li.albedo = a->albedo.as<float4>();
+        li.position = a->position.as<float4>();
+        KRT_HIP(hipMemsetAsync(a->ctr.p, 0, a->ctr.bytes, C.stream));
+        PassTimes times;
+        if (att->kind == "path_tracer") pt_render(C, s, li, att->history, a->ctr.as<unsigned long long>(), times);
+        else restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
+        unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        KRT_HIP(hipMemcpyAsync(c, a->ctr.p, sizeof(c), hipMemcpyDeviceToHost, C.stream));
+        KRT_HIP(hipStreamSynchronize(C.stream));
+        a->stats.closest_rays = c[0];
+        a->stats.shadow_rays = c[1] + c[2] + c[3];
+        a->stats.pass_rays[0] = c[0] + c[1];
+        a->stats.pass_rays[1] = c[2];
+        a->stats.pass_rays[2] = c[3];
+        a->stats.pass_rays[3] = 0;
+        a->stats.node_visits = c[4];
+        a->stats.tri_tests = c[5];
+        a->stats.samples = (uint64_t)a->w * a->h;
+        float tot = 0.0f;
+        for (int k = 0; k < 8; k++) {
+            a->stats.kernel_ms[k] = times.ms[k];
+            tot += times.ms[k];
+        }
+        a->stats.gpu_ms = tot;
+        a->samples++;  // armada.cu:779
+    });
+}
+
+int krt_armada_buffers(krt_armada *a, krt_buffers *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        out->color = a->color.p;
+        out->normal = a->normal.p;
+        out->albedo = a->albedo.p;
+        out->position = a->position.p;
+    });
+}
+
+static DevBuf &aov(krt_armada *a, int which) {
+    switch (which) {
+        case 0: return a->color;
+        case 1: return a->normal;
+        case 2: return a->albedo;
+        case 3: return a->position;
+    }
+    throw Error(KRT_ERR_INVALID, "buffer index must be 0..3");
+}
+
+int krt_armada_readback(krt_armada *a, int which, float *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        DevBuf &b = aov(a, which);
+        KRT_HIP(hipMemcpyAsync(out, b.p, b.bytes, hipMemcpyDeviceToHost, a->ctx->stream));
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+int krt_armada_upload(krt_armada *a, int which, const float *in) {
+    return guard([&] {
+        KRT_REQUIRE(a && in, KRT_ERR_INVALID, "null argument");
+        DevBuf &b = aov(a, which);
+        KRT_HIP(hipMemcpyAsync(b.p, in, b.bytes, hipMemcpyHostToDevice, a->ctx->stream));
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+int krt_armada_stats(krt_armada *a, krt_frame_stats *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        *out = a->stats;
+    });
+}
+
+int krt_armada_sync(krt_armada *a) {
+    return guard([&] {
+        KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+krt_attachment_ret krt_load_attachment(krt_context *ctx, const char *kind) {
+    krt_attachment_ret r{nullptr, nullptr};
+    if (krt_attachment_create(ctx, kind, &r.ptr) == KRT_OK) r.name = r.ptr->kind.c_str();
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+// Device math for the gfx950 kernels: vector helpers, pcg RNG, GGX/Glass BSDF,
+// triangle lights, Möller–Trumbore.  Every expression keeps the evaluation order
+// of the reference (include/cuda/math.cuh, random.cuh, brdf.cuh,
+// include/optix/lighting.cuh); the library is compiled with -ffp-contract=off
+// and IEEE division/sqrt, so results are bit-reproducible on any host that
+// evaluates the same expressions in IEEE single precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KRT_DEV __device__ __forceinline__
+
+namespace krt {
+
+struct f3 {
+    float x, y, z;
+};
+
+KRT_DEV f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+KRT_DEV f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+KRT_DEV f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+KRT_DEV f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+KRT_DEV f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+KRT_DEV f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+KRT_DEV f3 operator*(float s, f3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+KRT_DEV f3 operator/(f3 a, float s) {  // math.cuh:464-468: a * (1/s)
+    float inv = 1.0f / s;
+    return a * inv;
+}
+KRT_DEV f3 operator-(float a, f3 b) { return mk(a - b.x, a - b.y, a - b.z); }
+KRT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+KRT_DEV f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+KRT_DEV float length(f3 v) { return sqrtf(dot(v, v)); }
+KRT_DEV f3 normalize(f3 v) {
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return v * inv;
+}
+KRT_DEV f3 reflect(f3 i, f3 n) { return i - (2.0f * n) * dot(n, i); }
+KRT_DEV float fract(float x) { return x - floorf(x); }
+KRT_DEV float clampf(float x, float a, float b) {
+    x = (x < a) ? a : x;
+    return (x > b) ? b : x;
+}
+KRT_DEV float selmax(float a, float b) { return (a < b) ? b : a; }
+KRT_DEV float selmin(float a, float b) { return (b < a) ? b : a; }
+KRT_DEV bool isnan3(f3 v) { return isnan(v.x) || isnan(v.y) || isnan(v.z); }
+KRT_DEV f3 ld3(const float4 &v) { return mk(v.x, v.y, v.z); }
+KRT_DEV f3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+// glm::min / glm::max semantics
+KRT_DEV float gmin(float x, float y) { return (y < x) ? y : x; }
+KRT_DEV float gmax(float x, float y) { return (x < y) ? y : x; }
+
+// ---------------------------------------------------------------------------
+// RNG (random.cuh:14-58)
+// ---------------------------------------------------------------------------
+KRT_DEV void pcg3d(uint32_t &x, uint32_t &y, uint32_t &z) {
+    x = x * 1664525u + 1013904223u;
+    y = y * 1664525u + 1013904223u;
+    z = z * 1664525u + 1013904223u;
+    x += y * z;
+    y += z * x;
+    z += x * y;
+    x ^= x >> 16u;
+    y ^= y >> 16u;
+    z ^= z >> 16u;
+    x += y * z;
+    y += z * x;
+    z += x * y;
+}
+
+KRT_DEV f3 pcg3f(f3 &seed) {
+    uint32_t x = __float_as_uint(seed.x), y = __float_as_uint(seed.y), z = __float_as_uint(seed.z);
+    pcg3d(x, y, z);
+    x = (x & 0x007fffffu) | 0x3f800000u;
+    y = (y & 0x007fffffu) | 0x3f800000u;
+    z = (z & 0x007fffffu) | 0x3f800000u;
+    seed = mk(__uint_as_float(x) - 1.0f, __uint_as_float(y) - 1.0f, __uint_as_float(z) - 1.0f);
+    return seed;
+}
+
+KRT_DEV float rand_uniform(f3 &seed) {
+    seed = pcg3f(seed);
+    return fract(seed.x);
+}
+
+KRT_DEV f3 rand_uniform_3f(f3 &seed) {
+    seed = pcg3f(seed);
+    return mk(fract(seed.x), fract(seed.y), fract(seed.z));
+}
+
+// Seeds: three u32 words bit-cast into the float3 state.
+KRT_DEV f3 pixel_seed(uint32_t px, uint32_t py, uint32_t sample, uint32_t user) {
+    return mk(__uint_as_float(px), __uint_as_float(py), __uint_as_float(sample + 0x9E3779B9u * user));
+}
+KRT_DEV f3 reservoir_seed(uint32_t px, uint32_t py, uint32_t user) {
+    f3 s = mk(__uint_as_float(px), __uint_as_float(py), __uint_as_float(0x5EED0000u ^ (0x9E3779B9u * user)));
+    pcg3f(s);
+    return s;
+}
+
+// cos(2 pi u), sin(2 pi u), u in [0,1): exact quarter-turn reduction + Taylor
+// polynomials on [-pi/4, pi/4].
+KRT_DEV void sincos_turn(float u, float &c, float &s) {
+    float q = floorf(u * 4.0f + 0.5f);
+    float r = u - q * 0.25f;
+    float x = r * 6.28318530717958647692f;
+    float x2 = x * x;
+    float sp = x * (1.0f + x2 * (-0.166666672f + x2 * (0.00833333377f +
+                  x2 * (-0.000198412701f + x2 * 2.75573188e-06f))));
+    float cp = 1.0f + x2 * (-0.5f + x2 * (0.0416666679f + x2 * (-0.00138888892f +
+                  x2 * (2.48015876e-05f + x2 * -2.75573188e-07f))));
+    int qi = ((int)q) & 3;
+    c = (qi == 0) ? cp : (qi == 1) ? -sp : (qi == 2) ? -cp : sp;
+    s = (qi == 0) ? sp : (qi == 1) ? cp : (qi == 2) ? -sp : -cp;
+}
+
+KRT_DEV f3 rotate(f3 s, f3 n) {  // math.cuh:2653-2667
+    f3 w = n;
+    f3 a = mk(0.0f, 1.0f, 0.0f);
+    if (fabsf(dot(w, a)) > 0.999f) a = mk(0.0f, 0.0f, 1.0f);
+    f3 u = normalize(cross(w, a));
+    f3 v = normalize(cross(w, u));
+    return (u * s.x + v * s.y) + w * s.z;
+}
+
+// ---------------------------------------------------------------------------
+// Materials and BSDF (brdf.cuh)
+// ---------------------------------------------------------------------------
+enum : int { eReflection = 1, eTransmission = 2, eDiffuse = 4 };
+constexpr float PI_F = 3.14159265358979323846f;
+
+struct Material {
+    f3 diffuse, specular, emission;
+    float roughness, refraction;
+    int type;
+};
+
+// Device material table row: 4 x float4 = {diffuse, rough}, {specular, refr},
+// {emission, type-as-int-bits}, {1/PI-scaled diffuse, avg_Kd}.
+KRT_DEV Material load_material(const float4 *tab, uint32_t m) {
+    float4 a = tab[4 * m], b = tab[4 * m + 1], c = tab[4 * m + 2];
+    Material M;
+    M.diffuse = mk(a.x, a.y, a.z);
+    M.roughness = a.w;
+    M.specular = mk(b.x, b.y, b.z);
+    M.refraction = b.w;
+    M.emission = mk(c.x, c.y, c.z);
+    M.type = __float_as_int(c.w);
+    return M;
+}
+
+struct SurfaceHit {
+    Material mat;
+    bool entering;
+    f3 n, wo, x;
+};
+
+KRT_DEV float G1(f3 n, f3 v, const Material &mat) {  // brdf.cuh:180-192
+    if (dot(v, n) <= 0.0f) return 0.0f;
+    float alpha = mat.roughness;
+    float c = clampf(dot(n, v), 0.0f, 0.999f);
+    float c2 = c * c;
+    float tan2 = (1.0f - c2) / c2;
+    float denom = 1.0f + sqrtf(1.0f + alpha * alpha * tan2);
+    return 2.0f / denom;
+}
+
+KRT_DEV f3 shlick_F(f3 wi, f3 h, const Material &mat) {  // brdf.cuh:203-207
+    float x = 1.0f - dot(wi, h);
+    float x2 = x * x;
+    float k = x2 * x2 * x;
+    return mat.specular + (1.0f - mat.specular) * k;
+}
+
+KRT_DEV float ggx_D(f3 n, f3 h, const Material &mat) {  // brdf.cuh:211-219
+    float alpha = mat.roughness;
+    float a2 = alpha * alpha;
+    float c = clampf(dot(n, h), 0.0f, 1.0f);
+    float k = c * c * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI_F * k * k);
+}
+
+KRT_DEV f3 ggx_brdf(const Material &mat, f3 n, f3 wi, f3 wo) {  // brdf.cuh:337-353
+    if (dot(wo, n) <= 0.0f) n = -n;
+    f3 h = normalize(wi + wo);
+    f3 f = shlick_F(wi, h, mat);
+    float g = G1(n, wo, mat) * G1(n, wi, mat);
+    float d = ggx_D(n, h, mat);
+    f3 num = f * g * d;
+    float denom = 4.0f * dot(wi, n) * dot(wo, n);
+    return num / (denom + 1e-6f);
+}
+
+KRT_DEV float ggx_pdf(const Material &mat, f3 n, f3 wi, f3 wo) {  // brdf.cuh:357-382
+    if (dot(wo, n) <= 0.0f) n = -n;
+    if (dot(wi, n) <= 0.0f) return 0.0f;
+    f3 h = normalize(wi + wo);
+    float avg_Kd = (mat.diffuse.x + mat.diffuse.y + mat.diffuse.z) / 3.0f;
+    float avg_Ks = (mat.specular.x + mat.specular.y + mat.specular.z) / 3.0f;
+    float t = 1.0f;
+    if (avg_Kd + avg_Ks > 0.0f) t = selmax(avg_Ks / (avg_Kd + avg_Ks), 0.25f);
+    float term1 = dot(n, wi) / PI_F;
+    float term2 = ggx_D(n, h, mat) * dot(n, h) / (4.0f * dot(wi, h));
+    return (1.0f - t) * term1 + t * term2;
+}
+
+KRT_DEV f3 ggx_sample(const Material &mat, f3 n, f3 wo, f3 &seed) {  // brdf.cuh:386-433
+    if (dot(wo, n) <= 0.0f) n = -n;
+    float avg_Kd = (mat.diffuse.x + mat.diffuse.y + mat.diffuse.z) / 3.0f;
+    float avg_Ks = (mat.specular.x + mat.specular.y + mat.specular.z) / 3.0f;
+    float t = 1.0f;
+    if (avg_Kd + avg_Ks > 0.0f) t = avg_Ks / (avg_Kd + avg_Ks);
+    f3 eta = rand_uniform_3f(seed);
+    float cphi, sphi;
+    sincos_turn(eta.z, cphi, sphi);
+    float ct, st;
+    bool spec = eta.x < t;
+    if (spec) {
+        float k = sqrtf(eta.y / (1.0f - eta.y));
+        float tn = k * mat.roughness;
+        ct = 1.0f / sqrtf(1.0f + tn * tn);
+        st = tn * ct;
+    } else {
+        ct = sqrtf(eta.y);
+        st = sqrtf(1.0f - eta.y);
+    }
+    f3 v = rotate(mk(st * cphi, st * sphi, ct), n);
+    return spec ? reflect(-wo, v) : v;
+}
+
+KRT_DEV f3 glass_sample(const SurfaceHit &sh, f3 wi) {  // brdf.cuh:144-175
+    float eta = sh.mat.refraction;
+    float cos_theta = selmin(dot(-wi, sh.n), 1.0f);
+    float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    bool reflect_ray = (eta * sin_theta > 1.0f);
+    float ni_over_nt = 1.0f / eta;
+    if (!sh.entering) ni_over_nt = eta;
+    if (reflect_ray) return reflect(wi, sh.n);
+    f3 v = wi, n = sh.n;
+    float c = dot(-v, n);
+    f3 par = ni_over_nt * (v + c * n);
+    float len = length(par);
+    float len2 = len * len;
+    f3 perp = -sqrtf(fabsf(1.0f - len2)) * n;
+    return par + perp;
+}
+
+KRT_DEV f3 brdf(const SurfaceHit &sh, f3 wi, int out) {  // brdf.cuh:437-449
+    f3 diff = sh.mat.diffuse / PI_F;
+    if (out & eTransmission) return diff + mk(0.0f, 0.0f, 0.0f);
+    return diff + ggx_brdf(sh.mat, sh.n, wi, sh.wo);
+}
+
+KRT_DEV f3 eval_bsdf(const SurfaceHit &sh, f3 &wi, float &pdf, int &out, f3 &seed) {  // brdf.cuh:573-589
+    if (sh.mat.type & eTransmission) {
+        wi = glass_sample(sh, sh.wo);
+        pdf = 1.0f;
+        out = sh.mat.type;
+        return sh.mat.specular;
+    }
+    wi = ggx_sample(sh.mat, sh.n, sh.wo, seed);
+    out = eDiffuse;
+    if (length(wi) < 1e-6f) {
+        pdf = 0.0f;
+        return mk(0.0f, 0.0f, 0.0f);
+    }
+    pdf = ggx_pdf(sh.mat, sh.n, wi, sh.wo);
+    return brdf(sh, wi, out);
+}
+
+// ---------------------------------------------------------------------------
+// Triangle lights (lighting.cuh:96-146).  Device row: 4 x float4 =
+// {a, area}, {ab, 0}, {ac, 0}, {intensity, 0} plus a normal table; area and
+// normal are computed once with the same expressions as the per-sample form.
+// ---------------------------------------------------------------------------
+struct TriLight {
+    f3 a, ab, ac, intensity;
+    float area;
+    f3 normal;
+};
+
+KRT_DEV float light_area(f3 ab, f3 ac) { return length(cross(ab, ac)) * 0.5f; }
+KRT_DEV f3 light_normal(f3 ab, f3 ac) { return normalize(cross(ab, ac)); }
+
+KRT_DEV TriLight load_light(const float4 *tab, uint32_t i) {
+    float4 a = tab[4 * i], b = tab[4 * i + 1], c = tab[4 * i + 2], d = tab[4 * i + 3];
+    TriLight L;
+    L.a = mk(a.x, a.y, a.z);
+    L.area = a.w;
+    L.ab = mk(b.x, b.y, b.z);
+    L.normal.x = b.w;
+    L.ac = mk(c.x, c.y, c.z);
+    L.normal.y = c.w;
+    L.intensity = mk(d.x, d.y, d.z);
+    L.normal.z = d.w;
+    return L;
+}
+
+KRT_DEV f3 sample_area_light(const TriLight &l, f3 &seed) {
+    f3 r = pcg3f(seed);
+    float u = fract(r.x), v = fract(r.y);
+    if (u + v > 1.0f) {
+        u = 1.0f - u;
+        v = 1.0f - v;
+    }
+    return (l.a + u * l.ab) + v * l.ac;
+}
+
+KRT_DEV unsigned pick_light(f3 &seed, unsigned total) {
+    unsigned i = (unsigned)(rand_uniform(seed) * (float)total);
+    return i < total ? i : total - 1;
+}
+
+// Accumulation on a glm::vec4 (amadeus_path_tracer.cu:59-70): true division.
+KRT_DEV float4 accumulate4(float4 dst, float4 s, bool acc, float count) {
+    if (!acc) return s;
+    float c1 = count + 1.0f;
+    return make_float4((dst.x * count + s.x) / c1, (dst.y * count + s.y) / c1, (dst.z * count + s.z) / c1,
+                       (dst.w * count + s.w) / c1);
+}
+
+// Wave-level counter add (one atomic per wavefront).
+KRT_DEV void wave_count(unsigned long long *ctr, unsigned v) {
+    if (!ctr) return;
+    unsigned long long s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(ctr, s);
+}
+
+}  // namespace krt

@@ -1,0 +1,183 @@
+// Host-side internals shared by the krt translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/krt.h"
+
+namespace krt {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define KRT_HIP(call)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw ::krt::Error(KRT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define KRT_REQUIRE(cond, code, msg)                                                                 \
+    do {                                                                                             \
+        if (!(cond)) throw ::krt::Error((code), (msg));                                              \
+    } while (0)
+
+// Device allocation owned by RAII.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        KRT_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(p);
+    }
+};
+
+// Acceleration structure of one scene.
+struct Accel {
+    bool built = false;
+    uint32_t nt = 0;
+    uint32_t nser = 0;        // 2N-1 serialised nodes
+    uint32_t nnodes = 0;      // traversal nodes
+    uint32_t depth_bound = 0;
+    uint32_t levels = 0;
+    float build_ms = 0.0f;
+    DevBuf ser;        // float4 x 3 per node, bvh.cpp layout
+    DevBuf leaf_order; // int per rank
+    DevBuf tri;        // float4 x 3 per triangle in rank order: {v1, prim}, {e1}, {e2}
+    DevBuf nodes;      // float4 x 4 per traversal node
+};
+
+struct Scene {
+    uint32_t nv = 0, nt = 0, nm = 0, nl = 0;
+    DevBuf pos, nrm;   // float x 3 per vertex
+    DevBuf idx;        // uint x 3 per triangle
+    DevBuf tri_mat;    // uint per triangle
+    DevBuf mats;       // float4 x 4 per material
+    DevBuf lights;     // float4 x 4 per light
+    Accel accel;
+    uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
+    size_t bytes() const {
+        return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + accel.ser.bytes +
+               accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes;
+    }
+};
+
+// Device view of a scene passed to kernels by value.
+struct SceneView {
+    const float *pos;
+    const float *nrm;
+    const uint32_t *idx;
+    const uint32_t *tri_mat;
+    const float4 *mats;
+    const float4 *lights;
+    uint32_t nl;
+    const float4 *tri;
+    const float4 *nodes;
+};
+
+SceneView make_view(const Scene &s);
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<krt_handle, std::unique_ptr<Scene>> scenes;
+    krt_handle next_handle = 1;
+    DevBuf counters;  // scratch counters (unsigned long long x 16)
+    DevBuf spill;     // traversal stack spill slab (grown on demand)
+    Scene &scene(krt_handle h) {
+        auto it = scenes.find(h);
+        KRT_REQUIRE(it != scenes.end(), KRT_ERR_INVALID, "unknown scene handle");
+        return *it->second;
+    }
+};
+
+// Builders / kernels (defined in the .hip units)
+void accel_build(Context &ctx, Scene &s, int leaf_max);
+void upload_scene_derived(Context &ctx, Scene &s, const float *mats12, uint32_t nm);
+void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits);
+void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ);
+
+// Launch parameters of the rendering kernels (ArmadaLaunchInfo, armada.cuh:43-83)
+struct LaunchInfo {
+    uint32_t width, height;
+    float center[3], U[3], V[3], W[3];
+    float pv[16];       // this camera's projection*view (column-major)
+    float prev_pv[16];  // previous frame's, for temporal reprojection
+    int max_depth;
+    int samples;
+    int accumulate;
+    uint32_t user_seed;
+    float4 *color, *normal, *albedo, *position;
+};
+
+struct RestirOptions {
+    int candidates = 32;
+    int m_cap = 200;
+    int spatial_passes = 2;
+    int neighbours = 5;
+    int radius = 30;
+};
+
+struct RestirState {
+    uint32_t w = 0, h = 0;
+    DevBuf res[2];  // 4 float4 planes per buffer (SoA of float4)
+    int cur = 0;
+    DevBuf gmat;          // uint material id per pixel
+    DevBuf intermediate;  // float4 per pixel
+    DevBuf auxiliary;     // float4 per pixel
+    DevBuf history;       // per-pixel path history scratch
+    float prev_pv[16] = {0};
+};
+
+struct PassTimes {
+    float ms[8] = {0};
+};
+
+void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
+               PassTimes &times);
+void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
+void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
+                   unsigned long long *ctr, PassTimes &times);
+
+}  // namespace krt

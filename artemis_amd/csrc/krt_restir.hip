@@ -1,0 +1,506 @@
+// ReSTIR DI attachment: the three per-pixel passes of
+// source/optix/amadeus_restir.cu with the host sequence of
+// include/amadeus/restir.cuh:223-304, as fused gfx950 kernels.
+//
+//   initial  (__raygen__initial + __closesthit__initial, :82-144, :437-559):
+//            primary hit -> RIS over `candidates` light samples (target = sum
+//            of unshadowed RGB), one visibility ray, G-buffer material id; the
+//            rest of the path adds Le + Ld per bounce (intermediate buffer).
+//   temporal (__raygen__temporal, :213-291): merge with the previous frame's
+//            reservoir (M capped), occluded targets, resample.
+//   spatial  (__raygen__spatial, :317-434): `neighbours` reservoirs in a disk
+//            of `radius` px, 1/Z bias correction, shade, accumulate.
+//
+// Reservoirs (include/amadeus/reservoir.cuh:19-76) are stored as four float4
+// planes per buffer: {Le, type}, {normal, w}, {point, W}, {seed, M}.
+#include <hip/hip_runtime.h>
+
+#include "krt_device.h"
+#include "krt_internal.h"
+#include "krt_launch.h"
+#include "krt_shade.h"
+#include "krt_traverse.h"
+
+namespace krt {
+
+namespace {
+
+constexpr int kMaxNeighbours = 8;
+
+struct RSample {
+    f3 Le, normal, point;
+    int type;
+};
+
+struct Resv {
+    RSample data;
+    float w, W;
+    int M;
+    f3 seed;
+    KRT_DEV bool update(const RSample &s, float weight) {
+        w += weight;
+        float eta = rand_uniform(seed);
+        bool selected = (eta * w) < weight;
+        if (selected) data = s;
+        M++;
+        return selected;
+    }
+    KRT_DEV void resample(float target) {
+        float d = target * (float)M;
+        W = (d > 0.0f) ? w / d : 0.0f;
+    }
+};
+
+struct Planes {
+    float4 *p;
+    uint32_t n;
+    KRT_DEV Resv load(uint32_t i) const {
+        float4 a = p[i], b = p[n + i], c = p[2 * n + i], d = p[3 * n + i];
+        Resv r;
+        r.data.Le = mk(a.x, a.y, a.z);
+        r.data.type = __float_as_int(a.w);
+        r.data.normal = mk(b.x, b.y, b.z);
+        r.w = b.w;
+        r.data.point = mk(c.x, c.y, c.z);
+        r.W = c.w;
+        r.seed = mk(d.x, d.y, d.z);
+        r.M = __float_as_int(d.w);
+        return r;
+    }
+    KRT_DEV void store(uint32_t i, const Resv &r) const {
+        p[i] = make_float4(r.data.Le.x, r.data.Le.y, r.data.Le.z, __int_as_float(r.data.type));
+        p[n + i] = make_float4(r.data.normal.x, r.data.normal.y, r.data.normal.z, r.w);
+        p[2 * n + i] = make_float4(r.data.point.x, r.data.point.y, r.data.point.z, r.W);
+        p[3 * n + i] = make_float4(r.seed.x, r.seed.y, r.seed.z, __int_as_float(r.M));
+    }
+    // Reservoir::reset (reservoir.cuh:64-69): w = M = W = 0, data and seed kept
+    KRT_DEV void reset(uint32_t i) const {
+        p[n + i].w = 0.0f;
+        p[2 * n + i].w = 0.0f;
+        p[3 * n + i].w = __int_as_float(0);
+    }
+    KRT_DEV void store_M(uint32_t i, int M) const { p[3 * n + i].w = __int_as_float(M); }
+};
+
+// direct_unoccluded (amadeus_common.cuh:380-397)
+KRT_DEV f3 direct_li(const SurfaceHit &sh, f3 Le, f3 normal, int type, f3 D, float d) {
+    f3 rho = brdf(sh, D, eDiffuse);
+    float geometric = fabsf(dot(sh.n, D));
+    if (type != 2) {
+        float ldot = fabsf(dot(normal, D));
+        geometric *= ldot / (d * d);
+    }
+    return rho * Le * geometric;
+}
+
+// direct_occluded (:399-421)
+KRT_DEV f3 direct_occ(const SceneView &sv, const SurfaceHit &sh, const RSample &s, f3 D, float d,
+                      const TraceStack &stk, TravStats &ts, RayCount &rc, bool *occ_out = nullptr) {
+    rc.shadow++;
+    bool occ = occluded(sv, sh.x, D, d - kShadowEps, stk, ts);
+    if (occ_out) *occ_out = occ;
+    if (occ) return mk(0.0f, 0.0f, 0.0f);
+    return direct_li(sh, s.Le, s.normal, s.type, D, d);
+}
+
+// target / occluded_target (amadeus_restir.cu:148-178)
+KRT_DEV float target(const RSample &s, const SurfaceHit &hit) {
+    f3 D = s.point - hit.x;
+    float d = length(D);
+    f3 Li = direct_li(hit, s.Le, s.normal, s.type, D / d, d);
+    return Li.x + Li.y + Li.z;
+}
+
+KRT_DEV float occluded_target(const SceneView &sv, const RSample &s, const SurfaceHit &hit, const TraceStack &stk,
+                              TravStats &ts, RayCount &rc) {
+    f3 D = s.point - hit.x;
+    float d = length(D);
+    f3 Li = direct_occ(sv, hit, s, D / d, d, stk, ts, rc);
+    return Li.x + Li.y + Li.z;
+}
+
+struct RestirArgs {
+    LaunchInfo li;
+    SceneView sv;
+    Planes cur, prev;  // prev: temporal source / spatial destination
+    uint32_t *gmat;
+    float4 *intermediate, *auxiliary;
+    float4 *hist;
+    int *spill;
+    unsigned long long *ctr;
+    int candidates, m_cap, neighbours, radius;
+};
+
+// G-buffer surface of a pixel (amadeus_restir.cu:230-240, :323-336); the
+// normal is the accumulated normal AOV, as in the reference.
+KRT_DEV SurfaceHit gbuffer_hit(const RestirArgs &A, uint32_t i) {
+    SurfaceHit h;
+    h.mat = load_material(A.sv.mats, A.gmat[i]);
+    h.entering = false;
+    float4 n = A.li.normal[i], x = A.li.position[i];
+    h.n = mk(n.x, n.y, n.z);
+    h.x = mk(x.x, x.y, x.z);
+    h.wo = normalize(ld3(A.li.center) - h.x);
+    return h;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const SceneView &sv = A.sv;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    History hist{A.hist + gtid, nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    const float count = (float)li.samples;
+    const unsigned total = sv.nl;
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        const uint32_t px = pix % li.width, py = pix / li.width;
+        Resv res = A.cur.load(pix);
+        res.w = 0.0f;
+        res.M = 0;
+        res.W = 0.0f;
+        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+        f3 o, d;
+        make_ray(li, px, py, seed, false, o, d);
+        f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
+        int nlev = 0;
+        for (int level = 0; level <= li.max_depth; level++) {
+            Hit h;
+            rc.closest++;
+            traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            if (h.prim < 0) break;
+            Surface sf;
+            if (!load_surface(sv, h, d, sf)) break;
+            Material mat = load_material(sv.mats, sf.mat_id);
+            f3 n = sf.n;
+            f3 x = offset_point(sf.x, n, mat.type);
+            SurfaceHit sh{mat, sf.entering, n, -d, x};
+            f3 direct = mk(0, 0, 0);
+            if (level == 0) {
+                for (int i = 0; i < A.candidates; i++) {
+                    // sample_direct (amadeus_common.cuh:301-377), triangle lights
+                    RSample smp;
+                    smp.Le = smp.normal = smp.point = mk(0, 0, 0);
+                    smp.type = 0;
+                    float pdf = 0.0f;
+                    if (total > 0) {
+                        unsigned li_ = pick_light(seed, total);
+                        TriLight L = load_light(sv.lights, li_);
+                        smp.point = sample_area_light(L, seed);
+                        smp.Le = L.intensity;
+                        smp.normal = L.normal;
+                        smp.type = 1;
+                        pdf = 1.0f / (L.area * (float)total);
+                    } else {
+                        rand_uniform(seed);
+                    }
+                    f3 D = smp.point - sh.x;
+                    float dd = length(D);
+                    f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
+                    float tgt = Li.x + Li.y + Li.z;
+                    res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
+                }
+                RSample smp = res.data;
+                f3 D = smp.point - sh.x;
+                float dd = length(D);
+                bool occ;
+                f3 Li = direct_occ(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
+                res.resample(Li.x + Li.y + Li.z);
+                // the reference traces this same visibility ray a second time
+                res.W *= 1.0f - (occ ? 1.0f : 0.0f);
+                A.gmat[pix] = sf.mat_id;
+            } else {
+                direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+            }
+            int out;
+            f3 wi;
+            float pdf;
+            f3 f = eval_bsdf(sh, wi, pdf, out, seed);
+            bool has_pdf = pdf > 0.0f;
+            f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
+            hist.put(level, direct, true, T);  // value = direct + T * indirect
+            if (level == 0) {
+                pos0 = x;
+                n0 = n;
+                alb0 = mat.diffuse;
+            }
+            nlev = level + 1;
+            if (!has_pdf) break;
+            o = x;
+            d = wi;
+        }
+        f3 value = mk(0, 0, 0);
+        for (int k = nlev - 1; k >= 0; k--) {
+            f3 direct, T;
+            bool add;
+            hist.get(k, direct, add, T);
+            value = direct + T * value;
+        }
+        float4 col = make_float4(value.x, value.y, value.z, 1.0f);
+        if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+        A.intermediate[pix] = col;
+        A.auxiliary[pix] = make_float4(seed.x, seed.y, seed.z, 1.0f);
+        const bool acc = li.accumulate != 0;
+        li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
+        li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
+        li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
+        A.cur.store(pix, res);
+    }
+    wave_count(A.ctr, rc.closest);
+    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+// reproject (amadeus_restir.cu:182-210); pixel indices clamped to the image.
+KRT_DEV int reproject(const LaunchInfo &li, int index, f3 pos) {
+    if (li.samples > 0) return index;
+    const float *m = li.prev_pv;
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (m[0 + k] * pos.x + m[4 + k] * pos.y) + (m[8 + k] * pos.z + m[12 + k] * 1.0f);
+    float u = v[0] / v[3], vv = v[1] / v[3];
+    if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
+    u = (u + 1.0f) * 0.5f;
+    vv = (vv + 1.0f) * 0.5f;
+    int ix = (int)((double)(u * (float)li.width) + 0.5);
+    int iy = (int)((double)(vv * (float)li.height) + 0.5);
+    ix = min(max(ix, 0), (int)li.width - 1);
+    iy = min(max(iy, 0), (int)li.height - 1);
+    return iy * (int)li.width + ix;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        Resv cr = A.cur.load(pix);
+        if (cr.M <= 0) continue;
+        SurfaceHit hit = gbuffer_hit(A, pix);
+        int pindex = reproject(li, (int)pix, hit.x);
+        Resv merged;
+        merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
+        merged.data.type = 0;
+        merged.w = merged.W = 0.0f;
+        merged.M = 0;
+        merged.seed = ld3(A.auxiliary[pix]);
+        float t_current = occluded_target(A.sv, cr.data, hit, stk, ts, rc);
+        merged.update(cr.data, t_current * cr.W * (float)cr.M);
+        int N = cr.M;
+        if (pindex >= 0) {
+            Resv pr = A.prev.load((uint32_t)pindex);
+            if (pr.M > 0) {
+                pr.M = min(pr.M, A.m_cap);
+                A.prev.store_M((uint32_t)pindex, pr.M);
+                float t_prev = occluded_target(A.sv, pr.data, hit, stk, ts, rc);
+                merged.update(pr.data, t_prev * pr.W * (float)pr.M);
+                N += (t_prev > 0.0f ? 1 : 0) * pr.M;
+            }
+        }
+        merged.M = N;
+        merged.resample(target(merged.data, hit));
+        A.cur.store(pix, merged);
+    }
+    wave_count(A.ctr ? A.ctr + 2 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+// sample_spatial_neighborhood (amadeus_restir.cu:295-314)
+KRT_DEV uint32_t spatial_neighbour(const LaunchInfo &li, uint32_t index, f3 &seed, int radius) {
+    int width = (int)li.width, height = (int)li.height;
+    int x = (int)(index % li.width), y = (int)(index / li.width);
+    f3 off = rand_uniform_3f(seed);
+    float c, s;
+    sincos_turn(off.x, c, s);
+    float rr = (float)radius * sqrtf(off.y);
+    int x0 = (int)((float)x + rr * c);
+    int y0 = (int)((float)y + rr * s);
+    x0 = min(max(x0, 0), width - 1);
+    y0 = min(max(y0, 0), height - 1);
+    return (uint32_t)(x0 + y0 * width);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    const float count = (float)li.samples;
+    const int S = A.neighbours;
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        SurfaceHit hit = gbuffer_hit(A, pix);
+        Resv current = A.cur.load(pix);
+        f3 direct = mk(0, 0, 0);
+        if (current.M > 0) {
+            Resv merged;
+            merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
+            merged.data.type = 0;
+            merged.w = merged.W = 0.0f;
+            merged.M = 0;
+            merged.seed = ld3(A.auxiliary[pix]);
+            float t_current = target(current.data, hit);
+            merged.update(current.data, t_current * current.W * (float)current.M);
+            uint32_t nidx[kMaxNeighbours];
+            int sizes[kMaxNeighbours];
+            int cnt = 0;
+#pragma unroll
+            for (int i = 0; i < kMaxNeighbours; i++) {
+                nidx[i] = pix;
+                sizes[i] = 0;
+                if (i < S) {
+                    uint32_t i0 = spatial_neighbour(li, pix, merged.seed, A.radius);
+                    Resv nb = A.cur.load(i0);
+                    float t_nb = occluded_target(A.sv, nb.data, hit, stk, ts, rc);
+                    if (nb.M > 0) merged.update(nb.data, t_nb * nb.W * (float)nb.M);
+                    nidx[i] = i0;
+                    sizes[i] = nb.M;
+                    cnt += nb.M;
+                }
+            }
+            merged.M = current.M + cnt;
+            int Z = current.M;
+#pragma unroll
+            for (int i = 0; i < kMaxNeighbours; i++) {
+                if (i < S && sizes[i] > 0) {  // `entering` carries neighbour validity
+                    SurfaceHit nh;
+                    nh.mat = load_material(A.sv.mats, A.gmat[nidx[i]]);
+                    nh.entering = true;
+                    float4 nn = li.normal[nidx[i]], xx = li.position[nidx[i]];
+                    nh.n = mk(nn.x, nn.y, nn.z);
+                    nh.x = mk(xx.x, xx.y, xx.z);
+                    nh.wo = hit.wo;
+                    float t_nb = occluded_target(A.sv, merged.data, nh, stk, ts, rc);
+                    if (t_nb > 0.0f) Z += sizes[i];
+                }
+            }
+            float t_merged = target(merged.data, hit);
+            float denominator = t_merged * (float)Z;
+            merged.W = (denominator > 0.0f) ? merged.w / denominator : 0.0f;
+            f3 D = merged.data.point - hit.x;
+            float d = length(D);
+            f3 lighting = direct_occ(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
+            direct = hit.mat.emission + lighting * merged.W;
+            A.prev.store(pix, merged);
+        } else {
+            A.prev.reset(pix);
+        }
+        float4 im = A.intermediate[pix];
+        float4 col = make_float4(direct.x + im.x, direct.y + im.y, direct.z + im.z, 1.0f + im.w);
+        if (isnan(col.x) || isnan(col.y) || isnan(col.z) || isnan(col.w)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+        li.color[pix] = accumulate4(li.color[pix], col, li.accumulate != 0, count);
+    }
+    wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+__global__ void seed_kernel(Planes a, Planes b, uint32_t w, uint32_t h, uint32_t user) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    Resv r;
+    r.data.Le = r.data.normal = r.data.point = mk(0, 0, 0);
+    r.data.type = 0;
+    r.w = r.W = 0.0f;
+    r.M = 0;
+    r.seed = reservoir_seed(i % w, i / w, user);
+    a.store(i, r);
+    b.store(i, r);
+}
+
+}  // namespace
+
+void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed) {
+    size_t n = (size_t)w * h;
+    st.w = w;
+    st.h = h;
+    st.res[0].alloc(sizeof(float4) * 4 * n);
+    st.res[1].alloc(sizeof(float4) * 4 * n);
+    st.cur = 0;
+    st.gmat.alloc(sizeof(uint32_t) * n);
+    st.intermediate.alloc(sizeof(float4) * n);
+    st.auxiliary.alloc(sizeof(float4) * n);
+    KRT_HIP(hipMemsetAsync(st.gmat.p, 0, st.gmat.bytes, ctx.stream));
+    KRT_HIP(hipMemsetAsync(st.intermediate.p, 0, st.intermediate.bytes, ctx.stream));
+    KRT_HIP(hipMemsetAsync(st.auxiliary.p, 0, st.auxiliary.bytes, ctx.stream));
+    Planes a{st.res[0].as<float4>(), (uint32_t)n}, b{st.res[1].as<float4>(), (uint32_t)n};
+    seed_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx.stream>>>(a, b, w, h, user_seed);
+    KRT_HIP(hipGetLastError());
+    for (int k = 0; k < 16; k++) st.prev_pv[k] = 0.0f;
+}
+
+void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOptions &opt, RestirState &st,
+                   unsigned long long *ctr, PassTimes &times) {
+    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+    KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR buffers not loaded");
+    KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kMaxNeighbours, KRT_ERR_INVALID,
+                "restir neighbours must be in [0, 8]");
+    KRT_REQUIRE(opt.candidates >= 0 && opt.spatial_passes >= 0 && opt.m_cap >= 0 && opt.radius >= 0,
+                KRT_ERR_INVALID, "restir options must be non-negative");
+    LaunchInfo li = li_in;
+    for (int k = 0; k < 16; k++) li.prev_pv[k] = st.prev_pv[k];
+    const uint32_t npix = li.width * li.height;
+    uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
+    uint32_t nthreads = nb * kTraceBlock;
+    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
+    if (st.history.bytes < need) st.history.alloc(need);
+    RestirArgs A;
+    A.li = li;
+    A.sv = make_view(s);
+    A.gmat = st.gmat.as<uint32_t>();
+    A.intermediate = st.intermediate.as<float4>();
+    A.auxiliary = st.auxiliary.as<float4>();
+    A.hist = st.history.as<float4>();
+    A.spill = spill_buffer(ctx, s, nthreads);
+    A.ctr = ctr;
+    A.candidates = opt.candidates;
+    A.m_cap = opt.m_cap;
+    A.neighbours = opt.neighbours;
+    A.radius = opt.radius;
+    auto planes = [&](int b) { return Planes{st.res[b].as<float4>(), npix}; };
+    hipEvent_t ev[16];
+    int nev = 0;
+    auto mark = [&]() {
+        KRT_HIP(hipEventCreate(&ev[nev]));
+        KRT_HIP(hipEventRecord(ev[nev], ctx.stream));
+        nev++;
+    };
+    mark();
+    A.cur = planes(st.cur);
+    A.prev = planes(1 - st.cur);
+    initial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    KRT_HIP(hipGetLastError());
+    mark();
+    temporal_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    KRT_HIP(hipGetLastError());
+    mark();
+    for (int pass = 0; pass < opt.spatial_passes; pass++) {
+        A.cur = planes(st.cur);
+        A.prev = planes(1 - st.cur);
+        spatial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        KRT_HIP(hipGetLastError());
+        st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
+    }
+    mark();
+    st.cur = 1 - st.cur;  // final swap, restir.cuh:300
+    for (int k = 0; k < 16; k++) st.prev_pv[k] = li.pv[k];
+    KRT_HIP(hipEventSynchronize(ev[nev - 1]));
+    for (int k = 0; k + 1 < nev && k < 8; k++) KRT_HIP(hipEventElapsedTime(&times.ms[k], ev[k], ev[k + 1]));
+    for (int k = 0; k < nev; k++) (void)hipEventDestroy(ev[k]);
+}
+
+}  // namespace krt

@@ -1,0 +1,121 @@
+// Per-hit shading shared by the path-tracer and ReSTIR kernels:
+// surface reconstruction (LOAD_INTERSECTION_DATA, source/optix/amadeus_common.cuh:
+// 490-515 with calculate_normal :74-112 and calculate_intersection :62-71, for
+// world-space triangles), next-event estimation Ld (:255-286 with Ld_light,
+// include/optix/lighting.cuh:149-174) and the camera ray (make_ray,
+// source/optix/amadeus_path_tracer.cu:26-56).
+#pragma once
+
+#include "krt_device.h"
+#include "krt_internal.h"
+#include "krt_traverse.h"
+
+namespace krt {
+
+constexpr float kEps = 1e-3f;        // amadeus_common.cuh:22
+constexpr float kShadowEps = 0.05f;  // amadeus_common.cuh:193
+constexpr float kTraceTmax = 1e16f;  // amadeus_common.cuh:472
+
+struct RayCount {
+    unsigned closest = 0, shadow = 0;
+};
+
+struct Surface {
+    f3 x, n;
+    bool entering;
+    uint32_t mat_id;
+};
+
+KRT_DEV f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
+    float w = 1.0f - b1 - b2;
+    return (w * a + b1 * b) + b2 * c;
+}
+
+// Returns false when the shading normal is NaN (the reference then returns 0).
+KRT_DEV bool load_surface(const SceneView &sv, const Hit &h, f3 dir, Surface &sf) {
+    const uint32_t i0 = sv.idx[3 * h.prim], i1 = sv.idx[3 * h.prim + 1], i2 = sv.idx[3 * h.prim + 2];
+    f3 a = ld3(sv.pos + 3 * i0), b = ld3(sv.pos + 3 * i1), c = ld3(sv.pos + 3 * i2);
+    f3 ng = normalize(cross(b - a, c - a));
+    if (dot(ng, dir) > 0.0f) {
+        ng = -ng;
+        sf.entering = false;
+    } else {
+        sf.entering = true;
+    }
+    ng = normalize(ng);
+    f3 nn = interp(ld3(sv.nrm + 3 * i0), ld3(sv.nrm + 3 * i1), ld3(sv.nrm + 3 * i2), h.b1, h.b2);
+    if (dot(nn, ng) < 0.0f) nn = -nn;
+    sf.n = normalize(nn);
+    sf.x = interp(a, b, c, h.b1, h.b2);
+    sf.mat_id = sv.tri_mat[h.prim];
+    return !isnan3(sf.n);
+}
+
+KRT_DEV f3 offset_point(f3 x, f3 n, int type) {
+    return x + ((type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
+}
+
+KRT_DEV bool occluded(const SceneView &sv, f3 o, f3 d, float tmax, const TraceStack &stk, TravStats &ts) {
+    Hit h;
+    return traverse<true>(sv.nodes, sv.tri, o, d, tmax, stk, h, ts);
+}
+
+// Ld with one uniformly chosen triangle light (no quad lights, no envmap).
+KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceStack &stk, TravStats &ts,
+              RayCount &rc) {
+    const unsigned total = sv.nl;
+    if (total == 0) {
+        rand_uniform(seed);
+        return mk(0.0f, 0.0f, 0.0f);
+    }
+    unsigned i = pick_light(seed, total);
+    float pdf = 1.0f / (float)total;
+    TriLight L = load_light(sv.lights, i);
+    float light_pdf = 1.0f / L.area;
+    f3 lpos = sample_area_light(L, seed);
+    f3 wi = normalize(lpos - sh.x);
+    float R = length(lpos - sh.x);
+    rc.shadow++;
+    f3 contr;
+    if (occluded(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
+        contr = mk(0.0f, 0.0f, 0.0f);
+    } else {
+        f3 f = brdf(sh, wi, eDiffuse);
+        float ldot = fabsf(dot(L.normal, wi));
+        float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
+        contr = f * L.intensity * geometric;
+    }
+    pdf *= light_pdf;
+    return contr / pdf;
+}
+
+// Camera ray; the seed is advanced once by pcg3f (jittered or fixed -0.5 offset).
+KRT_DEV void make_ray(const LaunchInfo &li, uint32_t px, uint32_t py, f3 &seed, bool jitter, f3 &o, f3 &d) {
+    pcg3f(seed);
+    float xoff = jitter ? fract(seed.x) - 0.5f : -0.5f;
+    float yoff = jitter ? fract(seed.y) - 0.5f : -0.5f;
+    float dx = 2.0f * (((float)px + xoff) / (float)li.width) - 1.0f;
+    float dy = 2.0f * (((float)py + yoff) / (float)li.height) - 1.0f;
+    o = ld3(li.center);
+    d = normalize((dx * ld3(li.U) + dy * ld3(li.V)) + ld3(li.W));
+}
+
+// Path history of one lane: per level {direct, has_pdf} and {T}.  value is
+// then evaluated from the deepest level up, value_k = direct_k + T_k * value_k+1
+// exactly as the reference's recursion returns it.
+struct History {
+    float4 *base;
+    uint32_t stride;  // lanes in the grid
+    KRT_DEV void put(int level, f3 direct, bool has_pdf, f3 T) const {
+        base[(2 * level) * stride] = make_float4(direct.x, direct.y, direct.z, has_pdf ? 1.0f : 0.0f);
+        base[(2 * level + 1) * stride] = make_float4(T.x, T.y, T.z, 0.0f);
+    }
+    KRT_DEV void get(int level, f3 &direct, bool &has_pdf, f3 &T) const {
+        float4 a = base[(2 * level) * stride], b = base[(2 * level + 1) * stride];
+        direct = mk(a.x, a.y, a.z);
+        has_pdf = a.w != 0.0f;
+        T = mk(b.x, b.y, b.z);
+    }
+};
+
+}  // namespace krt

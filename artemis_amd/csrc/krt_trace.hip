@@ -1,0 +1,78 @@
+// Ray-query kernels behind krt_trace_closest / krt_trace_occluded.
+#include <hip/hip_runtime.h>
+
+#include "krt_device.h"
+#include "krt_internal.h"
+#include "krt_launch.h"
+#include "krt_traverse.h"
+
+namespace krt {
+
+namespace {
+
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__restrict__ nodes,
+                                                             const float4 *__restrict__ tri,
+                                                             const float4 *__restrict__ rays, uint32_t n,
+                                                             float4 *__restrict__ hits, int32_t *__restrict__ occ,
+                                                             int *spill, int spill_depth) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, spill + gtid, (int)nthreads};
+    (void)spill_depth;
+    TravStats ts;
+    for (uint32_t i = gtid; i < n; i += nthreads) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        Hit h;
+        bool r = traverse<ANY>(nodes, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts);
+        if (ANY) {
+            occ[i] = r ? 1 : 0;
+        } else {
+            hits[i] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
+        }
+    }
+}
+
+}  // namespace
+
+int *spill_buffer(Context &ctx, Scene &s, uint32_t threads) {
+    size_t depth = s.accel.depth_bound > (uint32_t)kLdsDepth ? s.accel.depth_bound - kLdsDepth + 1 : 1;
+    size_t need = sizeof(int) * depth * threads;
+    if (ctx.spill.bytes < need) ctx.spill.alloc(need);
+    return ctx.spill.as<int>();
+}
+
+uint32_t persistent_blocks(uint32_t work, uint32_t block, uint32_t max_blocks) {
+    uint32_t b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    return b < max_blocks ? b : max_blocks;
+}
+
+static void trace_common(Context &ctx, Scene &s, const float *rays_dev, uint32_t n, float *hits_dev,
+                         int32_t *occ_dev, bool any) {
+    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure (call krt_accel_build)");
+    if (n == 0) return;
+    uint32_t nb = persistent_blocks(n, kTraceBlock, kMaxResidentBlocks);
+    int *spill = spill_buffer(ctx, s, nb * kTraceBlock);
+    if (any)
+        trace_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.tri.as<float4>(),
+                                                              (const float4 *)rays_dev, n, nullptr, occ_dev, spill,
+                                                              (int)s.accel.depth_bound);
+    else
+        trace_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(),
+                                                               s.accel.tri.as<float4>(), (const float4 *)rays_dev, n,
+                                                               (float4 *)hits_dev, nullptr, spill,
+                                                               (int)s.accel.depth_bound);
+    KRT_HIP(hipGetLastError());
+}
+
+void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits) {
+    trace_common(ctx, s, rays, n, hits, nullptr, false);
+}
+
+void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ) {
+    trace_common(ctx, s, rays, n, nullptr, occ, true);
+}
+
+}  // namespace krt

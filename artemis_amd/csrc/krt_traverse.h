@@ -1,0 +1,151 @@
+// Closest-hit and any-hit traversal of the krt BVH layout on gfx950.
+//
+// Node: 4 x float4 = child boxes as {c0.min, c0.max, c1.min, c1.max} per axis
+// plus the two child links (internal node index, or a leaf range
+// 0x80000000 | (count-1) << 26 | first_rank).  Triangles sit in leaf-preorder
+// rank order as {v1, prim}, {e1}, {e2}.
+//
+// Ordered descent (near child first, far child on a per-lane stack kept in
+// LDS; entries past the LDS depth spill to a per-thread global slab).
+//
+// Result contract (the reference GLSL walk, shaders/rt/modules/bvh.glsl:52-127,
+// keeps the first leaf in preorder among equal t): the closest hit is the
+// lexicographic minimum of (t, rank) over triangles with 0 < t < tmax, with t,
+// b1, b2 from Möller–Trumbore exactly as intersect.glsl:14-33.  The box test is
+// conservative (relative slack, zero direction components treated as a
+// 1e-30 tilt), so traversal order never changes that minimum.
+#pragma once
+
+#include "krt_device.h"
+
+namespace krt {
+
+struct Hit {
+    float t, b1, b2;
+    int prim;
+    int rank;
+};
+
+struct TraceStack {
+    int *lds;      // this lane's column: lds[k * stride]
+    int stride;
+    int lds_depth;
+    int *gmem;     // this lane's spill column: gmem[k * gstride]
+    int gstride;
+    __device__ __forceinline__ void put(int k, int v) const {
+        if (k < lds_depth) lds[k * stride] = v;
+        else gmem[(k - lds_depth) * gstride] = v;
+    }
+    __device__ __forceinline__ int get(int k) const {
+        return k < lds_depth ? lds[k * stride] : gmem[(k - lds_depth) * gstride];
+    }
+};
+
+struct TravStats {
+    unsigned nodes = 0, tris = 0;
+};
+
+KRT_DEV f3 safe_inv(f3 d) {
+    float x = fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x);
+    float y = fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y);
+    float z = fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z);
+    return mk(1.0f / x, 1.0f / y, 1.0f / z);
+}
+
+// Conservative slab test of one child box (lo/hi per axis).
+KRT_DEV bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, f3 o, f3 inv, float tlim,
+                  float &tnear) {
+    float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
+    float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
+    float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    float tn_s = tn - (fabsf(tn) * 1e-5f + 1e-6f);
+    float tf_s = tf + (fabsf(tf) * 1e-5f + 1e-6f);
+    tnear = tn;
+    return tn_s <= tf_s && tf_s >= 0.0f && tn_s <= tlim;
+}
+
+// Möller–Trumbore on the packed triangle (intersect.glsl:14-33, e1/e2 precomputed
+// with the same subtraction).  Returns t or -1.
+KRT_DEV float mt_test(const float4 *tri, int r, f3 o, f3 d, float &b1, float &b2, int &prim) {
+    float4 A = tri[3 * r], B = tri[3 * r + 1], C = tri[3 * r + 2];
+    prim = __float_as_int(A.w);
+    f3 v1 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
+    f3 s1 = cross(d, e2);
+    float divisor = dot(s1, e1);
+    if (divisor == 0.0f) return -1.0f;
+    f3 s = o - v1;
+    float inv = 1.0f / divisor;
+    b1 = dot(s, s1) * inv;
+    if (b1 < 0.0f || b1 > 1.0f) return -1.0f;
+    f3 s2 = cross(s, e1);
+    b2 = dot(d, s2) * inv;
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return -1.0f;
+    return dot(e2, s2) * inv;
+}
+
+KRT_DEV bool is_leaf_link(int l) { return l < 0; }
+
+template <bool ANY>
+KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict__ tri, f3 o, f3 d, float tmax,
+                      const TraceStack &stk, Hit &hit, TravStats &ts) {
+    const f3 inv = safe_inv(d);
+    float best = tmax;
+    bool found = false;
+    hit.prim = -1;
+    hit.rank = 0x7fffffff;
+    hit.t = 0.0f;
+    hit.b1 = hit.b2 = 0.0f;
+    int sp = 0;
+    int cur = 0;
+    for (;;) {
+        if (!is_leaf_link(cur)) {
+            const float4 *nd = nodes + 4 * cur;
+            float4 X = nd[0], Y = nd[1], Z = nd[2], L = nd[3];
+            ts.nodes++;
+            float t0, t1;
+            bool h0 = slab(X.x, X.y, Y.x, Y.y, Z.x, Z.y, o, inv, best, t0);
+            bool h1 = slab(X.z, X.w, Y.z, Y.w, Z.z, Z.w, o, inv, best, t1);
+            int l0 = __float_as_int(L.x), l1 = __float_as_int(L.y);
+            if (h0 && h1) {
+                bool first0 = t0 <= t1;
+                stk.put(sp++, first0 ? l1 : l0);
+                cur = first0 ? l0 : l1;
+            } else if (h0) {
+                cur = l0;
+            } else if (h1) {
+                cur = l1;
+            } else {
+                if (sp == 0) break;
+                cur = stk.get(--sp);
+            }
+        } else {
+            const int first = cur & 0x03FFFFFF;
+            const int cnt = ((cur >> 26) & 31) + 1;
+            for (int k = 0; k < cnt; k++) {
+                int r = first + k;
+                float b1 = 0.0f, b2 = 0.0f;
+                int prim;
+                float t = mt_test(tri, r, o, d, b1, b2, prim);
+                ts.tris++;
+                if (ANY) {
+                    if (t > 0.0f && t < tmax) return true;
+                } else if (t > 0.0f && (t < best || (found && t == best && r < hit.rank))) {
+                    best = t;
+                    found = true;
+                    hit.t = t;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                    hit.prim = prim;
+                    hit.rank = r;
+                }
+            }
+            if (sp == 0) break;
+            cur = stk.get(--sp);
+        }
+    }
+    return found;
+}
+
+}  // namespace krt

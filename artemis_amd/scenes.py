@@ -1,0 +1,285 @@
+"""Seeded procedural scenes (world-space triangle soups).
+
+* cornell(): room 5 faces x 16x16 quads, two boxes x 10x10 quads per face, a
+  level-4 icosphere and one ceiling quad light (2 triangles, emission 15):
+  10,082 triangles.  Per object albedo U[0.2,0.8]^3, GGX roughness
+  U[0.05,1], specular 0 (include/material.hpp:22), type eDiffuse.
+* spheres(): `instances` copies of 16 seeded base meshes built like
+  Submesh::sphere(slices, stacks) (source/mesh.cpp:189-290, 2*slices*(stacks-1)
+  triangles) with radial noise, random rotation/scale, positions uniform in a
+  `box` m cube; `emissive` triangles chosen uniformly get their own material
+  with log-uniform emission in [0.1, 10].
+  sphere(64, 33) x 256 = 1,048,576 triangles; sphere(64, 65) x 256 = 2,097,152.
+
+Random numbers come from SplitMix64 so the scenes are identical on every host.
+Materials are rows of 12 floats {diffuse, specular, emission, roughness,
+refraction, type}.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+E_DIFFUSE = 4.0
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.s = seed & _M64
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & _M64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0, n=None):
+        if n is None:
+            return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+        return np.array([self.uniform(lo, hi) for _ in range(n)])
+
+    def uniform_array(self, n: int) -> np.ndarray:
+        """n uniforms in [0,1): vectorised SplitMix64 stream seeded from this one."""
+        base = np.uint64(self.next())
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = base + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+@dataclass
+class SceneData:
+    positions: np.ndarray     # (nv, 3) float32
+    normals: np.ndarray       # (nv, 3) float32
+    indices: np.ndarray       # (nt, 3) uint32
+    tri_material: np.ndarray  # (nt,) uint32
+    materials: np.ndarray     # (nm, 12) float32
+    camera: dict = field(default_factory=dict)
+    name: str = ""
+
+    @property
+    def triangles(self) -> int:
+        return len(self.indices)
+
+
+def material(diffuse, roughness, emission=(0.0, 0.0, 0.0), specular=(0.0, 0.0, 0.0), refraction=1.0,
+             type_=E_DIFFUSE):
+    return [*diffuse, *specular, *emission, roughness, refraction, type_]
+
+
+class _Builder:
+    def __init__(self):
+        self.p, self.n, self.i, self.m = [], [], [], []
+        self.nv = 0
+
+    def add(self, pos, nrm, idx, mat_id):
+        pos = np.asarray(pos, np.float64)
+        nrm = np.asarray(nrm, np.float64)
+        idx = np.asarray(idx, np.int64).reshape(-1, 3)
+        self.p.append(pos)
+        self.n.append(nrm)
+        self.i.append(idx + self.nv)
+        self.m.append(np.full(len(idx), mat_id, np.uint32) if np.isscalar(mat_id) else np.asarray(mat_id, np.uint32))
+        self.nv += len(pos)
+
+    def build(self, mats, **kw) -> SceneData:
+        return SceneData(np.concatenate(self.p).astype(np.float32), np.concatenate(self.n).astype(np.float32),
+                         np.concatenate(self.i).astype(np.uint32), np.concatenate(self.m),
+                         np.asarray(mats, np.float32), **kw)
+
+
+def grid_quad(origin, du, dv, n: int, normal):
+    """n x n subdivided parallelogram origin + s*du + t*dv (2 n^2 triangles)."""
+    origin, du, dv = (np.asarray(x, np.float64) for x in (origin, du, dv))
+    s = np.linspace(0.0, 1.0, n + 1)
+    S, T = np.meshgrid(s, s, indexing="ij")
+    pos = origin + S.reshape(-1, 1) * du + T.reshape(-1, 1) * dv
+    nrm = np.tile(np.asarray(normal, np.float64), (len(pos), 1))
+    idx = []
+    for a in range(n):
+        for b in range(n):
+            v00 = a * (n + 1) + b
+            v10 = (a + 1) * (n + 1) + b
+            v01 = a * (n + 1) + b + 1
+            v11 = (a + 1) * (n + 1) + b + 1
+            idx += [(v00, v10, v11), (v00, v11, v01)]
+    return pos, nrm, np.array(idx)
+
+
+def box_mesh(center, half, n: int):
+    c, h = np.asarray(center, np.float64), np.asarray(half, np.float64)
+    faces = []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+            o = c.copy()
+            o[axis] += sgn * h[axis]
+            o[a1] -= h[a1]
+            o[a2] -= h[a2]
+            du = np.zeros(3)
+            du[a1] = 2 * h[a1]
+            dv = np.zeros(3)
+            dv[a2] = 2 * h[a2]
+            nrm = np.zeros(3)
+            nrm[axis] = sgn
+            faces.append(grid_quad(o, du, dv, n, nrm))
+    return faces
+
+
+def icosphere(level: int):
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts), np.array(faces)
+
+
+def cornell(seed: int = 1) -> SceneData:
+    rng = SplitMix64(seed)
+
+    def rand_mat():
+        albedo = [rng.uniform(0.2, 0.8) for _ in range(3)]
+        return material(albedo, rng.uniform(0.05, 1.0))
+
+    mats, b = [], _Builder()
+    # room [-1,1] x [0,2] x [-1,1]: floor, ceiling, back, left, right (inward normals)
+    walls = [((-1, 0, -1), (2, 0, 0), (0, 0, 2), (0, 1, 0)),
+             ((-1, 2, -1), (0, 0, 2), (2, 0, 0), (0, -1, 0)),
+             ((-1, 0, -1), (0, 2, 0), (2, 0, 0), (0, 0, 1)),
+             ((-1, 0, -1), (0, 0, 2), (0, 2, 0), (1, 0, 0)),
+             ((1, 0, -1), (0, 2, 0), (0, 0, 2), (-1, 0, 0))]
+    for o, du, dv, nrm in walls:
+        mats.append(rand_mat())
+        b.add(*grid_quad(o, du, dv, 16, nrm), len(mats) - 1)
+    for center, half in (((-0.4, 0.6, -0.35), (0.3, 0.6, 0.3)), ((0.45, 0.3, 0.3), (0.3, 0.3, 0.3))):
+        mats.append(rand_mat())
+        for face in box_mesh(center, half, 10):
+            b.add(*face, len(mats) - 1)
+    mats.append(rand_mat())
+    v, f = icosphere(4)
+    r, c = 0.28, np.array([0.35, 0.88, 0.25])
+    b.add(c + r * v, v, f, len(mats) - 1)
+    mats.append(material((0.8, 0.8, 0.8), 1.0, emission=(15.0, 15.0, 15.0)))
+    b.add(np.array([(-0.25, 1.99, -0.25), (0.25, 1.99, -0.25), (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25)]),
+          np.tile((0.0, -1.0, 0.0), (4, 1)), np.array([(0, 2, 1), (0, 3, 2)]), len(mats) - 1)
+    cam = dict(eye=(0.0, 1.0, 3.5), lookat=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+    return b.build(mats, camera=cam, name="cornell")
+
+
+def sphere_mesh(slices: int, stacks: int):
+    """Submesh::sphere (source/mesh.cpp:189-290): unit sphere, 2*slices*(stacks-1) triangles."""
+    pos = [(0.0, 1.0, 0.0)]
+    for i in range(stacks - 1):
+        phi = np.pi * (i + 1) / stacks
+        for j in range(slices):
+            th = 2.0 * np.pi * j / slices
+            pos.append((np.sin(phi) * np.cos(th), np.cos(phi), np.sin(phi) * np.sin(th)))
+    pos.append((0.0, -1.0, 0.0))
+    pos = np.array(pos)
+    idx = []
+    last = len(pos) - 1
+    for i in range(slices):
+        idx.append((0, (i + 1) % slices + 1, i + 1))
+        idx.append((last, i + slices * (stacks - 2) + 1, (i + 1) % slices + slices * (stacks - 2) + 1))
+    for i in range(stacks - 2):
+        for j in range(slices):
+            i0 = i * slices + j + 1
+            i1 = i * slices + (j + 1) % slices + 1
+            i2 = (i + 1) * slices + (j + 1) % slices + 1
+            i3 = (i + 1) * slices + j + 1
+            idx += [(i0, i1, i2), (i0, i2, i3)]
+    return pos, np.array(idx)
+
+
+def _rotation(rng: SplitMix64) -> np.ndarray:
+    u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
+    q = np.array([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)])
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: int = 4096, box: float = 20.0,
+            seed: int = 2, base_meshes: int = 16) -> SceneData:
+    rng = SplitMix64(seed)
+    unit, idx = sphere_mesh(slices, stacks)
+    bases = []
+    for _ in range(base_meshes):
+        noise = 1.0 + 0.15 * (rng.uniform_array(len(unit)) - 0.5)
+        bases.append(unit * noise[:, None])
+    ntb = len(idx)
+    nt = instances * ntb
+    mats = []
+    P, N, I, M = [], [], [], []
+    for k in range(instances):
+        base = bases[k % base_meshes]
+        R = _rotation(rng)
+        s = rng.uniform(0.4, 1.2)
+        t = np.array([rng.uniform(-box / 2, box / 2) for _ in range(3)])
+        mats.append(material([rng.uniform(0.2, 0.8) for _ in range(3)], rng.uniform(0.05, 1.0)))
+        P.append((base * s) @ R.T + t)
+        N.append(unit @ R.T)
+        I.append(idx + k * len(unit))
+        M.append(np.full(ntb, len(mats) - 1, np.uint32))
+    tri_mat = np.concatenate(M)
+    if emissive > 0:
+        picks = np.unique((rng.uniform_array(emissive * 2) * nt).astype(np.int64))[:emissive]
+        k = len(picks)
+        while k < emissive:  # top up (duplicates are rare)
+            extra = (rng.uniform_array(emissive) * nt).astype(np.int64)
+            picks = np.unique(np.concatenate([picks, extra]))[:emissive]
+            k = len(picks)
+        levels = np.exp(np.log(0.1) + (np.log(10.0) - np.log(0.1)) * rng.uniform_array(emissive))
+        for tri, e in zip(picks, levels):
+            base = mats[tri_mat[tri]]
+            mats.append(material(base[0:3], base[9], emission=(e, e, e)))
+            tri_mat[tri] = len(mats) - 1
+    d = box * 0.9
+    cam = dict(eye=(0.0, 0.0, d), lookat=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=60.0)
+    return SceneData(np.concatenate(P).astype(np.float32), np.concatenate(N).astype(np.float32),
+                     np.concatenate(I).astype(np.uint32), tri_mat, np.asarray(mats, np.float32), camera=cam,
+                     name=f"spheres{instances}x{ntb}")
+
+
+def random_soup(n: int, seed: int = 3, extent: float = 1.0, size: float = 0.2) -> SceneData:
+    """n independent random triangles (parity tests of the build and traversal)."""
+    rng = SplitMix64(seed)
+    c = (rng.uniform_array(3 * n).reshape(n, 3) - 0.5) * 2 * extent
+    off = (rng.uniform_array(9 * n).reshape(n, 3, 3) - 0.5) * size
+    pos = (c[:, None, :] + off).reshape(-1, 3)
+    e1 = off[:, 1] - off[:, 0]
+    e2 = off[:, 2] - off[:, 0]
+    nrm = np.cross(e1, e2)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True) + 1e-30
+    nrm = np.repeat(nrm, 3, axis=0)
+    idx = np.arange(3 * n).reshape(n, 3)
+    mats = [material((0.5, 0.5, 0.5), 0.5)]
+    return SceneData(pos.astype(np.float32), nrm.astype(np.float32), idx.astype(np.uint32),
+                     np.zeros(n, np.uint32), np.asarray(mats, np.float32), name=f"soup{n}")

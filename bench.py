@@ -1,0 +1,195 @@
+"""Benchmark of the ray-traced rendering path on one MI355X.
+
+Default workload (the headline configuration): ReSTIR DI at 1920x1080 on the
+seeded 1,048,576-triangle instanced-sphere scene (256 x sphere(64, 33),
+positions uniform in a 20 m cube, 4,096 emissive triangles with log-uniform
+emission in [0.1, 10]); 32 initial candidates per pixel, temporal M-cap 20,
+one spatial pass of 5 neighbours in a 30 px disk, path depth 2 (the
+ArmadaRTX default), FP32 SoA reservoirs.  A step is one frame; the scene,
+BVH, reservoirs and AOV buffers are resident in HBM before timing starts.
+
+Prints one JSON line: Msamples/s (pixels x frames / s), MRays/s, p50/p99
+frame latency, the roofline of the dominant kernel (HIP-event kernel times)
+and the CPU oracle timed on the host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic resampling bytes per pixel of each ReSTIR pass (64 B reservoir,
+# 28 B G-buffer): initial 64+28, temporal 2*64+28+64, spatial (1 pass, 5
+# neighbours) 6*64+6*28+64+32.
+RESTIR_PIXEL_BYTES = {0: 64 + 28, 1: 2 * 64 + 28 + 64, 2: 6 * 64 + 6 * 28 + 64 + 32}
+PASS_NAMES = {0: "initial", 1: "temporal", 2: "spatial"}
+PT_PIXEL_BYTES = 112  # color/normal/albedo float4 read+write + position write
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def b_ray(name):
+    data = json.loads((ROOT / "tests" / "golden" / "bray.json").read_text())
+    return data[name]
+
+
+def cpu_baseline(workload, scene, W, H, seconds):
+    """The oracle (restated reference algorithm) on the host cores, on a
+    bounded sample: the same scene at a reduced resolution, frames until
+    about `seconds` of work."""
+    import oracle as O
+    from artemis_amd import look_at
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 64))
+    t0 = time.perf_counter()
+    nodes = O.bvh_build(O.tri_boxes(scene.positions, scene.indices))
+    build_s = time.perf_counter() - t0
+    osc = O.OracleScene(scene, nodes)
+    w, h = max(16, W // 8), max(16, H // 8)
+    cam = look_at(**scene.camera, aspect=w / h)
+    frames = 0
+    t0 = time.perf_counter()
+    if workload == "restir_di":
+        rs = O.OracleReSTIR(w, h)
+        while True:
+            rs.frame(osc, cam, max_depth=2, candidates=32, m_cap=20, spatial_passes=1, neighbours=5, radius=30,
+                     nthreads=cores)
+            frames += 1
+            if time.perf_counter() - t0 > seconds or frames >= 64:
+                break
+    else:
+        bufs = None
+        while True:
+            bufs, _ = osc.render_pt(cam, w, h, frames=1, max_depth=5, nthreads=cores, buffers=bufs,
+                                    first_sample=frames)
+            frames += 1
+            if time.perf_counter() - t0 > seconds or frames >= 64:
+                break
+    dt = time.perf_counter() - t0
+    return {"value": w * h * frames / dt / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "sample": f"{workload} {w}x{h} x {frames} frames on the same scene, {cores} threads; "
+                      f"oracle bvh.cpp build {build_s:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["restir_di", "pt"], default="restir_di")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from artemis_amd import ArmadaRTX, Context, PathTracer, ReSTIR, look_at, scenes
+
+    W, H = args.width, args.height
+    log(f"[bench] generating scene")
+    scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
+    ctx = Context(0)
+    log(f"[bench] uploading {scene.triangles} triangles + GPU BVH build")
+    sc = ctx.upload(scene)
+    bst = sc.stats()
+    log(f"[bench] bvh: {bst}")
+    cam = look_at(**scene.camera, aspect=W / H)
+    arm = ArmadaRTX(ctx, W, H)
+    if args.workload == "restir_di":
+        arm.attach("restir", ReSTIR(ctx))
+        arm.set_option("candidates", 32)
+        arm.set_option("m_cap", 20)
+        arm.set_option("spatial_passes", 1)
+        arm.set_option("neighbours", 5)
+        arm.set_option("radius", 30)
+        arm.set_depth(2)
+        bname = "restir_di_1080p_spheres1M"
+    else:
+        arm.attach("path_tracer", PathTracer(ctx))
+        arm.set_depth(5)
+        bname = "pt_1080p_spheres1M"
+    for i in range(args.warmup):
+        arm.render(sc, cam, accumulate=True)
+        log(f"[bench] warmup {i + 1}/{args.warmup}: {arm.stats()['gpu_ms']:.2f} ms")
+    arm.sync()
+    lat, rays, kms, prays, nodes, tris = [], 0, np.zeros(8), np.zeros(4), 0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        f0 = time.perf_counter()
+        arm.render(sc, cam, accumulate=True)  # returns after the frame completed on the GPU
+        lat.append(time.perf_counter() - f0)
+        st = arm.stats()
+        rays += st["closest_rays"] + st["shadow_rays"]
+        kms += np.array(st["kernel_ms"])
+        prays += np.array(st["pass_rays"], dtype=np.float64)
+        nodes += st["node_visits"]
+        tris += st["tri_tests"]
+    arm.sync()
+    dt = time.perf_counter() - t0
+    K = args.steps
+    npix = W * H
+    value = npix * K / dt / 1e6
+    kavg = kms / K
+    br = b_ray(bname)
+    if args.workload == "restir_di":
+        passes = [0, 1, 2]
+        per_pass_bytes = {p: RESTIR_PIXEL_BYTES[p] * npix + prays[p] / K * br["b_ray"] for p in passes}
+    else:
+        passes = [0]
+        per_pass_bytes = {0: PT_PIXEL_BYTES * npix + prays[0] / K * br["b_ray"]}
+    dom = max(passes, key=lambda p: kavg[p])
+    achieved = per_pass_bytes[dom] / (kavg[dom] * 1e-3) / 1e9
+    total_rays = rays / K
+    roof = {"bound": "hbm", "kernel": f"{args.workload}:{PASS_NAMES.get(dom, 'pt')}", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "b_ray": br["b_ray"], "ms": float(kavg[dom]),
+            "own_bytes_per_ray": float((64 * nodes + 48 * tris) / max(rays, 1))}
+    line = {
+        "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
+        "value": value,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": dt / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": f"{args.workload} {W}x{H} spheres 256x4096 tris (1,048,576), "
+                               + ("32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), depth 2"
+                                  if args.workload == "restir_di" else "1 spp/frame, max depth 5"),
+                   "width": W, "height": H, "triangles": int(scene.triangles), "parallelism": "single GPU"},
+        "mrays_per_s": rays / dt / 1e6,
+        "rays_per_sample": total_rays / npix,
+        "p50_frame_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_frame_ms": float(np.percentile(lat, 99) * 1e3),
+        "kernel_ms": {PASS_NAMES.get(p, str(p)): float(kavg[p]) for p in passes},
+        "bvh_build_ms": bst["build_ms"],
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle baseline")
+        line["cpu_baseline"] = cpu_baseline(args.workload, scene, W, H, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

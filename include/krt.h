@@ -1,0 +1,177 @@
+/*
+ * krt — C ABI of the MI355X ray-traced rendering path.
+ *
+ * The reference renderer (Kobra, vedavamadathil/artemis) exposes this path as
+ * C++ classes: ArmadaRTX (include/amadeus/armada.cuh:142-336), its plugin
+ * interface AttachmentRTX (armada.cuh:107-139) with the PathTracer
+ * (include/amadeus/path_tracer.cuh:17-167) and ReSTIR
+ * (include/amadeus/restir.cuh:35-305) attachments, and the acceleration
+ * structure layer amadeus::Accelerator (include/amadeus/accelerator.cuh:23-278).
+ * The entry points below are the plain-pointer equivalents of those methods;
+ * include/krt_armada.hpp re-creates the C++ classes on top of them.
+ *
+ * Conventions: every call returns KRT_OK (0) or an error code; the message of
+ * the last failure on the calling thread is krt_last_error().  The reference
+ * throws std::runtime_error / kobra::cuda::Exception instead
+ * (include/cuda/error.cuh:43-98).  A context is single-threaded, like the
+ * reference's one host thread (all launches on one stream).
+ */
+#ifndef KRT_H_
+#define KRT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    KRT_OK = 0,
+    KRT_ERR_INVALID = 1, /* bad argument or handle */
+    KRT_ERR_HIP = 2,     /* HIP runtime / launch failure */
+    KRT_ERR_BUDGET = 3,  /* request exceeds the HBM budget */
+    KRT_ERR_STATE = 4    /* call not valid in this state (no attachment, no BVH, ...) */
+};
+
+typedef struct krt_context krt_context;
+typedef struct krt_armada krt_armada;
+typedef struct krt_attachment krt_attachment;
+/* Scene + its acceleration structure; the analogue of OptixTraversableHandle. */
+typedef uint64_t krt_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char *krt_last_error(void);
+
+/* Context on HIP device `device`; owns one stream. */
+int krt_create(int device, krt_context **out);
+void krt_destroy(krt_context *ctx);
+
+/* ---------------------------------------------------------------- scenes --
+ * World-space triangle soup.  Replaces MeshDaemon::cache_cuda
+ * (source/daemons/mesh.cu:25-61), the material upload of
+ * ArmadaRTX::preprocess_scene (source/amadeus/armada.cu:560-572) and the
+ * triangle-light list (armada.cu:98-150: every triangle whose material has
+ * |emission| > 0 becomes a light, in triangle order).
+ * materials: 12 floats each {diffuse.rgb, specular.rgb, emission.rgb,
+ * roughness, refraction, type} with type a Shading bit set (4 = eDiffuse,
+ * 2 = eTransmission; include/types.hpp:18-26). */
+typedef struct {
+    const float *positions; /* nv x 3 */
+    const float *normals;   /* nv x 3 */
+    uint32_t nv;
+    const uint32_t *indices;      /* nt x 3 */
+    const uint32_t *tri_material; /* nt */
+    uint32_t nt;
+    const float *materials; /* nm x 12 */
+    uint32_t nm;
+} krt_scene_desc;
+
+int krt_scene_upload(krt_context *ctx, const krt_scene_desc *desc, krt_handle *scene);
+int krt_scene_free(krt_context *ctx, krt_handle scene);
+int krt_scene_light_count(krt_context *ctx, krt_handle scene, uint32_t *out);
+/* 12 floats per light {a, ab, ac, intensity} (include/optix/lighting.cuh:96-100) */
+int krt_scene_lights(krt_context *ctx, krt_handle scene, float *out, size_t nfloats);
+
+/* ----------------------------------------------------------- accelerator --
+ * GPU build of the reference's binary "SAH" BVH (source/bvh.cpp:32-210,
+ * serialised as bvh.cpp:213-260), then its collapse into the traversal layout
+ * (leaves of up to leaf_max triangles, 1..32).  Replaces
+ * Accelerator::build_gas/build_tlas (accelerator.cuh:46-277). */
+typedef struct {
+    uint32_t triangles;
+    uint32_t serialized_nodes; /* 2N-1 */
+    uint32_t traversal_nodes;
+    uint32_t depth_bound;  /* upper bound of the binary tree depth */
+    uint32_t build_levels; /* level-synchronous build passes */
+    float build_ms;        /* GPU time of the build (HIP events) */
+} krt_bvh_stats;
+
+int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max);
+int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out);
+/* bvh.cpp serialised layout, 12 floats (3 x vec4) per node */
+int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t nfloats);
+/* primitive id of each leaf in preorder (the closest-hit tie-break rank) */
+int krt_accel_leaf_order(krt_context *ctx, krt_handle scene, int32_t *out, size_t n);
+
+/* Ray queries on host buffers.  rays: 8 floats {o.xyz, tmax, d.xyz, pad};
+ * hits: 4 words {t, b1, b2, prim} with prim = -1 on a miss.  The closest hit
+ * is the lexicographic minimum of (t, preorder leaf rank) over triangles with
+ * 0 < t < tmax (Möller–Trumbore, shaders/rt/modules/intersect.glsl:14-33).
+ * Replaces optixTrace (source/optix/amadeus_common.cuh:467-478, :190-212). */
+int krt_trace_closest(krt_context *ctx, krt_handle scene, const float *rays, uint32_t n, float *hits);
+int krt_trace_occluded(krt_context *ctx, krt_handle scene, const float *rays, uint32_t n, int32_t *occluded);
+/* Same on device buffers (rays/hits already in HBM); enqueued on the
+ * context stream, no synchronisation. */
+int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n, void *hits_dev);
+
+/* ---------------------------------------------------------------- render --
+ * ArmadaRTX (armada.cuh:142-336) and its attachments. */
+typedef struct {
+    float center[3]; /* ArmadaLaunchInfo::Camera (armada.cuh:46-55) */
+    float U[3], V[3], W[3];
+    float pv[16]; /* projection * view, column-major (for reprojection) */
+} krt_camera;
+
+enum { KRT_OPT_NONE = 0, KRT_OPT_INT = 1, KRT_OPT_FLOAT = 2, KRT_OPT_BOOL = 3, KRT_OPT_STRING = 4 };
+/* AttachmentRTX::OptionValue = variant<int, float, bool, string> (armada.cuh:120) */
+typedef struct {
+    int kind;
+    int i;
+    float f;
+    int b;
+    const char *s;
+} krt_option;
+
+typedef struct {
+    void *color, *normal, *albedo, *position; /* W x H float4 each (armada.cu:70-73) */
+} krt_buffers;
+
+typedef struct {
+    uint64_t closest_rays; /* camera + bounce rays traced */
+    uint64_t shadow_rays;  /* visibility rays traced */
+    uint64_t samples;      /* path samples (pixels x frames) */
+    float gpu_ms;          /* GPU time of the last render call (HIP events) */
+    float kernel_ms[8];    /* per-pass GPU time: [0] path tracer / initial, [1] temporal, [2] spatial */
+    uint64_t pass_rays[4]; /* rays traced per pass, same indexing */
+    uint64_t node_visits;  /* traversal nodes fetched by all rays of the frame */
+    uint64_t tri_tests;    /* ray-triangle tests of the frame */
+} krt_frame_stats;
+
+/* ArmadaRTX(context, accelerator, mesh daemon, extent) (armada.cu:28-83) */
+int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out);
+void krt_armada_destroy(krt_armada *a);
+/* Attachment factory: kind "path_tracer" | "restir" | "restir_gi". */
+int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **out);
+void krt_attachment_destroy(krt_attachment *att);
+/* ArmadaRTX::attach / activate / set_option / get_option / set_depth
+ * (armada.cuh:275-320) */
+int krt_armada_attach(krt_armada *a, const char *name, krt_attachment *att);
+int krt_armada_activate(krt_armada *a, const char *name);
+int krt_armada_set_option(krt_armada *a, const char *field, const krt_option *value);
+int krt_armada_get_option(krt_armada *a, const char *field, krt_option *out);
+int krt_armada_set_depth(krt_armada *a, int depth);
+/* ArmadaRTX::render(system, entities, transforms, camera, transform,
+ * accumulate) (armada.cu:733-780): one frame of the active attachment. */
+int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *camera, int accumulate);
+/* color/normal/albedo/position_buffer() (armada.cuh:287-301) */
+int krt_armada_buffers(krt_armada *a, krt_buffers *out);
+/* which: 0 color, 1 normal, 2 albedo, 3 position; out: W*H*4 floats */
+int krt_armada_readback(krt_armada *a, int which, float *out);
+int krt_armada_upload(krt_armada *a, int which, const float *in);
+int krt_armada_stats(krt_armada *a, krt_frame_stats *out);
+int krt_armada_sync(krt_armada *a);
+
+/* Plugin entry of source/amadeus/naive_path_tracer.cu:280-344
+ * (`extern "C" ret load_attachment()`), by kind. */
+typedef struct {
+    const char *name;
+    krt_attachment *ptr;
+} krt_attachment_ret;
+krt_attachment_ret krt_load_attachment(krt_context *ctx, const char *kind);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,273 @@
+"""CPU oracle bindings — TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of liboracle.so (oracle.h).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only as the checker; the product (artemis_amd/) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+fp = C.POINTER(C.c_float)
+dp = C.POINTER(C.c_double)
+u32p = C.POINTER(C.c_uint32)
+i32p = C.POINTER(C.c_int32)
+u64p = C.POINTER(C.c_uint64)
+vp = C.c_void_p
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3), ("W", C.c_float * 3),
+                ("pv", C.c_float * 16)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("max_depth", C.c_int), ("samples", C.c_int),
+                ("accumulate", C.c_int), ("user_seed", C.c_uint32), ("candidates", C.c_int), ("m_cap", C.c_int),
+                ("spatial_passes", C.c_int), ("neighbours", C.c_int), ("radius", C.c_int)]
+
+
+_SIGS = {
+    "orc_scene_create": (vp, [fp, fp, C.c_int, u32p, u32p, C.c_int, fp, C.c_int, fp, C.c_int]),
+    "orc_scene_destroy": (None, [vp]),
+    "orc_scene_light_count": (C.c_int, [vp]),
+    "orc_scene_lights": (None, [vp, fp]),
+    "orc_pcg3d": (None, [u32p, u32p, C.c_int]),
+    "orc_pcg3f": (None, [fp, C.c_int]),
+    "orc_sincos_turn": (None, [fp, fp, fp, C.c_int]),
+    "orc_tri_boxes": (None, [fp, u32p, C.c_int, fp]),
+    "orc_bvh_build": (C.c_int, [fp, C.c_int, fp]),
+    "orc_bvh_leaf_order": (C.c_int, [fp, C.c_int, i32p]),
+    "orc_bvh_depth": (C.c_int, [fp, C.c_int]),
+    "orc_trace": (None, [vp, fp, C.c_int, C.c_int, fp, u64p]),
+    "orc_occluded": (None, [vp, fp, C.c_int, i32p]),
+    "orc_ggx_pdf": (C.c_float, [fp, fp, fp, fp, fp, C.c_float]),
+    "orc_ggx_brdf": (None, [fp, fp, fp, fp, fp, C.c_float, fp]),
+    "orc_ggx_sample": (None, [fp, fp, fp, fp, C.c_float, fp, fp]),
+    "orc_light_sample": (None, [fp, fp, fp]),
+    "orc_render_pt": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), fp, fp, fp, fp, C.c_int, u64p]),
+    "orc_render_pt_pixels": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), i32p, C.c_int, fp]),
+    "orc_accumulate_pt_f64": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int, C.c_int, dp,
+                                     C.c_int]),
+    "orc_restir_create": (vp, [C.c_int, C.c_int, C.c_uint32]),
+    "orc_restir_destroy": (None, [vp]),
+    "orc_restir_frame": (None, [vp, vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), fp, fp, fp, fp, C.c_int,
+                                u64p]),
+    "orc_restir_reservoirs": (None, [vp, C.c_int, fp]),
+}
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile liboracle.so with the committed Makefile."""
+    r = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(fp)
+
+
+def _u(a):
+    return a.ctypes.data_as(u32p)
+
+
+def pcg3d(words: np.ndarray) -> np.ndarray:
+    w = np.ascontiguousarray(words, np.uint32).reshape(-1, 3)
+    out = np.empty_like(w)
+    lib().orc_pcg3d(_u(w), _u(out), len(w))
+    return out
+
+
+def pcg3f(seeds: np.ndarray) -> np.ndarray:
+    s = np.ascontiguousarray(seeds, np.float32).reshape(-1, 3).copy()
+    lib().orc_pcg3f(_f(s), len(s))
+    return s
+
+
+def sincos_turn(u: np.ndarray):
+    u = np.ascontiguousarray(u, np.float32)
+    c = np.empty_like(u)
+    s = np.empty_like(u)
+    lib().orc_sincos_turn(_f(u), _f(c), _f(s), len(u))
+    return c, s
+
+
+def tri_boxes(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    out = np.empty((len(idx), 6), np.float32)
+    lib().orc_tri_boxes(_f(pos), _u(idx), len(idx), _f(out))
+    return out
+
+
+def bvh_build(boxes: np.ndarray) -> np.ndarray:
+    boxes = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    n = len(boxes)
+    out = np.zeros((max(2 * n - 1, 1), 12), np.float32)
+    k = lib().orc_bvh_build(_f(boxes), n, _f(out))
+    return out[:k]
+
+
+def bvh_leaf_order(nodes: np.ndarray) -> np.ndarray:
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    out = np.empty(len(nodes), np.int32)
+    k = lib().orc_bvh_leaf_order(_f(nodes), len(nodes), out.ctypes.data_as(i32p))
+    return out[:k]
+
+
+def bvh_depth(nodes: np.ndarray) -> int:
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    return lib().orc_bvh_depth(_f(nodes), len(nodes))
+
+
+def camera_struct(frame) -> OrcCamera:
+    c = OrcCamera()
+    center, U, V, W, pv = frame.as_arrays()
+    c.center[:] = [float(x) for x in center]
+    c.U[:] = [float(x) for x in U]
+    c.V[:] = [float(x) for x in V]
+    c.W[:] = [float(x) for x in W]
+    c.pv[:] = [float(x) for x in pv]
+    return c
+
+
+def params(width, height, max_depth=5, samples=0, accumulate=1, user_seed=0, candidates=32, m_cap=200,
+           spatial_passes=2, neighbours=5, radius=30) -> OrcParams:
+    return OrcParams(width, height, max_depth, samples, accumulate, user_seed, candidates, m_cap, spatial_passes,
+                     neighbours, radius)
+
+
+class OracleScene:
+    """Oracle view of a scene (positions, normals, indices, materials, BVH)."""
+
+    def __init__(self, data, nodes: np.ndarray | None = None):
+        self.data = data
+        self._keep = [np.ascontiguousarray(data.positions, np.float32),
+                      np.ascontiguousarray(data.normals, np.float32),
+                      np.ascontiguousarray(data.indices, np.uint32),
+                      np.ascontiguousarray(data.tri_material, np.uint32),
+                      np.ascontiguousarray(data.materials, np.float32)]
+        if nodes is None:
+            nodes = bvh_build(tri_boxes(self._keep[0], self._keep[2]))
+        self.nodes = np.ascontiguousarray(nodes, np.float32)
+        p, n, i, m, mats = self._keep
+        self.h = lib().orc_scene_create(_f(p), _f(n), len(p), _u(i), _u(m), len(i), _f(mats), len(mats),
+                                        _f(self.nodes), len(self.nodes))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def lights(self) -> np.ndarray:
+        n = lib().orc_scene_light_count(self.h)
+        out = np.empty((n, 12), np.float32)
+        if n:
+            lib().orc_scene_lights(self.h, _f(out))
+        return out
+
+    def trace(self, rays: np.ndarray, mode: int = 0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.empty((len(rays), 4), np.float32)
+        ctr = np.zeros(2, np.uint64)
+        lib().orc_trace(self.h, _f(rays), len(rays), mode, _f(hits), ctr.ctypes.data_as(u64p))
+        return hits, ctr
+
+    def occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty(len(rays), np.int32)
+        lib().orc_occluded(self.h, _f(rays), len(rays), out.ctypes.data_as(i32p))
+        return out
+
+    def render_pt(self, frame, width, height, frames=1, max_depth=5, user_seed=0, nthreads=8, buffers=None,
+                  first_sample=0):
+        """Accumulate `frames` frames like ArmadaRTX (samples counter from first_sample)."""
+        if buffers is None:
+            buffers = [np.zeros((height, width, 4), np.float32) for _ in range(4)]
+        cam = camera_struct(frame)
+        ctr = np.zeros(5, np.uint64)
+        for f in range(frames):
+            p = params(width, height, max_depth=max_depth, samples=first_sample + f, user_seed=user_seed)
+            lib().orc_render_pt(self.h, C.byref(cam), C.byref(p), *[_f(b) for b in buffers], nthreads,
+                                ctr.ctypes.data_as(u64p))
+        return buffers, ctr
+
+    def render_pt_pixels(self, frame, width, height, pixels, sample=0, max_depth=5, user_seed=0):
+        pixels = np.ascontiguousarray(pixels, np.int32)
+        out = np.empty((len(pixels), 4), np.float32)
+        cam = camera_struct(frame)
+        p = params(width, height, max_depth=max_depth, samples=sample, user_seed=user_seed)
+        lib().orc_render_pt_pixels(self.h, C.byref(cam), C.byref(p), pixels.ctypes.data_as(i32p), len(pixels),
+                                   _f(out))
+        return out
+
+    def accumulate_pt_f64(self, frame, width, height, first, count, max_depth=5, user_seed=0, nthreads=8, acc=None):
+        if acc is None:
+            acc = np.zeros((height, width, 3), np.float64)
+        cam = camera_struct(frame)
+        p = params(width, height, max_depth=max_depth, user_seed=user_seed)
+        lib().orc_accumulate_pt_f64(self.h, C.byref(cam), C.byref(p), first, count, acc.ctypes.data_as(dp),
+                                    nthreads)
+        return acc
+
+
+class OracleReSTIR:
+    def __init__(self, width, height, user_seed=0):
+        self.w, self.h_ = width, height
+        self.h = lib().orc_restir_create(width, height, user_seed)
+        self.user_seed = user_seed
+        self.buffers = [np.zeros((height, width, 4), np.float32) for _ in range(4)]
+        self.samples = 0
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_restir_destroy(self.h)
+        except Exception:
+            pass
+
+    def frame(self, scene: OracleScene, frame, max_depth=2, candidates=32, m_cap=200, spatial_passes=2,
+              neighbours=5, radius=30, nthreads=8, reset=False):
+        if reset:
+            self.samples = 0
+        cam = camera_struct(frame)
+        p = params(self.w, self.h_, max_depth=max_depth, samples=self.samples, user_seed=self.user_seed,
+                   candidates=candidates, m_cap=m_cap, spatial_passes=spatial_passes, neighbours=neighbours,
+                   radius=radius)
+        ctr = np.zeros(5, np.uint64)
+        lib().orc_restir_frame(self.h, scene.h, C.byref(cam), C.byref(p), *[_f(b) for b in self.buffers], nthreads,
+                               ctr.ctypes.data_as(u64p))
+        self.samples += 1
+        return ctr
+
+    def reservoirs(self, which=0) -> np.ndarray:
+        out = np.empty((self.h_, self.w, 16), np.float32)
+        lib().orc_restir_reservoirs(self.h, which, _f(out))
+        return out

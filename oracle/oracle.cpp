@@ -1,0 +1,1280 @@
+// CPU oracle — TEST INFRASTRUCTURE ONLY (see oracle.h).
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <thread>
+#include <vector>
+
+#include "rt.hpp"
+
+using namespace orc;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+struct Hit {
+    float t, b1, b2;
+    int prim;
+};
+
+struct Counters {
+    uint64_t closest = 0, shadow = 0, discarded = 0, nodes = 0, tris = 0;
+    void add(const Counters &o) {
+        closest += o.closest;
+        shadow += o.shadow;
+        discarded += o.discarded;
+        nodes += o.nodes;
+        tris += o.tris;
+    }
+};
+
+}  // namespace
+
+struct orc_scene {
+    std::vector<f3> pos, nrm;
+    std::vector<uint32_t> idx, tri_mat;
+    std::vector<Material> mats;
+    std::vector<float> nodes;  // 12 floats per node (3 x vec4)
+    int nnodes = 0;
+    std::vector<int32_t> leaf_order;
+    std::vector<TriLight> lights;
+};
+
+static inline f3 load3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv, const uint32_t *idx,
+                            const uint32_t *tri_mat, int nt, const float *mats, int nm,
+                            const float *bvh_nodes, int nnodes) {
+    orc_scene *s = new orc_scene();
+    s->pos.resize(nv);
+    s->nrm.resize(nv);
+    for (int i = 0; i < nv; i++) {
+        s->pos[i] = load3(pos + 3 * i);
+        s->nrm[i] = load3(nrm + 3 * i);
+    }
+    s->idx.assign(idx, idx + 3 * (size_t)nt);
+    s->tri_mat.assign(tri_mat, tri_mat + nt);
+    s->mats.resize(nm);
+    for (int m = 0; m < nm; m++) {
+        const float *q = mats + 12 * m;
+        Material &M = s->mats[m];
+        M.diffuse = load3(q);
+        M.specular = load3(q + 3);
+        M.emission = load3(q + 6);
+        M.roughness = q[9];
+        M.refraction = q[10];
+        M.type = (int)q[11];
+    }
+    s->nnodes = nnodes;
+    s->nodes.assign(bvh_nodes, bvh_nodes + 12 * (size_t)nnodes);
+    s->leaf_order.resize(nt);
+    orc_bvh_leaf_order(bvh_nodes, nnodes, s->leaf_order.data());
+    // Emissive triangles become triangle lights in triangle order
+    // (source/amadeus/armada.cu:117-140; emissive test armada.cu:240).
+    for (int t = 0; t < nt; t++) {
+        const Material &M = s->mats[s->tri_mat[t]];
+        if (length(M.emission) > 0.0f) {
+            f3 a = s->pos[s->idx[3 * t]], b = s->pos[s->idx[3 * t + 1]], c = s->pos[s->idx[3 * t + 2]];
+            s->lights.push_back(TriLight{a, b - a, c - a, M.emission});
+        }
+    }
+    return s;
+}
+
+void orc_scene_destroy(orc_scene *s) { delete s; }
+int orc_scene_light_count(const orc_scene *s) { return (int)s->lights.size(); }
+void orc_scene_lights(const orc_scene *s, float *out) {
+    for (size_t i = 0; i < s->lights.size(); i++) {
+        const TriLight &l = s->lights[i];
+        f3 v[4] = {l.a, l.ab, l.ac, l.intensity};
+        for (int k = 0; k < 4; k++) {
+            out[12 * i + 3 * k] = v[k].x;
+            out[12 * i + 3 * k + 1] = v[k].y;
+            out[12 * i + 3 * k + 2] = v[k].z;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RNG probes
+// ---------------------------------------------------------------------------
+void orc_pcg3d(const uint32_t *in, uint32_t *out, int n) {
+    for (int i = 0; i < n; i++) {
+        uint32_t v[3] = {in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+        pcg3d(v);
+        out[3 * i] = v[0];
+        out[3 * i + 1] = v[1];
+        out[3 * i + 2] = v[2];
+    }
+}
+
+void orc_pcg3f(float *seeds, int n) {
+    for (int i = 0; i < n; i++) {
+        f3 s = load3(seeds + 3 * i);
+        pcg3f(s);
+        seeds[3 * i] = s.x;
+        seeds[3 * i + 1] = s.y;
+        seeds[3 * i + 2] = s.z;
+    }
+}
+
+void orc_sincos_turn(const float *u, float *c, float *s, int n) {
+    for (int i = 0; i < n; i++) sincos_turn(u[i], c[i], s[i]);
+}
+
+// ---------------------------------------------------------------------------
+// BVH build — source/bvh.cpp restated iteratively
+// ---------------------------------------------------------------------------
+struct Box {
+    float mn[3], mx[3];
+};
+
+// include/mesh.hpp:66-71: min(p1, min(p2, p3)), max likewise (glm semantics)
+void orc_tri_boxes(const float *pos, const uint32_t *idx, int nt, float *boxes) {
+    for (int t = 0; t < nt; t++) {
+        const float *p1 = pos + 3 * idx[3 * t], *p2 = pos + 3 * idx[3 * t + 1], *p3 = pos + 3 * idx[3 * t + 2];
+        for (int k = 0; k < 3; k++) {
+            boxes[6 * t + k] = gmin(p1[k], gmin(p2[k], p3[k]));
+            boxes[6 * t + 3 + k] = gmax(p1[k], gmax(p2[k], p3[k]));
+        }
+    }
+}
+
+namespace {
+
+struct BNode {
+    Box box;
+    int object = -1;
+    int left = -1, right = -1;
+};
+
+// include/bbox.hpp:16-26
+inline float surface_area(const float mn[3], const float mx[3]) {
+    float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    float xy = dx * dy, yz = dy * dz, xz = dx * dz;
+    return 2.0f * (xy + yz + xz);
+}
+
+// bvh.cpp:32-43 (fold from the first element)
+Box union_of(const std::vector<BNode> &N, const std::vector<int> &items) {
+    Box b = N[items[0]].box;
+    for (size_t i = 1; i < items.size(); i++)
+        for (int k = 0; k < 3; k++) {
+            b.mn[k] = gmin(b.mn[k], N[items[i]].box.mn[k]);
+            b.mx[k] = gmax(b.mx[k], N[items[i]].box.mx[k]);
+        }
+    return b;
+}
+
+}  // namespace
+
+int orc_bvh_build(const float *boxes, int n, float *out) {
+    if (n <= 0) return 0;
+    std::vector<BNode> N(n);
+    N.reserve(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            N[i].box.mn[k] = boxes[6 * i + k];
+            N[i].box.mx[k] = boxes[6 * i + 3 + k];
+        }
+        N[i].object = i;  // bvh.cpp:198-207
+    }
+    struct Task {
+        std::vector<int> items;
+        int parent, side;
+    };
+    int root = -1;
+    std::vector<Task> stack;
+    {
+        Task t;
+        t.items.resize(n);
+        for (int i = 0; i < n; i++) t.items[i] = i;
+        t.parent = -1;
+        t.side = 0;
+        stack.push_back(std::move(t));
+    }
+    auto link = [&](int parent, int side, int node) {
+        if (parent < 0) root = node;
+        else if (side == 0) N[parent].left = node;
+        else N[parent].right = node;
+    };
+    while (!stack.empty()) {
+        Task task = std::move(stack.back());
+        stack.pop_back();
+        const std::vector<int> &items = task.items;
+        size_t cnt = items.size();
+        if (cnt == 1) {  // bvh.cpp:108-109
+            link(task.parent, task.side, items[0]);
+            continue;
+        }
+        if (cnt == 2) {  // bvh.cpp:111-117
+            BNode node;
+            node.box = union_of(N, items);
+            node.left = items[0];
+            node.right = items[1];
+            N.push_back(node);
+            link(task.parent, task.side, (int)N.size() - 1);
+            continue;
+        }
+        // Axis and interval of the single box with the largest extent (bvh.cpp:119-139)
+        int axis = 0;
+        float max_extent = 0.0f, min_value = FLT_MAX, max_value = -FLT_MAX;
+        for (size_t j = 0; j < cnt; j++) {
+            const Box &b = N[items[j]].box;
+            for (int i = 0; i < 3; i++) {
+                float extent = fabsf(b.mx[i] - b.mn[i]);
+                if (extent > max_extent) {
+                    max_extent = extent;
+                    min_value = b.mn[i];
+                    max_value = b.mx[i];
+                    axis = i;
+                }
+            }
+        }
+        // 10 candidate planes; sah_cost (bvh.cpp:46-98) evaluated for all ten in
+        // one pass — each candidate's fold visits the items in the same order.
+        const int bins = 10;
+        float split[bins];
+        for (int i = 0; i < bins; i++) split[i] = (max_value - min_value) / (float)bins * (float)i + min_value;
+        float lmn[bins][3], lmx[bins][3], rmn[bins][3], rmx[bins][3];
+        int pl[bins], pr[bins];
+        for (int i = 0; i < bins; i++) {
+            pl[i] = pr[i] = 0;
+            for (int k = 0; k < 3; k++) {
+                lmn[i][k] = rmn[i][k] = FLT_MAX;
+                lmx[i][k] = rmx[i][k] = -FLT_MAX;
+            }
+        }
+        float tmn[3], tmx[3];
+        for (int k = 0; k < 3; k++) {
+            tmn[k] = N[items[0]].box.mn[k];
+            tmx[k] = N[items[0]].box.mx[k];
+        }
+        for (size_t j = 0; j < cnt; j++) {
+            const Box &b = N[items[j]].box;
+            for (int k = 0; k < 3; k++) {
+                tmn[k] = gmin(tmn[k], b.mn[k]);
+                tmx[k] = gmax(tmx[k], b.mx[k]);
+            }
+            float value = (b.mn[axis] + b.mx[axis]) / 2.0f;
+            for (int i = 0; i < bins; i++) {
+                if (value < split[i]) {
+                    pl[i]++;
+                    for (int k = 0; k < 3; k++) {
+                        lmn[i][k] = gmin(lmn[i][k], b.mn[k]);
+                        lmx[i][k] = gmax(lmx[i][k], b.mx[k]);
+                    }
+                } else {
+                    pr[i]++;
+                    for (int k = 0; k < 3; k++) {
+                        rmn[i][k] = gmin(rmn[i][k], b.mn[k]);
+                        rmx[i][k] = gmax(rmx[i][k], b.mx[k]);
+                    }
+                }
+            }
+        }
+        float sa_total = surface_area(tmn, tmx);
+        float min_cost = FLT_MAX, min_split = 0.0f;
+        for (int i = 0; i < bins; i++) {
+            float cost;
+            if (pl[i] == 0 || pr[i] == 0) cost = FLT_MAX;
+            else
+                cost = 1.0f + ((float)pl[i] * surface_area(lmn[i], lmx[i]) +
+                               (float)pr[i] * surface_area(rmn[i], rmx[i])) / sa_total;
+            if (cost < min_cost) {  // bvh.cpp:150-153 (strict: first minimum)
+                min_cost = cost;
+                min_split = split[i];
+            }
+        }
+        Task L, R;
+        if (min_cost == FLT_MAX) {  // bvh.cpp:159-166 even/odd fallback
+            for (size_t j = 0; j < cnt; j++) (j % 2 == 0 ? L : R).items.push_back(items[j]);
+        } else {  // bvh.cpp:167-180
+            for (size_t j = 0; j < cnt; j++) {
+                const Box &b = N[items[j]].box;
+                float value = (b.mn[axis] + b.mx[axis]) / 2.0f;
+                (value < min_split ? L : R).items.push_back(items[j]);
+            }
+        }
+        BNode node;
+        node.box = union_of(N, items);
+        N.push_back(node);
+        int id = (int)N.size() - 1;
+        link(task.parent, task.side, id);
+        L.parent = R.parent = id;
+        L.side = 0;
+        R.side = 1;
+        stack.push_back(std::move(R));
+        stack.push_back(std::move(L));
+    }
+    // subtree node counts (bvh.cpp:18-22), post-order without recursion
+    std::vector<int> count(N.size(), 1);
+    {
+        std::vector<std::pair<int, int>> st;
+        st.push_back({root, 0});
+        while (!st.empty()) {
+            auto &top = st.back();
+            int v = top.first;
+            if (N[v].left < 0) {
+                st.pop_back();
+                continue;
+            }
+            if (top.second == 0) {
+                top.second = 1;
+                st.push_back({N[v].left, 0});
+            } else if (top.second == 1) {
+                top.second = 2;
+                st.push_back({N[v].right, 0});
+            } else {
+                count[v] = 1 + count[N[v].left] + count[N[v].right];
+                st.pop_back();
+            }
+        }
+    }
+    // serialize (bvh.cpp:213-260): preorder, 3 vec4 per node, links in vec4 units
+    std::vector<std::pair<int, int>> st;  // (node, miss)
+    st.push_back({root, -1});
+    int written = 0;
+    while (!st.empty()) {
+        auto [v, miss] = st.back();
+        st.pop_back();
+        int size = 3 * written;
+        int after = size + 3;
+        int left = N[v].left, right = N[v].right;
+        int left_index = left >= 0 ? after : -1;
+        int left_size = left >= 0 ? 3 * count[left] : 0;
+        int right_index = right >= 0 ? after + left_size : -1;
+        int hit = left_index;
+        if (left < 0 && right < 0) hit = miss;
+        int miss_left = right >= 0 ? right_index : miss;
+        float *o = out + 12 * (size_t)written;
+        o[0] = N[v].object != -1 ? 1.0f : 0.0f;
+        o[1] = bitsf((uint32_t)N[v].object);
+        o[2] = bitsf((uint32_t)hit);
+        o[3] = bitsf((uint32_t)miss);
+        for (int k = 0; k < 3; k++) {
+            o[4 + k] = N[v].box.mn[k];
+            o[8 + k] = N[v].box.mx[k];
+        }
+        o[7] = 0.0f;
+        o[11] = 0.0f;
+        written++;
+        if (right >= 0) st.push_back({right, miss});
+        if (left >= 0) st.push_back({left, miss_left});
+    }
+    return written;
+}
+
+int orc_bvh_leaf_order(const float *nodes, int nnodes, int32_t *order) {
+    int k = 0;
+    for (int i = 0; i < nnodes; i++) {
+        int32_t obj = (int32_t)fbits(nodes[12 * i + 1]);
+        if (obj != -1) order[k++] = obj;
+    }
+    return k;
+}
+
+int orc_bvh_depth(const float *nodes, int nnodes) {
+    // depth via preorder walk: a node's children follow it; track with a stack of
+    // remaining right-link targets.
+    std::vector<std::pair<int, int>> st;  // (vec4 index, depth)
+    if (nnodes == 0) return 0;
+    st.push_back({0, 1});
+    int best = 0;
+    while (!st.empty()) {
+        auto [v, d] = st.back();
+        st.pop_back();
+        best = std::max(best, d);
+        const float *h = nodes + 4 * (size_t)v;
+        int32_t obj = (int32_t)fbits(h[1]);
+        if (obj != -1) continue;
+        int left = (int32_t)fbits(h[2]);
+        int right = (int32_t)fbits(nodes[4 * (size_t)left + 3]);
+        st.push_back({left, d + 1});
+        st.push_back({right, d + 1});
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+namespace {
+
+inline void tri_verts(const orc_scene *s, int prim, f3 &a, f3 &b, f3 &c) {
+    a = s->pos[s->idx[3 * prim]];
+    b = s->pos[s->idx[3 * prim + 1]];
+    c = s->pos[s->idx[3 * prim + 2]];
+}
+
+// shaders/rt/modules/bbox.glsl:8-57 (returns t entry or -1)
+inline float glsl_box(f3 o, f3 d, const float *mn, const float *mx) {
+    float tmin = (mn[0] - o.x) / d.x, tmax = (mx[0] - o.x) / d.x;
+    if (tmin > tmax) std::swap(tmin, tmax);
+    float tymin = (mn[1] - o.y) / d.y, tymax = (mx[1] - o.y) / d.y;
+    if (tymin > tymax) std::swap(tymin, tymax);
+    if ((tmin > tymax) || (tymin > tmax)) return -1.0f;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = (mn[2] - o.z) / d.z, tzmax = (mx[2] - o.z) / d.z;
+    if (tzmin > tzmax) std::swap(tzmin, tzmax);
+    if ((tmin > tzmax) || (tzmin > tmax)) return -1.0f;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return tmin;
+}
+
+// bbox.glsl:60-66
+inline bool in_box(f3 p, const float *mn, const float *mx) {
+    return !(p.x < mn[0] || p.y < mn[1] || p.z < mn[2] || p.x > mx[0] || p.y > mx[1] || p.z > mx[2]);
+}
+
+// Conservative slab test: never rejects a box that the ray's segment [0, tlim]
+// meets (a zero direction component constrains only by the origin, a relative
+// slack absorbs rounding).  Over-acceptance only costs visits.
+inline bool robust_box(f3 o, f3 d, const float *mn, const float *mx, float tlim) {
+    float tn = -FLT_MAX, tf = FLT_MAX;
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    for (int a = 0; a < 3; a++) {
+        if (dd[a] == 0.0f) {
+            if (oo[a] < mn[a] || oo[a] > mx[a]) return false;
+            continue;
+        }
+        float t0 = (mn[a] - oo[a]) / dd[a], t1 = (mx[a] - oo[a]) / dd[a];
+        if (t0 > t1) std::swap(t0, t1);
+        tn = std::max(tn, t0);
+        tf = std::min(tf, t1);
+    }
+    float eps = 1e-4f * std::max(fabsf(tn), fabsf(tf)) + 1e-6f;
+    return tn - eps <= tf + eps && tf + eps >= 0.0f && tn - eps <= tlim;
+}
+
+// Closest hit: lexicographic minimum of (t, preorder leaf rank) over triangles
+// with 0 < t < tmax.  mode 0 brute force in rank order, 1 GLSL walk, 2 robust walk.
+Hit closest(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
+    Hit h{0.0f, 0.0f, 0.0f, -1};
+    float best = tmax;
+    if (mode == 0) {
+        for (int32_t prim : s->leaf_order) {
+            f3 a, b, cc;
+            tri_verts(s, prim, a, b, cc);
+            float b1 = 0.0f, b2 = 0.0f;
+            float t = intersect_tri(a, b, cc, o, d, b1, b2);
+            c.tris++;
+            if (t > 0.0f && t < best) {
+                best = t;
+                h = Hit{t, b1, b2, prim};
+            }
+        }
+        return h;
+    }
+    const float *nodes = s->nodes.data();
+    int node = s->nnodes > 0 ? 0 : -1;
+    while (node != -1) {
+        const float *hd = nodes + 4 * (size_t)node;
+        int32_t obj = (int32_t)fbits(hd[1]);
+        c.nodes++;
+        if (obj != -1) {
+            f3 a, b, cc;
+            tri_verts(s, obj, a, b, cc);
+            float b1 = 0.0f, b2 = 0.0f;
+            float t = intersect_tri(a, b, cc, o, d, b1, b2);
+            c.tris++;
+            if (t > 0.0f && t < best) {
+                best = t;
+                h = Hit{t, b1, b2, obj};
+            }
+            node = (int32_t)fbits(hd[3]);
+        } else {
+            const float *mn = hd + 4, *mx = hd + 8;
+            bool go;
+            if (mode == 1) {
+                float t = glsl_box(o, d, mn, mx);
+                go = (t > 0.0f && t < best) || in_box(o, mn, mx);
+            } else {
+                go = robust_box(o, d, mn, mx, best);
+            }
+            node = go ? (int32_t)fbits(hd[2]) : (int32_t)fbits(hd[3]);
+        }
+    }
+    return h;
+}
+
+// Any hit with 0 < t < tmax (OPTIX_RAY_FLAG_TERMINATE_ON_FIRST_HIT,
+// amadeus_common.cuh:190-212).
+bool occluded(const orc_scene *s, f3 o, f3 d, float tmax, Counters &c) {
+    const float *nodes = s->nodes.data();
+    int node = s->nnodes > 0 ? 0 : -1;
+    while (node != -1) {
+        const float *hd = nodes + 4 * (size_t)node;
+        int32_t obj = (int32_t)fbits(hd[1]);
+        c.nodes++;
+        if (obj != -1) {
+            f3 a, b, cc;
+            tri_verts(s, obj, a, b, cc);
+            float b1, b2;
+            float t = intersect_tri(a, b, cc, o, d, b1, b2);
+            c.tris++;
+            if (t > 0.0f && t < tmax) return true;
+            node = (int32_t)fbits(hd[3]);
+        } else {
+            bool go = robust_box(o, d, hd + 4, hd + 8, tmax);
+            node = go ? (int32_t)fbits(hd[2]) : (int32_t)fbits(hd[3]);
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+void orc_trace(const orc_scene *s, const float *rays, int n, int mode, float *hits, uint64_t *counters) {
+    Counters c;
+    for (int i = 0; i < n; i++) {
+        const float *r = rays + 8 * (size_t)i;
+        Hit h = closest(s, load3(r), load3(r + 4), r[3], mode, c);
+        hits[4 * i] = h.t;
+        hits[4 * i + 1] = h.b1;
+        hits[4 * i + 2] = h.b2;
+        hits[4 * i + 3] = bitsf((uint32_t)h.prim);
+    }
+    if (counters) {
+        counters[0] += c.nodes;
+        counters[1] += c.tris;
+    }
+}
+
+void orc_occluded(const orc_scene *s, const float *rays, int n, int32_t *out) {
+    Counters c;
+    for (int i = 0; i < n; i++) {
+        const float *r = rays + 8 * (size_t)i;
+        out[i] = occluded(s, load3(r), load3(r + 4), r[3], c) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BSDF probes
+// ---------------------------------------------------------------------------
+static Material probe_mat(const float *kd, const float *ks, float rough) {
+    Material m;
+    m.diffuse = load3(kd);
+    m.specular = load3(ks);
+    m.emission = mk(0, 0, 0);
+    m.roughness = rough;
+    m.refraction = 1.0f;
+    m.type = eDiffuse;
+    return m;
+}
+
+float orc_ggx_pdf(const float *n, const float *wi, const float *wo, const float *kd, const float *ks,
+                  float roughness) {
+    return ggx_pdf(probe_mat(kd, ks, roughness), load3(n), load3(wi), load3(wo));
+}
+
+void orc_ggx_brdf(const float *n, const float *wi, const float *wo, const float *kd, const float *ks,
+                  float roughness, float *out) {
+    SurfaceHit sh;
+    sh.mat = probe_mat(kd, ks, roughness);
+    sh.n = load3(n);
+    sh.wo = load3(wo);
+    sh.entering = true;
+    f3 f = brdf(sh, load3(wi), eDiffuse);
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+}
+
+void orc_ggx_sample(const float *n, const float *wo, const float *kd, const float *ks, float roughness,
+                    float *seed, float *wi_out) {
+    f3 sd = load3(seed);
+    f3 wi = ggx_sample(probe_mat(kd, ks, roughness), load3(n), load3(wo), sd);
+    seed[0] = sd.x;
+    seed[1] = sd.y;
+    seed[2] = sd.z;
+    wi_out[0] = wi.x;
+    wi_out[1] = wi.y;
+    wi_out[2] = wi.z;
+}
+
+void orc_light_sample(const float *l, float *seed, float *point_out) {
+    TriLight L{load3(l), load3(l + 3), load3(l + 6), load3(l + 9)};
+    f3 sd = load3(seed);
+    f3 p = sample_area_light(L, sd);
+    seed[0] = sd.x;
+    seed[1] = sd.y;
+    seed[2] = sd.z;
+    point_out[0] = p.x;
+    point_out[1] = p.y;
+    point_out[2] = p.z;
+}
+
+// ---------------------------------------------------------------------------
+// Shared per-hit logic (amadeus_common.cuh)
+// ---------------------------------------------------------------------------
+namespace {
+
+const float kEps = 1e-3f;          // amadeus_common.cuh:22
+const float kShadowEps = 0.05f;    // amadeus_common.cuh:193
+const float kTraceTmax = 1e16f;    // amadeus_common.cuh:472
+
+struct Ctx {
+    const orc_scene *s;
+    Counters c;
+};
+
+// LOAD_INTERSECTION_DATA (amadeus_common.cuh:490-515), calculate_normal
+// (:74-112), calculate_intersection (:62-71) for world-space triangles.
+struct Surface {
+    f3 x, n;
+    bool entering;
+    Material mat;
+};
+
+inline f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
+    float w = 1.0f - b1 - b2;
+    return (w * a + b1 * b) + b2 * c;
+}
+
+bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
+    f3 a, b, c;
+    tri_verts(s, h.prim, a, b, c);
+    f3 e1 = b - a, e2 = c - a;
+    f3 ng = normalize(cross(e1, e2));
+    if (dot(ng, dir) > 0.0f) {
+        ng = -ng;
+        sf.entering = false;
+    } else {
+        sf.entering = true;
+    }
+    ng = normalize(ng);
+    const uint32_t *ix = &s->idx[3 * h.prim];
+    f3 nn = interp(s->nrm[ix[0]], s->nrm[ix[1]], s->nrm[ix[2]], h.b1, h.b2);
+    if (dot(nn, ng) < 0.0f) nn = -nn;
+    sf.n = normalize(nn);
+    sf.x = interp(a, b, c, h.b1, h.b2);
+    sf.mat = s->mats[s->tri_mat[h.prim]];
+    return !isnan3(sf.n);
+}
+
+inline unsigned pick_light(f3 &seed, unsigned total) {
+    unsigned i = (unsigned)(rand_uniform(seed) * (float)total);
+    return i < total ? i : total - 1;  // rand*total may round up to total
+}
+
+// Ld (amadeus_common.cuh:255-286) with Ld_light (lighting.cuh:149-174)
+f3 Ld(Ctx &cx, const SurfaceHit &sh, f3 &seed) {
+    const orc_scene *s = cx.s;
+    int total = (int)s->lights.size();
+    if (total == 0) {
+        rand_uniform(seed);
+        return mk(0, 0, 0);
+    }
+    unsigned i = pick_light(seed, (unsigned)total);
+    float pdf = 1.0f / (float)total;
+    const TriLight &light = s->lights[i];
+    float light_pdf = 1.0f / light_area(light);
+    f3 lpos = sample_area_light(light, seed);
+    f3 wi = normalize(lpos - sh.x);
+    float R = length(lpos - sh.x);
+    cx.c.shadow++;
+    f3 contr;
+    if (occluded(s, sh.x, wi, R - kShadowEps, cx.c)) {
+        contr = mk(0, 0, 0);
+    } else {
+        f3 f = brdf(sh, wi, eDiffuse);
+        float ldot = fabsf(dot(light_normal(light), wi));
+        float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
+        contr = f * light.intensity * geometric;
+    }
+    pdf *= light_pdf;
+    return contr / pdf;
+}
+
+struct RayPacket {  // amadeus_path_tracer.cu:10-23
+    f3 value{0, 0, 0};
+    f3 position{0, 0, 0};
+    f3 normal{0, 0, 0};
+    f3 albedo{0, 0, 0};
+    f3 seed{0, 0, 0};
+    int depth = 0;
+};
+
+// __closesthit__ / __miss__ of amadeus_path_tracer.cu:133-220, recursive as in
+// the reference.  The ray past max_depth is not traced: its program only
+// writes value = 0 (LOAD_RAYPACKET, amadeus_common.cuh:480-488).
+void pt_trace(Ctx &cx, int max_depth, f3 o, f3 d, RayPacket &rp) {
+    if (rp.depth > max_depth) {
+        cx.c.discarded++;
+        rp.value = mk(0, 0, 0);
+        return;
+    }
+    cx.c.closest++;
+    Hit h = closest(cx.s, o, d, kTraceTmax, 2, cx.c);
+    if (h.prim < 0) {
+        rp.value = mk(0, 0, 0);  // no environment map
+        return;
+    }
+    Surface sf;
+    if (!load_surface(cx.s, h, d, sf)) {
+        rp.value = mk(0, 0, 0);
+        return;
+    }
+    const Material &material = sf.mat;
+    f3 n = sf.n;
+    f3 x = sf.x + ((material.type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
+    SurfaceHit sh{material, sf.entering, n, -d, x};
+    f3 direct = material.emission + Ld(cx, sh, rp.seed);
+    int out;
+    f3 wi;
+    float pdf;
+    f3 f = eval_bsdf(sh, wi, pdf, out, rp.seed);
+    f3 T = f * fabsf(dot(wi, n)) / pdf;
+    rp.depth++;
+    f3 indirect = mk(0, 0, 0);
+    if (pdf > 0.0f) {
+        pt_trace(cx, max_depth, x, wi, rp);
+        indirect = rp.value;
+    }
+    rp.value = direct;
+    if (pdf > 0.0f) rp.value = rp.value + T * indirect;
+    rp.position = x;
+    rp.normal = n;
+    rp.albedo = material.diffuse;
+}
+
+// make_ray (amadeus_path_tracer.cu:26-56; ReSTIR variant amadeus_restir.cu:31-65
+// uses a fixed -0.5 offset).  The seed is advanced once by pcg3f either way.
+void make_ray(const orc_camera *cam, const orc_params *p, uint32_t px, uint32_t py, f3 &seed, bool jitter,
+              f3 &o, f3 &d) {
+    pcg3f(seed);
+    float xoff = jitter ? fract(seed.x) - 0.5f : -0.5f;
+    float yoff = jitter ? fract(seed.y) - 0.5f : -0.5f;
+    float dx = 2.0f * (((float)px + xoff) / (float)p->width) - 1.0f;
+    float dy = 2.0f * (((float)py + yoff) / (float)p->height) - 1.0f;
+    o = load3(cam->center);
+    d = normalize((dx * load3(cam->U) + dy * load3(cam->V)) + load3(cam->W));
+}
+
+// accumulate (amadeus_path_tracer.cu:59-70) on a glm::vec4 (true division)
+inline void accumulate4(float *dst, const float *sample, const orc_params *p) {
+    if (p->accumulate) {
+        float count = (float)p->samples;
+        for (int k = 0; k < 4; k++) dst[k] = (dst[k] * count + sample[k]) / (count + 1.0f);
+    } else {
+        for (int k = 0; k < 4; k++) dst[k] = sample[k];
+    }
+}
+
+void pt_sample(Ctx &cx, const orc_camera *cam, const orc_params *p, uint32_t px, uint32_t py, RayPacket &rp) {
+    rp = RayPacket();
+    rp.seed = pixel_seed(px, py, (uint32_t)p->samples, p->user_seed);
+    f3 o, d;
+    make_ray(cam, p, px, py, rp.seed, true, o, d);
+    pt_trace(cx, p->max_depth, o, d, rp);
+}
+
+template <class F>
+void parallel_rows(int height, int nthreads, std::vector<Counters> &per, F fn) {
+    if (nthreads < 1) nthreads = 1;
+    per.assign(nthreads, Counters());
+    std::atomic<int> next{0};
+    auto work = [&](int tid) {
+        for (;;) {
+            int y = next.fetch_add(1);
+            if (y >= height) break;
+            fn(y, per[tid]);
+        }
+    };
+    if (nthreads == 1) {
+        work(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work, t);
+    for (auto &t : th) t.join();
+}
+
+void write_counters(const std::vector<Counters> &per, uint64_t *counters) {
+    if (!counters) return;
+    Counters c;
+    for (auto &x : per) c.add(x);
+    counters[0] += c.closest;
+    counters[1] += c.shadow;
+    counters[2] += c.discarded;
+    counters[3] += c.nodes;
+    counters[4] += c.tris;
+}
+
+}  // namespace
+
+void orc_render_pt(const orc_scene *s, const orc_camera *cam, const orc_params *p, float *color, float *normal,
+                   float *albedo, float *position, int nthreads, uint64_t *counters) {
+    std::vector<Counters> per;
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        for (int x = 0; x < p->width; x++) {
+            RayPacket rp;
+            pt_sample(cx, cam, p, (uint32_t)x, (uint32_t)y, rp);
+            size_t i = (size_t)y * p->width + x;
+            float col[4] = {rp.value.x, rp.value.y, rp.value.z, 1.0f};
+            if (isnan3(rp.value)) {  // amadeus_path_tracer.cu:118-120
+                col[0] = 1;
+                col[1] = 0;
+                col[2] = 1;
+                col[3] = 1;
+            }
+            float nrm[4] = {rp.normal.x, rp.normal.y, rp.normal.z, 0.0f};
+            float alb[4] = {rp.albedo.x, rp.albedo.y, rp.albedo.z, 0.0f};
+            accumulate4(color + 4 * i, col, p);
+            accumulate4(normal + 4 * i, nrm, p);
+            accumulate4(albedo + 4 * i, alb, p);
+            position[4 * i] = rp.position.x;
+            position[4 * i + 1] = rp.position.y;
+            position[4 * i + 2] = rp.position.z;
+            position[4 * i + 3] = 0.0f;
+        }
+        c.add(cx.c);
+    });
+    write_counters(per, counters);
+}
+
+void orc_render_pt_pixels(const orc_scene *s, const orc_camera *cam, const orc_params *p, const int32_t *pix, int n,
+                          float *out) {
+    Ctx cx{s, Counters()};
+    for (int i = 0; i < n; i++) {
+        RayPacket rp;
+        uint32_t x = (uint32_t)(pix[i] % p->width), y = (uint32_t)(pix[i] / p->width);
+        pt_sample(cx, cam, p, x, y, rp);
+        bool bad = isnan3(rp.value);
+        out[4 * i] = bad ? 1.0f : rp.value.x;
+        out[4 * i + 1] = bad ? 0.0f : rp.value.y;
+        out[4 * i + 2] = bad ? 1.0f : rp.value.z;
+        out[4 * i + 3] = 1.0f;
+    }
+}
+
+void orc_accumulate_pt_f64(const orc_scene *s, const orc_camera *cam, const orc_params *p, int first, int count,
+                           double *acc, int nthreads) {
+    std::vector<Counters> per;
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        orc_params q = *p;
+        for (int f = first; f < first + count; f++) {
+            q.samples = f;
+            for (int x = 0; x < p->width; x++) {
+                RayPacket rp;
+                pt_sample(cx, cam, &q, (uint32_t)x, (uint32_t)y, rp);
+                size_t i = (size_t)y * p->width + x;
+                if (isnan3(rp.value)) continue;
+                acc[3 * i] += rp.value.x;
+                acc[3 * i + 1] += rp.value.y;
+                acc[3 * i + 2] += rp.value.z;
+            }
+        }
+        c.add(cx.c);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// ReSTIR DI — source/optix/amadeus_restir.cu, include/amadeus/reservoir.cuh
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Sample {  // DirectLightingSample, reservoir.cuh:78-83
+    f3 Le{0, 0, 0}, normal{0, 0, 0}, point{0, 0, 0};
+    int type = 0;
+};
+
+struct Reservoir {  // reservoir.cuh:19-76
+    Sample data;
+    float w = 0.0f, W = 0.0f;
+    int M = 0;
+    f3 seed{0, 0, 0};
+    bool update(const Sample &s, float weight) {
+        w += weight;
+        float eta = rand_uniform(seed);
+        bool selected = (eta * w) < weight;
+        if (selected) data = s;
+        M++;
+        return selected;
+    }
+    void resample(float target) {
+        float d = target * (float)M;
+        W = (d > 0.0f) ? w / d : 0.0f;
+    }
+    void reset() {
+        w = 0.0f;
+        M = 0;
+        W = 0.0f;
+    }
+};
+
+// direct_unoccluded / direct_occluded, amadeus_common.cuh:380-421
+f3 direct_li(const SurfaceHit &sh, f3 Le, f3 normal, int type, f3 D, float d) {
+    f3 rho = brdf(sh, D, eDiffuse);
+    float geometric = fabsf(dot(sh.n, D));
+    if (type != 2) {
+        float ldot = fabsf(dot(normal, D));
+        geometric *= ldot / (d * d);
+    }
+    return rho * Le * geometric;
+}
+
+f3 direct_occ(Ctx &cx, const SurfaceHit &sh, f3 Le, f3 normal, int type, f3 D, float d, bool *occ_out = nullptr) {
+    cx.c.shadow++;
+    bool occ = occluded(cx.s, sh.x, D, d - kShadowEps, cx.c);
+    if (occ_out) *occ_out = occ;
+    if (occ) return mk(0, 0, 0);
+    return direct_li(sh, Le, normal, type, D, d);
+}
+
+// target / occluded_target, amadeus_restir.cu:148-178
+float target(const Sample &s, const SurfaceHit &hit) {
+    f3 D = s.point - hit.x;
+    float d = length(D);
+    f3 Li = direct_li(hit, s.Le, s.normal, s.type, D / d, d);
+    return Li.x + Li.y + Li.z;
+}
+
+float occluded_target(Ctx &cx, const Sample &s, const SurfaceHit &hit) {
+    f3 D = s.point - hit.x;
+    float d = length(D);
+    f3 Li = direct_occ(cx, hit, s.Le, s.normal, s.type, D / d, d);
+    return Li.x + Li.y + Li.z;
+}
+
+struct RestirRP {
+    f3 value{0, 0, 0}, position{0, 0, 0}, normal{0, 0, 0}, albedo{0, 0, 0}, seed{0, 0, 0};
+    int depth = 0;
+};
+
+}  // namespace
+
+struct orc_restir {
+    int w, h;
+    std::vector<Reservoir> buf[2];
+    int cur = 0;  // index of `current`; previous = 1 - cur
+    std::vector<Material> gmat;
+    std::vector<float> intermediate, auxiliary;  // float4 per pixel
+    float prev_pv[16];
+    bool have_prev_cam = false;
+};
+
+orc_restir *orc_restir_create(int width, int height, uint32_t user_seed) {
+    orc_restir *r = new orc_restir();
+    r->w = width;
+    r->h = height;
+    size_t n = (size_t)width * height;
+    for (int b = 0; b < 2; b++) {
+        r->buf[b].resize(n);
+        for (int y = 0; y < height; y++)
+            for (int x = 0; x < width; x++) r->buf[b][(size_t)y * width + x].seed = reservoir_seed(x, y, user_seed);
+    }
+    r->gmat.resize(n);
+    for (auto &m : r->gmat) m = Material{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), 0.0f, 1.0f, 0};
+    r->intermediate.assign(4 * n, 0.0f);
+    r->auxiliary.assign(4 * n, 0.0f);
+    std::fill(r->prev_pv, r->prev_pv + 16, 0.0f);
+    return r;
+}
+
+void orc_restir_destroy(orc_restir *r) { delete r; }
+
+namespace {
+
+// __closesthit__initial (amadeus_restir.cu:437-559) and the recursive bounces.
+void restir_trace(Ctx &cx, orc_restir *r, const orc_params *p, int index, f3 o, f3 d, RestirRP &rp) {
+    if (rp.depth > p->max_depth) {
+        cx.c.discarded++;
+        rp.value = mk(0, 0, 0);
+        return;
+    }
+    cx.c.closest++;
+    Hit h = closest(cx.s, o, d, kTraceTmax, 2, cx.c);
+    if (h.prim < 0) {
+        rp.value = mk(0, 0, 0);
+        return;
+    }
+    Surface sf;
+    if (!load_surface(cx.s, h, d, sf)) {
+        rp.value = mk(0, 0, 0);
+        return;
+    }
+    bool primary = rp.depth == 0;
+    const Material &material = sf.mat;
+    f3 n = sf.n;
+    f3 x = sf.x + ((material.type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
+    SurfaceHit sh{material, sf.entering, n, -d, x};
+    f3 direct = mk(0, 0, 0);
+    const orc_scene *s = cx.s;
+    if (primary) {
+        Reservoir &res = r->buf[r->cur][index];
+        res.reset();
+        unsigned total = (unsigned)s->lights.size();
+        for (int i = 0; i < p->candidates; i++) {
+            // sample_direct, amadeus_common.cuh:301-377 (triangle lights)
+            Sample smp;
+            float pdf = 0.0f;
+            if (total > 0) {
+                unsigned li = pick_light(rp.seed, total);
+                const TriLight &L = s->lights[li];
+                smp.point = sample_area_light(L, rp.seed);
+                smp.Le = L.intensity;
+                smp.normal = light_normal(L);
+                smp.type = 1;
+                pdf = 1.0f / (light_area(L) * (float)total);
+            } else {
+                rand_uniform(rp.seed);
+            }
+            f3 D = smp.point - sh.x;
+            float dd = length(D);
+            f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
+            float tgt = Li.x + Li.y + Li.z;
+            res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
+        }
+        Sample smp = res.data;
+        f3 D = smp.point - sh.x;
+        float dd = length(D);
+        bool occ;
+        f3 Li = direct_occ(cx, sh, smp.Le, smp.normal, smp.type, D / dd, dd, &occ);
+        res.resample(Li.x + Li.y + Li.z);
+        // amadeus_restir.cu:522-523 traces the identical shadow ray again
+        res.W *= 1.0f - (occ ? 1.0f : 0.0f);
+        r->gmat[index] = material;
+    } else {
+        direct = material.emission + Ld(cx, sh, rp.seed);
+    }
+    int out;
+    f3 wi;
+    float pdf;
+    f3 f = eval_bsdf(sh, wi, pdf, out, rp.seed);
+    f3 T = (pdf > 0.0f) ? f * fabsf(dot(wi, n)) / pdf : mk(0, 0, 0);
+    rp.depth++;
+    f3 indirect = mk(0, 0, 0);
+    if (pdf > 0.0f) {
+        restir_trace(cx, r, p, index, x, wi, rp);
+        indirect = rp.value;
+    }
+    rp.value = direct + T * indirect;
+    rp.position = x;
+    rp.normal = n;
+    rp.albedo = material.diffuse;
+}
+
+SurfaceHit gbuffer_hit(const orc_restir *r, const orc_camera *cam, const float *normal, const float *position,
+                       int index, f3 wo_of) {
+    (void)wo_of;
+    SurfaceHit hit;
+    hit.mat = r->gmat[index];
+    hit.entering = false;
+    hit.n = load3(normal + 4 * (size_t)index);
+    hit.x = load3(position + 4 * (size_t)index);
+    hit.wo = normalize(load3(cam->center) - hit.x);
+    return hit;
+}
+
+// reproject, amadeus_restir.cu:182-210 (ix/iy clamped to the image)
+int reproject(const orc_restir *r, const orc_params *p, int index, f3 pos) {
+    if (p->samples > 0) return index;
+    const float *m = r->prev_pv;
+    float v[4];
+    for (int k = 0; k < 4; k++)
+        v[k] = (m[0 + k] * pos.x + m[4 + k] * pos.y) + (m[8 + k] * pos.z + m[12 + k] * 1.0f);
+    float u = v[0] / v[3], vv = v[1] / v[3];
+    if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
+    u = (u + 1.0f) * 0.5f;
+    vv = (vv + 1.0f) * 0.5f;
+    int ix = (int)((double)(u * (float)p->width) + 0.5);
+    int iy = (int)((double)(vv * (float)p->height) + 0.5);
+    ix = std::min(std::max(ix, 0), p->width - 1);
+    iy = std::min(std::max(iy, 0), p->height - 1);
+    return iy * p->width + ix;
+}
+
+// sample_spatial_neighborhood, amadeus_restir.cu:295-314
+int spatial_neighbour(const orc_params *p, int index, f3 &seed, int radius) {
+    int width = p->width, height = p->height;
+    int x = index % width, y = index / width;
+    f3 off = rand_uniform_3f(seed);
+    float c, s;
+    sincos_turn(off.x, c, s);
+    float rr = (float)radius * sqrtf(off.y);
+    int x0 = (int)((float)x + rr * c);
+    int y0 = (int)((float)y + rr * s);
+    x0 = std::min(std::max(x0, 0), width - 1);
+    y0 = std::min(std::max(y0, 0), height - 1);
+    return x0 + y0 * width;
+}
+
+}  // namespace
+
+void orc_restir_frame(orc_restir *r, const orc_scene *s, const orc_camera *cam, const orc_params *p, float *color,
+                      float *normal, float *albedo, float *position, int nthreads, uint64_t *counters) {
+    const int W = p->width;
+    std::vector<Counters> all, per;
+    // Initial pass: __raygen__initial (amadeus_restir.cu:82-144)
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        for (int x = 0; x < W; x++) {
+            int index = y * W + x;
+            r->buf[r->cur][index].reset();
+            RestirRP rp;
+            rp.seed = pixel_seed((uint32_t)x, (uint32_t)y, (uint32_t)p->samples, p->user_seed);
+            f3 o, d;
+            make_ray(cam, p, (uint32_t)x, (uint32_t)y, rp.seed, false, o, d);
+            restir_trace(cx, r, p, index, o, d, rp);
+            float col[4] = {rp.value.x, rp.value.y, rp.value.z, 1.0f};
+            if (isnan3(rp.value)) {
+                col[0] = 1;
+                col[1] = 0;
+                col[2] = 1;
+            }
+            float *im = &r->intermediate[4 * (size_t)index];
+            for (int k = 0; k < 4; k++) im[k] = col[k];
+            float *ax = &r->auxiliary[4 * (size_t)index];
+            ax[0] = rp.seed.x;
+            ax[1] = rp.seed.y;
+            ax[2] = rp.seed.z;
+            ax[3] = 1.0f;
+            float nrm[4] = {rp.normal.x, rp.normal.y, rp.normal.z, 0.0f};
+            float alb[4] = {rp.albedo.x, rp.albedo.y, rp.albedo.z, 0.0f};
+            accumulate4(normal + 4 * (size_t)index, nrm, p);
+            accumulate4(albedo + 4 * (size_t)index, alb, p);
+            float *ps = position + 4 * (size_t)index;
+            ps[0] = rp.position.x;
+            ps[1] = rp.position.y;
+            ps[2] = rp.position.z;
+            ps[3] = 0.0f;
+        }
+        c.add(cx.c);
+    });
+    all.insert(all.end(), per.begin(), per.end());
+    // Temporal pass: __raygen__temporal (amadeus_restir.cu:213-291)
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        std::vector<Reservoir> &current = r->buf[r->cur];
+        std::vector<Reservoir> &previous = r->buf[1 - r->cur];
+        for (int x = 0; x < W; x++) {
+            int index = y * W + x;
+            Reservoir &cr = current[index];
+            if (cr.M <= 0) continue;
+            SurfaceHit hit = gbuffer_hit(r, cam, normal, position, index, mk(0, 0, 0));
+            int pindex = reproject(r, p, index, hit.x);
+            Reservoir *prev = pindex >= 0 ? &previous[pindex] : nullptr;
+            Reservoir merged;
+            merged.seed = load3(&r->auxiliary[4 * (size_t)index]);
+            float t_current = occluded_target(cx, cr.data, hit);
+            merged.update(cr.data, t_current * cr.W * (float)cr.M);
+            int Nn = cr.M;
+            if (prev && prev->M > 0) {
+                prev->M = std::min(prev->M, p->m_cap);
+                float t_prev = occluded_target(cx, prev->data, hit);
+                merged.update(prev->data, t_prev * prev->W * (float)prev->M);
+                Nn += (t_prev > 0.0f ? 1 : 0) * prev->M;
+            }
+            merged.M = Nn;
+            merged.resample(target(merged.data, hit));
+            cr = merged;
+        }
+        c.add(cx.c);
+    });
+    all.insert(all.end(), per.begin(), per.end());
+    // Spatial passes: __raygen__spatial (amadeus_restir.cu:317-434) with the
+    // host loop and swaps of restir.cuh:282-300.
+    for (int pass = 0; pass < p->spatial_passes; pass++) {
+        parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+            Ctx cx{s, Counters()};
+            const std::vector<Reservoir> &curbuf = r->buf[r->cur];
+            std::vector<Reservoir> &outbuf = r->buf[1 - r->cur];
+            const int S = p->neighbours;
+            for (int x = 0; x < W; x++) {
+                int index = y * W + x;
+                SurfaceHit hit = gbuffer_hit(r, cam, normal, position, index, mk(0, 0, 0));
+                Reservoir current = curbuf[index];
+                f3 direct = mk(0, 0, 0);
+                if (current.M > 0) {
+                    Reservoir merged;
+                    merged.seed = load3(&r->auxiliary[4 * (size_t)index]);
+                    float t_current = target(current.data, hit);
+                    merged.update(current.data, t_current * current.W * (float)current.M);
+                    SurfaceHit hits[16];
+                    int sizes[16];
+                    int count = 0;
+                    for (int i = 0; i < S; i++) {
+                        int index0 = spatial_neighbour(p, index, merged.seed, p->radius);
+                        const Reservoir &nb = curbuf[index0];
+                        float t_nb = occluded_target(cx, nb.data, hit);
+                        if (nb.M > 0) merged.update(nb.data, t_nb * nb.W * (float)nb.M);
+                        hits[i].mat = r->gmat[index0];
+                        hits[i].entering = nb.M > 0;
+                        hits[i].n = load3(normal + 4 * (size_t)index0);
+                        hits[i].wo = hit.wo;
+                        hits[i].x = load3(position + 4 * (size_t)index0);
+                        sizes[i] = nb.M;
+                        count += nb.M;
+                    }
+                    merged.M = current.M + count;
+                    int Z = current.M;
+                    for (int i = 0; i < S; i++) {
+                        float t_nb = hits[i].entering ? occluded_target(cx, merged.data, hits[i]) : 0.0f;
+                        if (t_nb > 0.0f) Z += sizes[i];
+                    }
+                    float t_merged = target(merged.data, hit);
+                    float denominator = t_merged * (float)Z;
+                    merged.W = (denominator > 0.0f) ? merged.w / denominator : 0.0f;
+                    const Sample &smp = merged.data;
+                    f3 D = smp.point - hit.x;
+                    float d = length(D);
+                    f3 lighting = direct_occ(cx, hit, smp.Le, smp.normal, smp.type, D / d, d);
+                    direct = hit.mat.emission + lighting * merged.W;
+                    outbuf[index] = merged;
+                } else {
+                    outbuf[index].reset();
+                }
+                const float *im = &r->intermediate[4 * (size_t)index];
+                float col[4] = {direct.x + im[0], direct.y + im[1], direct.z + im[2], 1.0f + im[3]};
+                if (std::isnan(col[0]) || std::isnan(col[1]) || std::isnan(col[2]) || std::isnan(col[3])) {
+                    col[0] = 1;
+                    col[1] = 0;
+                    col[2] = 1;
+                    col[3] = 1;
+                }
+                accumulate4(color + 4 * (size_t)index, col, p);
+            }
+            c.add(cx.c);
+        });
+        all.insert(all.end(), per.begin(), per.end());
+        r->cur = 1 - r->cur;  // swap(current, previous)
+    }
+    r->cur = 1 - r->cur;  // final swap (restir.cuh:300)
+    std::copy(cam->pv, cam->pv + 16, r->prev_pv);
+    r->have_prev_cam = true;
+    write_counters(all, counters);
+}
+
+void orc_restir_reservoirs(const orc_restir *r, int which, float *out) {
+    const std::vector<Reservoir> &b = r->buf[which == 0 ? 1 - r->cur : r->cur];
+    for (size_t i = 0; i < b.size(); i++) {
+        const Reservoir &q = b[i];
+        float *o = out + 16 * i;
+        o[0] = q.data.Le.x;
+        o[1] = q.data.Le.y;
+        o[2] = q.data.Le.z;
+        o[3] = q.data.normal.x;
+        o[4] = q.data.normal.y;
+        o[5] = q.data.normal.z;
+        o[6] = q.data.point.x;
+        o[7] = q.data.point.y;
+        o[8] = q.data.point.z;
+        o[9] = (float)q.data.type;
+        o[10] = q.w;
+        o[11] = q.W;
+        o[12] = (float)q.M;
+        o[13] = q.seed.x;
+        o[14] = q.seed.y;
+        o[15] = q.seed.z;
+    }
+}

@@ -1,0 +1,134 @@
+/*
+ * CPU oracle for the ray-traced rendering path — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++17 restatement of the reference renderer's algorithms:
+ *   - per-triangle AABBs           include/mesh.hpp:48-78
+ *   - "SAH" binary BVH build       source/bvh.cpp:32-210
+ *   - threaded serialisation       source/bvh.cpp:213-260
+ *   - stackless traversal          shaders/rt/modules/bvh.glsl:52-127,
+ *                                  shaders/rt/modules/bbox.glsl:8-66
+ *   - Möller–Trumbore              shaders/rt/modules/intersect.glsl:14-33
+ *   - pcg RNG, GGX/Glass BSDF      include/cuda/random.cuh, include/cuda/brdf.cuh
+ *   - NEE with triangle lights     source/optix/amadeus_common.cuh:190-464,
+ *                                  include/optix/lighting.cuh:96-174
+ *   - path tracer estimator        source/optix/amadeus_path_tracer.cu:26-228
+ *   - ReSTIR DI passes             source/optix/amadeus_restir.cu:82-559,
+ *                                  include/amadeus/reservoir.cuh:19-76,
+ *                                  include/amadeus/restir.cuh:223-304
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so, and only as the checker.  The product never links it.
+ *
+ * Pinning (what each function is checked against, tests/test_oracle_*.py):
+ *   pcg3d/pcg3f      JCGT 2020 integer hash recomputed in numpy + committed
+ *                    known-answer vectors.
+ *   bvh build        hand-computed trees for N = 1, 2, 3, 5; SAH cost closed
+ *                    form; invariants (2N-1 nodes, containment, threading).
+ *   traversal        brute force over all triangles, adversarial rays.
+ *   BSDF / lights    FP64 quadrature of the pdf, light-sample moments,
+ *                    closed-form irradiance of one small triangle light.
+ *   PT estimator     furnace-style closed forms on one-light scenes + the
+ *                    FP64 relMSE of a converged image (GPU tests).
+ *   ReSTIR DI        RIS unbiasedness with one light (closed form); reservoir
+ *                    stream known answers.
+ */
+#ifndef KRT_ORACLE_H_
+#define KRT_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scene: flattened world-space triangle soup.  mats: 12 floats per material
+ * {diffuse.rgb, specular.rgb, emission.rgb, roughness, refraction, type}. */
+typedef struct orc_scene orc_scene;
+
+orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv,
+                            const uint32_t *idx, const uint32_t *tri_mat, int nt,
+                            const float *mats, int nm,
+                            const float *bvh_nodes, int nnodes);
+void orc_scene_destroy(orc_scene *s);
+int orc_scene_light_count(const orc_scene *s);
+/* 12 floats per light: a, ab, ac, intensity (armada.cu:98-150 rule). */
+void orc_scene_lights(const orc_scene *s, float *out);
+
+void orc_pcg3d(const uint32_t *in, uint32_t *out, int n);
+/* n seeds of 3 floats, advanced in place by one pcg3f step each. */
+void orc_pcg3f(float *seeds, int n);
+void orc_sincos_turn(const float *u, float *c, float *s, int n);
+
+void orc_tri_boxes(const float *pos, const uint32_t *idx, int nt, float *boxes);
+/* Returns the node count (2n-1), writes 12 floats per node. */
+int orc_bvh_build(const float *boxes, int n, float *nodes);
+/* Leaf objects in preorder (the tie-break rank of a primitive). */
+int orc_bvh_leaf_order(const float *nodes, int nnodes, int32_t *order);
+int orc_bvh_depth(const float *nodes, int nnodes);
+
+/* rays: 8 floats {o.xyz, tmax, d.xyz, pad}; hits: 4 words {t, b1, b2, prim}
+ * mode 0 brute force, 1 GLSL stackless walk (exact reference box test),
+ * 2 stackless walk with a conservative box test.
+ * counters[0] += node visits, counters[1] += triangle tests. */
+void orc_trace(const orc_scene *s, const float *rays, int n, int mode, float *hits,
+               uint64_t *counters);
+void orc_occluded(const orc_scene *s, const float *rays, int n, int32_t *out);
+
+/* BSDF probes for tests: sh = 12 floats {n, wo, diffuse, specular}, mat
+ * params roughness; returns pdf / brdf / sample. */
+float orc_ggx_pdf(const float *n, const float *wi, const float *wo, const float *kd,
+                  const float *ks, float roughness);
+void orc_ggx_brdf(const float *n, const float *wi, const float *wo, const float *kd,
+                  const float *ks, float roughness, float *out);
+void orc_ggx_sample(const float *n, const float *wo, const float *kd, const float *ks,
+                    float roughness, float *seed, float *wi_out);
+void orc_light_sample(const float *light12, float *seed, float *point_out);
+
+typedef struct {
+    float center[3], U[3], V[3], W[3];
+    float pv[16]; /* projection*view of this camera, column-major */
+} orc_camera;
+
+typedef struct {
+    int width, height;
+    int max_depth;
+    int samples;    /* accumulated-frame counter (ArmadaLaunchInfo::samples) */
+    int accumulate;
+    uint32_t user_seed;
+    /* ReSTIR */
+    int candidates, m_cap, spatial_passes, neighbours, radius;
+} orc_params;
+
+/* counters: [0] closest rays, [1] shadow rays, [2] discarded depth-cut rays,
+ * [3] node visits, [4] triangle tests */
+void orc_render_pt(const orc_scene *s, const orc_camera *cam, const orc_params *p,
+                   float *color, float *normal, float *albedo, float *position,
+                   int nthreads, uint64_t *counters);
+
+/* Same as orc_render_pt for a list of pixels only (one sample each): used to
+ * spot-check large frames.  out: 4 floats per pixel (the un-accumulated
+ * color sample). */
+void orc_render_pt_pixels(const orc_scene *s, const orc_camera *cam, const orc_params *p,
+                          const int32_t *pix, int n, float *out);
+
+/* Closed-form-friendly FP64 accumulation of many frames into acc (3 doubles
+ * per pixel): renders frames [first, first+count) and adds each sample. */
+void orc_accumulate_pt_f64(const orc_scene *s, const orc_camera *cam, const orc_params *p,
+                           int first, int count, double *acc, int nthreads);
+
+typedef struct orc_restir orc_restir;
+orc_restir *orc_restir_create(int width, int height, uint32_t user_seed);
+void orc_restir_destroy(orc_restir *r);
+void orc_restir_frame(orc_restir *r, const orc_scene *s, const orc_camera *cam,
+                      const orc_params *p, float *color, float *normal, float *albedo,
+                      float *position, int nthreads, uint64_t *counters);
+/* 16 floats per pixel: Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz
+ * which: 0 = reservoir the next frame merges with (the last spatial output),
+ *        1 = the other buffer */
+void orc_restir_reservoirs(const orc_restir *r, int which, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

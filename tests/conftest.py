@@ -1,0 +1,29 @@
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libkrt.so on the device)")
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    import oracle
+
+    return oracle.lib()
+
+
+@pytest.fixture(scope="session")
+def krt_ctx():
+    import artemis_amd as A
+
+    ctx = A.Context(0)
+    yield ctx
+    ctx.close()

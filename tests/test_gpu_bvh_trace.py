@@ -1,0 +1,107 @@
+"""GPU BVH build and traversal vs the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from artemis_amd import scenes
+from tests.rays import adversarial_cornell, random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def degenerate_scene(kind):
+    s = scenes.random_soup(40, seed=11)
+    if kind == "duplicates":
+        s.positions[3:] = np.tile(s.positions[:3], (len(s.positions) // 3 - 1, 1))
+    elif kind == "points":
+        s.positions[:] = np.repeat(s.positions[::3], 3, axis=0)
+    elif kind == "collinear":
+        t = np.linspace(0, 1, len(s.positions), dtype=np.float32)
+        s.positions[:] = np.stack([t, 2 * t, -t], axis=1)
+    elif kind == "signed_zero":
+        s.positions[::2, 1] = -0.0
+        s.positions[1::2, 1] = 0.0
+    return s
+
+
+SCENES = {
+    "soup1": lambda: scenes.random_soup(1, seed=1),
+    "soup2": lambda: scenes.random_soup(2, seed=2),
+    "soup3": lambda: scenes.random_soup(3, seed=3),
+    "soup5": lambda: scenes.random_soup(5, seed=5),
+    "soup4096": lambda: scenes.random_soup(4096, seed=6),
+    "duplicates": lambda: degenerate_scene("duplicates"),
+    "points": lambda: degenerate_scene("points"),
+    "collinear": lambda: degenerate_scene("collinear"),
+    "signed_zero": lambda: degenerate_scene("signed_zero"),
+    "cornell": scenes.cornell,
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_build_bit_exact(krt_ctx, name):
+    s = SCENES[name]()
+    print(f"[bvh build] scene={name} tris={s.triangles}", flush=True)
+    ref = O.bvh_build(O.tri_boxes(s.positions, s.indices))
+    sc = krt_ctx.upload(s)
+    got = sc.serialized()
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(sc.leaf_order(), O.bvh_leaf_order(ref))
+    sc.free()
+
+
+def test_build_bit_exact_large(krt_ctx):
+    s = scenes.spheres(instances=16, slices=64, stacks=33, emissive=64)
+    print(f"[bvh build] spheres tris={s.triangles}", flush=True)
+    ref = O.bvh_build(O.tri_boxes(s.positions, s.indices))
+    sc = krt_ctx.upload(s)
+    np.testing.assert_array_equal(sc.serialized().view(np.uint32), ref.view(np.uint32))
+    sc.free()
+
+
+def compare_hits(got, ref):
+    g, r = got.view(np.uint32), ref.view(np.uint32)
+    bad = np.where((g != r).any(axis=1))[0]
+    assert len(bad) == 0, f"{len(bad)} mismatching rays, first {bad[:5]}: got {got[bad[:3]]} ref {ref[bad[:3]]}"
+
+
+@pytest.mark.parametrize("leaf_max", [1, 4, 8])
+def test_closest_hit_cornell(krt_ctx, leaf_max):
+    s = scenes.cornell()
+    sc = krt_ctx.upload(s, leaf_max=leaf_max)
+    osc = O.OracleScene(s, sc.serialized())
+    rays = np.concatenate([adversarial_cornell(0), random_rays(3000, [-0.95, 0.05, -0.95], [0.95, 1.95, 0.95], 1)])
+    print(f"[trace] cornell leaf_max={leaf_max} rays={len(rays)}", flush=True)
+    ref, _ = osc.trace(rays, mode=0)
+    compare_hits(sc.trace_closest(rays), ref)
+    occ_ref = osc.occluded(rays)
+    np.testing.assert_array_equal(sc.trace_occluded(rays), occ_ref)
+    sc.free()
+
+
+def test_closest_hit_soup(krt_ctx):
+    s = scenes.random_soup(20000, seed=21, extent=2.0, size=0.3)
+    sc = krt_ctx.upload(s)
+    osc = O.OracleScene(s, sc.serialized())
+    rays = random_rays(2000, -2.5, 2.5, 3)
+    print(f"[trace] soup20000 rays={len(rays)}", flush=True)
+    ref, _ = osc.trace(rays, mode=0)
+    compare_hits(sc.trace_closest(rays), ref)
+    sc.free()
+
+
+def test_trace_tmax_and_miss(krt_ctx):
+    s = scenes.cornell()
+    sc = krt_ctx.upload(s)
+    osc = O.OracleScene(s, sc.serialized())
+    rays = random_rays(1000, [-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], 5)
+    rays[:, 3] = np.linspace(1e-4, 0.5, len(rays))
+    ref, _ = osc.trace(rays, mode=0)
+    compare_hits(sc.trace_closest(rays), ref)
+    out = rays.copy()
+    out[:, 0:3] = [0.0, 1.0, 30.0]
+    out[:, 4:7] = [0.0, 0.0, 1.0]  # pointing away from the room
+    got = sc.trace_closest(out)
+    assert np.all(got[:, 3].view(np.int32) == -1)
+    sc.free()
