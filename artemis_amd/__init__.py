@@ -51,14 +51,14 @@ class Context:
         except Exception:
             pass
 
-    def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True) -> "Scene":
-        return Scene(self, scene, leaf_max=leaf_max, build=build)
+    def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True, layout: str = "sah") -> "Scene":
+        return Scene(self, scene, leaf_max=leaf_max, build=build, layout=layout)
 
 
 class Scene:
     """A resident scene and its acceleration structure (the "System" layer)."""
 
-    def __init__(self, ctx: Context, data: SceneData, leaf_max: int = 4, build: bool = True):
+    def __init__(self, ctx: Context, data: SceneData, leaf_max: int = 4, build: bool = True, layout: str = "sah"):
         self.ctx = ctx
         self.data = data
         lib = ctx._lib
@@ -74,11 +74,13 @@ class Scene:
         L.check(lib.krt_scene_upload(ctx.h, C.byref(d), C.byref(h)))
         self.h = h.value
         if build:
-            self.build(leaf_max)
+            self.build(leaf_max, layout)
 
-    def build(self, leaf_max: int = 4) -> None:
-        """Accelerator::update + build_tlas: GPU build of the bvh.cpp tree."""
-        L.check(self.ctx._lib.krt_accel_build(self.ctx.h, self.h, leaf_max))
+    def build(self, leaf_max: int = 4, layout: str = "sah") -> None:
+        """Accelerator::update + build_tlas: GPU build of the bvh.cpp tree plus
+        the traversal tree ("sah", or "reference" = collapse of the bvh.cpp tree)."""
+        code = {"sah": L.LAYOUT_SAH, "reference": L.LAYOUT_REFERENCE}[layout]
+        L.check(self.ctx._lib.krt_accel_build_layout(self.ctx.h, self.h, leaf_max, code))
 
     def stats(self) -> dict:
         s = L.BvhStats()
@@ -249,3 +251,13 @@ class ArmadaRTX:
 
     def sync(self) -> None:
         L.check(self.ctx._lib.krt_armada_sync(self.h))
+
+    def set_f64_accumulation(self, enable: bool) -> None:
+        """FP64 per-pixel sums of path-tracer samples (converged references)."""
+        L.check(self.ctx._lib.krt_armada_set_f64_accumulation(self.h, int(enable)))
+
+    def read_f64(self):
+        out = np.empty((self.height, self.width, 3), np.float64)
+        n = C.c_uint64()
+        L.check(self.ctx._lib.krt_armada_read_f64(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+        return out, n.value

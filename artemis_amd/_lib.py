@@ -13,6 +13,7 @@ KRT_ERR_BUDGET = 3
 KRT_ERR_STATE = 4
 
 OPT_NONE, OPT_INT, OPT_FLOAT, OPT_BOOL, OPT_STRING = 0, 1, 2, 3, 4
+LAYOUT_SAH, LAYOUT_REFERENCE = 0, 1
 
 fp = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
@@ -33,6 +34,7 @@ class BvhStats(C.Structure):
     _fields_ = [
         ("triangles", C.c_uint32), ("serialized_nodes", C.c_uint32), ("traversal_nodes", C.c_uint32),
         ("depth_bound", C.c_uint32), ("build_levels", C.c_uint32), ("build_ms", C.c_float),
+        ("layout", C.c_int),
     ]
 
 
@@ -68,6 +70,7 @@ _SIGS = {
     "krt_scene_light_count": (C.c_int, [vp, handle, u32p]),
     "krt_scene_lights": (C.c_int, [vp, handle, fp, C.c_size_t]),
     "krt_accel_build": (C.c_int, [vp, handle, C.c_int]),
+    "krt_accel_build_layout": (C.c_int, [vp, handle, C.c_int, C.c_int]),
     "krt_accel_stats": (C.c_int, [vp, handle, C.POINTER(BvhStats)]),
     "krt_accel_serialized": (C.c_int, [vp, handle, fp, C.c_size_t]),
     "krt_accel_leaf_order": (C.c_int, [vp, handle, i32p, C.c_size_t]),
@@ -89,6 +92,8 @@ _SIGS = {
     "krt_armada_upload": (C.c_int, [vp, C.c_int, fp]),
     "krt_armada_stats": (C.c_int, [vp, C.POINTER(FrameStats)]),
     "krt_armada_sync": (C.c_int, [vp]),
+    "krt_armada_set_f64_accumulation": (C.c_int, [vp, C.c_int]),
+    "krt_armada_read_f64": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "krt_load_attachment": (AttachmentRet, [vp, C.c_char_p]),
 }
 

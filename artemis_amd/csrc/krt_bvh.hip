@@ -431,24 +431,284 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int *flag, int *out, u
     if (tid == 1023) *total = s[1023];
 }
 
-__global__ void tri_pack_kernel(const float *pos, const uint32_t *idx, const int *leaf_order, uint32_t nt,
-                                float4 *tri) {
+// Triangles in traversal order: order[p] = primitive at position p; the
+// bvh.cpp rank of each primitive travels with it for the tie-break.
+__global__ void tri_pack_kernel(const float *pos, const uint32_t *idx, const int *order, const int *rank_of,
+                                uint32_t nt, float4 *tri) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nt) return;
-    int prim = leaf_order[r];
+    int prim = order[r];
     f3 v1 = ld3(pos + 3 * idx[3 * prim]), v2 = ld3(pos + 3 * idx[3 * prim + 1]), v3 = ld3(pos + 3 * idx[3 * prim + 2]);
     f3 e1 = v2 - v1, e2 = v3 - v1;  // as intersect.glsl:16-17 computes them
     tri[3 * r] = make_float4(v1.x, v1.y, v1.z, __int_as_float(prim));
-    tri[3 * r + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    tri[3 * r + 1] = make_float4(e1.x, e1.y, e1.z, __int_as_float(rank_of[prim]));
     tri[3 * r + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+}
+
+__global__ void invert_kernel(const int *order, uint32_t n, int *rank_of) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) rank_of[order[r]] = (int)r;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal tree: binned SAH (16 centroid bins on the axis of largest centroid
+// extent), level-synchronous, one workgroup per pending node.  Subtrees of at
+// most leaf_max primitives become leaf ranges.  Child boxes are the exact
+// unions of their primitives' boxes.
+// ---------------------------------------------------------------------------
+constexpr int SB = 16;
+
+struct SahSeg {
+    int start, count, node;
+};
+
+__device__ __forceinline__ uint32_t f2u(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float u2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+struct LdsBox {
+    uint32_t mn[3], mx[3];
+    __device__ void clear() {
+        for (int a = 0; a < 3; a++) {
+            mn[a] = 0xffffffffu;
+            mx[a] = 0u;
+        }
+    }
+    __device__ void add(const float *lo, const float *hi) {
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&mn[a], f2u(lo[a]));
+            atomicMax(&mx[a], f2u(hi[a]));
+        }
+    }
+};
+
+__device__ __forceinline__ float sa_of(const float *mn, const float *mx) {
+    float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    return 2.0f * (dx * dy + dy * dz + dx * dz);
+}
+
+__global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const int *perm_in, int *perm_out,
+                                                         int *perm_final, const float4 *bmin, const float4 *bmax,
+                                                         float4 *nodes, int *node_ctr, SahSeg *next,
+                                                         int *next_count, int leaf_max) {
+    const SahSeg sg = segs[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int n = sg.count;
+    __shared__ LdsBox s_cb;  // centroid bounds
+    __shared__ LdsBox s_bin[SB];
+    __shared__ int s_cnt[SB];
+    __shared__ LdsBox s_child[2];
+    __shared__ int s_nl, s_axis_split, s_median;
+    __shared__ float s_lo, s_scale;
+    __shared__ int s_axis;
+    __shared__ int s_wl[BB / 64];
+    if (tid == 0) s_cb.clear();
+    if (tid < SB) {
+        s_bin[tid].clear();
+        s_cnt[tid] = 0;
+    }
+    if (tid < 2) s_child[tid].clear();
+    __syncthreads();
+    // centroid bounds
+    {
+        float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (int j = tid; j < n; j += BB) {
+            int prim = perm_in[sg.start + j];
+            float4 a = bmin[prim], b = bmax[prim];
+            float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fminf(lo[k], c[k]);
+                hi[k] = fmaxf(hi[k], c[k]);
+            }
+        }
+        if (n > 0) s_cb.add(lo, hi);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float ext[3];
+        for (int k = 0; k < 3; k++) ext[k] = u2f(s_cb.mx[k]) - u2f(s_cb.mn[k]);
+        int ax = (ext[0] >= ext[1] && ext[0] >= ext[2]) ? 0 : (ext[1] >= ext[2] ? 1 : 2);
+        s_axis = ax;
+        s_lo = u2f(s_cb.mn[ax]);
+        s_scale = ext[ax] > 0.0f ? (float)SB / ext[ax] : 0.0f;
+    }
+    __syncthreads();
+    const int axis = s_axis;
+    const float clo = s_lo, scale = s_scale;
+    // bins
+    for (int j = tid; j < n; j += BB) {
+        int prim = perm_in[sg.start + j];
+        float4 a = bmin[prim], b = bmax[prim];
+        float c = 0.5f * (comp(a, axis) + comp(b, axis));
+        int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
+        atomicAdd(&s_cnt[bin], 1);
+        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+        s_bin[bin].add(lo, hi);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float pmn[SB][3], pmx[SB][3];
+        int pc[SB];
+        // prefix (left) sweep
+        float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        int c = 0;
+        for (int b = 0; b < SB; b++) {
+            c += s_cnt[b];
+            if (s_cnt[b])
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = fminf(mn[k], u2f(s_bin[b].mn[k]));
+                    mx[k] = fmaxf(mx[k], u2f(s_bin[b].mx[k]));
+                }
+            for (int k = 0; k < 3; k++) {
+                pmn[b][k] = mn[k];
+                pmx[b][k] = mx[k];
+            }
+            pc[b] = c;
+        }
+        float best = 3.4e38f;
+        int bsplit = -1;
+        float rmn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, rmx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        int rc = 0;
+        for (int b = SB - 1; b >= 1; b--) {
+            rc += s_cnt[b];
+            if (s_cnt[b])
+                for (int k = 0; k < 3; k++) {
+                    rmn[k] = fminf(rmn[k], u2f(s_bin[b].mn[k]));
+                    rmx[k] = fmaxf(rmx[k], u2f(s_bin[b].mx[k]));
+                }
+            int lc = pc[b - 1];
+            if (lc == 0 || rc == 0) continue;
+            float cost = (float)lc * sa_of(pmn[b - 1], pmx[b - 1]) + (float)rc * sa_of(rmn, rmx);
+            if (cost < best) {
+                best = cost;
+                bsplit = b - 1;
+            }
+        }
+        s_median = bsplit < 0;
+        s_axis_split = bsplit;
+        s_nl = bsplit < 0 ? n / 2 : pc[bsplit];
+    }
+    __syncthreads();
+    const bool median = s_median;
+    const int bsplit = s_axis_split;
+    const int nl = s_nl;
+    const int lane = tid & 63, wv = tid >> 6;
+    // stable partition (median split: by position)
+    int lbase = 0, rbase = 0;
+    for (int t0 = 0; t0 < n; t0 += BB) {
+        int j = t0 + tid;
+        bool valid = j < n;
+        int prim = valid ? perm_in[sg.start + j] : 0;
+        bool left = false;
+        if (valid) {
+            if (median) left = j < nl;
+            else {
+                float c = 0.5f * (comp(bmin[prim], axis) + comp(bmax[prim], axis));
+                int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
+                left = bin <= bsplit;
+            }
+        }
+        unsigned long long ball = __ballot(left);
+        int lpre = __popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wl[wv] = __popcll(ball);
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < BB / 64; w++) {
+            if (w < wv) woff += s_wl[w];
+            tot += s_wl[w];
+        }
+        if (valid) {
+            int lrank = woff + lpre;
+            if (left) perm_out[sg.start + lbase + lrank] = prim;
+            else perm_out[sg.start + nl + rbase + (tid - lrank)] = prim;
+        }
+        lbase += tot;
+        rbase += min(BB, n - t0) - tot;
+        __syncthreads();
+    }
+    // child boxes
+    {
+        float lo[2][3], hi[2][3];
+        for (int c = 0; c < 2; c++)
+            for (int k = 0; k < 3; k++) {
+                lo[c][k] = 3.4e38f;
+                hi[c][k] = -3.4e38f;
+            }
+        bool any[2] = {false, false};
+        for (int j = tid; j < n; j += BB) {
+            int prim = perm_out[sg.start + j];
+            int c = j < nl ? 0 : 1;
+            float4 a = bmin[prim], b = bmax[prim];
+            lo[c][0] = fminf(lo[c][0], a.x); lo[c][1] = fminf(lo[c][1], a.y); lo[c][2] = fminf(lo[c][2], a.z);
+            hi[c][0] = fmaxf(hi[c][0], b.x); hi[c][1] = fmaxf(hi[c][1], b.y); hi[c][2] = fmaxf(hi[c][2], b.z);
+            any[c] = true;
+        }
+        for (int c = 0; c < 2; c++)
+            if (any[c]) s_child[c].add(lo[c], hi[c]);
+    }
+    __syncthreads();
+    // leaves: copy their primitives to the final order
+    const int cnt[2] = {nl, n - nl};
+    const int st[2] = {sg.start, sg.start + nl};
+    for (int c = 0; c < 2; c++)
+        if (cnt[c] <= leaf_max)
+            for (int j = tid; j < cnt[c]; j += BB) perm_final[st[c] + j] = perm_out[st[c] + j];
+    if (tid == 0) {
+        int link[2];
+        for (int c = 0; c < 2; c++) {
+            if (cnt[c] <= leaf_max) {
+                link[c] = leaf_link(st[c], cnt[c]);
+            } else {
+                int id = atomicAdd(node_ctr, 1);
+                link[c] = id;
+                int slot = atomicAdd(next_count, 1);
+                next[slot] = SahSeg{st[c], cnt[c], id};
+            }
+        }
+        float mn[2][3], mx[2][3];
+        for (int c = 0; c < 2; c++)
+            for (int k = 0; k < 3; k++) {
+                mn[c][k] = u2f(s_child[c].mn[k]);
+                mx[c][k] = u2f(s_child[c].mx[k]);
+            }
+        float4 *nd = nodes + 4 * sg.node;
+        nd[0] = make_float4(mn[0][0], mx[0][0], mn[1][0], mx[1][0]);
+        nd[1] = make_float4(mn[0][1], mx[0][1], mn[1][1], mx[1][1]);
+        nd[2] = make_float4(mn[0][2], mx[0][2], mn[1][2], mx[1][2]);
+        nd[3] = make_float4(__int_as_float(link[0]), __int_as_float(link[1]), 0.0f, 0.0f);
+    }
+}
+
+// Root of a scene with at most leaf_max primitives: one node, both slots the
+// same leaf range.
+__global__ void sah_root_leaf_kernel(const float4 *bmin, const float4 *bmax, int n, int *perm_final,
+                                     float4 *nodes) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int i = 0; i < n; i++) {
+        perm_final[i] = i;
+        float4 a = bmin[i], b = bmax[i];
+        mn[0] = fminf(mn[0], a.x); mn[1] = fminf(mn[1], a.y); mn[2] = fminf(mn[2], a.z);
+        mx[0] = fmaxf(mx[0], b.x); mx[1] = fmaxf(mx[1], b.y); mx[2] = fmaxf(mx[2], b.z);
+    }
+    int l = leaf_link(0, n);
+    nodes[0] = make_float4(mn[0], mx[0], mn[0], mx[0]);
+    nodes[1] = make_float4(mn[1], mx[1], mn[1], mx[1]);
+    nodes[2] = make_float4(mn[2], mx[2], mn[2], mx[2]);
+    nodes[3] = make_float4(__int_as_float(l), __int_as_float(l), 0.0f, 0.0f);
 }
 
 inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-void accel_build(Context &ctx, Scene &s, int leaf_max) {
+void accel_build(Context &ctx, Scene &s, int leaf_max, int layout) {
     KRT_REQUIRE(leaf_max >= 1 && leaf_max <= 32, KRT_ERR_INVALID, "leaf_max must be in [1, 32]");
+    KRT_REQUIRE(layout == KRT_LAYOUT_SAH || layout == KRT_LAYOUT_REFERENCE, KRT_ERR_INVALID, "unknown BVH layout");
     KRT_REQUIRE(s.nt >= 1, KRT_ERR_STATE, "scene has no triangles");
     KRT_REQUIRE(s.nt < (1u << 26), KRT_ERR_INVALID, "too many triangles for the leaf link encoding");
     hipStream_t st = ctx.stream;
@@ -496,21 +756,68 @@ void accel_build(Context &ctx, Scene &s, int leaf_max) {
         }
     }
 
-    // traversal layout
-    DevBuf flag(sizeof(int) * nser), node_id(sizeof(int) * nser), total(sizeof(int));
-    mark_kernel<<<blocks(nser, 256), 256, 0, st>>>(A.ser.as<float4>(), leafcount.as<int>(), nser, leaf_max,
-                                                   flag.as<int>());
-    scan_kernel<<<1, 1024, 0, st>>>(flag.as<int>(), node_id.as<int>(), nser, total.as<int>());
+    // rank of each primitive in the bvh.cpp preorder (closest-hit tie-break)
+    DevBuf rank_of(sizeof(int) * n);
+    invert_kernel<<<blocks(n, 256), 256, 0, st>>>(A.leaf_order.as<int>(), n, rank_of.as<int>());
+    const int *order = A.leaf_order.as<int>();
     int nnodes = 0;
-    KRT_HIP(hipMemcpyAsync(&nnodes, total.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    KRT_HIP(hipStreamSynchronize(st));
-    A.nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes);
-    fill_nodes_kernel<<<blocks(nser, 256), 256, 0, st>>>(A.ser.as<float4>(), leafcount.as<int>(), rank.as<int>(),
-                                                         flag.as<int>(), node_id.as<int>(), nser, leaf_max,
-                                                         A.nodes.as<float4>());
+    uint32_t depth = levels + 2;
+    DevBuf perm_final;
+    if (layout == KRT_LAYOUT_REFERENCE) {
+        // collapse of the bvh.cpp tree itself
+        DevBuf flag(sizeof(int) * nser), node_id(sizeof(int) * nser), total(sizeof(int));
+        mark_kernel<<<blocks(nser, 256), 256, 0, st>>>(A.ser.as<float4>(), leafcount.as<int>(), nser, leaf_max,
+                                                       flag.as<int>());
+        scan_kernel<<<1, 1024, 0, st>>>(flag.as<int>(), node_id.as<int>(), nser, total.as<int>());
+        KRT_HIP(hipMemcpyAsync(&nnodes, total.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        KRT_HIP(hipStreamSynchronize(st));
+        A.nodes.alloc(sizeof(float4) * 4 * (size_t)nnodes);
+        fill_nodes_kernel<<<blocks(nser, 256), 256, 0, st>>>(A.ser.as<float4>(), leafcount.as<int>(),
+                                                             rank.as<int>(), flag.as<int>(), node_id.as<int>(), nser,
+                                                             leaf_max, A.nodes.as<float4>());
+    } else {
+        // binned-SAH traversal tree over the same primitives
+        perm_final.alloc(sizeof(int) * n);
+        A.nodes.alloc(sizeof(float4) * 4 * (size_t)(n + 1));
+        DevBuf sperm[2] = {DevBuf(sizeof(int) * n), DevBuf(sizeof(int) * n)};
+        DevBuf ssegs[2] = {DevBuf(sizeof(SahSeg) * (n / 2 + 2)), DevBuf(sizeof(SahSeg) * (n / 2 + 2))};
+        DevBuf nctr(sizeof(int));
+        if ((int)n <= leaf_max) {
+            sah_root_leaf_kernel<<<1, 64, 0, st>>>(bmin.as<float4>(), bmax.as<float4>(), (int)n,
+                                                   perm_final.as<int>(), A.nodes.as<float4>());
+            KRT_HIP(hipGetLastError());
+            nnodes = 1;
+            depth = 1;
+        } else {
+            iota_kernel<<<blocks(n, 256), 256, 0, st>>>(sperm[0].as<int>(), n);
+            SahSeg root{0, (int)n, 0};
+            int one = 1;
+            KRT_HIP(hipMemcpyAsync(ssegs[0].p, &root, sizeof(SahSeg), hipMemcpyHostToDevice, st));
+            KRT_HIP(hipMemcpyAsync(nctr.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
+            int nseg = 1;
+            uint32_t lv = 0;
+            while (nseg > 0) {
+                KRT_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+                sah_level_kernel<<<nseg, BB, 0, st>>>(ssegs[lv & 1].as<SahSeg>(), sperm[lv & 1].as<int>(),
+                                                      sperm[(lv + 1) & 1].as<int>(), perm_final.as<int>(),
+                                                      bmin.as<float4>(), bmax.as<float4>(), A.nodes.as<float4>(),
+                                                      nctr.as<int>(), ssegs[(lv + 1) & 1].as<SahSeg>(),
+                                                      cnt.as<int>(), leaf_max);
+                KRT_HIP(hipGetLastError());
+                KRT_HIP(hipMemcpyAsync(&nseg, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+                KRT_HIP(hipStreamSynchronize(st));
+                lv++;
+                KRT_REQUIRE(lv < 100000, KRT_ERR_HIP, "SAH build did not converge");
+            }
+            KRT_HIP(hipMemcpyAsync(&nnodes, nctr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            KRT_HIP(hipStreamSynchronize(st));
+            depth = lv + 1;
+        }
+        order = perm_final.as<int>();
+    }
     A.tri.alloc(sizeof(float4) * 3 * (size_t)n);
-    tri_pack_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), A.leaf_order.as<int>(),
-                                                    n, A.tri.as<float4>());
+    tri_pack_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), order,
+                                                    rank_of.as<int>(), n, A.tri.as<float4>());
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(e1, st));
     KRT_HIP(hipEventSynchronize(e1));
@@ -523,7 +830,8 @@ void accel_build(Context &ctx, Scene &s, int leaf_max) {
     A.nser = nser;
     A.nnodes = (uint32_t)nnodes;
     A.levels = levels;
-    A.depth_bound = levels + 2;
+    A.depth_bound = depth;
+    A.layout = layout;
     A.build_ms = ms;
     s.version++;
 }

@@ -100,6 +100,9 @@ struct krt_armada {
     uint64_t last_version = 0;
     DevBuf ctr;  // unsigned long long x 8: closest, shadow x3, nodes, tris
     krt_frame_stats stats{};
+    DevBuf acc64;  // FP64 sample sums (path tracer), enabled on request
+    bool f64 = false;
+    uint64_t f64_frames = 0;
 };
 
 struct krt_context {
@@ -254,12 +257,16 @@ int krt_scene_lights(krt_context *ctx, krt_handle scene, float *out, size_t nflo
     });
 }
 
-int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max) {
+int krt_accel_build_layout(krt_context *ctx, krt_handle scene, int leaf_max, int layout) {
     return guard([&] {
         KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
         KRT_HIP(hipSetDevice(ctx->impl.device));
-        accel_build(ctx->impl, ctx->impl.scene(scene), leaf_max);
+        accel_build(ctx->impl, ctx->impl.scene(scene), leaf_max, layout);
     });
+}
+
+int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max) {
+    return krt_accel_build_layout(ctx, scene, leaf_max, KRT_LAYOUT_SAH);
 }
 
 int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out) {
@@ -273,6 +280,7 @@ int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out) {
         out->depth_bound = A.depth_bound;
         out->build_levels = A.levels;
         out->build_ms = A.build_ms;
+        out->layout = A.layout;
     });
 }
 
@@ -473,7 +481,11 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         li.position = a->position.as<float4>();
         KRT_HIP(hipMemsetAsync(a->ctr.p, 0, a->ctr.bytes, C.stream));
         PassTimes times;
-        if (att->kind == "path_tracer") pt_render(C, s, li, att->history, a->ctr.as<unsigned long long>(), times);
+        if (att->kind == "path_tracer") {
+            pt_render(C, s, li, att->history, a->ctr.as<unsigned long long>(), times,
+                      a->f64 ? a->acc64.as<double>() : nullptr);
+            if (a->f64) a->f64_frames++;
+        }
         else restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
         unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         KRT_HIP(hipMemcpyAsync(c, a->ctr.p, sizeof(c), hipMemcpyDeviceToHost, C.stream));
@@ -532,6 +544,33 @@ int krt_armada_upload(krt_armada *a, int which, const float *in) {
         DevBuf &b = aov(a, which);
         KRT_HIP(hipMemcpyAsync(b.p, in, b.bytes, hipMemcpyHostToDevice, a->ctx->stream));
         KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+int krt_armada_set_f64_accumulation(krt_armada *a, int enable) {
+    return guard([&] {
+        KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
+        a->f64 = enable != 0;
+        a->f64_frames = 0;
+        if (a->f64) {
+            size_t bytes = sizeof(double) * 3 * (size_t)a->w * a->h;
+            if (a->acc64.bytes < bytes) a->acc64.alloc(bytes);
+            KRT_HIP(hipMemsetAsync(a->acc64.p, 0, bytes, a->ctx->stream));
+            KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+        } else {
+            a->acc64.release();
+        }
+    });
+}
+
+int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames) {
+    return guard([&] {
+        KRT_REQUIRE(a && out && frames, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(a->f64, KRT_ERR_STATE, "FP64 accumulation is not enabled");
+        KRT_HIP(hipMemcpyAsync(out, a->acc64.p, sizeof(double) * 3 * (size_t)a->w * a->h, hipMemcpyDeviceToHost,
+                               a->ctx->stream));
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+        *frames = a->f64_frames;
     });
 }
 

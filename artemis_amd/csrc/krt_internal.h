@@ -82,6 +82,7 @@ struct Accel {
     uint32_t depth_bound = 0;
     uint32_t levels = 0;
     float build_ms = 0.0f;
+    int layout = 0;
     DevBuf ser;        // float4 x 3 per node, bvh.cpp layout
     DevBuf leaf_order; // int per rank
     DevBuf tri;        // float4 x 3 per triangle in rank order: {v1, prim}, {e1}, {e2}
@@ -133,7 +134,7 @@ struct Context {
 };
 
 // Builders / kernels (defined in the .hip units)
-void accel_build(Context &ctx, Scene &s, int leaf_max);
+void accel_build(Context &ctx, Scene &s, int leaf_max, int layout);
 void upload_scene_derived(Context &ctx, Scene &s, const float *mats12, uint32_t nm);
 void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits);
 void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ);
@@ -175,7 +176,7 @@ struct PassTimes {
 };
 
 void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
-               PassTimes &times);
+               PassTimes &times, double *acc64 = nullptr);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times);

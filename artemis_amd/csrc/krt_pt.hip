@@ -26,6 +26,7 @@ struct PtArgs {
     float4 *hist;
     int *spill;
     unsigned long long *ctr;  // [0] closest, [1] shadow
+    double *acc64;            // optional FP64 running sum of samples (3 per pixel)
 };
 
 __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
@@ -87,6 +88,11 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         }
         float4 col = make_float4(value.x, value.y, value.z, 1.0f);
         if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+        else if (A.acc64) {  // converged-reference accumulator (FP64 only here)
+            A.acc64[3 * (size_t)pix] += (double)value.x;
+            A.acc64[3 * (size_t)pix + 1] += (double)value.y;
+            A.acc64[3 * (size_t)pix + 2] += (double)value.z;
+        }
         const bool acc = li.accumulate != 0;
         li.color[pix] = accumulate4(li.color[pix], col, acc, count);
         li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
 }  // namespace
 
 void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
-               PassTimes &times) {
+               PassTimes &times, double *acc64) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
     const uint32_t npix = li.width * li.height;
@@ -110,7 +116,7 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, un
     uint32_t nthreads = nb * kTraceBlock;
     size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
     if (history.bytes < need) history.alloc(need);
-    PtArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr};
+    PtArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, acc64};
     hipEvent_t e0, e1;
     KRT_HIP(hipEventCreate(&e0));
     KRT_HIP(hipEventCreate(&e1));

@@ -11,9 +11,12 @@
 // Result contract (the reference GLSL walk, shaders/rt/modules/bvh.glsl:52-127,
 // keeps the first leaf in preorder among equal t): the closest hit is the
 // lexicographic minimum of (t, rank) over triangles with 0 < t < tmax, with t,
-// b1, b2 from Möller–Trumbore exactly as intersect.glsl:14-33.  The box test is
-// conservative (relative slack, zero direction components treated as a
-// 1e-30 tilt), so traversal order never changes that minimum.
+// b1, b2 from Möller–Trumbore exactly as intersect.glsl:14-33, and rank the
+// triangle's leaf position in the preorder of the bvh.cpp tree (stored with
+// the triangle, so any traversal tree gives the same answer).  The box test is
+// conservative (relative slack; a zero direction component admits every t
+// while the origin lies inside that slab), so traversal order never changes
+// that minimum.
 #pragma once
 
 #include "krt_device.h"
@@ -46,20 +49,39 @@ struct TravStats {
 };
 
 KRT_DEV f3 safe_inv(f3 d) {
-    float x = fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x);
-    float y = fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y);
-    float z = fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z);
+    float x = d.x != 0.0f ? d.x : copysignf(1e-30f, d.x);
+    float y = d.y != 0.0f ? d.y : copysignf(1e-30f, d.y);
+    float z = d.z != 0.0f ? d.z : copysignf(1e-30f, d.z);
     return mk(1.0f / x, 1.0f / y, 1.0f / z);
 }
 
+// Per-ray mask of zero direction components (bit a set when d[a] == 0).
+KRT_DEV int zero_mask(f3 d) { return (d.x == 0.0f ? 1 : 0) | (d.y == 0.0f ? 2 : 0) | (d.z == 0.0f ? 4 : 0); }
+
+// A ray with d[a] == 0 stays in the plane o[a]: the slab of that axis admits
+// every t when lo <= o[a] <= hi and none otherwise.
+KRT_DEV void zero_axis(float lo, float hi, float o, float &tn_a, float &tf_a) {
+    bool in = o >= lo && o <= hi;
+    tn_a = in ? -3.4e38f : 3.4e38f;
+    tf_a = in ? 3.4e38f : -3.4e38f;
+}
+
 // Conservative slab test of one child box (lo/hi per axis).
-KRT_DEV bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, f3 o, f3 inv, float tlim,
-                  float &tnear) {
+KRT_DEV bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, f3 o, f3 inv, int zmask,
+                  float tlim, float &tnear) {
     float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
     float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
     float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    float nx = fminf(tx0, tx1), fx = fmaxf(tx0, tx1);
+    float ny = fminf(ty0, ty1), fy = fmaxf(ty0, ty1);
+    float nz = fminf(tz0, tz1), fz = fmaxf(tz0, tz1);
+    if (zmask) {  // rare: axis-parallel rays
+        if (zmask & 1) zero_axis(lox, hix, o.x, nx, fx);
+        if (zmask & 2) zero_axis(loy, hiy, o.y, ny, fy);
+        if (zmask & 4) zero_axis(loz, hiz, o.z, nz, fz);
+    }
+    float tn = fmaxf(fmaxf(nx, ny), nz);
+    float tf = fminf(fminf(fx, fy), fz);
     float tn_s = tn - (fabsf(tn) * 1e-5f + 1e-6f);
     float tf_s = tf + (fabsf(tf) * 1e-5f + 1e-6f);
     tnear = tn;
@@ -68,9 +90,10 @@ KRT_DEV bool slab(float lox, float hix, float loy, float hiy, float loz, float h
 
 // Möller–Trumbore on the packed triangle (intersect.glsl:14-33, e1/e2 precomputed
 // with the same subtraction).  Returns t or -1.
-KRT_DEV float mt_test(const float4 *tri, int r, f3 o, f3 d, float &b1, float &b2, int &prim) {
+KRT_DEV float mt_test(const float4 *tri, int r, f3 o, f3 d, float &b1, float &b2, int &prim, int &rank) {
     float4 A = tri[3 * r], B = tri[3 * r + 1], C = tri[3 * r + 2];
     prim = __float_as_int(A.w);
+    rank = __float_as_int(B.w);
     f3 v1 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
     f3 s1 = cross(d, e2);
     float divisor = dot(s1, e1);
@@ -91,6 +114,7 @@ template <bool ANY>
 KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict__ tri, f3 o, f3 d, float tmax,
                       const TraceStack &stk, Hit &hit, TravStats &ts) {
     const f3 inv = safe_inv(d);
+    const int zmask = zero_mask(d);
     float best = tmax;
     bool found = false;
     hit.prim = -1;
@@ -105,8 +129,8 @@ KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict
             float4 X = nd[0], Y = nd[1], Z = nd[2], L = nd[3];
             ts.nodes++;
             float t0, t1;
-            bool h0 = slab(X.x, X.y, Y.x, Y.y, Z.x, Z.y, o, inv, best, t0);
-            bool h1 = slab(X.z, X.w, Y.z, Y.w, Z.z, Z.w, o, inv, best, t1);
+            bool h0 = slab(X.x, X.y, Y.x, Y.y, Z.x, Z.y, o, inv, zmask, best, t0);
+            bool h1 = slab(X.z, X.w, Y.z, Y.w, Z.z, Z.w, o, inv, zmask, best, t1);
             int l0 = __float_as_int(L.x), l1 = __float_as_int(L.y);
             if (h0 && h1) {
                 bool first0 = t0 <= t1;
@@ -124,10 +148,9 @@ KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict
             const int first = cur & 0x03FFFFFF;
             const int cnt = ((cur >> 26) & 31) + 1;
             for (int k = 0; k < cnt; k++) {
-                int r = first + k;
                 float b1 = 0.0f, b2 = 0.0f;
-                int prim;
-                float t = mt_test(tri, r, o, d, b1, b2, prim);
+                int prim, r;
+                float t = mt_test(tri, first + k, o, d, b1, b2, prim, r);
                 ts.tris++;
                 if (ANY) {
                     if (t > 0.0f && t < tmax) return true;

@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--layout", choices=["sah", "reference"], default="sah",
+                    help="traversal tree (results are identical; 'reference' collapses the bvh.cpp tree)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,7 +108,7 @@ def main():
     scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
     ctx = Context(0)
     log(f"[bench] uploading {scene.triangles} triangles + GPU BVH build")
-    sc = ctx.upload(scene)
+    sc = ctx.upload(scene, layout=args.layout)
     bst = sc.stats()
     log(f"[bench] bvh: {bst}")
     cam = look_at(**scene.camera, aspect=W / H)
@@ -176,7 +178,8 @@ def main():
         "config": {"workload": f"{args.workload} {W}x{H} spheres 256x4096 tris (1,048,576), "
                                + ("32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), depth 2"
                                   if args.workload == "restir_di" else "1 spp/frame, max depth 5"),
-                   "width": W, "height": H, "triangles": int(scene.triangles), "parallelism": "single GPU"},
+                   "width": W, "height": H, "triangles": int(scene.triangles), "traversal_tree": args.layout,
+                   "parallelism": "single GPU"},
         "mrays_per_s": rays / dt / 1e6,
         "rays_per_sample": total_rays / npix,
         "p50_frame_ms": float(np.percentile(lat, 50) * 1e3),

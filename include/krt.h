@@ -82,12 +82,21 @@ typedef struct {
     uint32_t triangles;
     uint32_t serialized_nodes; /* 2N-1 */
     uint32_t traversal_nodes;
-    uint32_t depth_bound;  /* upper bound of the binary tree depth */
+    uint32_t depth_bound;  /* upper bound of the traversal tree depth */
     uint32_t build_levels; /* level-synchronous build passes */
     float build_ms;        /* GPU time of the build (HIP events) */
+    int layout;            /* KRT_LAYOUT_* of the traversal tree */
 } krt_bvh_stats;
 
-int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max);
+enum {
+    KRT_LAYOUT_SAH = 0,      /* binned-SAH traversal tree (default) */
+    KRT_LAYOUT_REFERENCE = 1 /* collapse of the bvh.cpp tree itself */
+};
+/* Builds the bvh.cpp tree (serialised layout and preorder ranks) and the
+ * traversal tree of `layout`.  Either layout gives bit-identical query results:
+ * every triangle carries its bvh.cpp preorder rank for the tie-break. */
+int krt_accel_build_layout(krt_context *ctx, krt_handle scene, int leaf_max, int layout);
+int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max); /* layout SAH */
 int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out);
 /* bvh.cpp serialised layout, 12 floats (3 x vec4) per node */
 int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t nfloats);
@@ -160,6 +169,13 @@ int krt_armada_buffers(krt_armada *a, krt_buffers *out);
 int krt_armada_readback(krt_armada *a, int which, float *out);
 int krt_armada_upload(krt_armada *a, int which, const float *in);
 int krt_armada_stats(krt_armada *a, krt_frame_stats *out);
+/* Converged-reference accumulator (no reference counterpart; the reference
+ * accumulates only the FP32 running mean of accumulate(), amadeus_path_tracer.cu:
+ * 59-70): while enabled, every path-tracer sample is also added to an FP64 sum
+ * per pixel.  Enabling (again) clears the sums.  read: 3 doubles per pixel and
+ * the number of frames summed. */
+int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
+int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames);
 int krt_armada_sync(krt_armada *a);
 
 /* Plugin entry of source/amadeus/naive_path_tracer.cu:280-344

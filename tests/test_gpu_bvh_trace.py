@@ -43,7 +43,7 @@ def test_build_bit_exact(krt_ctx, name):
     s = SCENES[name]()
     print(f"[bvh build] scene={name} tris={s.triangles}", flush=True)
     ref = O.bvh_build(O.tri_boxes(s.positions, s.indices))
-    sc = krt_ctx.upload(s)
+    sc = krt_ctx.upload(s, layout="reference")
     got = sc.serialized()
     assert got.shape == ref.shape
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
@@ -66,13 +66,15 @@ def compare_hits(got, ref):
     assert len(bad) == 0, f"{len(bad)} mismatching rays, first {bad[:5]}: got {got[bad[:3]]} ref {ref[bad[:3]]}"
 
 
-@pytest.mark.parametrize("leaf_max", [1, 4, 8])
-def test_closest_hit_cornell(krt_ctx, leaf_max):
+@pytest.mark.parametrize("layout,leaf_max", [("reference", 1), ("reference", 4), ("sah", 1), ("sah", 4),
+                                             ("sah", 8)])
+def test_closest_hit_cornell(krt_ctx, layout, leaf_max):
     s = scenes.cornell()
-    sc = krt_ctx.upload(s, leaf_max=leaf_max)
+    sc = krt_ctx.upload(s, leaf_max=leaf_max, layout=layout)
+    assert sc.stats()["layout"] == (0 if layout == "sah" else 1)
     osc = O.OracleScene(s, sc.serialized())
     rays = np.concatenate([adversarial_cornell(0), random_rays(3000, [-0.95, 0.05, -0.95], [0.95, 1.95, 0.95], 1)])
-    print(f"[trace] cornell leaf_max={leaf_max} rays={len(rays)}", flush=True)
+    print(f"[trace] cornell layout={layout} leaf_max={leaf_max} rays={len(rays)}", flush=True)
     ref, _ = osc.trace(rays, mode=0)
     compare_hits(sc.trace_closest(rays), ref)
     occ_ref = osc.occluded(rays)
@@ -80,15 +82,30 @@ def test_closest_hit_cornell(krt_ctx, leaf_max):
     sc.free()
 
 
-def test_closest_hit_soup(krt_ctx):
+@pytest.mark.parametrize("layout", ["reference", "sah"])
+def test_closest_hit_soup(krt_ctx, layout):
     s = scenes.random_soup(20000, seed=21, extent=2.0, size=0.3)
-    sc = krt_ctx.upload(s)
+    sc = krt_ctx.upload(s, layout=layout)
     osc = O.OracleScene(s, sc.serialized())
     rays = random_rays(2000, -2.5, 2.5, 3)
-    print(f"[trace] soup20000 rays={len(rays)}", flush=True)
+    print(f"[trace] soup20000 layout={layout} rays={len(rays)}", flush=True)
     ref, _ = osc.trace(rays, mode=0)
     compare_hits(sc.trace_closest(rays), ref)
     sc.free()
+
+
+@pytest.mark.parametrize("name", ["soup1", "soup2", "soup5", "points", "duplicates"])
+def test_small_and_degenerate_scenes_trace(krt_ctx, name):
+    s = SCENES[name]()
+    for layout in ("reference", "sah"):
+        sc = krt_ctx.upload(s, layout=layout)
+        osc = O.OracleScene(s, sc.serialized())
+        rays = random_rays(500, -1.2, 1.2, 8)
+        print(f"[trace] {name} layout={layout}", flush=True)
+        ref, _ = osc.trace(rays, mode=0)
+        compare_hits(sc.trace_closest(rays), ref)
+        np.testing.assert_array_equal(sc.trace_occluded(rays), osc.occluded(rays))
+        sc.free()
 
 
 def test_trace_tmax_and_miss(krt_ctx):
