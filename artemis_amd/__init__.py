@@ -9,7 +9,7 @@ library raises.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+import weakref
 
 import numpy as np
 
@@ -26,16 +26,23 @@ def _f(a):
 
 
 class Context:
-    """krt_create / krt_destroy: one HIP device, one stream."""
+    """krt_create / krt_destroy: one HIP device, one stream.  Objects created
+    on a context are released before it (close() releases them first)."""
 
     def __init__(self, device: int = 0):
         self._lib = L.lib()
         h = C.c_void_p()
         L.check(self._lib.krt_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch._release()
             self._lib.krt_destroy(self.h)
             self.h = None
 
@@ -50,6 +57,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def camera_rays(self, camera: CameraFrame, width: int, height: int, jitter: bool = False, sample: int = 0,
+                    user_seed: int = 0) -> np.ndarray:
+        """make_ray on the device for every pixel (8 floats per ray)."""
+        out = np.empty((width * height, 8), np.float32)
+        cam = camera.to_ctypes()
+        L.check(self._lib.krt_camera_rays(self.h, C.byref(cam), width, height, int(jitter), sample, user_seed,
+                                          _f(out)))
+        return out
 
     def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True, layout: str = "sah") -> "Scene":
         return Scene(self, scene, leaf_max=leaf_max, build=build, layout=layout)
@@ -73,6 +89,7 @@ class Scene:
         h = L.handle()
         L.check(lib.krt_scene_upload(ctx.h, C.byref(d), C.byref(h)))
         self.h = h.value
+        ctx._adopt(self)
         if build:
             self.build(leaf_max, layout)
 
@@ -120,9 +137,12 @@ class Scene:
         return out
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             L.check(self.ctx._lib.krt_scene_free(self.ctx.h, self.h))
-            self.h = 0
+        self.h = 0
+
+    def _release(self):
+        self.h = 0  # freed with the context
 
 
 class _Attachment:
@@ -133,11 +153,19 @@ class _Attachment:
         h = C.c_void_p()
         L.check(ctx._lib.krt_attachment_create(ctx.h, self.kind.encode(), C.byref(h)))
         self.h = h
+        self._users = weakref.WeakSet()
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            for u in list(self._users):
+                u._release()
+            self.ctx._lib.krt_attachment_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if self.h:
-                self.ctx._lib.krt_attachment_destroy(self.h)
+            self._release()
         except Exception:
             pass
 
@@ -167,11 +195,16 @@ class ArmadaRTX:
         L.check(ctx._lib.krt_armada_create(ctx.h, width, height, C.byref(h)))
         self.h = h
         self._attached = {}
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            self.ctx._lib.krt_armada_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if self.h:
-                self.ctx._lib.krt_armada_destroy(self.h)
+            self._release()
         except Exception:
             pass
 
@@ -184,6 +217,7 @@ class ArmadaRTX:
     def attach(self, name: str, attachment: _Attachment) -> None:
         L.check(self.ctx._lib.krt_armada_attach(self.h, name.encode(), attachment.h))
         self._attached[name] = attachment
+        attachment._users.add(self)
 
     def attachments(self):
         return sorted(self._attached)
@@ -251,6 +285,12 @@ class ArmadaRTX:
 
     def sync(self) -> None:
         L.check(self.ctx._lib.krt_armada_sync(self.h))
+
+    def reservoirs(self, which: int = 0) -> np.ndarray:
+        """ReSTIR reservoirs, 16 floats per pixel (see krt_armada_read_reservoirs)."""
+        out = np.empty((self.height, self.width, 16), np.float32)
+        L.check(self.ctx._lib.krt_armada_read_reservoirs(self.h, which, _f(out)))
+        return out
 
     def set_f64_accumulation(self, enable: bool) -> None:
         """FP64 per-pixel sums of path-tracer samples (converged references)."""

@@ -77,6 +77,8 @@ _SIGS = {
     "krt_trace_closest": (C.c_int, [vp, handle, fp, C.c_uint32, fp]),
     "krt_trace_occluded": (C.c_int, [vp, handle, fp, C.c_uint32, i32p]),
     "krt_trace_closest_device": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
+    "krt_camera_rays": (C.c_int, [vp, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
+                                  fp]),
     "krt_armada_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
     "krt_armada_destroy": (None, [vp]),
     "krt_attachment_create": (C.c_int, [vp, C.c_char_p, C.POINTER(vp)]),
@@ -93,6 +95,7 @@ _SIGS = {
     "krt_armada_stats": (C.c_int, [vp, C.POINTER(FrameStats)]),
     "krt_armada_sync": (C.c_int, [vp]),
     "krt_armada_set_f64_accumulation": (C.c_int, [vp, C.c_int]),
+    "krt_armada_read_reservoirs": (C.c_int, [vp, C.c_int, fp]),
     "krt_armada_read_f64": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "krt_load_attachment": (AttachmentRet, [vp, C.c_char_p]),
 }

@@ -337,6 +337,28 @@ int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *ray
     });
 }
 
+int krt_camera_rays(krt_context *ctx, const krt_camera *cam, uint32_t width, uint32_t height, int jitter,
+                    int sample, uint32_t user_seed, float *rays) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && cam && rays, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(width > 0 && height > 0, KRT_ERR_INVALID, "zero extent");
+        Context &C = ctx->impl;
+        LaunchInfo li{};
+        li.width = width;
+        li.height = height;
+        std::memcpy(li.center, cam->center, sizeof(li.center));
+        std::memcpy(li.U, cam->U, sizeof(li.U));
+        std::memcpy(li.V, cam->V, sizeof(li.V));
+        std::memcpy(li.W, cam->W, sizeof(li.W));
+        li.samples = sample;
+        li.user_seed = user_seed;
+        DevBuf r(sizeof(float) * 8 * (size_t)width * height);
+        camera_rays_dev(C, li, jitter, r.as<float>());
+        KRT_HIP(hipMemcpyAsync(rays, r.p, r.bytes, hipMemcpyDeviceToHost, C.stream));
+        KRT_HIP(hipStreamSynchronize(C.stream));
+    });
+}
+
 // ------------------------------------------------------------------- armada
 int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out) {
     return guard([&] {
@@ -369,9 +391,7 @@ int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_arm
 void krt_armada_destroy(krt_armada *a) {
     if (!a) return;
     (void)hipStreamSynchronize(a->ctx->stream);
-    for (auto &kv : a->attachments)
-        if (kv.second && kv.second->loaded) kv.second->unload();
-    delete a;
+    delete a;  // attachments keep their own buffers until krt_attachment_destroy
 }
 
 int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **out) {
@@ -544,6 +564,16 @@ int krt_armada_upload(krt_armada *a, int which, const float *in) {
         DevBuf &b = aov(a, which);
         KRT_HIP(hipMemcpyAsync(b.p, in, b.bytes, hipMemcpyHostToDevice, a->ctx->stream));
         KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+int krt_armada_read_reservoirs(krt_armada *a, int which, float *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(!a->previous.empty(), KRT_ERR_STATE, "nothing rendered yet");
+        krt_attachment *att = a->attachments.at(a->previous);
+        KRT_REQUIRE(att->kind == "restir" && att->loaded, KRT_ERR_STATE, "active attachment has no reservoirs");
+        restir_read(*a->ctx, att->rst, which, out);
     });
 }
 

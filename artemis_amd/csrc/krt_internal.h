@@ -178,6 +178,8 @@ struct PassTimes {
 void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
                PassTimes &times, double *acc64 = nullptr);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
+void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
+void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times);
 

@@ -422,7 +422,32 @@ __global__ void seed_kernel(Planes a, Planes b, uint32_t w, uint32_t h, uint32_t
     b.store(i, r);
 }
 
+// Reservoir planes -> 16 floats per pixel {Le, normal, point, type, w, W, M, seed}
+__global__ void unpack_kernel(Planes a, uint32_t n, float *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Resv r = a.load(i);
+    float *o = out + 16 * (size_t)i;
+    o[0] = r.data.Le.x; o[1] = r.data.Le.y; o[2] = r.data.Le.z;
+    o[3] = r.data.normal.x; o[4] = r.data.normal.y; o[5] = r.data.normal.z;
+    o[6] = r.data.point.x; o[7] = r.data.point.y; o[8] = r.data.point.z;
+    o[9] = (float)r.data.type;
+    o[10] = r.w; o[11] = r.W; o[12] = (float)r.M;
+    o[13] = r.seed.x; o[14] = r.seed.y; o[15] = r.seed.z;
+}
+
 }  // namespace
+
+void restir_read(Context &ctx, RestirState &st, int which, float *host_out) {
+    KRT_REQUIRE(st.w > 0, KRT_ERR_STATE, "ReSTIR buffers not loaded");
+    uint32_t n = st.w * st.h;
+    int b = which == 0 ? 1 - st.cur : st.cur;  // 0: the reservoirs the next frame merges with
+    DevBuf tmp(sizeof(float) * 16 * (size_t)n);
+    unpack_kernel<<<(n + 255) / 256, 256, 0, ctx.stream>>>(Planes{st.res[b].as<float4>(), n}, n, tmp.as<float>());
+    KRT_HIP(hipGetLastError());
+    KRT_HIP(hipMemcpyAsync(host_out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost, ctx.stream));
+    KRT_HIP(hipStreamSynchronize(ctx.stream));
+}
 
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed) {
     size_t n = (size_t)w * h;

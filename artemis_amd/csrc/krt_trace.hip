@@ -4,6 +4,7 @@
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
+#include "krt_shade.h"
 #include "krt_traverse.h"
 
 namespace krt {
@@ -34,7 +35,24 @@ __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__rest
     }
 }
 
+__global__ void camera_rays_kernel(LaunchInfo li, int jitter, float4 *rays) {
+    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= li.width * li.height) return;
+    uint32_t px = pix % li.width, py = pix / li.width;
+    f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+    f3 o, d;
+    make_ray(li, px, py, seed, jitter != 0, o, d);
+    rays[2 * pix] = make_float4(o.x, o.y, o.z, kTraceTmax);
+    rays[2 * pix + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+}
+
 }  // namespace
+
+void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays) {
+    uint32_t n = li.width * li.height;
+    camera_rays_kernel<<<(n + 255) / 256, 256, 0, ctx.stream>>>(li, jitter, (float4 *)rays);
+    KRT_HIP(hipGetLastError());
+}
 
 int *spill_buffer(Context &ctx, Scene &s, uint32_t threads) {
     size_t depth = s.accel.depth_bound > (uint32_t)kLdsDepth ? s.accel.depth_bound - kLdsDepth + 1 : 1;

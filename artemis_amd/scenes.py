@@ -159,7 +159,7 @@ def icosphere(level: int):
     return np.array(verts), np.array(faces)
 
 
-def cornell(seed: int = 1) -> SceneData:
+def cornell(seed: int = 1, light_half: float = 0.25, light_emission: float = 15.0) -> SceneData:
     rng = SplitMix64(seed)
 
     def rand_mat():
@@ -184,8 +184,9 @@ def cornell(seed: int = 1) -> SceneData:
     v, f = icosphere(4)
     r, c = 0.28, np.array([0.35, 0.88, 0.25])
     b.add(c + r * v, v, f, len(mats) - 1)
-    mats.append(material((0.8, 0.8, 0.8), 1.0, emission=(15.0, 15.0, 15.0)))
-    b.add(np.array([(-0.25, 1.99, -0.25), (0.25, 1.99, -0.25), (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25)]),
+    mats.append(material((0.8, 0.8, 0.8), 1.0, emission=(light_emission,) * 3))
+    h = light_half
+    b.add(np.array([(-h, 1.99, -h), (h, 1.99, -h), (h, 1.99, h), (-h, 1.99, h)]),
           np.tile((0.0, -1.0, 0.0), (4, 1)), np.array([(0, 2, 1), (0, 3, 2)]), len(mats) - 1)
     cam = dict(eye=(0.0, 1.0, 3.5), lookat=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
     return b.build(mats, camera=cam, name="cornell")
@@ -283,3 +284,21 @@ def random_soup(n: int, seed: int = 3, extent: float = 1.0, size: float = 0.2) -
     mats = [material((0.5, 0.5, 0.5), 0.5)]
     return SceneData(pos.astype(np.float32), nrm.astype(np.float32), idx.astype(np.uint32),
                      np.zeros(n, np.uint32), np.asarray(mats, np.float32), name=f"soup{n}")
+
+
+def sky_room(seed: int = 4) -> SceneData:
+    """Low-variance test scene: a 10 m floor, two boxes and a large emissive
+    ceiling 4 m up (no light closer than ~2 m to any lit surface)."""
+    rng = SplitMix64(seed)
+    mats, b = [], _Builder()
+    mats.append(material((0.6, 0.6, 0.6), 0.8))
+    b.add(*grid_quad((-5, 0, -5), (10, 0, 0), (0, 0, 10), 4, (0, 1, 0)), 0)
+    for center, half in (((-1.0, 0.7, -0.5), (0.6, 0.7, 0.6)), ((1.2, 0.5, 0.6), (0.5, 0.5, 0.5))):
+        mats.append(material([rng.uniform(0.3, 0.8) for _ in range(3)], rng.uniform(0.3, 1.0)))
+        for face in box_mesh(center, half, 2):
+            b.add(*face, len(mats) - 1)
+    mats.append(material((0.0, 0.0, 0.0), 1.0, emission=(1.0, 1.0, 1.0)))
+    b.add(np.array([(-5, 4, -5), (5, 4, -5), (5, 4, 5), (-5, 4, 5)]), np.tile((0.0, -1.0, 0.0), (4, 1)),
+          np.array([(0, 2, 1), (0, 3, 2)]), len(mats) - 1)
+    cam = dict(eye=(0.0, 3.0, 4.0), lookat=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=50.0)
+    return b.build(mats, camera=cam, name="sky_room")

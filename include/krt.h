@@ -122,6 +122,12 @@ typedef struct {
     float pv[16]; /* projection * view, column-major (for reprojection) */
 } krt_camera;
 
+/* Camera rays of make_ray (source/optix/amadeus_path_tracer.cu:26-56, jittered;
+ * amadeus_restir.cu:31-65, fixed -0.5 offset when jitter == 0) for frame
+ * `sample`, 8 floats per pixel as krt_trace_closest takes them. */
+int krt_camera_rays(krt_context *ctx, const krt_camera *cam, uint32_t width, uint32_t height, int jitter,
+                    int sample, uint32_t user_seed, float *rays);
+
 enum { KRT_OPT_NONE = 0, KRT_OPT_INT = 1, KRT_OPT_FLOAT = 2, KRT_OPT_BOOL = 3, KRT_OPT_STRING = 4 };
 /* AttachmentRTX::OptionValue = variant<int, float, bool, string> (armada.cuh:120) */
 typedef struct {
@@ -150,7 +156,9 @@ typedef struct {
 /* ArmadaRTX(context, accelerator, mesh daemon, extent) (armada.cu:28-83) */
 int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out);
 void krt_armada_destroy(krt_armada *a);
-/* Attachment factory: kind "path_tracer" | "restir" | "restir_gi". */
+/* Attachment factory: kind "path_tracer" | "restir" | "restir_gi".
+ * Lifetimes: destroy armadas before the attachments attached to them, and
+ * everything created on a context before krt_destroy. */
 int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **out);
 void krt_attachment_destroy(krt_attachment *att);
 /* ArmadaRTX::attach / activate / set_option / get_option / set_depth
@@ -175,6 +183,11 @@ int krt_armada_stats(krt_armada *a, krt_frame_stats *out);
  * per pixel.  Enabling (again) clears the sums.  read: 3 doubles per pixel and
  * the number of frames summed. */
 int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
+/* ReSTIR reservoirs (include/amadeus/reservoir.cuh:19-83) as 16 floats per
+ * pixel {Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz}.  which 0: the
+ * buffer the next frame's temporal pass merges with (last spatial output),
+ * 1: the other one. */
+int krt_armada_read_reservoirs(krt_armada *a, int which, float *out);
 int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames);
 int krt_armada_sync(krt_armada *a);
 

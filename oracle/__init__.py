@@ -31,7 +31,8 @@ class OrcCamera(C.Structure):
 class OrcParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("max_depth", C.c_int), ("samples", C.c_int),
                 ("accumulate", C.c_int), ("user_seed", C.c_uint32), ("candidates", C.c_int), ("m_cap", C.c_int),
-                ("spatial_passes", C.c_int), ("neighbours", C.c_int), ("radius", C.c_int)]
+                ("spatial_passes", C.c_int), ("neighbours", C.c_int), ("radius", C.c_int),
+                ("fixed_offset", C.c_int)]
 
 
 _SIGS = {
@@ -61,6 +62,12 @@ _SIGS = {
     "orc_restir_frame": (None, [vp, vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), fp, fp, fp, fp, C.c_int,
                                 u64p]),
     "orc_restir_reservoirs": (None, [vp, C.c_int, fp]),
+    "orc_gi_create": (vp, [C.c_int, C.c_int, C.c_uint32]),
+    "orc_gi_destroy": (None, [vp]),
+    "orc_gi_frame": (None, [vp, vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), fp, C.c_int, u64p]),
+    "orc_gi_reservoirs": (None, [vp, C.c_int, fp]),
+    "orc_gi_jacobian": (C.c_float, [fp, fp, fp, fp]),
+    "orc_camera_rays": (None, [C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int, fp]),
 }
 
 _lib = None
@@ -157,9 +164,9 @@ def camera_struct(frame) -> OrcCamera:
 
 
 def params(width, height, max_depth=5, samples=0, accumulate=1, user_seed=0, candidates=32, m_cap=200,
-           spatial_passes=2, neighbours=5, radius=30) -> OrcParams:
+           spatial_passes=2, neighbours=5, radius=30, fixed_offset=0) -> OrcParams:
     return OrcParams(width, height, max_depth, samples, accumulate, user_seed, candidates, m_cap, spatial_passes,
-                     neighbours, radius)
+                     neighbours, radius, fixed_offset)
 
 
 class OracleScene:
@@ -228,11 +235,12 @@ class OracleScene:
                                    _f(out))
         return out
 
-    def accumulate_pt_f64(self, frame, width, height, first, count, max_depth=5, user_seed=0, nthreads=8, acc=None):
+    def accumulate_pt_f64(self, frame, width, height, first, count, max_depth=5, user_seed=0, nthreads=8, acc=None,
+                          fixed_offset=False):
         if acc is None:
             acc = np.zeros((height, width, 3), np.float64)
         cam = camera_struct(frame)
-        p = params(width, height, max_depth=max_depth, user_seed=user_seed)
+        p = params(width, height, max_depth=max_depth, user_seed=user_seed, fixed_offset=int(fixed_offset))
         lib().orc_accumulate_pt_f64(self.h, C.byref(cam), C.byref(p), first, count, acc.ctypes.data_as(dp),
                                     nthreads)
         return acc
@@ -271,3 +279,52 @@ class OracleReSTIR:
         out = np.empty((self.h_, self.w, 16), np.float32)
         lib().orc_restir_reservoirs(self.h, which, _f(out))
         return out
+
+
+class OracleGI:
+    """ReSTIR GI restatement (our algorithm; oracle.cpp)."""
+
+    def __init__(self, width, height, user_seed=0):
+        self.w, self.h_ = width, height
+        self.h = lib().orc_gi_create(width, height, user_seed)
+        self.user_seed = user_seed
+        self.color = np.zeros((height, width, 4), np.float32)
+        self.samples = 0
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_gi_destroy(self.h)
+        except Exception:
+            pass
+
+    def frame(self, scene: OracleScene, frame, max_depth=3, m_cap=30, spatial_passes=2, neighbours=5, radius=30,
+              nthreads=8, accumulate=True):
+        if not accumulate:
+            self.samples = 0
+        cam = camera_struct(frame)
+        p = params(self.w, self.h_, max_depth=max_depth, samples=self.samples, user_seed=self.user_seed,
+                   m_cap=m_cap, spatial_passes=spatial_passes, neighbours=neighbours, radius=radius)
+        ctr = np.zeros(5, np.uint64)
+        lib().orc_gi_frame(self.h, scene.h, C.byref(cam), C.byref(p), _f(self.color), nthreads,
+                           ctr.ctypes.data_as(u64p))
+        self.samples += 1
+        return ctr
+
+    def reservoirs(self, which=0) -> np.ndarray:
+        out = np.empty((self.h_, self.w, 20), np.float32)
+        lib().orc_gi_reservoirs(self.h, which, _f(out))
+        return out
+
+
+def gi_jacobian(xq, xr, xs, ns) -> float:
+    a = [np.ascontiguousarray(v, np.float32) for v in (xq, xr, xs, ns)]
+    return float(lib().orc_gi_jacobian(*[_f(v) for v in a]))
+
+
+def camera_rays(frame, width, height, jitter=False, sample=0, user_seed=0) -> np.ndarray:
+    out = np.empty((height * width, 8), np.float32)
+    cam = camera_struct(frame)
+    p = params(width, height, samples=sample, user_seed=user_seed)
+    lib().orc_camera_rays(C.byref(cam), C.byref(p), int(jitter), _f(out))
+    return out

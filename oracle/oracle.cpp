@@ -700,6 +700,7 @@ struct RayPacket {  // amadeus_path_tracer.cu:10-23
     f3 albedo{0, 0, 0};
     f3 seed{0, 0, 0};
     int depth = 0;
+    float t = 0.0f;  // hit distance of the outermost level (ReSTIR GI sample point)
 };
 
 // __closesthit__ / __miss__ of amadeus_path_tracer.cu:133-220, recursive as in
@@ -743,6 +744,7 @@ void pt_trace(Ctx &cx, int max_depth, f3 o, f3 d, RayPacket &rp) {
     rp.position = x;
     rp.normal = n;
     rp.albedo = material.diffuse;
+    rp.t = h.t;
 }
 
 // make_ray (amadeus_path_tracer.cu:26-56; ReSTIR variant amadeus_restir.cu:31-65
@@ -772,7 +774,7 @@ void pt_sample(Ctx &cx, const orc_camera *cam, const orc_params *p, uint32_t px,
     rp = RayPacket();
     rp.seed = pixel_seed(px, py, (uint32_t)p->samples, p->user_seed);
     f3 o, d;
-    make_ray(cam, p, px, py, rp.seed, true, o, d);
+    make_ray(cam, p, px, py, rp.seed, p->fixed_offset == 0, o, d);
     pt_trace(cx, p->max_depth, o, d, rp);
 }
 
@@ -1277,4 +1279,342 @@ void orc_restir_reservoirs(const orc_restir *r, int which, float *out) {
         o[14] = q.seed.y;
         o[15] = q.seed.z;
     }
+}
+
+// ---------------------------------------------------------------------------
+// ReSTIR GI (no reference implementation exists; Ouyang et al. 2021, "ReSTIR
+// GI: Path Resampling for Real-Time Path Tracing").  Our algorithm, restated
+// identically by the HIP kernels:
+//  initial : primary hit x_v (fixed -0.5 pixel offset as the ReSTIR raygen),
+//            direct = Le + Ld (path-tracer NEE), one BSDF-sampled direction;
+//            its hit x_s with outgoing radiance L_o = path-tracer value of the
+//            rest of the path (levels 1..max_depth) is the sample.
+//            p_hat = sum(L_o) (zero if x_s is below x_v's tangent plane),
+//            w = p_hat / pdf.
+//  temporal: merge with the previous frame's reservoir (same pixel while
+//            accumulating, reprojected otherwise), M capped, reconnection
+//            Jacobian |cos phi_r| / |cos phi_q| * |x_q - x_s|^2 / |x_r - x_s|^2,
+//            visibility x_v -> x_s in the target of reused samples.
+//  spatial : `neighbours` reservoirs in a `radius` disk with normal/depth
+//            similarity, Jacobian and visibility; 1/Z bias correction.
+//  shading : direct + f(x_v, wo, wi_s) * L_o * |n_v . wi_s| * W.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct GISample {
+    f3 xs{0, 0, 0}, ns{0, 0, 0}, Lo{0, 0, 0};
+};
+
+struct GIRes {
+    GISample s;
+    f3 xo{0, 0, 0};
+    float w = 0.0f, W = 0.0f;
+    int M = 0;
+    f3 seed{0, 0, 0};
+    bool update(const GISample &x, float weight) {
+        w += weight;
+        float eta = rand_uniform(seed);
+        bool sel = (eta * w) < weight;
+        if (sel) s = x;
+        M++;
+        return sel;
+    }
+};
+
+struct GIG {  // G-buffer
+    int valid = 0;
+    f3 x{0, 0, 0}, n{0, 0, 0};
+    Material mat{};
+    float depth = 0.0f;
+};
+
+const float kGiMaxJacobian = 10.0f;
+const float kGiNormalCos = 0.9f;
+const float kGiDepthRel = 0.1f;
+
+float gi_target(const GISample &s, f3 xv, f3 nv) {
+    f3 D = s.xs - xv;
+    if (!(dot(nv, D) > 0.0f)) return 0.0f;
+    return s.Lo.x + s.Lo.y + s.Lo.z;
+}
+
+float gi_jacobian(f3 xq, f3 xr, const GISample &s) {
+    f3 dq = xq - s.xs, dr = xr - s.xs;
+    float lq2 = dot(dq, dq), lr2 = dot(dr, dr);
+    if (!(lq2 > 0.0f) || !(lr2 > 0.0f)) return 0.0f;
+    float cq = fabsf(dot(s.ns, dq)) / sqrtf(lq2);
+    float cr = fabsf(dot(s.ns, dr)) / sqrtf(lr2);
+    if (!(cq > 0.0f)) return 0.0f;
+    float J = (cr / cq) * (lq2 / lr2);
+    return (J > kGiMaxJacobian) ? 0.0f : J;
+}
+
+bool gi_visible(Ctx &cx, f3 xv, f3 xs) {
+    f3 D = xs - xv;
+    float d = length(D);
+    cx.c.shadow++;
+    return !occluded(cx.s, xv, D / d, d - kShadowEps, cx.c);
+}
+
+bool gi_similar(const GIG &a, const GIG &b) {
+    return b.valid && dot(a.n, b.n) > kGiNormalCos && fabsf(a.depth - b.depth) < kGiDepthRel * a.depth;
+}
+
+}  // namespace
+
+struct orc_gi {
+    int w, h;
+    std::vector<GIRes> buf[2];
+    int cur = 0;
+    std::vector<GIG> g[2];
+    int gcur = 0;
+    std::vector<f3> direct, aux;
+    float prev_pv[16];
+};
+
+orc_gi *orc_gi_create(int width, int height, uint32_t user_seed) {
+    orc_gi *r = new orc_gi();
+    r->w = width;
+    r->h = height;
+    size_t n = (size_t)width * height;
+    for (int b = 0; b < 2; b++) {
+        r->buf[b].resize(n);
+        r->g[b].resize(n);
+        for (int y = 0; y < height; y++)
+            for (int x = 0; x < width; x++)
+                r->buf[b][(size_t)y * width + x].seed = reservoir_seed(x, y, user_seed ^ 0x47490000u);
+    }
+    r->direct.assign(n, mk(0, 0, 0));
+    r->aux.assign(n, mk(0, 0, 0));
+    std::fill(r->prev_pv, r->prev_pv + 16, 0.0f);
+    return r;
+}
+
+void orc_gi_destroy(orc_gi *r) { delete r; }
+
+void orc_gi_frame(orc_gi *r, const orc_scene *s, const orc_camera *cam, const orc_params *p, float *color,
+                  int nthreads, uint64_t *counters) {
+    const int W = p->width;
+    std::vector<Counters> all, per;
+    std::vector<GIG> &G = r->g[r->gcur];
+    const std::vector<GIG> &PG = r->g[1 - r->gcur];
+    f3 center = load3(cam->center);
+    // initial
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        for (int x = 0; x < W; x++) {
+            int pix = y * W + x;
+            GIRes &R = r->buf[r->cur][pix];
+            R.w = 0.0f;
+            R.M = 0;
+            R.W = 0.0f;
+            f3 seed = pixel_seed((uint32_t)x, (uint32_t)y, (uint32_t)p->samples, p->user_seed);
+            f3 o, d;
+            make_ray(cam, p, (uint32_t)x, (uint32_t)y, seed, false, o, d);
+            GIG gb;
+            r->direct[pix] = mk(0, 0, 0);
+            cx.c.closest++;
+            Hit h = closest(s, o, d, kTraceTmax, 2, cx.c);
+            Surface sf;
+            if (h.prim >= 0 && load_surface(s, h, d, sf)) {
+                const Material &mat = sf.mat;
+                f3 n = sf.n;
+                f3 xv = sf.x + ((mat.type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
+                SurfaceHit sh{mat, sf.entering, n, -d, xv};
+                r->direct[pix] = mat.emission + Ld(cx, sh, seed);
+                int out;
+                f3 wi;
+                float pdf;
+                eval_bsdf(sh, wi, pdf, out, seed);
+                GISample smp;
+                smp.xs = xv + wi;
+                smp.ns = -wi;
+                if (pdf > 0.0f && p->max_depth >= 1) {
+                    RayPacket rp;
+                    rp.seed = seed;
+                    rp.depth = 1;
+                    pt_trace(cx, p->max_depth, xv, wi, rp);
+                    seed = rp.seed;
+                    if (dot(rp.normal, rp.normal) > 0.0f) {
+                        smp.xs = xv + wi * rp.t;  // on the ray, so xs - xv keeps the sampled direction
+                        smp.ns = rp.normal;
+                        smp.Lo = rp.value;
+                    }
+                }
+                float ph = gi_target(smp, xv, n);
+                R.update(smp, (pdf > 0.0f) ? ph / pdf : 0.0f);
+                float dd = ph * (float)R.M;
+                R.W = (dd > 0.0f) ? R.w / dd : 0.0f;
+                R.xo = xv;
+                gb.valid = 1;
+                gb.x = xv;
+                gb.n = n;
+                gb.mat = mat;
+                gb.depth = length(xv - center);
+            }
+            G[pix] = gb;
+            r->aux[pix] = seed;
+        }
+        c.add(cx.c);
+    });
+    all.insert(all.end(), per.begin(), per.end());
+    // temporal
+    parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        std::vector<GIRes> &current = r->buf[r->cur];
+        std::vector<GIRes> &previous = r->buf[1 - r->cur];
+        for (int x = 0; x < W; x++) {
+            int pix = y * W + x;
+            const GIG &gb = G[pix];
+            GIRes &R = current[pix];
+            if (!gb.valid || R.M <= 0) continue;
+            GIRes m;
+            m.seed = r->aux[pix];
+            m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
+            int N = R.M;
+            int q = pix;
+            if (p->samples == 0) {
+                const float *mm = r->prev_pv;
+                float v[4];
+                for (int k = 0; k < 4; k++)
+                    v[k] = (mm[0 + k] * gb.x.x + mm[4 + k] * gb.x.y) + (mm[8 + k] * gb.x.z + mm[12 + k] * 1.0f);
+                float u = v[0] / v[3], vv = v[1] / v[3];
+                q = -1;
+                if (u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f) {
+                    int ix = (int)((double)((u + 1.0f) * 0.5f * (float)p->width) + 0.5);
+                    int iy = (int)((double)((vv + 1.0f) * 0.5f * (float)p->height) + 0.5);
+                    ix = std::min(std::max(ix, 0), p->width - 1);
+                    iy = std::min(std::max(iy, 0), p->height - 1);
+                    q = iy * p->width + ix;
+                }
+            }
+            if (q >= 0 && gi_similar(gb, PG[q]) && previous[q].M > 0) {
+                GIRes &P = previous[q];
+                P.M = std::min(P.M, p->m_cap);
+                float J = gi_jacobian(P.xo, gb.x, P.s);
+                float t = 0.0f;
+                if (J > 0.0f && gi_target(P.s, gb.x, gb.n) > 0.0f && gi_visible(cx, gb.x, P.s.xs))
+                    t = gi_target(P.s, gb.x, gb.n);
+                m.update(P.s, t * P.W * (float)P.M * J);
+                N += (t > 0.0f ? 1 : 0) * P.M;
+            }
+            m.M = N;
+            float tm = gi_target(m.s, gb.x, gb.n) * (float)m.M;
+            m.W = (tm > 0.0f) ? m.w / tm : 0.0f;
+            m.xo = gb.x;
+            R = m;
+        }
+        c.add(cx.c);
+    });
+    all.insert(all.end(), per.begin(), per.end());
+    // spatial passes (+ shading in the last one; with none, shade the temporal result)
+    const int passes = p->spatial_passes;
+    auto shade = [&](int pix, const GIRes &R) {
+        const GIG &gb = G[pix];
+        f3 col = r->direct[pix];
+        if (gb.valid && R.W > 0.0f) {
+            f3 wi = normalize(R.s.xs - gb.x);
+            SurfaceHit sh{gb.mat, true, gb.n, normalize(center - gb.x), gb.x};
+            f3 f = brdf(sh, wi, eDiffuse);
+            col = col + (f * R.s.Lo * fabsf(dot(gb.n, wi))) * R.W;
+        }
+        float c4[4] = {col.x, col.y, col.z, 1.0f};
+        if (isnan3(col)) {
+            c4[0] = 1;
+            c4[1] = 0;
+            c4[2] = 1;
+        }
+        accumulate4(color + 4 * (size_t)pix, c4, p);
+    };
+    if (passes == 0) {
+        for (int pix = 0; pix < W * p->height; pix++) shade(pix, r->buf[r->cur][pix]);
+    }
+    for (int pass = 0; pass < passes; pass++) {
+        const bool last = pass == passes - 1;
+        parallel_rows(p->height, nthreads, per, [&](int y, Counters &c) {
+            Ctx cx{s, Counters()};
+            const std::vector<GIRes> &cb = r->buf[r->cur];
+            std::vector<GIRes> &ob = r->buf[1 - r->cur];
+            for (int x = 0; x < W; x++) {
+                int pix = y * W + x;
+                const GIG &gb = G[pix];
+                GIRes R = cb[pix];
+                if (gb.valid && R.M > 0) {
+                    GIRes m;
+                    m.seed = r->aux[pix];
+                    m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
+                    int used[16], Ms[16], nu = 0, cnt = 0;
+                    for (int i = 0; i < p->neighbours; i++) {
+                        int q = spatial_neighbour(p, pix, m.seed, p->radius);
+                        if (q == pix || !gi_similar(gb, G[q])) continue;
+                        const GIRes &Nb = cb[q];
+                        if (Nb.M <= 0) continue;
+                        float J = gi_jacobian(Nb.xo, gb.x, Nb.s);
+                        float t = 0.0f;
+                        if (J > 0.0f && gi_target(Nb.s, gb.x, gb.n) > 0.0f && gi_visible(cx, gb.x, Nb.s.xs))
+                            t = gi_target(Nb.s, gb.x, gb.n);
+                        m.update(Nb.s, t * Nb.W * (float)Nb.M * J);
+                        used[nu] = q;
+                        Ms[nu] = Nb.M;
+                        nu++;
+                        cnt += Nb.M;
+                    }
+                    int Z = R.M;
+                    for (int k = 0; k < nu; k++) {
+                        const GIG &gq = G[used[k]];
+                        if (gi_target(m.s, gq.x, gq.n) > 0.0f && gi_visible(cx, gq.x, m.s.xs)) Z += Ms[k];
+                    }
+                    m.M = R.M + cnt;
+                    float den = gi_target(m.s, gb.x, gb.n) * (float)Z;
+                    m.W = (den > 0.0f) ? m.w / den : 0.0f;
+                    m.xo = gb.x;
+                    ob[pix] = m;
+                    if (last) shade(pix, m);
+                } else {
+                    GIRes &o = ob[pix];
+                    o.w = 0.0f;
+                    o.M = 0;
+                    o.W = 0.0f;
+                    if (last) shade(pix, o);
+                }
+            }
+            c.add(cx.c);
+        });
+        all.insert(all.end(), per.begin(), per.end());
+        r->cur = 1 - r->cur;
+    }
+    r->cur = 1 - r->cur;  // the last output becomes `previous`
+    r->gcur = 1 - r->gcur;
+    std::copy(cam->pv, cam->pv + 16, r->prev_pv);
+    write_counters(all, counters);
+}
+
+void orc_gi_reservoirs(const orc_gi *r, int which, float *out) {
+    const std::vector<GIRes> &b = r->buf[which == 0 ? 1 - r->cur : r->cur];
+    for (size_t i = 0; i < b.size(); i++) {
+        const GIRes &q = b[i];
+        float v[20] = {q.s.xs.x, q.s.xs.y, q.s.xs.z, q.s.ns.x, q.s.ns.y, q.s.ns.z, q.s.Lo.x, q.s.Lo.y, q.s.Lo.z,
+                       q.xo.x,   q.xo.y,   q.xo.z,   q.w,      q.W,      (float)q.M, q.seed.x, q.seed.y, q.seed.z,
+                       0.0f,     0.0f};
+        std::copy(v, v + 20, out + 20 * i);
+    }
+}
+
+float orc_gi_jacobian(const float *xq, const float *xr, const float *xs, const float *ns) {
+    GISample s;
+    s.xs = load3(xs);
+    s.ns = load3(ns);
+    return gi_jacobian(load3(xq), load3(xr), s);
+}
+
+void orc_camera_rays(const orc_camera *cam, const orc_params *p, int jitter, float *rays) {
+    for (int y = 0; y < p->height; y++)
+        for (int x = 0; x < p->width; x++) {
+            f3 seed = pixel_seed((uint32_t)x, (uint32_t)y, (uint32_t)p->samples, p->user_seed);
+            f3 o, d;
+            make_ray(cam, p, (uint32_t)x, (uint32_t)y, seed, jitter != 0, o, d);
+            float *r = rays + 8 * ((size_t)y * p->width + x);
+            r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = kTraceTmax;
+            r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = 0.0f;
+        }
 }

@@ -97,6 +97,7 @@ typedef struct {
     uint32_t user_seed;
     /* ReSTIR */
     int candidates, m_cap, spatial_passes, neighbours, radius;
+    int fixed_offset; /* path tracer camera rays at the fixed -0.5 offset (as the ReSTIR raygen) */
 } orc_params;
 
 /* counters: [0] closest rays, [1] shadow rays, [2] discarded depth-cut rays,
@@ -126,6 +127,20 @@ void orc_restir_frame(orc_restir *r, const orc_scene *s, const orc_camera *cam,
  * which: 0 = reservoir the next frame merges with (the last spatial output),
  *        1 = the other buffer */
 void orc_restir_reservoirs(const orc_restir *r, int which, float *out);
+
+/* ReSTIR GI (our algorithm; see oracle.cpp).  color: float4 per pixel,
+ * accumulated like the other attachments.  Reservoirs: 20 floats per pixel
+ * {xs, ns, Lo, xo, w, W, M, seed, 0, 0}; which as orc_restir_reservoirs. */
+typedef struct orc_gi orc_gi;
+orc_gi *orc_gi_create(int width, int height, uint32_t user_seed);
+void orc_gi_destroy(orc_gi *r);
+void orc_gi_frame(orc_gi *r, const orc_scene *s, const orc_camera *cam, const orc_params *p, float *color,
+                  int nthreads, uint64_t *counters);
+void orc_gi_reservoirs(const orc_gi *r, int which, float *out);
+/* camera rays of make_ray (8 floats each), jittered (path tracer) or fixed -0.5 offset (ReSTIR) */
+void orc_camera_rays(const orc_camera *cam, const orc_params *p, int jitter, float *rays);
+/* reconnection-shift Jacobian d(omega_r)/d(omega_q) (0 when rejected) */
+float orc_gi_jacobian(const float *xq, const float *xr, const float *xs, const float *ns);
 
 #ifdef __cplusplus
 }

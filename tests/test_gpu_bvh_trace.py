@@ -122,3 +122,19 @@ def test_trace_tmax_and_miss(krt_ctx):
     got = sc.trace_closest(out)
     assert np.all(got[:, 3].view(np.int32) == -1)
     sc.free()
+
+
+@pytest.mark.parametrize("jitter", [False, True])
+def test_camera_rays_bit_exact(krt_ctx, jitter):
+    s = scenes.cornell()
+    from artemis_amd import look_at
+
+    cam = look_at(**s.camera, aspect=1.0)
+    for W, H, sample in ((32, 32, 0), (48, 40, 3)):
+        got = krt_ctx.camera_rays(cam, W, H, jitter=jitter, sample=sample)
+        ref = O.camera_rays(cam, W, H, jitter=jitter, sample=sample)
+        bad = np.where(np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1))[0]
+        print(f"[camera rays] jitter={jitter} {W}x{H}: {len(bad)} differ; first {bad[:4]}", flush=True)
+        if len(bad):
+            print(got[bad[:2]], ref[bad[:2]])
+        assert len(bad) == 0

@@ -1,0 +1,63 @@
+"""ReSTIR GI oracle (our algorithm; there is no reference implementation).
+Pinned by the reconnection Jacobian against the solid-angle ratio measured on
+a small surface patch, and by the estimator's mean against the converged path
+tracer of the same depth (ReSTIR GI reuses the path tracer's secondary paths,
+so its expectation is the path tracer's)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from artemis_amd import look_at, scenes
+
+
+def solid_angle_of_patch(x, corners):
+    # spherical area of a small quad seen from x (two spherical triangles)
+    def tri(a, b, c):
+        a, b, c = (v / np.linalg.norm(v) for v in (a, b, c))
+        num = abs(np.dot(a, np.cross(b, c)))
+        den = 1 + np.dot(a, b) + np.dot(b, c) + np.dot(c, a)
+        return 2 * np.arctan2(num, den)
+    p = [c - x for c in corners]
+    return tri(p[0], p[1], p[2]) + tri(p[0], p[2], p[3])
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_jacobian_equals_solid_angle_ratio(seed):
+    rng = np.random.default_rng(seed)
+    xs = rng.normal(size=3)
+    ns = rng.normal(size=3)
+    ns /= np.linalg.norm(ns)
+    t1 = np.cross(ns, [1.0, 0.0, 0.0] if abs(ns[0]) < 0.9 else [0.0, 1.0, 0.0])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(ns, t1)
+    # receivers on the front side of the patch, 1..3 units away
+    def receiver():
+        v = rng.normal(size=3)
+        v = v if v @ ns > 0.3 * np.linalg.norm(v) else v + 2 * ns
+        return xs + v / np.linalg.norm(v) * rng.uniform(1, 3)
+    xq, xr = receiver(), receiver()
+    h = 1e-3
+    corners = [xs + h * (a * t1 + b * t2) for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+    ratio = solid_angle_of_patch(xr, corners) / solid_angle_of_patch(xq, corners)
+    J = O.gi_jacobian(xq, xr, xs, ns)
+    if ratio > 10:
+        assert J == 0.0  # rejected shift
+    else:
+        assert J == pytest.approx(ratio, rel=2e-3)
+
+
+def test_gi_mean_matches_path_tracer():
+    s = scenes.cornell()
+    osc = O.OracleScene(s)
+    W = H = 24
+    cam = look_at(**s.camera)
+    frames = 512
+    gi = O.OracleGI(W, H)
+    for f in range(frames):
+        gi.frame(osc, cam, max_depth=3, m_cap=30, spatial_passes=2, neighbours=5, radius=8, nthreads=8)
+    acc = osc.accumulate_pt_f64(cam, W, H, 0, 2048, max_depth=3, nthreads=8) / 2048.0
+    g = gi.color[..., :3].astype(np.float64)
+    # same expectation: image means agree and per-pixel relative error is small
+    assert g.mean() == pytest.approx(acc.mean(), rel=0.03)
+    rel = np.mean((g - acc) ** 2 / (acc ** 2 + 1e-2))
+    assert rel < 0.02
