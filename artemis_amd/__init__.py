@@ -17,7 +17,7 @@ from . import _lib as L
 from .camera import CameraFrame, look_at
 from .scenes import SceneData
 
-__all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "CameraFrame", "look_at", "SceneData",
+__all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "CameraFrame", "look_at", "SceneData",
            "load_attachment"]
 
 
@@ -180,9 +180,14 @@ class ReSTIR(_Attachment):
     kind = "restir"
 
 
+class ReSTIRGI(_Attachment):
+    """ReSTIR GI (no reference counterpart; Ouyang et al. 2021)."""
+    kind = "restir_gi"
+
+
 def load_attachment(ctx: Context, kind: str):
     """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
-    return {"path_tracer": PathTracer, "restir": ReSTIR}[kind](ctx)
+    return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI}[kind](ctx)
 
 
 class ArmadaRTX:
@@ -286,9 +291,9 @@ class ArmadaRTX:
     def sync(self) -> None:
         L.check(self.ctx._lib.krt_armada_sync(self.h))
 
-    def reservoirs(self, which: int = 0) -> np.ndarray:
-        """ReSTIR reservoirs, 16 floats per pixel (see krt_armada_read_reservoirs)."""
-        out = np.empty((self.height, self.width, 16), np.float32)
+    def reservoirs(self, which: int = 0, gi: bool = False) -> np.ndarray:
+        """ReSTIR reservoirs, 16 (DI) or 20 (GI) floats per pixel (krt_armada_read_reservoirs)."""
+        out = np.empty((self.height, self.width, 20 if gi else 16), np.float32)
         L.check(self.ctx._lib.krt_armada_read_reservoirs(self.h, which, _f(out)))
         return out
 

@@ -38,6 +38,7 @@ struct krt_attachment {
     uint32_t user_seed = 0;
     RestirOptions ropt;
     RestirState rst;
+    GiState gst;
     DevBuf history;
     uint32_t w = 0, h = 0;
 
@@ -45,10 +46,12 @@ struct krt_attachment {
         w = width;
         h = height;
         if (kind == "restir") restir_alloc(*ctx, rst, width, height, user_seed);
+        if (kind == "restir_gi") gi_alloc(*ctx, gst, width, height, user_seed);
         loaded = true;
     }
     void unload() {  // restir.cuh:213-220
         rst = RestirState();
+        gst = GiState();
         history.release();
         loaded = false;
     }
@@ -66,7 +69,7 @@ struct krt_attachment {
             user_seed = (uint32_t)tmp;
             return true;
         }
-        if (kind != "restir") return true;  // unknown option names are ignored
+        if (kind != "restir" && kind != "restir_gi") return true;  // unknown option names are ignored
         if (f == "candidates") return as_int(ropt.candidates);
         if (f == "m_cap") return as_int(ropt.m_cap);
         if (f == "spatial_passes") return as_int(ropt.spatial_passes);
@@ -78,11 +81,11 @@ struct krt_attachment {
         krt_option o{};
         o.kind = KRT_OPT_INT;
         if (f == "seed") o.i = (int)user_seed;
-        else if (kind == "restir" && f == "candidates") o.i = ropt.candidates;
-        else if (kind == "restir" && f == "m_cap") o.i = ropt.m_cap;
-        else if (kind == "restir" && f == "spatial_passes") o.i = ropt.spatial_passes;
-        else if (kind == "restir" && f == "neighbours") o.i = ropt.neighbours;
-        else if (kind == "restir" && f == "radius") o.i = ropt.radius;
+        else if (kind != "path_tracer" && f == "candidates") o.i = ropt.candidates;
+        else if (kind != "path_tracer" && f == "m_cap") o.i = ropt.m_cap;
+        else if (kind != "path_tracer" && f == "spatial_passes") o.i = ropt.spatial_passes;
+        else if (kind != "path_tracer" && f == "neighbours") o.i = ropt.neighbours;
+        else if (kind != "path_tracer" && f == "radius") o.i = ropt.radius;
         else o.kind = KRT_OPT_NONE;
         return o;
     }
@@ -398,10 +401,15 @@ int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **o
     return guard([&] {
         KRT_REQUIRE(ctx && kind && out, KRT_ERR_INVALID, "null argument");
         std::string k(kind);
-        KRT_REQUIRE(k == "path_tracer" || k == "restir", KRT_ERR_INVALID, "unknown attachment kind: " + k);
+        KRT_REQUIRE(k == "path_tracer" || k == "restir" || k == "restir_gi", KRT_ERR_INVALID,
+                    "unknown attachment kind: " + k);
         auto *a = new krt_attachment();
         a->ctx = &ctx->impl;
         a->kind = k;
+        if (k == "restir_gi") {  // ReSTIR GI defaults: temporal M-cap 30, 2 spatial passes
+            a->ropt.m_cap = 30;
+            a->ropt.spatial_passes = 2;
+        }
         *out = a;
     });
 }
@@ -506,7 +514,8 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
                       a->f64 ? a->acc64.as<double>() : nullptr);
             if (a->f64) a->f64_frames++;
         }
-        else restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
+        else if (att->kind == "restir") restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
+        else gi_render(C, s, li, att->ropt, att->gst, a->ctr.as<unsigned long long>(), times);
         unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         KRT_HIP(hipMemcpyAsync(c, a->ctr.p, sizeof(c), hipMemcpyDeviceToHost, C.stream));
         KRT_HIP(hipStreamSynchronize(C.stream));
@@ -572,8 +581,9 @@ int krt_armada_read_reservoirs(krt_armada *a, int which, float *out) {
         KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
         KRT_REQUIRE(!a->previous.empty(), KRT_ERR_STATE, "nothing rendered yet");
         krt_attachment *att = a->attachments.at(a->previous);
-        KRT_REQUIRE(att->kind == "restir" && att->loaded, KRT_ERR_STATE, "active attachment has no reservoirs");
-        restir_read(*a->ctx, att->rst, which, out);
+        KRT_REQUIRE(att->kind != "path_tracer" && att->loaded, KRT_ERR_STATE, "active attachment has no reservoirs");
+        if (att->kind == "restir") restir_read(*a->ctx, att->rst, which, out);
+        else gi_read(*a->ctx, att->gst, which, out);
     });
 }
 

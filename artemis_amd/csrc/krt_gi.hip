@@ -1,0 +1,553 @@
+// ReSTIR GI attachment (no reference implementation: the reference has no GI
+// resampling; the algorithm follows Ouyang et al. 2021, "ReSTIR GI: Path
+// Resampling for Real-Time Path Tracing").
+//
+//  initial : primary hit x_v (fixed -0.5 pixel offset, as the ReSTIR raygen of
+//            source/optix/amadeus_restir.cu:31-65), direct = Le + Ld (the
+//            path tracer's NEE), one BSDF-sampled direction wi; its hit x_s =
+//            x_v + t*wi and the outgoing radiance L_o = the path tracer's value
+//            of the rest of the path (levels 1..max_depth) form the sample.
+//            p_hat = sum(L_o) (0 when x_s lies below x_v's tangent plane),
+//            w = p_hat / pdf(wi).
+//  temporal: merge with last frame's reservoir (same pixel while
+//            accumulating, reprojected otherwise; normal/depth similarity),
+//            M capped, reconnection Jacobian, visibility x_v -> x_s.
+//  spatial : `neighbours` reservoirs in a `radius` disk, same tests, 1/Z bias
+//            correction; the last pass shades
+//            direct + f(x_v, wo, wi_s) * L_o * |n_v . wi_s| * W.
+// Reservoirs: five float4 planes {x_s, M}, {n_s, w}, {L_o, W}, {x_owner}, {seed}.
+#include <hip/hip_runtime.h>
+
+#include "krt_device.h"
+#include "krt_internal.h"
+#include "krt_launch.h"
+#include "krt_shade.h"
+#include "krt_traverse.h"
+
+namespace krt {
+
+namespace {
+
+constexpr float kGiMaxJacobian = 10.0f;
+constexpr float kGiNormalCos = 0.9f;
+constexpr float kGiDepthRel = 0.1f;
+constexpr int kGiMaxNeighbours = 8;
+
+struct GSample {
+    f3 xs, ns, Lo;
+};
+
+struct GRes {
+    GSample s;
+    f3 xo;
+    float w, W;
+    int M;
+    f3 seed;
+    KRT_DEV bool update(const GSample &x, float weight) {
+        w += weight;
+        float eta = rand_uniform(seed);
+        bool sel = (eta * w) < weight;
+        if (sel) s = x;
+        M++;
+        return sel;
+    }
+};
+
+struct GPlanes {
+    float4 *p;
+    uint32_t n;
+    KRT_DEV GRes load(uint32_t i) const {
+        float4 a = p[i], b = p[n + i], c = p[2 * n + i], d = p[3 * n + i], e = p[4 * n + i];
+        GRes r;
+        r.s.xs = mk(a.x, a.y, a.z);
+        r.M = __float_as_int(a.w);
+        r.s.ns = mk(b.x, b.y, b.z);
+        r.w = b.w;
+        r.s.Lo = mk(c.x, c.y, c.z);
+        r.W = c.w;
+        r.xo = mk(d.x, d.y, d.z);
+        r.seed = mk(e.x, e.y, e.z);
+        return r;
+    }
+    KRT_DEV void store(uint32_t i, const GRes &r) const {
+        p[i] = make_float4(r.s.xs.x, r.s.xs.y, r.s.xs.z, __int_as_float(r.M));
+        p[n + i] = make_float4(r.s.ns.x, r.s.ns.y, r.s.ns.z, r.w);
+        p[2 * n + i] = make_float4(r.s.Lo.x, r.s.Lo.y, r.s.Lo.z, r.W);
+        p[3 * n + i] = make_float4(r.xo.x, r.xo.y, r.xo.z, 0.0f);
+        p[4 * n + i] = make_float4(r.seed.x, r.seed.y, r.seed.z, 0.0f);
+    }
+    KRT_DEV void reset(uint32_t i) const {
+        p[i].w = __int_as_float(0);
+        p[n + i].w = 0.0f;
+        p[2 * n + i].w = 0.0f;
+    }
+    KRT_DEV void store_M(uint32_t i, int M) const { p[i].w = __int_as_float(M); }
+};
+
+// G-buffer: {x, depth}, {n, valid}, material id
+struct GBuf {
+    float4 *xd, *nv;
+    uint32_t *mat;
+};
+
+struct GPix {
+    bool valid;
+    f3 x, n;
+    float depth;
+    uint32_t mat;
+};
+
+KRT_DEV GPix load_g(const GBuf &g, uint32_t i) {
+    float4 a = g.xd[i], b = g.nv[i];
+    GPix p;
+    p.x = mk(a.x, a.y, a.z);
+    p.depth = a.w;
+    p.n = mk(b.x, b.y, b.z);
+    p.valid = b.w != 0.0f;
+    p.mat = g.mat[i];
+    return p;
+}
+
+KRT_DEV float gi_target(const GSample &s, f3 xv, f3 nv) {
+    f3 D = s.xs - xv;
+    if (!(dot(nv, D) > 0.0f)) return 0.0f;
+    return s.Lo.x + s.Lo.y + s.Lo.z;
+}
+
+KRT_DEV float gi_jacobian(f3 xq, f3 xr, const GSample &s) {
+    f3 dq = xq - s.xs, dr = xr - s.xs;
+    float lq2 = dot(dq, dq), lr2 = dot(dr, dr);
+    if (!(lq2 > 0.0f) || !(lr2 > 0.0f)) return 0.0f;
+    float cq = fabsf(dot(s.ns, dq)) / sqrtf(lq2);
+    float cr = fabsf(dot(s.ns, dr)) / sqrtf(lr2);
+    if (!(cq > 0.0f)) return 0.0f;
+    float J = (cr / cq) * (lq2 / lr2);
+    return (J > kGiMaxJacobian) ? 0.0f : J;
+}
+
+KRT_DEV bool gi_similar(const GPix &a, const GPix &b) {
+    return b.valid && dot(a.n, b.n) > kGiNormalCos && fabsf(a.depth - b.depth) < kGiDepthRel * a.depth;
+}
+
+struct GiArgs {
+    LaunchInfo li;
+    SceneView sv;
+    GPlanes cur, prev;
+    GBuf g, pg;
+    float4 *direct, *aux;
+    float4 *hist;
+    int *spill;
+    unsigned long long *ctr;
+    int m_cap, neighbours, radius, shade;
+};
+
+KRT_DEV bool gi_visible(const SceneView &sv, f3 xv, f3 xs, const TraceStack &stk, TravStats &ts, RayCount &rc) {
+    f3 D = xs - xv;
+    float d = length(D);
+    rc.shadow++;
+    return !occluded(sv, xv, D / d, d - kShadowEps, stk, ts);
+}
+
+KRT_DEV void gi_shade(const GiArgs &A, uint32_t pix, const GPix &gb, const GRes &R) {
+    const LaunchInfo &li = A.li;
+    float4 dv = A.direct[pix];
+    f3 col = mk(dv.x, dv.y, dv.z);
+    if (gb.valid && R.W > 0.0f) {
+        f3 wi = normalize(R.s.xs - gb.x);
+        SurfaceHit sh;
+        sh.mat = load_material(A.sv.mats, gb.mat);
+        sh.entering = true;
+        sh.n = gb.n;
+        sh.wo = normalize(ld3(li.center) - gb.x);
+        sh.x = gb.x;
+        f3 f = brdf(sh, wi, eDiffuse);
+        col = col + (f * R.s.Lo * fabsf(dot(gb.n, wi))) * R.W;
+    }
+    float4 c4 = make_float4(col.x, col.y, col.z, 1.0f);
+    if (isnan3(col)) c4 = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+    li.color[pix] = accumulate4(li.color[pix], c4, li.accumulate != 0, (float)li.samples);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const SceneView &sv = A.sv;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    History hist{A.hist + gtid, nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    const f3 center = ld3(li.center);
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        const uint32_t px = pix % li.width, py = pix / li.width;
+        GRes R = A.cur.load(pix);
+        R.w = 0.0f;
+        R.M = 0;
+        R.W = 0.0f;
+        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+        f3 o, d;
+        make_ray(li, px, py, seed, false, o, d);
+        f3 direct = mk(0, 0, 0);
+        float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
+        uint32_t gmat = 0;
+        Hit h;
+        rc.closest++;
+        traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+        Surface sf;
+        if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
+            Material mat = load_material(sv.mats, sf.mat_id);
+            f3 n = sf.n;
+            f3 xv = offset_point(sf.x, n, mat.type);
+            SurfaceHit sh{mat, sf.entering, n, -d, xv};
+            direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+            int out;
+            f3 wi;
+            float pdf;
+            eval_bsdf(sh, wi, pdf, out, seed);
+            GSample smp;
+            smp.xs = xv + wi;
+            smp.ns = -wi;
+            smp.Lo = mk(0, 0, 0);
+            if (pdf > 0.0f && li.max_depth >= 1) {
+                // the path tracer from level 1 (source/optix/amadeus_path_tracer.cu:133-201)
+                f3 po = xv, pd = wi;
+                int nlev = 0;
+                bool valid1 = false;
+                float t1 = 0.0f;
+                f3 n1 = mk(0, 0, 0);
+                for (int level = 1; level <= li.max_depth; level++) {
+                    Hit hh;
+                    rc.closest++;
+                    traverse<false>(sv.nodes, sv.tri, po, pd, kTraceTmax, stk, hh, ts);
+                    if (hh.prim < 0) break;
+                    Surface s2;
+                    if (!load_surface(sv, hh, pd, s2)) break;
+                    Material m2 = load_material(sv.mats, s2.mat_id);
+                    f3 nn = s2.n;
+                    f3 xx = offset_point(s2.x, nn, m2.type);
+                    SurfaceHit sh2{m2, s2.entering, nn, -pd, xx};
+                    f3 dk = m2.emission + Ld(sv, sh2, seed, stk, ts, rc);
+                    int out2;
+                    f3 wi2;
+                    float pdf2;
+                    f3 f2 = eval_bsdf(sh2, wi2, pdf2, out2, seed);
+                    f3 T2 = f2 * fabsf(dot(wi2, nn)) / pdf2;
+                    bool has_pdf = pdf2 > 0.0f;
+                    hist.put(level - 1, dk, has_pdf, T2);
+                    if (level == 1) {
+                        valid1 = true;
+                        t1 = hh.t;
+                        n1 = nn;
+                    }
+                    nlev = level;
+                    if (!has_pdf) break;
+                    po = xx;
+                    pd = wi2;
+                }
+                f3 value = mk(0, 0, 0);
+                for (int k = nlev - 1; k >= 0; k--) {
+                    f3 dk, Tk;
+                    bool hp;
+                    hist.get(k, dk, hp, Tk);
+                    f3 v = dk;
+                    if (hp) v = v + Tk * value;
+                    value = v;
+                }
+                if (valid1) {
+                    smp.xs = xv + wi * t1;
+                    smp.ns = n1;
+                    smp.Lo = value;
+                }
+            }
+            float ph = gi_target(smp, xv, n);
+            R.update(smp, (pdf > 0.0f) ? ph / pdf : 0.0f);
+            float dd = ph * (float)R.M;
+            R.W = (dd > 0.0f) ? R.w / dd : 0.0f;
+            R.xo = xv;
+            gxd = make_float4(xv.x, xv.y, xv.z, length(xv - center));
+            gnv = make_float4(n.x, n.y, n.z, 1.0f);
+            gmat = sf.mat_id;
+        }
+        A.g.xd[pix] = gxd;
+        A.g.nv[pix] = gnv;
+        A.g.mat[pix] = gmat;
+        A.direct[pix] = make_float4(direct.x, direct.y, direct.z, 0.0f);
+        A.aux[pix] = make_float4(seed.x, seed.y, seed.z, 0.0f);
+        A.cur.store(pix, R);
+    }
+    wave_count(A.ctr, rc.closest);
+    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+KRT_DEV int gi_reproject(const LaunchInfo &li, uint32_t pix, f3 x) {
+    if (li.samples > 0) return (int)pix;
+    const float *m = li.prev_pv;
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (m[0 + k] * x.x + m[4 + k] * x.y) + (m[8 + k] * x.z + m[12 + k] * 1.0f);
+    float u = v[0] / v[3], vv = v[1] / v[3];
+    if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
+    int ix = (int)((double)((u + 1.0f) * 0.5f * (float)li.width) + 0.5);
+    int iy = (int)((double)((vv + 1.0f) * 0.5f * (float)li.height) + 0.5);
+    ix = min(max(ix, 0), (int)li.width - 1);
+    iy = min(max(iy, 0), (int)li.height - 1);
+    return iy * (int)li.width + ix;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void gi_temporal_kernel(GiArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        GPix gb = load_g(A.g, pix);
+        GRes R = A.cur.load(pix);
+        if (gb.valid && R.M > 0) {
+            GRes m;
+            m.s.xs = m.s.ns = m.s.Lo = mk(0, 0, 0);
+            m.xo = mk(0, 0, 0);
+            m.w = m.W = 0.0f;
+            m.M = 0;
+            float4 ax = A.aux[pix];
+            m.seed = mk(ax.x, ax.y, ax.z);
+            m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
+            int N = R.M;
+            int q = gi_reproject(li, pix, gb.x);
+            if (q >= 0) {
+                GPix pq = load_g(A.pg, (uint32_t)q);
+                if (gi_similar(gb, pq)) {
+                    GRes P = A.prev.load((uint32_t)q);
+                    if (P.M > 0) {
+                        P.M = min(P.M, A.m_cap);
+                        A.prev.store_M((uint32_t)q, P.M);
+                        float J = gi_jacobian(P.xo, gb.x, P.s);
+                        float t = 0.0f;
+                        if (J > 0.0f && gi_target(P.s, gb.x, gb.n) > 0.0f &&
+                            gi_visible(A.sv, gb.x, P.s.xs, stk, ts, rc))
+                            t = gi_target(P.s, gb.x, gb.n);
+                        m.update(P.s, t * P.W * (float)P.M * J);
+                        N += (t > 0.0f ? 1 : 0) * P.M;
+                    }
+                }
+            }
+            m.M = N;
+            float tm = gi_target(m.s, gb.x, gb.n) * (float)m.M;
+            m.W = (tm > 0.0f) ? m.w / tm : 0.0f;
+            m.xo = gb.x;
+            A.cur.store(pix, m);
+            R = m;
+        }
+        if (A.shade) gi_shade(A, pix, gb, R);
+    }
+    wave_count(A.ctr ? A.ctr + 2 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
+    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+    const LaunchInfo &li = A.li;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    TravStats ts;
+    RayCount rc;
+    const uint32_t npix = li.width * li.height;
+    const int S = A.neighbours;
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+        GPix gb = load_g(A.g, pix);
+        GRes R = A.cur.load(pix);
+        if (gb.valid && R.M > 0) {
+            GRes m;
+            m.s.xs = m.s.ns = m.s.Lo = mk(0, 0, 0);
+            m.xo = mk(0, 0, 0);
+            m.w = m.W = 0.0f;
+            m.M = 0;
+            float4 ax = A.aux[pix];
+            m.seed = mk(ax.x, ax.y, ax.z);
+            m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
+            uint32_t used[kGiMaxNeighbours];
+            int Ms[kGiMaxNeighbours];
+            int nu = 0, cnt = 0;
+#pragma unroll
+            for (int i = 0; i < kGiMaxNeighbours; i++) {
+                used[i] = pix;
+                Ms[i] = 0;
+            }
+            for (int i = 0; i < S; i++) {
+                uint32_t q = spatial_neighbour_px(li, pix, m.seed, A.radius);
+                if (q == pix) continue;
+                GPix gq = load_g(A.g, q);
+                if (!gi_similar(gb, gq)) continue;
+                GRes Nb = A.cur.load(q);
+                if (Nb.M <= 0) continue;
+                float J = gi_jacobian(Nb.xo, gb.x, Nb.s);
+                float t = 0.0f;
+                if (J > 0.0f && gi_target(Nb.s, gb.x, gb.n) > 0.0f && gi_visible(A.sv, gb.x, Nb.s.xs, stk, ts, rc))
+                    t = gi_target(Nb.s, gb.x, gb.n);
+                m.update(Nb.s, t * Nb.W * (float)Nb.M * J);
+#pragma unroll
+                for (int k = 0; k < kGiMaxNeighbours; k++)
+                    if (k == nu) {
+                        used[k] = q;
+                        Ms[k] = Nb.M;
+                    }
+                nu++;
+                cnt += Nb.M;
+            }
+            int Z = R.M;
+#pragma unroll
+            for (int k = 0; k < kGiMaxNeighbours; k++) {
+                if (k < nu) {
+                    GPix gq = load_g(A.g, used[k]);
+                    if (gi_target(m.s, gq.x, gq.n) > 0.0f && gi_visible(A.sv, gq.x, m.s.xs, stk, ts, rc)) Z += Ms[k];
+                }
+            }
+            m.M = R.M + cnt;
+            float den = gi_target(m.s, gb.x, gb.n) * (float)Z;
+            m.W = (den > 0.0f) ? m.w / den : 0.0f;
+            m.xo = gb.x;
+            A.prev.store(pix, m);
+            if (A.shade) gi_shade(A, pix, gb, m);
+        } else {
+            A.prev.reset(pix);
+            if (A.shade) {
+                GRes z;
+                z.W = 0.0f;
+                gi_shade(A, pix, gb, z);
+            }
+        }
+    }
+    wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
+    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+}
+
+__global__ void gi_seed_kernel(GPlanes a, GPlanes b, uint32_t w, uint32_t h, uint32_t user) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    GRes r;
+    r.s.xs = r.s.ns = r.s.Lo = mk(0, 0, 0);
+    r.xo = mk(0, 0, 0);
+    r.w = r.W = 0.0f;
+    r.M = 0;
+    r.seed = reservoir_seed(i % w, i / w, user ^ 0x47490000u);
+    a.store(i, r);
+    b.store(i, r);
+}
+
+__global__ void gi_unpack_kernel(GPlanes a, uint32_t n, float *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    GRes r = a.load(i);
+    float v[20] = {r.s.xs.x, r.s.xs.y, r.s.xs.z, r.s.ns.x, r.s.ns.y, r.s.ns.z, r.s.Lo.x, r.s.Lo.y, r.s.Lo.z,
+                   r.xo.x,   r.xo.y,   r.xo.z,   r.w,      r.W,      (float)r.M, r.seed.x, r.seed.y, r.seed.z,
+                   0.0f,     0.0f};
+    for (int k = 0; k < 20; k++) out[20 * (size_t)i + k] = v[k];
+}
+
+}  // namespace
+
+void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed) {
+    size_t n = (size_t)w * h;
+    st.w = w;
+    st.h = h;
+    for (int b = 0; b < 2; b++) {
+        st.res[b].alloc(sizeof(float4) * 5 * n);
+        st.gxd[b].alloc(sizeof(float4) * n);
+        st.gnv[b].alloc(sizeof(float4) * n);
+        st.gmat[b].alloc(sizeof(uint32_t) * n);
+        KRT_HIP(hipMemsetAsync(st.gxd[b].p, 0, st.gxd[b].bytes, ctx.stream));
+        KRT_HIP(hipMemsetAsync(st.gnv[b].p, 0, st.gnv[b].bytes, ctx.stream));
+        KRT_HIP(hipMemsetAsync(st.gmat[b].p, 0, st.gmat[b].bytes, ctx.stream));
+    }
+    st.direct.alloc(sizeof(float4) * n);
+    st.aux.alloc(sizeof(float4) * n);
+    st.cur = 0;
+    st.gcur = 0;
+    GPlanes a{st.res[0].as<float4>(), (uint32_t)n}, b{st.res[1].as<float4>(), (uint32_t)n};
+    gi_seed_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx.stream>>>(a, b, w, h, user_seed);
+    KRT_HIP(hipGetLastError());
+    for (int k = 0; k < 16; k++) st.prev_pv[k] = 0.0f;
+}
+
+void gi_read(Context &ctx, GiState &st, int which, float *host_out) {
+    KRT_REQUIRE(st.w > 0, KRT_ERR_STATE, "ReSTIR GI buffers not loaded");
+    uint32_t n = st.w * st.h;
+    int b = which == 0 ? 1 - st.cur : st.cur;
+    DevBuf tmp(sizeof(float) * 20 * (size_t)n);
+    gi_unpack_kernel<<<(n + 255) / 256, 256, 0, ctx.stream>>>(GPlanes{st.res[b].as<float4>(), n}, n, tmp.as<float>());
+    KRT_HIP(hipGetLastError());
+    KRT_HIP(hipMemcpyAsync(host_out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost, ctx.stream));
+    KRT_HIP(hipStreamSynchronize(ctx.stream));
+}
+
+void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOptions &opt, GiState &st,
+               unsigned long long *ctr, PassTimes &times) {
+    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+    KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR GI buffers not loaded");
+    KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kGiMaxNeighbours, KRT_ERR_INVALID,
+                "restir_gi neighbours must be in [0, 8]");
+    KRT_REQUIRE(opt.spatial_passes >= 0 && opt.m_cap >= 0 && opt.radius >= 0, KRT_ERR_INVALID,
+                "restir_gi options must be non-negative");
+    LaunchInfo li = li_in;
+    for (int k = 0; k < 16; k++) li.prev_pv[k] = st.prev_pv[k];
+    const uint32_t npix = li.width * li.height;
+    uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
+    uint32_t nthreads = nb * kTraceBlock;
+    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
+    if (st.history.bytes < need) st.history.alloc(need);
+    auto planes = [&](int b) { return GPlanes{st.res[b].as<float4>(), npix}; };
+    auto gbuf = [&](int b) { return GBuf{st.gxd[b].as<float4>(), st.gnv[b].as<float4>(), st.gmat[b].as<uint32_t>()}; };
+    GiArgs A;
+    A.li = li;
+    A.sv = make_view(s);
+    A.g = gbuf(st.gcur);
+    A.pg = gbuf(1 - st.gcur);
+    A.direct = st.direct.as<float4>();
+    A.aux = st.aux.as<float4>();
+    A.hist = st.history.as<float4>();
+    A.spill = spill_buffer(ctx, s, nthreads);
+    A.ctr = ctr;
+    A.m_cap = opt.m_cap;
+    A.neighbours = opt.neighbours;
+    A.radius = opt.radius;
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) KRT_HIP(hipEventCreate(&ev[k]));
+    KRT_HIP(hipEventRecord(ev[0], ctx.stream));
+    A.cur = planes(st.cur);
+    A.prev = planes(1 - st.cur);
+    A.shade = 0;
+    gi_initial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    KRT_HIP(hipGetLastError());
+    KRT_HIP(hipEventRecord(ev[1], ctx.stream));
+    A.shade = opt.spatial_passes == 0;
+    gi_temporal_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    KRT_HIP(hipGetLastError());
+    KRT_HIP(hipEventRecord(ev[2], ctx.stream));
+    for (int pass = 0; pass < opt.spatial_passes; pass++) {
+        A.cur = planes(st.cur);
+        A.prev = planes(1 - st.cur);
+        A.shade = pass == opt.spatial_passes - 1;
+        gi_spatial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        KRT_HIP(hipGetLastError());
+        st.cur = 1 - st.cur;
+    }
+    KRT_HIP(hipEventRecord(ev[3], ctx.stream));
+    st.cur = 1 - st.cur;
+    st.gcur = 1 - st.gcur;
+    for (int k = 0; k < 16; k++) st.prev_pv[k] = li.pv[k];
+    KRT_HIP(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; k++) KRT_HIP(hipEventElapsedTime(&times.ms[k], ev[k], ev[k + 1]));
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
+}
+
+}  // namespace krt

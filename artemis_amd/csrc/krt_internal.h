@@ -171,6 +171,17 @@ struct RestirState {
     float prev_pv[16] = {0};
 };
 
+struct GiState {
+    uint32_t w = 0, h = 0;
+    DevBuf res[2];  // 5 float4 planes per buffer
+    int cur = 0;
+    DevBuf gxd[2], gnv[2], gmat[2];  // G-buffer of this and the last frame
+    int gcur = 0;
+    DevBuf direct, aux;
+    DevBuf history;
+    float prev_pv[16] = {0};
+};
+
 struct PassTimes {
     float ms[8] = {0};
 };
@@ -179,6 +190,10 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, un
                PassTimes &times, double *acc64 = nullptr);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
+void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed);
+void gi_read(Context &ctx, GiState &st, int which, float *host_out);
+void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, GiState &st,
+               unsigned long long *ctr, PassTimes &times);
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times);

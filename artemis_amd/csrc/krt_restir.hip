@@ -316,21 +316,6 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
     wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
 }
 
-// sample_spatial_neighborhood (amadeus_restir.cu:295-314)
-KRT_DEV uint32_t spatial_neighbour(const LaunchInfo &li, uint32_t index, f3 &seed, int radius) {
-    int width = (int)li.width, height = (int)li.height;
-    int x = (int)(index % li.width), y = (int)(index / li.width);
-    f3 off = rand_uniform_3f(seed);
-    float c, s;
-    sincos_turn(off.x, c, s);
-    float rr = (float)radius * sqrtf(off.y);
-    int x0 = (int)((float)x + rr * c);
-    int y0 = (int)((float)y + rr * s);
-    x0 = min(max(x0, 0), width - 1);
-    y0 = min(max(y0, 0), height - 1);
-    return (uint32_t)(x0 + y0 * width);
-}
-
 __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
@@ -363,7 +348,7 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
                 nidx[i] = pix;
                 sizes[i] = 0;
                 if (i < S) {
-                    uint32_t i0 = spatial_neighbour(li, pix, merged.seed, A.radius);
+                    uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
                     Resv nb = A.cur.load(i0);
                     float t_nb = occluded_target(A.sv, nb.data, hit, stk, ts, rc);
                     if (nb.M > 0) merged.update(nb.data, t_nb * nb.W * (float)nb.M);

@@ -100,6 +100,22 @@ KRT_DEV void make_ray(const LaunchInfo &li, uint32_t px, uint32_t py, f3 &seed, 
     d = normalize((dx * ld3(li.U) + dy * ld3(li.V)) + ld3(li.W));
 }
 
+// sample_spatial_neighborhood (source/optix/amadeus_restir.cu:295-314): a
+// point in a disk of `radius` pixels, clamped to the image.
+KRT_DEV uint32_t spatial_neighbour_px(const LaunchInfo &li, uint32_t index, f3 &seed, int radius) {
+    int width = (int)li.width, height = (int)li.height;
+    int x = (int)(index % li.width), y = (int)(index / li.width);
+    f3 off = rand_uniform_3f(seed);
+    float c, s;
+    sincos_turn(off.x, c, s);
+    float rr = (float)radius * sqrtf(off.y);
+    int x0 = (int)((float)x + rr * c);
+    int y0 = (int)((float)y + rr * s);
+    x0 = min(max(x0, 0), width - 1);
+    y0 = min(max(y0, 0), height - 1);
+    return (uint32_t)(x0 + y0 * width);
+}
+
 // Path history of one lane: per level {direct, has_pdf} and {T}.  value is
 // then evaluated from the deepest level up, value_k = direct_k + T_k * value_k+1
 // exactly as the reference's recursion returns it.

@@ -115,12 +115,12 @@ KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict
                       const TraceStack &stk, Hit &hit, TravStats &ts) {
     const f3 inv = safe_inv(d);
     const int zmask = zero_mask(d);
+    // Only (t, rank, position) of the best candidate live through the walk;
+    // the hit record is recomputed from the winner at the end (the same
+    // Möller–Trumbore expressions give the same bits).
     float best = tmax;
-    bool found = false;
-    hit.prim = -1;
-    hit.rank = 0x7fffffff;
-    hit.t = 0.0f;
-    hit.b1 = hit.b2 = 0.0f;
+    int best_rank = 0x7fffffff;
+    int best_pos = -1;
     int sp = 0;
     int cur = 0;
     for (;;) {
@@ -148,27 +148,36 @@ KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict
             const int first = cur & 0x03FFFFFF;
             const int cnt = ((cur >> 26) & 31) + 1;
             for (int k = 0; k < cnt; k++) {
-                float b1 = 0.0f, b2 = 0.0f;
+                float b1, b2;
                 int prim, r;
                 float t = mt_test(tri, first + k, o, d, b1, b2, prim, r);
                 ts.tris++;
                 if (ANY) {
                     if (t > 0.0f && t < tmax) return true;
-                } else if (t > 0.0f && (t < best || (found && t == best && r < hit.rank))) {
-                    best = t;
-                    found = true;
-                    hit.t = t;
-                    hit.b1 = b1;
-                    hit.b2 = b2;
-                    hit.prim = prim;
-                    hit.rank = r;
+                } else {
+                    bool take = t > 0.0f && (t < best || (best_pos >= 0 && t == best && r < best_rank));
+                    best = take ? t : best;
+                    best_rank = take ? r : best_rank;
+                    best_pos = take ? first + k : best_pos;
                 }
             }
             if (sp == 0) break;
             cur = stk.get(--sp);
         }
     }
-    return found;
+    hit.prim = -1;
+    hit.rank = 0x7fffffff;
+    hit.t = 0.0f;
+    hit.b1 = hit.b2 = 0.0f;
+    if (ANY || best_pos < 0) return false;
+    float b1 = 0.0f, b2 = 0.0f;
+    int prim, r;
+    hit.t = mt_test(tri, best_pos, o, d, b1, b2, prim, r);
+    hit.b1 = b1;
+    hit.b2 = b2;
+    hit.prim = prim;
+    hit.rank = r;
+    return true;
 }
 
 }  // namespace krt

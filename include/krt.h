@@ -184,9 +184,10 @@ int krt_armada_stats(krt_armada *a, krt_frame_stats *out);
  * the number of frames summed. */
 int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
 /* ReSTIR reservoirs (include/amadeus/reservoir.cuh:19-83) as 16 floats per
- * pixel {Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz}.  which 0: the
- * buffer the next frame's temporal pass merges with (last spatial output),
- * 1: the other one. */
+ * pixel {Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz}; for
+ * "restir_gi" 20 floats {x_s, n_s, L_o, x_owner, w, W, M, seed.xyz, 0, 0}.
+ * which 0: the buffer the next frame's temporal pass merges with (last
+ * spatial output), 1: the other one. */
 int krt_armada_read_reservoirs(krt_armada *a, int which, float *out);
 int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames);
 int krt_armada_sync(krt_armada *a);
