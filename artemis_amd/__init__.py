@@ -58,6 +58,12 @@ class Context:
         except Exception:
             pass
 
+    def copy_probe(self, nbytes: int, reps: int = 10) -> float:
+        """Average ms of a device-to-device streaming copy kernel of nbytes."""
+        ms = C.c_float()
+        L.check(self._lib.krt_copy_probe(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
+
     def camera_rays(self, camera: CameraFrame, width: int, height: int, jitter: bool = False, sample: int = 0,
                     user_seed: int = 0) -> np.ndarray:
         """make_ray on the device for every pixel (8 floats per ray)."""
@@ -286,7 +292,8 @@ class ArmadaRTX:
         L.check(self.ctx._lib.krt_armada_stats(self.h, C.byref(s)))
         return {"closest_rays": s.closest_rays, "shadow_rays": s.shadow_rays, "samples": s.samples,
                 "gpu_ms": s.gpu_ms, "kernel_ms": list(s.kernel_ms), "pass_rays": list(s.pass_rays),
-                "node_visits": s.node_visits, "tri_tests": s.tri_tests}
+                "node_visits": s.node_visits, "tri_tests": s.tri_tests, "pass_nodes": list(s.pass_nodes),
+                "pass_tris": list(s.pass_tris)}
 
     def sync(self) -> None:
         L.check(self.ctx._lib.krt_armada_sync(self.h))

@@ -54,7 +54,8 @@ class Buffers(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
                 ("gpu_ms", C.c_float), ("kernel_ms", C.c_float * 8), ("pass_rays", C.c_uint64 * 4),
-                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("pass_nodes", C.c_uint64 * 4),
+                ("pass_tris", C.c_uint64 * 4)]
 
 
 class AttachmentRet(C.Structure):
@@ -79,6 +80,7 @@ _SIGS = {
     "krt_trace_closest_device": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
     "krt_camera_rays": (C.c_int, [vp, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
                                   fp]),
+    "krt_copy_probe": (C.c_int, [vp, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "krt_armada_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
     "krt_armada_destroy": (None, [vp]),
     "krt_attachment_create": (C.c_int, [vp, C.c_char_p, C.POINTER(vp)]),

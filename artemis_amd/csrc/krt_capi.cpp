@@ -101,7 +101,7 @@ struct krt_armada {
     int samples = 0;
     krt_handle last_scene = 0;
     uint64_t last_version = 0;
-    DevBuf ctr;  // unsigned long long x 8: closest, shadow x3, nodes, tris
+    DevBuf ctr;  // unsigned long long x 16: closest, shadow x3 passes, nodes x3 passes, tris x3 passes
     krt_frame_stats stats{};
     DevBuf acc64;  // FP64 sample sums (path tracer), enabled on request
     bool f64 = false;
@@ -362,6 +362,14 @@ int krt_camera_rays(krt_context *ctx, const krt_camera *cam, uint32_t width, uin
     });
 }
 
+int krt_copy_probe(krt_context *ctx, size_t bytes, int reps, float *ms) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && ms, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(bytes >= 16 && reps >= 1, KRT_ERR_INVALID, "bytes >= 16 and reps >= 1 required");
+        *ms = copy_probe(ctx->impl, bytes, reps);
+    });
+}
+
 // ------------------------------------------------------------------- armada
 int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out) {
     return guard([&] {
@@ -379,7 +387,7 @@ int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_arm
             a->normal.alloc(bytes);
             a->albedo.alloc(bytes);
             a->position.alloc(bytes);
-            a->ctr.alloc(sizeof(unsigned long long) * 8);
+            a->ctr.alloc(sizeof(unsigned long long) * 16);
             for (DevBuf *b : {&a->color, &a->normal, &a->albedo, &a->position})
                 KRT_HIP(hipMemsetAsync(b->p, 0, bytes, ctx->impl.stream));
             KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
@@ -516,7 +524,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         }
         else if (att->kind == "restir") restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
         else gi_render(C, s, li, att->ropt, att->gst, a->ctr.as<unsigned long long>(), times);
-        unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned long long c[16] = {0};
         KRT_HIP(hipMemcpyAsync(c, a->ctr.p, sizeof(c), hipMemcpyDeviceToHost, C.stream));
         KRT_HIP(hipStreamSynchronize(C.stream));
         a->stats.closest_rays = c[0];
@@ -525,8 +533,13 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         a->stats.pass_rays[1] = c[2];
         a->stats.pass_rays[2] = c[3];
         a->stats.pass_rays[3] = 0;
-        a->stats.node_visits = c[4];
-        a->stats.tri_tests = c[5];
+        a->stats.node_visits = c[4] + c[5] + c[6];
+        a->stats.tri_tests = c[7] + c[8] + c[9];
+        for (int k = 0; k < 3; k++) {
+            a->stats.pass_nodes[k] = c[4 + k];
+            a->stats.pass_tris[k] = c[7 + k];
+        }
+        a->stats.pass_nodes[3] = a->stats.pass_tris[3] = 0;
         a->stats.samples = (uint64_t)a->w * a->h;
         float tot = 0.0f;
         for (int k = 0; k < 8; k++) {

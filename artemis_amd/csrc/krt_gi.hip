@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
     wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
 }
 
 KRT_DEV int gi_reproject(const LaunchInfo &li, uint32_t pix, f3 x) {
@@ -348,8 +348,8 @@ __global__ __launch_bounds__(kTraceBlock) void gi_temporal_kernel(GiArgs A) {
         if (A.shade) gi_shade(A, pix, gb, R);
     }
     wave_count(A.ctr ? A.ctr + 2 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
@@ -408,7 +408,9 @@ __global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
             for (int k = 0; k < kGiMaxNeighbours; k++) {
                 if (k < nu) {
                     GPix gq = load_g(A.g, used[k]);
-                    if (gi_target(m.s, gq.x, gq.n) > 0.0f && gi_visible(A.sv, gq.x, m.s.xs, stk, ts, rc)) Z += Ms[k];
+                    if (gi_jacobian(gq.x, gb.x, m.s) > 0.0f && gi_target(m.s, gq.x, gq.n) > 0.0f &&
+                        gi_visible(A.sv, gq.x, m.s.xs, stk, ts, rc))
+                        Z += Ms[k];
                 }
             }
             m.M = R.M + cnt;
@@ -427,8 +429,8 @@ __global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
         }
     }
     wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 6 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 9 : nullptr, ts.tris);
 }
 
 __global__ void gi_seed_kernel(GPlanes a, GPlanes b, uint32_t w, uint32_t h, uint32_t user) {

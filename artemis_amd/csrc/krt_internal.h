@@ -194,6 +194,7 @@ void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_s
 void gi_read(Context &ctx, GiState &st, int which, float *host_out);
 void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, GiState &st,
                unsigned long long *ctr, PassTimes &times);
+float copy_probe(Context &ctx, size_t bytes, int reps);
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times);

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
     wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
 }
 
 }  // namespace

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
     wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
 }
 
 // reproject (amadeus_restir.cu:182-210); pixel indices clamped to the image.
@@ -312,8 +312,8 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
         A.cur.store(pix, merged);
     }
     wave_count(A.ctr ? A.ctr + 2 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
@@ -390,8 +390,8 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
         li.color[pix] = accumulate4(li.color[pix], col, li.accumulate != 0, count);
     }
     wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.tris);
+    wave_count(A.ctr ? A.ctr + 6 : nullptr, ts.nodes);
+    wave_count(A.ctr ? A.ctr + 9 : nullptr, ts.tris);
 }
 
 __global__ void seed_kernel(Planes a, Planes b, uint32_t w, uint32_t h, uint32_t user) {

@@ -46,7 +46,33 @@ __global__ void camera_rays_kernel(LaunchInfo li, int jitter, float4 *rays) {
     rays[2 * pix + 1] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
+// Plain streaming copy (HBM bandwidth and PMC byte-counter calibration).
+__global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                     size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+float copy_probe(Context &ctx, size_t bytes, int reps) {
+    size_t n = bytes / sizeof(float4);
+    DevBuf a(n * sizeof(float4)), b(n * sizeof(float4));
+    KRT_HIP(hipMemsetAsync(a.p, 0, a.bytes, ctx.stream));
+    hipEvent_t e0, e1;
+    KRT_HIP(hipEventCreate(&e0));
+    KRT_HIP(hipEventCreate(&e1));
+    copy_kernel<<<256 * 8, 256, 0, ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);  // warm
+    KRT_HIP(hipEventRecord(e0, ctx.stream));
+    for (int r = 0; r < reps; r++) copy_kernel<<<256 * 8, 256, 0, ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);
+    KRT_HIP(hipEventRecord(e1, ctx.stream));
+    KRT_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    KRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / (float)reps;
+}
 
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays) {
     uint32_t n = li.width * li.height;

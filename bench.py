@@ -44,6 +44,15 @@ def b_ray(name):
     return data[name]
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    return json.loads(path.read_text()).get(kernel)
+
+
 def cpu_baseline(workload, scene, W, H, seconds):
     """The oracle (restated reference algorithm) on the host cores, on a
     bounded sample: the same scene at a reduced resolution, frames until
@@ -131,6 +140,7 @@ def main():
         log(f"[bench] warmup {i + 1}/{args.warmup}: {arm.stats()['gpu_ms']:.2f} ms")
     arm.sync()
     lat, rays, kms, prays, nodes, tris = [], 0, np.zeros(8), np.zeros(4), 0, 0
+    pnodes, ptris = np.zeros(4), np.zeros(4)
     t0 = time.perf_counter()
     for i in range(args.steps):
         f0 = time.perf_counter()
@@ -142,6 +152,8 @@ def main():
         prays += np.array(st["pass_rays"], dtype=np.float64)
         nodes += st["node_visits"]
         tris += st["tri_tests"]
+        pnodes += np.array(st["pass_nodes"], dtype=np.float64)
+        ptris += np.array(st["pass_tris"], dtype=np.float64)
     arm.sync()
     dt = time.perf_counter() - t0
     K = args.steps
@@ -156,12 +168,31 @@ def main():
         passes = [0]
         per_pass_bytes = {0: PT_PIXEL_BYTES * npix + prays[0] / K * br["b_ray"]}
     dom = max(passes, key=lambda p: kavg[p])
-    achieved = per_pass_bytes[dom] / (kavg[dom] * 1e-3) / 1e9
+    # Algorithmic bytes of a launch: the per-pixel resampling traffic of the
+    # pass plus, per ray, the node (64 B) and triangle (48 B) records its
+    # traversal fetches (counted by the kernel).  The reference-walk figure of
+    # the same rays (48 N_node + 64 N_tri over the bvh.cpp tree, b_ray) is
+    # reported beside it as ref_*.
+    pn = np.zeros(4)
+    pt = np.zeros(4)
+    for k in range(3):
+        pn[k] = pnodes[k] / K
+        pt[k] = ptris[k] / K
+    pix_bytes = (RESTIR_PIXEL_BYTES[dom] if args.workload == "restir_di" else PT_PIXEL_BYTES) * npix
+    alg_bytes = pix_bytes + 64.0 * pn[dom] + 48.0 * pt[dom]
+    ref_bytes = per_pass_bytes[dom]
+    sec = kavg[dom] * 1e-3
+    achieved = alg_bytes / sec / 1e9
     total_rays = rays / K
-    roof = {"bound": "hbm", "kernel": f"{args.workload}:{PASS_NAMES.get(dom, 'pt')}", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "b_ray": br["b_ray"], "ms": float(kavg[dom]),
-            "own_bytes_per_ray": float((64 * nodes + 48 * tris) / max(rays, 1))}
+    kname = f"{args.workload}:{PASS_NAMES.get(dom, 'pt')}"
+    traffic = measured_traffic(kname)
+    copy_ms = ctx.copy_probe(1 << 30, 10)
+    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg_bytes,
+            "ms": float(kavg[dom]), "rays": float(prays[dom] / K),
+            "bytes_per_ray": float((64.0 * pn[dom] + 48.0 * pt[dom]) / max(prays[dom] / K, 1)),
+            "ref_bytes_per_ray": br["b_ray"], "ref_achieved": ref_bytes / sec / 1e9,
+            "copy_kernel_gbs": 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
     line = {
         "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
         "value": value,

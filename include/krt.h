@@ -114,6 +114,11 @@ int krt_trace_occluded(krt_context *ctx, krt_handle scene, const float *rays, ui
  * context stream, no synchronisation. */
 int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n, void *hits_dev);
 
+/* Streaming copy of `bytes` between two device buffers, `reps` times: average
+ * milliseconds per copy (HBM bandwidth and profiler byte-counter calibration;
+ * no reference counterpart). */
+int krt_copy_probe(krt_context *ctx, size_t bytes, int reps, float *ms);
+
 /* ---------------------------------------------------------------- render --
  * ArmadaRTX (armada.cuh:142-336) and its attachments. */
 typedef struct {
@@ -151,6 +156,8 @@ typedef struct {
     uint64_t pass_rays[4]; /* rays traced per pass, same indexing */
     uint64_t node_visits;  /* traversal nodes fetched by all rays of the frame */
     uint64_t tri_tests;    /* ray-triangle tests of the frame */
+    uint64_t pass_nodes[4];
+    uint64_t pass_tris[4];
 } krt_frame_stats;
 
 /* ArmadaRTX(context, accelerator, mesh daemon, extent) (armada.cu:28-83) */

@@ -1562,7 +1562,9 @@ void orc_gi_frame(orc_gi *r, const orc_scene *s, const orc_camera *cam, const or
                     int Z = R.M;
                     for (int k = 0; k < nu; k++) {
                         const GIG &gq = G[used[k]];
-                        if (gi_target(m.s, gq.x, gq.n) > 0.0f && gi_visible(cx, gq.x, m.s.xs)) Z += Ms[k];
+                        if (gi_jacobian(gq.x, gb.x, m.s) > 0.0f && gi_target(m.s, gq.x, gq.n) > 0.0f &&
+                            gi_visible(cx, gq.x, m.s.xs))
+                            Z += Ms[k];
                     }
                     m.M = R.M + cnt;
                     float den = gi_target(m.s, gb.x, gb.n) * (float)Z;

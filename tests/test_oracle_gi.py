@@ -1,7 +1,7 @@
 """ReSTIR GI oracle (our algorithm; there is no reference implementation).
 Pinned by the reconnection Jacobian against the solid-angle ratio measured on
 a small surface patch, and by the estimator's mean against the converged path
-tracer of the same depth (ReSTIR GI reuses the path tracer's secondary paths,
+tracer of the same depth with the same fixed pixel offset (ReSTIR GI reuses the path tracer's secondary paths,
 so its expectation is the path tracer's)."""
 import numpy as np
 import pytest
@@ -46,18 +46,32 @@ def test_jacobian_equals_solid_angle_ratio(seed):
         assert J == pytest.approx(ratio, rel=2e-3)
 
 
-def test_gi_mean_matches_path_tracer():
-    s = scenes.cornell()
+def _gi_mean_ratio(spatial_passes, seeds, frames=128, m_cap=5):
+    # sky_room: lights far from every lit surface, so both estimators have
+    # low variance and a few percent of bias is visible above the noise
+    s = scenes.sky_room()
     osc = O.OracleScene(s)
     W = H = 24
     cam = look_at(**s.camera)
-    frames = 512
-    gi = O.OracleGI(W, H)
-    for f in range(frames):
-        gi.frame(osc, cam, max_depth=3, m_cap=30, spatial_passes=2, neighbours=5, radius=8, nthreads=8)
-    acc = osc.accumulate_pt_f64(cam, W, H, 0, 2048, max_depth=3, nthreads=8) / 2048.0
-    g = gi.color[..., :3].astype(np.float64)
-    # same expectation: image means agree and per-pixel relative error is small
-    assert g.mean() == pytest.approx(acc.mean(), rel=0.03)
-    rel = np.mean((g - acc) ** 2 / (acc ** 2 + 1e-2))
-    assert rel < 0.02
+    ref = osc.accumulate_pt_f64(cam, W, H, 0, 2048, max_depth=3, nthreads=8, fixed_offset=True) / 2048.0
+    means = []
+    for seed in seeds:
+        gi = O.OracleGI(W, H, user_seed=seed)
+        for _ in range(frames):
+            gi.frame(osc, cam, max_depth=3, m_cap=m_cap, spatial_passes=spatial_passes, neighbours=5, radius=8,
+                     nthreads=8)
+        means.append(gi.color[..., :3].astype(np.float64).mean())
+    return np.mean(means) / ref.mean()
+
+
+def test_gi_temporal_reuse_mean_matches_path_tracer():
+    # static camera: every temporal neighbour is the same visible point, so the
+    # Jacobian is 1 and the reused estimator has the path tracer's expectation
+    assert _gi_mean_ratio(0, range(3)) == pytest.approx(1.0, abs=0.01)
+
+
+def test_gi_spatial_reuse_mean_matches_path_tracer():
+    # Two spatial passes at M-cap 20.  Counting a neighbour in 1/Z when the
+    # shift from it to this pixel is rejected (Jacobian above the cap) reads
+    # 0.96 here; the estimator as written reads 1.008.
+    assert _gi_mean_ratio(2, range(8), frames=96, m_cap=20) == pytest.approx(1.0, abs=0.025)
