@@ -18,7 +18,7 @@ from .camera import CameraFrame, look_at
 from .scenes import SceneData
 
 __all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "CameraFrame", "look_at", "SceneData",
-           "load_attachment"]
+           "load_attachment", "RadianceField"]
 
 
 def _f(a):
@@ -194,6 +194,100 @@ class ReSTIRGI(_Attachment):
 def load_attachment(ctx: Context, kind: str):
     """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
     return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI}[kind](ctx)
+
+
+class RadianceField:
+    """Online-trained radiance-cache MLP over krt_mlp_* (the reference's
+    RadianceField, editor/mlp.cuh:71-154, trained here).  Queries are
+    (n, 13) float32 {pos, dir, normal, albedo, roughness}; outputs and
+    targets (n, 3) float32."""
+
+    RAW, OUT, PARAMS = L.MLP_RAW, L.MLP_OUT, L.MLP_PARAMS
+
+    def __init__(self, ctx: Context, seed: int = 0, lr: float | None = None, loss_scale: float | None = None,
+                 growth_interval: int | None = None, pos_min=(0.0, 0.0, 0.0), pos_scale=(1.0, 1.0, 1.0),
+                 beta1: float | None = None, beta2: float | None = None, eps: float | None = None):
+        self.ctx = ctx
+        cfg = L.MlpConfig()
+        ctx._lib.krt_mlp_default_config(C.byref(cfg))
+        for name, val in (("lr", lr), ("loss_scale", loss_scale), ("growth_interval", growth_interval),
+                          ("beta1", beta1), ("beta2", beta2), ("eps", eps)):
+            if val is not None:
+                setattr(cfg, name, val)
+        cfg.pos_min[:] = [float(x) for x in pos_min]
+        cfg.pos_scale[:] = [float(x) for x in pos_scale]
+        self.config = cfg
+        h = C.c_void_p()
+        L.check(ctx._lib.krt_mlp_create(ctx.h, C.byref(cfg), seed, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            self.ctx._lib.krt_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _rows(a, width):
+        return np.ascontiguousarray(a, np.float32).reshape(-1, width)
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.PARAMS, np.float32)
+        L.check(self.ctx._lib.krt_mlp_get_params(self.h, _f(out)))
+        return out
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (self.PARAMS,)
+        L.check(self.ctx._lib.krt_mlp_set_params(self.h, _f(p)))
+
+    def set_encoding(self, pos_min, pos_scale) -> None:
+        a = np.ascontiguousarray(pos_min, np.float32)
+        b = np.ascontiguousarray(pos_scale, np.float32)
+        L.check(self.ctx._lib.krt_mlp_set_encoding(self.h, _f(a), _f(b)))
+
+    def infer(self, raw) -> np.ndarray:
+        raw = self._rows(raw, self.RAW)
+        out = np.empty((len(raw), self.OUT), np.float32)
+        L.check(self.ctx._lib.krt_mlp_infer(self.h, _f(raw), len(raw), _f(out)))
+        return out
+
+    def infer_device(self, raw_ptr: int, n: int, out_ptr: int) -> None:
+        L.check(self.ctx._lib.krt_mlp_infer_device(self.h, C.c_void_p(raw_ptr), n, C.c_void_p(out_ptr)))
+
+    def train_step(self, raw, target) -> float:
+        raw, target = self._rows(raw, self.RAW), self._rows(target, self.OUT)
+        assert len(raw) == len(target)
+        loss = C.c_float()
+        L.check(self.ctx._lib.krt_mlp_train_step(self.h, _f(raw), _f(target), len(raw), C.byref(loss)))
+        return loss.value
+
+    def train_step_device(self, raw_ptr: int, target_ptr: int, n: int) -> None:
+        L.check(self.ctx._lib.krt_mlp_train_step_device(self.h, C.c_void_p(raw_ptr), C.c_void_p(target_ptr), n))
+
+    def gradient(self, raw, target):
+        """(loss, gradient) of one batch without an optimiser step."""
+        raw, target = self._rows(raw, self.RAW), self._rows(target, self.OUT)
+        g = np.empty(self.PARAMS, np.float32)
+        loss = C.c_float()
+        L.check(self.ctx._lib.krt_mlp_gradient(self.h, _f(raw), _f(target), len(raw), _f(g), C.byref(loss)))
+        return loss.value, g
+
+    def apply_gradient(self, grad) -> None:
+        g = np.ascontiguousarray(grad, np.float32)
+        assert g.shape == (self.PARAMS,)
+        L.check(self.ctx._lib.krt_mlp_apply_gradient(self.h, _f(g)))
+
+    def state(self) -> dict:
+        st = L.MlpState()
+        L.check(self.ctx._lib.krt_mlp_get_state(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
 
 
 class ArmadaRTX:

@@ -51,6 +51,21 @@ class Buffers(C.Structure):
     _fields_ = [("color", vp), ("normal", vp), ("albedo", vp), ("position", vp)]
 
 
+class MlpConfig(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("loss_scale", C.c_float), ("growth_interval", C.c_int32), ("pos_min", C.c_float * 3),
+                ("pos_scale", C.c_float * 3)]
+
+
+class MlpState(C.Structure):
+    _fields_ = [("loss_scale", C.c_float), ("beta1_t", C.c_float), ("beta2_t", C.c_float),
+                ("last_loss", C.c_float), ("step", C.c_int32), ("good_steps", C.c_int32),
+                ("skipped_steps", C.c_int32), ("pending_nonfinite", C.c_int32)]
+
+
+MLP_RAW, MLP_OUT, MLP_PARAMS = 13, 3, 20867
+
+
 class FrameStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
                 ("gpu_ms", C.c_float), ("kernel_ms", C.c_float * 8), ("pass_rays", C.c_uint64 * 4),
@@ -100,6 +115,19 @@ _SIGS = {
     "krt_armada_read_reservoirs": (C.c_int, [vp, C.c_int, fp]),
     "krt_armada_read_f64": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "krt_load_attachment": (AttachmentRet, [vp, C.c_char_p]),
+    "krt_mlp_default_config": (None, [C.POINTER(MlpConfig)]),
+    "krt_mlp_create": (C.c_int, [vp, C.POINTER(MlpConfig), C.c_uint64, C.POINTER(vp)]),
+    "krt_mlp_destroy": (None, [vp]),
+    "krt_mlp_get_params": (C.c_int, [vp, fp]),
+    "krt_mlp_set_params": (C.c_int, [vp, fp]),
+    "krt_mlp_set_encoding": (C.c_int, [vp, fp, fp]),
+    "krt_mlp_infer": (C.c_int, [vp, fp, C.c_uint32, fp]),
+    "krt_mlp_infer_device": (C.c_int, [vp, vp, C.c_uint32, vp]),
+    "krt_mlp_train_step": (C.c_int, [vp, fp, fp, C.c_uint32, fp]),
+    "krt_mlp_train_step_device": (C.c_int, [vp, vp, vp, C.c_uint32]),
+    "krt_mlp_gradient": (C.c_int, [vp, fp, fp, C.c_uint32, fp, fp]),
+    "krt_mlp_apply_gradient": (C.c_int, [vp, fp]),
+    "krt_mlp_get_state": (C.c_int, [vp, C.POINTER(MlpState)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -648,3 +648,174 @@ krt_attachment_ret krt_load_attachment(krt_context *ctx, const char *kind) {
 }
 
 }  // extern "C"
+
+// --------------------------------------------------------- radiance cache
+struct krt_mlp {
+    Context *ctx = nullptr;
+    Mlp net;
+};
+
+namespace {
+
+struct HostArray {  // device copy of a host array for the host-pointer entry points
+    DevBuf buf;
+    HostArray(Context &ctx, const float *src, size_t count) : buf(sizeof(float) * count) {
+        KRT_HIP(hipMemcpyAsync(buf.p, src, sizeof(float) * count, hipMemcpyHostToDevice, ctx.stream));
+    }
+};
+
+}  // namespace
+
+void krt_mlp_default_config(krt_mlp_config *cfg) {
+    if (!cfg) return;
+    MlpConfig d;
+    MlpEncoding e;
+    cfg->lr = d.lr;
+    cfg->beta1 = d.b1;
+    cfg->beta2 = d.b2;
+    cfg->eps = d.eps;
+    cfg->loss_scale = d.loss_scale;
+    cfg->growth_interval = d.growth_interval;
+    for (int a = 0; a < 3; a++) {
+        cfg->pos_min[a] = e.pmin[a];
+        cfg->pos_scale[a] = e.pscale[a];
+    }
+}
+
+int krt_mlp_create(krt_context *ctx, const krt_mlp_config *cfg, uint64_t seed, krt_mlp **out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        krt_mlp_config c;
+        krt_mlp_default_config(&c);
+        if (cfg) c = *cfg;
+        KRT_REQUIRE(c.lr > 0.0f && c.beta1 >= 0.0f && c.beta1 < 1.0f && c.beta2 >= 0.0f && c.beta2 < 1.0f &&
+                        c.eps >= 0.0f && c.loss_scale > 0.0f && c.growth_interval > 0,
+                    KRT_ERR_INVALID, "invalid optimiser configuration");
+        auto *m = new krt_mlp();
+        m->ctx = &ctx->impl;
+        m->net.cfg = MlpConfig{c.lr, c.beta1, c.beta2, c.eps, c.loss_scale, c.growth_interval};
+        for (int a = 0; a < 3; a++) {
+            m->net.enc.pmin[a] = c.pos_min[a];
+            m->net.enc.pscale[a] = c.pos_scale[a];
+        }
+        try {
+            mlp_init(*m->ctx, m->net, m->net.cfg, seed);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+void krt_mlp_destroy(krt_mlp *net) {
+    if (!net) return;
+    (void)hipStreamSynchronize(net->ctx->stream);
+    delete net;
+}
+
+int krt_mlp_get_params(krt_mlp *net, float *params) {
+    return guard([&] {
+        KRT_REQUIRE(net && params, KRT_ERR_INVALID, "null argument");
+        mlp_get_params(*net->ctx, net->net, params);
+    });
+}
+
+int krt_mlp_set_params(krt_mlp *net, const float *params) {
+    return guard([&] {
+        KRT_REQUIRE(net && params, KRT_ERR_INVALID, "null argument");
+        mlp_set_params(*net->ctx, net->net, params);
+    });
+}
+
+int krt_mlp_set_encoding(krt_mlp *net, const float pos_min[3], const float pos_scale[3]) {
+    return guard([&] {
+        KRT_REQUIRE(net && pos_min && pos_scale, KRT_ERR_INVALID, "null argument");
+        for (int a = 0; a < 3; a++) {
+            net->net.enc.pmin[a] = pos_min[a];
+            net->net.enc.pscale[a] = pos_scale[a];
+        }
+    });
+}
+
+int krt_mlp_infer_device(krt_mlp *net, const float *raw_dev, uint32_t n, float *out_dev) {
+    return guard([&] {
+        KRT_REQUIRE(net && (n == 0 || (raw_dev && out_dev)), KRT_ERR_INVALID, "null argument");
+        mlp_infer_dev(*net->ctx, net->net, raw_dev, n, out_dev);
+    });
+}
+
+int krt_mlp_infer(krt_mlp *net, const float *raw, uint32_t n, float *out) {
+    return guard([&] {
+        KRT_REQUIRE(net && (n == 0 || (raw && out)), KRT_ERR_INVALID, "null argument");
+        if (n == 0) return;
+        Context &ctx = *net->ctx;
+        HostArray in(ctx, raw, (size_t)kMlpRaw * n);
+        DevBuf o(sizeof(float) * kMlpOut * (size_t)n);
+        mlp_infer_dev(ctx, net->net, in.buf.as<float>(), n, o.as<float>());
+        KRT_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * kMlpOut * (size_t)n, hipMemcpyDeviceToHost, ctx.stream));
+        KRT_HIP(hipStreamSynchronize(ctx.stream));
+    });
+}
+
+static void read_state(Context &ctx, Mlp &net, MlpDevState &st) {
+    KRT_HIP(hipMemcpyAsync(&st, net.state.p, sizeof(st), hipMemcpyDeviceToHost, ctx.stream));
+    KRT_HIP(hipStreamSynchronize(ctx.stream));
+}
+
+int krt_mlp_train_step_device(krt_mlp *net, const float *raw_dev, const float *target_dev, uint32_t n) {
+    return guard([&] {
+        KRT_REQUIRE(net && raw_dev && target_dev, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
+        mlp_gradient_dev(*net->ctx, net->net, raw_dev, target_dev, n);
+        mlp_apply_dev(*net->ctx, net->net);
+    });
+}
+
+int krt_mlp_train_step(krt_mlp *net, const float *raw, const float *target, uint32_t n, float *loss) {
+    return guard([&] {
+        KRT_REQUIRE(net && raw && target, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
+        Context &ctx = *net->ctx;
+        HostArray in(ctx, raw, (size_t)kMlpRaw * n), tg(ctx, target, (size_t)kMlpOut * n);
+        mlp_gradient_dev(ctx, net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
+        mlp_apply_dev(ctx, net->net);
+        MlpDevState st;
+        read_state(ctx, net->net, st);
+        if (loss) *loss = st.loss;
+    });
+}
+
+int krt_mlp_gradient(krt_mlp *net, const float *raw, const float *target, uint32_t n, float *grad, float *loss) {
+    return guard([&] {
+        KRT_REQUIRE(net && raw && target && grad, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
+        Context &ctx = *net->ctx;
+        HostArray in(ctx, raw, (size_t)kMlpRaw * n), tg(ctx, target, (size_t)kMlpOut * n);
+        mlp_gradient_dev(ctx, net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
+        KRT_HIP(hipMemcpyAsync(grad, net->net.grad.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost,
+                               ctx.stream));
+        mlp_clear_flag(ctx, net->net);
+        MlpDevState st;
+        read_state(ctx, net->net, st);
+        if (loss) *loss = st.loss;
+    });
+}
+
+int krt_mlp_apply_gradient(krt_mlp *net, const float *grad) {
+    return guard([&] {
+        KRT_REQUIRE(net && grad, KRT_ERR_INVALID, "null argument");
+        mlp_apply_host_gradient(*net->ctx, net->net, grad);
+        KRT_HIP(hipStreamSynchronize(net->ctx->stream));
+    });
+}
+
+int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state) {
+    return guard([&] {
+        KRT_REQUIRE(net && state, KRT_ERR_INVALID, "null argument");
+        MlpDevState st;
+        read_state(*net->ctx, net->net, st);
+        static_assert(sizeof(st) == sizeof(krt_mlp_state), "state layout");
+        std::memcpy(state, &st, sizeof(st));
+    });
+}

@@ -199,4 +199,50 @@ void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times);
 
+
+// Radiance-cache MLP (krt_mlp.hip).  Parameters in canonical order: per layer
+// W[out][in] row-major, then b[out].
+constexpr int kMlpRaw = 13;      // pos, dir, normal, albedo, roughness
+constexpr int kMlpEncoded = 62;  // frequency encoding width
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOut = 3;
+constexpr int kMlpLayers = 6;
+constexpr int kMlpParams = 20867;
+
+struct MlpEncoding {
+    float pmin[3] = {0.0f, 0.0f, 0.0f};
+    float pscale[3] = {1.0f, 1.0f, 1.0f};
+};
+
+struct MlpConfig {
+    float lr = 1e-2f, b1 = 0.9f, b2 = 0.99f, eps = 1e-15f;
+    float loss_scale = 1024.0f;
+    int growth_interval = 2000;
+};
+
+// Optimiser state kept on the device so that training steps need no host
+// round trip (layout of krt_mlp_state).
+struct MlpDevState {
+    float scale, b1t, b2t, loss;
+    int step, good, skipped, bad;
+};
+
+struct Mlp {
+    MlpConfig cfg;
+    MlpEncoding enc;
+    DevBuf params, m, v, grad;  // FP32 master weights and Adam moments
+    DevBuf frag_f, frag_b;      // bf16 MFMA fragments of W and W^T
+    DevBuf state;               // MlpDevState
+    DevBuf partial;             // per-workgroup gradient partials
+};
+
+void mlp_init(Context &ctx, Mlp &net, const MlpConfig &cfg, uint64_t seed);
+void mlp_set_params(Context &ctx, Mlp &net, const float *host);
+void mlp_get_params(Context &ctx, Mlp &net, float *host);
+void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *out);
+void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n);
+void mlp_apply_dev(Context &ctx, Mlp &net);
+void mlp_apply_host_gradient(Context &ctx, Mlp &net, const float *grad);
+void mlp_clear_flag(Context &ctx, Mlp &net);
+
 }  // namespace krt

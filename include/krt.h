@@ -207,6 +207,59 @@ typedef struct {
 } krt_attachment_ret;
 krt_attachment_ret krt_load_attachment(krt_context *ctx, const char *kind);
 
+/* ------------------------------------------------------- radiance cache --
+ * The online-trained radiance-cache MLP.  It takes the place of
+ * editor/mlp.cuh:71-154 (RadianceField, dev_radiance_field, forward<B>,
+ * release), whose network is a never-compiled FP32 forward pass; here
+ * 62 encoded inputs -> 5 x 64 ReLU -> 3 outputs, BF16 on MFMA with FP32
+ * accumulation, trained by Adam with dynamic loss scaling.
+ * Queries: KRT_MLP_RAW floats each {pos.xyz, dir.xyz, normal.xyz,
+ * albedo.rgb, roughness}; pos is mapped to [0,1] by
+ * (pos - pos_min) * pos_scale before encoding.  Outputs / targets: 3 floats
+ * (RGB radiance).  Parameters: KRT_MLP_PARAMS floats, per layer W[out][in]
+ * row-major then b[out] (mlp.cuh:48-67 ordering). */
+#define KRT_MLP_RAW 13
+#define KRT_MLP_OUT 3
+#define KRT_MLP_PARAMS 20867
+
+typedef struct krt_mlp krt_mlp;
+
+typedef struct {
+    float lr, beta1, beta2, eps;  /* Adam */
+    float loss_scale;             /* initial loss scale */
+    int32_t growth_interval;      /* finite steps before the scale doubles */
+    float pos_min[3], pos_scale[3];
+} krt_mlp_config;
+
+typedef struct {
+    float loss_scale, beta1_t, beta2_t, last_loss;
+    int32_t step, good_steps, skipped_steps, pending_nonfinite;
+} krt_mlp_state;
+
+void krt_mlp_default_config(krt_mlp_config *cfg);
+/* dev_radiance_field (mlp.cuh:109-130): weights and biases
+ * xavier(n, m) * (u - 0.5) with u from a seeded SplitMix64 stream. */
+int krt_mlp_create(krt_context *ctx, const krt_mlp_config *cfg, uint64_t seed, krt_mlp **out);
+void krt_mlp_destroy(krt_mlp *net); /* release (mlp.cuh:132-150) */
+int krt_mlp_get_params(krt_mlp *net, float *params);
+int krt_mlp_set_params(krt_mlp *net, const float *params);
+int krt_mlp_set_encoding(krt_mlp *net, const float pos_min[3], const float pos_scale[3]);
+/* forward<B> (mlp.cuh:153-163) for n queries; host or device arrays. */
+int krt_mlp_infer(krt_mlp *net, const float *raw, uint32_t n, float *out);
+int krt_mlp_infer_device(krt_mlp *net, const float *raw_dev, uint32_t n, float *out_dev);
+/* One optimiser step on n (query, target) pairs: relative L2 loss
+ * (y - t)^2 / (y^2 + 0.01) averaged over 3n outputs.  The host variant
+ * returns the batch loss; the device variant queues the step and returns. */
+int krt_mlp_train_step(krt_mlp *net, const float *raw, const float *target, uint32_t n, float *loss);
+int krt_mlp_train_step_device(krt_mlp *net, const float *raw_dev, const float *target_dev, uint32_t n);
+/* Gradient of the batch loss (unscaled, canonical order) without a step, and
+ * an Adam step with a caller-supplied gradient (e.g. after a cross-device
+ * reduction). */
+int krt_mlp_gradient(krt_mlp *net, const float *raw, const float *target, uint32_t n, float *grad,
+                     float *loss);
+int krt_mlp_apply_gradient(krt_mlp *net, const float *grad);
+int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,12 @@ _SIGS = {
     "orc_gi_reservoirs": (None, [vp, C.c_int, fp]),
     "orc_gi_jacobian": (C.c_float, [fp, fp, fp, fp]),
     "orc_camera_rays": (None, [C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int, fp]),
+    "orc_mlp_param_count": (C.c_int, []),
+    "orc_mlp_encode": (None, [fp, C.c_int, fp, fp, fp]),
+    "orc_mlp_forward": (None, [dp, fp, C.c_int, fp, fp, C.c_int, dp]),
+    "orc_mlp_loss": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, dp]),
+    "orc_mlp_grad": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, C.c_int, C.c_float, dp]),
+    "orc_mlp_adam": (None, [fp, fp, fp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp]),
 }
 
 _lib = None
@@ -328,3 +334,73 @@ def camera_rays(frame, width, height, jitter=False, sample=0, user_seed=0) -> np
     p = params(width, height, samples=sample, user_seed=user_seed)
     lib().orc_camera_rays(C.byref(cam), C.byref(p), int(jitter), _f(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Radiance-cache MLP (mlp.cpp)
+# ---------------------------------------------------------------------------
+MLP_RAW = 13
+MLP_FEATURES = 64
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _mlp_args(raw, pmin, pscale):
+    raw = np.ascontiguousarray(raw, np.float32).reshape(-1, MLP_RAW)
+    pmin = np.ascontiguousarray(pmin, np.float32).reshape(3)
+    pscale = np.ascontiguousarray(pscale, np.float32).reshape(3)
+    return raw, pmin, pscale
+
+
+def mlp_param_count() -> int:
+    return int(lib().orc_mlp_param_count())
+
+
+def mlp_encode(raw, pmin=(0, 0, 0), pscale=(1, 1, 1)) -> np.ndarray:
+    raw, pmin, pscale = _mlp_args(raw, pmin, pscale)
+    out = np.empty((len(raw), MLP_FEATURES), np.float32)
+    lib().orc_mlp_encode(_f(raw), len(raw), _f(pmin), _f(pscale), _f(out))
+    return out
+
+
+def mlp_forward(params, raw, pmin=(0, 0, 0), pscale=(1, 1, 1), bf16=False) -> np.ndarray:
+    params = np.ascontiguousarray(params, np.float64)
+    raw, pmin, pscale = _mlp_args(raw, pmin, pscale)
+    out = np.empty((len(raw), 3), np.float64)
+    lib().orc_mlp_forward(_d(params), _f(raw), len(raw), _f(pmin), _f(pscale), int(bf16), _d(out))
+    return out
+
+
+def mlp_loss(params, raw, target, pmin=(0, 0, 0), pscale=(1, 1, 1), den=None) -> float:
+    params = np.ascontiguousarray(params, np.float64)
+    raw, pmin, pscale = _mlp_args(raw, pmin, pscale)
+    target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    dptr = None
+    if den is not None:
+        den = np.ascontiguousarray(den, np.float64).reshape(-1, 3)
+        dptr = _d(den)
+    return float(lib().orc_mlp_loss(_d(params), _f(raw), _f(target), len(raw), _f(pmin), _f(pscale), dptr))
+
+
+def mlp_grad(params, raw, target, pmin=(0, 0, 0), pscale=(1, 1, 1), bf16=False, loss_scale=1.0):
+    """(loss, gradient) of the mean relative L2 loss, gradient in canonical order."""
+    params = np.ascontiguousarray(params, np.float64)
+    raw, pmin, pscale = _mlp_args(raw, pmin, pscale)
+    target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    g = np.empty(mlp_param_count(), np.float64)
+    loss = lib().orc_mlp_grad(_d(params), _f(raw), _f(target), len(raw), _f(pmin), _f(pscale), int(bf16),
+                              float(loss_scale), _d(g))
+    return float(loss), g
+
+
+def mlp_adam(params, m, v, grad, lr, b1, b2, eps, b1t, b2t):
+    """One FP32 Adam step in place; returns the advanced (b1t, b2t)."""
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    grad = np.ascontiguousarray(grad, np.float32)
+    t1 = np.array([b1t], np.float32)
+    t2 = np.array([b2t], np.float32)
+    lib().orc_mlp_adam(_f(params), _f(m), _f(v), _f(grad), len(params), lr, b1, b2, eps, _f(t1), _f(t2))
+    return float(t1[0]), float(t2[0])

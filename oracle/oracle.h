@@ -139,6 +139,21 @@ void orc_gi_frame(orc_gi *r, const orc_scene *s, const orc_camera *cam, const or
 void orc_gi_reservoirs(const orc_gi *r, int which, float *out);
 /* camera rays of make_ray (8 floats each), jittered (path tracer) or fixed -0.5 offset (ReSTIR) */
 void orc_camera_rays(const orc_camera *cam, const orc_params *p, int jitter, float *rays);
+
+/* Radiance-cache MLP (mlp.cpp): 13 raw inputs per query {pos, dir, normal,
+ * albedo, roughness}, pos mapped to [0,1] by (pos - pmin) * pscale; 64
+ * features (62 used); params in canonical order, per layer W[out][in] then
+ * b[out].  bf16 != 0 selects the kernels' rounding points. */
+int orc_mlp_param_count(void);
+void orc_mlp_encode(const float *raw, int n, const float *pmin, const float *pscale, float *feat);
+void orc_mlp_forward(const double *params, const float *raw, int n, const float *pmin, const float *pscale,
+                     int bf16, double *out);
+double orc_mlp_loss(const double *params, const float *raw, const float *target, int n, const float *pmin,
+                    const float *pscale, const double *den);
+double orc_mlp_grad(const double *params, const float *raw, const float *target, int n, const float *pmin,
+                    const float *pscale, int bf16, float loss_scale, double *grad);
+void orc_mlp_adam(float *params, float *m, float *v, const float *grad, int n, float lr, float b1, float b2,
+                  float eps, float *b1t, float *b2t);
 /* reconnection-shift Jacobian d(omega_r)/d(omega_q) (0 when rejected) */
 float orc_gi_jacobian(const float *xq, const float *xr, const float *xs, const float *ns);
 
