@@ -177,8 +177,16 @@ class _Attachment:
 
 
 class PathTracer(_Attachment):
-    """include/amadeus/path_tracer.cuh:17-167"""
+    """include/amadeus/path_tracer.cuh:17-167.  Options beyond the reference:
+    radiance_cache (bool), cache_depth, train_samples, cache_training."""
     kind = "path_tracer"
+
+    def radiance_cache(self):
+        """The cache network (a RadianceField view), or None before the first
+        cached frame."""
+        h = C.c_void_p()
+        L.check(self.ctx._lib.krt_attachment_cache(self.h, C.byref(h)))
+        return RadianceField._borrow(self, h) if h.value else None
 
 
 class ReSTIR(_Attachment):
@@ -222,10 +230,20 @@ class RadianceField:
         self.h = h
         ctx._adopt(self)
 
+    @classmethod
+    def _borrow(cls, owner, handle):
+        """View of a network owned by an attachment (not destroyed here)."""
+        obj = cls.__new__(cls)
+        obj.ctx = owner.ctx
+        obj.h = handle
+        obj._owner = owner
+        obj.config = None
+        return obj
+
     def _release(self):
-        if self.h:
+        if self.h and getattr(self, "_owner", None) is None:
             self.ctx._lib.krt_mlp_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

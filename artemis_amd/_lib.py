@@ -128,6 +128,7 @@ _SIGS = {
     "krt_mlp_gradient": (C.c_int, [vp, fp, fp, C.c_uint32, fp, fp]),
     "krt_mlp_apply_gradient": (C.c_int, [vp, fp]),
     "krt_mlp_get_state": (C.c_int, [vp, C.POINTER(MlpState)]),
+    "krt_attachment_cache": (C.c_int, [vp, C.POINTER(vp)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -7,8 +7,10 @@
 // unknown names are ignored, get_option of an unknown name is empty.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "krt_internal.h"
@@ -31,6 +33,14 @@ SceneView make_view(const Scene &s) {
 // ---------------------------------------------------------------------------
 // Attachment and armada state
 // ---------------------------------------------------------------------------
+// A radiance-cache network: owned (krt_mlp_create) or a borrowed view of an
+// attachment's cache (krt_attachment_cache; krt_mlp_destroy ignores it).
+struct krt_mlp {
+    Context *ctx = nullptr;
+    Mlp *net = nullptr;
+    std::unique_ptr<Mlp> owned;
+};
+
 struct krt_attachment {
     Context *ctx = nullptr;
     std::string kind;
@@ -39,6 +49,9 @@ struct krt_attachment {
     RestirOptions ropt;
     RestirState rst;
     GiState gst;
+    CacheOptions copt;
+    CacheState cst;
+    std::unique_ptr<krt_mlp> cache_handle;  // borrowed view of cst.net for krt_attachment_cache
     DevBuf history;
     uint32_t w = 0, h = 0;
 
@@ -52,6 +65,8 @@ struct krt_attachment {
     void unload() {  // restir.cuh:213-220
         rst = RestirState();
         gst = GiState();
+        cache_handle.reset();
+        cst = CacheState();
         history.release();
         loaded = false;
     }
@@ -69,6 +84,13 @@ struct krt_attachment {
             user_seed = (uint32_t)tmp;
             return true;
         }
+        if (kind == "path_tracer") {
+            if (f == "radiance_cache") return as_int(copt.enabled);
+            if (f == "cache_depth") return as_int(copt.depth);
+            if (f == "train_samples") return as_int(copt.train_samples);
+            if (f == "cache_training") return as_int(copt.training);
+            if (f == "cache_emission") return as_int(copt.learn_emission);
+        }
         if (kind != "restir" && kind != "restir_gi") return true;  // unknown option names are ignored
         if (f == "candidates") return as_int(ropt.candidates);
         if (f == "m_cap") return as_int(ropt.m_cap);
@@ -81,6 +103,11 @@ struct krt_attachment {
         krt_option o{};
         o.kind = KRT_OPT_INT;
         if (f == "seed") o.i = (int)user_seed;
+        else if (kind == "path_tracer" && f == "radiance_cache") o.i = copt.enabled;
+        else if (kind == "path_tracer" && f == "cache_depth") o.i = copt.depth;
+        else if (kind == "path_tracer" && f == "train_samples") o.i = copt.train_samples;
+        else if (kind == "path_tracer" && f == "cache_training") o.i = copt.training;
+        else if (kind == "path_tracer" && f == "cache_emission") o.i = copt.learn_emission;
         else if (kind != "path_tracer" && f == "candidates") o.i = ropt.candidates;
         else if (kind != "path_tracer" && f == "m_cap") o.i = ropt.m_cap;
         else if (kind != "path_tracer" && f == "spatial_passes") o.i = ropt.spatial_passes;
@@ -185,6 +212,12 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scen
         s->nt = d->nt;
         s->nm = d->nm;
         upload(C, s->pos, d->positions, sizeof(float) * 3 * d->nv);
+        for (int a = 0; a < 3; a++) s->bmin[a] = s->bmax[a] = d->positions[a];
+        for (uint32_t v = 0; v < d->nv; v++)
+            for (int a = 0; a < 3; a++) {
+                s->bmin[a] = std::min(s->bmin[a], d->positions[3 * v + a]);
+                s->bmax[a] = std::max(s->bmax[a], d->positions[3 * v + a]);
+            }
         upload(C, s->nrm, d->normals, sizeof(float) * 3 * d->nv);
         upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * d->nt);
         upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * d->nt);
@@ -517,7 +550,9 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         li.position = a->position.as<float4>();
         KRT_HIP(hipMemsetAsync(a->ctr.p, 0, a->ctr.bytes, C.stream));
         PassTimes times;
-        if (att->kind == "path_tracer") {
+        if (att->kind == "path_tracer" && att->copt.enabled) {
+            cache_pt_render(C, s, li, att->copt, att->cst, att->history, a->ctr.as<unsigned long long>(), times);
+        } else if (att->kind == "path_tracer") {
             pt_render(C, s, li, att->history, a->ctr.as<unsigned long long>(), times,
                       a->f64 ? a->acc64.as<double>() : nullptr);
             if (a->f64) a->f64_frames++;
@@ -650,10 +685,6 @@ krt_attachment_ret krt_load_attachment(krt_context *ctx, const char *kind) {
 }  // extern "C"
 
 // --------------------------------------------------------- radiance cache
-struct krt_mlp {
-    Context *ctx = nullptr;
-    Mlp net;
-};
 
 namespace {
 
@@ -691,40 +722,51 @@ int krt_mlp_create(krt_context *ctx, const krt_mlp_config *cfg, uint64_t seed, k
         KRT_REQUIRE(c.lr > 0.0f && c.beta1 >= 0.0f && c.beta1 < 1.0f && c.beta2 >= 0.0f && c.beta2 < 1.0f &&
                         c.eps >= 0.0f && c.loss_scale > 0.0f && c.growth_interval > 0,
                     KRT_ERR_INVALID, "invalid optimiser configuration");
-        auto *m = new krt_mlp();
+        auto m = std::make_unique<krt_mlp>();
         m->ctx = &ctx->impl;
-        m->net.cfg = MlpConfig{c.lr, c.beta1, c.beta2, c.eps, c.loss_scale, c.growth_interval};
+        m->owned = std::make_unique<Mlp>();
+        m->net = m->owned.get();
+        m->net->cfg = MlpConfig{c.lr, c.beta1, c.beta2, c.eps, c.loss_scale, c.growth_interval};
         for (int a = 0; a < 3; a++) {
-            m->net.enc.pmin[a] = c.pos_min[a];
-            m->net.enc.pscale[a] = c.pos_scale[a];
+            m->net->enc.pmin[a] = c.pos_min[a];
+            m->net->enc.pscale[a] = c.pos_scale[a];
         }
-        try {
-            mlp_init(*m->ctx, m->net, m->net.cfg, seed);
-        } catch (...) {
-            delete m;
-            throw;
-        }
-        *out = m;
+        mlp_init(*m->ctx, *m->net, m->net->cfg, seed);
+        *out = m.release();
     });
 }
 
 void krt_mlp_destroy(krt_mlp *net) {
-    if (!net) return;
+    if (!net || !net->owned) return;
     (void)hipStreamSynchronize(net->ctx->stream);
     delete net;
+}
+
+int krt_attachment_cache(krt_attachment *att, krt_mlp **out) {
+    return guard([&] {
+        KRT_REQUIRE(att && out, KRT_ERR_INVALID, "null argument");
+        *out = nullptr;
+        if (!att->cst.ready) return;
+        if (!att->cache_handle) {
+            att->cache_handle = std::make_unique<krt_mlp>();
+            att->cache_handle->ctx = att->ctx;
+            att->cache_handle->net = &att->cst.net;
+        }
+        *out = att->cache_handle.get();
+    });
 }
 
 int krt_mlp_get_params(krt_mlp *net, float *params) {
     return guard([&] {
         KRT_REQUIRE(net && params, KRT_ERR_INVALID, "null argument");
-        mlp_get_params(*net->ctx, net->net, params);
+        mlp_get_params(*net->ctx, *net->net, params);
     });
 }
 
 int krt_mlp_set_params(krt_mlp *net, const float *params) {
     return guard([&] {
         KRT_REQUIRE(net && params, KRT_ERR_INVALID, "null argument");
-        mlp_set_params(*net->ctx, net->net, params);
+        mlp_set_params(*net->ctx, *net->net, params);
     });
 }
 
@@ -732,8 +774,8 @@ int krt_mlp_set_encoding(krt_mlp *net, const float pos_min[3], const float pos_s
     return guard([&] {
         KRT_REQUIRE(net && pos_min && pos_scale, KRT_ERR_INVALID, "null argument");
         for (int a = 0; a < 3; a++) {
-            net->net.enc.pmin[a] = pos_min[a];
-            net->net.enc.pscale[a] = pos_scale[a];
+            net->net->enc.pmin[a] = pos_min[a];
+            net->net->enc.pscale[a] = pos_scale[a];
         }
     });
 }
@@ -741,7 +783,7 @@ int krt_mlp_set_encoding(krt_mlp *net, const float pos_min[3], const float pos_s
 int krt_mlp_infer_device(krt_mlp *net, const float *raw_dev, uint32_t n, float *out_dev) {
     return guard([&] {
         KRT_REQUIRE(net && (n == 0 || (raw_dev && out_dev)), KRT_ERR_INVALID, "null argument");
-        mlp_infer_dev(*net->ctx, net->net, raw_dev, n, out_dev);
+        mlp_infer_dev(*net->ctx, *net->net, raw_dev, n, out_dev);
     });
 }
 
@@ -752,7 +794,7 @@ int krt_mlp_infer(krt_mlp *net, const float *raw, uint32_t n, float *out) {
         Context &ctx = *net->ctx;
         HostArray in(ctx, raw, (size_t)kMlpRaw * n);
         DevBuf o(sizeof(float) * kMlpOut * (size_t)n);
-        mlp_infer_dev(ctx, net->net, in.buf.as<float>(), n, o.as<float>());
+        mlp_infer_dev(ctx, *net->net, in.buf.as<float>(), n, o.as<float>());
         KRT_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * kMlpOut * (size_t)n, hipMemcpyDeviceToHost, ctx.stream));
         KRT_HIP(hipStreamSynchronize(ctx.stream));
     });
@@ -767,8 +809,8 @@ int krt_mlp_train_step_device(krt_mlp *net, const float *raw_dev, const float *t
     return guard([&] {
         KRT_REQUIRE(net && raw_dev && target_dev, KRT_ERR_INVALID, "null argument");
         KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
-        mlp_gradient_dev(*net->ctx, net->net, raw_dev, target_dev, n);
-        mlp_apply_dev(*net->ctx, net->net);
+        mlp_gradient_dev(*net->ctx, *net->net, raw_dev, target_dev, n);
+        mlp_apply_dev(*net->ctx, *net->net);
     });
 }
 
@@ -778,10 +820,10 @@ int krt_mlp_train_step(krt_mlp *net, const float *raw, const float *target, uint
         KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
         Context &ctx = *net->ctx;
         HostArray in(ctx, raw, (size_t)kMlpRaw * n), tg(ctx, target, (size_t)kMlpOut * n);
-        mlp_gradient_dev(ctx, net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
-        mlp_apply_dev(ctx, net->net);
+        mlp_gradient_dev(ctx, *net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
+        mlp_apply_dev(ctx, *net->net);
         MlpDevState st;
-        read_state(ctx, net->net, st);
+        read_state(ctx, *net->net, st);
         if (loss) *loss = st.loss;
     });
 }
@@ -792,12 +834,12 @@ int krt_mlp_gradient(krt_mlp *net, const float *raw, const float *target, uint32
         KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
         Context &ctx = *net->ctx;
         HostArray in(ctx, raw, (size_t)kMlpRaw * n), tg(ctx, target, (size_t)kMlpOut * n);
-        mlp_gradient_dev(ctx, net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
-        KRT_HIP(hipMemcpyAsync(grad, net->net.grad.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost,
+        mlp_gradient_dev(ctx, *net->net, in.buf.as<float>(), tg.buf.as<float>(), n);
+        KRT_HIP(hipMemcpyAsync(grad, net->net->grad.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost,
                                ctx.stream));
-        mlp_clear_flag(ctx, net->net);
+        mlp_clear_flag(ctx, *net->net);
         MlpDevState st;
-        read_state(ctx, net->net, st);
+        read_state(ctx, *net->net, st);
         if (loss) *loss = st.loss;
     });
 }
@@ -805,7 +847,7 @@ int krt_mlp_gradient(krt_mlp *net, const float *raw, const float *target, uint32
 int krt_mlp_apply_gradient(krt_mlp *net, const float *grad) {
     return guard([&] {
         KRT_REQUIRE(net && grad, KRT_ERR_INVALID, "null argument");
-        mlp_apply_host_gradient(*net->ctx, net->net, grad);
+        mlp_apply_host_gradient(*net->ctx, *net->net, grad);
         KRT_HIP(hipStreamSynchronize(net->ctx->stream));
     });
 }
@@ -814,7 +856,7 @@ int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state) {
     return guard([&] {
         KRT_REQUIRE(net && state, KRT_ERR_INVALID, "null argument");
         MlpDevState st;
-        read_state(*net->ctx, net->net, st);
+        read_state(*net->ctx, *net->net, st);
         static_assert(sizeof(st) == sizeof(krt_mlp_state), "state layout");
         std::memcpy(state, &st, sizeof(st));
     });

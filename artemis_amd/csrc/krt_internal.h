@@ -97,6 +97,7 @@ struct Scene {
     DevBuf mats;       // float4 x 4 per material
     DevBuf lights;     // float4 x 4 per light
     Accel accel;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // vertex bounding box
     uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
     size_t bytes() const {
         return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + accel.ser.bytes +
@@ -240,9 +241,34 @@ void mlp_init(Context &ctx, Mlp &net, const MlpConfig &cfg, uint64_t seed);
 void mlp_set_params(Context &ctx, Mlp &net, const float *host);
 void mlp_get_params(Context &ctx, Mlp &net, float *host);
 void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *out);
-void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n);
+// n_dev (optional): the batch size on the device, at most n.
+void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n,
+                      const uint32_t *n_dev = nullptr);
 void mlp_apply_dev(Context &ctx, Mlp &net);
 void mlp_apply_host_gradient(Context &ctx, Mlp &net, const float *grad);
 void mlp_clear_flag(Context &ctx, Mlp &net);
+
+// Radiance cache attached to the path tracer (krt_cache.hip).
+struct CacheOptions {
+    int enabled = 0;
+    int depth = 1;              // level of the cache vertex (0 = primary hit)
+    int train_samples = 65536;  // training paths per frame
+    int training = 1;           // 0 freezes the network
+    int learn_emission = 0;     // 1: emission at the cache vertex is learned, not added
+};
+
+struct CacheState {
+    MlpConfig cfg;
+    Mlp net;
+    bool ready = false;
+    uint64_t frame = 0;
+    uint32_t npix = 0, slots = 0;
+    DevBuf query, pred, prefix, theta;             // per pixel
+    DevBuf slot_raw, slot_tgt, slot_ok;            // per training slot
+    DevBuf batch_raw, batch_tgt, batch_n;          // compacted training batch
+};
+
+void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
+                     DevBuf &history, unsigned long long *ctr, PassTimes &times);
 
 }  // namespace krt

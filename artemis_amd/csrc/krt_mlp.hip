@@ -229,7 +229,8 @@ struct TrainArgs {
     uint32_t n;
     MlpEncoding enc;
     const MlpDevState *state;
-    float *partial;  // per workgroup: kMlpParams gradient sums + 1 loss sum
+    float *partial;         // per workgroup: kMlpParams gradient sums + 1 loss sum
+    const uint32_t *n_dev;  // batch size on the device (overrides n when set)
 };
 
 __device__ __forceinline__ __bf16 &lds(__bf16 *base, int row, int col) { return base[row * kStride + col]; }
@@ -239,15 +240,17 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 sD[2 * 64 * kStride];
     __shared__ float sLoss[kTrainBlock / 64];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    if (blockIdx.x * kTrainSamples >= n) return;  // beyond the batch: no partial
     const uint32_t s0 = blockIdx.x * kTrainSamples + 32 * w;
-    const float gscale = a.state->scale / (float)(3u * a.n);
+    const float gscale = a.state->scale / (float)(3u * n);
     int col[2];
     bool valid[2];
     bf16x8 B[2][2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         const uint32_t q = s0 + 16 * t + r;
-        valid[t] = q < a.n;
+        valid[t] = q < n;
         col[t] = 32 * w + 16 * t + r;
         encode(a.raw + (size_t)kMlpRaw * (valid[t] ? q : 0), valid[t], a.enc, g, B[0][t], B[1][t]);
 #pragma unroll
@@ -388,11 +391,21 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
 }
 
 // Sum of the workgroup partials in workgroup order, unscaled; flags
-// non-finite gradients.
-__global__ void reduce_kernel(const float *__restrict__ partial, int nparts, uint32_t n, MlpDevState *st,
-                              float *__restrict__ grad) {
+// non-finite gradients (bad = 1) and an empty batch (bad = 2: no step, no
+// scale change).
+__global__ void reduce_kernel(const float *__restrict__ partial, uint32_t n_host, const uint32_t *n_dev,
+                              MlpDevState *st, float *__restrict__ grad) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > kMlpParams) return;
+    const uint32_t n = n_dev ? *n_dev : n_host;
+    if (n == 0) {
+        if (p == kMlpParams) {
+            st->loss = 0.0f;
+            st->bad = 2;
+        }
+        return;
+    }
+    const int nparts = (int)((n + kTrainSamples - 1) / kTrainSamples);
     float s = 0.0f;
     for (int k = 0; k < nparts; k++) s += partial[(size_t)k * (kMlpParams + 1) + p];
     if (p == kMlpParams) {
@@ -427,7 +440,9 @@ __global__ void adam_kernel(float *__restrict__ params, float *__restrict__ m, f
 __global__ void finalize_kernel(MlpDevState *st, MlpConfig cfg) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     MlpDevState s = *st;
-    if (s.bad) {
+    if (s.bad == 2) {
+        // empty batch: nothing to apply
+    } else if (s.bad) {
         s.scale = s.scale * 0.5f;
         s.good = 0;
         s.skipped += 1;
@@ -533,15 +548,16 @@ void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *
     KRT_HIP(hipGetLastError());
 }
 
-void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n) {
+void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n,
+                      const uint32_t *n_dev) {
     KRT_REQUIRE(n > 0, KRT_ERR_INVALID, "empty training batch");
     const uint32_t nb = (n + kTrainSamples - 1) / kTrainSamples;
     const size_t need = sizeof(float) * (size_t)nb * (kMlpParams + 1);
     if (net.partial.bytes < need) net.partial.alloc(need);
     TrainArgs a{net.frag_f.as<bf16x8>(), net.frag_b.as<bf16x8>(), net.params.as<float>(), raw, target, n, net.enc,
-                net.state.as<MlpDevState>(), net.partial.as<float>()};
+                net.state.as<MlpDevState>(), net.partial.as<float>(), n_dev};
     train_kernel<<<nb, kTrainBlock, 0, ctx.stream>>>(a);
-    reduce_kernel<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx.stream>>>(net.partial.as<float>(), (int)nb, n,
+    reduce_kernel<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx.stream>>>(net.partial.as<float>(), n, n_dev,
                                                                         net.state.as<MlpDevState>(),
                                                                         net.grad.as<float>());
     KRT_HIP(hipGetLastError());

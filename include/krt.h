@@ -259,6 +259,11 @@ int krt_mlp_gradient(krt_mlp *net, const float *raw, const float *target, uint32
                      float *loss);
 int krt_mlp_apply_gradient(krt_mlp *net, const float *grad);
 int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state);
+/* The network of a path_tracer attachment's radiance cache (options
+ * "radiance_cache", "cache_depth", "train_samples", "cache_training"):
+ * a borrowed handle, NULL before the first cached frame, valid until the
+ * attachment is unloaded or destroyed; krt_mlp_destroy ignores it. */
+int krt_attachment_cache(krt_attachment *att, krt_mlp **out);
 
 #ifdef __cplusplus
 }
