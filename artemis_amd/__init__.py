@@ -175,22 +175,25 @@ class _Attachment:
         except Exception:
             pass
 
-
-class PathTracer(_Attachment):
-    """include/amadeus/path_tracer.cuh:17-167.  Options beyond the reference:
-    radiance_cache (bool), cache_depth, train_samples, cache_training."""
-    kind = "path_tracer"
-
     def radiance_cache(self):
-        """The cache network (a RadianceField view), or None before the first
-        cached frame."""
+        """The radiance-cache network of this attachment (a RadianceField
+        view), or None before the first cached frame."""
         h = C.c_void_p()
         L.check(self.ctx._lib.krt_attachment_cache(self.h, C.byref(h)))
         return RadianceField._borrow(self, h) if h.value else None
 
 
+class PathTracer(_Attachment):
+    """include/amadeus/path_tracer.cuh:17-167.  Options beyond the reference:
+    radiance_cache (bool), cache_depth, train_samples, cache_training,
+    cache_emission."""
+    kind = "path_tracer"
+
+
 class ReSTIR(_Attachment):
-    """include/amadeus/restir.cuh:35-305"""
+    """include/amadeus/restir.cuh:35-305.  With the option radiance_cache the
+    indirect path stops at the first bounce and asks the cache (same options as
+    PathTracer)."""
     kind = "restir"
 
 

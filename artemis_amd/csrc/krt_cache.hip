@@ -18,6 +18,7 @@
 
 #include <algorithm>
 
+#include "krt_cache.h"
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
@@ -34,31 +35,9 @@ struct CacheArgs {
     float4 *hist;
     int *spill;
     unsigned long long *ctr;
-    float *query;       // npix x 13
-    float4 *prefix;     // npix: {P, 1 when the cache term applies}
-    float4 *theta;      // npix: {Theta, 0}
-    float *slot_raw;    // M x 13
-    float *slot_tgt;    // M x 3
-    uint32_t *slot_ok;  // M
-    uint32_t M, stride, offset;
-    int depth;
-    int learn_emission;  // 1: the cache also learns the vertex's emission
+    float4 *prefix;  // npix: P of value_0 = P + Theta * cache
+    CacheView cv;
 };
-
-KRT_DEV bool training_slot(const CacheArgs &A, uint32_t pix, uint32_t &slot) {
-    if (pix < A.offset) return false;
-    const uint32_t d = pix - A.offset;
-    if (d % A.stride) return false;
-    slot = d / A.stride;
-    return slot < A.M;
-}
-
-KRT_DEV void put_query(float *q, f3 x, f3 w, f3 n, const Material &mat) {
-    const float v[kMlpRaw] = {x.x, x.y, x.z, w.x, w.y, w.z, n.x, n.y, n.z,
-                              mat.diffuse.x, mat.diffuse.y, mat.diffuse.z, mat.roughness};
-#pragma unroll
-    for (int a = 0; a < kMlpRaw; a++) q[a] = v[a];
-}
 
 __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
@@ -75,15 +54,16 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
     for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
         const uint32_t px = pix % li.width, py = pix / li.width;
         uint32_t slot = 0;
-        const bool train = training_slot(A, pix, slot);
+        const CacheView &cv = A.cv;
+        const bool train = cv.training(pix, slot);
         f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
         f3 o, d;
-        make_ray(li, px, py, seed, true, o, d);
+        make_ray(li, px, py, seed, li.jitter != 0, o, d);
         f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
         int nlev = 0;
         bool queried = false;
         f3 le_c = mk(0, 0, 0);  // emission at the cache vertex, added exactly
-        float *qdst = train ? A.slot_raw + (size_t)kMlpRaw * slot : A.query + (size_t)kMlpRaw * pix;
+        float *qdst = train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix;
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
@@ -92,10 +72,10 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
             Surface sf;
             if (!load_surface(sv, h, d, sf)) break;
             Material mat = load_material(sv.mats, sf.mat_id);
-            if (level == A.depth) {
+            if (level == cv.depth) {
                 put_query(qdst, sf.x, -d, sf.n, mat);
                 queried = true;
-                le_c = A.learn_emission ? mk(0, 0, 0) : mat.emission;
+                le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
                 if (!train) break;
             }
             f3 n = sf.n;
@@ -131,8 +111,8 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
                 Th = T * Th;
             }
             P = P + Th * le_c;
-            pre = make_float4(P.x, P.y, P.z, 1.0f);
-            th = make_float4(Th.x, Th.y, Th.z, 0.0f);
+            pre = make_float4(P.x, P.y, P.z, 0.0f);
+            th = make_float4(Th.x, Th.y, Th.z, 1.0f);
         } else {
             f3 value = mk(0, 0, 0), target = mk(0, 0, 0);
             for (int k = nlev - 1; k >= 0; k--) {
@@ -142,21 +122,20 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
                 f3 v = direct;
                 if (has_pdf) v = v + T * value;
                 value = v;
-                if (k == A.depth) target = value - le_c;  // scattered radiance only
+                if (k == cv.depth) target = value - le_c;  // scattered radiance only
             }
             pre = make_float4(value.x, value.y, value.z, 0.0f);
             if (train) {
-                const bool ok = queried && nlev > A.depth && !isnan3(target) && isfinite(target.x) &&
-                                isfinite(target.y) && isfinite(target.z);
-                float *t = A.slot_tgt + 3 * (size_t)slot;
+                const bool ok = queried && nlev > cv.depth && finite3(target);
+                float *t = cv.slot_tgt + 3 * (size_t)slot;
                 t[0] = target.x;
                 t[1] = target.y;
                 t[2] = target.z;
-                A.slot_ok[slot] = ok ? 1u : 0u;
+                cv.slot_ok[slot] = ok ? 1u : 0u;
             }
         }
         A.prefix[pix] = pre;
-        A.theta[pix] = th;
+        cv.theta[pix] = th;
         const bool acc = li.accumulate != 0;
         li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
         li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
@@ -174,8 +153,8 @@ __global__ void composite_kernel(LaunchInfo li, const float4 *__restrict__ prefi
     if (pix >= li.width * li.height) return;
     const float4 p = prefix[pix];
     f3 value = mk(p.x, p.y, p.z);
-    if (p.w != 0.0f) {
-        const float4 t = theta[pix];
+    const float4 t = theta[pix];
+    if (t.w != 0.0f) {
         const f3 y = mk(fmaxf(pred[3 * (size_t)pix], 0.0f), fmaxf(pred[3 * (size_t)pix + 1], 0.0f),
                         fmaxf(pred[3 * (size_t)pix + 2], 0.0f));
         value = value + mk(t.x, t.y, t.z) * y;
@@ -183,6 +162,20 @@ __global__ void composite_kernel(LaunchInfo li, const float4 *__restrict__ prefi
     float4 col = make_float4(value.x, value.y, value.z, 1.0f);
     if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
     li.color[pix] = accumulate4(li.color[pix], col, li.accumulate != 0, (float)li.samples);
+}
+
+__global__ void fold_kernel(uint32_t npix, const float4 *__restrict__ theta, const float *__restrict__ pred,
+                            float4 *__restrict__ intermediate) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    const float4 t = theta[pix];
+    if (t.w == 0.0f) return;
+    float4 im = intermediate[pix];
+    im.x = im.x + t.x * fmaxf(pred[3 * (size_t)pix], 0.0f);
+    im.y = im.y + t.y * fmaxf(pred[3 * (size_t)pix + 1], 0.0f);
+    im.z = im.z + t.z * fmaxf(pred[3 * (size_t)pix + 2], 0.0f);
+    if (isnan(im.x) || isnan(im.y) || isnan(im.z)) im = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+    intermediate[pix] = im;
 }
 
 // Order-preserving compaction of the valid training slots (one workgroup).
@@ -226,9 +219,12 @@ uint32_t frame_offset(uint64_t frame, uint32_t seed, uint32_t stride) {
 
 }  // namespace
 
-void cache_prepare(Context &ctx, Scene &s, CacheState &cs, uint32_t npix, uint32_t seed) {
+CacheView cache_begin(Context &ctx, Scene &s, const CacheOptions &opt, CacheState &cs, uint32_t npix,
+                      uint32_t user_seed) {
+    KRT_REQUIRE(opt.depth >= 0, KRT_ERR_INVALID, "cache_depth must be >= 0");
+    KRT_REQUIRE(opt.train_samples >= 0, KRT_ERR_INVALID, "train_samples must be >= 0");
     if (!cs.ready) {
-        mlp_init(ctx, cs.net, cs.cfg, 0xC0FFEEull ^ seed);
+        mlp_init(ctx, cs.net, cs.cfg, 0xC0FFEEull ^ user_seed);
         cs.ready = true;
     }
     // positions mapped to [0, 1] by the scene's bounding cube
@@ -247,18 +243,7 @@ void cache_prepare(Context &ctx, Scene &s, CacheState &cs, uint32_t npix, uint32
         KRT_HIP(hipMemsetAsync(cs.query.p, 0, cs.query.bytes, ctx.stream));
         cs.npix = npix;
     }
-}
-
-void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
-                     DevBuf &history, unsigned long long *ctr, PassTimes &times) {
-    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
-    KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
-    KRT_REQUIRE(opt.depth >= 0, KRT_ERR_INVALID, "cache_depth must be >= 0");
-    KRT_REQUIRE(opt.train_samples >= 0, KRT_ERR_INVALID, "train_samples must be >= 0");
-    const uint32_t npix = li.width * li.height;
-    cache_prepare(ctx, s, cs, npix, li.user_seed);
     const uint32_t M = opt.training ? (uint32_t)std::min<uint64_t>((uint64_t)opt.train_samples, npix) : 0u;
-    const uint32_t stride = M ? npix / M : 1u;
     if (cs.slots < M || !cs.slot_ok.p) {
         const uint32_t cap = M ? M : 1u;
         cs.slot_raw.alloc(sizeof(float) * kMlpRaw * (size_t)cap);
@@ -269,49 +254,67 @@ void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOp
         cs.batch_n.alloc(sizeof(uint32_t));
         cs.slots = cap;
     }
-    const uint32_t offset = M ? frame_offset(cs.frame, li.user_seed, stride) : npix;
+    CacheView cv;
+    cv.enabled = 1;
+    cv.depth = opt.depth;
+    cv.learn_emission = opt.learn_emission;
+    cv.query = cs.query.as<float>();
+    cv.theta = cs.theta.as<float4>();
+    cv.slot_raw = cs.slot_raw.as<float>();
+    cv.slot_tgt = cs.slot_tgt.as<float>();
+    cv.slot_ok = cs.slot_ok.as<uint32_t>();
+    cv.M = M;
+    cv.stride = M ? npix / M : 1u;
+    cv.offset = M ? frame_offset(cs.frame, user_seed, cv.stride) : npix;
     cs.frame++;
+    return cv;
+}
+
+void cache_infer(Context &ctx, CacheState &cs, uint32_t npix) {
+    mlp_infer_dev(ctx, cs.net, cs.query.as<float>(), npix, cs.pred.as<float>());
+}
+
+void cache_fold_intermediate(Context &ctx, CacheState &cs, uint32_t npix, float4 *intermediate) {
+    fold_kernel<<<(npix + 255) / 256, 256, 0, ctx.stream>>>(npix, cs.theta.as<float4>(), cs.pred.as<float>(),
+                                                           intermediate);
+    KRT_HIP(hipGetLastError());
+}
+
+void cache_train(Context &ctx, CacheState &cs, const CacheView &cv) {
+    if (cv.M == 0) return;
+    compact_kernel<<<1, kCompactThreads, 0, ctx.stream>>>(cs.slot_raw.as<float>(), cs.slot_tgt.as<float>(),
+                                                         cs.slot_ok.as<uint32_t>(), cv.M, cs.batch_raw.as<float>(),
+                                                         cs.batch_tgt.as<float>(), cs.batch_n.as<uint32_t>());
+    KRT_HIP(hipGetLastError());
+    mlp_gradient_dev(ctx, cs.net, cs.batch_raw.as<float>(), cs.batch_tgt.as<float>(), cv.M, cs.batch_n.as<uint32_t>());
+    mlp_apply_dev(ctx, cs.net);
+}
+
+void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
+                     DevBuf &history, unsigned long long *ctr, PassTimes &times) {
+    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+    KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
+    const uint32_t npix = li.width * li.height;
+    const CacheView cv = cache_begin(ctx, s, opt, cs, npix, li.user_seed);
     uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
     uint32_t nthreads = nb * kTraceBlock;
     size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
     if (history.bytes < need) history.alloc(need);
-    CacheArgs A{li,
-                make_view(s),
-                history.as<float4>(),
-                spill_buffer(ctx, s, nthreads),
-                ctr,
-                cs.query.as<float>(),
-                cs.prefix.as<float4>(),
-                cs.theta.as<float4>(),
-                cs.slot_raw.as<float>(),
-                cs.slot_tgt.as<float>(),
-                cs.slot_ok.as<uint32_t>(),
-                M,
-                stride,
-                offset,
-                opt.depth,
-                opt.learn_emission};
+    CacheArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, cs.prefix.as<float4>(),
+                cv};
     hipEvent_t ev[5];
     for (auto &e : ev) KRT_HIP(hipEventCreate(&e));
     KRT_HIP(hipEventRecord(ev[0], ctx.stream));
     cache_path_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(ev[1], ctx.stream));
-    mlp_infer_dev(ctx, cs.net, cs.query.as<float>(), npix, cs.pred.as<float>());
+    cache_infer(ctx, cs, npix);
     KRT_HIP(hipEventRecord(ev[2], ctx.stream));
     composite_kernel<<<(npix + 255) / 256, 256, 0, ctx.stream>>>(li, cs.prefix.as<float4>(), cs.theta.as<float4>(),
                                                                 cs.pred.as<float>());
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(ev[3], ctx.stream));
-    if (M > 0) {
-        compact_kernel<<<1, kCompactThreads, 0, ctx.stream>>>(cs.slot_raw.as<float>(), cs.slot_tgt.as<float>(),
-                                                             cs.slot_ok.as<uint32_t>(), M, cs.batch_raw.as<float>(),
-                                                             cs.batch_tgt.as<float>(), cs.batch_n.as<uint32_t>());
-        KRT_HIP(hipGetLastError());
-        mlp_gradient_dev(ctx, cs.net, cs.batch_raw.as<float>(), cs.batch_tgt.as<float>(), M,
-                         cs.batch_n.as<uint32_t>());
-        mlp_apply_dev(ctx, cs.net);
-    }
+    cache_train(ctx, cs, cv);
     KRT_HIP(hipEventRecord(ev[4], ctx.stream));
     KRT_HIP(hipEventSynchronize(ev[4]));
     for (int k = 0; k < 4; k++) KRT_HIP(hipEventElapsedTime(&times.ms[k], ev[k], ev[k + 1]));

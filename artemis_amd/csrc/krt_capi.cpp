@@ -46,6 +46,7 @@ struct krt_attachment {
     std::string kind;
     bool loaded = false;
     uint32_t user_seed = 0;
+    int jitter = 1;  // path tracer pixel jitter (option "pixel_jitter")
     RestirOptions ropt;
     RestirState rst;
     GiState gst;
@@ -84,7 +85,8 @@ struct krt_attachment {
             user_seed = (uint32_t)tmp;
             return true;
         }
-        if (kind == "path_tracer") {
+        if (kind == "path_tracer" && f == "pixel_jitter") return as_int(jitter);
+        if (kind == "path_tracer" || kind == "restir") {
             if (f == "radiance_cache") return as_int(copt.enabled);
             if (f == "cache_depth") return as_int(copt.depth);
             if (f == "train_samples") return as_int(copt.train_samples);
@@ -103,11 +105,12 @@ struct krt_attachment {
         krt_option o{};
         o.kind = KRT_OPT_INT;
         if (f == "seed") o.i = (int)user_seed;
-        else if (kind == "path_tracer" && f == "radiance_cache") o.i = copt.enabled;
-        else if (kind == "path_tracer" && f == "cache_depth") o.i = copt.depth;
-        else if (kind == "path_tracer" && f == "train_samples") o.i = copt.train_samples;
-        else if (kind == "path_tracer" && f == "cache_training") o.i = copt.training;
-        else if (kind == "path_tracer" && f == "cache_emission") o.i = copt.learn_emission;
+        else if (kind == "path_tracer" && f == "pixel_jitter") o.i = jitter;
+        else if ((kind == "path_tracer" || kind == "restir") && f == "radiance_cache") o.i = copt.enabled;
+        else if ((kind == "path_tracer" || kind == "restir") && f == "cache_depth") o.i = copt.depth;
+        else if ((kind == "path_tracer" || kind == "restir") && f == "train_samples") o.i = copt.train_samples;
+        else if ((kind == "path_tracer" || kind == "restir") && f == "cache_training") o.i = copt.training;
+        else if ((kind == "path_tracer" || kind == "restir") && f == "cache_emission") o.i = copt.learn_emission;
         else if (kind != "path_tracer" && f == "candidates") o.i = ropt.candidates;
         else if (kind != "path_tracer" && f == "m_cap") o.i = ropt.m_cap;
         else if (kind != "path_tracer" && f == "spatial_passes") o.i = ropt.spatial_passes;
@@ -544,6 +547,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         li.samples = a->samples;
         li.accumulate = 1;  // ArmadaRTX launches always accumulate (armada.cu:58); samples==0 restarts
         li.user_seed = att->user_seed;
+        li.jitter = att->jitter;
         li.color = a->color.as<float4>();
         li.normal = a->normal.as<float4>();
         li.albedo = a->albedo.as<float4>();
@@ -557,7 +561,9 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
                       a->f64 ? a->acc64.as<double>() : nullptr);
             if (a->f64) a->f64_frames++;
         }
-        else if (att->kind == "restir") restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times);
+        else if (att->kind == "restir")
+            restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times, &att->copt,
+                          &att->cst);
         else gi_render(C, s, li, att->ropt, att->gst, a->ctr.as<unsigned long long>(), times);
         unsigned long long c[16] = {0};
         KRT_HIP(hipMemcpyAsync(c, a->ctr.p, sizeof(c), hipMemcpyDeviceToHost, C.stream));

@@ -150,6 +150,7 @@ struct LaunchInfo {
     int samples;
     int accumulate;
     uint32_t user_seed;
+    int jitter;  // path tracer: sub-pixel jitter (1, the reference's) or the fixed -0.5 offset (0)
     float4 *color, *normal, *albedo, *position;
 };
 
@@ -197,8 +198,7 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions
                unsigned long long *ctr, PassTimes &times);
 float copy_probe(Context &ctx, size_t bytes, int reps);
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
-void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
-                   unsigned long long *ctr, PassTimes &times);
+
 
 
 // Radiance-cache MLP (krt_mlp.hip).  Parameters in canonical order: per layer
@@ -270,5 +270,9 @@ struct CacheState {
 
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
                      DevBuf &history, unsigned long long *ctr, PassTimes &times);
+// ReSTIR DI; with a cache (copt->enabled) its indirect path stops at the cache vertex.
+void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
+                   unsigned long long *ctr, PassTimes &times, const CacheOptions *copt = nullptr,
+                   CacheState *cs = nullptr);
 
 }  // namespace krt

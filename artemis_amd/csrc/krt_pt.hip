@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         const uint32_t px = pix % li.width, py = pix / li.width;
         f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
         f3 o, d;
-        make_ray(li, px, py, seed, true, o, d);
+        make_ray(li, px, py, seed, li.jitter != 0, o, d);
         f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
         int nlev = 0;
         for (int level = 0; level <= li.max_depth; level++) {

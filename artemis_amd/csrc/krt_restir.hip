@@ -15,6 +15,7 @@
 // planes per buffer: {Le, type}, {normal, w}, {point, W}, {seed, M}.
 #include <hip/hip_runtime.h>
 
+#include "krt_cache.h"
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
@@ -129,6 +130,7 @@ struct RestirArgs {
     int *spill;
     unsigned long long *ctr;
     int candidates, m_cap, neighbours, radius;
+    CacheView cv;  // radiance cache for the indirect path (cv.enabled = 0: off)
 };
 
 // G-buffer surface of a pixel (amadeus_restir.cu:230-240, :323-336); the
@@ -168,6 +170,12 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
         make_ray(li, px, py, seed, false, o, d);
         f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
         int nlev = 0;
+        // radiance cache: the indirect path stops at level cv.depth (>= 1)
+        const CacheView &cv = A.cv;
+        uint32_t slot = 0;
+        const bool train = cv.enabled && cv.training(pix, slot);
+        bool queried = false;
+        f3 le_c = mk(0, 0, 0);
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
@@ -176,6 +184,13 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
             Surface sf;
             if (!load_surface(sv, h, d, sf)) break;
             Material mat = load_material(sv.mats, sf.mat_id);
+            if (cv.enabled && level == cv.depth) {
+                put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
+                          sf.n, mat);
+                queried = true;
+                le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
+                if (!train) break;
+            }
             f3 n = sf.n;
             f3 x = offset_point(sf.x, n, mat.type);
             SurfaceHit sh{mat, sf.entering, n, -d, x};
@@ -234,11 +249,36 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
             d = wi;
         }
         f3 value = mk(0, 0, 0);
-        for (int k = nlev - 1; k >= 0; k--) {
-            f3 direct, T;
-            bool add;
-            hist.get(k, direct, add, T);
-            value = direct + T * value;
+        if (queried && !train) {
+            // value = P + Theta * (Le_c + cache); the cache term is folded in
+            // after inference
+            f3 Th = mk(1.0f, 1.0f, 1.0f);
+            for (int k = nlev - 1; k >= 0; k--) {
+                f3 direct, T;
+                bool add;
+                hist.get(k, direct, add, T);
+                value = direct + T * value;
+                Th = T * Th;
+            }
+            value = value + Th * le_c;
+            cv.theta[pix] = make_float4(Th.x, Th.y, Th.z, 1.0f);
+        } else {
+            f3 target = mk(0, 0, 0);
+            for (int k = nlev - 1; k >= 0; k--) {
+                f3 direct, T;
+                bool add;
+                hist.get(k, direct, add, T);
+                value = direct + T * value;
+                if (cv.enabled && k == cv.depth) target = value - le_c;
+            }
+            if (cv.enabled) cv.theta[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (train) {
+                float *t = cv.slot_tgt + 3 * (size_t)slot;
+                t[0] = target.x;
+                t[1] = target.y;
+                t[2] = target.z;
+                cv.slot_ok[slot] = (queried && nlev > cv.depth && finite3(target)) ? 1u : 0u;
+            }
         }
         float4 col = make_float4(value.x, value.y, value.z, 1.0f);
         if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
@@ -454,7 +494,7 @@ void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_
 }
 
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOptions &opt, RestirState &st,
-                   unsigned long long *ctr, PassTimes &times) {
+                   unsigned long long *ctr, PassTimes &times, const CacheOptions *copt, CacheState *cs) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR buffers not loaded");
     KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kMaxNeighbours, KRT_ERR_INVALID,
@@ -481,6 +521,11 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.m_cap = opt.m_cap;
     A.neighbours = opt.neighbours;
     A.radius = opt.radius;
+    const bool cached = copt && cs && copt->enabled;
+    if (cached) {
+        KRT_REQUIRE(copt->depth >= 1, KRT_ERR_INVALID, "restir radiance cache needs cache_depth >= 1");
+        A.cv = cache_begin(ctx, s, *copt, *cs, npix, li.user_seed);
+    }
     auto planes = [&](int b) { return Planes{st.res[b].as<float4>(), npix}; };
     hipEvent_t ev[16];
     int nev = 0;
@@ -495,6 +540,11 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     initial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     mark();
+    if (cached) {
+        cache_infer(ctx, *cs, npix);
+        cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
+    }
+    mark();
     temporal_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     mark();
@@ -506,10 +556,19 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
     }
     mark();
+    if (cached) cache_train(ctx, *cs, A.cv);
+    mark();
     st.cur = 1 - st.cur;  // final swap, restir.cuh:300
     for (int k = 0; k < 16; k++) st.prev_pv[k] = li.pv[k];
     KRT_HIP(hipEventSynchronize(ev[nev - 1]));
-    for (int k = 0; k + 1 < nev && k < 8; k++) KRT_HIP(hipEventElapsedTime(&times.ms[k], ev[k], ev[k + 1]));
+    // segments: initial, cache inference + fold, temporal, spatial, training
+    float seg[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k + 1 < nev && k < 5; k++) KRT_HIP(hipEventElapsedTime(&seg[k], ev[k], ev[k + 1]));
+    times.ms[0] = seg[0];
+    times.ms[1] = seg[2];
+    times.ms[2] = seg[3];
+    times.ms[3] = seg[1];
+    times.ms[4] = seg[4];
     for (int k = 0; k < nev; k++) (void)hipEventDestroy(ev[k]);
 }
 
