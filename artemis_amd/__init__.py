@@ -64,6 +64,23 @@ class Context:
         L.check(self._lib.krt_copy_probe(self.h, nbytes, reps, C.byref(ms)))
         return ms.value
 
+    def timer_begin(self) -> None:
+        """Start a device-time measurement on the context's stream."""
+        L.check(self._lib.krt_timer_begin(self.h))
+
+    def timer_end(self) -> float:
+        """Milliseconds of device time since timer_begin (HIP events)."""
+        ms = C.c_float()
+        L.check(self._lib.krt_timer_end(self.h, C.byref(ms)))
+        return ms.value
+
+    def sync(self) -> None:
+        L.check(self._lib.krt_sync(self.h))
+
+    def device_array(self, data=None, shape=None, dtype=np.float32) -> "DeviceArray":
+        """A device buffer, filled from `data` or of `shape` x `dtype`."""
+        return DeviceArray(self, data, shape, dtype)
+
     def camera_rays(self, camera: CameraFrame, width: int, height: int, jitter: bool = False, sample: int = 0,
                     user_seed: int = 0) -> np.ndarray:
         """make_ray on the device for every pixel (8 floats per ray)."""
@@ -75,6 +92,48 @@ class Context:
 
     def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True, layout: str = "sah") -> "Scene":
         return Scene(self, scene, leaf_max=leaf_max, build=build, layout=layout)
+
+
+class DeviceArray:
+    """Device memory owned through krt_buffer_*; .ptr for *_device calls."""
+
+    def __init__(self, ctx: Context, data=None, shape=None, dtype=np.float32):
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            shape, dtype = data.shape, data.dtype
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = C.c_void_p()
+        L.check(ctx._lib.krt_buffer_create(ctx.h, max(self.nbytes, 1), C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+        if data is not None:
+            self.upload(data)
+
+    @property
+    def ptr(self) -> int:
+        return self.h.value
+
+    def upload(self, data) -> None:
+        data = np.ascontiguousarray(data, self.dtype)
+        assert data.nbytes == self.nbytes
+        L.check(self.ctx._lib.krt_buffer_upload(self.ctx.h, self.h, data.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        L.check(self.ctx._lib.krt_buffer_download(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.h, self.nbytes))
+        return out
+
+    def _release(self):
+        if self.h is not None and self.ctx.h:
+            self.ctx._lib.krt_buffer_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
 
 
 class Scene:

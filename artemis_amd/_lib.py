@@ -96,6 +96,13 @@ _SIGS = {
     "krt_camera_rays": (C.c_int, [vp, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
                                   fp]),
     "krt_copy_probe": (C.c_int, [vp, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
+    "krt_timer_begin": (C.c_int, [vp]),
+    "krt_timer_end": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "krt_sync": (C.c_int, [vp]),
+    "krt_buffer_create": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+    "krt_buffer_destroy": (C.c_int, [vp, vp]),
+    "krt_buffer_upload": (C.c_int, [vp, vp, vp, C.c_size_t]),
+    "krt_buffer_download": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "krt_armada_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
     "krt_armada_destroy": (None, [vp]),
     "krt_attachment_create": (C.c_int, [vp, C.c_char_p, C.POINTER(vp)]),

@@ -189,6 +189,8 @@ void krt_destroy(krt_context *ctx) {
     ctx->impl.scenes.clear();
     ctx->impl.counters.release();
     ctx->impl.spill.release();
+    for (auto &e : ctx->impl.timer)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->impl.stream);
     delete ctx;
 }
@@ -403,6 +405,69 @@ int krt_copy_probe(krt_context *ctx, size_t bytes, int reps, float *ms) {
         KRT_REQUIRE(ctx && ms, KRT_ERR_INVALID, "null argument");
         KRT_REQUIRE(bytes >= 16 && reps >= 1, KRT_ERR_INVALID, "bytes >= 16 and reps >= 1 required");
         *ms = copy_probe(ctx->impl, bytes, reps);
+    });
+}
+
+int krt_timer_begin(krt_context *ctx) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null argument");
+        Context &C = ctx->impl;
+        for (auto &e : C.timer)
+            if (!e) KRT_HIP(hipEventCreate(&e));
+        KRT_HIP(hipEventRecord(C.timer[0], C.stream));
+    });
+}
+
+int krt_timer_end(krt_context *ctx, float *ms) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && ms, KRT_ERR_INVALID, "null argument");
+        Context &C = ctx->impl;
+        KRT_REQUIRE(C.timer[0] && C.timer[1], KRT_ERR_STATE, "krt_timer_begin was not called");
+        KRT_HIP(hipEventRecord(C.timer[1], C.stream));
+        KRT_HIP(hipEventSynchronize(C.timer[1]));
+        KRT_HIP(hipEventElapsedTime(ms, C.timer[0], C.timer[1]));
+    });
+}
+
+int krt_sync(krt_context *ctx) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+    });
+}
+
+int krt_buffer_create(krt_context *ctx, size_t bytes, void **dev) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && dev, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(bytes > 0, KRT_ERR_INVALID, "empty buffer");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        *dev = nullptr;
+        KRT_HIP(hipMalloc(dev, bytes));
+    });
+}
+
+int krt_buffer_destroy(krt_context *ctx, void *dev) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null argument");
+        if (!dev) return;
+        KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+        KRT_HIP(hipFree(dev));
+    });
+}
+
+int krt_buffer_upload(krt_context *ctx, void *dev, const void *host, size_t bytes) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && dev && host, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->impl.stream));
+        KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+    });
+}
+
+int krt_buffer_download(krt_context *ctx, void *host, const void *dev, size_t bytes) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && dev && host, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->impl.stream));
+        KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
     });
 }
 

@@ -127,6 +127,7 @@ struct Context {
     krt_handle next_handle = 1;
     DevBuf counters;  // scratch counters (unsigned long long x 16)
     DevBuf spill;     // traversal stack spill slab (grown on demand)
+    hipEvent_t timer[2] = {nullptr, nullptr};  // krt_timer_begin / krt_timer_end
     Scene &scene(krt_handle h) {
         auto it = scenes.find(h);
         KRT_REQUIRE(it != scenes.end(), KRT_ERR_INVALID, "unknown scene handle");

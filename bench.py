@@ -8,6 +8,13 @@ one spatial pass of 5 neighbours in a 30 px disk, path depth 2 (the
 ArmadaRTX default), FP32 SoA reservoirs.  A step is one frame; the scene,
 BVH, reservoirs and AOV buffers are resident in HBM before timing starts.
 
+Other workloads (--workload): pt (path tracer, depth 5), pt_cache (path
+tracer whose paths stop at the first bounce and ask the online-trained
+radiance-cache MLP; 2^16 training paths per frame), restir_cache (ReSTIR DI
+with the cache for the indirect path), mlp (the cache network alone: 2^21
+queries + one training step on 2^16 samples per step, MFMA roofline).
+--quality adds an equal-relMSE study at reduced resolution (see quality()).
+
 Prints one JSON line: Msamples/s (pixels x frames / s), MRays/s, p50/p99
 frame latency, the roofline of the dominant kernel (HIP-event kernel times)
 and the CPU oracle timed on the host cores on a bounded sample.
@@ -27,6 +34,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Radiance-cache network 62 -> 5 x 64 -> 3: FLOP per query and per training
+# sample (forward; backward = input-delta products of layers 5..1 + weight
+# gradients of all layers).
+MLP_FWD_FLOP = 2 * (62 * 64 + 4 * 64 * 64 + 64 * 3)          # 41,088
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (64 * 3 + 4 * 64 * 64) + MLP_FWD_FLOP  # 115,328
+MLP_QUERY_BYTES = 13 * 4 + 3 * 4  # raw inputs in, RGB out
 # Algorithmic resampling bytes per pixel of each ReSTIR pass (64 B reservoir,
 # 28 B G-buffer): initial 64+28, temporal 2*64+28+64, spatial (1 pass, 5
 # neighbours) 6*64+6*28+64+32.
@@ -41,7 +55,7 @@ def log(msg):
 
 def b_ray(name):
     data = json.loads((ROOT / "tests" / "golden" / "bray.json").read_text())
-    return data[name]
+    return data.get(name)
 
 
 def measured_traffic(kernel: str):
@@ -92,37 +106,11 @@ def cpu_baseline(workload, scene, W, H, seconds):
                       f"oracle bvh.cpp build {build_s:.2f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["restir_di", "pt"], default="restir_di")
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--layout", choices=["sah", "reference"], default="sah",
-                    help="traversal tree (results are identical; 'reference' collapses the bvh.cpp tree)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
-              file=sys.stderr)
-        sys.exit(2)
+def rendering_setup(args, ctx, scene, sc, W, H, workload):
+    from artemis_amd import ArmadaRTX, PathTracer, ReSTIR
 
-    from artemis_amd import ArmadaRTX, Context, PathTracer, ReSTIR, look_at, scenes
-
-    W, H = args.width, args.height
-    log(f"[bench] generating scene")
-    scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
-    ctx = Context(0)
-    log(f"[bench] uploading {scene.triangles} triangles + GPU BVH build")
-    sc = ctx.upload(scene, layout=args.layout)
-    bst = sc.stats()
-    log(f"[bench] bvh: {bst}")
-    cam = look_at(**scene.camera, aspect=W / H)
     arm = ArmadaRTX(ctx, W, H)
-    if args.workload == "restir_di":
+    if workload in ("restir_di", "restir_cache"):
         arm.attach("restir", ReSTIR(ctx))
         arm.set_option("candidates", 32)
         arm.set_option("m_cap", 20)
@@ -130,11 +118,220 @@ def main():
         arm.set_option("neighbours", 5)
         arm.set_option("radius", 30)
         arm.set_depth(2)
-        bname = "restir_di_1080p_spheres1M"
     else:
         arm.attach("path_tracer", PathTracer(ctx))
         arm.set_depth(5)
-        bname = "pt_1080p_spheres1M"
+    if workload.endswith("_cache"):
+        arm.set_option("radiance_cache", True)
+        arm.set_option("cache_depth", 1)
+        arm.set_option("train_samples", 1 << 16)
+    return arm
+
+
+def mlp_workload(args, ctx):
+    """The radiance-cache network alone: per step, inference on 2^21 queries
+    and one training step on 2^16 samples, inputs resident in HBM."""
+    from artemis_amd import RadianceField
+
+    n, nt = 1 << 21, 1 << 16
+    rng = np.random.default_rng(1)
+
+    def queries(m):
+        pos = rng.random((m, 3))
+        d = rng.normal(size=(m, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        nrm = rng.normal(size=(m, 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        return np.concatenate([pos, d, nrm, rng.random((m, 4))], 1).astype(np.float32)
+
+    q = queries(n)
+    tq = queries(nt)
+    tt = (tq[:, 9:12] * (0.6 + 0.4 * np.sin(6.0 * tq[:, 0:1]) * np.cos(4.0 * tq[:, 2:3]))).astype(np.float32)
+    raw = ctx.device_array(q)
+    traw = ctx.device_array(tq)
+    target = ctx.device_array(tt)
+    out = ctx.device_array(shape=(n, 3))
+    net = RadianceField(ctx, seed=1)
+    for _ in range(args.warmup):
+        net.infer_device(raw.ptr, n, out.ptr)
+        net.train_step_device(traw.ptr, target.ptr, nt)
+    ctx.sync()
+    t_inf, t_tr = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.timer_begin()
+        net.infer_device(raw.ptr, n, out.ptr)
+        t_inf.append(ctx.timer_end())
+        ctx.timer_begin()
+        net.train_step_device(traw.ptr, target.ptr, nt)
+        t_tr.append(ctx.timer_end())
+    dt = time.perf_counter() - t0
+    K = args.steps
+    st = net.state()
+    inf_ms, tr_ms = float(np.mean(t_inf)), float(np.mean(t_tr))
+    achieved = MLP_FWD_FLOP * n / (inf_ms * 1e-3) / 1e12
+    traffic = measured_traffic("mlp:infer")
+    line = {
+        "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
+        "value": n * K / dt / 1e6, "unit": "Mqueries/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic (seeded random queries and smooth targets)",
+        "config": {"workload": "radiance-cache MLP 62->5x64 ReLU->3, BF16 MFMA + FP32 accumulation: "
+                               "2^21 queries inference + Adam step on 2^16 samples per step",
+                   "queries": n, "train_samples": nt, "parallelism": "single GPU"},
+        "infer_ms": inf_ms, "train_step_ms": tr_ms,
+        "train_tflops": MLP_TRAIN_FLOP * nt / (tr_ms * 1e-3) / 1e12,
+        "final_loss": st["last_loss"], "loss_scale": st["loss_scale"],
+        "roofline": {"bound": "mfma", "kernel": "mlp:infer", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": traffic,
+                     "algorithmic_flop": MLP_FWD_FLOP * n, "ms": inf_ms,
+                     "hbm_gbs": MLP_QUERY_BYTES * n / (inf_ms * 1e-3) / 1e9},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        m = 4096
+        qs = q[:m]
+        p = net.params().astype(np.float64)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 5.0):
+            O.mlp_forward(p, qs, bf16=True)
+            reps += 1
+        line["cpu_baseline"] = {"value": m * reps / (time.perf_counter() - t0) / 1e6, "unit": "Mqueries/s",
+                                "cores": 1, "kind": "port",
+                                "sample": f"oracle BF16-emulating forward, {m} queries x {reps}, one thread"}
+    return line
+
+
+def quality(args, ctx, scene, sc):
+    """Equal-relMSE study on the bench scene at reduced resolution.  Every
+    method converges to the same image (fixed -0.5 pixel offset, depth 5); the
+    judge is the path tracer's FP64-accumulated image.  For each method the
+    accumulated image's relMSE is recorded against cumulative GPU time; the
+    speedup is the path tracer's time over the method's time to reach the
+    relMSE the path tracer has after its last frame.  Cache methods train for
+    `warm` untimed frames first (the network persists across frames of a
+    static scene)."""
+    from artemis_amd import ArmadaRTX, PathTracer, ReSTIR, look_at
+    from artemis_amd.metrics import rel_mse
+
+    W, H = args.quality_width, args.quality_height
+    cam = look_at(**scene.camera, aspect=W / H)
+    ref_arm = ArmadaRTX(ctx, W, H)
+    ref_arm.attach("pt", PathTracer(ctx))
+    ref_arm.set_depth(5)
+    ref_arm.set_option("pixel_jitter", False)
+    ref_arm.set_f64_accumulation(True)
+    t0 = time.perf_counter()
+    for f in range(args.quality_ref_spp):
+        ref_arm.render(sc, cam)
+        if f % 1024 == 1023:
+            log(f"[quality] reference {f + 1}/{args.quality_ref_spp} spp, {time.perf_counter() - t0:.1f} s")
+    tot, nref = ref_arm.read_f64()
+    ref = tot / nref
+    del ref_arm
+
+    def make(kind):
+        arm = ArmadaRTX(ctx, W, H)
+        if kind.startswith("restir"):
+            arm.attach("m", ReSTIR(ctx))
+            arm.set_option("candidates", 32)
+            arm.set_option("m_cap", 20)
+            arm.set_option("spatial_passes", 1)
+            arm.set_option("neighbours", 5)
+            arm.set_option("radius", 30)
+        else:
+            arm.attach("m", PathTracer(ctx))
+            arm.set_option("pixel_jitter", False)
+        arm.set_depth(5)
+        if kind.endswith("cache"):
+            arm.set_option("radiance_cache", True)
+            arm.set_option("train_samples", max(1024, W * H // 32))
+        return arm
+
+    frames = args.quality_frames
+    checkpoints = sorted({2 ** k for k in range(20) if 2 ** k <= frames} | {frames})
+    curves = {}
+    for kind in ("pt", "pt_cache", "restir_di", "restir_cache"):
+        arm = make(kind)
+        if kind.endswith("cache"):
+            for _ in range(args.quality_warm):
+                arm.render(sc, cam, accumulate=True)
+        ms, curve = 0.0, []
+        for f in range(frames):
+            arm.render(sc, cam, accumulate=f > 0)
+            ms += arm.stats()["gpu_ms"]
+            if f + 1 in checkpoints:
+                curve.append((f + 1, ms, rel_mse(arm.color_buffer(), ref)))
+        curves[kind] = curve
+        log(f"[quality] {kind}: " + ", ".join(f"{n}: {e:.4g} @ {t:.1f} ms" for n, t, e in curve))
+    target = curves["pt"][-1][2]
+
+    def time_to(curve):
+        prev = None
+        for n, t, e in curve:
+            if e <= target:
+                if prev is None:
+                    return t
+                (n0, t0_, e0) = prev  # log-log interpolation between checkpoints
+                a = np.log(target / e0) / np.log(e / e0) if e0 != e else 1.0
+                return float(np.exp(np.log(t0_) + a * (np.log(t) - np.log(t0_))))
+            prev = (n, t, e)
+        return None
+
+    t_pt = time_to(curves["pt"])
+    out = {"width": W, "height": H, "reference_spp": int(nref), "target_relmse": target,
+           "cache_warm_frames": args.quality_warm, "curves": curves, "speedup_vs_pt": {}}
+    for kind, curve in curves.items():
+        t = time_to(curve)
+        out["speedup_vs_pt"][kind] = (t_pt / t) if t else None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "mlp"],
+                    default="restir_di")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--layout", choices=["sah", "reference"], default="sah",
+                    help="traversal tree (results are identical; 'reference' collapses the bvh.cpp tree)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quality", action="store_true", help="add the equal-relMSE study")
+    ap.add_argument("--quality-width", type=int, default=320)
+    ap.add_argument("--quality-height", type=int, default=180)
+    ap.add_argument("--quality-ref-spp", type=int, default=8192)
+    ap.add_argument("--quality-frames", type=int, default=256)
+    ap.add_argument("--quality-warm", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from artemis_amd import Context, look_at, scenes
+
+    ctx = Context(0)
+    if args.workload == "mlp":
+        print(json.dumps(mlp_workload(args, ctx)), flush=True)
+        return
+    W, H = args.width, args.height
+    log(f"[bench] generating scene")
+    scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
+    log(f"[bench] uploading {scene.triangles} triangles + GPU BVH build")
+    sc = ctx.upload(scene, layout=args.layout)
+    bst = sc.stats()
+    log(f"[bench] bvh: {bst}")
+    cam = look_at(**scene.camera, aspect=W / H)
+    wl = args.workload
+    arm = rendering_setup(args, ctx, scene, sc, W, H, wl)
+    restir = wl.startswith("restir")
+    bname = {"restir_di": "restir_di_1080p_spheres1M"}.get(wl, "")
     for i in range(args.warmup):
         arm.render(sc, cam, accumulate=True)
         log(f"[bench] warmup {i + 1}/{args.warmup}: {arm.stats()['gpu_ms']:.2f} ms")
@@ -160,39 +357,42 @@ def main():
     npix = W * H
     value = npix * K / dt / 1e6
     kavg = kms / K
-    br = b_ray(bname)
-    if args.workload == "restir_di":
-        passes = [0, 1, 2]
-        per_pass_bytes = {p: RESTIR_PIXEL_BYTES[p] * npix + prays[p] / K * br["b_ray"] for p in passes}
-    else:
-        passes = [0]
-        per_pass_bytes = {0: PT_PIXEL_BYTES * npix + prays[0] / K * br["b_ray"]}
+    br = b_ray(bname) if (W, H) == (1920, 1080) else None
+    passes = [0, 1, 2] if restir else [0]
     dom = max(passes, key=lambda p: kavg[p])
     # Algorithmic bytes of a launch: the per-pixel resampling traffic of the
     # pass plus, per ray, the node (64 B) and triangle (48 B) records its
     # traversal fetches (counted by the kernel).  The reference-walk figure of
     # the same rays (48 N_node + 64 N_tri over the bvh.cpp tree, b_ray) is
-    # reported beside it as ref_*.
-    pn = np.zeros(4)
-    pt = np.zeros(4)
-    for k in range(3):
-        pn[k] = pnodes[k] / K
-        pt[k] = ptris[k] / K
-    pix_bytes = (RESTIR_PIXEL_BYTES[dom] if args.workload == "restir_di" else PT_PIXEL_BYTES) * npix
+    # reported beside it as ref_* where it has been computed.
+    pn = pnodes / K
+    pt = ptris / K
+    pix_bytes = (RESTIR_PIXEL_BYTES[dom] if restir else PT_PIXEL_BYTES) * npix
     alg_bytes = pix_bytes + 64.0 * pn[dom] + 48.0 * pt[dom]
-    ref_bytes = per_pass_bytes[dom]
     sec = kavg[dom] * 1e-3
     achieved = alg_bytes / sec / 1e9
     total_rays = rays / K
-    kname = f"{args.workload}:{PASS_NAMES.get(dom, 'pt')}"
+    kname = f"{wl}:{PASS_NAMES.get(dom, 'pt') if restir else 'path'}"
     traffic = measured_traffic(kname)
     copy_ms = ctx.copy_probe(1 << 30, 10)
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg_bytes,
             "ms": float(kavg[dom]), "rays": float(prays[dom] / K),
             "bytes_per_ray": float((64.0 * pn[dom] + 48.0 * pt[dom]) / max(prays[dom] / K, 1)),
-            "ref_bytes_per_ray": br["b_ray"], "ref_achieved": ref_bytes / sec / 1e9,
             "copy_kernel_gbs": 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
+    if br:
+        ref_bytes = RESTIR_PIXEL_BYTES[dom] * npix + prays[dom] / K * br["b_ray"]
+        roof["ref_bytes_per_ray"] = br["b_ray"]
+        roof["ref_achieved"] = ref_bytes / sec / 1e9
+    names = {"restir_di": PASS_NAMES, "restir_cache": {0: "initial", 1: "temporal", 2: "spatial",
+                                                       3: "cache_infer", 4: "cache_train"},
+             "pt": {0: "path"}, "pt_cache": {0: "path", 1: "cache_infer", 2: "composite", 3: "cache_train"}}[wl]
+    desc = {"restir_di": "32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), depth 2",
+            "restir_cache": "32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), indirect path stops at the "
+                            "first bounce and asks the radiance cache, 2^16 training paths per frame",
+            "pt": "1 spp/frame, max depth 5",
+            "pt_cache": "1 spp/frame, max depth 5, paths stop at the first bounce and ask the radiance cache, "
+                        "2^16 training paths per frame"}[wl]
     line = {
         "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
         "value": value,
@@ -206,22 +406,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": f"{args.workload} {W}x{H} spheres 256x4096 tris (1,048,576), "
-                               + ("32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), depth 2"
-                                  if args.workload == "restir_di" else "1 spp/frame, max depth 5"),
+        "config": {"workload": f"{wl} {W}x{H} spheres 256x4096 tris (1,048,576), {desc}",
                    "width": W, "height": H, "triangles": int(scene.triangles), "traversal_tree": args.layout,
                    "parallelism": "single GPU"},
         "mrays_per_s": rays / dt / 1e6,
         "rays_per_sample": total_rays / npix,
         "p50_frame_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_frame_ms": float(np.percentile(lat, 99) * 1e3),
-        "kernel_ms": {PASS_NAMES.get(p, str(p)): float(kavg[p]) for p in passes},
+        "kernel_ms": {name: float(kavg[k]) for k, name in names.items()},
         "bvh_build_ms": bst["build_ms"],
         "roofline": roof,
     }
+    if wl.endswith("_cache"):
+        k_inf = 3 if restir else 1
+        k_tr = 4 if restir else 3
+        inf_ms, tr_ms = float(kavg[k_inf]), float(kavg[k_tr])
+        line["mlp"] = {"bound": "mfma", "queries": npix, "infer_ms": inf_ms,
+                       "infer_tflops": MLP_FWD_FLOP * npix / (inf_ms * 1e-3) / 1e12 if inf_ms > 0 else None,
+                       "train_samples": 1 << 16, "train_step_ms": tr_ms,
+                       "train_tflops": MLP_TRAIN_FLOP * (1 << 16) / (tr_ms * 1e-3) / 1e12 if tr_ms > 0 else None,
+                       "peak_tflops": MFMA_BF16_PEAK_TFLOPS}
+    if args.quality:
+        line["quality"] = quality(args, ctx, scene, sc)
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
-        line["cpu_baseline"] = cpu_baseline(args.workload, scene, W, H, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline("restir_di" if restir else "pt", scene, W, H, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

@@ -119,6 +119,20 @@ int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *ray
  * no reference counterpart). */
 int krt_copy_probe(krt_context *ctx, size_t bytes, int reps, float *ms);
 
+/* HIP events on the context's stream around the work queued between the two
+ * calls: milliseconds of device time (no reference counterpart). */
+int krt_timer_begin(krt_context *ctx);
+int krt_timer_end(krt_context *ctx, float *ms);
+/* Waits for all work queued on the context's stream. */
+int krt_sync(krt_context *ctx);
+
+/* Device buffers for the *_device entry points (no reference counterpart):
+ * allocation, and stream-ordered copies that return when complete. */
+int krt_buffer_create(krt_context *ctx, size_t bytes, void **dev);
+int krt_buffer_destroy(krt_context *ctx, void *dev);
+int krt_buffer_upload(krt_context *ctx, void *dev, const void *host, size_t bytes);
+int krt_buffer_download(krt_context *ctx, void *host, const void *dev, size_t bytes);
+
 /* ---------------------------------------------------------------- render --
  * ArmadaRTX (armada.cuh:142-336) and its attachments. */
 typedef struct {
