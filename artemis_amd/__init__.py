@@ -18,7 +18,7 @@ from .camera import CameraFrame, look_at
 from .scenes import SceneData
 
 __all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "CameraFrame", "look_at", "SceneData",
-           "load_attachment", "RadianceField"]
+           "load_attachment", "RadianceField", "Server", "DeviceArray"]
 
 
 def _f(a):
@@ -367,6 +367,72 @@ class RadianceField:
     def state(self) -> dict:
         st = L.MlpState()
         L.check(self.ctx._lib.krt_mlp_get_state(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+class Server:
+    """Render-request server over krt_server_* (continuous batching of
+    path-tracer requests on resident scenes, admission by HBM budget)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int | None = None, max_active: int | None = None,
+                 use_graph: bool = True, max_queue_ms: float = 0.0):
+        self.ctx = ctx
+        cfg = L.ServerConfig()
+        ctx._lib.krt_server_default_config(C.byref(cfg))
+        if hbm_budget is not None:
+            cfg.hbm_budget = hbm_budget
+        if max_active is not None:
+            cfg.max_active = max_active
+        cfg.use_graph = int(use_graph)
+        cfg.max_queue_ms = max_queue_ms
+        h = C.c_void_p()
+        L.check(ctx._lib.krt_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._shape = {}
+        ctx._adopt(self)
+
+    def _release(self):
+        if self.h:
+            self.ctx._lib.krt_server_destroy(self.h)
+            self.h = None
+
+    close = _release
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    def submit(self, scene: Scene, camera: CameraFrame, width: int, height: int, spp: int = 1,
+               max_depth: int = 5, seed: int = 0) -> int:
+        r = L.Request()
+        r.scene = scene.h
+        r.camera = camera.to_ctypes()
+        r.width, r.height, r.spp, r.max_depth, r.seed = width, height, spp, max_depth, seed
+        rid = C.c_uint64()
+        L.check(self.ctx._lib.krt_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._shape[rid.value] = (height, width)
+        return rid.value
+
+    def wait(self, rid: int):
+        """(colour (H, W, 4) float32, latency ms), or (None, latency ms) if
+        the request was rejected by the queueing deadline."""
+        h, w = self._shape.pop(rid)
+        out = np.empty((h, w, 4), np.float32)
+        lat = C.c_double()
+        rc = self.ctx._lib.krt_server_wait(self.h, rid, _f(out), C.byref(lat))
+        if rc == L.KRT_ERR_BUDGET:
+            return None, lat.value
+        L.check(rc)
+        return out, lat.value
+
+    def drain(self) -> None:
+        L.check(self.ctx._lib.krt_server_drain(self.h))
+
+    def stats(self) -> dict:
+        st = L.ServerStats()
+        L.check(self.ctx._lib.krt_server_get_stats(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
 

@@ -66,6 +66,25 @@ class MlpState(C.Structure):
 MLP_RAW, MLP_OUT, MLP_PARAMS = 13, 3, 20867
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_uint32), ("use_graph", C.c_int32),
+                ("max_queue_ms", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_uint64), ("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("max_depth", C.c_int32), ("seed", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("steps", C.c_uint64), ("active", C.c_uint32), ("queued", C.c_uint32),
+                ("max_concurrent", C.c_uint32), ("reserved", C.c_uint32), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
+                ("samples", C.c_double), ("busy_ms", C.c_double), ("wall_ms", C.c_double),
+                ("hbm_scratch", C.c_uint64), ("hbm_in_use", C.c_uint64), ("hbm_peak", C.c_uint64)]
+
+
 class FrameStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
                 ("gpu_ms", C.c_float), ("kernel_ms", C.c_float * 8), ("pass_rays", C.c_uint64 * 4),
@@ -136,6 +155,13 @@ _SIGS = {
     "krt_mlp_apply_gradient": (C.c_int, [vp, fp]),
     "krt_mlp_get_state": (C.c_int, [vp, C.POINTER(MlpState)]),
     "krt_attachment_cache": (C.c_int, [vp, C.POINTER(vp)]),
+    "krt_server_default_config": (None, [C.POINTER(ServerConfig)]),
+    "krt_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+    "krt_server_destroy": (None, [vp]),
+    "krt_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "krt_server_wait": (C.c_int, [vp, C.c_uint64, fp, C.POINTER(C.c_double)]),
+    "krt_server_drain": (C.c_int, [vp]),
+    "krt_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
 }
 
 EXPORTED = tuple(_SIGS)

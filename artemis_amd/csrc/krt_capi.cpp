@@ -14,6 +14,7 @@
 #include <string>
 
 #include "krt_internal.h"
+#include "krt_serve.h"
 
 using namespace krt;
 
@@ -930,5 +931,77 @@ int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state) {
         read_state(*net->ctx, *net->net, st);
         static_assert(sizeof(st) == sizeof(krt_mlp_state), "state layout");
         std::memcpy(state, &st, sizeof(st));
+    });
+}
+
+// ---------------------------------------------------------------- serving
+struct krt_server {
+    krt_context *ctx = nullptr;
+    Server *impl = nullptr;
+};
+
+void krt_server_default_config(krt_server_config *cfg) {
+    if (!cfg) return;
+    ServeConfig d;
+    cfg->hbm_budget = d.hbm_budget;
+    cfg->max_active = d.max_active;
+    cfg->use_graph = d.use_graph;
+    cfg->max_queue_ms = d.max_queue_ms;
+}
+
+int krt_server_create(krt_context *ctx, const krt_server_config *cfg, krt_server **out) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
+        krt_server_config c;
+        krt_server_default_config(&c);
+        if (cfg) c = *cfg;
+        ServeConfig sc;
+        sc.hbm_budget = c.hbm_budget;
+        sc.max_active = c.max_active;
+        sc.use_graph = c.use_graph;
+        sc.max_queue_ms = c.max_queue_ms;
+        auto srv = std::make_unique<krt_server>();
+        srv->ctx = ctx;
+        srv->impl = serve_create(ctx->impl, sc);
+        *out = srv.release();
+    });
+}
+
+void krt_server_destroy(krt_server *srv) {
+    if (!srv) return;
+    serve_destroy(srv->impl);
+    delete srv;
+}
+
+int krt_server_submit(krt_server *srv, const krt_request *req, uint64_t *id) {
+    return guard([&] {
+        KRT_REQUIRE(srv && req && id, KRT_ERR_INVALID, "null argument");
+        Scene &s = srv->ctx->impl.scene(req->scene);
+        KRT_HIP(hipStreamSynchronize(srv->ctx->impl.stream));  // scene uploads and builds are complete
+        *id = serve_submit(srv->impl, s, req->camera, req->width, req->height, req->spp, req->max_depth, req->seed);
+    });
+}
+
+int krt_server_wait(krt_server *srv, uint64_t id, float *color, double *latency_ms) {
+    int rc = KRT_OK;
+    int g = guard([&] {
+        KRT_REQUIRE(srv, KRT_ERR_INVALID, "null argument");
+        rc = serve_wait(srv->impl, id, color, latency_ms, nullptr, nullptr);
+        if (rc != KRT_OK) throw Error(rc, "request rejected (queueing deadline)");
+    });
+    return g;
+}
+
+int krt_server_drain(krt_server *srv) {
+    return guard([&] {
+        KRT_REQUIRE(srv, KRT_ERR_INVALID, "null argument");
+        serve_drain(srv->impl);
+    });
+}
+
+int krt_server_get_stats(krt_server *srv, krt_server_stats *out) {
+    return guard([&] {
+        KRT_REQUIRE(srv && out, KRT_ERR_INVALID, "null argument");
+        serve_stats(srv->impl, *out);
     });
 }

@@ -1,0 +1,32 @@
+// Render-request server (krt_serve.hip): continuous batching of path-tracer
+// requests over resident scenes, admission by HBM budget.
+#pragma once
+
+#include "krt_internal.h"
+
+namespace krt {
+
+constexpr int kServeMaxActive = 64;           // requests sharing one step
+constexpr int kServeMaxDepth = 8;             // path depth cap (history slab)
+constexpr int kServeSpillDepth = 64;          // traversal stack levels beyond LDS
+constexpr uint32_t kServeBlocks = 256 * 8;    // persistent grid: 8 workgroups per CU
+
+struct ServeConfig {
+    size_t hbm_budget = (size_t)8 << 30;  // fixed scratch + per-request AOV buffers
+    uint32_t max_active = 16;
+    int use_graph = 1;
+    double max_queue_ms = 0.0;  // > 0: queued requests older than this are rejected
+};
+
+struct Server;
+Server *serve_create(Context &ctx, const ServeConfig &cfg);
+void serve_destroy(Server *s);
+uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w, uint32_t h, uint32_t spp,
+                      int max_depth, uint32_t seed);
+// KRT_OK with the colour buffer (W x H x float4) or KRT_ERR_BUDGET when the
+// request was rejected; the request is forgotten afterwards.
+int serve_wait(Server *s, uint64_t id, float *color, double *latency_ms, uint32_t *w, uint32_t *h);
+void serve_drain(Server *s);
+void serve_stats(Server *s, krt_server_stats &out);
+
+}  // namespace krt

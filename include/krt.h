@@ -279,6 +279,53 @@ int krt_mlp_get_state(krt_mlp *net, krt_mlp_state *state);
  * attachment is unloaded or destroyed; krt_mlp_destroy ignores it. */
 int krt_attachment_cache(krt_attachment *att, krt_mlp **out);
 
+/* ------------------------------------------------------------- serving --
+ * Render-request server (no reference counterpart): continuous batching of
+ * path-tracer requests.  A request's answer is the colour buffer a
+ * path_tracer attachment holds after `spp` accumulated frames of the same
+ * scene, camera, depth and seed (ArmadaRTX::render, armada.cu:733-780), bit
+ * for bit, whatever else runs beside it.  Every step renders one frame of
+ * each active request in one persistent launch (a captured hipGraph);
+ * requests are admitted FIFO as slots and HBM budget free up.  Scenes must
+ * outlive the requests that use them. */
+typedef struct krt_server krt_server;
+
+typedef struct {
+    uint64_t hbm_budget;   /* bytes: fixed scratch + 64 B per pixel of each active request */
+    uint32_t max_active;   /* requests per step, 1..64 */
+    int32_t use_graph;     /* 1: replay a captured hipGraph per step */
+    double max_queue_ms;   /* > 0: reject requests queued longer than this */
+} krt_server_config;
+
+typedef struct {
+    krt_handle scene;
+    krt_camera camera;
+    uint32_t width, height, spp;
+    int32_t max_depth;     /* 0..8 */
+    uint32_t seed;         /* the attachment's "seed" option */
+} krt_request;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, steps;
+    uint32_t active, queued, max_concurrent, reserved;
+    double p50_ms, p99_ms;           /* request latency, submit -> result ready */
+    double step_p50_ms, step_p99_ms; /* device time of one batched step */
+    double samples;                  /* pixels x spp of completed requests */
+    double busy_ms, wall_ms;         /* device time of all steps; first submit -> last completion */
+    uint64_t hbm_scratch, hbm_in_use, hbm_peak;
+} krt_server_stats;
+
+void krt_server_default_config(krt_server_config *cfg);
+int krt_server_create(krt_context *ctx, const krt_server_config *cfg, krt_server **out);
+void krt_server_destroy(krt_server *srv);
+/* KRT_ERR_BUDGET when the request alone exceeds the budget. */
+int krt_server_submit(krt_server *srv, const krt_request *req, uint64_t *id);
+/* Blocks until the request completes (color: width x height x 4 floats, may be
+ * NULL) or was rejected (KRT_ERR_BUDGET); the id is released afterwards. */
+int krt_server_wait(krt_server *srv, uint64_t id, float *color, double *latency_ms);
+int krt_server_drain(krt_server *srv);
+int krt_server_get_stats(krt_server *srv, krt_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,119 @@
+"""Render-request server (krt_serve.hip) through the C ABI: a request's answer
+is the path-tracer attachment's colour buffer after spp frames, bit for bit;
+it is the same alone and in a mix of other scenes, resolutions and spp; the
+HBM budget bounds what runs at once; requests too large for the budget are
+refused and queueing deadlines shed load."""
+import numpy as np
+import pytest
+
+from artemis_amd import ArmadaRTX, PathTracer, Server, look_at, scenes
+from artemis_amd._lib import KRT_ERR_BUDGET, KrtError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def two_scenes(krt_ctx):
+    a = scenes.cornell()
+    b = scenes.sky_room()
+    return [(krt_ctx.upload(a), a), (krt_ctx.upload(b), b)]
+
+
+def armada_pt(ctx, sc, data, w, h, spp, depth, seed):
+    arm = ArmadaRTX(ctx, w, h)
+    arm.attach("pt", PathTracer(ctx))
+    arm.set_depth(depth)
+    arm.set_option("seed", seed)
+    cam = look_at(**data.camera, aspect=w / h)
+    for _ in range(spp):
+        arm.render(sc, cam, accumulate=True)
+    return arm.color_buffer()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_request_equals_path_tracer_attachment(krt_ctx, two_scenes, use_graph):
+    sc, data = two_scenes[0]
+    w, h, spp, depth, seed = 72, 40, 5, 3, 7
+    srv = Server(krt_ctx, use_graph=use_graph)
+    rid = srv.submit(sc, look_at(**data.camera, aspect=w / h), w, h, spp=spp, max_depth=depth, seed=seed)
+    img, lat = srv.wait(rid)
+    srv.close()
+    ref = armada_pt(krt_ctx, sc, data, w, h, spp, depth, seed)
+    assert lat > 0
+    assert np.array_equal(bits(img), bits(ref))
+
+
+def test_result_is_the_same_alone_and_in_the_mix(krt_ctx, two_scenes):
+    srv = Server(krt_ctx, max_active=8)
+    (sa, da), (sb, db) = two_scenes
+    probe = dict(scene=sa, camera=look_at(**da.camera, aspect=1.0), width=64, height=64, spp=6, max_depth=4, seed=11)
+    rid = srv.submit(**probe)
+    alone, _ = srv.wait(rid)
+    rng = np.random.default_rng(0)
+    reqs = []
+    for k in range(10):
+        sc, d = (sa, da) if k % 2 else (sb, db)
+        w, h = [(32, 32), (96, 54), (128, 72), (50, 50)][k % 4]
+        spp, depth = int(rng.integers(1, 9)), int(rng.integers(1, 6))
+        rid = srv.submit(sc, look_at(**d.camera, aspect=w / h), w, h, spp=spp, max_depth=depth, seed=k)
+        reqs.append((rid, sc, d, w, h, spp, depth, k))
+        if k == 4:
+            mixed_id = srv.submit(**probe)
+    mixed, _ = srv.wait(mixed_id)
+    out = {rid: srv.wait(rid)[0] for rid, *_ in reqs}
+    st = srv.stats()
+    srv.close()
+    assert st["max_concurrent"] > 1 and st["completed"] == 12 and st["rejected"] == 0
+    assert np.array_equal(bits(alone), bits(mixed))
+    # the other members of the mix are their own attachment renders too
+    for rid, sc, d, w, h, spp, depth, seed in reqs[:3]:
+        assert np.array_equal(bits(out[rid]), bits(armada_pt(krt_ctx, sc, d, w, h, spp, depth, seed)))
+
+
+def test_hbm_budget_bounds_concurrency(krt_ctx, two_scenes):
+    probe = Server(krt_ctx)
+    scratch = probe.stats()["hbm_scratch"]
+    probe.close()
+    req_bytes = 4 * 16 * 128 * 128
+    srv = Server(krt_ctx, hbm_budget=scratch + int(1.5 * req_bytes), max_active=8)
+    sc, d = two_scenes[1]
+    cam = look_at(**d.camera, aspect=1.0)
+    ids = [srv.submit(sc, cam, 128, 128, spp=3, max_depth=2, seed=s) for s in range(5)]
+    imgs = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert all(im is not None for im in imgs)
+    assert st["max_concurrent"] == 1
+    assert st["hbm_peak"] == req_bytes and st["hbm_in_use"] == 0
+    assert st["completed"] == 5
+
+
+def test_request_larger_than_budget_is_refused(krt_ctx, two_scenes):
+    probe = Server(krt_ctx)
+    scratch = probe.stats()["hbm_scratch"]
+    probe.close()
+    srv = Server(krt_ctx, hbm_budget=scratch + 4 * 16 * 64 * 64)
+    sc, d = two_scenes[0]
+    with pytest.raises(KrtError) as e:
+        srv.submit(sc, look_at(**d.camera), 1920, 1080, spp=1)
+    assert e.value.code == KRT_ERR_BUDGET
+    rid = srv.submit(sc, look_at(**d.camera), 64, 64, spp=1)
+    assert srv.wait(rid)[0] is not None
+    srv.close()
+
+
+def test_queueing_deadline_sheds_load(krt_ctx, two_scenes):
+    srv = Server(krt_ctx, max_active=1, max_queue_ms=1.0)
+    sc, d = two_scenes[0]
+    cam = look_at(**d.camera, aspect=1.0)
+    ids = [srv.submit(sc, cam, 256, 256, spp=16, max_depth=5, seed=s) for s in range(12)]
+    res = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert res[0] is not None
+    assert st["rejected"] >= 1 and st["rejected"] + st["completed"] == 12
+    assert sum(r is None for r in res) == st["rejected"]
