@@ -375,7 +375,7 @@ class Server:
     path-tracer requests on resident scenes, admission by HBM budget)."""
 
     def __init__(self, ctx: Context, hbm_budget: int | None = None, max_active: int | None = None,
-                 use_graph: bool = True, max_queue_ms: float = 0.0):
+                 use_graph: bool = True, max_queue_ms: float = 0.0, step_pixels: int | None = None):
         self.ctx = ctx
         cfg = L.ServerConfig()
         ctx._lib.krt_server_default_config(C.byref(cfg))
@@ -385,6 +385,8 @@ class Server:
             cfg.max_active = max_active
         cfg.use_graph = int(use_graph)
         cfg.max_queue_ms = max_queue_ms
+        if step_pixels is not None:
+            cfg.step_pixels = step_pixels
         h = C.c_void_p()
         L.check(ctx._lib.krt_server_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

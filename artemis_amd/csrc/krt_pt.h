@@ -18,11 +18,14 @@
 
 namespace krt {
 
-KRT_DEV void pt_pixel(const LaunchInfo &li, const SceneView &sv, uint32_t pix, const TraceStack &stk,
+// `sample` is the frame's sample index (li.samples for one frame per launch;
+// the serving kernel runs several consecutive frames of a pixel in one thread,
+// which leaves every accumulation step as a launch per frame would make it).
+KRT_DEV void pt_pixel(const LaunchInfo &li, const SceneView &sv, uint32_t pix, int sample, const TraceStack &stk,
                       const History &hist, TravStats &ts, RayCount &rc, double *acc64) {
-    const float count = (float)li.samples;
+    const float count = (float)sample;
     const uint32_t px = pix % li.width, py = pix / li.width;
-    f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+    f3 seed = pixel_seed(px, py, (uint32_t)sample, li.user_seed);
     f3 o, d;
     make_ray(li, px, py, seed, li.jitter != 0, o, d);
     f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     TravStats ts;
     RayCount rc;
     const uint32_t npix = li.width * li.height;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) pt_pixel(li, sv, pix, stk, hist, ts, rc, A.acc64);
+    for (uint32_t pix = gtid; pix < npix; pix += nthreads)
+        pt_pixel(li, sv, pix, li.samples, stk, hist, ts, rc, A.acc64);
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
     wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);

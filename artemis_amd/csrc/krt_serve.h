@@ -10,12 +10,14 @@ constexpr int kServeMaxActive = 64;           // requests sharing one step
 constexpr int kServeMaxDepth = 8;             // path depth cap (history slab)
 constexpr int kServeSpillDepth = 64;          // traversal stack levels beyond LDS
 constexpr uint32_t kServeBlocks = 256 * 8;    // persistent grid: 8 workgroups per CU
+constexpr uint32_t kServeMaxFrames = 64;      // frames of one request in one step
 
 struct ServeConfig {
     size_t hbm_budget = (size_t)8 << 30;  // fixed scratch + per-request AOV buffers
     uint32_t max_active = 16;
     int use_graph = 1;
     double max_queue_ms = 0.0;  // > 0: queued requests older than this are rejected
+    uint64_t step_pixels = 1ull << 22;  // pixel-frames one step aims at, shared fairly
 };
 
 struct Server;

@@ -5,10 +5,14 @@
 // accumulated frames (ArmadaRTX::render, source/amadeus/armada.cu:733-780,
 // with the estimator of source/optix/amadeus_path_tracer.cu:73-228).
 //
-// One host thread runs the loop.  Each step renders one more frame of every
-// active request with a single persistent launch: the grid strides over the
-// concatenated pixels of all active requests and finds each pixel's request by
-// binary search in a device table {LaunchInfo, SceneView, first pixel}.  The
+// One host thread runs the loop.  Each step advances every active request with
+// a single persistent launch: the grid strides over the concatenated pixels of
+// all active requests and finds each pixel's request by binary search in a
+// device table {LaunchInfo, SceneView, first pixel, frames}.  A request's share
+// of a step is a fair share of `step_pixels` pixel-frames: small requests run
+// several consecutive frames of a pixel in one thread (the accumulation is per
+// pixel, so the result is the one a launch per frame gives) and finish in one
+// step; large ones advance a frame per step.  The
 // step (table upload + kernel) is a hipGraph captured once; only the table's
 // contents change between steps.  Requests arriving while others run join at
 // the next step as soon as a slot and HBM budget are free (FIFO admission,
@@ -41,9 +45,10 @@ namespace krt {
 namespace {
 
 struct ServeDesc {
-    LaunchInfo li;
+    LaunchInfo li;  // li.samples: the first frame of this step
     SceneView sv;
     uint32_t pix_begin;
+    uint32_t frames;  // consecutive frames this step
 };
 
 struct ServeTable {
@@ -69,7 +74,8 @@ __global__ __launch_bounds__(kTraceBlock) void serve_kernel(const ServeTable *__
             else hi = mid - 1;
         }
         const ServeDesc &D = tab->d[lo];
-        pt_pixel(D.li, D.sv, g - D.pix_begin, stk, hist, ts, rc, nullptr);
+        const int first = D.li.samples, frames = (int)D.frames;
+        for (int f = 0; f < frames; f++) pt_pixel(D.li, D.sv, g - D.pix_begin, first + f, stk, hist, ts, rc, nullptr);
     }
 }
 
@@ -87,7 +93,7 @@ struct ServeRequest {
     int max_depth = 5;
     krt_camera cam{};
     int state = 0;  // 0 queued, 1 active, 2 done, 3 rejected
-    uint32_t frames = 0;
+    uint32_t frames = 0, step_frames = 1;
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t bytes = 0;
     DevBuf aov;  // colour, normal, albedo, position planes (float4 x W x H each)
@@ -153,6 +159,7 @@ void Server::step(std::vector<ServeRequest *> &batch) {
         li.position = planes + 3 * n;
         d.sv = make_view(*r.scene);
         d.pix_begin = total;
+        d.frames = r.step_frames;
         total += r.w * r.h;
     }
     table_host->total = total;
@@ -199,6 +206,10 @@ void Server::run() {
                 ServeRequest *r = queue.front();
                 if (scratch + in_use + r->bytes > cfg.hbm_budget) break;
                 r->aov.alloc(r->bytes);
+                // frame 0 accumulates dst * 0 + sample: recycled memory holding
+                // a non-finite value would poison it (stream-ordered, outside the
+                // captured step)
+                KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->bytes, stream));
                 in_use += r->bytes;
                 peak = std::max(peak, in_use);
                 r->state = 1;
@@ -210,11 +221,17 @@ void Server::run() {
             if (active.empty()) continue;
             max_concurrent = std::max<uint64_t>(max_concurrent, active.size());
             std::vector<ServeRequest *> batch = active;
+            const double share = (double)cfg.step_pixels / (double)batch.size();
+            for (ServeRequest *r : batch) {
+                const double k = std::floor(share / ((double)r->w * r->h));
+                const uint32_t left = r->spp - r->frames;
+                r->step_frames = (uint32_t)std::max(1.0, std::min<double>({k, (double)left, (double)kServeMaxFrames}));
+            }
             lk.unlock();
             step(batch);
             std::vector<ServeRequest *> finished;
             for (ServeRequest *r : batch) {
-                r->frames++;
+                r->frames += r->step_frames;
                 if (r->frames >= r->spp) {
                     r->result.resize(4 * (size_t)r->w * r->h);
                     KRT_HIP(hipMemcpyAsync(r->result.data(), r->aov.p, sizeof(float) * r->result.size(),

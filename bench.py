@@ -204,6 +204,95 @@ def mlp_workload(args, ctx):
     return line
 
 
+SERVE_SCENES = [  # (instances, slices, stacks): 10k .. 2M triangles
+    (10, 32, 17), (12, 64, 33), (25, 64, 33), (61, 64, 33), (122, 64, 33), (256, 64, 33), (366, 64, 33),
+    (256, 64, 65)]
+SERVE_RES = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+SERVE_SPP = [1, 4, 16, 64]
+
+
+def serve_workload(args, ctx):
+    """Render-request serving: Poisson arrivals (mean --serve-rate per s) for
+    --serve-seconds, resolution uniform over SERVE_RES, spp over SERVE_SPP, a
+    uniformly chosen one of 8 resident scenes, path depth 5.  The server
+    batches every active request into one launch per step, admits by HBM
+    budget and sheds requests queued longer than --serve-deadline-ms."""
+    import threading
+
+    from artemis_amd import Server, look_at, scenes
+
+    resident = []
+    for k, (inst, sl, st) in enumerate(SERVE_SCENES):
+        nt = inst * 2 * sl * (st - 1)
+        data = scenes.spheres(instances=inst, slices=sl, stacks=st, emissive=max(64, nt // 256), seed=10 + k)
+        resident.append((ctx.upload(data), data))
+        log(f"[serve] scene {k}: {data.triangles} triangles resident")
+    srv = Server(ctx, hbm_budget=args.serve_budget_gb << 30, max_active=args.serve_active,
+                 max_queue_ms=args.serve_deadline_ms)
+    rng = np.random.default_rng(5)
+    n = int(rng.poisson(args.serve_rate * args.serve_seconds))
+    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    plan = [(float(t), int(rng.integers(len(resident))), SERVE_RES[int(rng.integers(4))],
+             SERVE_SPP[int(rng.integers(4))]) for t in np.cumsum(gaps)]
+    ids, done = [], []
+    lock = threading.Lock()
+
+    def consumer():
+        k = 0
+        while True:
+            with lock:
+                if k < len(ids):
+                    rid = ids[k]
+                elif k >= n:
+                    return
+                else:
+                    rid = None
+            if rid is None:
+                time.sleep(0.001)
+                continue
+            img, lat = srv.wait(rid)
+            done.append((img is not None, lat))
+            k += 1
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    t0 = time.perf_counter()
+    offered = 0.0
+    for i, (t, si, (w, h), spp) in enumerate(plan):
+        delay = t - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        sc, data = resident[si]
+        rid = srv.submit(sc, look_at(**data.camera, aspect=w / h), w, h, spp=spp, max_depth=5, seed=i)
+        offered += w * h * spp
+        with lock:
+            ids.append(rid)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat_ok = [l for ok, l in done if ok]
+    return {
+        "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
+        "value": st["samples"] / wall / 1e6 if wall > 0 else 0.0,
+        "unit": "Msamples/s", "n_gpus": 1, "steps": int(st["steps"]), "warmup": 0,
+        "ms_per_step": st["busy_ms"] / max(st["steps"], 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes, seeded arrivals)",
+        "config": {"workload": f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s, "
+                               "resolution uniform over 256^2/512^2/1280x720/1920x1080, spp over 1/4/16/64, "
+                               "8 resident scenes of 10k-2M triangles, path depth 5",
+                   "hbm_budget_gb": args.serve_budget_gb, "max_active": args.serve_active,
+                   "queue_deadline_ms": args.serve_deadline_ms, "parallelism": "single GPU"},
+        "requests": n, "completed": int(st["completed"]), "rejected": int(st["rejected"]),
+        "offered_msamples_per_s": offered / wall / 1e6,
+        "p50_latency_ms": float(np.percentile(lat_ok, 50)) if lat_ok else None,
+        "p99_latency_ms": float(np.percentile(lat_ok, 99)) if lat_ok else None,
+        "p50_step_ms": st["step_p50_ms"], "p99_step_ms": st["step_p99_ms"],
+        "gpu_busy_frac": st["busy_ms"] / (wall * 1e3), "max_concurrent": int(st["max_concurrent"]),
+        "hbm_peak_gb": st["hbm_peak"] / 2 ** 30, "wall_s": wall,
+    }
+
+
 def quality(args, ctx, scene, sc):
     """Equal-relMSE study on the bench scene at reduced resolution.  Every
     method converges to the same image (fixed -0.5 pixel offset, depth 5); the
@@ -294,8 +383,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "mlp"],
+    ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "mlp", "serve"],
                     default="restir_di")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-deadline-ms", type=float, default=2000.0)
+    ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -319,6 +413,9 @@ def main():
     ctx = Context(0)
     if args.workload == "mlp":
         print(json.dumps(mlp_workload(args, ctx)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(serve_workload(args, ctx)), flush=True)
         return
     W, H = args.width, args.height
     log(f"[bench] generating scene")

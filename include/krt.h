@@ -284,9 +284,10 @@ int krt_attachment_cache(krt_attachment *att, krt_mlp **out);
  * path-tracer requests.  A request's answer is the colour buffer a
  * path_tracer attachment holds after `spp` accumulated frames of the same
  * scene, camera, depth and seed (ArmadaRTX::render, armada.cu:733-780), bit
- * for bit, whatever else runs beside it.  Every step renders one frame of
- * each active request in one persistent launch (a captured hipGraph);
- * requests are admitted FIFO as slots and HBM budget free up.  Scenes must
+ * for bit, whatever else runs beside it.  Every step advances each active
+ * request by its share of `step_pixels` pixel-frames (at least one frame) in
+ * one persistent launch (a captured hipGraph); requests are admitted FIFO as
+ * slots and HBM budget free up.  Scenes must
  * outlive the requests that use them. */
 typedef struct krt_server krt_server;
 
@@ -295,6 +296,7 @@ typedef struct {
     uint32_t max_active;   /* requests per step, 1..64 */
     int32_t use_graph;     /* 1: replay a captured hipGraph per step */
     double max_queue_ms;   /* > 0: reject requests queued longer than this */
+    uint64_t step_pixels;  /* pixel-frames per step, shared fairly among active requests */
 } krt_server_config;
 
 typedef struct {

@@ -74,6 +74,20 @@ def test_result_is_the_same_alone_and_in_the_mix(krt_ctx, two_scenes):
         assert np.array_equal(bits(out[rid]), bits(armada_pt(krt_ctx, sc, d, w, h, spp, depth, seed)))
 
 
+@pytest.mark.parametrize("step_pixels", [1, 1 << 22])
+def test_frames_per_step_do_not_change_the_result(krt_ctx, two_scenes, step_pixels):
+    # step_pixels = 1: one frame per step; 2^22: the whole request in one step
+    sc, data = two_scenes[1]
+    w, h, spp, depth, seed = 80, 60, 9, 4, 3
+    srv = Server(krt_ctx, step_pixels=step_pixels)
+    img, _ = srv.wait(srv.submit(sc, look_at(**data.camera, aspect=w / h), w, h, spp=spp, max_depth=depth,
+                                 seed=seed))
+    steps = srv.stats()["steps"]
+    srv.close()
+    assert steps == (spp if step_pixels == 1 else 1)
+    assert np.array_equal(bits(img), bits(armada_pt(krt_ctx, sc, data, w, h, spp, depth, seed)))
+
+
 def test_hbm_budget_bounds_concurrency(krt_ctx, two_scenes):
     probe = Server(krt_ctx)
     scratch = probe.stats()["hbm_scratch"]
@@ -110,7 +124,7 @@ def test_queueing_deadline_sheds_load(krt_ctx, two_scenes):
     srv = Server(krt_ctx, max_active=1, max_queue_ms=1.0)
     sc, d = two_scenes[0]
     cam = look_at(**d.camera, aspect=1.0)
-    ids = [srv.submit(sc, cam, 256, 256, spp=16, max_depth=5, seed=s) for s in range(12)]
+    ids = [srv.submit(sc, cam, 512, 512, spp=64, max_depth=5, seed=s) for s in range(12)]
     res = [srv.wait(i)[0] for i in ids]
     st = srv.stats()
     srv.close()
