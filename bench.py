@@ -469,7 +469,8 @@ def main():
     sec = kavg[dom] * 1e-3
     achieved = alg_bytes / sec / 1e9
     total_rays = rays / K
-    kname = f"{wl}:{PASS_NAMES.get(dom, 'pt') if restir else 'path'}"
+    # rocprofv3 kernel names (profiles/traffic.json keys)
+    kname = f"{wl}:{PASS_NAMES[dom]}" if restir else {"pt": "pt:pt", "pt_cache": "pt_cache:cache_path"}[wl]
     traffic = measured_traffic(kname)
     copy_ms = ctx.copy_probe(1 << 30, 10)
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
