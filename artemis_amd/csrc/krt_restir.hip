@@ -383,6 +383,10 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
             uint32_t nidx[kMaxNeighbours];
             int sizes[kMaxNeighbours];
             int cnt = 0;
+            // Visibility of the selected sample from this pixel, when one of
+            // the neighbour rays below already traced it (same origin,
+            // direction and extent as the final shading ray).
+            bool sel_known = false, sel_occ = false;
 #pragma unroll
             for (int i = 0; i < kMaxNeighbours; i++) {
                 nidx[i] = pix;
@@ -390,8 +394,19 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
                 if (i < S) {
                     uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
                     Resv nb = A.cur.load(i0);
-                    float t_nb = occluded_target(A.sv, nb.data, hit, stk, ts, rc);
-                    if (nb.M > 0) merged.update(nb.data, t_nb * nb.W * (float)nb.M);
+                    // the reference traces this ray also for an empty
+                    // neighbour, whose target is then unused
+                    if (nb.M > 0) {
+                        f3 Dn = nb.data.point - hit.x;
+                        float dn = length(Dn);
+                        bool occ;
+                        f3 Ln = direct_occ(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
+                        float t_nb = Ln.x + Ln.y + Ln.z;
+                        if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
+                            sel_known = true;
+                            sel_occ = occ;
+                        }
+                    }
                     nidx[i] = i0;
                     sizes[i] = nb.M;
                     cnt += nb.M;
@@ -418,7 +433,13 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
             merged.W = (denominator > 0.0f) ? merged.w / denominator : 0.0f;
             f3 D = merged.data.point - hit.x;
             float d = length(D);
-            f3 lighting = direct_occ(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
+            f3 lighting;
+            if (sel_known) {
+                lighting = sel_occ ? mk(0.0f, 0.0f, 0.0f)
+                                   : direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
+            } else {
+                lighting = direct_occ(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
+            }
             direct = hit.mat.emission + lighting * merged.W;
             A.prev.store(pix, merged);
         } else {

@@ -29,12 +29,18 @@ struct Hit {
     int rank;
 };
 
+// LDS-typed pointer: keeps stack traffic on ds_read/ds_write instead of flat
+// loads through the generic aperture.
+typedef __attribute__((address_space(3))) int lds_int;
+
 struct TraceStack {
-    int *lds;      // this lane's column: lds[k * stride]
+    lds_int *lds;  // this lane's column: lds[k * stride]
     int stride;
     int lds_depth;
     int *gmem;     // this lane's spill column: gmem[k * gstride]
     int gstride;
+    __device__ __forceinline__ TraceStack(int *lds_col, int stride_, int depth, int *gcol, int gstride_)
+        : lds((lds_int *)lds_col), stride(stride_), lds_depth(depth), gmem(gcol), gstride(gstride_) {}
     __device__ __forceinline__ void put(int k, int v) const {
         if (k < lds_depth) lds[k * stride] = v;
         else gmem[(k - lds_depth) * gstride] = v;
