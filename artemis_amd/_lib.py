@@ -34,7 +34,7 @@ class BvhStats(C.Structure):
     _fields_ = [
         ("triangles", C.c_uint32), ("serialized_nodes", C.c_uint32), ("traversal_nodes", C.c_uint32),
         ("depth_bound", C.c_uint32), ("build_levels", C.c_uint32), ("build_ms", C.c_float),
-        ("layout", C.c_int),
+        ("layout", C.c_int), ("binary_nodes", C.c_uint32), ("node_bytes", C.c_uint32),
     ]
 
 

@@ -20,6 +20,8 @@
 // equal values), so the serialised bits equal a sequential build's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cfloat>
 #include <cstring>
 
@@ -704,7 +706,122 @@ __global__ void sah_root_leaf_kernel(const float4 *bmin, const float4 *bmax, int
 
 inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// ---------------------------------------------------------------------------
+// 4-wide collapse of a binary traversal tree (either layout).  A wide node
+// starts from a binary node's two children and twice replaces the internal
+// child of largest surface area by its two children; every internal child
+// that remains becomes a wide node of the next level.  Node: 8 float4 (one
+// 128 B cache line) = {min.x}, {max.x}, {min.y}, {max.y}, {min.z}, {max.z} of
+// the four children, their links (wide node index, leaf range, or 0 = empty:
+// the root is nobody's child), pad.
+// ---------------------------------------------------------------------------
+struct WideItem {
+    int bin;   // binary node
+    int wide;  // its wide node
+};
+
+KRT_DEV float half_area(float4 mn, float4 mx) {
+    float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void collapse_kernel(const float4 *__restrict__ bin, float4 *__restrict__ wide,
+                                const WideItem *__restrict__ front_in, const int *n_in, WideItem *front_out,
+                                int *n_out, int *wcount) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_in) return;
+    const WideItem it = front_in[i];
+    float4 mn[4], mx[4];
+    int link[4];
+    auto child = [&](int b, int k, float4 &a, float4 &c, int &l) {
+        const float4 X = bin[4 * b], Y = bin[4 * b + 1], Z = bin[4 * b + 2], L = bin[4 * b + 3];
+        a = k ? make_float4(X.z, Y.z, Z.z, 0.0f) : make_float4(X.x, Y.x, Z.x, 0.0f);
+        c = k ? make_float4(X.w, Y.w, Z.w, 0.0f) : make_float4(X.y, Y.y, Z.y, 0.0f);
+        l = __float_as_int(k ? L.y : L.x);
+    };
+    child(it.bin, 0, mn[0], mx[0], link[0]);
+    child(it.bin, 1, mn[1], mx[1], link[1]);
+    int nc = 2;
+    for (int round = 0; round < 2; round++) {
+        int pick = -1;
+        float best = -1.0f;
+        for (int j = 0; j < nc; j++) {
+            if (link[j] <= 0) continue;  // leaf range
+            const float a = half_area(mn[j], mx[j]);
+            if (a > best) {
+                best = a;
+                pick = j;
+            }
+        }
+        if (pick < 0) break;
+        const int b = link[pick];
+        child(b, 0, mn[pick], mx[pick], link[pick]);
+        child(b, 1, mn[nc], mx[nc], link[nc]);
+        nc++;
+    }
+    int ninternal = 0;
+    for (int j = 0; j < nc; j++) ninternal += link[j] > 0 ? 1 : 0;
+    int base = ninternal ? atomicAdd(wcount, ninternal) : 0;
+    int slot = ninternal ? atomicAdd(n_out, ninternal) : 0;
+    int out[4] = {0, 0, 0, 0};
+    for (int j = 0; j < nc; j++) {
+        if (link[j] > 0) {
+            out[j] = base;
+            front_out[slot++] = WideItem{link[j], base};
+            base++;
+        } else {
+            out[j] = link[j];
+        }
+    }
+    for (int j = nc; j < 4; j++) {
+        mn[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        mx[j] = mn[j];
+    }
+    float4 *w = wide + 8 * (size_t)it.wide;
+    w[0] = make_float4(mn[0].x, mn[1].x, mn[2].x, mn[3].x);
+    w[1] = make_float4(mx[0].x, mx[1].x, mx[2].x, mx[3].x);
+    w[2] = make_float4(mn[0].y, mn[1].y, mn[2].y, mn[3].y);
+    w[3] = make_float4(mx[0].y, mx[1].y, mx[2].y, mx[3].y);
+    w[4] = make_float4(mn[0].z, mn[1].z, mn[2].z, mn[3].z);
+    w[5] = make_float4(mx[0].z, mx[1].z, mx[2].z, mx[3].z);
+    w[6] = make_float4(__int_as_float(out[0]), __int_as_float(out[1]), __int_as_float(out[2]),
+                       __int_as_float(out[3]));
+    w[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
+
+// Returns the number of wide levels; A.wide / A.nwide filled.
+static uint32_t collapse_wide(Context &ctx, Accel &A, int nbin) {
+    hipStream_t st = ctx.stream;
+    A.wide.alloc(sizeof(float4) * 8 * (size_t)nbin);
+    DevBuf front[2] = {DevBuf(sizeof(WideItem) * (size_t)nbin), DevBuf(sizeof(WideItem) * (size_t)nbin)};
+    DevBuf counts(sizeof(int) * 3);  // n[0], n[1], wide count
+    const int init[3] = {1, 0, 1};
+    const WideItem root{0, 0};
+    KRT_HIP(hipMemcpyAsync(front[0].p, &root, sizeof(root), hipMemcpyHostToDevice, st));
+    KRT_HIP(hipMemcpyAsync(counts.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+    int *cnt = counts.as<int>();
+    int n = 1;
+    uint32_t lv = 0;
+    while (n > 0) {
+        KRT_HIP(hipMemsetAsync(cnt + ((lv + 1) & 1), 0, sizeof(int), st));
+        collapse_kernel<<<blocks((size_t)n, 256), 256, 0, st>>>(A.nodes.as<float4>(), A.wide.as<float4>(),
+                                                              front[lv & 1].as<WideItem>(), cnt + (lv & 1),
+                                                              front[(lv + 1) & 1].as<WideItem>(), cnt + ((lv + 1) & 1),
+                                                              cnt + 2);
+        KRT_HIP(hipGetLastError());
+        KRT_HIP(hipMemcpyAsync(&n, cnt + ((lv + 1) & 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        KRT_HIP(hipStreamSynchronize(st));
+        lv++;
+        KRT_REQUIRE(lv < 100000, KRT_ERR_HIP, "wide collapse did not converge");
+    }
+    int nw = 0;
+    KRT_HIP(hipMemcpyAsync(&nw, cnt + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    KRT_HIP(hipStreamSynchronize(st));
+    A.nwide = (uint32_t)nw;
+    return lv;
+}
 
 void accel_build(Context &ctx, Scene &s, int leaf_max, int layout) {
     KRT_REQUIRE(leaf_max >= 1 && leaf_max <= 32, KRT_ERR_INVALID, "leaf_max must be in [1, 32]");
@@ -819,6 +936,8 @@ void accel_build(Context &ctx, Scene &s, int leaf_max, int layout) {
     tri_pack_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), order,
                                                     rank_of.as<int>(), n, A.tri.as<float4>());
     KRT_HIP(hipGetLastError());
+    // each wide level pushes at most three siblings
+    const uint32_t wide_levels = collapse_wide(ctx, A, nnodes);
     KRT_HIP(hipEventRecord(e1, st));
     KRT_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -830,7 +949,7 @@ void accel_build(Context &ctx, Scene &s, int leaf_max, int layout) {
     A.nser = nser;
     A.nnodes = (uint32_t)nnodes;
     A.levels = levels;
-    A.depth_bound = depth;
+    A.depth_bound = std::max(depth, 3 * wide_levels + 1);
     A.layout = layout;
     A.build_ms = ms;
     s.version++;

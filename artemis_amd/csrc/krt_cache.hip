@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
-            traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
             if (h.prim < 0) break;
             Surface sf;
             if (!load_surface(sv, h, d, sf)) break;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
             f3 n = sf.n;
             f3 x = offset_point(sf.x, n, mat.type);
             SurfaceHit sh{mat, sf.entering, n, -d, x};
-            f3 direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+            f3 direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);  // binary: occupancy
             int out;
             f3 wi;
             float pdf;

@@ -26,7 +26,8 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 SceneView make_view(const Scene &s) {
     return SceneView{s.pos.as<float>(),   s.nrm.as<float>(),    s.idx.as<uint32_t>(),
                      s.tri_mat.as<uint32_t>(), s.mats.as<float4>(), s.lights.as<float4>(),
-                     s.nl,                s.accel.tri.as<float4>(), s.accel.nodes.as<float4>()};
+                     s.nl,                s.accel.tri.as<float4>(), s.accel.nodes.as<float4>(),
+                     s.accel.wide.as<float4>()};
 }
 
 }  // namespace krt
@@ -318,11 +319,13 @@ int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out) {
         KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
         out->triangles = A.nt;
         out->serialized_nodes = A.nser;
-        out->traversal_nodes = A.nnodes;
+        out->traversal_nodes = A.nwide;
         out->depth_bound = A.depth_bound;
         out->build_levels = A.levels;
         out->build_ms = A.build_ms;
         out->layout = A.layout;
+        out->binary_nodes = A.nnodes;
+        out->node_bytes = 128;
     });
 }
 

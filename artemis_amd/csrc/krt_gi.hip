@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
         uint32_t gmat = 0;
         Hit h;
         rc.closest++;
-        traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+        traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
         Surface sf;
         if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
             Material mat = load_material(sv.mats, sf.mat_id);
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
                 for (int level = 1; level <= li.max_depth; level++) {
                     Hit hh;
                     rc.closest++;
-                    traverse<false>(sv.nodes, sv.tri, po, pd, kTraceTmax, stk, hh, ts);
+                    traverse<false>(sv.nodes, sv.wide, sv.tri, po, pd, kTraceTmax, stk, hh, ts);
                     if (hh.prim < 0) break;
                     Surface s2;
                     if (!load_surface(sv, hh, pd, s2)) break;

@@ -86,7 +86,9 @@ struct Accel {
     DevBuf ser;        // float4 x 3 per node, bvh.cpp layout
     DevBuf leaf_order; // int per rank
     DevBuf tri;        // float4 x 3 per triangle in rank order: {v1, prim}, {e1}, {e2}
-    DevBuf nodes;      // float4 x 4 per traversal node
+    DevBuf nodes;      // float4 x 4 per binary traversal node (collapse input)
+    DevBuf wide;       // float4 x 8 per 4-wide traversal node (what the kernels walk)
+    uint32_t nwide = 0;
 };
 
 struct Scene {
@@ -101,7 +103,7 @@ struct Scene {
     uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
     size_t bytes() const {
         return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + accel.ser.bytes +
-               accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes;
+               accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes + accel.wide.bytes;
     }
 };
 
@@ -115,7 +117,8 @@ struct SceneView {
     const float4 *lights;
     uint32_t nl;
     const float4 *tri;
-    const float4 *nodes;
+    const float4 *nodes;  // binary traversal tree
+    const float4 *wide;   // its 4-wide collapse
 };
 
 SceneView make_view(const Scene &s);

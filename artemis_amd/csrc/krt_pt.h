@@ -33,7 +33,7 @@ KRT_DEV void pt_pixel(const LaunchInfo &li, const SceneView &sv, uint32_t pix, i
     for (int level = 0; level <= li.max_depth; level++) {
         Hit h;
         rc.closest++;
-        traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+        traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
         if (h.prim < 0) break;  // __miss__: no environment map, value 0
         Surface sf;
         if (!load_surface(sv, h, d, sf)) break;

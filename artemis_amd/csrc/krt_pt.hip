@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
-            traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
             if (h.prim < 0) break;  // __miss__: no environment map, value 0
             Surface sf;
             if (!load_surface(sv, h, d, sf)) break;

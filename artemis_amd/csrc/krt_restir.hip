@@ -95,10 +95,11 @@ KRT_DEV f3 direct_li(const SurfaceHit &sh, f3 Le, f3 normal, int type, f3 D, flo
 }
 
 // direct_occluded (:399-421)
+template <bool WIDE = true>
 KRT_DEV f3 direct_occ(const SceneView &sv, const SurfaceHit &sh, const RSample &s, f3 D, float d,
                       const TraceStack &stk, TravStats &ts, RayCount &rc, bool *occ_out = nullptr) {
     rc.shadow++;
-    bool occ = occluded(sv, sh.x, D, d - kShadowEps, stk, ts);
+    bool occ = occluded<WIDE>(sv, sh.x, D, d - kShadowEps, stk, ts);
     if (occ_out) *occ_out = occ;
     if (occ) return mk(0.0f, 0.0f, 0.0f);
     return direct_li(sh, s.Le, s.normal, s.type, D, d);
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
-            traverse<false>(sv.nodes, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
             if (h.prim < 0) break;
             Surface sf;
             if (!load_surface(sv, h, d, sf)) break;
@@ -223,13 +224,13 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
                 f3 D = smp.point - sh.x;
                 float dd = length(D);
                 bool occ;
-                f3 Li = direct_occ(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
+                f3 Li = direct_occ<false>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
                 res.resample(Li.x + Li.y + Li.z);
                 // the reference traces this same visibility ray a second time
                 res.W *= 1.0f - (occ ? 1.0f : 0.0f);
                 A.gmat[pix] = sf.mat_id;
             } else {
-                direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+                direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);
             }
             int out;
             f3 wi;

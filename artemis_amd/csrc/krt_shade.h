@@ -55,12 +55,14 @@ KRT_DEV f3 offset_point(f3 x, f3 n, int type) {
     return x + ((type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
 }
 
+template <bool WIDE = true>
 KRT_DEV bool occluded(const SceneView &sv, f3 o, f3 d, float tmax, const TraceStack &stk, TravStats &ts) {
     Hit h;
-    return traverse<true>(sv.nodes, sv.tri, o, d, tmax, stk, h, ts);
+    return traverse<true, WIDE>(sv.nodes, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
 }
 
 // Ld with one uniformly chosen triangle light (no quad lights, no envmap).
+template <bool WIDE = true>
 KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceStack &stk, TravStats &ts,
               RayCount &rc) {
     const unsigned total = sv.nl;
@@ -77,7 +79,7 @@ KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceSt
     float R = length(lpos - sh.x);
     rc.shadow++;
     f3 contr;
-    if (occluded(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
+    if (occluded<WIDE>(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
         contr = mk(0.0f, 0.0f, 0.0f);
     } else {
         f3 f = brdf(sh, wi, eDiffuse);

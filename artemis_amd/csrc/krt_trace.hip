@@ -13,6 +13,7 @@ namespace {
 
 template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__restrict__ nodes,
+                                                             const float4 *__restrict__ wide,
                                                              const float4 *__restrict__ tri,
                                                              const float4 *__restrict__ rays, uint32_t n,
                                                              float4 *__restrict__ hits, int32_t *__restrict__ occ,
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__rest
     for (uint32_t i = gtid; i < n; i += nthreads) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         Hit h;
-        bool r = traverse<ANY>(nodes, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts);
+        bool r = traverse<ANY>(nodes, wide, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts);
         if (ANY) {
             occ[i] = r ? 1 : 0;
         } else {
@@ -100,11 +101,12 @@ static void trace_common(Context &ctx, Scene &s, const float *rays_dev, uint32_t
     uint32_t nb = persistent_blocks(n, kTraceBlock, kMaxResidentBlocks);
     int *spill = spill_buffer(ctx, s, nb * kTraceBlock);
     if (any)
-        trace_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.tri.as<float4>(),
+        trace_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.wide.as<float4>(),
+                                                              s.accel.tri.as<float4>(),
                                                               (const float4 *)rays_dev, n, nullptr, occ_dev, spill,
                                                               (int)s.accel.depth_bound);
     else
-        trace_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(),
+        trace_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.wide.as<float4>(),
                                                                s.accel.tri.as<float4>(), (const float4 *)rays_dev, n,
                                                                (float4 *)hits_dev, nullptr, spill,
                                                                (int)s.accel.depth_bound);

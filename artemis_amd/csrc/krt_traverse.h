@@ -1,12 +1,14 @@
 // Closest-hit and any-hit traversal of the krt BVH layout on gfx950.
 //
-// Node: 4 x float4 = child boxes as {c0.min, c0.max, c1.min, c1.max} per axis
-// plus the two child links (internal node index, or a leaf range
-// 0x80000000 | (count-1) << 26 | first_rank).  Triangles sit in leaf-preorder
-// rank order as {v1, prim}, {e1}, {e2}.
+// Node (4-wide, one 128 B line): 8 x float4 = the four children's min.x,
+// max.x, min.y, max.y, min.z, max.z as SoA float4s, their links (wide node
+// index, a leaf range 0x80000000 | (count-1) << 26 | first_rank, or 0 for an
+// empty slot), pad.  Triangles sit in leaf-preorder rank order as
+// {v1, prim}, {e1}, {e2}.
 //
-// Ordered descent (near child first, far child on a per-lane stack kept in
-// LDS; entries past the LDS depth spill to a per-thread global slab).
+// Ordered descent (nearest child next, the other hit children on a per-lane
+// stack kept in LDS, far first; entries past the LDS depth spill to a
+// per-thread global slab).
 //
 // Result contract (the reference GLSL walk, shaders/rt/modules/bvh.glsl:52-127,
 // keeps the first leaf in preorder among equal t): the closest hit is the
@@ -116,8 +118,103 @@ KRT_DEV float mt_test(const float4 *tri, int r, f3 o, f3 d, float &b1, float &b2
 
 KRT_DEV bool is_leaf_link(int l) { return l < 0; }
 
+// Ascending compare-exchange of (key, link) pairs.
+KRT_DEV void cswap(float &ka, int &la, float &kb, int &lb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    const int l = sw ? lb : la;
+    kb = sw ? ka : kb;
+    lb = sw ? la : lb;
+    ka = k;
+    la = l;
+}
+
 template <bool ANY>
-KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict__ tri, f3 o, f3 d, float tmax,
+KRT_DEV bool traverse4(const float4 *__restrict__ nodes, const float4 *__restrict__ tri, f3 o, f3 d, float tmax,
+                      const TraceStack &stk, Hit &hit, TravStats &ts) {
+    const f3 inv = safe_inv(d);
+    const int zmask = zero_mask(d);
+    // Only (t, rank, position) of the best candidate live through the walk;
+    // the hit record is recomputed from the winner at the end (the same
+    // Möller–Trumbore expressions give the same bits).
+    float best = tmax;
+    int best_rank = 0x7fffffff;
+    int best_pos = -1;
+    int sp = 0;
+    int cur = 0;  // the root; no child link is 0, so 0 also marks an empty slot
+    for (;;) {
+        if (!is_leaf_link(cur)) {
+            const float4 *nd = nodes + 8 * (size_t)cur;
+            const float4 MNX = nd[0], MXX = nd[1], MNY = nd[2], MXY = nd[3], MNZ = nd[4], MXZ = nd[5], L = nd[6];
+            ts.nodes++;
+            float k0, k1, k2, k3;
+            int l0 = __float_as_int(L.x), l1 = __float_as_int(L.y), l2 = __float_as_int(L.z),
+                l3 = __float_as_int(L.w);
+            const float inf = 3.4e38f;
+            bool h;
+            h = l0 != 0 && slab(MNX.x, MXX.x, MNY.x, MXY.x, MNZ.x, MXZ.x, o, inv, zmask, best, k0);
+            k0 = h ? k0 : inf;
+            h = l1 != 0 && slab(MNX.y, MXX.y, MNY.y, MXY.y, MNZ.y, MXZ.y, o, inv, zmask, best, k1);
+            k1 = h ? k1 : inf;
+            h = l2 != 0 && slab(MNX.z, MXX.z, MNY.z, MXY.z, MNZ.z, MXZ.z, o, inv, zmask, best, k2);
+            k2 = h ? k2 : inf;
+            h = l3 != 0 && slab(MNX.w, MXX.w, MNY.w, MXY.w, MNZ.w, MXZ.w, o, inv, zmask, best, k3);
+            k3 = h ? k3 : inf;
+            // near first: sort the four (entry, link) pairs, missed boxes last
+            cswap(k0, l0, k1, l1);
+            cswap(k2, l2, k3, l3);
+            cswap(k0, l0, k2, l2);
+            cswap(k1, l1, k3, l3);
+            cswap(k1, l1, k2, l2);
+            if (k0 == inf) {
+                if (sp == 0) break;
+                cur = stk.get(--sp);
+                continue;
+            }
+            if (k3 != inf) stk.put(sp++, l3);
+            if (k2 != inf) stk.put(sp++, l2);
+            if (k1 != inf) stk.put(sp++, l1);
+            cur = l0;
+        } else {
+            const int first = cur & 0x03FFFFFF;
+            const int cnt = ((cur >> 26) & 31) + 1;
+            for (int k = 0; k < cnt; k++) {
+                float b1, b2;
+                int prim, r;
+                float t = mt_test(tri, first + k, o, d, b1, b2, prim, r);
+                ts.tris++;
+                if (ANY) {
+                    if (t > 0.0f && t < tmax) return true;
+                } else {
+                    bool take = t > 0.0f && (t < best || (best_pos >= 0 && t == best && r < best_rank));
+                    best = take ? t : best;
+                    best_rank = take ? r : best_rank;
+                    best_pos = take ? first + k : best_pos;
+                }
+            }
+            if (sp == 0) break;
+            cur = stk.get(--sp);
+        }
+    }
+    hit.prim = -1;
+    hit.rank = 0x7fffffff;
+    hit.t = 0.0f;
+    hit.b1 = hit.b2 = 0.0f;
+    if (ANY || best_pos < 0) return false;
+    float b1 = 0.0f, b2 = 0.0f;
+    int prim, r;
+    hit.t = mt_test(tri, best_pos, o, d, b1, b2, prim, r);
+    hit.b1 = b1;
+    hit.b2 = b2;
+    hit.prim = prim;
+    hit.rank = r;
+    return true;
+}
+
+// Binary walk: node = 4 x float4 {c0.min, c0.max, c1.min, c1.max} per axis
+// + two links; near child next, far child on the stack.
+template <bool ANY>
+KRT_DEV bool traverse2(const float4 *__restrict__ nodes, const float4 *__restrict__ tri, f3 o, f3 d, float tmax,
                       const TraceStack &stk, Hit &hit, TravStats &ts) {
     const f3 inv = safe_inv(d);
     const int zmask = zero_mask(d);
@@ -184,6 +281,18 @@ KRT_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict
     hit.prim = prim;
     hit.rank = r;
     return true;
+}
+
+// Which tree a query walks.  Closest-hit rays keep the binary tree.  Shadow
+// rays take the 4-wide tree (four boxes per 128 B fetch, fewer dependent
+// loads) unless the kernel opts out: the wide walk needs more registers, and a
+// kernel already near the occupancy cliff (ReSTIR's initial pass, the cached
+// path kernel) runs faster on the binary tree at higher occupancy.
+template <bool ANY, bool WIDE = ANY>
+KRT_DEV bool traverse(const float4 *__restrict__ bin, const float4 *__restrict__ wide, const float4 *__restrict__ tri,
+                      f3 o, f3 d, float tmax, const TraceStack &stk, Hit &hit, TravStats &ts) {
+    if (WIDE) return traverse4<ANY>(wide, tri, o, d, tmax, stk, hit, ts);
+    return traverse2<ANY>(bin, tri, o, d, tmax, stk, hit, ts);
 }
 
 }  // namespace krt

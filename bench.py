@@ -458,14 +458,15 @@ def main():
     passes = [0, 1, 2] if restir else [0]
     dom = max(passes, key=lambda p: kavg[p])
     # Algorithmic bytes of a launch: the per-pixel resampling traffic of the
-    # pass plus, per ray, the node (64 B) and triangle (48 B) records its
-    # traversal fetches (counted by the kernel).  The reference-walk figure of
+    # pass plus, per ray, the node (128 B 4-wide line) and triangle (48 B)
+    # records its traversal fetches (counted by the kernel).  The reference-walk figure of
     # the same rays (48 N_node + 64 N_tri over the bvh.cpp tree, b_ray) is
     # reported beside it as ref_* where it has been computed.
     pn = pnodes / K
     pt = ptris / K
     pix_bytes = (RESTIR_PIXEL_BYTES[dom] if restir else PT_PIXEL_BYTES) * npix
-    alg_bytes = pix_bytes + 64.0 * pn[dom] + 48.0 * pt[dom]
+    node_b = float(bst.get("node_bytes", 64))
+    alg_bytes = pix_bytes + node_b * pn[dom] + 48.0 * pt[dom]
     sec = kavg[dom] * 1e-3
     achieved = alg_bytes / sec / 1e9
     total_rays = rays / K
@@ -476,7 +477,7 @@ def main():
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg_bytes,
             "ms": float(kavg[dom]), "rays": float(prays[dom] / K),
-            "bytes_per_ray": float((64.0 * pn[dom] + 48.0 * pt[dom]) / max(prays[dom] / K, 1)),
+            "bytes_per_ray": float((node_b * pn[dom] + 48.0 * pt[dom]) / max(prays[dom] / K, 1)),
             "copy_kernel_gbs": 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
     if br:
         ref_bytes = RESTIR_PIXEL_BYTES[dom] * npix + prays[dom] / K * br["b_ray"]

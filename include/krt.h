@@ -81,11 +81,13 @@ int krt_scene_lights(krt_context *ctx, krt_handle scene, float *out, size_t nflo
 typedef struct {
     uint32_t triangles;
     uint32_t serialized_nodes; /* 2N-1 */
-    uint32_t traversal_nodes;
-    uint32_t depth_bound;  /* upper bound of the traversal tree depth */
+    uint32_t traversal_nodes;  /* 4-wide nodes the kernels walk */
+    uint32_t depth_bound;  /* upper bound of the traversal stack depth */
     uint32_t build_levels; /* level-synchronous build passes */
     float build_ms;        /* GPU time of the build (HIP events) */
-    int layout;            /* KRT_LAYOUT_* of the traversal tree */
+    int layout;            /* KRT_LAYOUT_* of the binary tree the wide tree collapses */
+    uint32_t binary_nodes; /* binary traversal tree before the 4-wide collapse */
+    uint32_t node_bytes;   /* bytes fetched per traversal-node visit */
 } krt_bvh_stats;
 
 enum {
