@@ -77,6 +77,12 @@ class Context:
     def sync(self) -> None:
         L.check(self._lib.krt_sync(self.h))
 
+    def hdr_to_ldr(self, pixels_ptr: int, target_ptr: int, width: int, height: int, tonemapping: int = 0) -> None:
+        """hdr_to_ldr (include/cuda/color.cuh:82-121) on device buffers: width x
+        height float4 -> packed RGBA8 uint32, rows flipped; 0 = ACES, 1 = sRGB.
+        Queued on the context stream."""
+        L.check(self._lib.krt_hdr_to_ldr(self.h, pixels_ptr, target_ptr, width, height, tonemapping))
+
     def device_array(self, data=None, shape=None, dtype=np.float32) -> "DeviceArray":
         """A device buffer, filled from `data` or of `shape` x `dtype`."""
         return DeviceArray(self, data, shape, dtype)
@@ -519,6 +525,13 @@ class ArmadaRTX:
     def position_buffer(self):
         return self._read(3)
 
+    def ldr_buffer(self, tonemapping: int = 0) -> np.ndarray:
+        """The colour buffer through hdr_to_ldr: (H, W) uint32 RGBA8, rows
+        flipped (ACES = 0, sRGB = 1)."""
+        out = np.empty((self.height, self.width), np.uint32)
+        L.check(self.ctx._lib.krt_armada_read_ldr(self.h, tonemapping, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
     def upload_buffer(self, which: int, data: np.ndarray) -> None:
         data = np.ascontiguousarray(data, np.float32)
         assert data.size == self.width * self.height * 4
@@ -555,3 +568,9 @@ class ArmadaRTX:
         n = C.c_uint64()
         L.check(self.ctx._lib.krt_armada_read_f64(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
         return out, n.value
+
+    def read_f64_squares(self) -> np.ndarray:
+        """FP64 per-pixel sums of squared path-tracer samples (H, W, 3)."""
+        out = np.empty((self.height, self.width, 3), np.float64)
+        L.check(self.ctx._lib.krt_armada_read_f64_squares(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out

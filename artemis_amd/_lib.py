@@ -140,6 +140,9 @@ _SIGS = {
     "krt_armada_set_f64_accumulation": (C.c_int, [vp, C.c_int]),
     "krt_armada_read_reservoirs": (C.c_int, [vp, C.c_int, fp]),
     "krt_armada_read_f64": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "krt_armada_read_f64_squares": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "krt_hdr_to_ldr": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int]),
+    "krt_armada_read_ldr": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
     "krt_load_attachment": (AttachmentRet, [vp, C.c_char_p]),
     "krt_mlp_default_config": (None, [C.POINTER(MlpConfig)]),
     "krt_mlp_create": (C.c_int, [vp, C.POINTER(MlpConfig), C.c_uint64, C.POINTER(vp)]),

@@ -49,10 +49,12 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
     History hist{A.hist + gtid, nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const float count = (float)li.samples;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
-        const uint32_t px = pix % li.width, py = pix / li.width;
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         uint32_t slot = 0;
         const CacheView &cv = A.cv;
         const bool train = cv.training(pix, slot);

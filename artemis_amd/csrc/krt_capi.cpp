@@ -127,6 +127,7 @@ struct krt_armada {
     Context *ctx = nullptr;
     uint32_t w = 0, h = 0;
     DevBuf color, normal, albedo, position;
+    DevBuf ldr;  // RGBA8 staging of krt_armada_read_ldr
     std::map<std::string, krt_attachment *> attachments;
     std::string active, previous;
     int max_depth = 2;  // armada.cu:56
@@ -716,7 +717,7 @@ int krt_armada_set_f64_accumulation(krt_armada *a, int enable) {
         a->f64 = enable != 0;
         a->f64_frames = 0;
         if (a->f64) {
-            size_t bytes = sizeof(double) * 3 * (size_t)a->w * a->h;
+            size_t bytes = sizeof(double) * 6 * (size_t)a->w * a->h;  // sums, then sums of squares
             if (a->acc64.bytes < bytes) a->acc64.alloc(bytes);
             KRT_HIP(hipMemsetAsync(a->acc64.p, 0, bytes, a->ctx->stream));
             KRT_HIP(hipStreamSynchronize(a->ctx->stream));
@@ -734,6 +735,40 @@ int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames) {
                                a->ctx->stream));
         KRT_HIP(hipStreamSynchronize(a->ctx->stream));
         *frames = a->f64_frames;
+    });
+}
+
+int krt_armada_read_f64_squares(krt_armada *a, double *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(a->f64, KRT_ERR_STATE, "FP64 accumulation is not enabled");
+        const size_t n3 = 3 * (size_t)a->w * a->h;
+        KRT_HIP(hipMemcpyAsync(out, a->acc64.as<double>() + n3, sizeof(double) * n3, hipMemcpyDeviceToHost,
+                               a->ctx->stream));
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+    });
+}
+
+int krt_hdr_to_ldr(krt_context *ctx, const void *pixels_dev, void *target_dev, uint32_t width, uint32_t height,
+                   int tonemapping) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && ((pixels_dev && target_dev) || width == 0 || height == 0), KRT_ERR_INVALID,
+                    "null argument");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        hdr_to_ldr(ctx->impl, (const float4 *)pixels_dev, (uint32_t *)target_dev, width, height, tonemapping);
+    });
+}
+
+int krt_armada_read_ldr(krt_armada *a, int tonemapping, uint32_t *out) {
+    return guard([&] {
+        KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");
+        Context &C = *a->ctx;
+        KRT_HIP(hipSetDevice(C.device));
+        const size_t n = (size_t)a->w * a->h;
+        if (a->ldr.bytes < sizeof(uint32_t) * n) a->ldr.alloc(sizeof(uint32_t) * n);
+        hdr_to_ldr(C, a->color.as<float4>(), a->ldr.as<uint32_t>(), a->w, a->h, tonemapping);
+        KRT_HIP(hipMemcpyAsync(out, a->ldr.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, C.stream));
+        KRT_HIP(hipStreamSynchronize(C.stream));
     });
 }
 

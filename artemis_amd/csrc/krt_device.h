@@ -92,6 +92,22 @@ KRT_DEV f3 rand_uniform_3f(f3 &seed) {
     return mk(fract(seed.x), fract(seed.y), fract(seed.z));
 }
 
+// Work item -> pixel of a grid-stride pixel loop, in 8x8 tiles taken in
+// row-major tile order: the 64 lanes of a wavefront shade one tile, so their
+// primary rays, G-buffer reads and light samples stay close on screen.  Items
+// of a partial edge tile that fall outside the image return false.
+struct TileOrder {
+    uint32_t width, height, tiles_x, items;
+    KRT_DEV TileOrder(uint32_t w, uint32_t h)
+        : width(w), height(h), tiles_x((w + 7u) >> 3), items(((w + 7u) >> 3) * ((h + 7u) >> 3) * 64u) {}
+    KRT_DEV bool pixel(uint32_t item, uint32_t &px, uint32_t &py) const {
+        const uint32_t tile = item >> 6, t = item & 63u;
+        px = (tile % tiles_x) * 8u + (t & 7u);
+        py = (tile / tiles_x) * 8u + (t >> 3);
+        return px < width && py < height;
+    }
+};
+
 // Seeds: three u32 words bit-cast into the float3 state.
 KRT_DEV f3 pixel_seed(uint32_t px, uint32_t py, uint32_t sample, uint32_t user) {
     return mk(__uint_as_float(px), __uint_as_float(py), __uint_as_float(sample + 0x9E3779B9u * user));

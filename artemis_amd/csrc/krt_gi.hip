@@ -178,10 +178,12 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
     History hist{A.hist + gtid, nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const f3 center = ld3(li.center);
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
-        const uint32_t px = pix % li.width, py = pix / li.width;
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         GRes R = A.cur.load(pix);
         R.w = 0.0f;
         R.M = 0;
@@ -306,8 +308,11 @@ __global__ __launch_bounds__(kTraceBlock) void gi_temporal_kernel(GiArgs A) {
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         GPix gb = load_g(A.g, pix);
         GRes R = A.cur.load(pix);
         if (gb.valid && R.M > 0) {
@@ -360,9 +365,12 @@ __global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const int S = A.neighbours;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         GPix gb = load_g(A.g, pix);
         GRes R = A.cur.load(pix);
         if (gb.valid && R.M > 0) {

@@ -202,6 +202,9 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions
                unsigned long long *ctr, PassTimes &times);
 float copy_probe(Context &ctx, size_t bytes, int reps);
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
+// hdr_to_ldr (include/cuda/color.cuh:82-121) on the context stream; no sync.
+void hdr_to_ldr(Context &ctx, const float4 *pixels, uint32_t *target, uint32_t width, uint32_t height,
+                int tonemapping);
 
 
 

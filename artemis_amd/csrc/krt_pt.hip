@@ -20,7 +20,7 @@ struct PtArgs {
     float4 *hist;
     int *spill;
     unsigned long long *ctr;  // [0] closest, [1] shadow
-    double *acc64;            // optional FP64 running sum of samples (3 per pixel)
+    double *acc64;            // optional FP64 sums of samples, then of their squares (3 + 3 per pixel)
 };
 
 __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
@@ -33,10 +33,12 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     History hist{A.hist + gtid, nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const float count = (float)li.samples;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
-        const uint32_t px = pix % li.width, py = pix / li.width;
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
         f3 o, d;
         make_ray(li, px, py, seed, li.jitter != 0, o, d);
@@ -83,9 +85,14 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         float4 col = make_float4(value.x, value.y, value.z, 1.0f);
         if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
         else if (A.acc64) {  // converged-reference accumulator (FP64 only here)
-            A.acc64[3 * (size_t)pix] += (double)value.x;
-            A.acc64[3 * (size_t)pix + 1] += (double)value.y;
-            A.acc64[3 * (size_t)pix + 2] += (double)value.z;
+            double *sum = A.acc64 + 3 * (size_t)pix;
+            double *sq = A.acc64 + 3 * ((size_t)li.width * li.height + pix);  // per-pixel sample variance
+            const double v[3] = {(double)value.x, (double)value.y, (double)value.z};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                sum[c] += v[c];
+                sq[c] += v[c] * v[c];
+            }
         }
         const bool acc = li.accumulate != 0;
         li.color[pix] = accumulate4(li.color[pix], col, acc, count);

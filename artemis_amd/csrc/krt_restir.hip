@@ -81,6 +81,7 @@ struct Planes {
         p[3 * n + i].w = __int_as_float(0);
     }
     KRT_DEV void store_M(uint32_t i, int M) const { p[3 * n + i].w = __int_as_float(M); }
+    KRT_DEV int M(uint32_t i) const { return __float_as_int(p[3 * n + i].w); }
 };
 
 // direct_unoccluded (amadeus_common.cuh:380-397)
@@ -147,7 +148,9 @@ KRT_DEV SurfaceHit gbuffer_hit(const RestirArgs &A, uint32_t i) {
     return h;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
+// Four waves per SIMD: 128 VGPRs with a few dwords spilled to scratch runs
+// faster here than 139 VGPRs at three waves (7.5 vs 8.7 ms at 1080p).
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void initial_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
     const SceneView &sv = A.sv;
@@ -157,19 +160,21 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
     History hist{A.hist + gtid, nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const float count = (float)li.samples;
     const unsigned total = sv.nl;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
-        const uint32_t px = pix % li.width, py = pix / li.width;
-        Resv res = A.cur.load(pix);
-        res.w = 0.0f;
-        res.M = 0;
-        res.W = 0.0f;
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
         f3 o, d;
         make_ray(li, px, py, seed, false, o, d);
-        f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
+        // The reservoir and the level-0 AOVs are written as soon as they are
+        // final (level 0), so they hold no registers through the indirect
+        // path; a pixel whose primary ray misses writes them after the loop.
+        bool level0_done = false;
+        const bool acc = li.accumulate != 0;
         int nlev = 0;
         // radiance cache: the indirect path stops at level cv.depth (>= 1)
         const CacheView &cv = A.cv;
@@ -197,6 +202,10 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
             SurfaceHit sh{mat, sf.entering, n, -d, x};
             f3 direct = mk(0, 0, 0);
             if (level == 0) {
+                Resv res = A.cur.load(pix);
+                res.w = 0.0f;
+                res.M = 0;
+                res.W = 0.0f;
                 for (int i = 0; i < A.candidates; i++) {
                     // sample_direct (amadeus_common.cuh:301-377), triangle lights
                     RSample smp;
@@ -228,6 +237,7 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
                 res.resample(Li.x + Li.y + Li.z);
                 // the reference traces this same visibility ray a second time
                 res.W *= 1.0f - (occ ? 1.0f : 0.0f);
+                A.cur.store(pix, res);
                 A.gmat[pix] = sf.mat_id;
             } else {
                 direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);
@@ -240,9 +250,12 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
             f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
             hist.put(level, direct, true, T);  // value = direct + T * indirect
             if (level == 0) {
-                pos0 = x;
-                n0 = n;
-                alb0 = mat.diffuse;
+                li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
+                li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z,
+                                                                         0.0f),
+                                             acc, count);
+                li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
+                level0_done = true;
             }
             nlev = level + 1;
             if (!has_pdf) break;
@@ -285,11 +298,19 @@ __global__ __launch_bounds__(kTraceBlock) void initial_kernel(RestirArgs A) {
         if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
         A.intermediate[pix] = col;
         A.auxiliary[pix] = make_float4(seed.x, seed.y, seed.z, 1.0f);
-        const bool acc = li.accumulate != 0;
-        li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
-        li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
-        li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
-        A.cur.store(pix, res);
+        if (!level0_done) {
+            if (nlev == 0) {  // no reservoir written: primary miss or no surface
+                Resv res = A.cur.load(pix);
+                res.w = 0.0f;
+                res.M = 0;
+                res.W = 0.0f;
+                A.cur.store(pix, res);
+            }
+            const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
+            li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
+            li.position[pix] = zero;
+        }
     }
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
@@ -323,8 +344,11 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         Resv cr = A.cur.load(pix);
         if (cr.M <= 0) continue;
         SurfaceHit hit = gbuffer_hit(A, pix);
@@ -357,7 +381,9 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
     wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
+// Four waves per SIMD (128 VGPRs, no spills): the pass is latency-bound on
+// its eleven shadow walks per pixel, so waves in flight matter most.
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void spatial_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -365,10 +391,13 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
     TravStats ts;
     RayCount rc;
-    const uint32_t npix = li.width * li.height;
     const float count = (float)li.samples;
     const int S = A.neighbours;
-    for (uint32_t pix = gtid; pix < npix; pix += nthreads) {
+    const TileOrder tiles(li.width, li.height);
+    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
+        uint32_t px, py;
+        if (!tiles.pixel(item, px, py)) continue;
+        const uint32_t pix = py * li.width + px;
         SurfaceHit hit = gbuffer_hit(A, pix);
         Resv current = A.cur.load(pix);
         f3 direct = mk(0, 0, 0);
@@ -381,52 +410,52 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
             merged.seed = ld3(A.auxiliary[pix]);
             float t_current = target(current.data, hit);
             merged.update(current.data, t_current * current.W * (float)current.M);
-            uint32_t nidx[kMaxNeighbours];
-            int sizes[kMaxNeighbours];
+            // The 1/Z loop below replays the neighbour picks from this seed
+            // state (re-derived there from the auxiliary seed) instead of
+            // keeping indices and sizes of the neighbours in registers across
+            // the traversals; the cur planes are read-only in this pass, so
+            // the replay sees the same M.
             int cnt = 0;
             // Visibility of the selected sample from this pixel, when one of
             // the neighbour rays below already traced it (same origin,
             // direction and extent as the final shading ray).
             bool sel_known = false, sel_occ = false;
-#pragma unroll
-            for (int i = 0; i < kMaxNeighbours; i++) {
-                nidx[i] = pix;
-                sizes[i] = 0;
-                if (i < S) {
-                    uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
-                    Resv nb = A.cur.load(i0);
-                    // the reference traces this ray also for an empty
-                    // neighbour, whose target is then unused
-                    if (nb.M > 0) {
-                        f3 Dn = nb.data.point - hit.x;
-                        float dn = length(Dn);
-                        bool occ;
-                        f3 Ln = direct_occ(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
-                        float t_nb = Ln.x + Ln.y + Ln.z;
-                        if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
-                            sel_known = true;
-                            sel_occ = occ;
-                        }
+            for (int i = 0; i < S; i++) {
+                uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
+                Resv nb = A.cur.load(i0);
+                // the reference traces this ray also for an empty
+                // neighbour, whose target is then unused
+                if (nb.M > 0) {
+                    f3 Dn = nb.data.point - hit.x;
+                    float dn = length(Dn);
+                    bool occ;
+                    f3 Ln = direct_occ(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
+                    float t_nb = Ln.x + Ln.y + Ln.z;
+                    if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
+                        sel_known = true;
+                        sel_occ = occ;
                     }
-                    nidx[i] = i0;
-                    sizes[i] = nb.M;
-                    cnt += nb.M;
                 }
+                cnt += nb.M;
             }
             merged.M = current.M + cnt;
             int Z = current.M;
-#pragma unroll
-            for (int i = 0; i < kMaxNeighbours; i++) {
-                if (i < S && sizes[i] > 0) {  // `entering` carries neighbour validity
+            f3 rs = ld3(A.auxiliary[pix]);
+            rand_uniform(rs);  // the draw of the first merged.update
+            for (int i = 0; i < S; i++) {
+                const uint32_t i0 = spatial_neighbour_px(li, pix, rs, A.radius);
+                const int m = A.cur.M(i0);
+                if (m > 0) {  // `entering` carries neighbour validity
+                    rand_uniform(rs);  // the draw of merged.update above
                     SurfaceHit nh;
-                    nh.mat = load_material(A.sv.mats, A.gmat[nidx[i]]);
+                    nh.mat = load_material(A.sv.mats, A.gmat[i0]);
                     nh.entering = true;
-                    float4 nn = li.normal[nidx[i]], xx = li.position[nidx[i]];
+                    float4 nn = li.normal[i0], xx = li.position[i0];
                     nh.n = mk(nn.x, nn.y, nn.z);
                     nh.x = mk(xx.x, xx.y, xx.z);
                     nh.wo = hit.wo;
                     float t_nb = occluded_target(A.sv, merged.data, nh, stk, ts, rc);
-                    if (t_nb > 0.0f) Z += sizes[i];
+                    if (t_nb > 0.0f) Z += m;
                 }
             }
             float t_merged = target(merged.data, hit);
@@ -441,7 +470,8 @@ __global__ __launch_bounds__(kTraceBlock) void spatial_kernel(RestirArgs A) {
             } else {
                 lighting = direct_occ(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
             }
-            direct = hit.mat.emission + lighting * merged.W;
+            // emission re-read here rather than held in registers through the walks
+            direct = load_material(A.sv.mats, A.gmat[pix]).emission + lighting * merged.W;
             A.prev.store(pix, merged);
         } else {
             A.prev.reset(pix);

@@ -13,7 +13,9 @@ tracer whose paths stop at the first bounce and ask the online-trained
 radiance-cache MLP; 2^16 training paths per frame), restir_cache (ReSTIR DI
 with the cache for the indirect path), mlp (the cache network alone: 2^21
 queries + one training step on 2^16 samples per step, MFMA roofline).
---quality adds an equal-relMSE study at reduced resolution (see quality()).
+--quality adds the equal-relMSE study (quality()): path tracer, ReSTIR DI, and
+both with the radiance cache, compared at 1920x1080 against the path
+tracer's FP64 reference.
 
 Prints one JSON line: Msamples/s (pixels x frames / s), MRays/s, p50/p99
 frame latency, the roofline of the dominant kernel (HIP-event kernel times)
@@ -45,6 +47,14 @@ MLP_QUERY_BYTES = 13 * 4 + 3 * 4  # raw inputs in, RGB out
 # 28 B G-buffer): initial 64+28, temporal 2*64+28+64, spatial (1 pass, 5
 # neighbours) 6*64+6*28+64+32.
 RESTIR_PIXEL_BYTES = {0: 64 + 28, 1: 2 * 64 + 28 + 64, 2: 6 * 64 + 6 * 28 + 64 + 32}
+# ReSTIR GI (80 B reservoir = 5 float4 planes, 36 B G-buffer {x, depth}, {n},
+# material id): initial writes G-buffer + reservoir + direct + seed; temporal
+# reads both frames' G-buffer and reservoirs and writes one reservoir; a
+# spatial pass (5 neighbours) reads 6 G-buffers + 6 reservoirs and writes one;
+# the last pass also reads direct and reads/writes the colour.
+GI_SPATIAL_PASSES = 2
+GI_PIXEL_BYTES = {0: 36 + 80 + 16 + 16, 1: 2 * (36 + 80) + 80,
+                  2: GI_SPATIAL_PASSES * (6 * (36 + 80) + 80) + 16 + 32}
 PASS_NAMES = {0: "initial", 1: "temporal", 2: "spatial"}
 PT_PIXEL_BYTES = 112  # color/normal/albedo float4 read+write + position write
 
@@ -92,6 +102,14 @@ def cpu_baseline(workload, scene, W, H, seconds):
             frames += 1
             if time.perf_counter() - t0 > seconds or frames >= 64:
                 break
+    elif workload == "restir_gi":
+        gi = O.OracleGI(w, h)
+        while True:
+            gi.frame(osc, pan_camera(scene.camera, w, h, frames % 64), max_depth=3, m_cap=30,
+                     spatial_passes=GI_SPATIAL_PASSES, neighbours=5, radius=30, accumulate=False, nthreads=cores)
+            frames += 1
+            if time.perf_counter() - t0 > seconds or frames >= 64:
+                break
     else:
         bufs = None
         while True:
@@ -106,11 +124,37 @@ def cpu_baseline(workload, scene, W, H, seconds):
                       f"oracle bvh.cpp build {build_s:.2f} s"}
 
 
+def pan_camera(camera: dict, W: int, H: int, px: float):
+    """The scene camera turned right by `px` pixels (the pixel pitch of the
+    image plane at the look-at distance)."""
+    from artemis_amd import look_at
+
+    eye = np.asarray(camera["eye"], np.float64)
+    at = np.asarray(camera["lookat"], np.float64)
+    fov = camera.get("fov", 45.0)
+    w = at - eye
+    up = np.asarray(camera.get("up", (0.0, 1.0, 0.0)), np.float64)
+    right = np.cross(w, up)
+    right /= np.linalg.norm(right)
+    pitch = 2.0 * np.linalg.norm(w) * np.tan(np.radians(fov) / 2.0) / H
+    kw = {k: v for k, v in camera.items() if k not in ("eye", "lookat")}
+    return look_at(eye=eye, lookat=at + right * pitch * px, aspect=W / H, **kw)
+
+
 def rendering_setup(args, ctx, scene, sc, W, H, workload):
     from artemis_amd import ArmadaRTX, PathTracer, ReSTIR
 
     arm = ArmadaRTX(ctx, W, H)
-    if workload in ("restir_di", "restir_cache"):
+    if workload == "restir_gi":
+        from artemis_amd import ReSTIRGI
+
+        arm.attach("restir_gi", ReSTIRGI(ctx))
+        arm.set_option("m_cap", 30)
+        arm.set_option("spatial_passes", GI_SPATIAL_PASSES)
+        arm.set_option("neighbours", 5)
+        arm.set_option("radius", 30)
+        arm.set_depth(3)
+    elif workload in ("restir_di", "restir_cache"):
         arm.attach("restir", ReSTIR(ctx))
         arm.set_option("candidates", 32)
         arm.set_option("m_cap", 20)
@@ -293,33 +337,70 @@ def serve_workload(args, ctx):
     }
 
 
+def tonemap_timing(ctx, arm, W, H, reps=50):
+    """hdr_to_ldr (ACES) of the frame's colour buffer into a device RGBA8
+    image: HIP-event time per launch and its HBM roofline (16 B read + 4 B
+    written per pixel)."""
+    dst = ctx.device_array(shape=(H, W), dtype=np.uint32)
+    src = arm.device_buffers()["color"]
+    ctx.hdr_to_ldr(src, dst.ptr, W, H, 0)  # warm
+    ctx.timer_begin()
+    for _ in range(reps):
+        ctx.hdr_to_ldr(src, dst.ptr, W, H, 0)
+    ms = ctx.timer_end() / reps
+    gbs = 20.0 * W * H / (ms * 1e-3) / 1e9
+    return {"kernel": "hdr_to_ldr_kernel", "ms": ms, "bytes": 20 * W * H, "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic("tonemap:hdr_to_ldr")}
+
+
 def quality(args, ctx, scene, sc):
-    """Equal-relMSE study on the bench scene at reduced resolution.  Every
-    method converges to the same image (fixed -0.5 pixel offset, depth 5); the
-    judge is the path tracer's FP64-accumulated image.  For each method the
-    accumulated image's relMSE is recorded against cumulative GPU time; the
-    speedup is the path tracer's time over the method's time to reach the
-    relMSE the path tracer has after its last frame.  Cache methods train for
-    `warm` untimed frames first (the network persists across frames of a
-    static scene)."""
+    """Equal-relMSE study on the bench scene.
+
+    Every method converges to the same image (fixed -0.5 pixel offset, path
+    depth 5).  The judge is the path tracer's FP64-accumulated image of
+    N_ref = `quality_ref_spp` samples per pixel; the same run accumulates the
+    squares of the samples, which gives the per-pixel variance of ONE path
+    tracer sample and so its relMSE e_pt1 = mean(var / (ref^2 + eps)) over all
+    N_ref x W x H samples.  The path tracer's samples are independent, so its
+    relMSE after N samples per pixel is e_pt1 / N.  The error of a method is
+    the expected relMSE of ONE displayed frame in steady state (static
+    camera, after `quality_warm` frames that fill the temporal reservoirs and
+    train the radiance cache), averaged over `quality_frames` frames with
+    distinct seeds.  The path tracer needs N_eq = e_pt1 / e_m samples per
+    pixel to match a method's frame, so
+
+        speedup at equal relMSE = N_eq * t_pt / t_m,
+        equivalent Msamples/s   = N_eq * W * H / t_m,
+
+    with t the GPU time of one frame (sum of the frame's kernel times).  The
+    reference's own noise adds e_pt1 / N_ref to every measured relMSE in
+    expectation and is subtracted.  The frames' relMSE is heavy-tailed
+    (fireflies of next-event estimation at 1/R^2); `*_low` uses the per-frame
+    mean plus two standard errors."""
     from artemis_amd import ArmadaRTX, PathTracer, ReSTIR, look_at
-    from artemis_amd.metrics import rel_mse
 
     W, H = args.quality_width, args.quality_height
+    depth = 5
     cam = look_at(**scene.camera, aspect=W / H)
     ref_arm = ArmadaRTX(ctx, W, H)
     ref_arm.attach("pt", PathTracer(ctx))
-    ref_arm.set_depth(5)
+    ref_arm.set_depth(depth)
     ref_arm.set_option("pixel_jitter", False)
     ref_arm.set_f64_accumulation(True)
     t0 = time.perf_counter()
     for f in range(args.quality_ref_spp):
         ref_arm.render(sc, cam)
-        if f % 1024 == 1023:
+        if f % 512 == 511:
             log(f"[quality] reference {f + 1}/{args.quality_ref_spp} spp, {time.perf_counter() - t0:.1f} s")
     tot, nref = ref_arm.read_f64()
-    ref = tot / nref
+    sq = ref_arm.read_f64_squares()
     del ref_arm
+    ref = tot / nref
+    eps = 1e-2
+    den = ref * ref + eps
+    var1 = np.maximum(sq / nref - ref * ref, 0.0) * (nref / (nref - 1.0))
+    e_pt1 = float(np.mean(var1 / den))
+    del tot, sq, var1
 
     def make(kind):
         arm = ArmadaRTX(ctx, W, H)
@@ -333,48 +414,50 @@ def quality(args, ctx, scene, sc):
         else:
             arm.attach("m", PathTracer(ctx))
             arm.set_option("pixel_jitter", False)
-        arm.set_depth(5)
+        arm.set_depth(depth)
         if kind.endswith("cache"):
             arm.set_option("radiance_cache", True)
-            arm.set_option("train_samples", max(1024, W * H // 32))
+            arm.set_option("train_samples", 1 << 16)
         return arm
 
-    frames = args.quality_frames
-    checkpoints = sorted({2 ** k for k in range(20) if 2 ** k <= frames} | {frames})
-    curves = {}
-    for kind in ("pt", "pt_cache", "restir_di", "restir_cache"):
+    K = args.quality_frames
+    out = {"width": W, "height": H, "depth": depth, "reference_spp": int(nref), "frames": K,
+           "warm_frames": args.quality_warm, "relmse_eps": eps, "pt_relmse_1spp": e_pt1,
+           "pt_relmse_1spp_source": "per-pixel variance of the reference's samples", "methods": {}}
+    t_pt = None
+    for kind in ("pt", "restir_di", "pt_cache", "restir_cache"):
         arm = make(kind)
-        if kind.endswith("cache"):
-            for _ in range(args.quality_warm):
-                arm.render(sc, cam, accumulate=True)
-        ms, curve = 0.0, []
-        for f in range(frames):
-            arm.render(sc, cam, accumulate=f > 0)
-            ms += arm.stats()["gpu_ms"]
-            if f + 1 in checkpoints:
-                curve.append((f + 1, ms, rel_mse(arm.color_buffer(), ref)))
-        curves[kind] = curve
-        log(f"[quality] {kind}: " + ", ".join(f"{n}: {e:.4g} @ {t:.1f} ms" for n, t, e in curve))
-    target = curves["pt"][-1][2]
-
-    def time_to(curve):
-        prev = None
-        for n, t, e in curve:
-            if e <= target:
-                if prev is None:
-                    return t
-                (n0, t0_, e0) = prev  # log-log interpolation between checkpoints
-                a = np.log(target / e0) / np.log(e / e0) if e0 != e else 1.0
-                return float(np.exp(np.log(t0_) + a * (np.log(t) - np.log(t0_))))
-            prev = (n, t, e)
-        return None
-
-    t_pt = time_to(curves["pt"])
-    out = {"width": W, "height": H, "reference_spp": int(nref), "target_relmse": target,
-           "cache_warm_frames": args.quality_warm, "curves": curves, "speedup_vs_pt": {}}
-    for kind, curve in curves.items():
-        t = time_to(curve)
-        out["speedup_vs_pt"][kind] = (t_pt / t) if t else None
+        seed = 1
+        for _ in range(0 if kind == "pt" else args.quality_warm):
+            arm.set_option("seed", seed)
+            seed += 1
+            arm.render(sc, cam, accumulate=False)
+        errs, ms = [], []
+        for _ in range(K):
+            arm.set_option("seed", seed)
+            seed += 1
+            arm.render(sc, cam, accumulate=False)
+            ms.append(arm.stats()["gpu_ms"])
+            d = arm.color_buffer()[..., :3].astype(np.float64) - ref
+            errs.append(float(np.mean(d * d / den)))
+        del arm
+        errs = np.array(errs)
+        t_m = float(np.mean(ms))
+        if kind == "pt":
+            t_pt = t_m
+        e_raw = float(errs.mean())
+        se = float(errs.std(ddof=1) / np.sqrt(K)) if K > 1 else 0.0
+        e_m = max(e_raw - e_pt1 / nref, 1e-12)
+        n_eq = e_pt1 / e_m
+        n_eq_low = e_pt1 / (e_m + 2.0 * se)
+        out["methods"][kind] = {
+            "relmse_per_frame": e_m, "relmse_stderr": se, "frame_ms": t_m,
+            "relmse_frame_quantiles": [float(q) for q in np.quantile(errs, [0.0, 0.25, 0.5, 0.75, 1.0])],
+            "pt_spp_equivalent": n_eq, "speedup_vs_pt": n_eq * t_pt / t_m,
+            "speedup_vs_pt_low": n_eq_low * t_pt / t_m,
+            "equivalent_msamples_per_s": n_eq * W * H / (t_m * 1e-3) / 1e6}
+        log(f"[quality] {kind}: relMSE/frame {e_m:.4g} +- {se:.2g} (median frame {np.median(errs):.3g}), "
+            f"{t_m:.2f} ms/frame, speedup at equal relMSE {n_eq * t_pt / t_m:.2f} (low {n_eq_low * t_pt / t_m:.2f})")
     return out
 
 
@@ -383,7 +466,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "mlp", "serve"],
+    ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "restir_gi", "mlp",
+                                           "serve"],
                     default="restir_di")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -397,11 +481,11 @@ def main():
                     help="traversal tree (results are identical; 'reference' collapses the bvh.cpp tree)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quality", action="store_true", help="add the equal-relMSE study")
-    ap.add_argument("--quality-width", type=int, default=320)
-    ap.add_argument("--quality-height", type=int, default=180)
-    ap.add_argument("--quality-ref-spp", type=int, default=8192)
-    ap.add_argument("--quality-frames", type=int, default=256)
-    ap.add_argument("--quality-warm", type=int, default=200)
+    ap.add_argument("--quality-width", type=int, default=1920)
+    ap.add_argument("--quality-height", type=int, default=1080)
+    ap.add_argument("--quality-ref-spp", type=int, default=1024)
+    ap.add_argument("--quality-frames", type=int, default=32)
+    ap.add_argument("--quality-warm", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -419,18 +503,26 @@ def main():
         return
     W, H = args.width, args.height
     log(f"[bench] generating scene")
-    scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
+    if args.workload == "restir_gi":  # 256 x sphere(64, 65) = 2,097,152 triangles
+        scene = scenes.spheres(instances=256, slices=64, stacks=65, emissive=4096)
+    else:
+        scene = scenes.spheres(instances=256, slices=64, stacks=33, emissive=4096)
     log(f"[bench] uploading {scene.triangles} triangles + GPU BVH build")
     sc = ctx.upload(scene, layout=args.layout)
     bst = sc.stats()
     log(f"[bench] bvh: {bst}")
     cam = look_at(**scene.camera, aspect=W / H)
     wl = args.workload
+    # ReSTIR GI: the camera pans by one pixel per frame over a 64-frame cycle;
+    # every frame is then a new view (no accumulation) and the temporal pass
+    # reprojects.  The other workloads keep a static camera and accumulate.
+    moving = wl == "restir_gi"
+    cams = [pan_camera(scene.camera, W, H, f % 64) for f in range(64)] if moving else [cam]
     arm = rendering_setup(args, ctx, scene, sc, W, H, wl)
     restir = wl.startswith("restir")
     bname = {"restir_di": "restir_di_1080p_spheres1M"}.get(wl, "")
     for i in range(args.warmup):
-        arm.render(sc, cam, accumulate=True)
+        arm.render(sc, cams[i % len(cams)], accumulate=not moving)
         log(f"[bench] warmup {i + 1}/{args.warmup}: {arm.stats()['gpu_ms']:.2f} ms")
     arm.sync()
     lat, rays, kms, prays, nodes, tris = [], 0, np.zeros(8), np.zeros(4), 0, 0
@@ -438,7 +530,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         f0 = time.perf_counter()
-        arm.render(sc, cam, accumulate=True)  # returns after the frame completed on the GPU
+        arm.render(sc, cams[(args.warmup + i) % len(cams)], accumulate=not moving)  # returns when complete
         lat.append(time.perf_counter() - f0)
         st = arm.stats()
         rays += st["closest_rays"] + st["shadow_rays"]
@@ -464,7 +556,8 @@ def main():
     # reported beside it as ref_* where it has been computed.
     pn = pnodes / K
     pt = ptris / K
-    pix_bytes = (RESTIR_PIXEL_BYTES[dom] if restir else PT_PIXEL_BYTES) * npix
+    pix_table = GI_PIXEL_BYTES if wl == "restir_gi" else RESTIR_PIXEL_BYTES
+    pix_bytes = (pix_table[dom] if restir else PT_PIXEL_BYTES) * npix
     node_b = float(bst.get("node_bytes", 64))
     alg_bytes = pix_bytes + node_b * pn[dom] + 48.0 * pt[dom]
     sec = kavg[dom] * 1e-3
@@ -483,10 +576,13 @@ def main():
         ref_bytes = RESTIR_PIXEL_BYTES[dom] * npix + prays[dom] / K * br["b_ray"]
         roof["ref_bytes_per_ray"] = br["b_ray"]
         roof["ref_achieved"] = ref_bytes / sec / 1e9
-    names = {"restir_di": PASS_NAMES, "restir_cache": {0: "initial", 1: "temporal", 2: "spatial",
+    names = {"restir_di": PASS_NAMES, "restir_gi": PASS_NAMES,
+             "restir_cache": {0: "initial", 1: "temporal", 2: "spatial",
                                                        3: "cache_infer", 4: "cache_train"},
              "pt": {0: "path"}, "pt_cache": {0: "path", 1: "cache_infer", 2: "composite", 3: "cache_train"}}[wl]
     desc = {"restir_di": "32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), depth 2",
+            "restir_gi": "1 secondary path of depth 3 per pixel, temporal M-cap 30 + 2 spatial passes "
+                         "(5 nbrs, 30 px) with Jacobian reweighting, camera pans 1 px/frame over 64 frames",
             "restir_cache": "32 candidates, M-cap 20, 1 spatial pass (5 nbrs, 30 px), indirect path stops at the "
                             "first bounce and asks the radiance cache, 2^16 training paths per frame",
             "pt": "1 spp/frame, max depth 5",
@@ -525,11 +621,13 @@ def main():
                        "train_samples": 1 << 16, "train_step_ms": tr_ms,
                        "train_tflops": MLP_TRAIN_FLOP * (1 << 16) / (tr_ms * 1e-3) / 1e12 if tr_ms > 0 else None,
                        "peak_tflops": MFMA_BF16_PEAK_TFLOPS}
-    if args.quality:
-        line["quality"] = quality(args, ctx, scene, sc)
+    line["tonemap"] = tonemap_timing(ctx, arm, W, H)
+    if args.quality and wl == "restir_di":
+        line["equal_relmse"] = quality(args, ctx, scene, sc)
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
-        line["cpu_baseline"] = cpu_baseline("restir_di" if restir else "pt", scene, W, H, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(wl if wl in ("restir_di", "restir_gi") else "pt", scene, W, H,
+                                            args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

@@ -202,9 +202,11 @@ int krt_armada_upload(krt_armada *a, int which, const float *in);
 int krt_armada_stats(krt_armada *a, krt_frame_stats *out);
 /* Converged-reference accumulator (no reference counterpart; the reference
  * accumulates only the FP32 running mean of accumulate(), amadeus_path_tracer.cu:
- * 59-70): while enabled, every path-tracer sample is also added to an FP64 sum
- * per pixel.  Enabling (again) clears the sums.  read: 3 doubles per pixel and
- * the number of frames summed. */
+ * 59-70): while enabled, every path-tracer sample and its square are also
+ * added to FP64 sums per pixel.  Enabling (again) clears the sums.  read: 3
+ * doubles per pixel and the number of frames summed; read_f64_squares: the 3
+ * sums of squares per pixel (the per-pixel sample variance of one frame is
+ * squares/n - (sums/n)^2). */
 int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
 /* ReSTIR reservoirs (include/amadeus/reservoir.cuh:19-83) as 16 floats per
  * pixel {Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz}; for
@@ -213,7 +215,24 @@ int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
  * spatial output), 1: the other one. */
 int krt_armada_read_reservoirs(krt_armada *a, int which, float *out);
 int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames);
+int krt_armada_read_f64_squares(krt_armada *a, double *out);
 int krt_armada_sync(krt_armada *a);
+
+/* ---------------------------------------------------------- tonemapping --
+ * hdr_to_ldr (include/cuda/color.cuh:82-121, called by bench.cu:205 and
+ * motion_capture.cuh:358 on the colour buffer): float4 HDR pixels ->
+ * RGBA8 packed in a uint32 as r | g << 8 | b << 16 | 255 << 24
+ * (include/cuda/cast.cuh:16-20), rows flipped vertically (color.cuh:92-93),
+ * through ACES (color.cuh:35-41) or sRGB (color.cuh:14-31), each channel
+ * quantized by quantize_unsigned_8_bits (color.cuh:43-56).  Device pointers
+ * (width x height float4 in, width x height uint32 out); enqueued on the
+ * context stream without synchronisation, as the reference's launch. */
+enum { KRT_TONEMAP_ACES = 0, KRT_TONEMAP_SRGB = 1 };
+int krt_hdr_to_ldr(krt_context *ctx, const void *pixels_dev, void *target_dev, uint32_t width, uint32_t height,
+                   int tonemapping);
+/* The armada's colour buffer through krt_hdr_to_ldr into a host array of
+ * W*H uint32; returns when the copy is complete. */
+int krt_armada_read_ldr(krt_armada *a, int tonemapping, uint32_t *out);
 
 /* Plugin entry of source/amadeus/naive_path_tracer.cu:280-344
  * (`extern "C" ret load_attachment()`), by kind. */

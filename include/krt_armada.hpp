@@ -159,6 +159,14 @@ public:
     }
 };
 
+// kobra::cuda::hdr_to_ldr (include/cuda/color.cuh:104-118): same arguments,
+// plus the context whose stream the launch is queued on.
+enum { eTonemappingACES = KRT_TONEMAP_ACES, eTonemappingSRGB = KRT_TONEMAP_SRGB };
+inline void hdr_to_ldr(Context &ctx, const void *pixels, uint32_t *target, int width, int height,
+                       int tonemapping = eTonemappingACES) {
+    check(krt_hdr_to_ldr(ctx.get(), pixels, target, (uint32_t)width, (uint32_t)height, tonemapping));
+}
+
 }  // namespace krt_cpp
 
 #endif

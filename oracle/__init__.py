@@ -74,6 +74,7 @@ _SIGS = {
     "orc_mlp_loss": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, dp]),
     "orc_mlp_grad": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, C.c_int, C.c_float, dp]),
     "orc_mlp_adam": (None, [fp, fp, fp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp]),
+    "orc_hdr_to_ldr": (None, [fp, u32p, C.c_int, C.c_int, C.c_int]),
 }
 
 _lib = None
@@ -128,6 +129,15 @@ def sincos_turn(u: np.ndarray):
     s = np.empty_like(u)
     lib().orc_sincos_turn(_f(u), _f(c), _f(s), len(u))
     return c, s
+
+
+def hdr_to_ldr(pixels: np.ndarray, tonemapping: int = 0) -> np.ndarray:
+    """(H, W, 4) float32 -> (H, W) uint32 RGBA8, rows flipped (color.cuh:82-121)."""
+    p = np.ascontiguousarray(pixels, np.float32)
+    H, W = p.shape[:2]
+    out = np.empty((H, W), np.uint32)
+    lib().orc_hdr_to_ldr(_f(p), _u(out), W, H, tonemapping)
+    return out
 
 
 def tri_boxes(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:

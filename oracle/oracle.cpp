@@ -1620,3 +1620,45 @@ void orc_camera_rays(const orc_camera *cam, const orc_params *p, int jitter, flo
             r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = 0.0f;
         }
 }
+
+// ---------------------------------------------------------------------------
+// HDR -> LDR — include/cuda/color.cuh:14-121, include/cuda/cast.cuh:16-20
+// ---------------------------------------------------------------------------
+// clamp (include/cuda/math.cuh:101-104): fmaxf(a, fminf(f, b)); a NaN input
+// therefore clamps to b.
+static float ldr_clamp(float f, float a, float b) { return std::fmax(a, std::fmin(f, b)); }
+
+// aces (color.cuh:35-41), per channel: a = c (2.51 c + 0.03),
+// b = c (2.43 c + 0.59) + 0.14, clamp(a / b, 0, 1)
+static float ldr_aces(float c) {
+    float a = c * (2.51f * c + 0.03f);
+    float b = c * (2.43f * c + 0.59f) + 0.14f;
+    return ldr_clamp(a / b, 0.0f, 1.0f);
+}
+
+// srgb (color.cuh:14-31): 12.92 c below 0.0031308, else 1.055 c^(1/2.4) - 0.055
+static float ldr_srgb(float c) {
+    const float inv_gamma = 1.0f / 2.4f;
+    float powed = std::pow(c, inv_gamma);
+    float s = c < 0.0031308f ? 12.92f * c : 1.055f * powed - 0.055f;
+    return ldr_clamp(s, 0.0f, 1.0f);
+}
+
+// quantize_unsigned_8_bits (color.cuh:43-56): min((uint)(x * 256), 255)
+static uint32_t ldr_quantize(float x) {
+    x = ldr_clamp(x, 0.0f, 1.0f);
+    uint32_t q = (uint32_t)(x * 256.0f);
+    return q < 255u ? q : 255u;
+}
+
+// hdr_to_ldr_kernel (color.cuh:82-100): out row = height - y - 1; to_ui32
+// packs x | y << 8 | z << 16 | w << 24 with w = 255 (make_color, :58-67).
+void orc_hdr_to_ldr(const float *pixels, uint32_t *target, int width, int height, int tonemapping) {
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            const float *p = pixels + 4 * ((size_t)y * width + x);
+            uint32_t ch[3];
+            for (int k = 0; k < 3; k++) ch[k] = ldr_quantize(tonemapping == 0 ? ldr_aces(p[k]) : ldr_srgb(p[k]));
+            target[(size_t)(height - y - 1) * width + x] = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (255u << 24);
+        }
+}

@@ -157,6 +157,10 @@ void orc_mlp_adam(float *params, float *m, float *v, const float *grad, int n, f
 /* reconnection-shift Jacobian d(omega_r)/d(omega_q) (0 when rejected) */
 float orc_gi_jacobian(const float *xq, const float *xr, const float *xs, const float *ns);
 
+/* hdr_to_ldr (include/cuda/color.cuh:82-121): float4 pixels -> packed RGBA8,
+ * rows flipped; tonemapping 0 = ACES (:35-41), 1 = sRGB (:14-31). */
+void orc_hdr_to_ldr(const float *pixels, uint32_t *target, int width, int height, int tonemapping);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,8 @@ def test_cpp_drop_in_layer_compiles_and_links(tmp_path):
                 a.set_depth(5);
                 krt_cpp::Camera cam{};
                 a.render(h, cam, true);
+                uint32_t *ldr = nullptr;
+                krt_cpp::hdr_to_ldr(ctx, a.color_buffer(), ldr, 64, 64, krt_cpp::eTonemappingSRGB);
                 krt_attachment_ret r = krt_load_attachment(ctx.get(), "restir");
                 std::printf("%s\\n", r.name);
             }

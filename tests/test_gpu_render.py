@@ -117,3 +117,37 @@ def test_restir_spheres_spot(krt_ctx):
     print(f"  color MAD={m:.3e} exact={exact_fraction(arm.color_buffer(), ors.buffers[0]):.4f}")
     assert m <= MAD_TOL
     sc.free()
+
+
+def test_f64_accumulator_sums_and_squares(krt_ctx, cornell_pair):
+    """The FP64 accumulator holds sum v and sum v^2 of the per-frame samples;
+    the samples are the oracle's bit for bit, so the sums agree to FP64
+    rounding of the addition order."""
+    s, sc, osc = cornell_pair
+    W = H = 32
+    frames = 6
+    cam = look_at(**s.camera, aspect=W / H)
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("pt", PathTracer(krt_ctx))
+    arm.set_depth(5)
+    arm.set_f64_accumulation(True)
+    # one sample per frame under a new seed, not accumulated: the colour
+    # buffer of each oracle frame is then the sample itself
+    for f in range(frames):
+        arm.set_option("seed", f + 1)
+        arm.render(sc, cam, accumulate=False)
+    tot, n = arm.read_f64()
+    sq = arm.read_f64_squares()
+    assert n == frames
+    want_s = np.zeros((H, W, 3))
+    want_q = np.zeros((H, W, 3))
+    for f in range(frames):
+        bufs, _ = osc.render_pt(cam, W, H, frames=1, max_depth=5, user_seed=f + 1)
+        v = bufs[0][..., :3].astype(np.float64)
+        ok = ~np.isnan(v).any(axis=-1, keepdims=True)
+        want_s += np.where(ok, v, 0.0)
+        want_q += np.where(ok, v * v, 0.0)
+    np.testing.assert_allclose(tot, want_s, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(sq, want_q, rtol=1e-12, atol=1e-12)
+    var = sq / n - (tot / n) ** 2
+    assert var.min() > -1e-9 and var.max() > 0
