@@ -96,7 +96,8 @@ class Context:
                                           _f(out)))
         return out
 
-    def upload(self, scene: SceneData, leaf_max: int = 4, build: bool = True, layout: str = "sah") -> "Scene":
+    def upload(self, scene, leaf_max: int = 4, build: bool = True, layout: str = "sah") -> "Scene":
+        """A SceneData (world-space soup) or an InstancedSceneData (two-level)."""
         return Scene(self, scene, leaf_max=leaf_max, build=build, layout=layout)
 
 
@@ -149,6 +150,13 @@ class Scene:
         self.ctx = ctx
         self.data = data
         lib = ctx._lib
+        self.instanced = hasattr(data, "inst_mesh")
+        if self.instanced:
+            self._upload_instanced(data)
+            ctx._adopt(self)
+            if build:
+                self.build(leaf_max, layout)
+            return
         self._keep = [np.ascontiguousarray(data.positions, np.float32),
                       np.ascontiguousarray(data.normals, np.float32),
                       np.ascontiguousarray(data.indices, np.uint32),
@@ -163,6 +171,41 @@ class Scene:
         ctx._adopt(self)
         if build:
             self.build(leaf_max, layout)
+
+    def _upload_instanced(self, d) -> None:
+        """krt_scene_upload_instanced (Accelerator::update, accelerator.cuh:145-190)."""
+        u32 = lambda a: a.ctypes.data_as(L.u32p)
+        pos = np.ascontiguousarray(np.concatenate([m[0] for m in d.meshes]), np.float32)
+        nrm = np.ascontiguousarray(np.concatenate([m[1] for m in d.meshes]), np.float32)
+        idx = np.ascontiguousarray(np.concatenate([m[2] for m in d.meshes]), np.uint32)
+        nv = np.array([len(m[0]) for m in d.meshes], np.uint32)
+        nt = np.array([len(m[2]) for m in d.meshes], np.uint32)
+        im = np.ascontiguousarray(d.inst_mesh, np.uint32)
+        xf = np.ascontiguousarray(d.transforms, np.float32).reshape(-1, 12)
+        tm = np.ascontiguousarray(d.tri_material, np.uint32)
+        mats = np.ascontiguousarray(d.materials, np.float32)
+        self._keep = [pos, nrm, idx, nv, nt, im, xf, tm, mats]
+        desc = L.InstancedDesc(_f(pos), _f(nrm), u32(nv), u32(idx), u32(nt), len(nv), u32(im), _f(xf), len(im),
+                               u32(tm), _f(mats), len(mats))
+        h = L.handle()
+        L.check(self.ctx._lib.krt_scene_upload_instanced(self.ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h.value
+
+    def mesh_serialized(self, mesh: int) -> np.ndarray:
+        """Instanced scenes: the bvh.cpp tree of one base mesh (12 floats per node)."""
+        n = C.c_uint32()
+        L.check(self.ctx._lib.krt_accel_mesh_serialized(self.ctx.h, self.h, mesh, None, 0, C.byref(n)))
+        out = np.empty((n.value, 12), np.float32)
+        L.check(self.ctx._lib.krt_accel_mesh_serialized(self.ctx.h, self.h, mesh, _f(out), out.size, C.byref(n)))
+        return out
+
+    def instance_tree(self) -> np.ndarray:
+        """Instanced scenes: the bvh.cpp tree over the instances' world boxes."""
+        n = C.c_uint32()
+        L.check(self.ctx._lib.krt_accel_instance_tree(self.ctx.h, self.h, None, 0, C.byref(n)))
+        out = np.empty((n.value, 12), np.float32)
+        L.check(self.ctx._lib.krt_accel_instance_tree(self.ctx.h, self.h, _f(out), out.size, C.byref(n)))
+        return out
 
     def build(self, leaf_max: int = 4, layout: str = "sah") -> None:
         """Accelerator::update + build_tlas: GPU build of the bvh.cpp tree plus

@@ -22,6 +22,16 @@ vp = C.c_void_p
 handle = C.c_uint64
 
 
+class InstancedDesc(C.Structure):
+    _fields_ = [
+        ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+        ("mesh_nv", C.POINTER(C.c_uint32)), ("indices", C.POINTER(C.c_uint32)),
+        ("mesh_nt", C.POINTER(C.c_uint32)), ("nmesh", C.c_uint32),
+        ("instance_mesh", C.POINTER(C.c_uint32)), ("transforms", C.POINTER(C.c_float)), ("ninst", C.c_uint32),
+        ("tri_material", C.POINTER(C.c_uint32)), ("materials", C.POINTER(C.c_float)), ("nm", C.c_uint32),
+    ]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("positions", fp), ("normals", fp), ("nv", C.c_uint32),
@@ -108,6 +118,9 @@ _SIGS = {
     "krt_accel_build_layout": (C.c_int, [vp, handle, C.c_int, C.c_int]),
     "krt_accel_stats": (C.c_int, [vp, handle, C.POINTER(BvhStats)]),
     "krt_accel_serialized": (C.c_int, [vp, handle, fp, C.c_size_t]),
+    "krt_scene_upload_instanced": (C.c_int, [vp, C.POINTER(InstancedDesc), C.POINTER(handle)]),
+    "krt_accel_mesh_serialized": (C.c_int, [vp, handle, C.c_uint32, fp, C.c_size_t, u32p]),
+    "krt_accel_instance_tree": (C.c_int, [vp, handle, fp, C.c_size_t, u32p]),
     "krt_accel_leaf_order": (C.c_int, [vp, handle, i32p, C.c_size_t]),
     "krt_trace_closest": (C.c_int, [vp, handle, fp, C.c_uint32, fp]),
     "krt_trace_occluded": (C.c_int, [vp, handle, fp, C.c_uint32, i32p]),

@@ -295,6 +295,7 @@ void cache_train(Context &ctx, CacheState &cs, const CacheView &cv) {
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
                      DevBuf &history, unsigned long long *ctr, PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+    KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "the path tracer's radiance cache renders flat scenes (krt_scene_upload)");
     KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
     const uint32_t npix = li.width * li.height;
     const CacheView cv = cache_begin(ctx, s, opt, cs, npix, li.user_seed);

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,7 +28,7 @@ SceneView make_view(const Scene &s) {
     return SceneView{s.pos.as<float>(),   s.nrm.as<float>(),    s.idx.as<uint32_t>(),
                      s.tri_mat.as<uint32_t>(), s.mats.as<float4>(), s.lights.as<float4>(),
                      s.nl,                s.accel.tri.as<float4>(), s.accel.nodes.as<float4>(),
-                     s.accel.wide.as<float4>()};
+                     s.accel.wide.as<float4>(), s.tlas.wide.as<float4>(), s.inst_rec.as<float4>()};
 }
 
 }  // namespace krt
@@ -203,6 +204,38 @@ static void upload(Context &C, DevBuf &b, const void *src, size_t bytes) {
     if (bytes) KRT_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, C.stream));
 }
 
+// Material rows: {diffuse, rough}, {specular, refr}, {emission, type}, {0}.
+static std::vector<float> material_rows(const float *materials, uint32_t nm) {
+    std::vector<float> mt(16 * (size_t)nm, 0.0f);
+    for (uint32_t m = 0; m < nm; m++) {
+        const float *q = materials + 12 * m;
+        float *o = &mt[16 * m];
+        o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[9];
+        o[4] = q[3]; o[5] = q[4]; o[6] = q[5]; o[7] = q[10];
+        o[8] = q[6]; o[9] = q[7]; o[10] = q[8];
+        int type = (int)q[11];
+        std::memcpy(&o[11], &type, 4);
+    }
+    return mt;
+}
+
+static bool emissive(const float *e) { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.0f; }
+
+// Triangle light row (armada.cu:117-140) with area and normal precomputed by
+// the per-sample expressions of lighting.cuh:102-110.
+static void light_row(const float *a, const float *b, const float *c, const float *e, std::vector<float> &lt) {
+    float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+    float dd = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+    float area = std::sqrt(dd) * 0.5f;
+    float inv = 1.0f / std::sqrt(dd);
+    float nrm[3] = {cr[0] * inv, cr[1] * inv, cr[2] * inv};
+    float row[16] = {a[0], a[1], a[2], area, ab[0], ab[1], ab[2], nrm[0],
+                     ac[0], ac[1], ac[2], nrm[1], e[0], e[1], e[2], nrm[2]};
+    lt.insert(lt.end(), row, row + 16);
+}
+
 int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scene) {
     return guard([&] {
         KRT_REQUIRE(ctx && d && scene, KRT_ERR_INVALID, "null argument");
@@ -229,37 +262,15 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scen
         upload(C, s->nrm, d->normals, sizeof(float) * 3 * d->nv);
         upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * d->nt);
         upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * d->nt);
-        // material rows: {diffuse, rough}, {specular, refr}, {emission, type}, {0}
-        std::vector<float> mt(16 * (size_t)d->nm, 0.0f);
-        for (uint32_t m = 0; m < d->nm; m++) {
-            const float *q = d->materials + 12 * m;
-            float *o = &mt[16 * m];
-            o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[9];
-            o[4] = q[3]; o[5] = q[4]; o[6] = q[5]; o[7] = q[10];
-            o[8] = q[6]; o[9] = q[7]; o[10] = q[8];
-            int type = (int)q[11];
-            std::memcpy(&o[11], &type, 4);
-        }
+        std::vector<float> mt = material_rows(d->materials, d->nm);
         upload(C, s->mats, mt.data(), sizeof(float) * mt.size());
-        // triangle lights (armada.cu:117-140, emissive test :240), area and
-        // normal precomputed with the per-sample expressions of lighting.cuh:102-110
+        // triangle lights in triangle order (armada.cu:117-140, emissive test :240)
         std::vector<float> lt;
         for (uint32_t t = 0; t < d->nt; t++) {
             const float *e = d->materials + 12 * d->tri_material[t] + 6;
-            if (!(std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.0f)) continue;
-            const float *a = d->positions + 3 * d->indices[3 * t];
-            const float *b = d->positions + 3 * d->indices[3 * t + 1];
-            const float *c = d->positions + 3 * d->indices[3 * t + 2];
-            float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-            float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-            float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
-            float dd = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
-            float area = std::sqrt(dd) * 0.5f;
-            float inv = 1.0f / std::sqrt(dd);
-            float nrm[3] = {cr[0] * inv, cr[1] * inv, cr[2] * inv};
-            float row[16] = {a[0], a[1], a[2], area, ab[0], ab[1], ab[2], nrm[0],
-                             ac[0], ac[1], ac[2], nrm[1], e[0], e[1], e[2], nrm[2]};
-            lt.insert(lt.end(), row, row + 16);
+            if (!emissive(e)) continue;
+            light_row(d->positions + 3 * d->indices[3 * t], d->positions + 3 * d->indices[3 * t + 1],
+                      d->positions + 3 * d->indices[3 * t + 2], e, lt);
         }
         s->nl = (uint32_t)(lt.size() / 16);
         upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
@@ -267,6 +278,132 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scen
         krt_handle h = C.next_handle++;
         C.scenes[h] = std::move(s);
         *scene = h;
+    });
+}
+
+int krt_scene_upload_instanced(krt_context *ctx, const krt_instanced_desc *d, krt_handle *scene) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && d && scene, KRT_ERR_INVALID, "null argument");
+        KRT_REQUIRE(d->nmesh > 0 && d->ninst > 0 && d->nm > 0, KRT_ERR_INVALID, "empty instanced scene");
+        KRT_REQUIRE(d->positions && d->normals && d->mesh_nv && d->indices && d->mesh_nt && d->instance_mesh &&
+                        d->transforms && d->tri_material && d->materials,
+                    KRT_ERR_INVALID, "null scene array");
+        Context &C = ctx->impl;
+        KRT_HIP(hipSetDevice(C.device));
+        auto s = std::make_unique<Scene>();
+        s->instanced = true;
+        s->nmesh = d->nmesh;
+        s->ninst = d->ninst;
+        s->nm = d->nm;
+        uint64_t nv = 0, ntm = 0;
+        for (uint32_t m = 0; m < d->nmesh; m++) {
+            KRT_REQUIRE(d->mesh_nv[m] > 0 && d->mesh_nt[m] > 0, KRT_ERR_INVALID, "empty mesh");
+            s->mesh_vbase.push_back((uint32_t)nv);
+            s->mesh_tbase.push_back((uint32_t)ntm);
+            s->mesh_nv.push_back(d->mesh_nv[m]);
+            s->mesh_nt.push_back(d->mesh_nt[m]);
+            for (uint64_t i = 3 * ntm; i < 3 * (ntm + d->mesh_nt[m]); i++)
+                KRT_REQUIRE(d->indices[i] < d->mesh_nv[m], KRT_ERR_INVALID, "vertex index out of range");
+            nv += d->mesh_nv[m];
+            ntm += d->mesh_nt[m];
+            KRT_REQUIRE(d->mesh_nt[m] < (1u << 26), KRT_ERR_INVALID, "too many triangles in a mesh");
+        }
+        KRT_REQUIRE(nv < (1ull << 31) && ntm < (1ull << 31), KRT_ERR_INVALID, "meshes too large");
+        uint64_t nt = 0;
+        for (uint32_t i = 0; i < d->ninst; i++) {
+            KRT_REQUIRE(d->instance_mesh[i] < d->nmesh, KRT_ERR_INVALID, "instance mesh out of range");
+            for (int k = 0; k < 12; k++)
+                KRT_REQUIRE(std::isfinite(d->transforms[12 * (size_t)i + k]), KRT_ERR_INVALID,
+                            "instance transform has non-finite entries");
+            float inv[12];
+            affine_inverse(d->transforms + 12 * (size_t)i, inv);  // rejects singular transforms
+            s->inst_mesh.push_back((int32_t)d->instance_mesh[i]);
+            nt += d->mesh_nt[d->instance_mesh[i]];
+        }
+        KRT_REQUIRE(nt < (1ull << 31), KRT_ERR_INVALID, "too many instanced triangles");
+        KRT_REQUIRE(d->ninst < (1u << 26), KRT_ERR_INVALID, "too many instances");
+        for (uint64_t t = 0; t < nt; t++)
+            KRT_REQUIRE(d->tri_material[t] < d->nm, KRT_ERR_INVALID, "material index out of range");
+        s->xform.assign(d->transforms, d->transforms + 12 * (size_t)d->ninst);
+        s->nv = (uint32_t)nv;
+        s->nt = (uint32_t)nt;
+        upload(C, s->pos, d->positions, sizeof(float) * 3 * nv);
+        upload(C, s->nrm, d->normals, sizeof(float) * 3 * nv);
+        upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * ntm);
+        upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * nt);
+        std::vector<float> mt = material_rows(d->materials, d->nm);
+        upload(C, s->mats, mt.data(), sizeof(float) * mt.size());
+        // world-space triangle lights, instance order then triangle order, the
+        // vertices through the instance transform (Transform::apply,
+        // armada.cu:131-133); vertex bounding box of the world-space scene
+        std::vector<float> lt;
+        for (int a = 0; a < 3; a++) {
+            s->bmin[a] = FLT_MAX;
+            s->bmax[a] = -FLT_MAX;
+        }
+        uint64_t g = 0;
+        for (uint32_t i = 0; i < d->ninst; i++) {
+            const uint32_t m = d->instance_mesh[i];
+            const float *M = d->transforms + 12 * (size_t)i;
+            const float *P = d->positions + 3 * (size_t)s->mesh_vbase[m];
+            const uint32_t *I = d->indices + 3 * (size_t)s->mesh_tbase[m];
+            auto xf = [&](uint32_t v, float *w) {
+                const float *p = P + 3 * (size_t)v;
+                for (int r = 0; r < 3; r++)
+                    w[r] = (M[4 * r] * p[0] + M[4 * r + 1] * p[1]) + (M[4 * r + 2] * p[2] + M[4 * r + 3] * 1.0f);
+            };
+            for (uint32_t v = 0; v < d->mesh_nv[m]; v++) {
+                float w[3];
+                xf(v, w);
+                for (int a = 0; a < 3; a++) {
+                    s->bmin[a] = std::min(s->bmin[a], w[a]);
+                    s->bmax[a] = std::max(s->bmax[a], w[a]);
+                }
+            }
+            for (uint32_t t = 0; t < d->mesh_nt[m]; t++, g++) {
+                const float *e = d->materials + 12 * d->tri_material[g] + 6;
+                if (!emissive(e)) continue;
+                float a[3], b[3], c[3];
+                xf(I[3 * t], a);
+                xf(I[3 * t + 1], b);
+                xf(I[3 * t + 2], c);
+                light_row(a, b, c, e, lt);
+            }
+        }
+        s->nl = (uint32_t)(lt.size() / 16);
+        upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
+        KRT_HIP(hipStreamSynchronize(C.stream));
+        krt_handle h = C.next_handle++;
+        C.scenes[h] = std::move(s);
+        *scene = h;
+    });
+}
+
+int krt_accel_instance_tree(krt_context *ctx, krt_handle scene, float *out, size_t nfloats, uint32_t *nodes) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && nodes, KRT_ERR_INVALID, "null argument");
+        const Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(s.instanced && s.tlas.built, KRT_ERR_STATE, "not a built instanced scene");
+        *nodes = s.tlas.nser;
+        if (!out) return;
+        KRT_REQUIRE(nfloats >= 12 * (size_t)s.tlas.nser, KRT_ERR_INVALID, "output too small");
+        KRT_HIP(hipMemcpy(out, s.tlas.ser.p, sizeof(float) * 12 * (size_t)s.tlas.nser, hipMemcpyDeviceToHost));
+    });
+}
+
+int krt_accel_mesh_serialized(krt_context *ctx, krt_handle scene, uint32_t mesh, float *out, size_t nfloats,
+                              uint32_t *nodes) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && nodes, KRT_ERR_INVALID, "null argument");
+        const Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(s.instanced && s.accel.built, KRT_ERR_STATE, "not a built instanced scene");
+        KRT_REQUIRE(mesh < s.nmesh, KRT_ERR_INVALID, "mesh index out of range");
+        const uint32_t n = 2 * s.mesh_nt[mesh] - 1;
+        *nodes = n;
+        if (!out) return;
+        KRT_REQUIRE(nfloats >= 12 * (size_t)n, KRT_ERR_INVALID, "output too small");
+        KRT_HIP(hipMemcpy(out, s.mesh_ser.as<float4>() + 3 * (size_t)s.mesh_ser_base[mesh],
+                          sizeof(float) * 12 * (size_t)n, hipMemcpyDeviceToHost));
     });
 }
 
@@ -335,6 +472,8 @@ int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t 
         KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
         const Accel &A = ctx->impl.scene(scene).accel;
         KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
+        KRT_REQUIRE(!ctx->impl.scene(scene).instanced, KRT_ERR_STATE,
+                    "instanced scene: use krt_accel_mesh_serialized / krt_accel_instance_tree");
         KRT_REQUIRE(nfloats >= 12 * (size_t)A.nser, KRT_ERR_INVALID, "output too small");
         KRT_HIP(hipMemcpy(out, A.ser.p, sizeof(float) * 12 * (size_t)A.nser, hipMemcpyDeviceToHost));
     });
@@ -345,6 +484,7 @@ int krt_accel_leaf_order(krt_context *ctx, krt_handle scene, int32_t *out, size_
         KRT_REQUIRE(ctx && out, KRT_ERR_INVALID, "null argument");
         const Accel &A = ctx->impl.scene(scene).accel;
         KRT_REQUIRE(A.built, KRT_ERR_STATE, "acceleration structure not built");
+        KRT_REQUIRE(!ctx->impl.scene(scene).instanced, KRT_ERR_STATE, "instanced scene: no single leaf order");
         KRT_REQUIRE(n >= A.nt, KRT_ERR_INVALID, "output too small");
         KRT_HIP(hipMemcpy(out, A.leaf_order.p, sizeof(int32_t) * A.nt, hipMemcpyDeviceToHost));
     });
@@ -1018,6 +1158,7 @@ int krt_server_submit(krt_server *srv, const krt_request *req, uint64_t *id) {
     return guard([&] {
         KRT_REQUIRE(srv && req && id, KRT_ERR_INVALID, "null argument");
         Scene &s = srv->ctx->impl.scene(req->scene);
+        KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "the server renders flat scenes (krt_scene_upload)");
         KRT_HIP(hipStreamSynchronize(srv->ctx->impl.stream));  // scene uploads and builds are complete
         *id = serve_submit(srv->impl, s, req->camera, req->width, req->height, req->spp, req->max_depth, req->seed);
     });

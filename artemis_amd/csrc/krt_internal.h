@@ -101,9 +101,25 @@ struct Scene {
     Accel accel;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // vertex bounding box
     uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
+    // Two-level scenes (krt_scene_upload_instanced).  pos / nrm / idx then hold
+    // the base meshes back to back in their own space (indices local to their
+    // mesh); tri_mat is per global triangle (instance i's triangles follow
+    // those of instances 0..i-1); accel.wide / accel.tri hold every mesh's
+    // 4-wide tree and packed triangles back to back; tlas is the tree over the
+    // instances and inst_rec their records in its traversal order.
+    bool instanced = false;
+    uint32_t nmesh = 0, ninst = 0;
+    std::vector<uint32_t> mesh_nv, mesh_nt, mesh_vbase, mesh_tbase;
+    std::vector<int32_t> inst_mesh;
+    std::vector<float> xform;                // 12 floats per instance (row-major 3x4)
+    std::vector<uint32_t> mesh_ser_base;     // first serialised node of each mesh in mesh_ser
+    DevBuf mesh_ser;                         // every mesh's bvh.cpp tree, back to back
+    Accel tlas;
+    DevBuf inst_rec;
     size_t bytes() const {
         return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + accel.ser.bytes +
-               accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes + accel.wide.bytes;
+               accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes + accel.wide.bytes + mesh_ser.bytes +
+               tlas.ser.bytes + tlas.leaf_order.bytes + tlas.nodes.bytes + tlas.wide.bytes + inst_rec.bytes;
     }
 };
 
@@ -118,7 +134,9 @@ struct SceneView {
     uint32_t nl;
     const float4 *tri;
     const float4 *nodes;  // binary traversal tree
-    const float4 *wide;   // its 4-wide collapse
+    const float4 *wide;   // its 4-wide collapse (instanced scenes: every mesh's, concatenated)
+    const float4 *tlas;   // instanced scenes: 4-wide instance tree
+    const float4 *inst;   // instanced scenes: instance records (krt_traverse.h)
 };
 
 SceneView make_view(const Scene &s);
@@ -202,6 +220,9 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions
                unsigned long long *ctr, PassTimes &times);
 float copy_probe(Context &ctx, size_t bytes, int reps);
 void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays);
+// Inverse of a row-major 3x4 affine transform (FP64 cofactors, rounded once);
+// throws on a singular transform.
+void affine_inverse(const float *M, float *Minv);
 // hdr_to_ldr (include/cuda/color.cuh:82-121) on the context stream; no sync.
 void hdr_to_ldr(Context &ctx, const float4 *pixels, uint32_t *target, uint32_t width, uint32_t height,
                 int tonemapping);

@@ -23,6 +23,7 @@ struct PtArgs {
     double *acc64;            // optional FP64 sums of samples, then of their squares (3 + 3 per pixel)
 };
 
+template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
@@ -47,15 +48,15 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
-            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
             if (h.prim < 0) break;  // __miss__: no environment map, value 0
             Surface sf;
-            if (!load_surface(sv, h, d, sf)) break;
+            if (!load_surface<INST>(sv, h, d, sf)) break;
             Material mat = load_material(sv.mats, sf.mat_id);
             f3 n = sf.n;
             f3 x = offset_point(sf.x, n, mat.type);
             SurfaceHit sh{mat, sf.entering, n, -d, x};
-            f3 direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+            f3 direct = mat.emission + Ld<true, INST>(sv, sh, seed, stk, ts, rc);
             int out;
             f3 wi;
             float pdf;
@@ -122,7 +123,8 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, un
     KRT_HIP(hipEventCreate(&e0));
     KRT_HIP(hipEventCreate(&e1));
     KRT_HIP(hipEventRecord(e0, ctx.stream));
-    pt_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    if (s.instanced) pt_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    else pt_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(e1, ctx.stream));
     KRT_HIP(hipEventSynchronize(e1));

@@ -96,11 +96,11 @@ KRT_DEV f3 direct_li(const SurfaceHit &sh, f3 Le, f3 normal, int type, f3 D, flo
 }
 
 // direct_occluded (:399-421)
-template <bool WIDE = true>
+template <bool WIDE = true, bool INST = false>
 KRT_DEV f3 direct_occ(const SceneView &sv, const SurfaceHit &sh, const RSample &s, f3 D, float d,
                       const TraceStack &stk, TravStats &ts, RayCount &rc, bool *occ_out = nullptr) {
     rc.shadow++;
-    bool occ = occluded<WIDE>(sv, sh.x, D, d - kShadowEps, stk, ts);
+    bool occ = occluded<WIDE, INST>(sv, sh.x, D, d - kShadowEps, stk, ts);
     if (occ_out) *occ_out = occ;
     if (occ) return mk(0.0f, 0.0f, 0.0f);
     return direct_li(sh, s.Le, s.normal, s.type, D, d);
@@ -114,11 +114,12 @@ KRT_DEV float target(const RSample &s, const SurfaceHit &hit) {
     return Li.x + Li.y + Li.z;
 }
 
+template <bool INST>
 KRT_DEV float occluded_target(const SceneView &sv, const RSample &s, const SurfaceHit &hit, const TraceStack &stk,
                               TravStats &ts, RayCount &rc) {
     f3 D = s.point - hit.x;
     float d = length(D);
-    f3 Li = direct_occ(sv, hit, s, D / d, d, stk, ts, rc);
+    f3 Li = direct_occ<true, INST>(sv, hit, s, D / d, d, stk, ts, rc);
     return Li.x + Li.y + Li.z;
 }
 
@@ -150,6 +151,7 @@ KRT_DEV SurfaceHit gbuffer_hit(const RestirArgs &A, uint32_t i) {
 
 // Four waves per SIMD: 128 VGPRs with a few dwords spilled to scratch runs
 // faster here than 139 VGPRs at three waves (7.5 vs 8.7 ms at 1080p).
+template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void initial_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
@@ -185,10 +187,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
         for (int level = 0; level <= li.max_depth; level++) {
             Hit h;
             rc.closest++;
-            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
+            trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
             if (h.prim < 0) break;
             Surface sf;
-            if (!load_surface(sv, h, d, sf)) break;
+            if (!load_surface<INST>(sv, h, d, sf)) break;
             Material mat = load_material(sv.mats, sf.mat_id);
             if (cv.enabled && level == cv.depth) {
                 put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
@@ -233,14 +235,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                 f3 D = smp.point - sh.x;
                 float dd = length(D);
                 bool occ;
-                f3 Li = direct_occ<false>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
+                f3 Li = direct_occ<false, INST>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
                 res.resample(Li.x + Li.y + Li.z);
                 // the reference traces this same visibility ray a second time
                 res.W *= 1.0f - (occ ? 1.0f : 0.0f);
                 A.cur.store(pix, res);
                 A.gmat[pix] = sf.mat_id;
             } else {
-                direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);
+                direct = mat.emission + Ld<false, INST>(sv, sh, seed, stk, ts, rc);
             }
             int out;
             f3 wi;
@@ -336,6 +338,7 @@ KRT_DEV int reproject(const LaunchInfo &li, int index, f3 pos) {
     return iy * (int)li.width + ix;
 }
 
+template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
         merged.w = merged.W = 0.0f;
         merged.M = 0;
         merged.seed = ld3(A.auxiliary[pix]);
-        float t_current = occluded_target(A.sv, cr.data, hit, stk, ts, rc);
+        float t_current = occluded_target<INST>(A.sv, cr.data, hit, stk, ts, rc);
         merged.update(cr.data, t_current * cr.W * (float)cr.M);
         int N = cr.M;
         if (pindex >= 0) {
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
             if (pr.M > 0) {
                 pr.M = min(pr.M, A.m_cap);
                 A.prev.store_M((uint32_t)pindex, pr.M);
-                float t_prev = occluded_target(A.sv, pr.data, hit, stk, ts, rc);
+                float t_prev = occluded_target<INST>(A.sv, pr.data, hit, stk, ts, rc);
                 merged.update(pr.data, t_prev * pr.W * (float)pr.M);
                 N += (t_prev > 0.0f ? 1 : 0) * pr.M;
             }
@@ -383,6 +386,7 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
 
 // Four waves per SIMD (128 VGPRs, no spills): the pass is latency-bound on
 // its eleven shadow walks per pixel, so waves in flight matter most.
+template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void spatial_kernel(RestirArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                     f3 Dn = nb.data.point - hit.x;
                     float dn = length(Dn);
                     bool occ;
-                    f3 Ln = direct_occ(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
+                    f3 Ln = direct_occ<true, INST>(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
                     float t_nb = Ln.x + Ln.y + Ln.z;
                     if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
                         sel_known = true;
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                     nh.n = mk(nn.x, nn.y, nn.z);
                     nh.x = mk(xx.x, xx.y, xx.z);
                     nh.wo = hit.wo;
-                    float t_nb = occluded_target(A.sv, merged.data, nh, stk, ts, rc);
+                    float t_nb = occluded_target<INST>(A.sv, merged.data, nh, stk, ts, rc);
                     if (t_nb > 0.0f) Z += m;
                 }
             }
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                 lighting = sel_occ ? mk(0.0f, 0.0f, 0.0f)
                                    : direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
             } else {
-                lighting = direct_occ(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
+                lighting = direct_occ<true, INST>(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
             }
             // emission re-read here rather than held in registers through the walks
             direct = load_material(A.sv.mats, A.gmat[pix]).emission + lighting * merged.W;
@@ -589,7 +593,8 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     mark();
     A.cur = planes(st.cur);
     A.prev = planes(1 - st.cur);
-    initial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    if (s.instanced) initial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    else initial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     mark();
     if (cached) {
@@ -597,13 +602,15 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
     }
     mark();
-    temporal_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    if (s.instanced) temporal_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    else temporal_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     mark();
     for (int pass = 0; pass < opt.spatial_passes; pass++) {
         A.cur = planes(st.cur);
         A.prev = planes(1 - st.cur);
-        spatial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        if (s.instanced) spatial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        else spatial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
         KRT_HIP(hipGetLastError());
         st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
     }

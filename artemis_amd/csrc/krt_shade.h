@@ -31,8 +31,40 @@ KRT_DEV f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
     return (w * a + b1 * b) + b2 * c;
 }
 
+// Instanced hit: calculate_intersection / calculate_normal with the
+// instance's model matrix (amadeus_common.cuh:62-112): x = M (interp, 1), the
+// edges and the interpolated normal through M with w = 0.
+KRT_DEV bool load_surface_inst(const SceneView &sv, const Hit &h, f3 dir, Surface &sf) {
+    const float4 *rec = sv.inst + kInstRec * (size_t)h.inst;
+    const float4 m0 = rec[3], m1 = rec[4], m2 = rec[5], meta = rec[6], geo = rec[7];
+    const uint32_t local = (uint32_t)(h.prim - __float_as_int(meta.z));
+    const uint32_t *ix = sv.idx + 3 * ((size_t)__float_as_int(geo.y) + local);
+    const uint32_t vb = (uint32_t)__float_as_int(geo.x);
+    const uint32_t i0 = vb + ix[0], i1 = vb + ix[1], i2 = vb + ix[2];
+    f3 a = ld3(sv.pos + 3 * (size_t)i0), b = ld3(sv.pos + 3 * (size_t)i1), c = ld3(sv.pos + 3 * (size_t)i2);
+    f3 ng = normalize(cross(xform(m0, m1, m2, b - a, 0.0f), xform(m0, m1, m2, c - a, 0.0f)));
+    if (dot(ng, dir) > 0.0f) {
+        ng = -ng;
+        sf.entering = false;
+    } else {
+        sf.entering = true;
+    }
+    ng = normalize(ng);
+    f3 nn = xform(m0, m1, m2,
+                  interp(ld3(sv.nrm + 3 * (size_t)i0), ld3(sv.nrm + 3 * (size_t)i1), ld3(sv.nrm + 3 * (size_t)i2),
+                         h.b1, h.b2),
+                  0.0f);
+    if (dot(nn, ng) < 0.0f) nn = -nn;
+    sf.n = normalize(nn);
+    sf.x = xform(m0, m1, m2, interp(a, b, c, h.b1, h.b2), 1.0f);
+    sf.mat_id = sv.tri_mat[h.prim];
+    return !isnan3(sf.n);
+}
+
 // Returns false when the shading normal is NaN (the reference then returns 0).
+template <bool INST = false>
 KRT_DEV bool load_surface(const SceneView &sv, const Hit &h, f3 dir, Surface &sf) {
+    if (INST) return load_surface_inst(sv, h, dir, sf);
     const uint32_t i0 = sv.idx[3 * h.prim], i1 = sv.idx[3 * h.prim + 1], i2 = sv.idx[3 * h.prim + 2];
     f3 a = ld3(sv.pos + 3 * i0), b = ld3(sv.pos + 3 * i1), c = ld3(sv.pos + 3 * i2);
     f3 ng = normalize(cross(b - a, c - a));
@@ -55,14 +87,24 @@ KRT_DEV f3 offset_point(f3 x, f3 n, int type) {
     return x + ((type == eTransmission ? -1.0f : 1.0f) * n) * kEps;
 }
 
-template <bool WIDE = true>
+// Closest hit of the scene: one-level walk of the binary tree, or the
+// two-level walk of an instanced scene.
+template <bool INST = false>
+KRT_DEV bool trace_closest(const SceneView &sv, f3 o, f3 d, float tmax, const TraceStack &stk, Hit &h,
+                           TravStats &ts) {
+    if (INST) return traverse_inst<false>(sv.tlas, sv.inst, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
+    return traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
+}
+
+template <bool WIDE = true, bool INST = false>
 KRT_DEV bool occluded(const SceneView &sv, f3 o, f3 d, float tmax, const TraceStack &stk, TravStats &ts) {
     Hit h;
+    if (INST) return traverse_inst<true>(sv.tlas, sv.inst, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
     return traverse<true, WIDE>(sv.nodes, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
 }
 
 // Ld with one uniformly chosen triangle light (no quad lights, no envmap).
-template <bool WIDE = true>
+template <bool WIDE = true, bool INST = false>
 KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceStack &stk, TravStats &ts,
               RayCount &rc) {
     const unsigned total = sv.nl;
@@ -79,7 +121,7 @@ KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceSt
     float R = length(lpos - sh.x);
     rc.shadow++;
     f3 contr;
-    if (occluded<WIDE>(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
+    if (occluded<WIDE, INST>(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
         contr = mk(0.0f, 0.0f, 0.0f);
     } else {
         f3 f = brdf(sh, wi, eDiffuse);

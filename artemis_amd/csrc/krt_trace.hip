@@ -11,10 +11,12 @@ namespace krt {
 
 namespace {
 
-template <bool ANY>
+template <bool ANY, bool INST>
 __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__restrict__ nodes,
                                                              const float4 *__restrict__ wide,
                                                              const float4 *__restrict__ tri,
+                                                             const float4 *__restrict__ tlas,
+                                                             const float4 *__restrict__ inst,
                                                              const float4 *__restrict__ rays, uint32_t n,
                                                              float4 *__restrict__ hits, int32_t *__restrict__ occ,
                                                              int *spill, int spill_depth) {
@@ -27,7 +29,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const float4 *__rest
     for (uint32_t i = gtid; i < n; i += nthreads) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         Hit h;
-        bool r = traverse<ANY>(nodes, wide, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts);
+        bool r = INST ? traverse_inst<ANY>(tlas, inst, wide, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts)
+                      : traverse<ANY>(nodes, wide, tri, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, h, ts);
         if (ANY) {
             occ[i] = r ? 1 : 0;
         } else {
@@ -100,16 +103,22 @@ static void trace_common(Context &ctx, Scene &s, const float *rays_dev, uint32_t
     if (n == 0) return;
     uint32_t nb = persistent_blocks(n, kTraceBlock, kMaxResidentBlocks);
     int *spill = spill_buffer(ctx, s, nb * kTraceBlock);
-    if (any)
-        trace_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.wide.as<float4>(),
-                                                              s.accel.tri.as<float4>(),
-                                                              (const float4 *)rays_dev, n, nullptr, occ_dev, spill,
-                                                              (int)s.accel.depth_bound);
+    const float4 *nodes = s.accel.nodes.as<float4>(), *wide = s.accel.wide.as<float4>(),
+                 *tri = s.accel.tri.as<float4>(), *tlas = s.tlas.wide.as<float4>(), *inst = s.inst_rec.as<float4>();
+    const float4 *rays = (const float4 *)rays_dev;
+    const int depth = (int)s.accel.depth_bound;
+    if (any && s.instanced)
+        trace_kernel<true, true><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst, rays, n, nullptr,
+                                                                    occ_dev, spill, depth);
+    else if (any)
+        trace_kernel<true, false><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst, rays, n, nullptr,
+                                                                     occ_dev, spill, depth);
+    else if (s.instanced)
+        trace_kernel<false, true><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst, rays, n,
+                                                                     (float4 *)hits_dev, nullptr, spill, depth);
     else
-        trace_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(s.accel.nodes.as<float4>(), s.accel.wide.as<float4>(),
-                                                               s.accel.tri.as<float4>(), (const float4 *)rays_dev, n,
-                                                               (float4 *)hits_dev, nullptr, spill,
-                                                               (int)s.accel.depth_bound);
+        trace_kernel<false, false><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst, rays, n,
+                                                                      (float4 *)hits_dev, nullptr, spill, depth);
     KRT_HIP(hipGetLastError());
 }
 

@@ -29,6 +29,7 @@ struct Hit {
     float t, b1, b2;
     int prim;
     int rank;
+    int inst;  // instance record of a two-level scene's hit (-1 otherwise)
 };
 
 // LDS-typed pointer: keeps stack traffic on ds_read/ds_write instead of flat
@@ -198,6 +199,7 @@ KRT_DEV bool traverse4(const float4 *__restrict__ nodes, const float4 *__restric
     }
     hit.prim = -1;
     hit.rank = 0x7fffffff;
+    hit.inst = -1;
     hit.t = 0.0f;
     hit.b1 = hit.b2 = 0.0f;
     if (ANY || best_pos < 0) return false;
@@ -270,6 +272,7 @@ KRT_DEV bool traverse2(const float4 *__restrict__ nodes, const float4 *__restric
     }
     hit.prim = -1;
     hit.rank = 0x7fffffff;
+    hit.inst = -1;
     hit.t = 0.0f;
     hit.b1 = hit.b2 = 0.0f;
     if (ANY || best_pos < 0) return false;
@@ -280,6 +283,144 @@ KRT_DEV bool traverse2(const float4 *__restrict__ nodes, const float4 *__restric
     hit.b2 = b2;
     hit.prim = prim;
     hit.rank = r;
+    return true;
+}
+
+// Row-major 3x4 affine transform of a point (w = 1) or direction (w = 0)
+// with glm's pairing of mat4 x vec4: (c0 x + c1 y) + (c2 z + c3 w).
+KRT_DEV f3 xform(const float4 &r0, const float4 &r1, const float4 &r2, f3 p, float w) {
+    return mk((r0.x * p.x + r0.y * p.y) + (r0.z * p.z + r0.w * w), (r1.x * p.x + r1.y * p.y) + (r1.z * p.z + r1.w * w),
+              (r2.x * p.x + r2.y * p.y) + (r2.z * p.z + r2.w * w));
+}
+
+// Instance record (8 float4, in instance-tree traversal order): rows of the
+// inverse transform, rows of the transform, {4-wide node base, packed
+// triangle base, global id of the first triangle, bvh.cpp rank of the
+// instance}, {vertex base, triangle base of the mesh, mesh, 0}.
+constexpr int kInstRec = 8;
+
+// Two-level walk (Accelerator::build_tlas + optixTrace semantics,
+// include/amadeus/accelerator.cuh:197-277): the world ray walks the 4-wide
+// instance tree (leaves hold one instance); at an instance the ray is taken
+// into the mesh's space by the inverse transform (the direction is not
+// renormalised, so t is shared) and walks that mesh's 4-wide tree on the same
+// stack.  The closest hit is the lexicographic minimum of (t, instance rank,
+// triangle rank): the first instance in instance-tree preorder, then the first
+// triangle in mesh preorder, among equal t.
+template <bool ANY>
+KRT_DEV bool traverse_inst(const float4 *__restrict__ tlas, const float4 *__restrict__ inst,
+                           const float4 *__restrict__ wide, const float4 *__restrict__ tri, const f3 wo, const f3 wd,
+                           float tmax, const TraceStack &stk, Hit &hit, TravStats &ts) {
+    const f3 winv = safe_inv(wd);
+    const int wz = zero_mask(wd);
+    f3 o = wo, d = wd, inv = winv;
+    int zmask = wz;
+    const float4 *nodes = tlas;
+    const float4 *tris = tri;
+    float best = tmax;
+    int best_rank = 0x7fffffff, best_irank = 0x7fffffff;
+    int best_pos = -1, best_inst = -1;
+    int cur_inst = -1, cur_irank = 0;
+    int base_sp = -1;  // stack height when the current instance was entered (-1: in the instance tree)
+    int sp = 0;
+    int cur = 0;
+    for (;;) {
+        if (!is_leaf_link(cur)) {
+            const float4 *nd = nodes + 8 * (size_t)cur;
+            const float4 MNX = nd[0], MXX = nd[1], MNY = nd[2], MXY = nd[3], MNZ = nd[4], MXZ = nd[5], L = nd[6];
+            ts.nodes++;
+            float k0, k1, k2, k3;
+            int l0 = __float_as_int(L.x), l1 = __float_as_int(L.y), l2 = __float_as_int(L.z),
+                l3 = __float_as_int(L.w);
+            const float inf = 3.4e38f;
+            bool h;
+            h = l0 != 0 && slab(MNX.x, MXX.x, MNY.x, MXY.x, MNZ.x, MXZ.x, o, inv, zmask, best, k0);
+            k0 = h ? k0 : inf;
+            h = l1 != 0 && slab(MNX.y, MXX.y, MNY.y, MXY.y, MNZ.y, MXZ.y, o, inv, zmask, best, k1);
+            k1 = h ? k1 : inf;
+            h = l2 != 0 && slab(MNX.z, MXX.z, MNY.z, MXY.z, MNZ.z, MXZ.z, o, inv, zmask, best, k2);
+            k2 = h ? k2 : inf;
+            h = l3 != 0 && slab(MNX.w, MXX.w, MNY.w, MXY.w, MNZ.w, MXZ.w, o, inv, zmask, best, k3);
+            k3 = h ? k3 : inf;
+            cswap(k0, l0, k1, l1);
+            cswap(k2, l2, k3, l3);
+            cswap(k0, l0, k2, l2);
+            cswap(k1, l1, k3, l3);
+            cswap(k1, l1, k2, l2);
+            if (k0 != inf) {
+                if (k3 != inf) stk.put(sp++, l3);
+                if (k2 != inf) stk.put(sp++, l2);
+                if (k1 != inf) stk.put(sp++, l1);
+                cur = l0;
+                continue;
+            }
+        } else if (base_sp < 0) {
+            // instance leaf: enter the mesh of instance record `first`
+            const int r = cur & 0x03FFFFFF;
+            const float4 *rec = inst + kInstRec * (size_t)r;
+            const float4 i0 = rec[0], i1 = rec[1], i2 = rec[2], meta = rec[6];
+            o = xform(i0, i1, i2, wo, 1.0f);
+            d = xform(i0, i1, i2, wd, 0.0f);
+            inv = safe_inv(d);
+            zmask = zero_mask(d);
+            nodes = wide + 8 * (size_t)__float_as_int(meta.x);
+            tris = tri + 3 * (size_t)__float_as_int(meta.y);
+            cur_inst = r;
+            cur_irank = __float_as_int(meta.w);
+            base_sp = sp;
+            cur = 0;
+            continue;
+        } else {
+            const int first = cur & 0x03FFFFFF;
+            const int cnt = ((cur >> 26) & 31) + 1;
+            for (int k = 0; k < cnt; k++) {
+                float b1, b2;
+                int prim, rk;
+                float t = mt_test(tris, first + k, o, d, b1, b2, prim, rk);
+                ts.tris++;
+                if (ANY) {
+                    if (t > 0.0f && t < tmax) return true;
+                } else {
+                    bool take = t > 0.0f &&
+                                (t < best || (best_pos >= 0 && t == best &&
+                                              (cur_irank < best_irank || (cur_irank == best_irank && rk < best_rank))));
+                    best = take ? t : best;
+                    best_rank = take ? rk : best_rank;
+                    best_irank = take ? cur_irank : best_irank;
+                    best_pos = take ? first + k : best_pos;
+                    best_inst = take ? cur_inst : best_inst;
+                }
+            }
+        }
+        // pop; leaving an instance when its part of the stack is exhausted
+        if (base_sp >= 0 && sp == base_sp) {
+            o = wo;
+            d = wd;
+            inv = winv;
+            zmask = wz;
+            nodes = tlas;
+            base_sp = -1;
+        }
+        if (sp == 0) break;
+        cur = stk.get(--sp);
+    }
+    hit.prim = -1;
+    hit.rank = 0x7fffffff;
+    hit.inst = -1;
+    hit.t = 0.0f;
+    hit.b1 = hit.b2 = 0.0f;
+    if (ANY || best_pos < 0) return false;
+    const float4 *rec = inst + kInstRec * (size_t)best_inst;
+    const float4 i0 = rec[0], i1 = rec[1], i2 = rec[2], meta = rec[6];
+    const f3 bo = xform(i0, i1, i2, wo, 1.0f), bd = xform(i0, i1, i2, wd, 0.0f);
+    float b1 = 0.0f, b2 = 0.0f;
+    int prim, rk;
+    hit.t = mt_test(tri + 3 * (size_t)__float_as_int(meta.y), best_pos, bo, bd, b1, b2, prim, rk);
+    hit.b1 = b1;
+    hit.b2 = b2;
+    hit.prim = __float_as_int(meta.z) + prim;
+    hit.rank = rk;
+    hit.inst = best_inst;
     return true;
 }
 

@@ -68,6 +68,52 @@ class SceneData:
         return len(self.indices)
 
 
+def xform_points(M: np.ndarray, p: np.ndarray, w: float = 1.0) -> np.ndarray:
+    """Row-major 3x4 transform of float32 points (w = 1) or directions (w = 0)
+    with glm's pairing (c0 x + c1 y) + (c2 z + c3 w), in float32."""
+    M = np.asarray(M, np.float32)
+    p = np.asarray(p, np.float32)
+    w = np.float32(w)
+    out = np.empty_like(p)
+    for r in range(3):
+        out[:, r] = (M[r, 0] * p[:, 0] + M[r, 1] * p[:, 1]) + (M[r, 2] * p[:, 2] + M[r, 3] * w)
+    return out
+
+
+@dataclass
+class InstancedSceneData:
+    """Base meshes in their own space and instances {mesh, row-major 3x4
+    transform} (Accelerator::update / build_tlas,
+    include/amadeus/accelerator.cuh:145-277).  Triangle ids are global:
+    instance i's triangles follow those of instances 0..i-1; tri_material
+    holds the material of every global triangle."""
+    meshes: list              # [(positions (nv, 3) f32, normals (nv, 3) f32, indices (nt, 3) u32)]
+    inst_mesh: np.ndarray     # (ni,) int32
+    transforms: np.ndarray    # (ni, 3, 4) float32
+    tri_material: np.ndarray  # (sum of the instances' triangles,) uint32
+    materials: np.ndarray     # (nm, 12) float32
+    camera: dict = field(default_factory=dict)
+    name: str = ""
+
+    @property
+    def triangles(self) -> int:
+        return int(sum(len(self.meshes[m][2]) for m in self.inst_mesh))
+
+    def flatten(self) -> SceneData:
+        """The same scene as one world-space soup (vertices through the
+        transforms in float32, normals through the 3x3 part)."""
+        P, N, I = [], [], []
+        nv = 0
+        for i, m in enumerate(self.inst_mesh):
+            pos, nrm, idx = self.meshes[m]
+            P.append(xform_points(self.transforms[i], pos, 1.0))
+            N.append(xform_points(self.transforms[i], nrm, 0.0))
+            I.append(idx + nv)
+            nv += len(pos)
+        return SceneData(np.concatenate(P), np.concatenate(N), np.concatenate(I).astype(np.uint32),
+                         self.tri_material, self.materials, camera=self.camera, name=self.name + "_flat")
+
+
 def material(diffuse, roughness, emission=(0.0, 0.0, 0.0), specular=(0.0, 0.0, 0.0), refraction=1.0,
              type_=E_DIFFUSE):
     return [*diffuse, *specular, *emission, roughness, refraction, type_]
@@ -228,7 +274,9 @@ def _rotation(rng: SplitMix64) -> np.ndarray:
 
 
 def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: int = 4096, box: float = 20.0,
-            seed: int = 2, base_meshes: int = 16) -> SceneData:
+            seed: int = 2, base_meshes: int = 16, instanced: bool = False):
+    """The instanced-sphere scene; instanced=True returns it as base meshes +
+    transforms (InstancedSceneData), same random stream, same materials."""
     rng = SplitMix64(seed)
     unit, idx = sphere_mesh(slices, stacks)
     bases = []
@@ -238,7 +286,7 @@ def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: 
     ntb = len(idx)
     nt = instances * ntb
     mats = []
-    P, N, I, M = [], [], [], []
+    P, N, I, M, X = [], [], [], [], []
     for k in range(instances):
         base = bases[k % base_meshes]
         R = _rotation(rng)
@@ -249,6 +297,7 @@ def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: 
         N.append(unit @ R.T)
         I.append(idx + k * len(unit))
         M.append(np.full(ntb, len(mats) - 1, np.uint32))
+        X.append(np.concatenate([s * R, t[:, None]], axis=1))
     tri_mat = np.concatenate(M)
     if emissive > 0:
         picks = np.unique((rng.uniform_array(emissive * 2) * nt).astype(np.int64))[:emissive]
@@ -264,6 +313,11 @@ def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: 
             tri_mat[tri] = len(mats) - 1
     d = box * 0.9
     cam = dict(eye=(0.0, 0.0, d), lookat=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=60.0)
+    if instanced:
+        meshes = [(b.astype(np.float32), unit.astype(np.float32), idx.astype(np.uint32)) for b in bases]
+        return InstancedSceneData(meshes, np.array([k % base_meshes for k in range(instances)], np.int32),
+                                  np.asarray(X, np.float32), tri_mat, np.asarray(mats, np.float32), camera=cam,
+                                  name=f"spheres{instances}x{ntb}_instanced")
     return SceneData(np.concatenate(P).astype(np.float32), np.concatenate(N).astype(np.float32),
                      np.concatenate(I).astype(np.uint32), tri_mat, np.asarray(mats, np.float32), camera=cam,
                      name=f"spheres{instances}x{ntb}")

@@ -68,6 +68,38 @@ typedef struct {
 } krt_scene_desc;
 
 int krt_scene_upload(krt_context *ctx, const krt_scene_desc *desc, krt_handle *scene);
+
+/* Two-level scene: base meshes in their own space and instances of them with
+ * a row-major 3x4 transform (OptixInstance::transform).  Replaces
+ * Accelerator::update (one GAS per entity submesh,
+ * include/amadeus/accelerator.cuh:145-190) and build_tlas (instances,
+ * :197-277).  Meshes are given back to back (indices local to their mesh).
+ * Triangle ids are global: instance i's triangles follow those of instances
+ * 0..i-1, in its mesh's order, and tri_material gives the material of every
+ * global triangle (the reference gives each entity's submesh its own
+ * material).  Hits report global ids; shading follows
+ * calculate_intersection / calculate_normal with the instance transform as
+ * the model matrix (source/optix/amadeus_common.cuh:62-112); lights are the
+ * emissive triangles through the transform, in instance then triangle order
+ * (armada.cu:117-140).  The path tracer, ReSTIR DI and the ray queries take
+ * instanced scenes; ReSTIR GI, the radiance cache and the server take flat
+ * scenes. */
+typedef struct {
+    const float *positions;     /* sum(mesh_nv) x 3 */
+    const float *normals;       /* sum(mesh_nv) x 3 */
+    const uint32_t *mesh_nv;    /* nmesh */
+    const uint32_t *indices;    /* sum(mesh_nt) x 3, local to their mesh */
+    const uint32_t *mesh_nt;    /* nmesh */
+    uint32_t nmesh;
+    const uint32_t *instance_mesh; /* ninst */
+    const float *transforms;       /* ninst x 12, row-major 3x4, non-singular */
+    uint32_t ninst;
+    const uint32_t *tri_material;  /* one per global triangle */
+    const float *materials;        /* nm x 12 */
+    uint32_t nm;
+} krt_instanced_desc;
+
+int krt_scene_upload_instanced(krt_context *ctx, const krt_instanced_desc *desc, krt_handle *scene);
 int krt_scene_free(krt_context *ctx, krt_handle scene);
 int krt_scene_light_count(krt_context *ctx, krt_handle scene, uint32_t *out);
 /* 12 floats per light {a, ab, ac, intensity} (include/optix/lighting.cuh:96-100) */
@@ -104,6 +136,15 @@ int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out);
 int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t nfloats);
 /* primitive id of each leaf in preorder (the closest-hit tie-break rank) */
 int krt_accel_leaf_order(krt_context *ctx, krt_handle scene, int32_t *out, size_t n);
+/* Instanced scenes: the bvh.cpp tree of one base mesh, and the bvh.cpp tree
+ * over the instances' world boxes (the 8 corners of the mesh's root box
+ * through the transform, widened by 1e-4 of the largest extent + 1e-5 of the
+ * largest coordinate); 12 floats per node, *nodes = the node count (out may
+ * be NULL to query it).  Instanced closest hits are the lexicographic
+ * minimum of (t, instance rank in this tree, triangle rank in its mesh's). */
+int krt_accel_mesh_serialized(krt_context *ctx, krt_handle scene, uint32_t mesh, float *out, size_t nfloats,
+                              uint32_t *nodes);
+int krt_accel_instance_tree(krt_context *ctx, krt_handle scene, float *out, size_t nfloats, uint32_t *nodes);
 
 /* Ray queries on host buffers.  rays: 8 floats {o.xyz, tmax, d.xyz, pad};
  * hits: 4 words {t, b1, b2, prim} with prim = -1 on a miss.  The closest hit

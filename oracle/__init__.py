@@ -75,6 +75,10 @@ _SIGS = {
     "orc_mlp_grad": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, C.c_int, C.c_float, dp]),
     "orc_mlp_adam": (None, [fp, fp, fp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp]),
     "orc_hdr_to_ldr": (None, [fp, u32p, C.c_int, C.c_int, C.c_int]),
+    "orc_scene_create_instanced": (vp, [C.c_int, i32p, i32p, fp, fp, u32p, C.c_int, i32p, fp, u32p, fp, C.c_int]),
+    "orc_scene_tlas": (C.c_int, [vp, fp]),
+    "orc_affine_inverse": (None, [fp, fp]),
+    "orc_instance_box": (None, [fp, fp, fp, fp]),
 }
 
 _lib = None
@@ -140,6 +144,13 @@ def hdr_to_ldr(pixels: np.ndarray, tonemapping: int = 0) -> np.ndarray:
     return out
 
 
+def affine_inverse(M: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(M, np.float32).reshape(12)
+    out = np.empty(12, np.float32)
+    lib().orc_affine_inverse(_f(m), _f(out))
+    return out.reshape(3, 4)
+
+
 def tri_boxes(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
     pos = np.ascontiguousarray(pos, np.float32)
     idx = np.ascontiguousarray(idx, np.uint32)
@@ -190,6 +201,9 @@ class OracleScene:
 
     def __init__(self, data, nodes: np.ndarray | None = None):
         self.data = data
+        if hasattr(data, "inst_mesh"):  # InstancedSceneData: two-level scene
+            self._init_instanced(data)
+            return
         self._keep = [np.ascontiguousarray(data.positions, np.float32),
                       np.ascontiguousarray(data.normals, np.float32),
                       np.ascontiguousarray(data.indices, np.uint32),
@@ -201,6 +215,29 @@ class OracleScene:
         p, n, i, m, mats = self._keep
         self.h = lib().orc_scene_create(_f(p), _f(n), len(p), _u(i), _u(m), len(i), _f(mats), len(mats),
                                         _f(self.nodes), len(self.nodes))
+
+    def _init_instanced(self, d):
+        pos = np.ascontiguousarray(np.concatenate([m[0] for m in d.meshes]), np.float32)
+        nrm = np.ascontiguousarray(np.concatenate([m[1] for m in d.meshes]), np.float32)
+        idx = np.ascontiguousarray(np.concatenate([m[2] for m in d.meshes]), np.uint32)
+        nv = np.array([len(m[0]) for m in d.meshes], np.int32)
+        nt = np.array([len(m[2]) for m in d.meshes], np.int32)
+        im = np.ascontiguousarray(d.inst_mesh, np.int32)
+        xf = np.ascontiguousarray(d.transforms, np.float32).reshape(-1, 12)
+        tm = np.ascontiguousarray(d.tri_material, np.uint32)
+        mats = np.ascontiguousarray(d.materials, np.float32)
+        self._keep = [pos, nrm, idx, nv, nt, im, xf, tm, mats]
+        i32 = lambda a: a.ctypes.data_as(i32p)
+        self.h = lib().orc_scene_create_instanced(len(nv), i32(nv), i32(nt), _f(pos), _f(nrm), _u(idx), len(im),
+                                                  i32(im), _f(xf), _u(tm), _f(mats), len(mats))
+        self.nodes = None
+
+    def tlas(self) -> np.ndarray:
+        """The bvh.cpp tree over the instances (12 floats per node)."""
+        n = lib().orc_scene_tlas(self.h, None)
+        out = np.empty((n, 12), np.float32)
+        lib().orc_scene_tlas(self.h, _f(out))
+        return out
 
     def __del__(self):
         try:

@@ -34,6 +34,28 @@ struct Counters {
 
 }  // namespace
 
+// One mesh in its own space with its bvh.cpp tree (a flat scene is one such
+// mesh in world space; an instanced scene has one per base mesh).
+struct OrcMesh {
+    std::vector<f3> pos, nrm;
+    std::vector<uint32_t> idx;
+    std::vector<float> nodes;  // 12 floats per node (3 x vec4)
+    int nnodes = 0;
+    std::vector<int32_t> leaf_order;
+};
+
+// Instance of a base mesh: row-major 3x4 transform (OptixInstance::transform,
+// include/amadeus/accelerator.cuh:204-212), its inverse, the global id of the
+// mesh's first triangle and its rank in the preorder of the instance tree.
+// Materials are per global triangle (orc_scene::tri_mat), as the reference
+// gives every entity's submesh its own material and its own GAS.
+struct OrcInst {
+    int mesh;
+    float M[12], Minv[12];
+    uint32_t prim_base;
+    int rank;
+};
+
 struct orc_scene {
     std::vector<f3> pos, nrm;
     std::vector<uint32_t> idx, tri_mat;
@@ -42,9 +64,31 @@ struct orc_scene {
     int nnodes = 0;
     std::vector<int32_t> leaf_order;
     std::vector<TriLight> lights;
+    // two-level scenes (Accelerator::update / build_tlas, accelerator.cuh:145-277)
+    bool instanced = false;
+    std::vector<OrcMesh> meshes;
+    std::vector<OrcInst> inst;
+    std::vector<float> tlas;  // bvh.cpp tree over the instances' world boxes
+    int ntlas = 0;
+    std::vector<int32_t> inst_order;  // instances in preorder of the instance tree
 };
 
 static inline f3 load3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+// glm mat4 x vec4 (column sums paired as (c0 x + c1 y) + (c2 z + c3 w)) for a
+// row-major 3x4 affine transform; w = 1 for points, 0 for directions.
+static inline f3 xform_point(const float *M, f3 p) {
+    return mk((M[0] * p.x + M[1] * p.y) + (M[2] * p.z + M[3] * 1.0f),
+              (M[4] * p.x + M[5] * p.y) + (M[6] * p.z + M[7] * 1.0f),
+              (M[8] * p.x + M[9] * p.y) + (M[10] * p.z + M[11] * 1.0f));
+}
+static inline f3 xform_dir(const float *M, f3 v) {
+    return mk((M[0] * v.x + M[1] * v.y) + (M[2] * v.z + M[3] * 0.0f),
+              (M[4] * v.x + M[5] * v.y) + (M[6] * v.z + M[7] * 0.0f),
+              (M[8] * v.x + M[9] * v.y) + (M[10] * v.z + M[11] * 0.0f));
+}
+
+
 
 orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv, const uint32_t *idx,
                             const uint32_t *tri_mat, int nt, const float *mats, int nm,
@@ -83,6 +127,126 @@ orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv, const ui
         }
     }
     return s;
+}
+
+// Inverse of a row-major 3x4 affine transform: cofactors of the 3x3 part in
+// FP64, translation -R^-1 t, rounded to float once.
+void orc_affine_inverse(const float *M, float *Minv) {
+    const double a = M[0], b = M[1], c = M[2], d = M[4], e = M[5], f = M[6], g = M[8], h = M[9], k = M[10];
+    const double A = e * k - f * h, B = -(d * k - f * g), C = d * h - e * g;
+    const double det = a * A + b * B + c * C;
+    const double r[9] = {A / det, -(b * k - c * h) / det, (b * f - c * e) / det,
+                         B / det, (a * k - c * g) / det, -(a * f - c * d) / det,
+                         C / det, -(a * h - b * g) / det, (a * e - b * d) / det};
+    const double t[3] = {M[3], M[7], M[11]};
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) Minv[4 * i + j] = (float)r[3 * i + j];
+        Minv[4 * i + 3] = (float)(-(r[3 * i] * t[0] + r[3 * i + 1] * t[1] + r[3 * i + 2] * t[2]));
+    }
+}
+
+// World box of an instance: the 8 corners of its mesh's root box through M,
+// widened by 1e-4 of the largest extent + 1e-5 of the largest coordinate so
+// that rounding of the ray transforms can never cull a hit.
+void orc_instance_box(const float *root_min, const float *root_max, const float *M, float *box6) {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int k = 0; k < 8; k++) {
+        f3 p = mk((k & 1) ? root_max[0] : root_min[0], (k & 2) ? root_max[1] : root_min[1],
+                  (k & 4) ? root_max[2] : root_min[2]);
+        f3 w = xform_point(M, p);
+        const float v[3] = {w.x, w.y, w.z};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = gmin(lo[a], v[a]);
+            hi[a] = gmax(hi[a], v[a]);
+        }
+    }
+    float ext = 0.0f, mag = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        ext = gmax(ext, hi[a] - lo[a]);
+        mag = gmax(mag, gmax(fabsf(lo[a]), fabsf(hi[a])));
+    }
+    const float pad = 1e-4f * ext + 1e-5f * mag;
+    for (int a = 0; a < 3; a++) {
+        box6[a] = lo[a] - pad;
+        box6[3 + a] = hi[a] + pad;
+    }
+}
+
+orc_scene *orc_scene_create_instanced(int nmesh, const int *mesh_nv, const int *mesh_nt, const float *pos,
+                                      const float *nrm, const uint32_t *idx, int ninst, const int *inst_mesh,
+                                      const float *inst_xform, const uint32_t *tri_mat, const float *mats, int nm) {
+    orc_scene *s = new orc_scene();
+    s->instanced = true;
+    s->meshes.resize(nmesh);
+    size_t vo = 0, to = 0;
+    for (int m = 0; m < nmesh; m++) {
+        OrcMesh &M = s->meshes[m];
+        M.pos.resize(mesh_nv[m]);
+        M.nrm.resize(mesh_nv[m]);
+        for (int i = 0; i < mesh_nv[m]; i++) {
+            M.pos[i] = load3(pos + 3 * (vo + i));
+            M.nrm[i] = load3(nrm + 3 * (vo + i));
+        }
+        M.idx.assign(idx + 3 * to, idx + 3 * (to + mesh_nt[m]));
+        std::vector<float> boxes(6 * (size_t)mesh_nt[m]);
+        orc_tri_boxes(pos + 3 * vo, M.idx.data(), mesh_nt[m], boxes.data());
+        M.nodes.resize(12 * (size_t)(2 * mesh_nt[m] - 1));
+        M.nnodes = orc_bvh_build(boxes.data(), mesh_nt[m], M.nodes.data());
+        M.leaf_order.resize(mesh_nt[m]);
+        orc_bvh_leaf_order(M.nodes.data(), M.nnodes, M.leaf_order.data());
+        vo += mesh_nv[m];
+        to += mesh_nt[m];
+    }
+    s->mats.resize(nm);
+    for (int m = 0; m < nm; m++) {
+        const float *q = mats + 12 * m;
+        Material &M = s->mats[m];
+        M.diffuse = load3(q);
+        M.specular = load3(q + 3);
+        M.emission = load3(q + 6);
+        M.roughness = q[9];
+        M.refraction = q[10];
+        M.type = (int)q[11];
+    }
+    s->inst.resize(ninst);
+    std::vector<float> ibox(6 * (size_t)ninst);
+    uint32_t base = 0;
+    for (int i = 0; i < ninst; i++) {
+        OrcInst &I = s->inst[i];
+        I.mesh = inst_mesh[i];
+        std::memcpy(I.M, inst_xform + 12 * (size_t)i, sizeof(I.M));
+        orc_affine_inverse(I.M, I.Minv);
+        I.prim_base = base;
+        base += (uint32_t)s->meshes[I.mesh].idx.size() / 3;
+        const float *root = s->meshes[I.mesh].nodes.data();
+        orc_instance_box(root + 4, root + 8, I.M, ibox.data() + 6 * (size_t)i);
+    }
+    s->tri_mat.assign(tri_mat, tri_mat + base);
+    s->tlas.resize(12 * (size_t)(2 * ninst - 1));
+    s->ntlas = orc_bvh_build(ibox.data(), ninst, s->tlas.data());
+    s->inst_order.resize(ninst);
+    orc_bvh_leaf_order(s->tlas.data(), s->ntlas, s->inst_order.data());
+    for (int r = 0; r < ninst; r++) s->inst[s->inst_order[r]].rank = r;
+    // triangle lights in instance order, then triangle order; vertices through
+    // the instance transform (Transform::apply, armada.cu:131-133)
+    for (int i = 0; i < ninst; i++) {
+        const OrcInst &I = s->inst[i];
+        const OrcMesh &m = s->meshes[I.mesh];
+        for (size_t t = 0; t < m.idx.size() / 3; t++) {
+            const Material &M = s->mats[s->tri_mat[I.prim_base + t]];
+            if (length(M.emission) > 0.0f) {
+                f3 a = xform_point(I.M, m.pos[m.idx[3 * t]]), b = xform_point(I.M, m.pos[m.idx[3 * t + 1]]),
+                   c = xform_point(I.M, m.pos[m.idx[3 * t + 2]]);
+                s->lights.push_back(TriLight{a, b - a, c - a, M.emission});
+            }
+        }
+    }
+    return s;
+}
+
+int orc_scene_tlas(const orc_scene *s, float *nodes) {
+    if (nodes) std::memcpy(nodes, s->tlas.data(), sizeof(float) * s->tlas.size());
+    return s->ntlas;
 }
 
 void orc_scene_destroy(orc_scene *s) { delete s; }
@@ -405,11 +569,30 @@ int orc_bvh_depth(const float *nodes, int nnodes) {
 // ---------------------------------------------------------------------------
 namespace {
 
-inline void tri_verts(const orc_scene *s, int prim, f3 &a, f3 &b, f3 &c) {
-    a = s->pos[s->idx[3 * prim]];
-    b = s->pos[s->idx[3 * prim + 1]];
-    c = s->pos[s->idx[3 * prim + 2]];
+// Triangles + bvh.cpp tree of one mesh (or of a flat scene).
+struct Geo {
+    const f3 *pos;
+    const uint32_t *idx;
+    const float *nodes;
+    int nnodes;
+    const int32_t *leaf_order;
+    int nt;
+};
+
+inline Geo geo_of(const orc_scene *s) {
+    return Geo{s->pos.data(), s->idx.data(), s->nodes.data(), s->nnodes, s->leaf_order.data(),
+               (int)s->leaf_order.size()};
 }
+inline Geo geo_of(const OrcMesh &m) {
+    return Geo{m.pos.data(), m.idx.data(), m.nodes.data(), m.nnodes, m.leaf_order.data(), (int)m.leaf_order.size()};
+}
+
+inline void tri_verts(const Geo &g, int prim, f3 &a, f3 &b, f3 &c) {
+    a = g.pos[g.idx[3 * prim]];
+    b = g.pos[g.idx[3 * prim + 1]];
+    c = g.pos[g.idx[3 * prim + 2]];
+}
+inline void tri_verts(const orc_scene *s, int prim, f3 &a, f3 &b, f3 &c) { tri_verts(geo_of(s), prim, a, b, c); }
 
 // shaders/rt/modules/bbox.glsl:8-57 (returns t entry or -1)
 inline float glsl_box(f3 o, f3 d, const float *mn, const float *mx) {
@@ -455,11 +638,12 @@ inline bool robust_box(f3 o, f3 d, const float *mn, const float *mx, float tlim)
 
 // Closest hit: lexicographic minimum of (t, preorder leaf rank) over triangles
 // with 0 < t < tmax.  mode 0 brute force in rank order, 1 GLSL walk, 2 robust walk.
-Hit closest(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
+Hit closest_geo(const Geo &s, f3 o, f3 d, float tmax, int mode, Counters &c) {
     Hit h{0.0f, 0.0f, 0.0f, -1};
     float best = tmax;
     if (mode == 0) {
-        for (int32_t prim : s->leaf_order) {
+        for (int k = 0; k < s.nt; k++) {
+            const int32_t prim = s.leaf_order[k];
             f3 a, b, cc;
             tri_verts(s, prim, a, b, cc);
             float b1 = 0.0f, b2 = 0.0f;
@@ -472,8 +656,8 @@ Hit closest(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
         }
         return h;
     }
-    const float *nodes = s->nodes.data();
-    int node = s->nnodes > 0 ? 0 : -1;
+    const float *nodes = s.nodes;
+    int node = s.nnodes > 0 ? 0 : -1;
     while (node != -1) {
         const float *hd = nodes + 4 * (size_t)node;
         int32_t obj = (int32_t)fbits(hd[1]);
@@ -506,9 +690,9 @@ Hit closest(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
 
 // Any hit with 0 < t < tmax (OPTIX_RAY_FLAG_TERMINATE_ON_FIRST_HIT,
 // amadeus_common.cuh:190-212).
-bool occluded(const orc_scene *s, f3 o, f3 d, float tmax, Counters &c) {
-    const float *nodes = s->nodes.data();
-    int node = s->nnodes > 0 ? 0 : -1;
+bool occluded_geo(const Geo &s, f3 o, f3 d, float tmax, Counters &c) {
+    const float *nodes = s.nodes;
+    int node = s.nnodes > 0 ? 0 : -1;
     while (node != -1) {
         const float *hd = nodes + 4 * (size_t)node;
         int32_t obj = (int32_t)fbits(hd[1]);
@@ -527,6 +711,84 @@ bool occluded(const orc_scene *s, f3 o, f3 d, float tmax, Counters &c) {
         }
     }
     return false;
+}
+
+// Two-level closest hit: the world ray walks the instance tree; at an
+// instance it is taken into the mesh's space by the inverse transform (t is
+// unchanged: the direction is not renormalised) and walks the mesh tree with
+// the best t so far.  Ties keep the first instance in instance-tree preorder,
+// then the first triangle in mesh preorder: the lexicographic minimum of
+// (t, instance rank, triangle rank).
+Hit closest_inst(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
+    Hit h{0.0f, 0.0f, 0.0f, -1};
+    float best = tmax;
+    auto visit = [&](int i) {
+        const OrcInst &I = s->inst[i];
+        Hit hm = closest_geo(geo_of(s->meshes[I.mesh]), xform_point(I.Minv, o), xform_dir(I.Minv, d), best,
+                             mode == 0 ? 0 : 2, c);
+        if (hm.prim >= 0) {
+            best = hm.t;
+            h = hm;
+            h.prim = (int)(I.prim_base + (uint32_t)hm.prim);
+        }
+    };
+    if (mode == 0) {
+        for (int32_t i : s->inst_order) visit(i);
+        return h;
+    }
+    const float *nodes = s->tlas.data();
+    int node = s->ntlas > 0 ? 0 : -1;
+    while (node != -1) {
+        const float *hd = nodes + 4 * (size_t)node;
+        int32_t obj = (int32_t)fbits(hd[1]);
+        c.nodes++;
+        if (obj != -1) {
+            visit(obj);
+            node = (int32_t)fbits(hd[3]);
+        } else {
+            bool go = robust_box(o, d, hd + 4, hd + 8, best);
+            node = go ? (int32_t)fbits(hd[2]) : (int32_t)fbits(hd[3]);
+        }
+    }
+    return h;
+}
+
+bool occluded_inst(const orc_scene *s, f3 o, f3 d, float tmax, Counters &c) {
+    const float *nodes = s->tlas.data();
+    int node = s->ntlas > 0 ? 0 : -1;
+    while (node != -1) {
+        const float *hd = nodes + 4 * (size_t)node;
+        int32_t obj = (int32_t)fbits(hd[1]);
+        c.nodes++;
+        if (obj != -1) {
+            const OrcInst &I = s->inst[obj];
+            if (occluded_geo(geo_of(s->meshes[I.mesh]), xform_point(I.Minv, o), xform_dir(I.Minv, d), tmax, c))
+                return true;
+            node = (int32_t)fbits(hd[3]);
+        } else {
+            bool go = robust_box(o, d, hd + 4, hd + 8, tmax);
+            node = go ? (int32_t)fbits(hd[2]) : (int32_t)fbits(hd[3]);
+        }
+    }
+    return false;
+}
+
+Hit closest(const orc_scene *s, f3 o, f3 d, float tmax, int mode, Counters &c) {
+    return s->instanced ? closest_inst(s, o, d, tmax, mode, c) : closest_geo(geo_of(s), o, d, tmax, mode, c);
+}
+bool occluded(const orc_scene *s, f3 o, f3 d, float tmax, Counters &c) {
+    return s->instanced ? occluded_inst(s, o, d, tmax, c) : occluded_geo(geo_of(s), o, d, tmax, c);
+}
+
+// Instance of a global triangle id (instances hold consecutive id ranges).
+inline int instance_of(const orc_scene *s, int prim) {
+    int lo = 0, hi = (int)s->inst.size() - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (s->inst[mid].prim_base <= (uint32_t)prim) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 }  // namespace
@@ -638,7 +900,34 @@ inline f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
     return (w * a + b1 * b) + b2 * c;
 }
 
+// Instanced hit: calculate_intersection / calculate_normal with the
+// instance's model matrix (amadeus_common.cuh:62-112): x = M (interp, 1),
+// edges and the interpolated normal by M (w = 0).
+bool load_surface_inst(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
+    const OrcInst &I = s->inst[instance_of(s, h.prim)];
+    const OrcMesh &m = s->meshes[I.mesh];
+    const int prim = h.prim - (int)I.prim_base;
+    const uint32_t *ix = &m.idx[3 * prim];
+    f3 a = m.pos[ix[0]], b = m.pos[ix[1]], c = m.pos[ix[2]];
+    f3 e1 = xform_dir(I.M, b - a), e2 = xform_dir(I.M, c - a);
+    f3 ng = normalize(cross(e1, e2));
+    if (dot(ng, dir) > 0.0f) {
+        ng = -ng;
+        sf.entering = false;
+    } else {
+        sf.entering = true;
+    }
+    ng = normalize(ng);
+    f3 nn = xform_dir(I.M, interp(m.nrm[ix[0]], m.nrm[ix[1]], m.nrm[ix[2]], h.b1, h.b2));
+    if (dot(nn, ng) < 0.0f) nn = -nn;
+    sf.n = normalize(nn);
+    sf.x = xform_point(I.M, interp(a, b, c, h.b1, h.b2));
+    sf.mat = s->mats[s->tri_mat[h.prim]];
+    return !isnan3(sf.n);
+}
+
 bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
+    if (s->instanced) return load_surface_inst(s, h, dir, sf);
     f3 a, b, c;
     tri_verts(s, h.prim, a, b, c);
     f3 e1 = b - a, e2 = c - a;

@@ -49,6 +49,20 @@ orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv,
                             const uint32_t *idx, const uint32_t *tri_mat, int nt,
                             const float *mats, int nm,
                             const float *bvh_nodes, int nnodes);
+/* Two-level scene (Accelerator::update / build_tlas,
+ * include/amadeus/accelerator.cuh:145-277): nmesh base meshes given as
+ * concatenated arrays (indices local to their mesh), ninst instances
+ * {mesh, row-major 3x4 transform}.  Builds every mesh's bvh.cpp tree and the
+ * bvh.cpp tree over the instances' padded world boxes (orc_instance_box).
+ * Triangle ids are global: instance i's triangles follow those of instances
+ * 0..i-1; tri_mat gives the material of every global triangle. */
+orc_scene *orc_scene_create_instanced(int nmesh, const int *mesh_nv, const int *mesh_nt, const float *pos,
+                                      const float *nrm, const uint32_t *idx, int ninst, const int *inst_mesh,
+                                      const float *inst_xform, const uint32_t *tri_mat, const float *mats, int nm);
+/* the instance tree (12 floats per node, returns the node count) */
+int orc_scene_tlas(const orc_scene *s, float *nodes);
+void orc_affine_inverse(const float *M, float *Minv);
+void orc_instance_box(const float *root_min, const float *root_max, const float *M, float *box6);
 void orc_scene_destroy(orc_scene *s);
 int orc_scene_light_count(const orc_scene *s);
 /* 12 floats per light: a, ab, ac, intensity (armada.cu:98-150 rule). */

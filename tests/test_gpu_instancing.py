@@ -1,0 +1,130 @@
+"""Two-level scenes on the GPU (krt_scene_upload_instanced) vs the oracle's
+two-level restatement: per-mesh bvh.cpp trees and the instance tree bit for
+bit, closest hits / occlusion bit for bit, path-tracer and ReSTIR DI frames
+at the same seeds."""
+import numpy as np
+import pytest
+
+import oracle as O
+from artemis_amd import ArmadaRTX, PathTracer, ReSTIR, ReSTIRGI, look_at, scenes
+from artemis_amd._lib import KrtError
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="two-level GPU path switched off: its first GPU run left a memory fault whose cause "
+                            "has not been found; krt_accel_build refuses instanced scenes until it is"),
+]
+
+
+@pytest.fixture(scope="module")
+def inst_pair(krt_ctx):
+    data = scenes.spheres(instances=24, slices=16, stacks=9, emissive=64, box=8.0, base_meshes=4, instanced=True)
+    sc = krt_ctx.upload(data)
+    yield data, sc, O.OracleScene(data)
+    sc.free()
+
+
+def rays_at_instances(data, n, seed):
+    rng = np.random.default_rng(seed)
+    centers = data.transforms[:, :, 3]
+    o = rng.uniform(-6, 6, (n, 3))
+    tgt = centers[rng.integers(len(centers), size=n)] + rng.normal(scale=0.4, size=(n, 3))
+    d = tgt - o
+    d[: n // 8] /= np.linalg.norm(d[: n // 8], axis=1, keepdims=True)
+    o[n // 8: n // 4] = tgt[n // 8: n // 4] + rng.normal(scale=0.05, size=(n // 4 - n // 8, 3))
+    axes = np.eye(3)[rng.integers(3, size=n // 16)] * rng.choice([-1.0, 1.0], size=(n // 16, 1))
+    d[n // 4: n // 4 + n // 16] = axes  # zero direction components
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 3] = 1e16
+    rays[:, 4:7] = d
+    rays[: n // 16, 3] = 5.0
+    return rays
+
+
+def test_trees_match_oracle(inst_pair):
+    data, sc, osc = inst_pair
+    for m, (pos, _, idx) in enumerate(data.meshes):
+        want = O.bvh_build(O.tri_boxes(pos, idx))
+        assert np.array_equal(sc.mesh_serialized(m).view(np.uint32), want.view(np.uint32)), m
+    assert np.array_equal(sc.instance_tree().view(np.uint32), osc.tlas().view(np.uint32))
+    st = sc.stats()
+    assert st["triangles"] == data.triangles
+
+
+def test_closest_and_occluded_bit_exact(inst_pair):
+    data, sc, osc = inst_pair
+    rays = rays_at_instances(data, 20000, 5)
+    want, _ = osc.trace(rays, mode=0)
+    got = sc.trace_closest(rays)
+    bad = np.any(got.view(np.uint32) != want.view(np.uint32), axis=1)
+    print(f"  {int(bad.sum())} of {len(rays)} hits differ; {np.mean(want[:, 3].view(np.int32) >= 0):.3f} hit",
+          flush=True)
+    assert not bad.any()
+    occ = sc.trace_occluded(rays)
+    assert np.array_equal(occ, osc.occluded(rays))
+    assert np.array_equal(occ, (want[:, 3].view(np.int32) >= 0).astype(np.int32))
+
+
+def test_lights_match_oracle(inst_pair):
+    _, sc, osc = inst_pair
+    got, want = sc.lights(), osc.lights()
+    assert len(want) == 64 and np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("layout", ["sah", "reference"])
+def test_path_tracer_frames(krt_ctx, inst_pair, layout):
+    data, sc, osc = inst_pair
+    sc.build(layout=layout)
+    W, H = 48, 40
+    cam = look_at(**data.camera, aspect=W / H)
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("pt", PathTracer(krt_ctx))
+    arm.set_depth(5)
+    for _ in range(3):
+        arm.render(sc, cam)
+    ref, _ = osc.render_pt(cam, W, H, frames=3, max_depth=5)
+    g = arm.color_buffer()
+    mad = float(np.abs(g.astype(np.float64) - ref[0]).mean())
+    ex = float(np.mean(np.all(g.view(np.uint32) == ref[0].view(np.uint32), axis=-1)))
+    print(f"  [{layout}] color MAD {mad:.3e} exact {ex:.4f}", flush=True)
+    assert mad <= 1e-3 and ex >= 0.99
+    for k in (1, 2, 3):
+        assert np.array_equal(arm._read(k).view(np.uint32), ref[k].view(np.uint32)), k
+    sc.build(layout="sah")
+
+
+def test_restir_frames(krt_ctx, inst_pair):
+    data, sc, osc = inst_pair
+    W, H = 40, 32
+    cam = look_at(**data.camera, aspect=W / H)
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("restir", ReSTIR(krt_ctx))
+    arm.set_option("m_cap", 20)
+    arm.set_option("spatial_passes", 1)
+    ors = O.OracleReSTIR(W, H)
+    for _ in range(2):
+        arm.render(sc, cam)
+        ors.frame(osc, cam, max_depth=2, m_cap=20, spatial_passes=1)
+    for which in (0, 1):
+        g, r = arm.reservoirs(which), ors.reservoirs(which)
+        frac = float(np.mean(np.any(g.view(np.uint32) != r.view(np.uint32), axis=2)))
+        print(f"  reservoirs[{which}] differing pixels {frac:.4f}")
+        assert frac == 0.0
+    g = arm.color_buffer()
+    ex = float(np.mean(np.all(g.view(np.uint32) == ors.buffers[0].view(np.uint32), axis=-1)))
+    assert ex >= 0.99
+
+
+def test_flat_only_paths_refuse_instanced_scenes(krt_ctx, inst_pair):
+    data, sc, _ = inst_pair
+    cam = look_at(**data.camera)
+    arm = ArmadaRTX(krt_ctx, 16, 16)
+    arm.attach("gi", ReSTIRGI(krt_ctx))
+    with pytest.raises(KrtError):
+        arm.render(sc, cam)
+    arm2 = ArmadaRTX(krt_ctx, 16, 16)
+    arm2.attach("pt", PathTracer(krt_ctx))
+    arm2.set_option("radiance_cache", True)
+    with pytest.raises(KrtError):
+        arm2.render(sc, cam)
