@@ -323,6 +323,49 @@ def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: 
                      name=f"spheres{instances}x{ntb}")
 
 
+def spheres_lamps(instances: int = 256, slices: int = 64, stacks: int = 33, lamps: int = 4096, box: float = 20.0,
+                  seed: int = 2, lamp_size: float = 0.15, clearance: float = 0.3) -> SceneData:
+    """The instanced-sphere scene with its 4,096 emitters as separate lamps:
+    small equilateral triangles (side `lamp_size`, random orientation,
+    log-uniform emission in [0.1, 10]) at least `clearance` metres from every
+    sphere's bounding sphere.  No shading point comes closer to an emitter
+    than the clearance, so next-event estimation has finite variance (in
+    spheres() the emitters are triangles of the sphere meshes and the 1/R^2
+    term has none)."""
+    base = spheres(instances=instances, slices=slices, stacks=stacks, emissive=0, box=box, seed=seed)
+    rng = SplitMix64(seed + 1000)
+    nv = len(base.positions)
+    # bounding spheres of the instances (vertex ranges are consecutive per instance)
+    per = nv // instances
+    P = base.positions.reshape(instances, per, 3).astype(np.float64)
+    centers = P.mean(axis=1)
+    radii = np.linalg.norm(P - centers[:, None, :], axis=2).max(axis=1)
+    mats = [list(m) for m in base.materials]
+    pos, nrm, idx, tmat = [], [], [], []
+    placed = 0
+    while placed < lamps:
+        c = np.array([rng.uniform(-box / 2, box / 2) for _ in range(3)])
+        if np.any(np.linalg.norm(centers - c, axis=1) < radii + clearance + lamp_size):
+            continue
+        R = _rotation(rng)
+        tri = np.array([[0.0, 1.0, 0.0], [-0.866, -0.5, 0.0], [0.866, -0.5, 0.0]]) * (lamp_size / 1.732)
+        v = tri @ R.T + c
+        n = np.cross(v[1] - v[0], v[2] - v[0])
+        n /= np.linalg.norm(n)
+        e = float(np.exp(np.log(0.1) + (np.log(10.0) - np.log(0.1)) * rng.uniform()))
+        mats.append(material((0.5, 0.5, 0.5), 1.0, emission=(e, e, e)))
+        pos.append(v)
+        nrm.append(np.tile(n, (3, 1)))
+        idx.append(np.arange(3) + nv + 3 * placed)
+        tmat.append(len(mats) - 1)
+        placed += 1
+    return SceneData(np.concatenate([base.positions, np.concatenate(pos).astype(np.float32)]),
+                     np.concatenate([base.normals, np.concatenate(nrm).astype(np.float32)]),
+                     np.concatenate([base.indices, np.array(idx, np.uint32)]).astype(np.uint32),
+                     np.concatenate([base.tri_material, np.array(tmat, np.uint32)]),
+                     np.asarray(mats, np.float32), camera=base.camera, name=f"spheres_lamps{instances}")
+
+
 def random_soup(n: int, seed: int = 3, extent: float = 1.0, size: float = 0.2) -> SceneData:
     """n independent random triangles (parity tests of the build and traversal)."""
     rng = SplitMix64(seed)
