@@ -326,12 +326,11 @@ def spheres(instances: int = 256, slices: int = 64, stacks: int = 33, emissive: 
 def spheres_lamps(instances: int = 256, slices: int = 64, stacks: int = 33, lamps: int = 4096, box: float = 20.0,
                   seed: int = 2, lamp_size: float = 0.15, clearance: float = 0.3) -> SceneData:
     """The instanced-sphere scene with its 4,096 emitters as separate lamps:
-    small equilateral triangles (side `lamp_size`, random orientation,
+    small black equilateral triangles (side `lamp_size`, random orientation,
     log-uniform emission in [0.1, 10]) at least `clearance` metres from every
-    sphere's bounding sphere.  No shading point comes closer to an emitter
-    than the clearance, so next-event estimation has finite variance (in
-    spheres() the emitters are triangles of the sphere meshes and the 1/R^2
-    term has none)."""
+    sphere's bounding sphere and from each other, so that next-event
+    estimation never sees an emitter closer than about the clearance (in
+    spheres() the emitters are triangles of the sphere meshes)."""
     base = spheres(instances=instances, slices=slices, stacks=stacks, emissive=0, box=box, seed=seed)
     rng = SplitMix64(seed + 1000)
     nv = len(base.positions)
@@ -343,17 +342,21 @@ def spheres_lamps(instances: int = 256, slices: int = 64, stacks: int = 33, lamp
     mats = [list(m) for m in base.materials]
     pos, nrm, idx, tmat = [], [], [], []
     placed = 0
+    lamp_c = np.zeros((lamps, 3))
     while placed < lamps:
         c = np.array([rng.uniform(-box / 2, box / 2) for _ in range(3)])
         if np.any(np.linalg.norm(centers - c, axis=1) < radii + clearance + lamp_size):
             continue
+        if placed and np.any(np.linalg.norm(lamp_c[:placed] - c, axis=1) < clearance + lamp_size):
+            continue
+        lamp_c[placed] = c
         R = _rotation(rng)
         tri = np.array([[0.0, 1.0, 0.0], [-0.866, -0.5, 0.0], [0.866, -0.5, 0.0]]) * (lamp_size / 1.732)
         v = tri @ R.T + c
         n = np.cross(v[1] - v[0], v[2] - v[0])
         n /= np.linalg.norm(n)
         e = float(np.exp(np.log(0.1) + (np.log(10.0) - np.log(0.1)) * rng.uniform()))
-        mats.append(material((0.5, 0.5, 0.5), 1.0, emission=(e, e, e)))
+        mats.append(material((0.0, 0.0, 0.0), 1.0, emission=(e, e, e)))
         pos.append(v)
         nrm.append(np.tile(n, (3, 1)))
         idx.append(np.arange(3) + nv + 3 * placed)
