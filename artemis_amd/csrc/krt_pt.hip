@@ -21,8 +21,16 @@ struct PtArgs {
     int *spill;
     unsigned long long *ctr;  // [0] closest, [1] shadow
     double *acc64;            // optional FP64 sums of samples, then of their squares (3 + 3 per pixel)
+    uint32_t *work;           // work-item counter of the launch (zeroed before it)
 };
 
+// Path regeneration on a persistent grid: a lane whose path has ended takes
+// the next pixel at once instead of idling until the longest path of its
+// wavefront ends.  Lanes that need work are counted with a wave ballot and
+// the wave takes that many consecutive work items with one atomic, so the
+// pixels a wave starts together come from one 8x8 tile.  Every pixel's
+// estimate is computed exactly as before; only its lane and start time
+// change.
 template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
@@ -30,28 +38,52 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
     const SceneView &sv = A.sv;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63u;
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
     History hist{A.hist + gtid, nthreads};
     TravStats ts;
     RayCount rc;
     const float count = (float)li.samples;
     const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-        f3 o, d;
-        make_ray(li, px, py, seed, li.jitter != 0, o, d);
-        f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
-        int nlev = 0;
-        for (int level = 0; level <= li.max_depth; level++) {
-            Hit h;
-            rc.closest++;
-            trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
-            if (h.prim < 0) break;  // __miss__: no environment map, value 0
-            Surface sf;
-            if (!load_surface<INST>(sv, h, d, sf)) break;
+    bool active = false, done = false;
+    uint32_t pix = 0;
+    int level = 0, nlev = 0;
+    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0);
+    f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
+    for (;;) {
+        // refill: one atomic per wavefront for all lanes without a path
+        const bool need = !active && !done;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                uint32_t px, py;
+                if (item >= tiles.items) {
+                    done = true;
+                } else if (tiles.pixel(item, px, py)) {
+                    pix = py * li.width + px;
+                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+                    make_ray(li, px, py, seed, li.jitter != 0, o, d);
+                    pos0 = n0 = alb0 = mk(0, 0, 0);
+                    level = 0;
+                    nlev = 0;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (!active) continue;
+        // one level of the path (amadeus_path_tracer.cu:133-201)
+        bool end = true;
+        Hit h;
+        rc.closest++;
+        trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
+        Surface sf;
+        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {  // a miss (no environment map) or a NaN normal end it
             Material mat = load_material(sv.mats, sf.mat_id);
             f3 n = sf.n;
             f3 x = offset_point(sf.x, n, mat.type);
@@ -70,16 +102,21 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
                 alb0 = mat.diffuse;
             }
             nlev = level + 1;
-            if (!has_pdf) break;
-            o = x;
-            d = wi;
+            if (has_pdf && level < li.max_depth) {
+                o = x;
+                d = wi;
+                level++;
+                end = false;
+            }
         }
+        if (!end) continue;
+        // the path is complete: fold its levels from the deepest up
         f3 value = mk(0, 0, 0);
         for (int k = nlev - 1; k >= 0; k--) {
-            f3 direct, T;
+            f3 dk, T;
             bool has_pdf;
-            hist.get(k, direct, has_pdf, T);
-            f3 v = direct;
+            hist.get(k, dk, has_pdf, T);
+            f3 v = dk;
             if (has_pdf) v = v + T * value;
             value = v;
         }
@@ -100,6 +137,7 @@ __global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
         li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
         li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
         li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
+        active = false;
     }
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
@@ -118,7 +156,9 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, un
     uint32_t nthreads = nb * kTraceBlock;
     size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
     if (history.bytes < need) history.alloc(need);
-    PtArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, acc64};
+    uint32_t *work = ctx.counters.as<uint32_t>();
+    KRT_HIP(hipMemsetAsync(work, 0, sizeof(uint32_t), ctx.stream));
+    PtArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, acc64, work};
     hipEvent_t e0, e1;
     KRT_HIP(hipEventCreate(&e0));
     KRT_HIP(hipEventCreate(&e1));
