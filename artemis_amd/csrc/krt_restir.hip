@@ -134,6 +134,7 @@ struct RestirArgs {
     unsigned long long *ctr;
     int candidates, m_cap, neighbours, radius;
     CacheView cv;  // radiance cache for the indirect path (cv.enabled = 0: off)
+    uint32_t *work;  // initial pass: work-item counter (zeroed before the launch)
 };
 
 // G-buffer surface of a pixel (amadeus_restir.cu:230-240, :323-336); the
@@ -165,115 +166,145 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
     const float count = (float)li.samples;
     const unsigned total = sv.nl;
     const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-        f3 o, d;
-        make_ray(li, px, py, seed, false, o, d);
-        // The reservoir and the level-0 AOVs are written as soon as they are
-        // final (level 0), so they hold no registers through the indirect
-        // path; a pixel whose primary ray misses writes them after the loop.
-        bool level0_done = false;
-        const bool acc = li.accumulate != 0;
-        int nlev = 0;
-        // radiance cache: the indirect path stops at level cv.depth (>= 1)
-        const CacheView &cv = A.cv;
-        uint32_t slot = 0;
-        const bool train = cv.enabled && cv.training(pix, slot);
-        bool queried = false;
-        f3 le_c = mk(0, 0, 0);
-        for (int level = 0; level <= li.max_depth; level++) {
-            Hit h;
-            rc.closest++;
-            trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
-            if (h.prim < 0) break;
-            Surface sf;
-            if (!load_surface<INST>(sv, h, d, sf)) break;
+    const CacheView &cv = A.cv;
+    const bool acc = li.accumulate != 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    // Path regeneration (as the path tracer's kernel): a lane whose path has
+    // ended takes the next pixel at once; the wave takes its refills with one
+    // atomic, consecutive work items in 8x8-tile order.  The reservoir and the
+    // level-0 AOVs are written as soon as they are final (level 0); a pixel
+    // whose primary ray misses writes them when its path ends.
+    bool active = false, done = false;
+    uint32_t pix = 0, slot = 0;
+    int level = 0, nlev = 0;
+    bool level0_done = false, train = false, queried = false;
+    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), le_c = mk(0, 0, 0);
+    for (;;) {
+        const bool need = !active && !done;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                uint32_t px, py;
+                if (item >= tiles.items) {
+                    done = true;
+                } else if (tiles.pixel(item, px, py)) {
+                    pix = py * li.width + px;
+                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+                    make_ray(li, px, py, seed, false, o, d);
+                    level = 0;
+                    nlev = 0;
+                    level0_done = false;
+                    // radiance cache: the indirect path stops at level cv.depth (>= 1)
+                    slot = 0;
+                    train = cv.enabled && cv.training(pix, slot);
+                    queried = false;
+                    le_c = mk(0, 0, 0);
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (!active) continue;
+        bool end = true;
+        Hit h;
+        rc.closest++;
+        trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
+        Surface sf;
+        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {
             Material mat = load_material(sv.mats, sf.mat_id);
+            bool cache_stop = false;
             if (cv.enabled && level == cv.depth) {
                 put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
                           sf.n, mat);
                 queried = true;
                 le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
-                if (!train) break;
+                cache_stop = !train;
             }
-            f3 n = sf.n;
-            f3 x = offset_point(sf.x, n, mat.type);
-            SurfaceHit sh{mat, sf.entering, n, -d, x};
-            f3 direct = mk(0, 0, 0);
-            if (level == 0) {
-                Resv res = A.cur.load(pix);
-                res.w = 0.0f;
-                res.M = 0;
-                res.W = 0.0f;
-                for (int i = 0; i < A.candidates; i++) {
-                    // sample_direct (amadeus_common.cuh:301-377), triangle lights
-                    RSample smp;
-                    smp.Le = smp.normal = smp.point = mk(0, 0, 0);
-                    smp.type = 0;
-                    float pdf = 0.0f;
-                    if (total > 0) {
-                        unsigned li_ = pick_light(seed, total);
-                        TriLight L = load_light(sv.lights, li_);
-                        smp.point = sample_area_light(L, seed);
-                        smp.Le = L.intensity;
-                        smp.normal = L.normal;
-                        smp.type = 1;
-                        pdf = 1.0f / (L.area * (float)total);
-                    } else {
-                        rand_uniform(seed);
+            if (!cache_stop) {
+                f3 n = sf.n;
+                f3 x = offset_point(sf.x, n, mat.type);
+                SurfaceHit sh{mat, sf.entering, n, -d, x};
+                f3 direct = mk(0, 0, 0);
+                if (level == 0) {
+                    Resv res = A.cur.load(pix);
+                    res.w = 0.0f;
+                    res.M = 0;
+                    res.W = 0.0f;
+                    for (int i = 0; i < A.candidates; i++) {
+                        // sample_direct (amadeus_common.cuh:301-377), triangle lights
+                        RSample smp;
+                        smp.Le = smp.normal = smp.point = mk(0, 0, 0);
+                        smp.type = 0;
+                        float pdf = 0.0f;
+                        if (total > 0) {
+                            unsigned li_ = pick_light(seed, total);
+                            TriLight L = load_light(sv.lights, li_);
+                            smp.point = sample_area_light(L, seed);
+                            smp.Le = L.intensity;
+                            smp.normal = L.normal;
+                            smp.type = 1;
+                            pdf = 1.0f / (L.area * (float)total);
+                        } else {
+                            rand_uniform(seed);
+                        }
+                        f3 D = smp.point - sh.x;
+                        float dd = length(D);
+                        f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
+                        float tgt = Li.x + Li.y + Li.z;
+                        res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
                     }
+                    RSample smp = res.data;
                     f3 D = smp.point - sh.x;
                     float dd = length(D);
-                    f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
-                    float tgt = Li.x + Li.y + Li.z;
-                    res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
+                    bool occ;
+                    f3 Li = direct_occ<false, INST>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
+                    res.resample(Li.x + Li.y + Li.z);
+                    // the reference traces this same visibility ray a second time
+                    res.W *= 1.0f - (occ ? 1.0f : 0.0f);
+                    A.cur.store(pix, res);
+                    A.gmat[pix] = sf.mat_id;
+                } else {
+                    direct = mat.emission + Ld<false, INST>(sv, sh, seed, stk, ts, rc);
                 }
-                RSample smp = res.data;
-                f3 D = smp.point - sh.x;
-                float dd = length(D);
-                bool occ;
-                f3 Li = direct_occ<false, INST>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
-                res.resample(Li.x + Li.y + Li.z);
-                // the reference traces this same visibility ray a second time
-                res.W *= 1.0f - (occ ? 1.0f : 0.0f);
-                A.cur.store(pix, res);
-                A.gmat[pix] = sf.mat_id;
-            } else {
-                direct = mat.emission + Ld<false, INST>(sv, sh, seed, stk, ts, rc);
+                int out;
+                f3 wi;
+                float pdf;
+                f3 f = eval_bsdf(sh, wi, pdf, out, seed);
+                bool has_pdf = pdf > 0.0f;
+                f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
+                hist.put(level, direct, true, T);  // value = direct + T * indirect
+                if (level == 0) {
+                    li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
+                    li.albedo[pix] = accumulate4(
+                        li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z, 0.0f), acc, count);
+                    li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
+                    level0_done = true;
+                }
+                nlev = level + 1;
+                if (has_pdf && level < li.max_depth) {
+                    o = x;
+                    d = wi;
+                    level++;
+                    end = false;
+                }
             }
-            int out;
-            f3 wi;
-            float pdf;
-            f3 f = eval_bsdf(sh, wi, pdf, out, seed);
-            bool has_pdf = pdf > 0.0f;
-            f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
-            hist.put(level, direct, true, T);  // value = direct + T * indirect
-            if (level == 0) {
-                li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
-                li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z,
-                                                                         0.0f),
-                                             acc, count);
-                li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
-                level0_done = true;
-            }
-            nlev = level + 1;
-            if (!has_pdf) break;
-            o = x;
-            d = wi;
         }
+        if (!end) continue;
         f3 value = mk(0, 0, 0);
         if (queried && !train) {
             // value = P + Theta * (Le_c + cache); the cache term is folded in
             // after inference
             f3 Th = mk(1.0f, 1.0f, 1.0f);
             for (int k = nlev - 1; k >= 0; k--) {
-                f3 direct, T;
+                f3 dk, T;
                 bool add;
-                hist.get(k, direct, add, T);
-                value = direct + T * value;
+                hist.get(k, dk, add, T);
+                value = dk + T * value;
                 Th = T * Th;
             }
             value = value + Th * le_c;
@@ -281,10 +312,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
         } else {
             f3 target = mk(0, 0, 0);
             for (int k = nlev - 1; k >= 0; k--) {
-                f3 direct, T;
+                f3 dk, T;
                 bool add;
-                hist.get(k, direct, add, T);
-                value = direct + T * value;
+                hist.get(k, dk, add, T);
+                value = dk + T * value;
                 if (cv.enabled && k == cv.depth) target = value - le_c;
             }
             if (cv.enabled) cv.theta[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -313,6 +344,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
             li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
             li.position[pix] = zero;
         }
+        active = false;
     }
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
@@ -573,6 +605,8 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.hist = st.history.as<float4>();
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
+    A.work = ctx.counters.as<uint32_t>();
+    KRT_HIP(hipMemsetAsync(A.work, 0, sizeof(uint32_t), ctx.stream));
     A.candidates = opt.candidates;
     A.m_cap = opt.m_cap;
     A.neighbours = opt.neighbours;
