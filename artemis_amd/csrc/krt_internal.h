@@ -192,6 +192,10 @@ struct RestirState {
     DevBuf intermediate;  // float4 per pixel
     DevBuf auxiliary;     // float4 per pixel
     DevBuf history;       // per-pixel path history scratch
+    // spatial pass, split around its shadow rays: ray list (2 float4 per ray),
+    // one occlusion byte per (ray slot, pixel), 1/Z weights per (neighbour,
+    // pixel), selection flags per pixel
+    DevBuf rays, occ, zm, flags;
     float prev_pv[16] = {0};
 };
 

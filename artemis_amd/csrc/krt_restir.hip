@@ -416,110 +416,264 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
     wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
 }
 
-// Four waves per SIMD (128 VGPRs, no spills): the pass is latency-bound on
-// its eleven shadow walks per pixel, so waves in flight matter most.
-template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void spatial_kernel(RestirArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+// The spatial pass (__raygen__spatial, amadeus_restir.cu:317-434) split around
+// its shadow rays.  Per pixel the reference traces one ray towards every
+// non-empty neighbour's sample (the resampling target), then one from every
+// non-empty neighbour's surface towards the merged sample (1/Z), then the
+// shading ray.  Here:
+//   spatial_rays_kernel     neighbour picks -> ray list (slot i = neighbour i)
+//   shadow_list_kernel      any-hit walks of the list -> occ[slot][pixel]
+//   spatial_merge_kernel    resampling with those visibilities; 1/Z rays
+//                           (slot i) and the shading ray (slot S) -> ray list
+//   shadow_list_kernel
+//   spatial_resolve_kernel  Z, W, shading, accumulation
+// Every ray has the origin, direction and extent of the reference's, walked
+// by the same traversal, so each visibility is the one the fused loop saw;
+// the walks run in a register-lean kernel at eight waves per SIMD instead of
+// inside the shading code's 128-VGPR frame.
+//
+// The neighbour picks are replayed from the pixel's auxiliary seed in each
+// kernel: a pick draws three numbers and every non-empty neighbour's
+// reservoir update one more, whatever its visibility, so the seed sequence is
+// known before any ray is traced.
+
+struct RayList {
+    float4 *rays;     // {origin, tmax}, {direction, slot as uint bits}
+    uint32_t *count;  // rays appended (zeroed before the producing kernel)
+};
+
+// Wave-aggregated append: one atomic per wavefront; every lane of the wave
+// must call it (lanes without a ray pass valid = false).
+KRT_DEV void emit_ray(const RayList &L, bool valid, f3 o, f3 d, float tmax, uint32_t slot) {
+    const unsigned long long want = __ballot(valid);
+    if (!want) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)want) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(L.count, (uint32_t)__popcll(want));
+    base = __shfl(base, leader, 64);
+    if (valid) {
+        const uint32_t k = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        L.rays[2 * (size_t)k] = make_float4(o.x, o.y, o.z, tmax);
+        L.rays[2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
+    }
+}
+
+struct SpatialBufs {
+    RayList list;
+    uint8_t *occ;    // [slot * npix + pixel]: 1 when the slot's ray was blocked
+    int32_t *zm;     // [i * npix + pixel]: neighbour i's M when its unoccluded 1/Z target is > 0, else 0
+    uint8_t *flags;  // per pixel: 1 sample selected from a neighbour whose ray is known, 2 that ray was blocked
+};
+
+constexpr int kSpatialBlock = 256;
+
+// Neighbour picks -> the resampling-target rays (origin: this pixel's
+// surface, towards the neighbour's sample).
+__global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs A, SpatialBufs B) {
     const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px = 0, py = 0, pix = 0;
+    bool live = item < tiles.items && tiles.pixel(item, px, py);
+    if (live) {
+        pix = py * li.width + px;
+        live = A.cur.M(pix) > 0;
+    }
+    f3 x = mk(0, 0, 0), rs = mk(0, 0, 0);
+    if (live) {
+        const float4 xx = li.position[pix];
+        x = mk(xx.x, xx.y, xx.z);
+        rs = ld3(A.auxiliary[pix]);
+        rand_uniform(rs);  // the current reservoir's update
+    }
+    for (int i = 0; i < A.neighbours; i++) {
+        bool valid = false;
+        f3 dir = mk(0, 0, 0);
+        float tmax = 0.0f;
+        if (live) {
+            const uint32_t i0 = spatial_neighbour_px(li, pix, rs, A.radius);
+            if (A.cur.M(i0) > 0) {
+                rand_uniform(rs);  // that neighbour's update
+                const float4 c = A.cur.p[2 * (size_t)A.cur.n + i0];
+                const f3 Dn = mk(c.x, c.y, c.z) - x;
+                const float dn = length(Dn);
+                dir = Dn / dn;
+                tmax = dn - kShadowEps;
+                valid = true;
+            }
+        }
+        emit_ray(B.list, valid, x, dir, tmax, (uint32_t)i * npix + pix);
+    }
+}
+
+// Any-hit walks of a ray list (occluded(), amadeus_common.cuh:467-478): the
+// list's count is read on the device; grid-stride over the rays.
+template <bool INST>
+__global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, const float4 *__restrict__ rays,
+                                                                  const uint32_t *__restrict__ count,
+                                                                  uint8_t *__restrict__ occ, int *spill,
+                                                                  unsigned long long *ctr) {
+    __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepthLean, spill + gtid, (int)nthreads};
     TravStats ts;
-    RayCount rc;
-    const float count = (float)li.samples;
-    const int S = A.neighbours;
-    const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        SurfaceHit hit = gbuffer_hit(A, pix);
-        Resv current = A.cur.load(pix);
-        f3 direct = mk(0, 0, 0);
-        if (current.M > 0) {
-            Resv merged;
-            merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
-            merged.data.type = 0;
-            merged.w = merged.W = 0.0f;
-            merged.M = 0;
-            merged.seed = ld3(A.auxiliary[pix]);
-            float t_current = target(current.data, hit);
-            merged.update(current.data, t_current * current.W * (float)current.M);
-            // The 1/Z loop below replays the neighbour picks from this seed
-            // state (re-derived there from the auxiliary seed) instead of
-            // keeping indices and sizes of the neighbours in registers across
-            // the traversals; the cur planes are read-only in this pass, so
-            // the replay sees the same M.
-            int cnt = 0;
-            // Visibility of the selected sample from this pixel, when one of
-            // the neighbour rays below already traced it (same origin,
-            // direction and extent as the final shading ray).
-            bool sel_known = false, sel_occ = false;
-            for (int i = 0; i < S; i++) {
-                uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
-                Resv nb = A.cur.load(i0);
-                // the reference traces this ray also for an empty
-                // neighbour, whose target is then unused
-                if (nb.M > 0) {
-                    f3 Dn = nb.data.point - hit.x;
-                    float dn = length(Dn);
-                    bool occ;
-                    f3 Ln = direct_occ<true, INST>(A.sv, hit, nb.data, Dn / dn, dn, stk, ts, rc, &occ);
-                    float t_nb = Ln.x + Ln.y + Ln.z;
-                    if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
-                        sel_known = true;
-                        sel_occ = occ;
-                    }
-                }
-                cnt += nb.M;
-            }
-            merged.M = current.M + cnt;
-            int Z = current.M;
-            f3 rs = ld3(A.auxiliary[pix]);
-            rand_uniform(rs);  // the draw of the first merged.update
-            for (int i = 0; i < S; i++) {
-                const uint32_t i0 = spatial_neighbour_px(li, pix, rs, A.radius);
-                const int m = A.cur.M(i0);
-                if (m > 0) {  // `entering` carries neighbour validity
-                    rand_uniform(rs);  // the draw of merged.update above
-                    SurfaceHit nh;
-                    nh.mat = load_material(A.sv.mats, A.gmat[i0]);
-                    nh.entering = true;
-                    float4 nn = li.normal[i0], xx = li.position[i0];
-                    nh.n = mk(nn.x, nn.y, nn.z);
-                    nh.x = mk(xx.x, xx.y, xx.z);
-                    nh.wo = hit.wo;
-                    float t_nb = occluded_target<INST>(A.sv, merged.data, nh, stk, ts, rc);
-                    if (t_nb > 0.0f) Z += m;
-                }
-            }
-            float t_merged = target(merged.data, hit);
-            float denominator = t_merged * (float)Z;
-            merged.W = (denominator > 0.0f) ? merged.w / denominator : 0.0f;
-            f3 D = merged.data.point - hit.x;
-            float d = length(D);
-            f3 lighting;
-            if (sel_known) {
-                lighting = sel_occ ? mk(0.0f, 0.0f, 0.0f)
-                                   : direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
-            } else {
-                lighting = direct_occ<true, INST>(A.sv, hit, merged.data, D / d, d, stk, ts, rc);
-            }
-            // emission re-read here rather than held in registers through the walks
-            direct = load_material(A.sv.mats, A.gmat[pix]).emission + lighting * merged.W;
-            A.prev.store(pix, merged);
-        } else {
-            A.prev.reset(pix);
-        }
-        float4 im = A.intermediate[pix];
-        float4 col = make_float4(direct.x + im.x, direct.y + im.y, direct.z + im.z, 1.0f + im.w);
-        if (isnan(col.x) || isnan(col.y) || isnan(col.z) || isnan(col.w)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-        li.color[pix] = accumulate4(li.color[pix], col, li.accumulate != 0, count);
+    unsigned nr = 0;
+    const uint32_t n = *count;
+    for (uint32_t k = gtid; k < n; k += nthreads) {
+        const float4 a = rays[2 * (size_t)k], b = rays[2 * (size_t)k + 1];
+        const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, ts);
+        occ[__float_as_uint(b.w)] = r ? 1 : 0;
+        nr++;
     }
-    wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 6 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 9 : nullptr, ts.tris);
+    wave_count(ctr ? ctr + 3 : nullptr, nr);
+    wave_count(ctr ? ctr + 6 : nullptr, ts.nodes);
+    wave_count(ctr ? ctr + 9 : nullptr, ts.tris);
+}
+
+// Resampling over the neighbours with the traced visibilities; emits the 1/Z
+// rays (from each non-empty neighbour's surface towards the merged sample)
+// and, when the merged sample's visibility from this pixel is not already
+// known, the shading ray.  The merged reservoir (W still open) goes to the
+// destination planes.
+__global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs A, SpatialBufs B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    const int S = A.neighbours;
+    uint32_t px = 0, py = 0, pix = 0;
+    bool live = item < tiles.items && tiles.pixel(item, px, py);
+    if (live) {
+        pix = py * li.width + px;
+        live = A.cur.M(pix) > 0;
+    }
+    Resv merged;
+    merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
+    merged.data.type = 0;
+    merged.w = merged.W = 0.0f;
+    merged.M = 0;
+    merged.seed = mk(0, 0, 0);
+    SurfaceHit hit;
+    bool sel_known = false, sel_occ = false;
+    if (live) {
+        hit = gbuffer_hit(A, pix);
+        const Resv current = A.cur.load(pix);
+        merged.seed = ld3(A.auxiliary[pix]);
+        const float t_current = target(current.data, hit);
+        merged.update(current.data, t_current * current.W * (float)current.M);
+        int cnt = 0;
+        for (int i = 0; i < S; i++) {
+            const uint32_t i0 = spatial_neighbour_px(li, pix, merged.seed, A.radius);
+            const Resv nb = A.cur.load(i0);
+            if (nb.M > 0) {
+                const f3 Dn = nb.data.point - hit.x;
+                const float dn = length(Dn);
+                const bool occ = B.occ[(size_t)i * npix + pix] != 0;
+                const f3 Ln = occ ? mk(0.0f, 0.0f, 0.0f)
+                                  : direct_li(hit, nb.data.Le, nb.data.normal, nb.data.type, Dn / dn, dn);
+                const float t_nb = Ln.x + Ln.y + Ln.z;
+                if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
+                    sel_known = true;
+                    sel_occ = occ;
+                }
+            }
+            cnt += nb.M;
+        }
+        merged.M = current.M + cnt;
+        A.prev.store(pix, merged);
+        B.flags[pix] = (uint8_t)((sel_known ? 1 : 0) | (sel_occ ? 2 : 0));
+    }
+    // 1/Z rays: replay the picks from the auxiliary seed
+    f3 rs = mk(0, 0, 0);
+    if (live) {
+        rs = ld3(A.auxiliary[pix]);
+        rand_uniform(rs);
+    }
+    for (int i = 0; i < S; i++) {
+        bool valid = false;
+        f3 o = mk(0, 0, 0), dir = mk(0, 0, 0);
+        float tmax = 0.0f;
+        if (live) {
+            const uint32_t i0 = spatial_neighbour_px(li, pix, rs, A.radius);
+            const int m = A.cur.M(i0);
+            int zm = 0;
+            if (m > 0) {  // `entering` carries neighbour validity
+                rand_uniform(rs);
+                SurfaceHit nh;
+                nh.mat = load_material(A.sv.mats, A.gmat[i0]);
+                nh.entering = true;
+                const float4 nn = li.normal[i0], xx = li.position[i0];
+                nh.n = mk(nn.x, nn.y, nn.z);
+                nh.x = mk(xx.x, xx.y, xx.z);
+                nh.wo = hit.wo;
+                const f3 D = merged.data.point - nh.x;
+                const float d = length(D);
+                const f3 Li = direct_li(nh, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
+                zm = (Li.x + Li.y + Li.z > 0.0f) ? m : 0;
+                o = nh.x;
+                dir = D / d;
+                tmax = d - kShadowEps;
+                valid = true;
+            }
+            B.zm[(size_t)i * npix + pix] = zm;
+        }
+        emit_ray(B.list, valid, o, dir, tmax, (uint32_t)i * npix + pix);
+    }
+    bool valid = false;
+    f3 dir = mk(0, 0, 0);
+    float tmax = 0.0f;
+    if (live && !sel_known) {
+        const f3 D = merged.data.point - hit.x;
+        const float d = length(D);
+        dir = D / d;
+        tmax = d - kShadowEps;
+        valid = true;
+    }
+    emit_ray(B.list, valid, hit.x, dir, tmax, (uint32_t)S * npix + pix);
+}
+
+// Z from the 1/Z visibilities, W, shading with the merged sample, colour.
+__global__ __launch_bounds__(kSpatialBlock) void spatial_resolve_kernel(RestirArgs A, SpatialBufs B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const int S = A.neighbours;
+    const int Mc = A.cur.M(pix);
+    f3 direct = mk(0, 0, 0);
+    if (Mc > 0) {
+        const SurfaceHit hit = gbuffer_hit(A, pix);
+        Resv merged = A.prev.load(pix);
+        int Z = Mc;
+        for (int i = 0; i < S; i++) {
+            const size_t k = (size_t)i * npix + pix;
+            const int zm = B.zm[k];
+            if (zm > 0 && B.occ[k] == 0) Z += zm;
+        }
+        const float t_merged = target(merged.data, hit);
+        const float denominator = t_merged * (float)Z;
+        merged.W = (denominator > 0.0f) ? merged.w / denominator : 0.0f;
+        const f3 D = merged.data.point - hit.x;
+        const float d = length(D);
+        const uint8_t fl = B.flags[pix];
+        const bool occ = (fl & 1) ? (fl & 2) != 0 : B.occ[(size_t)S * npix + pix] != 0;
+        const f3 lighting = occ ? mk(0.0f, 0.0f, 0.0f)
+                                : direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
+        direct = hit.mat.emission + lighting * merged.W;
+        A.prev.store(pix, merged);
+    } else {
+        A.prev.reset(pix);
+    }
+    const float4 im = A.intermediate[pix];
+    float4 col = make_float4(direct.x + im.x, direct.y + im.y, direct.z + im.z, 1.0f + im.w);
+    if (isnan(col.x) || isnan(col.y) || isnan(col.z) || isnan(col.w)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+    li.color[pix] = accumulate4(li.color[pix], col, li.accumulate != 0, (float)li.samples);
 }
 
 __global__ void seed_kernel(Planes a, Planes b, uint32_t w, uint32_t h, uint32_t user) {
@@ -603,6 +757,9 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.intermediate = st.intermediate.as<float4>();
     A.auxiliary = st.auxiliary.as<float4>();
     A.hist = st.history.as<float4>();
+    // one spill slab serves the pass kernels (32-entry LDS stacks) and the
+    // spatial shadow-list walks (16 entries): size it for both first
+    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
     A.work = ctx.counters.as<uint32_t>();
@@ -640,13 +797,51 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     else temporal_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     mark();
-    for (int pass = 0; pass < opt.spatial_passes; pass++) {
-        A.cur = planes(st.cur);
-        A.prev = planes(1 - st.cur);
-        if (s.instanced) spatial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-        else spatial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-        KRT_HIP(hipGetLastError());
-        st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
+    if (opt.spatial_passes > 0) {
+        const int S = opt.neighbours;
+        const size_t slots = (size_t)(S + 1) * npix;
+        if (st.rays.bytes < sizeof(float4) * 2 * slots) st.rays.alloc(sizeof(float4) * 2 * slots);
+        if (st.occ.bytes < slots) st.occ.alloc(slots);
+        if (st.zm.bytes < sizeof(int32_t) * ((size_t)S * npix + 1)) st.zm.alloc(sizeof(int32_t) * ((size_t)S * npix + 1));
+        if (st.flags.bytes < npix) st.flags.alloc(npix);
+        SpatialBufs B;
+        B.list.rays = st.rays.as<float4>();
+        B.list.count = ctx.counters.as<uint32_t>() + 1;
+        B.occ = st.occ.as<uint8_t>();
+        B.zm = st.zm.as<int32_t>();
+        B.flags = st.flags.as<uint8_t>();
+        const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
+        const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;
+        const uint32_t lb = persistent_blocks((uint32_t)std::min<size_t>(slots, 0xffffffffu), kTraceBlock,
+                                              kMaxResidentBlocks);
+        int *lean_spill = A.spill;  // sized above for lb <= kMaxResidentBlocks workgroups
+        for (int pass = 0; pass < opt.spatial_passes; pass++) {
+            A.cur = planes(st.cur);
+            A.prev = planes(1 - st.cur);
+            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            spatial_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            if (s.instanced)
+                shadow_list_kernel<true><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
+                                                                            lean_spill, ctr);
+            else
+                shadow_list_kernel<false><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
+                                                                             lean_spill, ctr);
+            KRT_HIP(hipGetLastError());
+            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            spatial_merge_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            if (s.instanced)
+                shadow_list_kernel<true><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
+                                                                            lean_spill, ctr);
+            else
+                shadow_list_kernel<false><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
+                                                                             lean_spill, ctr);
+            KRT_HIP(hipGetLastError());
+            spatial_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
+        }
     }
     mark();
     if (cached) cache_train(ctx, *cs, A.cv);

@@ -84,8 +84,8 @@ void camera_rays_dev(Context &ctx, const LaunchInfo &li, int jitter, float *rays
     KRT_HIP(hipGetLastError());
 }
 
-int *spill_buffer(Context &ctx, Scene &s, uint32_t threads) {
-    size_t depth = s.accel.depth_bound > (uint32_t)kLdsDepth ? s.accel.depth_bound - kLdsDepth + 1 : 1;
+int *spill_buffer(Context &ctx, Scene &s, uint32_t threads, int lds_depth) {
+    size_t depth = s.accel.depth_bound > (uint32_t)lds_depth ? s.accel.depth_bound - lds_depth + 1 : 1;
     size_t need = sizeof(int) * depth * threads;
     if (ctx.spill.bytes < need) ctx.spill.alloc(need);
     return ctx.spill.as<int>();
