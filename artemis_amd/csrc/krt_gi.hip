@@ -22,6 +22,7 @@
 #include "krt_internal.h"
 #include "krt_launch.h"
 #include "krt_shade.h"
+#include "krt_shadow.h"
 #include "krt_traverse.h"
 
 namespace krt {
@@ -82,6 +83,7 @@ struct GPlanes {
         p[2 * n + i].w = 0.0f;
     }
     KRT_DEV void store_M(uint32_t i, int M) const { p[i].w = __int_as_float(M); }
+    KRT_DEV int M(uint32_t i) const { return __float_as_int(p[i].w); }
 };
 
 // G-buffer: {x, depth}, {n, valid}, material id
@@ -399,88 +401,193 @@ __global__ __launch_bounds__(kTraceBlock) void gi_temporal_kernel(GiArgs A) {
     wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void gi_spatial_kernel(GiArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
+// Spatial pass, split around its visibility rays as the ReSTIR DI spatial
+// pass is: gi_spatial_rays_kernel appends the x_v -> x_s rays of the
+// neighbours whose Jacobian and target are positive (slot i = neighbour i),
+// a shadow-list walk answers them, gi_spatial_merge_kernel resamples with
+// those answers and appends the 1/Z rays from each used neighbour's surface
+// towards the merged sample (slot k = k-th used neighbour), a second walk
+// answers them and gi_spatial_resolve_kernel forms Z, W and shades.  The seed
+// sequence does not depend on any visibility (a pick draws three numbers, an
+// accepted neighbour's update one), so every kernel replays the picks.
+struct GiSpatialBufs {
+    RayList list;
+    uint8_t *occ;  // [slot * npix + pixel]
+    int32_t *zm;   // [k * npix + pixel]: M of the k-th used neighbour when its 1/Z ray was cast, else 0
+};
+
+constexpr int kGiSpatialBlock = 256;
+
+__global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_rays_kernel(GiArgs A, GiSpatialBufs B) {
     const LaunchInfo &li = A.li;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    TravStats ts;
-    RayCount rc;
-    const int S = A.neighbours;
     const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        GPix gb = load_g(A.g, pix);
-        GRes R = A.cur.load(pix);
-        if (gb.valid && R.M > 0) {
-            GRes m;
-            m.s.xs = m.s.ns = m.s.Lo = mk(0, 0, 0);
-            m.xo = mk(0, 0, 0);
-            m.w = m.W = 0.0f;
-            m.M = 0;
-            float4 ax = A.aux[pix];
-            m.seed = mk(ax.x, ax.y, ax.z);
-            m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
-            uint32_t used[kGiMaxNeighbours];
-            int Ms[kGiMaxNeighbours];
-            int nu = 0, cnt = 0;
-#pragma unroll
-            for (int i = 0; i < kGiMaxNeighbours; i++) {
-                used[i] = pix;
-                Ms[i] = 0;
-            }
-            for (int i = 0; i < S; i++) {
-                uint32_t q = spatial_neighbour_px(li, pix, m.seed, A.radius);
-                if (q == pix) continue;
-                GPix gq = load_g(A.g, q);
-                if (!gi_similar(gb, gq)) continue;
-                GRes Nb = A.cur.load(q);
-                if (Nb.M <= 0) continue;
-                float J = gi_jacobian(Nb.xo, gb.x, Nb.s);
-                float t = 0.0f;
-                if (J > 0.0f && gi_target(Nb.s, gb.x, gb.n) > 0.0f && gi_visible(A.sv, gb.x, Nb.s.xs, stk, ts, rc))
-                    t = gi_target(Nb.s, gb.x, gb.n);
-                m.update(Nb.s, t * Nb.W * (float)Nb.M * J);
-#pragma unroll
-                for (int k = 0; k < kGiMaxNeighbours; k++)
-                    if (k == nu) {
-                        used[k] = q;
-                        Ms[k] = Nb.M;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px = 0, py = 0, pix = 0;
+    bool live = item < tiles.items && tiles.pixel(item, px, py);
+    GPix gb;
+    gb.valid = false;
+    gb.x = gb.n = mk(0, 0, 0);
+    gb.depth = 0.0f;
+    gb.mat = 0;
+    f3 seed = mk(0, 0, 0);
+    if (live) {
+        pix = py * li.width + px;
+        gb = load_g(A.g, pix);
+        live = gb.valid && A.cur.M(pix) > 0;
+    }
+    if (live) {
+        const float4 ax = A.aux[pix];
+        seed = mk(ax.x, ax.y, ax.z);
+        rand_uniform(seed);  // the pixel's own reservoir's update
+    }
+    for (int i = 0; i < A.neighbours; i++) {
+        bool valid = false;
+        f3 dir = mk(0, 0, 0);
+        float tmax = 0.0f;
+        if (live) {
+            const uint32_t q = spatial_neighbour_px(li, pix, seed, A.radius);
+            if (q != pix) {
+                const GPix gq = load_g(A.g, q);
+                if (gi_similar(gb, gq)) {
+                    const GRes Nb = A.cur.load(q);
+                    if (Nb.M > 0) {
+                        const float J = gi_jacobian(Nb.xo, gb.x, Nb.s);
+                        if (J > 0.0f && gi_target(Nb.s, gb.x, gb.n) > 0.0f) {
+                            const f3 D = Nb.s.xs - gb.x;
+                            const float d = length(D);
+                            dir = D / d;
+                            tmax = d - kShadowEps;
+                            valid = true;
+                        }
+                        rand_uniform(seed);  // that neighbour's update
                     }
-                nu++;
-                cnt += Nb.M;
-            }
-            int Z = R.M;
-#pragma unroll
-            for (int k = 0; k < kGiMaxNeighbours; k++) {
-                if (k < nu) {
-                    GPix gq = load_g(A.g, used[k]);
-                    if (gi_jacobian(gq.x, gb.x, m.s) > 0.0f && gi_target(m.s, gq.x, gq.n) > 0.0f &&
-                        gi_visible(A.sv, gq.x, m.s.xs, stk, ts, rc))
-                        Z += Ms[k];
                 }
             }
-            m.M = R.M + cnt;
-            float den = gi_target(m.s, gb.x, gb.n) * (float)Z;
-            m.W = (den > 0.0f) ? m.w / den : 0.0f;
-            m.xo = gb.x;
-            A.prev.store(pix, m);
-            if (A.shade) gi_shade(A, pix, gb, m);
-        } else {
-            A.prev.reset(pix);
-            if (A.shade) {
-                GRes z;
-                z.W = 0.0f;
-                gi_shade(A, pix, gb, z);
+        }
+        emit_ray(B.list, valid, gb.x, dir, tmax, (uint32_t)i * npix + pix);
+    }
+}
+
+__global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_merge_kernel(GiArgs A, GiSpatialBufs B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    const int S = A.neighbours;
+    uint32_t px = 0, py = 0, pix = 0;
+    bool live = item < tiles.items && tiles.pixel(item, px, py);
+    GPix gb;
+    gb.valid = false;
+    gb.x = gb.n = mk(0, 0, 0);
+    gb.depth = 0.0f;
+    gb.mat = 0;
+    if (live) {
+        pix = py * li.width + px;
+        gb = load_g(A.g, pix);
+        live = gb.valid && A.cur.M(pix) > 0;
+    }
+    GRes m;
+    m.s.xs = m.s.ns = m.s.Lo = mk(0, 0, 0);
+    m.xo = mk(0, 0, 0);
+    m.w = m.W = 0.0f;
+    m.M = 0;
+    m.seed = mk(0, 0, 0);
+    uint32_t used[kGiMaxNeighbours];
+    int Ms[kGiMaxNeighbours];
+    int nu = 0;
+#pragma unroll
+    for (int i = 0; i < kGiMaxNeighbours; i++) {
+        used[i] = pix;
+        Ms[i] = 0;
+    }
+    if (live) {
+        const GRes R = A.cur.load(pix);
+        const float4 ax = A.aux[pix];
+        m.seed = mk(ax.x, ax.y, ax.z);
+        m.update(R.s, gi_target(R.s, gb.x, gb.n) * R.W * (float)R.M);
+        int cnt = 0;
+        for (int i = 0; i < S; i++) {
+            const uint32_t q = spatial_neighbour_px(li, pix, m.seed, A.radius);
+            if (q == pix) continue;
+            const GPix gq = load_g(A.g, q);
+            if (!gi_similar(gb, gq)) continue;
+            const GRes Nb = A.cur.load(q);
+            if (Nb.M <= 0) continue;
+            const float J = gi_jacobian(Nb.xo, gb.x, Nb.s);
+            float t = 0.0f;
+            if (J > 0.0f && gi_target(Nb.s, gb.x, gb.n) > 0.0f && B.occ[(size_t)i * npix + pix] == 0)
+                t = gi_target(Nb.s, gb.x, gb.n);
+            m.update(Nb.s, t * Nb.W * (float)Nb.M * J);
+#pragma unroll
+            for (int k = 0; k < kGiMaxNeighbours; k++)
+                if (k == nu) {
+                    used[k] = q;
+                    Ms[k] = Nb.M;
+                }
+            nu++;
+            cnt += Nb.M;
+        }
+        m.M = R.M + cnt;
+        A.prev.store(pix, m);
+    }
+#pragma unroll
+    for (int k = 0; k < kGiMaxNeighbours; k++) {
+        if (k >= S) break;
+        bool valid = false;
+        f3 o = mk(0, 0, 0), dir = mk(0, 0, 0);
+        float tmax = 0.0f;
+        if (live) {
+            int zm = 0;
+            if (k < nu) {
+                const GPix gq = load_g(A.g, used[k]);
+                if (gi_jacobian(gq.x, gb.x, m.s) > 0.0f && gi_target(m.s, gq.x, gq.n) > 0.0f) {
+                    const f3 D = m.s.xs - gq.x;
+                    const float d = length(D);
+                    o = gq.x;
+                    dir = D / d;
+                    tmax = d - kShadowEps;
+                    valid = true;
+                    zm = Ms[k];
+                }
             }
+            B.zm[(size_t)k * npix + pix] = zm;
+        }
+        emit_ray(B.list, valid, o, dir, tmax, (uint32_t)k * npix + pix);
+    }
+}
+
+__global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_resolve_kernel(GiArgs A, GiSpatialBufs B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const GPix gb = load_g(A.g, pix);
+    const int RM = A.cur.M(pix);
+    if (gb.valid && RM > 0) {
+        GRes m = A.prev.load(pix);
+        int Z = RM;
+        for (int k = 0; k < A.neighbours; k++) {
+            const size_t j = (size_t)k * npix + pix;
+            const int zm = B.zm[j];
+            if (zm > 0 && B.occ[j] == 0) Z += zm;
+        }
+        const float den = gi_target(m.s, gb.x, gb.n) * (float)Z;
+        m.W = (den > 0.0f) ? m.w / den : 0.0f;
+        m.xo = gb.x;
+        A.prev.store(pix, m);
+        if (A.shade) gi_shade(A, pix, gb, m);
+    } else {
+        A.prev.reset(pix);
+        if (A.shade) {
+            GRes z;
+            z.W = 0.0f;
+            gi_shade(A, pix, gb, z);
         }
     }
-    wave_count(A.ctr ? A.ctr + 3 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 6 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 9 : nullptr, ts.tris);
 }
 
 __global__ void gi_seed_kernel(GPlanes a, GPlanes b, uint32_t w, uint32_t h, uint32_t user) {
@@ -568,6 +675,9 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     A.direct = st.direct.as<float4>();
     A.aux = st.aux.as<float4>();
     A.hist = st.history.as<float4>();
+    // one spill slab for the pass kernels (32-entry LDS stacks) and the
+    // spatial shadow-list walks (16 entries): size it for both first
+    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
     A.work = ctx.counters.as<uint32_t>();
@@ -588,13 +698,37 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     gi_temporal_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(ev[2], ctx.stream));
-    for (int pass = 0; pass < opt.spatial_passes; pass++) {
-        A.cur = planes(st.cur);
-        A.prev = planes(1 - st.cur);
-        A.shade = pass == opt.spatial_passes - 1;
-        gi_spatial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-        KRT_HIP(hipGetLastError());
-        st.cur = 1 - st.cur;
+    if (opt.spatial_passes > 0) {
+        const int S = opt.neighbours;
+        const size_t slots = (size_t)S * npix + 1;
+        if (st.rays.bytes < sizeof(float4) * 2 * slots) st.rays.alloc(sizeof(float4) * 2 * slots);
+        if (st.occ.bytes < slots) st.occ.alloc(slots);
+        if (st.zm.bytes < sizeof(int32_t) * slots) st.zm.alloc(sizeof(int32_t) * slots);
+        GiSpatialBufs B;
+        B.list.rays = st.rays.as<float4>();
+        B.list.count = ctx.counters.as<uint32_t>() + 1;
+        B.occ = st.occ.as<uint8_t>();
+        B.zm = st.zm.as<int32_t>();
+        const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
+        const uint32_t gb = (items + kGiSpatialBlock - 1) / kGiSpatialBlock;
+        const uint32_t lb = persistent_blocks((uint32_t)std::min<size_t>(slots, 0xffffffffu), kTraceBlock,
+                                              kMaxResidentBlocks);
+        for (int pass = 0; pass < opt.spatial_passes; pass++) {
+            A.cur = planes(st.cur);
+            A.prev = planes(1 - st.cur);
+            A.shade = pass == opt.spatial_passes - 1;
+            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            gi_spatial_rays_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, lb);
+            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            gi_spatial_merge_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, lb);
+            gi_spatial_resolve_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
+            KRT_HIP(hipGetLastError());
+            st.cur = 1 - st.cur;
+        }
     }
     KRT_HIP(hipEventRecord(ev[3], ctx.stream));
     st.cur = 1 - st.cur;

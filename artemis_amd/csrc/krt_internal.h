@@ -207,6 +207,7 @@ struct GiState {
     int gcur = 0;
     DevBuf direct, aux;
     DevBuf history;
+    DevBuf rays, occ, zm;  // spatial pass: shadow-ray list, occlusion bytes, 1/Z weights
     float prev_pv[16] = {0};
 };
 

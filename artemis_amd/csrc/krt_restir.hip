@@ -20,6 +20,7 @@
 #include "krt_internal.h"
 #include "krt_launch.h"
 #include "krt_shade.h"
+#include "krt_shadow.h"
 #include "krt_traverse.h"
 
 namespace krt {
@@ -437,28 +438,6 @@ __global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
 // reservoir update one more, whatever its visibility, so the seed sequence is
 // known before any ray is traced.
 
-struct RayList {
-    float4 *rays;     // {origin, tmax}, {direction, slot as uint bits}
-    uint32_t *count;  // rays appended (zeroed before the producing kernel)
-};
-
-// Wave-aggregated append: one atomic per wavefront; every lane of the wave
-// must call it (lanes without a ray pass valid = false).
-KRT_DEV void emit_ray(const RayList &L, bool valid, f3 o, f3 d, float tmax, uint32_t slot) {
-    const unsigned long long want = __ballot(valid);
-    if (!want) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __ffsll((unsigned long long)want) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(L.count, (uint32_t)__popcll(want));
-    base = __shfl(base, leader, 64);
-    if (valid) {
-        const uint32_t k = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-        L.rays[2 * (size_t)k] = make_float4(o.x, o.y, o.z, tmax);
-        L.rays[2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
-    }
-}
-
 struct SpatialBufs {
     RayList list;
     uint8_t *occ;    // [slot * npix + pixel]: 1 when the slot's ray was blocked
@@ -506,31 +485,6 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
         }
         emit_ray(B.list, valid, x, dir, tmax, (uint32_t)i * npix + pix);
     }
-}
-
-// Any-hit walks of a ray list (occluded(), amadeus_common.cuh:467-478): the
-// list's count is read on the device; grid-stride over the rays.
-template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, const float4 *__restrict__ rays,
-                                                                  const uint32_t *__restrict__ count,
-                                                                  uint8_t *__restrict__ occ, int *spill,
-                                                                  unsigned long long *ctr) {
-    __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepthLean, spill + gtid, (int)nthreads};
-    TravStats ts;
-    unsigned nr = 0;
-    const uint32_t n = *count;
-    for (uint32_t k = gtid; k < n; k += nthreads) {
-        const float4 a = rays[2 * (size_t)k], b = rays[2 * (size_t)k + 1];
-        const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, ts);
-        occ[__float_as_uint(b.w)] = r ? 1 : 0;
-        nr++;
-    }
-    wave_count(ctr ? ctr + 3 : nullptr, nr);
-    wave_count(ctr ? ctr + 6 : nullptr, ts.nodes);
-    wave_count(ctr ? ctr + 9 : nullptr, ts.tris);
 }
 
 // Resampling over the neighbours with the traced visibilities; emits the 1/Z
@@ -821,23 +775,11 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
             KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
             spatial_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            if (s.instanced)
-                shadow_list_kernel<true><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
-                                                                            lean_spill, ctr);
-            else
-                shadow_list_kernel<false><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
-                                                                             lean_spill, ctr);
-            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, lb);
             KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
             spatial_merge_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            if (s.instanced)
-                shadow_list_kernel<true><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
-                                                                            lean_spill, ctr);
-            else
-                shadow_list_kernel<false><<<lb, kTraceBlock, 0, ctx.stream>>>(A.sv, B.list.rays, B.list.count, B.occ,
-                                                                             lean_spill, ctr);
-            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, lb);
             spatial_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
