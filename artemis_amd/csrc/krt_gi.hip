@@ -419,6 +419,9 @@ struct GiSpatialBufs {
 constexpr int kGiSpatialBlock = 256;
 
 __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_rays_kernel(GiArgs A, GiSpatialBufs B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -465,11 +468,15 @@ __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_rays_kernel(GiArgs
                 }
             }
         }
-        emit_ray(B.list, valid, gb.x, dir, tmax, (uint32_t)i * npix + pix);
+        emit_ray(B.list, &nray, valid, gb.x, dir, tmax, (uint32_t)i * npix + pix);
     }
+    publish_rays(B.list, &nray);
 }
 
 __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_merge_kernel(GiArgs A, GiSpatialBufs B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -553,8 +560,9 @@ __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_merge_kernel(GiArg
             }
             B.zm[(size_t)k * npix + pix] = zm;
         }
-        emit_ray(B.list, valid, o, dir, tmax, (uint32_t)k * npix + pix);
+        emit_ray(B.list, &nray, valid, o, dir, tmax, (uint32_t)k * npix + pix);
     }
+    publish_rays(B.list, &nray);
 }
 
 __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_resolve_kernel(GiArgs A, GiSpatialBufs B) {
@@ -676,7 +684,10 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     A.aux = st.aux.as<float4>();
     A.hist = st.history.as<float4>();
     // one spill slab for the pass kernels (32-entry LDS stacks) and the
-    // spatial shadow-list walks (16 entries): size it for both first
+    // spatial shadow-list walks (16 entries, one workgroup per list
+    // segment): size it for both first
+    const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
+    const uint32_t gb = (items + kGiSpatialBlock - 1) / kGiSpatialBlock;  // spatial producers = list segments
     spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
@@ -700,31 +711,33 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     KRT_HIP(hipEventRecord(ev[2], ctx.stream));
     if (opt.spatial_passes > 0) {
         const int S = opt.neighbours;
+        const size_t list_slots = (size_t)gb * kGiSpatialBlock * (size_t)S + 1;
         const size_t slots = (size_t)S * npix + 1;
-        if (st.rays.bytes < sizeof(float4) * 2 * slots) st.rays.alloc(sizeof(float4) * 2 * slots);
+        if (st.rays.bytes < sizeof(float4) * 2 * list_slots) st.rays.alloc(sizeof(float4) * 2 * list_slots);
+        if (st.seg_counts.bytes < sizeof(uint32_t) * gb) st.seg_counts.alloc(sizeof(uint32_t) * gb);
+        if (st.seg_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.seg_prefix.alloc(sizeof(uint32_t) * (gb + 1));
+        if (st.dense.bytes < sizeof(float4) * 2 * list_slots) st.dense.alloc(sizeof(float4) * 2 * list_slots);
         if (st.occ.bytes < slots) st.occ.alloc(slots);
         if (st.zm.bytes < sizeof(int32_t) * slots) st.zm.alloc(sizeof(int32_t) * slots);
         GiSpatialBufs B;
         B.list.rays = st.rays.as<float4>();
-        B.list.count = ctx.counters.as<uint32_t>() + 1;
+        B.list.counts = st.seg_counts.as<uint32_t>();
+        B.list.prefix = st.seg_prefix.as<uint32_t>();
+        B.list.dense = st.dense.as<float4>();
+        B.list.seg = kGiSpatialBlock * (uint32_t)S;
+        B.list.nseg = gb;
         B.occ = st.occ.as<uint8_t>();
         B.zm = st.zm.as<int32_t>();
-        const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
-        const uint32_t gb = (items + kGiSpatialBlock - 1) / kGiSpatialBlock;
-        const uint32_t lb = persistent_blocks((uint32_t)std::min<size_t>(slots, 0xffffffffu), kTraceBlock,
-                                              kMaxResidentBlocks);
         for (int pass = 0; pass < opt.spatial_passes; pass++) {
             A.cur = planes(st.cur);
             A.prev = planes(1 - st.cur);
             A.shade = pass == opt.spatial_passes - 1;
-            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
             gi_spatial_rays_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, lb);
-            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr);
             gi_spatial_merge_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, lb);
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr);
             gi_spatial_resolve_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;

@@ -192,10 +192,11 @@ struct RestirState {
     DevBuf intermediate;  // float4 per pixel
     DevBuf auxiliary;     // float4 per pixel
     DevBuf history;       // per-pixel path history scratch
-    // spatial pass, split around its shadow rays: ray list (2 float4 per ray),
-    // one occlusion byte per (ray slot, pixel), 1/Z weights per (neighbour,
-    // pixel), selection flags per pixel
-    DevBuf rays, occ, zm, flags;
+    // spatial pass, split around its shadow rays: segmented ray list (2 float4
+    // per ray), its segment counts and offsets, the packed list, one occlusion
+    // byte per (ray slot, pixel), 1/Z weights per (neighbour, pixel),
+    // selection flags per pixel
+    DevBuf rays, seg_counts, seg_prefix, dense, occ, zm, flags;
     float prev_pv[16] = {0};
 };
 
@@ -207,7 +208,9 @@ struct GiState {
     int gcur = 0;
     DevBuf direct, aux;
     DevBuf history;
-    DevBuf rays, occ, zm;  // spatial pass: shadow-ray list, occlusion bytes, 1/Z weights
+    // spatial pass: segmented shadow-ray list, segment counts and offsets,
+    // packed list, occlusion bytes, 1/Z weights
+    DevBuf rays, seg_counts, seg_prefix, dense, occ, zm;
     float prev_pv[16] = {0};
 };
 

@@ -450,6 +450,9 @@ constexpr int kSpatialBlock = 256;
 // Neighbour picks -> the resampling-target rays (origin: this pixel's
 // surface, towards the neighbour's sample).
 __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs A, SpatialBufs B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,8 +486,9 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
                 valid = true;
             }
         }
-        emit_ray(B.list, valid, x, dir, tmax, (uint32_t)i * npix + pix);
+        emit_ray(B.list, &nray, valid, x, dir, tmax, (uint32_t)i * npix + pix);
     }
+    publish_rays(B.list, &nray);
 }
 
 // Resampling over the neighbours with the traced visibilities; emits the 1/Z
@@ -493,6 +497,9 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
 // known, the shading ray.  The merged reservoir (W still open) goes to the
 // destination planes.
 __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs A, SpatialBufs B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,7 +581,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
             }
             B.zm[(size_t)i * npix + pix] = zm;
         }
-        emit_ray(B.list, valid, o, dir, tmax, (uint32_t)i * npix + pix);
+        emit_ray(B.list, &nray, valid, o, dir, tmax, (uint32_t)i * npix + pix);
     }
     bool valid = false;
     f3 dir = mk(0, 0, 0);
@@ -586,7 +593,8 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
         tmax = d - kShadowEps;
         valid = true;
     }
-    emit_ray(B.list, valid, hit.x, dir, tmax, (uint32_t)S * npix + pix);
+    emit_ray(B.list, &nray, valid, hit.x, dir, tmax, (uint32_t)S * npix + pix);
+    publish_rays(B.list, &nray);
 }
 
 // Z from the 1/Z visibilities, W, shading with the merged sample, colour.
@@ -712,7 +720,10 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.auxiliary = st.auxiliary.as<float4>();
     A.hist = st.history.as<float4>();
     // one spill slab serves the pass kernels (32-entry LDS stacks) and the
-    // spatial shadow-list walks (16 entries): size it for both first
+    // spatial shadow-list walks (16 entries, one workgroup per list segment):
+    // size it for both first
+    const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
+    const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;  // spatial producers = list segments
     spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
@@ -753,33 +764,37 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     mark();
     if (opt.spatial_passes > 0) {
         const int S = opt.neighbours;
+        // segments sized for the merge kernel's S + 1 rays per lane
+        const size_t list_slots = (size_t)gb * kSpatialBlock * (size_t)(S + 1);
         const size_t slots = (size_t)(S + 1) * npix;
-        if (st.rays.bytes < sizeof(float4) * 2 * slots) st.rays.alloc(sizeof(float4) * 2 * slots);
+        if (st.rays.bytes < sizeof(float4) * 2 * list_slots) st.rays.alloc(sizeof(float4) * 2 * list_slots);
+        if (st.seg_counts.bytes < sizeof(uint32_t) * gb) st.seg_counts.alloc(sizeof(uint32_t) * gb);
+        if (st.seg_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.seg_prefix.alloc(sizeof(uint32_t) * (gb + 1));
+        if (st.dense.bytes < sizeof(float4) * 2 * list_slots) st.dense.alloc(sizeof(float4) * 2 * list_slots);
         if (st.occ.bytes < slots) st.occ.alloc(slots);
         if (st.zm.bytes < sizeof(int32_t) * ((size_t)S * npix + 1)) st.zm.alloc(sizeof(int32_t) * ((size_t)S * npix + 1));
         if (st.flags.bytes < npix) st.flags.alloc(npix);
         SpatialBufs B;
         B.list.rays = st.rays.as<float4>();
-        B.list.count = ctx.counters.as<uint32_t>() + 1;
+        B.list.counts = st.seg_counts.as<uint32_t>();
+        B.list.prefix = st.seg_prefix.as<uint32_t>();
+        B.list.dense = st.dense.as<float4>();
+        B.list.nseg = gb;
         B.occ = st.occ.as<uint8_t>();
         B.zm = st.zm.as<int32_t>();
         B.flags = st.flags.as<uint8_t>();
-        const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
-        const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;
-        const uint32_t lb = persistent_blocks((uint32_t)std::min<size_t>(slots, 0xffffffffu), kTraceBlock,
-                                              kMaxResidentBlocks);
-        int *lean_spill = A.spill;  // sized above for lb <= kMaxResidentBlocks workgroups
+        int *lean_spill = A.spill;  // sized above for gb walk workgroups
         for (int pass = 0; pass < opt.spatial_passes; pass++) {
             A.cur = planes(st.cur);
             A.prev = planes(1 - st.cur);
-            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            B.list.seg = kSpatialBlock * (uint32_t)S;
             spatial_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, lb);
-            KRT_HIP(hipMemsetAsync(B.list.count, 0, sizeof(uint32_t), ctx.stream));
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr);
+            B.list.seg = kSpatialBlock * (uint32_t)(S + 1);
             spatial_merge_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, lb);
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr);
             spatial_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294

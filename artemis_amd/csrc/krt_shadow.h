@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
@@ -16,46 +18,100 @@
 namespace krt {
 namespace {
 
+// Producers append with no global atomics: workgroup b of a producing launch
+// owns the segment [b * seg, (b + 1) * seg) of `rays` and counts its rays in
+// LDS; counts[b] is written by every producing workgroup, so nothing needs
+// clearing between launches.  Before the walk, the segments are packed into
+// one dense list (`dense`, offsets from an exclusive scan of the counts into
+// `prefix`), so every walking lane gets the same share of rays.
 struct RayList {
-    float4 *rays;     // {origin, tmax}, {direction, slot as uint bits}
-    uint32_t *count;  // rays appended (zeroed before the producing kernel)
+    float4 *rays;      // per ray {origin, tmax}, {direction, slot as uint bits}
+    uint32_t *counts;  // rays in each segment
+    uint32_t seg;      // slots per segment (lanes per workgroup x rays per lane)
+    uint32_t nseg;     // producing workgroups
+    uint32_t *prefix;  // nseg + 1: segment offsets in the dense list, total last
+    float4 *dense;
 };
 
-// Wave-aggregated append: one atomic per wavefront; every lane of the wave
-// must call it (lanes without a ray pass valid = false).
-KRT_DEV void emit_ray(const RayList &L, bool valid, f3 o, f3 d, float tmax, uint32_t slot) {
+// Wave-aggregated append to the workgroup's segment: one LDS atomic per
+// wavefront.  Every lane of the wave must call it (lanes without a ray pass
+// valid = false); `nray` is the workgroup's LDS counter, zeroed before the
+// first call and published with publish_rays after the last.
+KRT_DEV void emit_ray(const RayList &L, uint32_t *nray, bool valid, f3 o, f3 d, float tmax, uint32_t slot) {
     const unsigned long long want = __ballot(valid);
     if (!want) return;
     const uint32_t lane = threadIdx.x & 63u;
     const int leader = __ffsll((unsigned long long)want) - 1;
     uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(L.count, (uint32_t)__popcll(want));
+    if ((int)lane == leader) base = atomicAdd(nray, (uint32_t)__popcll(want));
     base = __shfl(base, leader, 64);
     if (valid) {
-        const uint32_t k = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-        L.rays[2 * (size_t)k] = make_float4(o.x, o.y, o.z, tmax);
-        L.rays[2 * (size_t)k + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
+        const size_t k = (size_t)blockIdx.x * L.seg + base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        L.rays[2 * k] = make_float4(o.x, o.y, o.z, tmax);
+        L.rays[2 * k + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
     }
 }
 
-// Any-hit walks of a ray list (occluded(), amadeus_common.cuh:467-478): the
-// list's count is read on the device; grid-stride over the rays.
+KRT_DEV void publish_rays(const RayList &L, const uint32_t *nray) {
+    __syncthreads();
+    if (threadIdx.x == 0) L.counts[blockIdx.x] = *nray;
+}
+
+constexpr int kScanBlock = 1024;
+
+// Exclusive scan of the segment counts in one workgroup: each lane sums a run
+// of consecutive segments, the run sums are scanned in LDS, the lanes write
+// their runs' offsets; prefix[nseg] = total.
+__global__ __launch_bounds__(kScanBlock) void seg_scan_kernel(RayList L) {
+    __shared__ uint32_t part[kScanBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (L.nseg + kScanBlock - 1) / kScanBlock;
+    const uint32_t b0 = min(t * per, L.nseg), b1 = min(b0 + per, L.nseg);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += L.counts[b];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint32_t b = b0; b < b1; b++) {
+        L.prefix[b] = run;
+        run += L.counts[b];
+    }
+    if (t == kScanBlock - 1) L.prefix[L.nseg] = part[kScanBlock - 1];
+}
+
+constexpr int kCompactBlock = 256;
+
+// Segment b -> dense[prefix[b] ...], one workgroup per segment.
+__global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n2 = 2 * L.counts[b];
+    const float4 *src = L.rays + 2 * (size_t)b * L.seg;
+    float4 *dst = L.dense + 2 * (size_t)L.prefix[b];
+    for (uint32_t k = threadIdx.x; k < n2; k += kCompactBlock) dst[k] = src[k];
+}
+
+// Any-hit walks of the dense list (occluded(), amadeus_common.cuh:467-478),
+// grid-stride; the count is read on the device.
 template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, const float4 *__restrict__ rays,
-                                                                  const uint32_t *__restrict__ count,
-                                                                  uint8_t *__restrict__ occ, int *spill,
-                                                                  unsigned long long *ctr) {
+__global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, RayList L, uint8_t *__restrict__ occ,
+                                                                  int *spill, unsigned long long *ctr) {
     __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepthLean, spill + gtid, (int)nthreads};
     TravStats ts;
     unsigned nr = 0;
-    const uint32_t n = *count;
+    const uint32_t n = L.prefix[L.nseg];
     for (uint32_t k = gtid; k < n; k += nthreads) {
-        const float4 a = rays[2 * (size_t)k], b = rays[2 * (size_t)k + 1];
-        const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, stk, ts);
-        occ[__float_as_uint(b.w)] = r ? 1 : 0;
+        const float4 a = L.dense[2 * (size_t)k], c = L.dense[2 * (size_t)k + 1];
+        const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, ts);
+        occ[__float_as_uint(c.w)] = r ? 1 : 0;
         nr++;
     }
     wave_count(ctr ? ctr + 3 : nullptr, nr);
@@ -63,14 +119,24 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, 
     wave_count(ctr ? ctr + 9 : nullptr, ts.tris);
 }
 
-// Walks a list on the context stream with `blocks` workgroups of kTraceBlock
-// lanes, whose spill columns the caller sized for kLdsDepthLean.
+// Workgroups of the walk: persistent grid over the list's capacity.
+inline uint32_t walk_blocks(const RayList &L) {
+    const size_t cap = (size_t)L.nseg * L.seg;
+    return persistent_blocks((uint32_t)std::min<size_t>(cap, 0xffffffffu), kTraceBlock, kMaxResidentBlocks);
+}
+
+// Packs and walks a list on the context stream; the caller sized the spill
+// slab for kMaxResidentBlocks * kTraceBlock lanes at kLdsDepthLean.
 inline void walk_shadow_list(Context &ctx, bool instanced, const SceneView &sv, const RayList &L, uint8_t *occ,
-                             int *spill, unsigned long long *ctr, uint32_t blocks) {
-    if (instanced)
-        shadow_list_kernel<true><<<blocks, kTraceBlock, 0, ctx.stream>>>(sv, L.rays, L.count, occ, spill, ctr);
-    else
-        shadow_list_kernel<false><<<blocks, kTraceBlock, 0, ctx.stream>>>(sv, L.rays, L.count, occ, spill, ctr);
+                             int *spill, unsigned long long *ctr) {
+    if (L.nseg == 0) return;
+    seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
+    KRT_HIP(hipGetLastError());
+    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
+    KRT_HIP(hipGetLastError());
+    const uint32_t nb = walk_blocks(L);
+    if (instanced) shadow_list_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr);
+    else shadow_list_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr);
     KRT_HIP(hipGetLastError());
 }
 
