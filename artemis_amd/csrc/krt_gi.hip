@@ -734,10 +734,10 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
             A.shade = pass == opt.spatial_passes - 1;
             gi_spatial_rays_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr);
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, 2);
             gi_spatial_merge_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr);
+            walk_shadow_list(ctx, false, A.sv, B.list, B.occ, A.spill, ctr, 2);
             gi_spatial_resolve_kernel<<<gb, kGiSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;

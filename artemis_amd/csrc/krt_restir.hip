@@ -371,52 +371,6 @@ KRT_DEV int reproject(const LaunchInfo &li, int index, f3 pos) {
     return iy * (int)li.width + ix;
 }
 
-template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) void temporal_kernel(RestirArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
-    const LaunchInfo &li = A.li;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    TravStats ts;
-    RayCount rc;
-    const TileOrder tiles(li.width, li.height);
-    for (uint32_t item = gtid; item < tiles.items; item += nthreads) {
-        uint32_t px, py;
-        if (!tiles.pixel(item, px, py)) continue;
-        const uint32_t pix = py * li.width + px;
-        Resv cr = A.cur.load(pix);
-        if (cr.M <= 0) continue;
-        SurfaceHit hit = gbuffer_hit(A, pix);
-        int pindex = reproject(li, (int)pix, hit.x);
-        Resv merged;
-        merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
-        merged.data.type = 0;
-        merged.w = merged.W = 0.0f;
-        merged.M = 0;
-        merged.seed = ld3(A.auxiliary[pix]);
-        float t_current = occluded_target<INST>(A.sv, cr.data, hit, stk, ts, rc);
-        merged.update(cr.data, t_current * cr.W * (float)cr.M);
-        int N = cr.M;
-        if (pindex >= 0) {
-            Resv pr = A.prev.load((uint32_t)pindex);
-            if (pr.M > 0) {
-                pr.M = min(pr.M, A.m_cap);
-                A.prev.store_M((uint32_t)pindex, pr.M);
-                float t_prev = occluded_target<INST>(A.sv, pr.data, hit, stk, ts, rc);
-                merged.update(pr.data, t_prev * pr.W * (float)pr.M);
-                N += (t_prev > 0.0f ? 1 : 0) * pr.M;
-            }
-        }
-        merged.M = N;
-        merged.resample(target(merged.data, hit));
-        A.cur.store(pix, merged);
-    }
-    wave_count(A.ctr ? A.ctr + 2 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 5 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 8 : nullptr, ts.tris);
-}
-
 // The spatial pass (__raygen__spatial, amadeus_restir.cu:317-434) split around
 // its shadow rays.  Per pixel the reference traces one ray towards every
 // non-empty neighbour's sample (the resampling target), then one from every
@@ -446,6 +400,101 @@ struct SpatialBufs {
 };
 
 constexpr int kSpatialBlock = 256;
+
+// Temporal pass (__raygen__temporal, amadeus_restir.cu:213-291), split around
+// its two visibility rays as the spatial pass is: temporal_rays_kernel casts
+// the current sample's ray (slot 0) and the reprojected previous sample's
+// (slot 1) from the pixel's surface into the list, a walk answers them, and
+// temporal_resolve_kernel merges with those answers.  The previous
+// reservoir's M is capped in place in the first kernel, as the reference
+// does before its ray.
+__global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs A, SpatialBufs B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px = 0, py = 0, pix = 0;
+    bool live = item < tiles.items && tiles.pixel(item, px, py);
+    if (live) {
+        pix = py * li.width + px;
+        live = A.cur.M(pix) > 0;
+    }
+    f3 x = mk(0, 0, 0), d0 = mk(0, 0, 0), d1 = mk(0, 0, 0);
+    float t0 = 0.0f, t1 = 0.0f;
+    bool v1 = false;
+    if (live) {
+        const float4 xx = li.position[pix];
+        x = mk(xx.x, xx.y, xx.z);
+        const float4 c = A.cur.p[2 * (size_t)A.cur.n + pix];
+        f3 D = mk(c.x, c.y, c.z) - x;
+        float d = length(D);
+        d0 = D / d;
+        t0 = d - kShadowEps;
+        const int pindex = reproject(li, (int)pix, x);
+        if (pindex >= 0) {
+            const int pm = A.prev.M((uint32_t)pindex);
+            if (pm > 0) {
+                A.prev.store_M((uint32_t)pindex, min(pm, A.m_cap));
+                const float4 p = A.prev.p[2 * (size_t)A.prev.n + (uint32_t)pindex];
+                D = mk(p.x, p.y, p.z) - x;
+                d = length(D);
+                d1 = D / d;
+                t1 = d - kShadowEps;
+                v1 = true;
+            }
+        }
+    }
+    emit_ray(B.list, &nray, live, x, d0, t0, pix);
+    emit_ray(B.list, &nray, v1, x, d1, t1, npix + pix);
+    publish_rays(B.list, &nray);
+}
+
+__global__ __launch_bounds__(kSpatialBlock) void temporal_resolve_kernel(RestirArgs A, SpatialBufs B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const Resv cr = A.cur.load(pix);
+    if (cr.M <= 0) return;
+    const SurfaceHit hit = gbuffer_hit(A, pix);
+    const int pindex = reproject(li, (int)pix, hit.x);
+    Resv merged;
+    merged.data.Le = merged.data.normal = merged.data.point = mk(0, 0, 0);
+    merged.data.type = 0;
+    merged.w = merged.W = 0.0f;
+    merged.M = 0;
+    merged.seed = ld3(A.auxiliary[pix]);
+    // occluded_target: 0 when the ray was blocked, else the unshadowed sum
+    f3 D = cr.data.point - hit.x;
+    float d = length(D);
+    f3 Li = B.occ[pix] ? mk(0.0f, 0.0f, 0.0f) : direct_li(hit, cr.data.Le, cr.data.normal, cr.data.type, D / d, d);
+    const float t_current = Li.x + Li.y + Li.z;
+    merged.update(cr.data, t_current * cr.W * (float)cr.M);
+    int N = cr.M;
+    if (pindex >= 0) {
+        Resv pr = A.prev.load((uint32_t)pindex);
+        if (pr.M > 0) {
+            pr.M = min(pr.M, A.m_cap);
+            D = pr.data.point - hit.x;
+            d = length(D);
+            Li = B.occ[npix + pix] ? mk(0.0f, 0.0f, 0.0f)
+                                   : direct_li(hit, pr.data.Le, pr.data.normal, pr.data.type, D / d, d);
+            const float t_prev = Li.x + Li.y + Li.z;
+            merged.update(pr.data, t_prev * pr.W * (float)pr.M);
+            N += (t_prev > 0.0f ? 1 : 0) * pr.M;
+        }
+    }
+    merged.M = N;
+    merged.resample(target(merged.data, hit));
+    A.cur.store(pix, merged);
+}
+
 
 // Neighbour picks -> the resampling-target rays (origin: this pixel's
 // surface, towards the neighbour's sample).
@@ -758,43 +807,48 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
     }
     mark();
-    if (s.instanced) temporal_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-    else temporal_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    // shadow-ray lists of the temporal and spatial passes: segments sized for
+    // the largest per-lane count (2 temporal rays, S + 1 in the spatial merge)
+    const int S = opt.neighbours;
+    const uint32_t per_lane = (uint32_t)std::max(S + 1, 2);
+    const size_t list_slots = (size_t)gb * kSpatialBlock * per_lane;
+    const size_t slots = (size_t)per_lane * npix;
+    if (st.rays.bytes < sizeof(float4) * 2 * list_slots) st.rays.alloc(sizeof(float4) * 2 * list_slots);
+    if (st.seg_counts.bytes < sizeof(uint32_t) * gb) st.seg_counts.alloc(sizeof(uint32_t) * gb);
+    if (st.seg_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.seg_prefix.alloc(sizeof(uint32_t) * (gb + 1));
+    if (st.dense.bytes < sizeof(float4) * 2 * list_slots) st.dense.alloc(sizeof(float4) * 2 * list_slots);
+    if (st.occ.bytes < slots) st.occ.alloc(slots);
+    if (st.zm.bytes < sizeof(int32_t) * ((size_t)S * npix + 1)) st.zm.alloc(sizeof(int32_t) * ((size_t)S * npix + 1));
+    if (st.flags.bytes < npix) st.flags.alloc(npix);
+    SpatialBufs B;
+    B.list.rays = st.rays.as<float4>();
+    B.list.counts = st.seg_counts.as<uint32_t>();
+    B.list.prefix = st.seg_prefix.as<uint32_t>();
+    B.list.dense = st.dense.as<float4>();
+    B.list.nseg = gb;
+    B.occ = st.occ.as<uint8_t>();
+    B.zm = st.zm.as<int32_t>();
+    B.flags = st.flags.as<uint8_t>();
+    int *lean_spill = A.spill;  // sized above for the walks too
+    B.list.seg = kSpatialBlock * 2u;
+    temporal_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
+    KRT_HIP(hipGetLastError());
+    walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, 1);
+    temporal_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
     KRT_HIP(hipGetLastError());
     mark();
     if (opt.spatial_passes > 0) {
-        const int S = opt.neighbours;
-        // segments sized for the merge kernel's S + 1 rays per lane
-        const size_t list_slots = (size_t)gb * kSpatialBlock * (size_t)(S + 1);
-        const size_t slots = (size_t)(S + 1) * npix;
-        if (st.rays.bytes < sizeof(float4) * 2 * list_slots) st.rays.alloc(sizeof(float4) * 2 * list_slots);
-        if (st.seg_counts.bytes < sizeof(uint32_t) * gb) st.seg_counts.alloc(sizeof(uint32_t) * gb);
-        if (st.seg_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.seg_prefix.alloc(sizeof(uint32_t) * (gb + 1));
-        if (st.dense.bytes < sizeof(float4) * 2 * list_slots) st.dense.alloc(sizeof(float4) * 2 * list_slots);
-        if (st.occ.bytes < slots) st.occ.alloc(slots);
-        if (st.zm.bytes < sizeof(int32_t) * ((size_t)S * npix + 1)) st.zm.alloc(sizeof(int32_t) * ((size_t)S * npix + 1));
-        if (st.flags.bytes < npix) st.flags.alloc(npix);
-        SpatialBufs B;
-        B.list.rays = st.rays.as<float4>();
-        B.list.counts = st.seg_counts.as<uint32_t>();
-        B.list.prefix = st.seg_prefix.as<uint32_t>();
-        B.list.dense = st.dense.as<float4>();
-        B.list.nseg = gb;
-        B.occ = st.occ.as<uint8_t>();
-        B.zm = st.zm.as<int32_t>();
-        B.flags = st.flags.as<uint8_t>();
-        int *lean_spill = A.spill;  // sized above for gb walk workgroups
         for (int pass = 0; pass < opt.spatial_passes; pass++) {
             A.cur = planes(st.cur);
             A.prev = planes(1 - st.cur);
             B.list.seg = kSpatialBlock * (uint32_t)S;
             spatial_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr);
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, 2);
             B.list.seg = kSpatialBlock * (uint32_t)(S + 1);
             spatial_merge_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr);
+            walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, 2);
             spatial_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294

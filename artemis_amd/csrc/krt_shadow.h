@@ -97,10 +97,12 @@ __global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L) {
 }
 
 // Any-hit walks of the dense list (occluded(), amadeus_common.cuh:467-478),
-// grid-stride; the count is read on the device.
+// grid-stride; the count is read on the device.  Rays, node visits and
+// triangle tests are added to the pass's counters (ctr[1 + pass],
+// ctr[4 + pass], ctr[7 + pass]).
 template <bool INST>
 __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, RayList L, uint8_t *__restrict__ occ,
-                                                                  int *spill, unsigned long long *ctr) {
+                                                                  int *spill, unsigned long long *ctr, int pass) {
     __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -114,9 +116,9 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, 
         occ[__float_as_uint(c.w)] = r ? 1 : 0;
         nr++;
     }
-    wave_count(ctr ? ctr + 3 : nullptr, nr);
-    wave_count(ctr ? ctr + 6 : nullptr, ts.nodes);
-    wave_count(ctr ? ctr + 9 : nullptr, ts.tris);
+    wave_count(ctr ? ctr + 1 + pass : nullptr, nr);
+    wave_count(ctr ? ctr + 4 + pass : nullptr, ts.nodes);
+    wave_count(ctr ? ctr + 7 + pass : nullptr, ts.tris);
 }
 
 // Workgroups of the walk: persistent grid over the list's capacity.
@@ -128,15 +130,15 @@ inline uint32_t walk_blocks(const RayList &L) {
 // Packs and walks a list on the context stream; the caller sized the spill
 // slab for kMaxResidentBlocks * kTraceBlock lanes at kLdsDepthLean.
 inline void walk_shadow_list(Context &ctx, bool instanced, const SceneView &sv, const RayList &L, uint8_t *occ,
-                             int *spill, unsigned long long *ctr) {
+                             int *spill, unsigned long long *ctr, int pass) {
     if (L.nseg == 0) return;
     seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
     seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
     const uint32_t nb = walk_blocks(L);
-    if (instanced) shadow_list_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr);
-    else shadow_list_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr);
+    if (instanced) shadow_list_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    else shadow_list_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
     KRT_HIP(hipGetLastError());
 }
 
