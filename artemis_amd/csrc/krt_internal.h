@@ -197,6 +197,10 @@ struct RestirState {
     // byte per (ray slot, pixel), 1/Z weights per (neighbour, pixel),
     // selection flags per pixel
     DevBuf rays, seg_counts, seg_prefix, dense, occ, zm, flags;
+    // wavefront initial pass: hits, hit instances, {seed, levels} per pixel,
+    // per-level history and occlusion bytes, the closest-ray list
+    DevBuf wf_hits, wf_inst, wf_state, wf_hist, wf_occ;
+    DevBuf cl_rays, cl_counts, cl_prefix, cl_dense;
     float prev_pv[16] = {0};
 };
 

@@ -401,6 +401,232 @@ struct SpatialBufs {
 
 constexpr int kSpatialBlock = 256;
 
+// Wavefront form of the initial pass (radiance cache off): the estimator of
+// initial_kernel split at every ray.  Per level L of the path,
+//   walk_closest_list      the level's rays (slot = pixel) -> hits
+//   init_shade_kernel<L>   surface, material; L = 0: RIS over the
+//                          candidates, the reservoir with W for an unblocked
+//                          sample, the sample's visibility ray; L >= 1: the
+//                          light sample of Ld and its shadow ray; the BSDF
+//                          sample -> the next level's ray; per-level history
+//   walk_shadow_list       -> occlusion byte of (L, pixel)
+// and init_compose_kernel folds the history from the deepest level up
+// (value_k = direct_k + T_k value_k+1), zeroes W of reservoirs whose sample
+// is blocked, and writes the intermediate and auxiliary buffers.  Every
+// number is computed by the expressions of the fused kernel in the same
+// order (the random sequence included), so the two agree bit for bit; the
+// walks run in the register-lean list kernels.
+struct InitWave {
+    RayList cl;      // closest rays of the current level
+    RayList sl;      // shadow rays of the current level
+    float4 *hits;    // per pixel: {t, b1, b2, prim bits} of the current level's ray
+    int *hit_inst;   // per pixel: instance of that hit (two-level scenes)
+    float4 *state;   // per pixel: {seed, levels with history (int bits)}
+    float4 *hist;    // per level, per pixel: {emission, 0}, {T, Ld flag}, {Ld contribution unblocked, Ld pdf}
+    uint8_t *occ;    // per level, per pixel
+};
+
+__global__ __launch_bounds__(kSpatialBlock) void init_camera_kernel(RestirArgs A, InitWave B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t px = 0, py = 0, pix = 0;
+    const bool live = item < tiles.items && tiles.pixel(item, px, py);
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    if (live) {
+        pix = py * li.width + px;
+        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+        make_ray(li, px, py, seed, false, o, d);
+        B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(0));
+    }
+    emit_ray(B.cl, &nray, live, o, d, kTraceTmax, pix);
+    publish_rays(B.cl, &nray);
+}
+
+template <bool INST>
+__global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A, InitWave B, int level) {
+    __shared__ uint32_t ncl, nsl;
+    if (threadIdx.x == 0) ncl = nsl = 0;
+    __syncthreads();
+    const LaunchInfo &li = A.li;
+    const SceneView &sv = A.sv;
+    const uint32_t npix = li.width * li.height;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float count = (float)li.samples;
+    const bool acc = li.accumulate != 0;
+    const unsigned total = sv.nl;
+    const bool live = k < B.cl.prefix[B.cl.nseg];
+    bool shadow = false, next = false;
+    f3 so = mk(0, 0, 0), sd = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
+    float stmax = 0.0f;
+    uint32_t pix = 0;
+    if (live) {
+        const float4 ra = B.cl.dense[2 * (size_t)k], rb = B.cl.dense[2 * (size_t)k + 1];
+        (void)ra;
+        const f3 d = mk(rb.x, rb.y, rb.z);
+        pix = __float_as_uint(rb.w);
+        const float4 st = B.state[pix];
+        f3 seed = mk(st.x, st.y, st.z);
+        const float4 hh = B.hits[pix];
+        Hit h;
+        h.t = hh.x;
+        h.b1 = hh.y;
+        h.b2 = hh.z;
+        h.prim = __float_as_int(hh.w);
+        h.rank = 0;
+        h.inst = INST ? B.hit_inst[pix] : -1;
+        Surface sf;
+        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {
+            const Material mat = load_material(sv.mats, sf.mat_id);
+            const f3 n = sf.n;
+            const f3 x = offset_point(sf.x, n, mat.type);
+            const SurfaceHit sh{mat, sf.entering, n, -d, x};
+            f3 emission = mk(0, 0, 0), contr = mk(0, 0, 0);
+            float lpdf = 0.0f;
+            bool ld = false;
+            if (level == 0) {
+                Resv res = A.cur.load(pix);
+                res.w = 0.0f;
+                res.M = 0;
+                res.W = 0.0f;
+                for (int i = 0; i < A.candidates; i++) {
+                    // sample_direct (amadeus_common.cuh:301-377), triangle lights
+                    RSample smp;
+                    smp.Le = smp.normal = smp.point = mk(0, 0, 0);
+                    smp.type = 0;
+                    float pdf = 0.0f;
+                    if (total > 0) {
+                        unsigned li_ = pick_light(seed, total);
+                        TriLight L = load_light(sv.lights, li_);
+                        smp.point = sample_area_light(L, seed);
+                        smp.Le = L.intensity;
+                        smp.normal = L.normal;
+                        smp.type = 1;
+                        pdf = 1.0f / (L.area * (float)total);
+                    } else {
+                        rand_uniform(seed);
+                    }
+                    f3 D = smp.point - sh.x;
+                    float dd = length(D);
+                    f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
+                    float tgt = Li.x + Li.y + Li.z;
+                    res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
+                }
+                const RSample smp = res.data;
+                const f3 D = smp.point - sh.x;
+                const float dd = length(D);
+                const f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
+                res.resample(Li.x + Li.y + Li.z);  // W for an unblocked sample; the compose kernel applies the ray
+                A.cur.store(pix, res);
+                A.gmat[pix] = sf.mat_id;
+                so = sh.x;
+                sd = D / dd;
+                stmax = dd - kShadowEps;
+                shadow = true;
+            } else {
+                // Ld (krt_shade.h) with its shadow ray deferred
+                emission = mat.emission;
+                if (total == 0) {
+                    rand_uniform(seed);
+                } else {
+                    unsigned i = pick_light(seed, total);
+                    float pdf = 1.0f / (float)total;
+                    TriLight L = load_light(sv.lights, i);
+                    float light_pdf = 1.0f / L.area;
+                    f3 lpos = sample_area_light(L, seed);
+                    f3 wi = normalize(lpos - sh.x);
+                    float R = length(lpos - sh.x);
+                    f3 f = brdf(sh, wi, eDiffuse);
+                    float ldot = fabsf(dot(L.normal, wi));
+                    float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
+                    contr = f * L.intensity * geometric;
+                    pdf *= light_pdf;
+                    lpdf = pdf;
+                    ld = true;
+                    so = sh.x;
+                    sd = wi;
+                    stmax = R - kShadowEps;
+                    shadow = true;
+                }
+            }
+            int out;
+            f3 wi;
+            float pdf;
+            f3 f = eval_bsdf(sh, wi, pdf, out, seed);
+            bool has_pdf = pdf > 0.0f;
+            f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
+            float4 *hl = B.hist + 3 * (size_t)level * npix;
+            hl[pix] = make_float4(emission.x, emission.y, emission.z, 0.0f);
+            hl[npix + pix] = make_float4(T.x, T.y, T.z, ld ? 1.0f : 0.0f);
+            hl[2 * (size_t)npix + pix] = make_float4(contr.x, contr.y, contr.z, lpdf);
+            if (level == 0) {
+                li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
+                li.albedo[pix] =
+                    accumulate4(li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z, 0.0f), acc,
+                                count);
+                li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
+            }
+            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(level + 1));
+            if (has_pdf && level < li.max_depth) {
+                no = x;
+                nd = wi;
+                next = true;
+            }
+        }
+    }
+    emit_ray(B.sl, &nsl, shadow, so, sd, stmax, pix);
+    emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        B.sl.counts[blockIdx.x] = nsl;
+        B.cl.counts[blockIdx.x] = ncl;
+    }
+}
+
+__global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs A, InitWave B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const float4 st = B.state[pix];
+    const int nlev = __float_as_int(st.w);
+    f3 value = mk(0, 0, 0);
+    for (int k = nlev - 1; k >= 0; k--) {
+        const float4 *hl = B.hist + 3 * (size_t)k * npix;
+        const float4 e = hl[pix], t = hl[npix + pix], c = hl[2 * (size_t)npix + pix];
+        f3 Ldv = mk(0.0f, 0.0f, 0.0f);
+        if (t.w != 0.0f) {
+            const f3 contr = B.occ[(size_t)k * npix + pix] ? mk(0.0f, 0.0f, 0.0f) : mk(c.x, c.y, c.z);
+            Ldv = contr / c.w;
+        }
+        const f3 dk = mk(e.x, e.y, e.z) + Ldv;
+        value = dk + mk(t.x, t.y, t.z) * value;
+    }
+    float4 col = make_float4(value.x, value.y, value.z, 1.0f);
+    if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+    A.intermediate[pix] = col;
+    A.auxiliary[pix] = make_float4(st.x, st.y, st.z, 1.0f);
+    const bool acc = li.accumulate != 0;
+    const float count = (float)li.samples;
+    if (nlev > 0) {
+        // the reference traces this visibility ray a second time: W *= 1 - occluded
+        float4 &w = A.cur.p[2 * (size_t)A.cur.n + pix];
+        w.w = B.occ[pix] ? 0.0f : w.w * (1.0f - 0.0f);
+    } else {  // no reservoir written: primary miss or no surface
+        A.cur.reset(pix);
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
+        li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
+        li.position[pix] = zero;
+    }
+}
+
 // Temporal pass (__raygen__temporal, amadeus_restir.cu:213-291), split around
 // its two visibility rays as the spatial pass is: temporal_rays_kernel casts
 // the current sample's ray (slot 0) and the reprojected previous sample's
@@ -795,20 +1021,9 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         KRT_HIP(hipEventRecord(ev[nev], ctx.stream));
         nev++;
     };
-    mark();
-    A.cur = planes(st.cur);
-    A.prev = planes(1 - st.cur);
-    if (s.instanced) initial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-    else initial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-    KRT_HIP(hipGetLastError());
-    mark();
-    if (cached) {
-        cache_infer(ctx, *cs, npix);
-        cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
-    }
-    mark();
-    // shadow-ray lists of the temporal and spatial passes: segments sized for
-    // the largest per-lane count (2 temporal rays, S + 1 in the spatial merge)
+    // shadow-ray lists of the passes: segments sized for the largest per-lane
+    // count (1 in the initial pass, 2 temporal rays, S + 1 in the spatial
+    // merge); the initial pass's closest rays have a list of their own
     const int S = opt.neighbours;
     const uint32_t per_lane = (uint32_t)std::max(S + 1, 2);
     const size_t list_slots = (size_t)gb * kSpatialBlock * per_lane;
@@ -829,7 +1044,54 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     B.occ = st.occ.as<uint8_t>();
     B.zm = st.zm.as<int32_t>();
     B.flags = st.flags.as<uint8_t>();
-    int *lean_spill = A.spill;  // sized above for the walks too
+    int *lean_spill = A.spill;  // sized above for the list walks too
+    mark();
+    A.cur = planes(st.cur);
+    A.prev = planes(1 - st.cur);
+    if (cached) {
+        if (s.instanced) initial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        else initial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+        KRT_HIP(hipGetLastError());
+    } else {
+        const size_t levels = (size_t)li.max_depth + 1;
+        const size_t cl_slots = (size_t)gb * kSpatialBlock;
+        if (st.wf_hits.bytes < sizeof(float4) * npix) st.wf_hits.alloc(sizeof(float4) * npix);
+        if (s.instanced && st.wf_inst.bytes < sizeof(int) * npix) st.wf_inst.alloc(sizeof(int) * npix);
+        if (st.wf_state.bytes < sizeof(float4) * npix) st.wf_state.alloc(sizeof(float4) * npix);
+        if (st.wf_hist.bytes < sizeof(float4) * 3 * levels * npix) st.wf_hist.alloc(sizeof(float4) * 3 * levels * npix);
+        if (st.wf_occ.bytes < levels * npix) st.wf_occ.alloc(levels * npix);
+        if (st.cl_rays.bytes < sizeof(float4) * 2 * cl_slots) st.cl_rays.alloc(sizeof(float4) * 2 * cl_slots);
+        if (st.cl_counts.bytes < sizeof(uint32_t) * gb) st.cl_counts.alloc(sizeof(uint32_t) * gb);
+        if (st.cl_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.cl_prefix.alloc(sizeof(uint32_t) * (gb + 1));
+        if (st.cl_dense.bytes < sizeof(float4) * 2 * cl_slots) st.cl_dense.alloc(sizeof(float4) * 2 * cl_slots);
+        InitWave W;
+        W.cl = RayList{st.cl_rays.as<float4>(), st.cl_counts.as<uint32_t>(), (uint32_t)kSpatialBlock, gb,
+                       st.cl_prefix.as<uint32_t>(), st.cl_dense.as<float4>()};
+        W.sl = B.list;
+        W.sl.seg = kSpatialBlock;
+        W.hits = st.wf_hits.as<float4>();
+        W.hit_inst = s.instanced ? st.wf_inst.as<int>() : nullptr;
+        W.state = st.wf_state.as<float4>();
+        W.hist = st.wf_hist.as<float4>();
+        W.occ = st.wf_occ.as<uint8_t>();
+        init_camera_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
+        KRT_HIP(hipGetLastError());
+        for (int level = 0; level <= li.max_depth; level++) {
+            walk_closest_list(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, lean_spill, ctr, 0);
+            if (s.instanced) init_shade_kernel<true><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
+            else init_shade_kernel<false><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
+            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, s.instanced, A.sv, W.sl, W.occ + (size_t)level * npix, lean_spill, ctr, 0);
+        }
+        init_compose_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
+        KRT_HIP(hipGetLastError());
+    }
+    mark();
+    if (cached) {
+        cache_infer(ctx, *cs, npix);
+        cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
+    }
+    mark();
     B.list.seg = kSpatialBlock * 2u;
     temporal_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
     KRT_HIP(hipGetLastError());

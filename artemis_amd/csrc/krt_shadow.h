@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L) {
 // grid-stride; the count is read on the device.  Rays, node visits and
 // triangle tests are added to the pass's counters (ctr[1 + pass],
 // ctr[4 + pass], ctr[7 + pass]).
-template <bool INST>
+template <bool INST, bool WIDE>
 __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, RayList L, uint8_t *__restrict__ occ,
                                                                   int *spill, unsigned long long *ctr, int pass) {
     __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
@@ -112,11 +112,40 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, 
     const uint32_t n = L.prefix[L.nseg];
     for (uint32_t k = gtid; k < n; k += nthreads) {
         const float4 a = L.dense[2 * (size_t)k], c = L.dense[2 * (size_t)k + 1];
-        const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, ts);
+        const bool r = occluded<WIDE, INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, ts);
         occ[__float_as_uint(c.w)] = r ? 1 : 0;
         nr++;
     }
     wave_count(ctr ? ctr + 1 + pass : nullptr, nr);
+    wave_count(ctr ? ctr + 4 + pass : nullptr, ts.nodes);
+    wave_count(ctr ? ctr + 7 + pass : nullptr, ts.tris);
+}
+
+// Closest-hit walks of the dense list (the binary tree, as the fused kernels
+// walk it): hits[slot] = {t, b1, b2, prim bits}, hit_inst[slot] = instance
+// (two-level scenes).  LDS stack depth D: 16 entries keep eight waves per
+// SIMD resident, 32 keep five with fewer spills.
+template <bool INST, int D>
+__global__ __launch_bounds__(kTraceBlock) void closest_list_kernel(SceneView sv, RayList L, float4 *__restrict__ hits,
+                                                                   int *__restrict__ hit_inst, int *spill,
+                                                                   unsigned long long *ctr, int pass) {
+    __shared__ int lds_stack[D * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, D, spill + gtid, (int)nthreads};
+    TravStats ts;
+    unsigned nr = 0;
+    const uint32_t n = L.prefix[L.nseg];
+    for (uint32_t k = gtid; k < n; k += nthreads) {
+        const float4 a = L.dense[2 * (size_t)k], c = L.dense[2 * (size_t)k + 1];
+        Hit h;
+        trace_closest<INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, h, ts);
+        const uint32_t slot = __float_as_uint(c.w);
+        hits[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
+        if (INST) hit_inst[slot] = h.inst;
+        nr++;
+    }
+    wave_count(ctr, nr);
     wave_count(ctr ? ctr + 4 + pass : nullptr, ts.nodes);
     wave_count(ctr ? ctr + 7 + pass : nullptr, ts.tris);
 }
@@ -127,18 +156,44 @@ inline uint32_t walk_blocks(const RayList &L) {
     return persistent_blocks((uint32_t)std::min<size_t>(cap, 0xffffffffu), kTraceBlock, kMaxResidentBlocks);
 }
 
-// Packs and walks a list on the context stream; the caller sized the spill
-// slab for kMaxResidentBlocks * kTraceBlock lanes at kLdsDepthLean.
+// Packs and walks a list on the context stream (the 4-wide tree unless
+// `wide` is false); the caller sized the spill slab for kMaxResidentBlocks *
+// kTraceBlock lanes at kLdsDepthLean.
 inline void walk_shadow_list(Context &ctx, bool instanced, const SceneView &sv, const RayList &L, uint8_t *occ,
-                             int *spill, unsigned long long *ctr, int pass) {
+                             int *spill, unsigned long long *ctr, int pass, bool wide = true) {
     if (L.nseg == 0) return;
     seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
     seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
     const uint32_t nb = walk_blocks(L);
-    if (instanced) shadow_list_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
-    else shadow_list_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    if (instanced && wide)
+        shadow_list_kernel<true, true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    else if (instanced)
+        shadow_list_kernel<true, false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    else if (wide)
+        shadow_list_kernel<false, true><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    else
+        shadow_list_kernel<false, false><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, occ, spill, ctr, pass);
+    KRT_HIP(hipGetLastError());
+}
+
+// Packs a list, then walks it for closest hits on the context stream (spill
+// slab sized for kMaxResidentBlocks * kTraceBlock lanes at kLdsDepthLean).
+inline void walk_closest_list(Context &ctx, bool instanced, const SceneView &sv, const RayList &L, float4 *hits,
+                              int *hit_inst, int *spill, unsigned long long *ctr, int pass) {
+    if (L.nseg == 0) return;
+    seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
+    KRT_HIP(hipGetLastError());
+    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
+    KRT_HIP(hipGetLastError());
+    const uint32_t nb = walk_blocks(L);
+    if (instanced)
+        closest_list_kernel<true, kLdsDepthLean><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, hits, hit_inst, spill, ctr,
+                                                                                     pass);
+    else
+        closest_list_kernel<false, kLdsDepthLean><<<nb, kTraceBlock, 0, ctx.stream>>>(sv, L, hits, hit_inst, spill,
+                                                                                      ctr, pass);
     KRT_HIP(hipGetLastError());
 }
 
