@@ -767,7 +767,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         if (att->kind == "path_tracer" && att->copt.enabled) {
             cache_pt_render(C, s, li, att->copt, att->cst, att->history, a->ctr.as<unsigned long long>(), times);
         } else if (att->kind == "path_tracer") {
-            pt_render(C, s, li, att->history, a->ctr.as<unsigned long long>(), times,
+            pt_render(C, s, li, a->ctr.as<unsigned long long>(), times,
                       a->f64 ? a->acc64.as<double>() : nullptr);
             if (a->f64) a->f64_frames++;
         }

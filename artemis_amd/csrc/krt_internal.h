@@ -141,6 +141,13 @@ struct SceneView {
 
 SceneView make_view(const Scene &s);
 
+// Per-context scratch of the wavefront path tracer (grown on demand; every
+// launch sequence uses it from start to end on the context stream).
+struct WaveScratch {
+    DevBuf hits, inst, state, hist, occ;
+    DevBuf rays[2], counts[2], prefix[2], dense[2];  // closest-ray list, shadow-ray list
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -148,6 +155,7 @@ struct Context {
     krt_handle next_handle = 1;
     DevBuf counters;  // scratch counters (unsigned long long x 16)
     DevBuf spill;     // traversal stack spill slab (grown on demand)
+    WaveScratch wave;
     hipEvent_t timer[2] = {nullptr, nullptr};  // krt_timer_begin / krt_timer_end
     Scene &scene(krt_handle h) {
         auto it = scenes.find(h);
@@ -222,8 +230,8 @@ struct PassTimes {
     float ms[8] = {0};
 };
 
-void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
-               PassTimes &times, double *acc64 = nullptr);
+void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,
+               double *acc64);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
 void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed);

@@ -1,170 +1,254 @@
-// Path tracer attachment kernel: one pixel sample per lane over a persistent
-// grid, the estimator of pt_pixel (krt_pt.h) written out in the kernel: the
-// inlined call compiles to a 30 % slower loop here (same registers, different
-// block layout), and the two are held equal bit for bit by the serving tests.
+// Path tracer attachment: the estimator of pt_pixel (krt_pt.h,
+// amadeus_path_tracer.cu:73-228) in wavefront form, held equal to pt_pixel
+// bit for bit by the serving tests.
 #include <hip/hip_runtime.h>
 
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
 #include "krt_shade.h"
+#include "krt_shadow.h"
 #include "krt_traverse.h"
 
 namespace krt {
 
 namespace {
 
-struct PtArgs {
-    LaunchInfo li;
-    SceneView sv;
-    float4 *hist;
-    int *spill;
-    unsigned long long *ctr;  // [0] closest, [1] shadow
-    double *acc64;            // optional FP64 sums of samples, then of their squares (3 + 3 per pixel)
-    uint32_t *work;           // work-item counter of the launch (zeroed before it)
+// The path is split at every ray.  Per level L,
+//   walk_closest_list     the level's rays (slot = pixel) -> hits
+//   pt_shade_kernel       surface, material, the light sample of Ld and its
+//                         shadow ray, the BSDF sample -> the next level's ray,
+//                         the level's history {Le, has_pdf}, {T, Ld flag},
+//                         {Ld contribution unblocked, Ld pdf}; level 0 writes
+//                         the normal / albedo / position AOVs
+//   walk_shadow_list      -> occlusion byte of (L, pixel)
+// and pt_compose_kernel folds the history from the deepest level up and
+// accumulates the colour (and the FP64 sums).  Every number comes from the
+// expressions of pt_pixel in the same order, the random sequence included,
+// so the two agree bit for bit; the walks run in the register-lean list
+// kernels at up to eight waves per SIMD.
+struct PtWave {
+    RayList cl, sl;   // closest and shadow rays of the current level
+    float4 *hits;     // per pixel: {t, b1, b2, prim bits}
+    int *hit_inst;    // per pixel: instance (two-level scenes)
+    float4 *state;    // per pixel: {seed, levels with history (int bits)}
+    float4 *hist;     // per level, per pixel: 3 float4
+    uint8_t *occ;     // per level, per pixel
 };
 
-// Path regeneration on a persistent grid: a lane whose path has ended takes
-// the next pixel at once instead of idling until the longest path of its
-// wavefront ends.  Lanes that need work are counted with a wave ballot and
-// the wave takes that many consecutive work items with one atomic, so the
-// pixels a wave starts together come from one 8x8 tile.  Every pixel's
-// estimate is computed exactly as before; only its lane and start time
-// change.
-template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) void pt_kernel(PtArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
-    const LaunchInfo &li = A.li;
-    const SceneView &sv = A.sv;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    History hist{A.hist + gtid, nthreads};
-    TravStats ts;
-    RayCount rc;
-    const float count = (float)li.samples;
+constexpr int kWaveBlock = 256;
+
+__global__ __launch_bounds__(kWaveBlock) void pt_camera_kernel(LaunchInfo li, PtWave B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const TileOrder tiles(li.width, li.height);
-    bool active = false, done = false;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t px = 0, py = 0, pix = 0;
+    const bool live = item < tiles.items && tiles.pixel(item, px, py);
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    if (live) {
+        pix = py * li.width + px;
+        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+        make_ray(li, px, py, seed, li.jitter != 0, o, d);
+        B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(0));
+    }
+    emit_ray(B.cl, &nray, live, o, d, kTraceTmax, pix);
+    publish_rays(B.cl, &nray);
+}
+
+template <bool INST>
+__global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, SceneView sv, PtWave B, int level) {
+    __shared__ uint32_t ncl, nsl;
+    if (threadIdx.x == 0) ncl = nsl = 0;
+    __syncthreads();
+    const uint32_t npix = li.width * li.height;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float count = (float)li.samples;
+    const bool acc = li.accumulate != 0;
+    const unsigned total = sv.nl;
+    const bool live = k < B.cl.prefix[B.cl.nseg];
+    bool shadow = false, next = false;
+    f3 so = mk(0, 0, 0), sd = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
+    float stmax = 0.0f;
     uint32_t pix = 0;
-    int level = 0, nlev = 0;
-    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0);
-    f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
-    for (;;) {
-        // refill: one atomic per wavefront for all lanes without a path
-        const bool need = !active && !done;
-        const unsigned long long want = __ballot(need);
-        if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                uint32_t px, py;
-                if (item >= tiles.items) {
-                    done = true;
-                } else if (tiles.pixel(item, px, py)) {
-                    pix = py * li.width + px;
-                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-                    make_ray(li, px, py, seed, li.jitter != 0, o, d);
-                    pos0 = n0 = alb0 = mk(0, 0, 0);
-                    level = 0;
-                    nlev = 0;
-                    active = true;
-                }
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (!active) continue;
-        // one level of the path (amadeus_path_tracer.cu:133-201)
-        bool end = true;
+    if (live) {
+        const float4 rb = B.cl.dense[2 * (size_t)k + 1];
+        const f3 d = mk(rb.x, rb.y, rb.z);
+        pix = __float_as_uint(rb.w);
+        const float4 st = B.state[pix];
+        f3 seed = mk(st.x, st.y, st.z);
+        const float4 hh = B.hits[pix];
         Hit h;
-        rc.closest++;
-        trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
+        h.t = hh.x;
+        h.b1 = hh.y;
+        h.b2 = hh.z;
+        h.prim = __float_as_int(hh.w);
+        h.rank = 0;
+        h.inst = INST ? B.hit_inst[pix] : -1;
         Surface sf;
-        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {  // a miss (no environment map) or a NaN normal end it
-            Material mat = load_material(sv.mats, sf.mat_id);
-            f3 n = sf.n;
-            f3 x = offset_point(sf.x, n, mat.type);
-            SurfaceHit sh{mat, sf.entering, n, -d, x};
-            f3 direct = mat.emission + Ld<true, INST>(sv, sh, seed, stk, ts, rc);
+        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {  // a miss or a NaN normal ends the path
+            const Material mat = load_material(sv.mats, sf.mat_id);
+            const f3 n = sf.n;
+            const f3 x = offset_point(sf.x, n, mat.type);
+            const SurfaceHit sh{mat, sf.entering, n, -d, x};
+            // Ld (krt_shade.h) with its shadow ray deferred
+            f3 contr = mk(0, 0, 0);
+            float lpdf = 0.0f;
+            bool ld = false;
+            if (total == 0) {
+                rand_uniform(seed);
+            } else {
+                unsigned i = pick_light(seed, total);
+                float pdf = 1.0f / (float)total;
+                TriLight L = load_light(sv.lights, i);
+                float light_pdf = 1.0f / L.area;
+                f3 lpos = sample_area_light(L, seed);
+                f3 wi = normalize(lpos - sh.x);
+                float R = length(lpos - sh.x);
+                f3 f = brdf(sh, wi, eDiffuse);
+                float ldot = fabsf(dot(L.normal, wi));
+                float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
+                contr = f * L.intensity * geometric;
+                pdf *= light_pdf;
+                lpdf = pdf;
+                ld = true;
+                so = sh.x;
+                sd = wi;
+                stmax = R - kShadowEps;
+                shadow = true;
+            }
             int out;
             f3 wi;
             float pdf;
             f3 f = eval_bsdf(sh, wi, pdf, out, seed);
             f3 T = f * fabsf(dot(wi, n)) / pdf;
             bool has_pdf = pdf > 0.0f;
-            hist.put(level, direct, has_pdf, T);
+            float4 *hl = B.hist + 3 * (size_t)level * npix;
+            hl[pix] = make_float4(mat.emission.x, mat.emission.y, mat.emission.z, has_pdf ? 1.0f : 0.0f);
+            hl[npix + pix] = make_float4(T.x, T.y, T.z, ld ? 1.0f : 0.0f);
+            hl[2 * (size_t)npix + pix] = make_float4(contr.x, contr.y, contr.z, lpdf);
             if (level == 0) {
-                pos0 = x;
-                n0 = n;
-                alb0 = mat.diffuse;
+                li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
+                li.albedo[pix] =
+                    accumulate4(li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z, 0.0f), acc,
+                                count);
+                li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
             }
-            nlev = level + 1;
+            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(level + 1));
             if (has_pdf && level < li.max_depth) {
-                o = x;
-                d = wi;
-                level++;
-                end = false;
+                no = x;
+                nd = wi;
+                next = true;
             }
         }
-        if (!end) continue;
-        // the path is complete: fold its levels from the deepest up
-        f3 value = mk(0, 0, 0);
-        for (int k = nlev - 1; k >= 0; k--) {
-            f3 dk, T;
-            bool has_pdf;
-            hist.get(k, dk, has_pdf, T);
-            f3 v = dk;
-            if (has_pdf) v = v + T * value;
-            value = v;
-        }
-        float4 col = make_float4(value.x, value.y, value.z, 1.0f);
-        if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-        else if (A.acc64) {  // converged-reference accumulator (FP64 only here)
-            double *sum = A.acc64 + 3 * (size_t)pix;
-            double *sq = A.acc64 + 3 * ((size_t)li.width * li.height + pix);  // per-pixel sample variance
-            const double v[3] = {(double)value.x, (double)value.y, (double)value.z};
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                sum[c] += v[c];
-                sq[c] += v[c] * v[c];
-            }
-        }
-        const bool acc = li.accumulate != 0;
-        li.color[pix] = accumulate4(li.color[pix], col, acc, count);
-        li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
-        li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
-        li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
-        active = false;
     }
-    wave_count(A.ctr, rc.closest);
-    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
+    emit_ray(B.sl, &nsl, shadow, so, sd, stmax, pix);
+    emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        B.sl.counts[blockIdx.x] = nsl;
+        B.cl.counts[blockIdx.x] = ncl;
+    }
+}
+
+__global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, PtWave B, double *acc64) {
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const int nlev = __float_as_int(B.state[pix].w);
+    f3 value = mk(0, 0, 0);
+    for (int k = nlev - 1; k >= 0; k--) {
+        const float4 *hl = B.hist + 3 * (size_t)k * npix;
+        const float4 e = hl[pix], t = hl[npix + pix], c = hl[2 * (size_t)npix + pix];
+        f3 Ldv = mk(0.0f, 0.0f, 0.0f);
+        if (t.w != 0.0f) {
+            const f3 contr = B.occ[(size_t)k * npix + pix] ? mk(0.0f, 0.0f, 0.0f) : mk(c.x, c.y, c.z);
+            Ldv = contr / c.w;
+        }
+        f3 v = mk(e.x, e.y, e.z) + Ldv;
+        if (e.w != 0.0f) v = v + mk(t.x, t.y, t.z) * value;
+        value = v;
+    }
+    float4 col = make_float4(value.x, value.y, value.z, 1.0f);
+    if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+    else if (acc64) {  // converged-reference accumulator (FP64 only here)
+        double *sum = acc64 + 3 * (size_t)pix;
+        double *sq = acc64 + 3 * ((size_t)npix + pix);  // per-pixel sample variance
+        const double v[3] = {(double)value.x, (double)value.y, (double)value.z};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sum[c] += v[c];
+            sq[c] += v[c] * v[c];
+        }
+    }
+    const bool acc = li.accumulate != 0;
+    const float count = (float)li.samples;
+    li.color[pix] = accumulate4(li.color[pix], col, acc, count);
+    if (nlev == 0) {  // the primary ray missed: the AOVs take zeros, as pt_pixel's
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
+        li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
+        li.position[pix] = zero;
+    }
 }
 
 }  // namespace
 
-void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, DevBuf &history, unsigned long long *ctr,
-               PassTimes &times, double *acc64) {
+void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,
+               double *acc64) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
     const uint32_t npix = li.width * li.height;
-    uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
-    uint32_t nthreads = nb * kTraceBlock;
-    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
-    if (history.bytes < need) history.alloc(need);
-    uint32_t *work = ctx.counters.as<uint32_t>();
-    KRT_HIP(hipMemsetAsync(work, 0, sizeof(uint32_t), ctx.stream));
-    PtArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, acc64, work};
+    const SceneView sv = make_view(s);
+    WaveScratch &ws = ctx.wave;
+    const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
+    const uint32_t gb = (items + kWaveBlock - 1) / kWaveBlock;
+    const size_t levels = (size_t)li.max_depth + 1;
+    const size_t cap = (size_t)gb * kWaveBlock;
+    auto grow = [](DevBuf &b, size_t bytes) {
+        if (b.bytes < bytes) b.alloc(bytes);
+    };
+    grow(ws.hits, sizeof(float4) * npix);
+    if (s.instanced) grow(ws.inst, sizeof(int) * npix);
+    grow(ws.state, sizeof(float4) * npix);
+    grow(ws.hist, sizeof(float4) * 3 * levels * npix);
+    grow(ws.occ, levels * npix);
+    for (int l = 0; l < 2; l++) {
+        grow(ws.rays[l], sizeof(float4) * 2 * cap);
+        grow(ws.counts[l], sizeof(uint32_t) * gb);
+        grow(ws.prefix[l], sizeof(uint32_t) * (gb + 1));
+        grow(ws.dense[l], sizeof(float4) * 2 * cap);
+    }
+    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
+    int *spill = ctx.spill.as<int>();
+    PtWave B;
+    B.cl = RayList{ws.rays[0].as<float4>(), ws.counts[0].as<uint32_t>(), (uint32_t)kWaveBlock, gb,
+                   ws.prefix[0].as<uint32_t>(), ws.dense[0].as<float4>()};
+    B.sl = RayList{ws.rays[1].as<float4>(), ws.counts[1].as<uint32_t>(), (uint32_t)kWaveBlock, gb,
+                   ws.prefix[1].as<uint32_t>(), ws.dense[1].as<float4>()};
+    B.hits = ws.hits.as<float4>();
+    B.hit_inst = s.instanced ? ws.inst.as<int>() : nullptr;
+    B.state = ws.state.as<float4>();
+    B.hist = ws.hist.as<float4>();
+    B.occ = ws.occ.as<uint8_t>();
     hipEvent_t e0, e1;
     KRT_HIP(hipEventCreate(&e0));
     KRT_HIP(hipEventCreate(&e1));
     KRT_HIP(hipEventRecord(e0, ctx.stream));
-    if (s.instanced) pt_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-    else pt_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    pt_camera_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B);
+    KRT_HIP(hipGetLastError());
+    for (int level = 0; level <= li.max_depth; level++) {
+        walk_closest_list(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, spill, ctr, 0);
+        if (s.instanced) pt_shade_kernel<true><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
+        else pt_shade_kernel<false><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
+        KRT_HIP(hipGetLastError());
+        walk_shadow_list(ctx, s.instanced, sv, B.sl, B.occ + (size_t)level * npix, spill, ctr, 0);
+    }
+    pt_compose_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B, acc64);
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(e1, ctx.stream));
     KRT_HIP(hipEventSynchronize(e1));

@@ -57,6 +57,28 @@ GI_SPATIAL_PASSES = 2
 GI_PIXEL_BYTES = {0: 36 + 80 + 16 + 16, 1: 2 * (36 + 80) + 80,
                   2: GI_SPATIAL_PASSES * (6 * (36 + 80) + 80) + 16 + 32}
 PASS_NAMES = {0: "initial", 1: "temporal", 2: "spatial"}
+# Launches of each timed pass (one HIP-event segment each).  The passes split
+# at their rays: producer kernels append rays to per-workgroup list segments,
+# seg_scan + seg_compact pack them, closest_list / shadow_list walk them.
+LIST = ["seg_scan_kernel", "seg_compact_kernel"]
+PASS_KERNELS = {
+    "restir_di:initial": ["init_camera_kernel", "closest_list_kernel", "init_shade_kernel", "shadow_list_kernel",
+                          "init_compose_kernel"] + LIST,
+    "restir_di:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
+    "restir_di:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
+                          "spatial_resolve_kernel"] + LIST,
+    "restir_cache:initial": ["initial_kernel"],
+    "restir_cache:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
+    "restir_cache:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
+                             "spatial_resolve_kernel"] + LIST,
+    "pt:path": ["pt_camera_kernel", "closest_list_kernel", "pt_shade_kernel", "shadow_list_kernel",
+                "pt_compose_kernel"] + LIST,
+    "restir_gi:initial": ["gi_initial_kernel"],
+    "restir_gi:temporal": ["gi_temporal_kernel"],
+    "restir_gi:spatial": ["gi_spatial_rays_kernel", "shadow_list_kernel", "gi_spatial_merge_kernel",
+                          "gi_spatial_resolve_kernel"] + LIST,
+    "pt_cache:path": ["cache_path_kernel"],
+}
 PT_PIXEL_BYTES = 112  # color/normal/albedo float4 read+write + position write
 
 
@@ -637,14 +659,12 @@ def main():
     sec = kavg[dom] * 1e-3
     achieved = alg_bytes / sec / 1e9
     total_rays = rays / K
-    # rocprofv3 kernel names (profiles/traffic.json keys)
-    if wl == "restir_gi":  # both spatial passes of a frame (profiles/traffic.json)
-        kname = {0: "restir_gi:gi_initial", 1: "restir_gi:gi_temporal", 2: "restir_gi:gi_spatial_frame"}[dom]
-    else:
-        kname = f"{wl}:{PASS_NAMES[dom]}" if restir else {"pt": "pt:pt", "pt_cache": "pt_cache:cache_path"}[wl]
+    # the dominant pass (profiles/traffic.json keys: HBM bytes of all its
+    # launches in one frame; ReSTIR GI: both spatial passes)
+    kname = f"{wl}:{PASS_NAMES[dom]}" if restir else f"{wl}:path"
     traffic = measured_traffic(kname)
     copy_ms = ctx.copy_probe(1 << 30, 10)
-    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roof = {"bound": "hbm", "kernel": kname, "launches": PASS_KERNELS.get(kname), "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg_bytes,
             "ms": float(kavg[dom]), "rays": float(prays[dom] / K),
             "bytes_per_ray": float((node_b * pn[dom] + 48.0 * pt[dom]) / max(prays[dom] / K, 1)),
@@ -678,7 +698,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": f"{wl} {W}x{H} spheres 256x4096 tris (1,048,576), {desc}",
+        "config": {"workload": f"{wl} {W}x{H} spheres {scene.triangles:,} triangles, {desc}",
                    "width": W, "height": H, "triangles": int(scene.triangles), "traversal_tree": args.layout,
                    "acceleration": "two-level (16 meshes, 256 instances)" if args.instanced else "flat",
                    "parallelism": "single GPU"},
