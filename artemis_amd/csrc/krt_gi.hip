@@ -137,11 +137,9 @@ struct GiArgs {
     GPlanes cur, prev;
     GBuf g, pg;
     float4 *direct, *aux;
-    float4 *hist;
     int *spill;
     unsigned long long *ctr;
     int m_cap, neighbours, radius, shade;
-    uint32_t *work;  // initial pass: work-item counter (zeroed before the launch)
 };
 
 KRT_DEV bool gi_visible(const SceneView &sv, f3 xv, f3 xs, const TraceStack &stk, TravStats &ts, RayCount &rc) {
@@ -171,162 +169,222 @@ KRT_DEV void gi_shade(const GiArgs &A, uint32_t pix, const GPix &gb, const GRes 
     li.color[pix] = accumulate4(li.color[pix], c4, li.accumulate != 0, (float)li.samples);
 }
 
-// Four waves per SIMD (128 VGPRs, 28 bytes of scratch spills): 7.3 ms vs 8.7 ms
-// at three waves on the 1080p, 2M-triangle frame.
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void gi_initial_kernel(GiArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
-    const LaunchInfo &li = A.li;
-    const SceneView &sv = A.sv;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    History hist{A.hist + gtid, nthreads};
-    TravStats ts;
-    RayCount rc;
-    const f3 center = ld3(li.center);
+// The initial pass in wavefront form, split at every ray as the path
+// tracer is.  Level 0 is the primary hit x_v (direct
+// = Le + Ld, the BSDF sample wi_0); levels 1..max_depth are the secondary
+// path, whose value is the sample's L_o.  Per level: a closest walk of the
+// level's rays, gi_shade_kernel (Ld's light sample and shadow ray, the BSDF
+// sample -> the next level's ray, the level's history), a shadow walk; then
+// gi_compose_kernel folds the history and forms the sample, the reservoir and
+// the G-buffer.  The same expressions in the same order (the random sequence
+// included) in every launch; the oracle holds it bit for bit.
+struct GiWave {
+    RayList cl, sl;
+    float4 *hits;   // per pixel: {t, b1, b2, prim bits}
+    float4 *state;  // per pixel: {seed, secondary levels with history (int bits)}
+    float4 *hist;   // per level, per pixel: {Le, has_pdf}, {T, Ld flag}, {Ld contribution unblocked, Ld pdf}
+    float4 *v0;     // per pixel: {x_v, pdf0}, {n_v, material id bits}, {wi_0, flags: 1 hit0, 2 valid1}, {n_1, t_1}
+    uint8_t *occ;   // per level, per pixel
+};
+
+constexpr int kGiWaveBlock = 256;
+
+__global__ __launch_bounds__(kGiWaveBlock) void gi_camera_kernel(LaunchInfo li, GiWave B) {
+    __shared__ uint32_t nray;
+    if (threadIdx.x == 0) nray = 0;
+    __syncthreads();
     const TileOrder tiles(li.width, li.height);
-    const uint32_t lane = threadIdx.x & 63u;
-    // Path regeneration (as the path tracer's kernel): a lane whose pixel is
-    // finished takes the next one at once; the wave takes its refills with
-    // one atomic, consecutive work items in 8x8-tile order.  A pixel is two
-    // stages: the primary hit (level 0) and the secondary path (levels
-    // 1..max_depth), one level per iteration.
-    bool active = false, done = false, hit0 = false;
-    uint32_t pix = 0, gmat = 0;
-    int level = 0, nlev = 0;
-    bool valid1 = false;
-    float t1 = 0.0f, pdf0 = 0.0f;
-    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), direct = mk(0, 0, 0);
-    f3 xv = mk(0, 0, 0), nv = mk(0, 0, 0), wi0 = mk(0, 0, 0), n1 = mk(0, 0, 0);
-    for (;;) {
-        const bool need = !active && !done;
-        const unsigned long long want = __ballot(need);
-        if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                uint32_t px, py;
-                if (item >= tiles.items) {
-                    done = true;
-                } else if (tiles.pixel(item, px, py)) {
-                    pix = py * li.width + px;
-                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-                    make_ray(li, px, py, seed, false, o, d);
-                    direct = mk(0, 0, 0);
-                    level = 0;
-                    nlev = 0;
-                    hit0 = false;
-                    valid1 = false;
-                    active = true;
-                }
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (!active) continue;
-        bool end = true;
-        Hit h;
-        rc.closest++;
-        traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
-        Surface sf;
-        const bool surf = h.prim >= 0 && load_surface(sv, h, d, sf);
-        if (level == 0) {
-            if (surf) {
-                Material mat = load_material(sv.mats, sf.mat_id);
-                nv = sf.n;
-                xv = offset_point(sf.x, nv, mat.type);
-                SurfaceHit sh{mat, sf.entering, nv, -d, xv};
-                direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
-                int out;
-                eval_bsdf(sh, wi0, pdf0, out, seed);
-                gmat = sf.mat_id;
-                hit0 = true;
-                if (pdf0 > 0.0f && li.max_depth >= 1) {
-                    // the path tracer from level 1 (source/optix/amadeus_path_tracer.cu:133-201)
-                    o = xv;
-                    d = wi0;
-                    level = 1;
-                    end = false;
-                }
-            }
-        } else if (surf) {
-            Material m2 = load_material(sv.mats, sf.mat_id);
-            f3 nn = sf.n;
-            f3 xx = offset_point(sf.x, nn, m2.type);
-            SurfaceHit sh2{m2, sf.entering, nn, -d, xx};
-            f3 dk = m2.emission + Ld(sv, sh2, seed, stk, ts, rc);
-            int out2;
-            f3 wi2;
-            float pdf2;
-            f3 f2 = eval_bsdf(sh2, wi2, pdf2, out2, seed);
-            f3 T2 = f2 * fabsf(dot(wi2, nn)) / pdf2;
-            bool has_pdf = pdf2 > 0.0f;
-            hist.put(level - 1, dk, has_pdf, T2);
-            if (level == 1) {
-                valid1 = true;
-                t1 = h.t;
-                n1 = nn;
-            }
-            nlev = level;
-            if (has_pdf && level < li.max_depth) {
-                o = xx;
-                d = wi2;
-                level++;
-                end = false;
-            }
-        }
-        if (!end) continue;
-        // the pixel is complete: its sample, reservoir and G-buffer
-        GRes R = A.cur.load(pix);
-        R.w = 0.0f;
-        R.M = 0;
-        R.W = 0.0f;
-        float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
-        uint32_t gm = 0;
-        if (hit0) {
-            GSample smp;
-            smp.xs = xv + wi0;
-            smp.ns = -wi0;
-            smp.Lo = mk(0, 0, 0);
-            if (pdf0 > 0.0f && li.max_depth >= 1) {
-                f3 value = mk(0, 0, 0);
-                for (int k = nlev - 1; k >= 0; k--) {
-                    f3 dk, Tk;
-                    bool hp;
-                    hist.get(k, dk, hp, Tk);
-                    f3 v = dk;
-                    if (hp) v = v + Tk * value;
-                    value = v;
-                }
-                if (valid1) {
-                    smp.xs = xv + wi0 * t1;
-                    smp.ns = n1;
-                    smp.Lo = value;
-                }
-            }
-            float ph = gi_target(smp, xv, nv);
-            R.update(smp, (pdf0 > 0.0f) ? ph / pdf0 : 0.0f);
-            float dd = ph * (float)R.M;
-            R.W = (dd > 0.0f) ? R.w / dd : 0.0f;
-            R.xo = xv;
-            gxd = make_float4(xv.x, xv.y, xv.z, length(xv - center));
-            gnv = make_float4(nv.x, nv.y, nv.z, 1.0f);
-            gm = gmat;
-        }
-        A.g.xd[pix] = gxd;
-        A.g.nv[pix] = gnv;
-        A.g.mat[pix] = gm;
-        A.direct[pix] = make_float4(direct.x, direct.y, direct.z, 0.0f);
-        A.aux[pix] = make_float4(seed.x, seed.y, seed.z, 0.0f);
-        A.cur.store(pix, R);
-        active = false;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px = 0, py = 0, pix = 0;
+    const bool live = item < tiles.items && tiles.pixel(item, px, py);
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    if (live) {
+        pix = py * li.width + px;
+        f3 seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
+        make_ray(li, px, py, seed, false, o, d);
+        B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(0));
+        B.v0[2 * (size_t)npix + pix] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
     }
-    wave_count(A.ctr, rc.closest);
-    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
+    emit_ray(B.cl, &nray, live, o, d, kTraceTmax, pix);
+    publish_rays(B.cl, &nray);
+}
+
+__global__ __launch_bounds__(kGiWaveBlock) void gi_shade_kernel(LaunchInfo li, SceneView sv, GiWave B, int level) {
+    __shared__ uint32_t ncl, nsl;
+    if (threadIdx.x == 0) ncl = nsl = 0;
+    __syncthreads();
+    const uint32_t npix = li.width * li.height;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned total = sv.nl;
+    const bool live = k < B.cl.prefix[B.cl.nseg];
+    bool shadow = false, next = false;
+    f3 so = mk(0, 0, 0), sd = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
+    float stmax = 0.0f;
+    uint32_t pix = 0;
+    if (live) {
+        const float4 rb = B.cl.dense[2 * (size_t)k + 1];
+        const f3 d = mk(rb.x, rb.y, rb.z);
+        pix = __float_as_uint(rb.w);
+        const float4 st = B.state[pix];
+        f3 seed = mk(st.x, st.y, st.z);
+        const float4 hh = B.hits[pix];
+        Hit h;
+        h.t = hh.x;
+        h.b1 = hh.y;
+        h.b2 = hh.z;
+        h.prim = __float_as_int(hh.w);
+        h.rank = 0;
+        h.inst = -1;
+        Surface sf;
+        if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
+            const Material mat = load_material(sv.mats, sf.mat_id);
+            const f3 n = sf.n;
+            const f3 x = offset_point(sf.x, n, mat.type);
+            const SurfaceHit sh{mat, sf.entering, n, -d, x};
+            // Ld (krt_shade.h) with its shadow ray deferred
+            f3 contr = mk(0, 0, 0);
+            float lpdf = 0.0f;
+            bool ld = false;
+            if (total == 0) {
+                rand_uniform(seed);
+            } else {
+                unsigned i = pick_light(seed, total);
+                float pdf = 1.0f / (float)total;
+                TriLight L = load_light(sv.lights, i);
+                float light_pdf = 1.0f / L.area;
+                f3 lpos = sample_area_light(L, seed);
+                f3 wi = normalize(lpos - sh.x);
+                float R = length(lpos - sh.x);
+                f3 f = brdf(sh, wi, eDiffuse);
+                float ldot = fabsf(dot(L.normal, wi));
+                float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
+                contr = f * L.intensity * geometric;
+                pdf *= light_pdf;
+                lpdf = pdf;
+                ld = true;
+                so = sh.x;
+                sd = wi;
+                stmax = R - kShadowEps;
+                shadow = true;
+            }
+            int out;
+            f3 wi;
+            float pdf;
+            f3 f = eval_bsdf(sh, wi, pdf, out, seed);
+            float4 *hl = B.hist + 3 * (size_t)level * npix;
+            hl[2 * (size_t)npix + pix] = make_float4(contr.x, contr.y, contr.z, lpdf);
+            if (level == 0) {
+                hl[pix] = make_float4(mat.emission.x, mat.emission.y, mat.emission.z, 0.0f);
+                hl[npix + pix] = make_float4(0.0f, 0.0f, 0.0f, ld ? 1.0f : 0.0f);
+                B.v0[pix] = make_float4(x.x, x.y, x.z, pdf);
+                B.v0[npix + pix] = make_float4(n.x, n.y, n.z, __uint_as_float(sf.mat_id));
+                B.v0[2 * (size_t)npix + pix] = make_float4(wi.x, wi.y, wi.z, __int_as_float(1));
+                B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(0));
+                if (pdf > 0.0f && li.max_depth >= 1) {
+                    no = x;
+                    nd = wi;
+                    next = true;
+                }
+            } else {
+                f3 T = f * fabsf(dot(wi, n)) / pdf;
+                bool has_pdf = pdf > 0.0f;
+                hl[pix] = make_float4(mat.emission.x, mat.emission.y, mat.emission.z, has_pdf ? 1.0f : 0.0f);
+                hl[npix + pix] = make_float4(T.x, T.y, T.z, ld ? 1.0f : 0.0f);
+                if (level == 1) {
+                    float4 &w0 = B.v0[2 * (size_t)npix + pix];
+                    w0.w = __int_as_float(__float_as_int(w0.w) | 2);
+                    B.v0[3 * (size_t)npix + pix] = make_float4(n.x, n.y, n.z, h.t);
+                }
+                B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(level));
+                if (has_pdf && level < li.max_depth) {
+                    no = x;
+                    nd = wi;
+                    next = true;
+                }
+            }
+        }
+    }
+    emit_ray(B.sl, &nsl, shadow, so, sd, stmax, pix);
+    emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        B.sl.counts[blockIdx.x] = nsl;
+        B.cl.counts[blockIdx.x] = ncl;
+    }
+}
+
+__global__ __launch_bounds__(kGiWaveBlock) void gi_compose_kernel(GiArgs A, GiWave B) {
+    const LaunchInfo &li = A.li;
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const float4 st = B.state[pix];
+    const int nlev = __float_as_int(st.w);
+    const float4 w0 = B.v0[2 * (size_t)npix + pix];
+    const int flags = __float_as_int(w0.w);
+    const bool hit0 = (flags & 1) != 0, valid1 = (flags & 2) != 0;
+    // Le + Ld of the given level, the shadow ray's answer applied
+    auto direct_of = [&](int lv) {
+        const float4 *hl = B.hist + 3 * (size_t)lv * npix;
+        const float4 e = hl[pix], t = hl[npix + pix], c = hl[2 * (size_t)npix + pix];
+        f3 Ldv = mk(0.0f, 0.0f, 0.0f);
+        if (t.w != 0.0f) {
+            const f3 contr = B.occ[(size_t)lv * npix + pix] ? mk(0.0f, 0.0f, 0.0f) : mk(c.x, c.y, c.z);
+            Ldv = contr / c.w;
+        }
+        return mk(e.x, e.y, e.z) + Ldv;
+    };
+    f3 direct = mk(0, 0, 0);
+    if (hit0) direct = direct_of(0);
+    GRes R = A.cur.load(pix);
+    R.w = 0.0f;
+    R.M = 0;
+    R.W = 0.0f;
+    float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
+    uint32_t gm = 0;
+    if (hit0) {
+        const float4 a = B.v0[pix], b = B.v0[npix + pix];
+        const f3 xv = mk(a.x, a.y, a.z), nv = mk(b.x, b.y, b.z), wi0 = mk(w0.x, w0.y, w0.z);
+        const float pdf0 = a.w;
+        GSample smp;
+        smp.xs = xv + wi0;
+        smp.ns = -wi0;
+        smp.Lo = mk(0, 0, 0);
+        if (pdf0 > 0.0f && li.max_depth >= 1) {
+            f3 value = mk(0, 0, 0);
+            for (int k = nlev - 1; k >= 0; k--) {
+                const float4 *hl = B.hist + 3 * (size_t)(k + 1) * npix;
+                const float4 e = hl[pix], t = hl[npix + pix];
+                f3 v = direct_of(k + 1);
+                if (e.w != 0.0f) v = v + mk(t.x, t.y, t.z) * value;
+                value = v;
+            }
+            if (valid1) {
+                const float4 q = B.v0[3 * (size_t)npix + pix];
+                smp.xs = xv + wi0 * q.w;
+                smp.ns = mk(q.x, q.y, q.z);
+                smp.Lo = value;
+            }
+        }
+        float ph = gi_target(smp, xv, nv);
+        R.update(smp, (pdf0 > 0.0f) ? ph / pdf0 : 0.0f);
+        float dd = ph * (float)R.M;
+        R.W = (dd > 0.0f) ? R.w / dd : 0.0f;
+        R.xo = xv;
+        gxd = make_float4(xv.x, xv.y, xv.z, length(xv - ld3(li.center)));
+        gnv = make_float4(nv.x, nv.y, nv.z, 1.0f);
+        gm = __float_as_uint(b.w);
+    }
+    A.g.xd[pix] = gxd;
+    A.g.nv[pix] = gnv;
+    A.g.mat[pix] = gm;
+    A.direct[pix] = make_float4(direct.x, direct.y, direct.z, 0.0f);
+    A.aux[pix] = make_float4(st.x, st.y, st.z, 0.0f);
+    A.cur.store(pix, R);
 }
 
 KRT_DEV int gi_reproject(const LaunchInfo &li, uint32_t pix, f3 x) {
@@ -671,8 +729,6 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     const uint32_t npix = li.width * li.height;
     uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
     uint32_t nthreads = nb * kTraceBlock;
-    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
-    if (st.history.bytes < need) st.history.alloc(need);
     auto planes = [&](int b) { return GPlanes{st.res[b].as<float4>(), npix}; };
     auto gbuf = [&](int b) { return GBuf{st.gxd[b].as<float4>(), st.gnv[b].as<float4>(), st.gmat[b].as<uint32_t>()}; };
     GiArgs A;
@@ -682,17 +738,14 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     A.pg = gbuf(1 - st.gcur);
     A.direct = st.direct.as<float4>();
     A.aux = st.aux.as<float4>();
-    A.hist = st.history.as<float4>();
-    // one spill slab for the pass kernels (32-entry LDS stacks) and the
-    // spatial shadow-list walks (16 entries, one workgroup per list
-    // segment): size it for both first
+    // one spill slab for the temporal kernel (32-entry LDS stacks) and the
+    // list walks (16 entries, up to kMaxResidentBlocks workgroups): size it
+    // for both first
     const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
     const uint32_t gb = (items + kGiSpatialBlock - 1) / kGiSpatialBlock;  // spatial producers = list segments
     spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
-    A.work = ctx.counters.as<uint32_t>();
-    KRT_HIP(hipMemsetAsync(A.work, 0, sizeof(uint32_t), ctx.stream));
     A.m_cap = opt.m_cap;
     A.neighbours = opt.neighbours;
     A.radius = opt.radius;
@@ -702,7 +755,44 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     A.cur = planes(st.cur);
     A.prev = planes(1 - st.cur);
     A.shade = 0;
-    gi_initial_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
+    {
+        WaveScratch &ws = ctx.wave;
+        const size_t levels = (size_t)li.max_depth + 1;
+        const size_t cap = (size_t)gb * kGiWaveBlock;
+        auto grow = [](DevBuf &b, size_t bytes) {
+            if (b.bytes < bytes) b.alloc(bytes);
+        };
+        grow(ws.hits, sizeof(float4) * npix);
+        grow(ws.state, sizeof(float4) * npix);
+        grow(ws.hist, sizeof(float4) * 3 * levels * npix);
+        grow(ws.occ, levels * npix);
+        grow(ws.v0, sizeof(float4) * 4 * npix);
+        for (int l = 0; l < 2; l++) {
+            grow(ws.rays[l], sizeof(float4) * 2 * cap);
+            grow(ws.counts[l], sizeof(uint32_t) * gb);
+            grow(ws.prefix[l], sizeof(uint32_t) * (gb + 1));
+            grow(ws.dense[l], sizeof(float4) * 2 * cap);
+        }
+        GiWave W;
+        W.cl = RayList{ws.rays[0].as<float4>(), ws.counts[0].as<uint32_t>(), (uint32_t)kGiWaveBlock, gb,
+                       ws.prefix[0].as<uint32_t>(), ws.dense[0].as<float4>()};
+        W.sl = RayList{ws.rays[1].as<float4>(), ws.counts[1].as<uint32_t>(), (uint32_t)kGiWaveBlock, gb,
+                       ws.prefix[1].as<uint32_t>(), ws.dense[1].as<float4>()};
+        W.hits = ws.hits.as<float4>();
+        W.state = ws.state.as<float4>();
+        W.hist = ws.hist.as<float4>();
+        W.v0 = ws.v0.as<float4>();
+        W.occ = ws.occ.as<uint8_t>();
+        gi_camera_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(li, W);
+        KRT_HIP(hipGetLastError());
+        for (int level = 0; level <= li.max_depth; level++) {
+            walk_closest_list(ctx, false, A.sv, W.cl, W.hits, nullptr, A.spill, ctr, 0);
+            gi_shade_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(li, A.sv, W, level);
+            KRT_HIP(hipGetLastError());
+            walk_shadow_list(ctx, false, A.sv, W.sl, W.occ + (size_t)level * npix, A.spill, ctr, 0);
+        }
+        gi_compose_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(A, W);
+    }
     KRT_HIP(hipGetLastError());
     KRT_HIP(hipEventRecord(ev[1], ctx.stream));
     A.shade = opt.spatial_passes == 0;

@@ -145,6 +145,7 @@ SceneView make_view(const Scene &s);
 // launch sequence uses it from start to end on the context stream).
 struct WaveScratch {
     DevBuf hits, inst, state, hist, occ;
+    DevBuf v0;  // ReSTIR GI: primary-hit state (4 float4 per pixel)
     DevBuf rays[2], counts[2], prefix[2], dense[2];  // closest-ray list, shadow-ray list
 };
 
@@ -219,7 +220,6 @@ struct GiState {
     DevBuf gxd[2], gnv[2], gmat[2];  // G-buffer of this and the last frame
     int gcur = 0;
     DevBuf direct, aux;
-    DevBuf history;
     // spatial pass: segmented shadow-ray list, segment counts and offsets,
     // packed list, occlusion bytes, 1/Z weights
     DevBuf rays, seg_counts, seg_prefix, dense, occ, zm;

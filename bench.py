@@ -73,7 +73,8 @@ PASS_KERNELS = {
                              "spatial_resolve_kernel"] + LIST,
     "pt:path": ["pt_camera_kernel", "closest_list_kernel", "pt_shade_kernel", "shadow_list_kernel",
                 "pt_compose_kernel"] + LIST,
-    "restir_gi:initial": ["gi_initial_kernel"],
+    "restir_gi:initial": ["gi_camera_kernel", "closest_list_kernel", "gi_shade_kernel", "shadow_list_kernel",
+                          "gi_compose_kernel"] + LIST,
     "restir_gi:temporal": ["gi_temporal_kernel"],
     "restir_gi:spatial": ["gi_spatial_rays_kernel", "shadow_list_kernel", "gi_spatial_merge_kernel",
                           "gi_spatial_resolve_kernel"] + LIST,
