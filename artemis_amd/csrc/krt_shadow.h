@@ -33,6 +33,14 @@ struct RayList {
     float4 *dense;
 };
 
+// Ray records are read once per walk: streaming loads keep them from
+// displacing tree nodes in L2.
+KRT_DEV float4 stream_load(const float4 *p) {
+    const float *f = reinterpret_cast<const float *>(p);
+    return make_float4(__builtin_nontemporal_load(f), __builtin_nontemporal_load(f + 1),
+                       __builtin_nontemporal_load(f + 2), __builtin_nontemporal_load(f + 3));
+}
+
 // Wave-aggregated append to the workgroup's segment: one LDS atomic per
 // wavefront.  Every lane of the wave must call it (lanes without a ray pass
 // valid = false); `nray` is the workgroup's LDS counter, zeroed before the
@@ -111,9 +119,9 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_list_kernel(SceneView sv, 
     unsigned nr = 0;
     const uint32_t n = L.prefix[L.nseg];
     for (uint32_t k = gtid; k < n; k += nthreads) {
-        const float4 a = L.dense[2 * (size_t)k], c = L.dense[2 * (size_t)k + 1];
+        const float4 a = stream_load(L.dense + 2 * (size_t)k), c = stream_load(L.dense + 2 * (size_t)k + 1);
         const bool r = occluded<WIDE, INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, ts);
-        occ[__float_as_uint(c.w)] = r ? 1 : 0;
+        __builtin_nontemporal_store((uint8_t)(r ? 1 : 0), occ + __float_as_uint(c.w));
         nr++;
     }
     wave_count(ctr ? ctr + 1 + pass : nullptr, nr);
@@ -137,11 +145,15 @@ __global__ __launch_bounds__(kTraceBlock) void closest_list_kernel(SceneView sv,
     unsigned nr = 0;
     const uint32_t n = L.prefix[L.nseg];
     for (uint32_t k = gtid; k < n; k += nthreads) {
-        const float4 a = L.dense[2 * (size_t)k], c = L.dense[2 * (size_t)k + 1];
+        const float4 a = stream_load(L.dense + 2 * (size_t)k), c = stream_load(L.dense + 2 * (size_t)k + 1);
         Hit h;
         trace_closest<INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, h, ts);
         const uint32_t slot = __float_as_uint(c.w);
-        hits[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
+        float *hp = reinterpret_cast<float *>(hits + slot);
+        __builtin_nontemporal_store(h.t, hp);
+        __builtin_nontemporal_store(h.b1, hp + 1);
+        __builtin_nontemporal_store(h.b2, hp + 2);
+        __builtin_nontemporal_store(__int_as_float(h.prim), hp + 3);
         if (INST) hit_inst[slot] = h.inst;
         nr++;
     }
