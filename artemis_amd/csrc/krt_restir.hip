@@ -115,6 +115,14 @@ KRT_DEV float target(const RSample &s, const SurfaceHit &hit) {
     return Li.x + Li.y + Li.z;
 }
 
+// A resampling ray whose answer cannot change anything: the candidate's
+// weight is target * W * M, and with W = 0 and a finite unshadowed target it
+// is 0 whether the ray is blocked or not (a zero-weight candidate is never
+// selected).  Producers skip such rays; consumers take them as unblocked.
+KRT_DEV bool moot_ray(float W, f3 unshadowed) {
+    return W == 0.0f && isfinite(unshadowed.x + unshadowed.y + unshadowed.z);
+}
+
 template <bool INST>
 KRT_DEV float occluded_target(const SceneView &sv, const RSample &s, const SurfaceHit &hit, const TraceStack &stk,
                               TravStats &ts, RayCount &rc) {
@@ -651,14 +659,16 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs
     f3 x = mk(0, 0, 0), d0 = mk(0, 0, 0), d1 = mk(0, 0, 0);
     float t0 = 0.0f, t1 = 0.0f;
     bool v1 = false;
+    bool v0 = false;
     if (live) {
-        const float4 xx = li.position[pix];
-        x = mk(xx.x, xx.y, xx.z);
-        const float4 c = A.cur.p[2 * (size_t)A.cur.n + pix];
-        f3 D = mk(c.x, c.y, c.z) - x;
+        const SurfaceHit hit = gbuffer_hit(A, pix);
+        x = hit.x;
+        const Resv cr = A.cur.load(pix);
+        f3 D = cr.data.point - x;
         float d = length(D);
         d0 = D / d;
         t0 = d - kShadowEps;
+        v0 = !moot_ray(cr.W, direct_li(hit, cr.data.Le, cr.data.normal, cr.data.type, d0, d));
         const int pindex = reproject(li, (int)pix, x);
         if (pindex >= 0) {
             const int pm = A.prev.M((uint32_t)pindex);
@@ -673,7 +683,7 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs
             }
         }
     }
-    emit_ray(B.list, &nray, live, x, d0, t0, pix);
+    emit_ray(B.list, &nray, v0, x, d0, t0, pix);
     emit_ray(B.list, &nray, v1, x, d1, t1, npix + pix);
     publish_rays(B.list, &nray);
 }
@@ -699,7 +709,8 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_resolve_kernel(RestirA
     // occluded_target: 0 when the ray was blocked, else the unshadowed sum
     f3 D = cr.data.point - hit.x;
     float d = length(D);
-    f3 Li = B.occ[pix] ? mk(0.0f, 0.0f, 0.0f) : direct_li(hit, cr.data.Le, cr.data.normal, cr.data.type, D / d, d);
+    const f3 Lu = direct_li(hit, cr.data.Le, cr.data.normal, cr.data.type, D / d, d);
+    f3 Li = (!moot_ray(cr.W, Lu) && B.occ[pix]) ? mk(0.0f, 0.0f, 0.0f) : Lu;
     const float t_current = Li.x + Li.y + Li.z;
     merged.update(cr.data, t_current * cr.W * (float)cr.M);
     int N = cr.M;
@@ -739,9 +750,10 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
         live = A.cur.M(pix) > 0;
     }
     f3 x = mk(0, 0, 0), rs = mk(0, 0, 0);
+    SurfaceHit hit;
     if (live) {
-        const float4 xx = li.position[pix];
-        x = mk(xx.x, xx.y, xx.z);
+        hit = gbuffer_hit(A, pix);
+        x = hit.x;
         rs = ld3(A.auxiliary[pix]);
         rand_uniform(rs);  // the current reservoir's update
     }
@@ -751,14 +763,14 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
         float tmax = 0.0f;
         if (live) {
             const uint32_t i0 = spatial_neighbour_px(li, pix, rs, A.radius);
-            if (A.cur.M(i0) > 0) {
+            const Resv nb = A.cur.load(i0);
+            if (nb.M > 0) {
                 rand_uniform(rs);  // that neighbour's update
-                const float4 c = A.cur.p[2 * (size_t)A.cur.n + i0];
-                const f3 Dn = mk(c.x, c.y, c.z) - x;
+                const f3 Dn = nb.data.point - x;
                 const float dn = length(Dn);
                 dir = Dn / dn;
                 tmax = dn - kShadowEps;
-                valid = true;
+                valid = !moot_ray(nb.W, direct_li(hit, nb.data.Le, nb.data.normal, nb.data.type, dir, dn));
             }
         }
         emit_ray(B.list, &nray, valid, x, dir, tmax, (uint32_t)i * npix + pix);
@@ -807,9 +819,9 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
             if (nb.M > 0) {
                 const f3 Dn = nb.data.point - hit.x;
                 const float dn = length(Dn);
-                const bool occ = B.occ[(size_t)i * npix + pix] != 0;
-                const f3 Ln = occ ? mk(0.0f, 0.0f, 0.0f)
-                                  : direct_li(hit, nb.data.Le, nb.data.normal, nb.data.type, Dn / dn, dn);
+                const f3 Lu = direct_li(hit, nb.data.Le, nb.data.normal, nb.data.type, Dn / dn, dn);
+                const bool occ = !moot_ray(nb.W, Lu) && B.occ[(size_t)i * npix + pix] != 0;
+                const f3 Ln = occ ? mk(0.0f, 0.0f, 0.0f) : Lu;
                 const float t_nb = Ln.x + Ln.y + Ln.z;
                 if (merged.update(nb.data, t_nb * nb.W * (float)nb.M)) {
                     sel_known = true;
