@@ -805,7 +805,8 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
     merged.M = 0;
     merged.seed = mk(0, 0, 0);
     SurfaceHit hit;
-    bool sel_known = false, sel_occ = false;
+    bool sel_known = false, sel_occ = false, z_moot = false, final_moot = false;
+    float t_merged = 0.0f;
     if (live) {
         hit = gbuffer_hit(A, pix);
         const Resv current = A.cur.load(pix);
@@ -832,7 +833,18 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
         }
         merged.M = current.M + cnt;
         A.prev.store(pix, merged);
-        B.flags[pix] = (uint8_t)((sel_known ? 1 : 0) | (sel_occ ? 2 : 0));
+        // W = w / (target * Z) is 0 whatever Z when w = 0 or the target is
+        // not positive: then no 1/Z ray can change it, nor the shading ray
+        // (finite lighting x 0)
+        t_merged = target(merged.data, hit);
+        z_moot = merged.w == 0.0f || !(t_merged > 0.0f);
+        if (z_moot && !sel_known) {
+            const f3 D = merged.data.point - hit.x;
+            const float d = length(D);
+            const f3 Lf = direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
+            final_moot = isfinite(Lf.x) && isfinite(Lf.y) && isfinite(Lf.z);
+        }
+        B.flags[pix] = (uint8_t)((sel_known ? 1 : 0) | (sel_occ ? 2 : 0) | (final_moot ? 4 : 0));
     }
     // 1/Z rays: replay the picks from the auxiliary seed
     f3 rs = mk(0, 0, 0);
@@ -860,11 +872,11 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
                 const f3 D = merged.data.point - nh.x;
                 const float d = length(D);
                 const f3 Li = direct_li(nh, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
-                zm = (Li.x + Li.y + Li.z > 0.0f) ? m : 0;
+                zm = (!z_moot && Li.x + Li.y + Li.z > 0.0f) ? m : 0;
                 o = nh.x;
                 dir = D / d;
                 tmax = d - kShadowEps;
-                valid = true;
+                valid = zm > 0;  // the ray counts only towards a positive target
             }
             B.zm[(size_t)i * npix + pix] = zm;
         }
@@ -873,7 +885,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
     bool valid = false;
     f3 dir = mk(0, 0, 0);
     float tmax = 0.0f;
-    if (live && !sel_known) {
+    if (live && !sel_known && !final_moot) {
         const f3 D = merged.data.point - hit.x;
         const float d = length(D);
         dir = D / d;
@@ -911,7 +923,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_resolve_kernel(RestirAr
         const f3 D = merged.data.point - hit.x;
         const float d = length(D);
         const uint8_t fl = B.flags[pix];
-        const bool occ = (fl & 1) ? (fl & 2) != 0 : B.occ[(size_t)S * npix + pix] != 0;
+        const bool occ = (fl & 1) ? (fl & 2) != 0 : ((fl & 4) ? false : B.occ[(size_t)S * npix + pix] != 0);
         const f3 lighting = occ ? mk(0.0f, 0.0f, 0.0f)
                                 : direct_li(hit, merged.data.Le, merged.data.normal, merged.data.type, D / d, d);
         direct = hit.mat.emission + lighting * merged.W;
