@@ -95,13 +95,45 @@ __global__ __launch_bounds__(kScanBlock) void seg_scan_kernel(RayList L) {
 
 constexpr int kCompactBlock = 256;
 
-// Segment b -> dense[prefix[b] ...], one workgroup per segment.
-__global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L) {
+// Segment b -> dense[prefix[b] ...], one workgroup per segment.  Shadow rays
+// (`by_octant`) are grouped by direction octant with a counting sort in LDS,
+// so that a wavefront of the walk starts 64 rays of one octant (their
+// targets are scattered light samples); closest rays keep their order, in
+// which a wavefront's rays come from one 8x8 tile.  Results are written by
+// slot, so the order changes no answer.
+__global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L, bool by_octant) {
+    __shared__ uint32_t bins[8], base[8];
     const uint32_t b = blockIdx.x;
-    const uint32_t n2 = 2 * L.counts[b];
+    const uint32_t n = L.counts[b];
     const float4 *src = L.rays + 2 * (size_t)b * L.seg;
     float4 *dst = L.dense + 2 * (size_t)L.prefix[b];
-    for (uint32_t k = threadIdx.x; k < n2; k += kCompactBlock) dst[k] = src[k];
+    if (!by_octant) {
+        for (uint32_t k = threadIdx.x; k < 2 * n; k += kCompactBlock) dst[k] = src[k];
+        return;
+    }
+    if (threadIdx.x < 8) bins[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += kCompactBlock) {
+        const float4 c = src[2 * k + 1];
+        const uint32_t oct = (c.x < 0.0f ? 1u : 0u) | (c.y < 0.0f ? 2u : 0u) | (c.z < 0.0f ? 4u : 0u);
+        atomicAdd(&bins[oct], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int o = 0; o < 8; o++) {
+            base[o] = run;
+            run += bins[o];
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += kCompactBlock) {
+        const float4 a = src[2 * k], c = src[2 * k + 1];
+        const uint32_t oct = (c.x < 0.0f ? 1u : 0u) | (c.y < 0.0f ? 2u : 0u) | (c.z < 0.0f ? 4u : 0u);
+        const uint32_t at = atomicAdd(&base[oct], 1u);
+        dst[2 * at] = a;
+        dst[2 * at + 1] = c;
+    }
 }
 
 // Any-hit walks of the dense list (occluded(), amadeus_common.cuh:467-478),
@@ -176,7 +208,7 @@ inline void walk_shadow_list(Context &ctx, bool instanced, const SceneView &sv, 
     if (L.nseg == 0) return;
     seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
-    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
+    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L, true);
     KRT_HIP(hipGetLastError());
     const uint32_t nb = walk_blocks(L);
     if (instanced && wide)
@@ -197,7 +229,7 @@ inline void walk_closest_list(Context &ctx, bool instanced, const SceneView &sv,
     if (L.nseg == 0) return;
     seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(L);
     KRT_HIP(hipGetLastError());
-    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L);
+    seg_compact_kernel<<<L.nseg, kCompactBlock, 0, ctx.stream>>>(L, false);
     KRT_HIP(hipGetLastError());
     const uint32_t nb = walk_blocks(L);
     if (instanced)
