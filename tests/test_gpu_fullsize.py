@@ -1,0 +1,87 @@
+"""Checks at the benchmark's full size: 1920x1080 on the 1,048,576-triangle
+sphere scene (256 x sphere(64, 33), 4,096 emitters), the configuration the
+performance figures are quoted on.  The oracle cannot render whole frames of
+it inside a test, so the checks are the ones that hold at any size: the BVH
+bit for bit, path-tracer pixels spot-checked bit for bit against the oracle
+(pixels are independent), and determinism of ReSTIR DI / GI frames (two
+fresh runs give the same bits; reservoirs stay finite and within the M-cap)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from artemis_amd import ArmadaRTX, PathTracer, ReSTIR, ReSTIRGI, look_at, scenes
+
+pytestmark = pytest.mark.gpu
+
+W, H = 1920, 1080
+
+
+@pytest.fixture(scope="module")
+def bench_scene(krt_ctx):
+    data = scenes.spheres()
+    assert data.triangles == 1_048_576
+    print(f"[fullsize] uploading {data.triangles} triangles", flush=True)
+    sc = krt_ctx.upload(data)
+    yield data, sc
+    sc.free()
+
+
+def test_bvh_1m_bit_exact(bench_scene):
+    data, sc = bench_scene
+    ref = O.bvh_build(O.tri_boxes(data.positions, data.indices))
+    np.testing.assert_array_equal(sc.serialized().view(np.uint32), ref.view(np.uint32))
+
+
+def test_pt_1080p_pixels_match_oracle(krt_ctx, bench_scene):
+    data, sc = bench_scene
+    cam = look_at(**data.camera, aspect=W / H)
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("path_tracer", PathTracer(krt_ctx))
+    arm.set_depth(5)
+    arm.render(sc, cam, accumulate=False)  # sample 0: the colour buffer holds the sample itself
+    img = arm.color_buffer().reshape(-1, 4)
+    osc = O.OracleScene(data, sc.serialized())
+    rng = np.random.default_rng(11)
+    pix = np.unique(rng.integers(0, W * H, 3000)).astype(np.int32)
+    ref = osc.render_pt_pixels(cam, W, H, pix, sample=0, max_depth=5)
+    got = img[pix]
+    same = np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1)
+    print(f"  {int(same.sum())} of {len(pix)} pixels bit-exact; "
+          f"{np.mean(np.any(ref[:, :3] != 0, axis=1)):.3f} non-black", flush=True)
+    assert same.all(), f"first differing pixels {pix[~same][:5]}: got {got[~same][:2]} want {ref[~same][:2]}"
+
+
+def _restir_frames(ctx, sc, cam, kind, frames):
+    arm = ArmadaRTX(ctx, W, H)
+    if kind == "restir":
+        arm.attach("restir", ReSTIR(ctx))
+        arm.set_option("m_cap", 20)
+        arm.set_option("spatial_passes", 1)
+        arm.set_depth(2)
+    else:
+        arm.attach("restir_gi", ReSTIRGI(ctx))
+        arm.set_option("m_cap", 30)
+        arm.set_option("spatial_passes", 2)
+        arm.set_depth(3)
+    for _ in range(frames):
+        arm.render(sc, cam, accumulate=True)
+    return arm
+
+
+@pytest.mark.parametrize("kind", ["restir", "restir_gi"])
+def test_restir_1080p_deterministic(krt_ctx, bench_scene, kind):
+    data, sc = bench_scene
+    cam = look_at(**data.camera, aspect=W / H)
+    a = _restir_frames(krt_ctx, sc, cam, kind, 3)
+    b = _restir_frames(krt_ctx, sc, cam, kind, 3)
+    ca, cb = a.color_buffer(), b.color_buffer()
+    assert np.array_equal(ca.view(np.uint32), cb.view(np.uint32))
+    assert np.isfinite(ca).all()
+    r = a.reservoirs(1, gi=kind == "restir_gi")
+    m_col = 12 if kind == "restir" else 14
+    M = r.reshape(-1, r.shape[-1])[:, m_col]
+    print(f"  {kind}: mean colour {ca[..., :3].mean():.4f}, M in [{M.min():.0f}, {M.max():.0f}]", flush=True)
+    assert np.isfinite(r).all()
+    assert M.min() >= 0
+    st = a.stats()
+    assert st["shadow_rays"] > 0 and st["closest_rays"] >= W * H
