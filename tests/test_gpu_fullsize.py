@@ -4,7 +4,7 @@ performance figures are quoted on.  The oracle cannot render whole frames of
 it inside a test, so the checks are the ones that hold at any size: the BVH
 bit for bit, path-tracer pixels spot-checked bit for bit against the oracle
 (pixels are independent), and determinism of ReSTIR DI / GI frames (two
-fresh runs give the same bits; reservoirs stay finite and within the M-cap)."""
+fresh runs give the same bits; reservoirs and colours stay finite)."""
 import numpy as np
 import pytest
 
