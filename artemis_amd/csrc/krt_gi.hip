@@ -743,7 +743,7 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     // for both first
     const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
     const uint32_t gb = (items + kGiSpatialBlock - 1) / kGiSpatialBlock;  // spatial producers = list segments
-    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
+    spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
     A.m_cap = opt.m_cap;
@@ -785,11 +785,16 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
         W.occ = ws.occ.as<uint8_t>();
         gi_camera_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(li, W);
         KRT_HIP(hipGetLastError());
+        walk_closest_list(ctx, false, A.sv, W.cl, W.hits, nullptr, A.spill, ctr, 0);
         for (int level = 0; level <= li.max_depth; level++) {
-            walk_closest_list(ctx, false, A.sv, W.cl, W.hits, nullptr, A.spill, ctr, 0);
             gi_shade_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(li, A.sv, W, level);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, false, A.sv, W.sl, W.occ + (size_t)level * npix, A.spill, ctr, 0);
+            // the level's shadow rays and the next level's closest rays
+            if (level < li.max_depth)
+                walk_pair(ctx, false, A.sv, W.cl, W.hits, nullptr, W.sl, W.occ + (size_t)level * npix, A.spill, ctr,
+                          0);
+            else
+                walk_shadow_list(ctx, false, A.sv, W.sl, W.occ + (size_t)level * npix, A.spill, ctr, 0);
         }
         gi_compose_kernel<<<gb, kGiWaveBlock, 0, ctx.stream>>>(A, W);
     }

@@ -223,7 +223,7 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long 
         grow(ws.prefix[l], sizeof(uint32_t) * (gb + 1));
         grow(ws.dense[l], sizeof(float4) * 2 * cap);
     }
-    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
+    spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     int *spill = ctx.spill.as<int>();
     PtWave B;
     B.cl = RayList{ws.rays[0].as<float4>(), ws.counts[0].as<uint32_t>(), (uint32_t)kWaveBlock, gb,
@@ -241,12 +241,17 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long 
     KRT_HIP(hipEventRecord(e0, ctx.stream));
     pt_camera_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B);
     KRT_HIP(hipGetLastError());
+    walk_closest_list(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, spill, ctr, 0);
     for (int level = 0; level <= li.max_depth; level++) {
-        walk_closest_list(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, spill, ctr, 0);
         if (s.instanced) pt_shade_kernel<true><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
         else pt_shade_kernel<false><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
         KRT_HIP(hipGetLastError());
-        walk_shadow_list(ctx, s.instanced, sv, B.sl, B.occ + (size_t)level * npix, spill, ctr, 0);
+        // the level's shadow rays and the next level's closest rays
+        if (level < li.max_depth)
+            walk_pair(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, B.sl, B.occ + (size_t)level * npix, spill, ctr,
+                      0);
+        else
+            walk_shadow_list(ctx, s.instanced, sv, B.sl, B.occ + (size_t)level * npix, spill, ctr, 0);
     }
     pt_compose_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B, acc64);
     KRT_HIP(hipGetLastError());

@@ -1023,7 +1023,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     // size it for both first
     const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
     const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;  // spatial producers = list segments
-    spill_buffer(ctx, s, kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
+    spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.spill = spill_buffer(ctx, s, nthreads);
     A.ctr = ctr;
     A.work = ctx.counters.as<uint32_t>();
@@ -1100,12 +1100,17 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         W.occ = st.wf_occ.as<uint8_t>();
         init_camera_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
         KRT_HIP(hipGetLastError());
+        walk_closest_list(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, lean_spill, ctr, 0);
         for (int level = 0; level <= li.max_depth; level++) {
-            walk_closest_list(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, lean_spill, ctr, 0);
             if (s.instanced) init_shade_kernel<true><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
             else init_shade_kernel<false><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
             KRT_HIP(hipGetLastError());
-            walk_shadow_list(ctx, s.instanced, A.sv, W.sl, W.occ + (size_t)level * npix, lean_spill, ctr, 0);
+            // the level's shadow rays and the next level's closest rays
+            if (level < li.max_depth)
+                walk_pair(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, W.sl, W.occ + (size_t)level * npix,
+                          lean_spill, ctr, 0);
+            else
+                walk_shadow_list(ctx, s.instanced, A.sv, W.sl, W.occ + (size_t)level * npix, lean_spill, ctr, 0);
         }
         init_compose_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
         KRT_HIP(hipGetLastError());

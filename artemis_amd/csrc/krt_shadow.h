@@ -241,5 +241,78 @@ inline void walk_closest_list(Context &ctx, bool instanced, const SceneView &sv,
     KRT_HIP(hipGetLastError());
 }
 
+// One launch for a level's two walks (the shadow rays of level L and the
+// closest rays of level L + 1, both produced by the level-L shading kernel):
+// the first `nbc` workgroups walk the closest list, the rest the shadow list
+// (4-wide tree), so the two walks' tails overlap and one launch gap goes.
+// Spill columns are indexed over the whole grid.
+template <bool INST>
+__global__ __launch_bounds__(kTraceBlock) void walk_pair_kernel(SceneView sv, RayList cl, float4 *__restrict__ hits,
+                                                                int *__restrict__ hit_inst, RayList sl,
+                                                                uint8_t *__restrict__ occ, int *spill,
+                                                                unsigned long long *ctr, int pass, uint32_t nbc) {
+    __shared__ int lds_stack[kLdsDepthLean * kTraceBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepthLean, spill + gid, (int)(gridDim.x * blockDim.x)};
+    TravStats ts;
+    unsigned nr = 0;
+    if (blockIdx.x < nbc) {
+        const uint32_t nthreads = nbc * blockDim.x;
+        const uint32_t n = cl.prefix[cl.nseg];
+        for (uint32_t k = gid; k < n; k += nthreads) {
+            const float4 a = stream_load(cl.dense + 2 * (size_t)k), c = stream_load(cl.dense + 2 * (size_t)k + 1);
+            Hit h;
+            trace_closest<INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, h, ts);
+            const uint32_t slot = __float_as_uint(c.w);
+            float *hp = reinterpret_cast<float *>(hits + slot);
+            __builtin_nontemporal_store(h.t, hp);
+            __builtin_nontemporal_store(h.b1, hp + 1);
+            __builtin_nontemporal_store(h.b2, hp + 2);
+            __builtin_nontemporal_store(__int_as_float(h.prim), hp + 3);
+            if (INST) hit_inst[slot] = h.inst;
+            nr++;
+        }
+        wave_count(ctr, nr);
+        wave_count(ctr ? ctr + 4 + pass : nullptr, ts.nodes);
+        wave_count(ctr ? ctr + 7 + pass : nullptr, ts.tris);
+    } else {
+        const uint32_t nthreads = (gridDim.x - nbc) * blockDim.x;
+        const uint32_t n = sl.prefix[sl.nseg];
+        for (uint32_t k = gid - nbc * blockDim.x; k < n; k += nthreads) {
+            const float4 a = stream_load(sl.dense + 2 * (size_t)k), c = stream_load(sl.dense + 2 * (size_t)k + 1);
+            const bool r = occluded<true, INST>(sv, mk(a.x, a.y, a.z), mk(c.x, c.y, c.z), a.w, stk, ts);
+            __builtin_nontemporal_store((uint8_t)(r ? 1 : 0), occ + __float_as_uint(c.w));
+            nr++;
+        }
+        wave_count(ctr ? ctr + 1 + pass : nullptr, nr);
+        wave_count(ctr ? ctr + 4 + pass : nullptr, ts.nodes);
+        wave_count(ctr ? ctr + 7 + pass : nullptr, ts.tris);
+    }
+}
+
+// Packs both lists, then walks them in one launch (walk_pair_kernel); the
+// caller sized the spill slab for 2 * kMaxResidentBlocks * kTraceBlock lanes
+// at kLdsDepthLean.
+inline void walk_pair(Context &ctx, bool instanced, const SceneView &sv, const RayList &cl, float4 *hits,
+                      int *hit_inst, const RayList &sl, uint8_t *occ, int *spill, unsigned long long *ctr, int pass) {
+    if (cl.nseg == 0 || sl.nseg == 0) return;
+    seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(cl);
+    KRT_HIP(hipGetLastError());
+    seg_compact_kernel<<<cl.nseg, kCompactBlock, 0, ctx.stream>>>(cl, false);
+    KRT_HIP(hipGetLastError());
+    seg_scan_kernel<<<1, kScanBlock, 0, ctx.stream>>>(sl);
+    KRT_HIP(hipGetLastError());
+    seg_compact_kernel<<<sl.nseg, kCompactBlock, 0, ctx.stream>>>(sl, true);
+    KRT_HIP(hipGetLastError());
+    const uint32_t nbc = walk_blocks(cl), nbs = walk_blocks(sl);
+    if (instanced)
+        walk_pair_kernel<true><<<nbc + nbs, kTraceBlock, 0, ctx.stream>>>(sv, cl, hits, hit_inst, sl, occ, spill, ctr,
+                                                                          pass, nbc);
+    else
+        walk_pair_kernel<false><<<nbc + nbs, kTraceBlock, 0, ctx.stream>>>(sv, cl, hits, hit_inst, sl, occ, spill,
+                                                                           ctr, pass, nbc);
+    KRT_HIP(hipGetLastError());
+}
+
 }  // namespace
 }  // namespace krt
