@@ -59,11 +59,13 @@ GI_PIXEL_BYTES = {0: 36 + 80 + 16 + 16, 1: 2 * (36 + 80) + 80,
 PASS_NAMES = {0: "initial", 1: "temporal", 2: "spatial"}
 # Launches of each timed pass (one HIP-event segment each).  The passes split
 # at their rays: producer kernels append rays to per-workgroup list segments,
-# seg_scan + seg_compact pack them, closest_list / shadow_list walk them.
+# seg_scan + seg_compact pack them, closest_list / shadow_list walk them
+# (walk_pair: a level's shadow rays and the next level's closest rays in one
+# launch).
 LIST = ["seg_scan_kernel", "seg_compact_kernel"]
 PASS_KERNELS = {
-    "restir_di:initial": ["init_camera_kernel", "closest_list_kernel", "init_shade_kernel", "shadow_list_kernel",
-                          "init_compose_kernel"] + LIST,
+    "restir_di:initial": ["init_camera_kernel", "closest_list_kernel", "init_shade_kernel", "walk_pair_kernel",
+                          "shadow_list_kernel", "init_compose_kernel"] + LIST,
     "restir_di:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
     "restir_di:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
                           "spatial_resolve_kernel"] + LIST,
@@ -71,10 +73,10 @@ PASS_KERNELS = {
     "restir_cache:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
     "restir_cache:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
                              "spatial_resolve_kernel"] + LIST,
-    "pt:path": ["pt_camera_kernel", "closest_list_kernel", "pt_shade_kernel", "shadow_list_kernel",
-                "pt_compose_kernel"] + LIST,
-    "restir_gi:initial": ["gi_camera_kernel", "closest_list_kernel", "gi_shade_kernel", "shadow_list_kernel",
-                          "gi_compose_kernel"] + LIST,
+    "pt:path": ["pt_camera_kernel", "closest_list_kernel", "pt_shade_kernel", "walk_pair_kernel",
+                "shadow_list_kernel", "pt_compose_kernel"] + LIST,
+    "restir_gi:initial": ["gi_camera_kernel", "closest_list_kernel", "gi_shade_kernel", "walk_pair_kernel",
+                          "shadow_list_kernel", "gi_compose_kernel"] + LIST,
     "restir_gi:temporal": ["gi_temporal_kernel"],
     "restir_gi:spatial": ["gi_spatial_rays_kernel", "shadow_list_kernel", "gi_spatial_merge_kernel",
                           "gi_spatial_resolve_kernel"] + LIST,
