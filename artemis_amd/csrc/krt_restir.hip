@@ -1,6 +1,9 @@
 // ReSTIR DI attachment: the three per-pixel passes of
 // source/optix/amadeus_restir.cu with the host sequence of
-// include/amadeus/restir.cuh:223-304, as fused gfx950 kernels.
+// include/amadeus/restir.cuh:223-304.  Each pass is split at its rays into
+// producer / consumer kernels around the ray-list walks of krt_shadow.h (the
+// initial pass with the radiance cache is one fused kernel with path
+// regeneration).
 //
 //   initial  (__raygen__initial + __closesthit__initial, :82-144, :437-559):
 //            primary hit -> RIS over `candidates` light samples (target = sum
