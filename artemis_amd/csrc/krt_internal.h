@@ -200,15 +200,15 @@ struct RestirState {
     DevBuf gmat;          // uint material id per pixel
     DevBuf intermediate;  // float4 per pixel
     DevBuf auxiliary;     // float4 per pixel
-    DevBuf history;       // per-pixel path history scratch
     // spatial pass, split around its shadow rays: segmented ray list (2 float4
     // per ray), its segment counts and offsets, the packed list, one occlusion
     // byte per (ray slot, pixel), 1/Z weights per (neighbour, pixel),
     // selection flags per pixel
     DevBuf rays, seg_counts, seg_prefix, dense, occ, zm, flags;
     // wavefront initial pass: hits, hit instances, {seed, levels} per pixel,
-    // per-level history and occlusion bytes, the closest-ray list
-    DevBuf wf_hits, wf_inst, wf_state, wf_hist, wf_occ;
+    // per-level history and occlusion bytes, the cache vertex's emission, the
+    // closest-ray list
+    DevBuf wf_hits, wf_inst, wf_state, wf_hist, wf_occ, wf_lec;
     DevBuf cl_rays, cl_counts, cl_prefix, cl_dense;
     float prev_pv[16] = {0};
 };

@@ -141,12 +141,10 @@ struct RestirArgs {
     Planes cur, prev;  // prev: temporal source / spatial destination
     uint32_t *gmat;
     float4 *intermediate, *auxiliary;
-    float4 *hist;
     int *spill;
     unsigned long long *ctr;
     int candidates, m_cap, neighbours, radius;
     CacheView cv;  // radiance cache for the indirect path (cv.enabled = 0: off)
-    uint32_t *work;  // initial pass: work-item counter (zeroed before the launch)
 };
 
 // G-buffer surface of a pixel (amadeus_restir.cu:230-240, :323-336); the
@@ -160,208 +158,6 @@ KRT_DEV SurfaceHit gbuffer_hit(const RestirArgs &A, uint32_t i) {
     h.x = mk(x.x, x.y, x.z);
     h.wo = normalize(ld3(A.li.center) - h.x);
     return h;
-}
-
-// Four waves per SIMD: 128 VGPRs with a few dwords spilled to scratch runs
-// faster here than 139 VGPRs at three waves (7.5 vs 8.7 ms at 1080p).
-template <bool INST>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void initial_kernel(RestirArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
-    const LaunchInfo &li = A.li;
-    const SceneView &sv = A.sv;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    History hist{A.hist + gtid, nthreads};
-    TravStats ts;
-    RayCount rc;
-    const float count = (float)li.samples;
-    const unsigned total = sv.nl;
-    const TileOrder tiles(li.width, li.height);
-    const CacheView &cv = A.cv;
-    const bool acc = li.accumulate != 0;
-    const uint32_t lane = threadIdx.x & 63u;
-    // Path regeneration (as the path tracer's kernel): a lane whose path has
-    // ended takes the next pixel at once; the wave takes its refills with one
-    // atomic, consecutive work items in 8x8-tile order.  The reservoir and the
-    // level-0 AOVs are written as soon as they are final (level 0); a pixel
-    // whose primary ray misses writes them when its path ends.
-    bool active = false, done = false;
-    uint32_t pix = 0, slot = 0;
-    int level = 0, nlev = 0;
-    bool level0_done = false, train = false, queried = false;
-    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), le_c = mk(0, 0, 0);
-    for (;;) {
-        const bool need = !active && !done;
-        const unsigned long long want = __ballot(need);
-        if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                uint32_t px, py;
-                if (item >= tiles.items) {
-                    done = true;
-                } else if (tiles.pixel(item, px, py)) {
-                    pix = py * li.width + px;
-                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-                    make_ray(li, px, py, seed, false, o, d);
-                    level = 0;
-                    nlev = 0;
-                    level0_done = false;
-                    // radiance cache: the indirect path stops at level cv.depth (>= 1)
-                    slot = 0;
-                    train = cv.enabled && cv.training(pix, slot);
-                    queried = false;
-                    le_c = mk(0, 0, 0);
-                    active = true;
-                }
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (!active) continue;
-        bool end = true;
-        Hit h;
-        rc.closest++;
-        trace_closest<INST>(sv, o, d, kTraceTmax, stk, h, ts);
-        Surface sf;
-        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {
-            Material mat = load_material(sv.mats, sf.mat_id);
-            bool cache_stop = false;
-            if (cv.enabled && level == cv.depth) {
-                put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
-                          sf.n, mat);
-                queried = true;
-                le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
-                cache_stop = !train;
-            }
-            if (!cache_stop) {
-                f3 n = sf.n;
-                f3 x = offset_point(sf.x, n, mat.type);
-                SurfaceHit sh{mat, sf.entering, n, -d, x};
-                f3 direct = mk(0, 0, 0);
-                if (level == 0) {
-                    Resv res = A.cur.load(pix);
-                    res.w = 0.0f;
-                    res.M = 0;
-                    res.W = 0.0f;
-                    for (int i = 0; i < A.candidates; i++) {
-                        // sample_direct (amadeus_common.cuh:301-377), triangle lights
-                        RSample smp;
-                        smp.Le = smp.normal = smp.point = mk(0, 0, 0);
-                        smp.type = 0;
-                        float pdf = 0.0f;
-                        if (total > 0) {
-                            unsigned li_ = pick_light(seed, total);
-                            TriLight L = load_light(sv.lights, li_);
-                            smp.point = sample_area_light(L, seed);
-                            smp.Le = L.intensity;
-                            smp.normal = L.normal;
-                            smp.type = 1;
-                            pdf = 1.0f / (L.area * (float)total);
-                        } else {
-                            rand_uniform(seed);
-                        }
-                        f3 D = smp.point - sh.x;
-                        float dd = length(D);
-                        f3 Li = direct_li(sh, smp.Le, smp.normal, smp.type, D / dd, dd);
-                        float tgt = Li.x + Li.y + Li.z;
-                        res.update(smp, (pdf > 0.0f) ? tgt / pdf : 0.0f);
-                    }
-                    RSample smp = res.data;
-                    f3 D = smp.point - sh.x;
-                    float dd = length(D);
-                    bool occ;
-                    f3 Li = direct_occ<false, INST>(sv, sh, smp, D / dd, dd, stk, ts, rc, &occ);
-                    res.resample(Li.x + Li.y + Li.z);
-                    // the reference traces this same visibility ray a second time
-                    res.W *= 1.0f - (occ ? 1.0f : 0.0f);
-                    A.cur.store(pix, res);
-                    A.gmat[pix] = sf.mat_id;
-                } else {
-                    direct = mat.emission + Ld<false, INST>(sv, sh, seed, stk, ts, rc);
-                }
-                int out;
-                f3 wi;
-                float pdf;
-                f3 f = eval_bsdf(sh, wi, pdf, out, seed);
-                bool has_pdf = pdf > 0.0f;
-                f3 T = has_pdf ? f * fabsf(dot(wi, n)) / pdf : mk(0.0f, 0.0f, 0.0f);
-                hist.put(level, direct, true, T);  // value = direct + T * indirect
-                if (level == 0) {
-                    li.normal[pix] = accumulate4(li.normal[pix], make_float4(n.x, n.y, n.z, 0.0f), acc, count);
-                    li.albedo[pix] = accumulate4(
-                        li.albedo[pix], make_float4(mat.diffuse.x, mat.diffuse.y, mat.diffuse.z, 0.0f), acc, count);
-                    li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
-                    level0_done = true;
-                }
-                nlev = level + 1;
-                if (has_pdf && level < li.max_depth) {
-                    o = x;
-                    d = wi;
-                    level++;
-                    end = false;
-                }
-            }
-        }
-        if (!end) continue;
-        f3 value = mk(0, 0, 0);
-        if (queried && !train) {
-            // value = P + Theta * (Le_c + cache); the cache term is folded in
-            // after inference
-            f3 Th = mk(1.0f, 1.0f, 1.0f);
-            for (int k = nlev - 1; k >= 0; k--) {
-                f3 dk, T;
-                bool add;
-                hist.get(k, dk, add, T);
-                value = dk + T * value;
-                Th = T * Th;
-            }
-            value = value + Th * le_c;
-            cv.theta[pix] = make_float4(Th.x, Th.y, Th.z, 1.0f);
-        } else {
-            f3 target = mk(0, 0, 0);
-            for (int k = nlev - 1; k >= 0; k--) {
-                f3 dk, T;
-                bool add;
-                hist.get(k, dk, add, T);
-                value = dk + T * value;
-                if (cv.enabled && k == cv.depth) target = value - le_c;
-            }
-            if (cv.enabled) cv.theta[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (train) {
-                float *t = cv.slot_tgt + 3 * (size_t)slot;
-                t[0] = target.x;
-                t[1] = target.y;
-                t[2] = target.z;
-                cv.slot_ok[slot] = (queried && nlev > cv.depth && finite3(target)) ? 1u : 0u;
-            }
-        }
-        float4 col = make_float4(value.x, value.y, value.z, 1.0f);
-        if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-        A.intermediate[pix] = col;
-        A.auxiliary[pix] = make_float4(seed.x, seed.y, seed.z, 1.0f);
-        if (!level0_done) {
-            if (nlev == 0) {  // no reservoir written: primary miss or no surface
-                Resv res = A.cur.load(pix);
-                res.w = 0.0f;
-                res.M = 0;
-                res.W = 0.0f;
-                A.cur.store(pix, res);
-            }
-            const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
-            li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
-            li.position[pix] = zero;
-        }
-        active = false;
-    }
-    wave_count(A.ctr, rc.closest);
-    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
 }
 
 // reproject (amadeus_restir.cu:182-210); pixel indices clamped to the image.
@@ -412,21 +208,26 @@ struct SpatialBufs {
 
 constexpr int kSpatialBlock = 256;
 
-// Wavefront form of the initial pass (radiance cache off): the estimator of
-// initial_kernel split at every ray.  Per level L of the path,
-//   walk_closest_list      the level's rays (slot = pixel) -> hits
+// The initial pass (__raygen__initial + __closesthit__initial,
+// amadeus_restir.cu:82-144, :437-559) split at every ray.  Per level L of
+// the path,
+//   walk_closest_list / walk_pair   the level's rays (slot = pixel) -> hits
 //   init_shade_kernel<L>   surface, material; L = 0: RIS over the
 //                          candidates, the reservoir with W for an unblocked
 //                          sample, the sample's visibility ray; L >= 1: the
 //                          light sample of Ld and its shadow ray; the BSDF
-//                          sample -> the next level's ray; per-level history
-//   walk_shadow_list       -> occlusion byte of (L, pixel)
+//                          sample -> the next level's ray; per-level history.
+//                          With the radiance cache, the level `cache_depth`
+//                          hit records the MLP query and ends the path
+//                          (training paths go on to full depth).
+//   walk_shadow_list / walk_pair    -> occlusion byte of (L, pixel)
 // and init_compose_kernel folds the history from the deepest level up
-// (value_k = direct_k + T_k value_k+1), zeroes W of reservoirs whose sample
-// is blocked, and writes the intermediate and auxiliary buffers.  Every
-// number is computed by the expressions of the fused kernel in the same
-// order (the random sequence included), so the two agree bit for bit; the
-// walks run in the register-lean list kernels.
+// (value_k = direct_k + T_k value_k+1, or P + Theta (Le + cache) with the
+// cache), zeroes W of reservoirs whose sample is blocked, and writes the
+// intermediate and auxiliary buffers.  Every number is computed by the
+// expressions of the reference's programs in the same order, the random
+// sequence included (the oracle holds it bit for bit); the walks run in the
+// register-lean list kernels.
 struct InitWave {
     RayList cl;      // closest rays of the current level
     RayList sl;      // shadow rays of the current level
@@ -435,7 +236,12 @@ struct InitWave {
     float4 *state;   // per pixel: {seed, levels with history (int bits)}
     float4 *hist;    // per level, per pixel: {emission, 0}, {T, Ld flag}, {Ld contribution unblocked, Ld pdf}
     uint8_t *occ;    // per level, per pixel
+    float4 *lec;     // radiance cache: per pixel, the emission added exactly at the cache vertex
 };
+
+// state.w of the wavefront initial pass: levels with history in the low 16
+// bits, bit 16 set when the path recorded a radiance-cache query
+constexpr int kQueried = 1 << 16;
 
 __global__ __launch_bounds__(kSpatialBlock) void init_camera_kernel(RestirArgs A, InitWave B) {
     __shared__ uint32_t nray;
@@ -490,7 +296,25 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
         h.rank = 0;
         h.inst = INST ? B.hit_inst[pix] : -1;
         Surface sf;
-        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {
+        const CacheView &cv = A.cv;
+        const bool surf = h.prim >= 0 && load_surface<INST>(sv, h, d, sf);
+        int queried = __float_as_int(st.w) & kQueried;
+        bool cache_stop = false;
+        if (surf && cv.enabled && level == cv.depth) {
+            // the cache vertex: record the query; the path stops here unless
+            // the pixel is a training path
+            const Material mat = load_material(sv.mats, sf.mat_id);
+            uint32_t slot = 0;
+            const bool train = cv.training(pix, slot);
+            put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d, sf.n,
+                      mat);
+            const f3 le = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
+            B.lec[pix] = make_float4(le.x, le.y, le.z, 0.0f);
+            queried = kQueried;
+            B.state[pix] = make_float4(st.x, st.y, st.z, __int_as_float(__float_as_int(st.w) | kQueried));
+            cache_stop = !train;
+        }
+        if (surf && !cache_stop) {
             const Material mat = load_material(sv.mats, sf.mat_id);
             const f3 n = sf.n;
             const f3 x = offset_point(sf.x, n, mat.type);
@@ -580,7 +404,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
                                 count);
                 li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
             }
-            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(level + 1));
+            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float((level + 1) | queried));
             if (has_pdf && level < li.max_depth) {
                 no = x;
                 nd = wi;
@@ -606,8 +430,18 @@ __global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs 
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
     const uint32_t pix = py * li.width + px;
     const float4 st = B.state[pix];
-    const int nlev = __float_as_int(st.w);
-    f3 value = mk(0, 0, 0);
+    const int sw = __float_as_int(st.w);
+    const int nlev = sw & (kQueried - 1);
+    const bool queried = (sw & kQueried) != 0;
+    const CacheView &cv = A.cv;
+    uint32_t slot = 0;
+    const bool train = cv.enabled && cv.training(pix, slot);
+    f3 le_c = mk(0, 0, 0);
+    if (queried) {
+        const float4 l = B.lec[pix];
+        le_c = mk(l.x, l.y, l.z);
+    }
+    f3 value = mk(0, 0, 0), Th = mk(1.0f, 1.0f, 1.0f), target = mk(0, 0, 0);
     for (int k = nlev - 1; k >= 0; k--) {
         const float4 *hl = B.hist + 3 * (size_t)k * npix;
         const float4 e = hl[pix], t = hl[npix + pix], c = hl[2 * (size_t)npix + pix];
@@ -617,7 +451,24 @@ __global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs 
             Ldv = contr / c.w;
         }
         const f3 dk = mk(e.x, e.y, e.z) + Ldv;
-        value = dk + mk(t.x, t.y, t.z) * value;
+        const f3 T = mk(t.x, t.y, t.z);
+        value = dk + T * value;
+        Th = T * Th;
+        if (cv.enabled && k == cv.depth) target = value - le_c;
+    }
+    if (queried && !train) {
+        // value = P + Theta * (Le_c + cache); the cache term is folded in after inference
+        value = value + Th * le_c;
+        cv.theta[pix] = make_float4(Th.x, Th.y, Th.z, 1.0f);
+    } else if (cv.enabled) {
+        cv.theta[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (train) {
+            float *tg = cv.slot_tgt + 3 * (size_t)slot;
+            tg[0] = target.x;
+            tg[1] = target.y;
+            tg[2] = target.z;
+            cv.slot_ok[slot] = (queried && nlev > cv.depth && finite3(target)) ? 1u : 0u;
+        }
     }
     float4 col = make_float4(value.x, value.y, value.z, 1.0f);
     if (isnan3(value)) col = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
@@ -1010,27 +861,18 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     LaunchInfo li = li_in;
     for (int k = 0; k < 16; k++) li.prev_pv[k] = st.prev_pv[k];
     const uint32_t npix = li.width * li.height;
-    uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
-    uint32_t nthreads = nb * kTraceBlock;
-    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
-    if (st.history.bytes < need) st.history.alloc(need);
     RestirArgs A;
     A.li = li;
     A.sv = make_view(s);
     A.gmat = st.gmat.as<uint32_t>();
     A.intermediate = st.intermediate.as<float4>();
     A.auxiliary = st.auxiliary.as<float4>();
-    A.hist = st.history.as<float4>();
-    // one spill slab serves the pass kernels (32-entry LDS stacks) and the
-    // spatial shadow-list walks (16 entries, one workgroup per list segment):
-    // size it for both first
+    // producer / consumer workgroups (= list segments) and the walks' spill
+    // slab (16-entry LDS stacks, up to two roles of kMaxResidentBlocks)
     const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
-    const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;  // spatial producers = list segments
-    spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
-    A.spill = spill_buffer(ctx, s, nthreads);
+    const uint32_t gb = (items + kSpatialBlock - 1) / kSpatialBlock;
+    A.spill = spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     A.ctr = ctr;
-    A.work = ctx.counters.as<uint32_t>();
-    KRT_HIP(hipMemsetAsync(A.work, 0, sizeof(uint32_t), ctx.stream));
     A.candidates = opt.candidates;
     A.m_cap = opt.m_cap;
     A.neighbours = opt.neighbours;
@@ -1075,11 +917,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     mark();
     A.cur = planes(st.cur);
     A.prev = planes(1 - st.cur);
-    if (cached) {
-        if (s.instanced) initial_kernel<true><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-        else initial_kernel<false><<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-        KRT_HIP(hipGetLastError());
-    } else {
+    {
         const size_t levels = (size_t)li.max_depth + 1;
         const size_t cl_slots = (size_t)gb * kSpatialBlock;
         if (st.wf_hits.bytes < sizeof(float4) * npix) st.wf_hits.alloc(sizeof(float4) * npix);
@@ -1087,6 +925,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         if (st.wf_state.bytes < sizeof(float4) * npix) st.wf_state.alloc(sizeof(float4) * npix);
         if (st.wf_hist.bytes < sizeof(float4) * 3 * levels * npix) st.wf_hist.alloc(sizeof(float4) * 3 * levels * npix);
         if (st.wf_occ.bytes < levels * npix) st.wf_occ.alloc(levels * npix);
+        if (cached && st.wf_lec.bytes < sizeof(float4) * npix) st.wf_lec.alloc(sizeof(float4) * npix);
         if (st.cl_rays.bytes < sizeof(float4) * 2 * cl_slots) st.cl_rays.alloc(sizeof(float4) * 2 * cl_slots);
         if (st.cl_counts.bytes < sizeof(uint32_t) * gb) st.cl_counts.alloc(sizeof(uint32_t) * gb);
         if (st.cl_prefix.bytes < sizeof(uint32_t) * (gb + 1)) st.cl_prefix.alloc(sizeof(uint32_t) * (gb + 1));
@@ -1101,6 +940,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         W.state = st.wf_state.as<float4>();
         W.hist = st.wf_hist.as<float4>();
         W.occ = st.wf_occ.as<uint8_t>();
+        W.lec = st.wf_lec.as<float4>();
         init_camera_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
         KRT_HIP(hipGetLastError());
         walk_closest_list(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, lean_spill, ctr, 0);
