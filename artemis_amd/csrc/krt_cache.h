@@ -44,10 +44,10 @@ KRT_DEV bool finite3(f3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v
 // picks this frame's training pixels.
 CacheView cache_begin(Context &ctx, Scene &s, const CacheOptions &opt, CacheState &cs, uint32_t npix,
                       uint32_t user_seed);
-// MLP inference over every pixel's query record.
-void cache_infer(Context &ctx, CacheState &cs, uint32_t npix);
+// MLP inference over every pixel's query record, on `stream`.
+void cache_infer(CacheState &cs, uint32_t npix, hipStream_t stream);
 // intermediate[pix].xyz += Theta * max(prediction, 0) where the term applies.
-void cache_fold_intermediate(Context &ctx, CacheState &cs, uint32_t npix, float4 *intermediate);
+void cache_fold_intermediate(CacheState &cs, uint32_t npix, float4 *intermediate, hipStream_t stream);
 // Compaction of the valid training slots and one optimiser step.
 void cache_train(Context &ctx, CacheState &cs, const CacheView &cv);
 

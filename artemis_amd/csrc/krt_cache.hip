@@ -1,18 +1,19 @@
 // Path tracer with an online-trained radiance cache.
 //
-// Every pixel runs the path tracer's estimator (the loop of krt_pt.hip, after
-// source/optix/amadeus_path_tracer.cu:73-228) up to the cache vertex, the hit
-// at level `depth`; there the path stops and records a query {x, -d, n,
-// albedo, roughness} for the MLP, whose prediction stands in for the radiance
-// the rest of the path would have scattered toward the previous vertex:
+// Every pixel runs the path tracer's estimator (the passes of krt_pt.hip,
+// after source/optix/amadeus_path_tracer.cu:73-228) up to the cache vertex,
+// the hit at level `depth`; there the path stops and records a query {x, -d,
+// n, albedo, roughness} for the MLP, whose prediction stands in for the
+// radiance the rest of the path would have scattered toward the previous
+// vertex:
 //   value_0 = P + Theta * (Le(x_depth) + max(cache(x_depth, -d), 0)),
 // with (P, Theta) folded from the levels before the cache vertex in the
 // estimator's order.  The vertex's own emission is added exactly rather than
-// learned: a light's edge is a step the encoding cannot resolve.  A strided subset of pixels (a new offset every frame)
-// traces the full path instead, shows that unbiased value, and hands the value
-// at its cache vertex less that vertex's emission to the trainer as the target
-// for the same query.  Per
-// frame: path kernel -> MLP inference -> composite -> compaction of the
+// learned: a light's edge is a step the encoding cannot resolve.  A strided
+// subset of pixels (a new offset every frame) traces the full path instead,
+// shows that unbiased value, and hands the value at its cache vertex less that
+// vertex's emission to the trainer as the target for the same query.  Per
+// frame: path-tracer passes -> MLP inference -> composite -> compaction of the
 // training slots -> one optimiser step; nothing waits for the host.
 #include <hip/hip_runtime.h>
 
@@ -22,167 +23,10 @@
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
-#include "krt_shade.h"
-#include "krt_traverse.h"
 
 namespace krt {
 
 namespace {
-
-struct CacheArgs {
-    LaunchInfo li;
-    SceneView sv;
-    float4 *hist;
-    int *spill;
-    unsigned long long *ctr;
-    float4 *prefix;  // npix: P of value_0 = P + Theta * cache
-    CacheView cv;
-    uint32_t *work;  // work-item counter of the launch (zeroed before it)
-};
-
-__global__ __launch_bounds__(kTraceBlock) void cache_path_kernel(CacheArgs A) {
-    __shared__ int lds_stack[kLdsDepth * kTraceBlock];
-    const LaunchInfo &li = A.li;
-    const SceneView &sv = A.sv;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    TraceStack stk{lds_stack + threadIdx.x, kTraceBlock, kLdsDepth, A.spill + gtid, (int)nthreads};
-    History hist{A.hist + gtid, nthreads};
-    TravStats ts;
-    RayCount rc;
-    const float count = (float)li.samples;
-    const TileOrder tiles(li.width, li.height);
-    const CacheView &cv = A.cv;
-    const bool acc = li.accumulate != 0;
-    const uint32_t lane = threadIdx.x & 63u;
-    // Path regeneration (as the path tracer's kernel): most paths stop at the
-    // cache vertex while the strided training paths run to full depth, so a
-    // lane whose path has ended takes the next pixel at once instead of
-    // waiting for a training path of its wavefront.
-    bool active = false, done = false, train = false, queried = false;
-    uint32_t pix = 0, slot = 0;
-    int level = 0, nlev = 0;
-    f3 seed = mk(0, 0, 0), o = mk(0, 0, 0), d = mk(0, 0, 0), le_c = mk(0, 0, 0);
-    f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
-    for (;;) {
-        const bool need = !active && !done;
-        const unsigned long long want = __ballot(need);
-        if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.work, (uint32_t)__popcll(want));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                uint32_t px, py;
-                if (item >= tiles.items) {
-                    done = true;
-                } else if (tiles.pixel(item, px, py)) {
-                    pix = py * li.width + px;
-                    slot = 0;
-                    train = cv.training(pix, slot);
-                    seed = pixel_seed(px, py, (uint32_t)li.samples, li.user_seed);
-                    make_ray(li, px, py, seed, li.jitter != 0, o, d);
-                    pos0 = n0 = alb0 = mk(0, 0, 0);
-                    level = 0;
-                    nlev = 0;
-                    queried = false;
-                    le_c = mk(0, 0, 0);  // emission at the cache vertex, added exactly
-                    active = true;
-                }
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (!active) continue;
-        bool end = true;
-        Hit h;
-        rc.closest++;
-        traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
-        Surface sf;
-        if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
-            Material mat = load_material(sv.mats, sf.mat_id);
-            bool cache_stop = false;
-            if (level == cv.depth) {
-                put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d,
-                          sf.n, mat);
-                queried = true;
-                le_c = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
-                cache_stop = !train;
-            }
-            if (!cache_stop) {
-                f3 n = sf.n;
-                f3 x = offset_point(sf.x, n, mat.type);
-                SurfaceHit sh{mat, sf.entering, n, -d, x};
-                f3 direct = mat.emission + Ld<false>(sv, sh, seed, stk, ts, rc);  // binary: occupancy
-                int out;
-                f3 wi;
-                float pdf;
-                f3 f = eval_bsdf(sh, wi, pdf, out, seed);
-                f3 T = f * fabsf(dot(wi, n)) / pdf;
-                bool has_pdf = pdf > 0.0f;
-                hist.put(level, direct, has_pdf, T);
-                if (level == 0) {
-                    pos0 = x;
-                    n0 = n;
-                    alb0 = mat.diffuse;
-                }
-                nlev = level + 1;
-                if (has_pdf && level < li.max_depth) {
-                    o = x;
-                    d = wi;
-                    level++;
-                    end = false;
-                }
-            }
-        }
-        if (!end) continue;
-        float4 pre, th = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (queried && !train) {
-            // value_0 = P + Theta * cache, levels 0 .. depth-1 folded outside in
-            f3 P = mk(0, 0, 0), Th = mk(1.0f, 1.0f, 1.0f);
-            for (int k = nlev - 1; k >= 0; k--) {
-                f3 dk, T;
-                bool has_pdf;
-                hist.get(k, dk, has_pdf, T);
-                P = dk + T * P;
-                Th = T * Th;
-            }
-            P = P + Th * le_c;
-            pre = make_float4(P.x, P.y, P.z, 0.0f);
-            th = make_float4(Th.x, Th.y, Th.z, 1.0f);
-        } else {
-            f3 value = mk(0, 0, 0), target = mk(0, 0, 0);
-            for (int k = nlev - 1; k >= 0; k--) {
-                f3 dk, T;
-                bool has_pdf;
-                hist.get(k, dk, has_pdf, T);
-                f3 v = dk;
-                if (has_pdf) v = v + T * value;
-                value = v;
-                if (k == cv.depth) target = value - le_c;  // scattered radiance only
-            }
-            pre = make_float4(value.x, value.y, value.z, 0.0f);
-            if (train) {
-                const bool ok = queried && nlev > cv.depth && finite3(target);
-                float *t = cv.slot_tgt + 3 * (size_t)slot;
-                t[0] = target.x;
-                t[1] = target.y;
-                t[2] = target.z;
-                cv.slot_ok[slot] = ok ? 1u : 0u;
-            }
-        }
-        A.prefix[pix] = pre;
-        cv.theta[pix] = th;
-        li.normal[pix] = accumulate4(li.normal[pix], make_float4(n0.x, n0.y, n0.z, 0.0f), acc, count);
-        li.albedo[pix] = accumulate4(li.albedo[pix], make_float4(alb0.x, alb0.y, alb0.z, 0.0f), acc, count);
-        li.position[pix] = make_float4(pos0.x, pos0.y, pos0.z, 0.0f);
-        active = false;
-    }
-    wave_count(A.ctr, rc.closest);
-    wave_count(A.ctr ? A.ctr + 1 : nullptr, rc.shadow);
-    wave_count(A.ctr ? A.ctr + 4 : nullptr, ts.nodes);
-    wave_count(A.ctr ? A.ctr + 7 : nullptr, ts.tris);
-}
 
 __global__ void composite_kernel(LaunchInfo li, const float4 *__restrict__ prefix, const float4 *__restrict__ theta,
                                  const float *__restrict__ pred) {
@@ -215,35 +59,75 @@ __global__ void fold_kernel(uint32_t npix, const float4 *__restrict__ theta, con
     intermediate[pix] = im;
 }
 
-// Order-preserving compaction of the valid training slots (one workgroup).
-constexpr int kCompactThreads = 1024;
-__global__ __launch_bounds__(kCompactThreads) void compact_kernel(const float *__restrict__ slot_raw,
-                                                                  const float *__restrict__ slot_tgt,
-                                                                  const uint32_t *__restrict__ slot_ok, uint32_t M,
-                                                                  float *__restrict__ raw, float *__restrict__ tgt,
-                                                                  uint32_t *n_out) {
-    __shared__ uint32_t s[kCompactThreads];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (M + kCompactThreads - 1) / kCompactThreads;
-    const uint32_t b = tid * per, e = min(b + per, M);
-    uint32_t cnt = 0;
-    for (uint32_t i = b; i < e; i++) cnt += slot_ok[i];
-    s[tid] = cnt;
+// Order-preserving compaction of the valid training slots, in chunks of
+// kChunk slots: chunk_count_kernel counts each chunk's valid slots,
+// chunk_scan_kernel turns the counts into the chunks' first batch rows (and
+// the batch size), chunk_scatter_kernel ranks each chunk's valid slots and
+// copies their records, the chunk's threads sharing the copy word by word.
+constexpr int kChunk = 256;
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kChunk) void chunk_count_kernel(const uint32_t *__restrict__ slot_ok, uint32_t M,
+                                                             uint32_t *__restrict__ chunk_n) {
+    const uint32_t i = blockIdx.x * kChunk + threadIdx.x;
+    const int ok = i < M && slot_ok[i] != 0u;
+    const int n = __syncthreads_count(ok);
+    if (threadIdx.x == 0) chunk_n[blockIdx.x] = (uint32_t)n;
+}
+
+__global__ __launch_bounds__(kScanThreads) void chunk_scan_kernel(uint32_t *__restrict__ chunk_n, uint32_t nchunk,
+                                                                  uint32_t *__restrict__ n_out) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nchunk + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = min(t * per, nchunk), b1 = min(b0 + per, nchunk);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += chunk_n[b];
+    part[t] = sum;
     __syncthreads();
-    for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
-        const uint32_t v = tid >= off ? s[tid - off] : 0u;
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
         __syncthreads();
-        s[tid] += v;
+        part[t] += v;
         __syncthreads();
     }
-    uint32_t dst = s[tid] - cnt;
-    for (uint32_t i = b; i < e; i++) {
-        if (!slot_ok[i]) continue;
-        for (int a = 0; a < kMlpRaw; a++) raw[(size_t)kMlpRaw * dst + a] = slot_raw[(size_t)kMlpRaw * i + a];
-        for (int a = 0; a < 3; a++) tgt[3 * (size_t)dst + a] = slot_tgt[3 * (size_t)i + a];
-        dst++;
+    uint32_t run = part[t] - sum;  // exclusive: rows before this thread's chunks
+    for (uint32_t b = b0; b < b1; b++) {
+        const uint32_t n = chunk_n[b];
+        chunk_n[b] = run;
+        run += n;
     }
-    if (tid == kCompactThreads - 1) *n_out = s[tid];
+    if (t == kScanThreads - 1) *n_out = part[t];
+}
+
+__global__ __launch_bounds__(kChunk) void chunk_scatter_kernel(const float *__restrict__ slot_raw,
+                                                               const float *__restrict__ slot_tgt,
+                                                               const uint32_t *__restrict__ slot_ok, uint32_t M,
+                                                               const uint32_t *__restrict__ chunk_first,
+                                                               float *__restrict__ raw, float *__restrict__ tgt) {
+    constexpr int kWaves = kChunk / 64;
+    __shared__ uint32_t wave_n[kWaves];
+    __shared__ int dst[kChunk];  // batch row of each slot of the chunk, -1 when invalid
+    const uint32_t base = blockIdx.x * kChunk;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t i = base + t;
+    const bool ok = i < M && slot_ok[i] != 0u;
+    const unsigned long long bal = __ballot(ok);
+    if (lane == 0) wave_n[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t row = chunk_first[blockIdx.x] + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    for (uint32_t k = 0; k < w; k++) row += wave_n[k];
+    dst[t] = ok ? (int)row : -1;
+    __syncthreads();
+    const uint32_t nslot = min((uint32_t)kChunk, M - base);
+    for (uint32_t j = t; j < nslot * kMlpRaw; j += kChunk) {
+        const int r = dst[j / kMlpRaw];
+        if (r >= 0) raw[(size_t)kMlpRaw * r + j % kMlpRaw] = slot_raw[(size_t)kMlpRaw * base + j];
+    }
+    for (uint32_t j = t; j < nslot * 3; j += kChunk) {
+        const int r = dst[j / 3];
+        if (r >= 0) tgt[3 * (size_t)r + j % 3] = slot_tgt[3 * (size_t)base + j];
+    }
 }
 
 uint32_t frame_offset(uint64_t frame, uint32_t seed, uint32_t stride) {
@@ -289,6 +173,7 @@ CacheView cache_begin(Context &ctx, Scene &s, const CacheOptions &opt, CacheStat
         cs.batch_raw.alloc(sizeof(float) * kMlpRaw * (size_t)cap);
         cs.batch_tgt.alloc(sizeof(float) * 3 * (size_t)cap);
         cs.batch_n.alloc(sizeof(uint32_t));
+        cs.chunk_n.alloc(sizeof(uint32_t) * ((cap + kChunk - 1) / kChunk));
         cs.slots = cap;
     }
     CacheView cv;
@@ -307,48 +192,57 @@ CacheView cache_begin(Context &ctx, Scene &s, const CacheOptions &opt, CacheStat
     return cv;
 }
 
-void cache_infer(Context &ctx, CacheState &cs, uint32_t npix) {
-    mlp_infer_dev(ctx, cs.net, cs.query.as<float>(), npix, cs.pred.as<float>());
+void cache_infer(CacheState &cs, uint32_t npix, hipStream_t stream) {
+    mlp_infer_dev(cs.net, cs.query.as<float>(), npix, cs.pred.as<float>(), stream);
 }
 
-void cache_fold_intermediate(Context &ctx, CacheState &cs, uint32_t npix, float4 *intermediate) {
-    fold_kernel<<<(npix + 255) / 256, 256, 0, ctx.stream>>>(npix, cs.theta.as<float4>(), cs.pred.as<float>(),
-                                                           intermediate);
+void cache_fold_intermediate(CacheState &cs, uint32_t npix, float4 *intermediate, hipStream_t stream) {
+    fold_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(npix, cs.theta.as<float4>(), cs.pred.as<float>(),
+                                                       intermediate);
     KRT_HIP(hipGetLastError());
 }
 
 void cache_train(Context &ctx, CacheState &cs, const CacheView &cv) {
     if (cv.M == 0) return;
-    compact_kernel<<<1, kCompactThreads, 0, ctx.stream>>>(cs.slot_raw.as<float>(), cs.slot_tgt.as<float>(),
-                                                         cs.slot_ok.as<uint32_t>(), cv.M, cs.batch_raw.as<float>(),
-                                                         cs.batch_tgt.as<float>(), cs.batch_n.as<uint32_t>());
+    const uint32_t nchunk = (cv.M + kChunk - 1) / kChunk;
+    uint32_t *chunk = cs.chunk_n.as<uint32_t>();
+    chunk_count_kernel<<<nchunk, kChunk, 0, ctx.stream>>>(cs.slot_ok.as<uint32_t>(), cv.M, chunk);
+    KRT_HIP(hipGetLastError());
+    chunk_scan_kernel<<<1, kScanThreads, 0, ctx.stream>>>(chunk, nchunk, cs.batch_n.as<uint32_t>());
+    KRT_HIP(hipGetLastError());
+    chunk_scatter_kernel<<<nchunk, kChunk, 0, ctx.stream>>>(cs.slot_raw.as<float>(), cs.slot_tgt.as<float>(),
+                                                           cs.slot_ok.as<uint32_t>(), cv.M, chunk,
+                                                           cs.batch_raw.as<float>(), cs.batch_tgt.as<float>());
     KRT_HIP(hipGetLastError());
     mlp_gradient_dev(ctx, cs.net, cs.batch_raw.as<float>(), cs.batch_tgt.as<float>(), cv.M, cs.batch_n.as<uint32_t>());
     mlp_apply_dev(ctx, cs.net);
 }
 
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
-                     DevBuf &history, unsigned long long *ctr, PassTimes &times) {
+                     unsigned long long *ctr, PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "the path tracer's radiance cache renders flat scenes (krt_scene_upload)");
     KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
     const uint32_t npix = li.width * li.height;
     const CacheView cv = cache_begin(ctx, s, opt, cs, npix, li.user_seed);
-    uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
-    uint32_t nthreads = nb * kTraceBlock;
-    size_t need = sizeof(float4) * 2 * (size_t)(li.max_depth + 1) * nthreads;
-    if (history.bytes < need) history.alloc(need);
-    uint32_t *work = ctx.counters.as<uint32_t>();
-    KRT_HIP(hipMemsetAsync(work, 0, sizeof(uint32_t), ctx.stream));
-    CacheArgs A{li, make_view(s), history.as<float4>(), spill_buffer(ctx, s, nthreads), ctr, cs.prefix.as<float4>(),
-                cv, work};
-    hipEvent_t ev[5];
+    // The inference reads only the query records, all written by the
+    // cache-vertex shading pass: it runs on the side stream while the main
+    // stream walks the training paths' remaining levels, and the composite
+    // waits for it.
+    hipStream_t side = ctx.side ? ctx.side : ctx.stream;
+    hipEvent_t ev[5], queried, inf0, inf1;
     for (auto &e : ev) KRT_HIP(hipEventCreate(&e));
+    KRT_HIP(hipEventCreate(&queried));
+    KRT_HIP(hipEventCreate(&inf0));
+    KRT_HIP(hipEventCreate(&inf1));
     KRT_HIP(hipEventRecord(ev[0], ctx.stream));
-    cache_path_kernel<<<nb, kTraceBlock, 0, ctx.stream>>>(A);
-    KRT_HIP(hipGetLastError());
+    pt_wave(ctx, s, li, ctr, nullptr, &cv, cs.prefix.as<float4>(), queried);
     KRT_HIP(hipEventRecord(ev[1], ctx.stream));
-    cache_infer(ctx, cs, npix);
+    KRT_HIP(hipStreamWaitEvent(side, queried, 0));
+    KRT_HIP(hipEventRecord(inf0, side));
+    cache_infer(cs, npix, side);
+    KRT_HIP(hipEventRecord(inf1, side));
+    KRT_HIP(hipStreamWaitEvent(ctx.stream, inf1, 0));
     KRT_HIP(hipEventRecord(ev[2], ctx.stream));
     composite_kernel<<<(npix + 255) / 256, 256, 0, ctx.stream>>>(li, cs.prefix.as<float4>(), cs.theta.as<float4>(),
                                                                 cs.pred.as<float>());
@@ -357,8 +251,14 @@ void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOp
     cache_train(ctx, cs, cv);
     KRT_HIP(hipEventRecord(ev[4], ctx.stream));
     KRT_HIP(hipEventSynchronize(ev[4]));
-    for (int k = 0; k < 4; k++) KRT_HIP(hipEventElapsedTime(&times.ms[k], ev[k], ev[k + 1]));
+    KRT_HIP(hipEventElapsedTime(&times.ms[0], ev[0], ev[1]));
+    KRT_HIP(hipEventElapsedTime(&times.ms[1], inf0, inf1));
+    KRT_HIP(hipEventElapsedTime(&times.ms[2], ev[2], ev[3]));
+    KRT_HIP(hipEventElapsedTime(&times.ms[3], ev[3], ev[4]));
     for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(queried);
+    (void)hipEventDestroy(inf0);
+    (void)hipEventDestroy(inf1);
 }
 
 }  // namespace krt

@@ -56,7 +56,6 @@ struct krt_attachment {
     CacheOptions copt;
     CacheState cst;
     std::unique_ptr<krt_mlp> cache_handle;  // borrowed view of cst.net for krt_attachment_cache
-    DevBuf history;
     uint32_t w = 0, h = 0;
 
     void load(uint32_t width, uint32_t height) {  // restir.cuh:203-211
@@ -71,7 +70,6 @@ struct krt_attachment {
         gst = GiState();
         cache_handle.reset();
         cst = CacheState();
-        history.release();
         loaded = false;
     }
     bool set_option(const std::string &f, const krt_option &v) {
@@ -181,6 +179,7 @@ int krt_create(int device, krt_context **out) {
         auto *c = new krt_context();
         c->impl.device = device;
         KRT_HIP(hipStreamCreate(&c->impl.stream));
+        KRT_HIP(hipStreamCreate(&c->impl.side));
         c->impl.counters.alloc(sizeof(unsigned long long) * 16);
         *out = c;
     });
@@ -190,12 +189,14 @@ void krt_destroy(krt_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->impl.device);
     (void)hipStreamSynchronize(ctx->impl.stream);
+    if (ctx->impl.side) (void)hipStreamSynchronize(ctx->impl.side);
     ctx->impl.scenes.clear();
     ctx->impl.counters.release();
     ctx->impl.spill.release();
     for (auto &e : ctx->impl.timer)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->impl.stream);
+    if (ctx->impl.side) (void)hipStreamDestroy(ctx->impl.side);
     delete ctx;
 }
 
@@ -765,7 +766,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         KRT_HIP(hipMemsetAsync(a->ctr.p, 0, a->ctr.bytes, C.stream));
         PassTimes times;
         if (att->kind == "path_tracer" && att->copt.enabled) {
-            cache_pt_render(C, s, li, att->copt, att->cst, att->history, a->ctr.as<unsigned long long>(), times);
+            cache_pt_render(C, s, li, att->copt, att->cst, a->ctr.as<unsigned long long>(), times);
         } else if (att->kind == "path_tracer") {
             pt_render(C, s, li, a->ctr.as<unsigned long long>(), times,
                       a->f64 ? a->acc64.as<double>() : nullptr);

@@ -146,12 +146,14 @@ SceneView make_view(const Scene &s);
 struct WaveScratch {
     DevBuf hits, inst, state, hist, occ;
     DevBuf v0;  // ReSTIR GI: primary-hit state (4 float4 per pixel)
+    DevBuf lec;  // path tracer with the radiance cache: emission at the cache vertex (float4 per pixel)
     DevBuf rays[2], counts[2], prefix[2], dense[2];  // closest-ray list, shadow-ray list
 };
 
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream for work that overlaps the main stream's (radiance-cache inference)
     std::map<krt_handle, std::unique_ptr<Scene>> scenes;
     krt_handle next_handle = 1;
     DevBuf counters;  // scratch counters (unsigned long long x 16)
@@ -232,6 +234,13 @@ struct PassTimes {
 
 void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,
                double *acc64);
+struct CacheView;
+// The path tracer's passes on ctx.stream, untimed.  With a cache view (cv and
+// cv->enabled) paths stop at the cache vertex and the pass leaves prefix and
+// cv->theta for the composite instead of accumulating colour, and records
+// `queried` (when given) on ctx.stream once every query record is written.
+void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, double *acc64,
+             const CacheView *cv, float4 *prefix, hipEvent_t queried = nullptr);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
 void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed);
@@ -289,6 +298,7 @@ void mlp_init(Context &ctx, Mlp &net, const MlpConfig &cfg, uint64_t seed);
 void mlp_set_params(Context &ctx, Mlp &net, const float *host);
 void mlp_get_params(Context &ctx, Mlp &net, float *host);
 void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *out);
+void mlp_infer_dev(Mlp &net, const float *raw, uint32_t n, float *out, hipStream_t stream);
 // n_dev (optional): the batch size on the device, at most n.
 void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n,
                       const uint32_t *n_dev = nullptr);
@@ -314,10 +324,11 @@ struct CacheState {
     DevBuf query, pred, prefix, theta;             // per pixel
     DevBuf slot_raw, slot_tgt, slot_ok;            // per training slot
     DevBuf batch_raw, batch_tgt, batch_n;          // compacted training batch
+    DevBuf chunk_n;                                // per chunk of slots: valid count, then first batch row
 };
 
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
-                     DevBuf &history, unsigned long long *ctr, PassTimes &times);
+                     unsigned long long *ctr, PassTimes &times);
 // ReSTIR DI; with a cache (copt->enabled) its indirect path stops at the cache vertex.
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times, const CacheOptions *copt = nullptr,

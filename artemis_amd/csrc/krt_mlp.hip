@@ -540,12 +540,15 @@ uint32_t mlp_infer_blocks(uint32_t n) {
     return b < 1 ? 1 : (b < cap ? b : cap);
 }
 
-void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *out) {
+void mlp_infer_dev(Mlp &net, const float *raw, uint32_t n, float *out, hipStream_t stream) {
     if (n == 0) return;
-    infer_kernel<<<mlp_infer_blocks(n), kInferBlock, 0, ctx.stream>>>(net.frag_f.as<bf16x8>(),
-                                                                       net.params.as<float>(), raw, n, out,
-                                                                       net.enc);
+    infer_kernel<<<mlp_infer_blocks(n), kInferBlock, 0, stream>>>(net.frag_f.as<bf16x8>(), net.params.as<float>(),
+                                                                   raw, n, out, net.enc);
     KRT_HIP(hipGetLastError());
+}
+
+void mlp_infer_dev(Context &ctx, Mlp &net, const float *raw, uint32_t n, float *out) {
+    mlp_infer_dev(net, raw, n, out, ctx.stream);
 }
 
 void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *target, uint32_t n,

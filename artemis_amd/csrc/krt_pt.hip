@@ -3,6 +3,7 @@
 // bit for bit by the serving tests.
 #include <hip/hip_runtime.h>
 
+#include "krt_cache.h"
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
@@ -34,9 +35,13 @@ struct PtWave {
     float4 *state;    // per pixel: {seed, levels with history (int bits)}
     float4 *hist;     // per level, per pixel: 3 float4
     uint8_t *occ;     // per level, per pixel
+    float4 *lec;      // radiance cache: the emission added exactly at the cache vertex
 };
 
 constexpr int kWaveBlock = 256;
+// state.w: levels with history in the low 16 bits, bit 16 set when the path
+// recorded a radiance-cache query
+constexpr int kQueried = 1 << 16;
 
 __global__ __launch_bounds__(kWaveBlock) void pt_camera_kernel(LaunchInfo li, PtWave B) {
     __shared__ uint32_t nray;
@@ -58,7 +63,8 @@ __global__ __launch_bounds__(kWaveBlock) void pt_camera_kernel(LaunchInfo li, Pt
 }
 
 template <bool INST>
-__global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, SceneView sv, PtWave B, int level) {
+__global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, SceneView sv, PtWave B, int level,
+                                                              CacheView cv) {
     __shared__ uint32_t ncl, nsl;
     if (threadIdx.x == 0) ncl = nsl = 0;
     __syncthreads();
@@ -87,7 +93,24 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
         h.rank = 0;
         h.inst = INST ? B.hit_inst[pix] : -1;
         Surface sf;
-        if (h.prim >= 0 && load_surface<INST>(sv, h, d, sf)) {  // a miss or a NaN normal ends the path
+        const bool surf = h.prim >= 0 && load_surface<INST>(sv, h, d, sf);  // a miss or a NaN normal ends the path
+        int queried = __float_as_int(st.w) & kQueried;
+        bool cache_stop = false;
+        if (surf && cv.enabled && level == cv.depth) {
+            // the cache vertex (krt_cache.hip): record the query; the path stops
+            // here unless the pixel is a training path
+            const Material mat = load_material(sv.mats, sf.mat_id);
+            uint32_t slot = 0;
+            const bool train = cv.training(pix, slot);
+            put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d, sf.n,
+                      mat);
+            const f3 le = cv.learn_emission ? mk(0, 0, 0) : mat.emission;
+            B.lec[pix] = make_float4(le.x, le.y, le.z, 0.0f);
+            queried = kQueried;
+            B.state[pix] = make_float4(st.x, st.y, st.z, __int_as_float(__float_as_int(st.w) | kQueried));
+            cache_stop = !train;
+        }
+        if (surf && !cache_stop) {
             const Material mat = load_material(sv.mats, sf.mat_id);
             const f3 n = sf.n;
             const f3 x = offset_point(sf.x, n, mat.type);
@@ -135,7 +158,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
                                 count);
                 li.position[pix] = make_float4(x.x, x.y, x.z, 0.0f);
             }
-            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(level + 1));
+            B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float((level + 1) | queried));
             if (has_pdf && level < li.max_depth) {
                 no = x;
                 nd = wi;
@@ -152,6 +175,86 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
     }
 }
 
+// Le + Ld of level k of pixel pix, the shadow ray's answer applied.
+KRT_DEV f3 level_direct(const PtWave &B, uint32_t npix, uint32_t pix, int k, float4 &e, float4 &t) {
+    const float4 *hl = B.hist + 3 * (size_t)k * npix;
+    e = hl[pix];
+    t = hl[npix + pix];
+    const float4 c = hl[2 * (size_t)npix + pix];
+    f3 Ldv = mk(0.0f, 0.0f, 0.0f);
+    if (t.w != 0.0f) {
+        const f3 contr = B.occ[(size_t)k * npix + pix] ? mk(0.0f, 0.0f, 0.0f) : mk(c.x, c.y, c.z);
+        Ldv = contr / c.w;
+    }
+    return mk(e.x, e.y, e.z) + Ldv;
+}
+
+// With the radiance cache (krt_cache.hip): a path that stopped at the cache
+// vertex leaves value_0 = P + Theta * (Le + cache) as prefix P (Le folded in)
+// and Theta for the composite after inference; any other path leaves its
+// value, and a training path its target (the value at the cache vertex less
+// that vertex's emission).  The AOVs are written as without the cache.
+__global__ __launch_bounds__(kWaveBlock) void pt_cache_compose_kernel(LaunchInfo li, PtWave B, CacheView cv,
+                                                                      float4 *prefix) {
+    const TileOrder tiles(li.width, li.height);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = li.width * li.height;
+    uint32_t px, py;
+    if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
+    const uint32_t pix = py * li.width + px;
+    const int sw = __float_as_int(B.state[pix].w);
+    const int nlev = sw & (kQueried - 1);
+    const bool queried = (sw & kQueried) != 0;
+    uint32_t slot = 0;
+    const bool train = cv.training(pix, slot);
+    f3 le_c = mk(0, 0, 0);
+    if (queried) {
+        const float4 l = B.lec[pix];
+        le_c = mk(l.x, l.y, l.z);
+    }
+    float4 pre, th = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (queried && !train) {
+        f3 P = mk(0, 0, 0), Th = mk(1.0f, 1.0f, 1.0f);
+        for (int k = nlev - 1; k >= 0; k--) {
+            float4 e, t;
+            const f3 dk = level_direct(B, npix, pix, k, e, t);
+            const f3 T = mk(t.x, t.y, t.z);
+            P = dk + T * P;
+            Th = T * Th;
+        }
+        P = P + Th * le_c;
+        pre = make_float4(P.x, P.y, P.z, 0.0f);
+        th = make_float4(Th.x, Th.y, Th.z, 1.0f);
+    } else {
+        f3 value = mk(0, 0, 0), target = mk(0, 0, 0);
+        for (int k = nlev - 1; k >= 0; k--) {
+            float4 e, t;
+            f3 v = level_direct(B, npix, pix, k, e, t);
+            if (e.w != 0.0f) v = v + mk(t.x, t.y, t.z) * value;
+            value = v;
+            if (k == cv.depth) target = value - le_c;  // scattered radiance only
+        }
+        pre = make_float4(value.x, value.y, value.z, 0.0f);
+        if (train) {
+            float *tg = cv.slot_tgt + 3 * (size_t)slot;
+            tg[0] = target.x;
+            tg[1] = target.y;
+            tg[2] = target.z;
+            cv.slot_ok[slot] = (queried && nlev > cv.depth && finite3(target)) ? 1u : 0u;
+        }
+    }
+    prefix[pix] = pre;
+    cv.theta[pix] = th;
+    if (nlev == 0) {  // the primary ray missed: the AOVs take zeros
+        const bool acc = li.accumulate != 0;
+        const float count = (float)li.samples;
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
+        li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);
+        li.position[pix] = zero;
+    }
+}
+
 __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, PtWave B, double *acc64) {
     const TileOrder tiles(li.width, li.height);
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,17 +262,11 @@ __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, P
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
     const uint32_t pix = py * li.width + px;
-    const int nlev = __float_as_int(B.state[pix].w);
+    const int nlev = __float_as_int(B.state[pix].w) & (kQueried - 1);
     f3 value = mk(0, 0, 0);
     for (int k = nlev - 1; k >= 0; k--) {
-        const float4 *hl = B.hist + 3 * (size_t)k * npix;
-        const float4 e = hl[pix], t = hl[npix + pix], c = hl[2 * (size_t)npix + pix];
-        f3 Ldv = mk(0.0f, 0.0f, 0.0f);
-        if (t.w != 0.0f) {
-            const f3 contr = B.occ[(size_t)k * npix + pix] ? mk(0.0f, 0.0f, 0.0f) : mk(c.x, c.y, c.z);
-            Ldv = contr / c.w;
-        }
-        f3 v = mk(e.x, e.y, e.z) + Ldv;
+        float4 e, t;
+        f3 v = level_direct(B, npix, pix, k, e, t);
         if (e.w != 0.0f) v = v + mk(t.x, t.y, t.z) * value;
         value = v;
     }
@@ -198,11 +295,12 @@ __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, P
 
 }  // namespace
 
-void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,
-               double *acc64) {
+void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, double *acc64,
+             const CacheView *cv, float4 *prefix, hipEvent_t queried) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(li.max_depth >= 0 && li.max_depth <= 64, KRT_ERR_INVALID, "max_depth must be in [0, 64]");
     const uint32_t npix = li.width * li.height;
+    const CacheView cview = cv ? *cv : CacheView{};
     const SceneView sv = make_view(s);
     WaveScratch &ws = ctx.wave;
     const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
@@ -217,6 +315,7 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long 
     grow(ws.state, sizeof(float4) * npix);
     grow(ws.hist, sizeof(float4) * 3 * levels * npix);
     grow(ws.occ, levels * npix);
+    if (cview.enabled) grow(ws.lec, sizeof(float4) * npix);
     for (int l = 0; l < 2; l++) {
         grow(ws.rays[l], sizeof(float4) * 2 * cap);
         grow(ws.counts[l], sizeof(uint32_t) * gb);
@@ -235,26 +334,51 @@ void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long 
     B.state = ws.state.as<float4>();
     B.hist = ws.hist.as<float4>();
     B.occ = ws.occ.as<uint8_t>();
+    B.lec = cview.enabled ? ws.lec.as<float4>() : nullptr;
+    pt_camera_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B);
+    KRT_HIP(hipGetLastError());
+    walk_closest_list(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, spill, ctr, 0);
+    // Past the cache vertex only the training paths (at most cv.M) go on, so
+    // those levels' shading grids, and the segments of the lists they emit,
+    // shrink to the training paths' count; with no training paths they end.
+    const uint32_t train_blocks = (cview.M + kWaveBlock - 1) / kWaveBlock;
+    uint32_t in_seg = gb;  // segments of the closest list the next shading pass reads
+    for (int level = 0; level <= li.max_depth; level++) {
+        uint32_t grid = gb;
+        if (cview.enabled && level > cview.depth) {
+            if (train_blocks == 0) break;
+            grid = std::min(gb, train_blocks);
+        }
+        PtWave Bl = B;
+        Bl.cl.nseg = in_seg;
+        if (s.instanced) pt_shade_kernel<true><<<grid, kWaveBlock, 0, ctx.stream>>>(li, sv, Bl, level, cview);
+        else pt_shade_kernel<false><<<grid, kWaveBlock, 0, ctx.stream>>>(li, sv, Bl, level, cview);
+        KRT_HIP(hipGetLastError());
+        if (queried && cview.enabled && level == cview.depth) {
+            KRT_HIP(hipEventRecord(queried, ctx.stream));
+            queried = nullptr;
+        }
+        Bl.cl.nseg = Bl.sl.nseg = in_seg = grid;
+        // the level's shadow rays and the next level's closest rays
+        if (level < li.max_depth)
+            walk_pair(ctx, s.instanced, sv, Bl.cl, Bl.hits, Bl.hit_inst, Bl.sl, Bl.occ + (size_t)level * npix, spill,
+                      ctr, 0);
+        else
+            walk_shadow_list(ctx, s.instanced, sv, Bl.sl, Bl.occ + (size_t)level * npix, spill, ctr, 0);
+    }
+    if (cview.enabled) pt_cache_compose_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B, cview, prefix);
+    else pt_compose_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B, acc64);
+    KRT_HIP(hipGetLastError());
+    if (queried) KRT_HIP(hipEventRecord(queried, ctx.stream));  // the cache vertex lies beyond the path's depth
+}
+
+void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,
+               double *acc64) {
     hipEvent_t e0, e1;
     KRT_HIP(hipEventCreate(&e0));
     KRT_HIP(hipEventCreate(&e1));
     KRT_HIP(hipEventRecord(e0, ctx.stream));
-    pt_camera_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B);
-    KRT_HIP(hipGetLastError());
-    walk_closest_list(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, spill, ctr, 0);
-    for (int level = 0; level <= li.max_depth; level++) {
-        if (s.instanced) pt_shade_kernel<true><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
-        else pt_shade_kernel<false><<<gb, kWaveBlock, 0, ctx.stream>>>(li, sv, B, level);
-        KRT_HIP(hipGetLastError());
-        // the level's shadow rays and the next level's closest rays
-        if (level < li.max_depth)
-            walk_pair(ctx, s.instanced, sv, B.cl, B.hits, B.hit_inst, B.sl, B.occ + (size_t)level * npix, spill, ctr,
-                      0);
-        else
-            walk_shadow_list(ctx, s.instanced, sv, B.sl, B.occ + (size_t)level * npix, spill, ctr, 0);
-    }
-    pt_compose_kernel<<<gb, kWaveBlock, 0, ctx.stream>>>(li, B, acc64);
-    KRT_HIP(hipGetLastError());
+    pt_wave(ctx, s, li, ctr, acc64, nullptr, nullptr);
     KRT_HIP(hipEventRecord(e1, ctx.stream));
     KRT_HIP(hipEventSynchronize(e1));
     KRT_HIP(hipEventElapsedTime(&times.ms[0], e0, e1));

@@ -914,6 +914,28 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     B.zm = st.zm.as<int32_t>();
     B.flags = st.flags.as<uint8_t>();
     int *lean_spill = A.spill;  // sized above for the list walks too
+    // With the cache, the inference reads only the query records (all written
+    // by the cache-vertex shading pass) and runs on the side stream while the
+    // main stream walks the training paths' remaining levels and the temporal
+    // pass; the fold into the intermediate buffer follows the initial pass's
+    // compose there, and the first reader of that buffer waits for it.
+    hipStream_t side = ctx.side ? ctx.side : ctx.stream;
+    hipEvent_t queried = nullptr, inf0 = nullptr, inf1 = nullptr, folded = nullptr;
+    if (cached) {
+        KRT_HIP(hipEventCreate(&queried));
+        KRT_HIP(hipEventCreate(&inf0));
+        KRT_HIP(hipEventCreate(&inf1));
+        KRT_HIP(hipEventCreate(&folded));
+    }
+    bool inferring = false;
+    auto start_infer = [&]() {
+        KRT_HIP(hipEventRecord(queried, ctx.stream));
+        KRT_HIP(hipStreamWaitEvent(side, queried, 0));
+        KRT_HIP(hipEventRecord(inf0, side));
+        cache_infer(*cs, npix, side);
+        KRT_HIP(hipEventRecord(inf1, side));
+        inferring = true;
+    };
     mark();
     A.cur = planes(st.cur);
     A.prev = planes(1 - st.cur);
@@ -944,26 +966,41 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
         init_camera_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
         KRT_HIP(hipGetLastError());
         walk_closest_list(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, lean_spill, ctr, 0);
+        // Past the cache vertex only the training paths (at most cv.M) go on, so
+        // those levels' shading grids, and the segments of the lists they emit,
+        // shrink to the training paths' count; with no training paths they end.
+        const uint32_t train_blocks = (A.cv.M + kSpatialBlock - 1) / kSpatialBlock;
+        uint32_t in_seg = gb;  // segments of the closest list the next shading pass reads
         for (int level = 0; level <= li.max_depth; level++) {
-            if (s.instanced) init_shade_kernel<true><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
-            else init_shade_kernel<false><<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W, level);
+            uint32_t grid = gb;
+            if (cached && level > A.cv.depth) {
+                if (train_blocks == 0) break;
+                grid = std::min(gb, train_blocks);
+            }
+            InitWave Wl = W;
+            Wl.cl.nseg = in_seg;
+            if (s.instanced) init_shade_kernel<true><<<grid, kSpatialBlock, 0, ctx.stream>>>(A, Wl, level);
+            else init_shade_kernel<false><<<grid, kSpatialBlock, 0, ctx.stream>>>(A, Wl, level);
             KRT_HIP(hipGetLastError());
+            if (cached && level == A.cv.depth) start_infer();
+            Wl.cl.nseg = Wl.sl.nseg = in_seg = grid;
             // the level's shadow rays and the next level's closest rays
             if (level < li.max_depth)
-                walk_pair(ctx, s.instanced, A.sv, W.cl, W.hits, W.hit_inst, W.sl, W.occ + (size_t)level * npix,
+                walk_pair(ctx, s.instanced, A.sv, Wl.cl, Wl.hits, Wl.hit_inst, Wl.sl, Wl.occ + (size_t)level * npix,
                           lean_spill, ctr, 0);
             else
-                walk_shadow_list(ctx, s.instanced, A.sv, W.sl, W.occ + (size_t)level * npix, lean_spill, ctr, 0);
+                walk_shadow_list(ctx, s.instanced, A.sv, Wl.sl, Wl.occ + (size_t)level * npix, lean_spill, ctr, 0);
         }
         init_compose_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, W);
         KRT_HIP(hipGetLastError());
     }
     mark();
     if (cached) {
-        cache_infer(ctx, *cs, npix);
-        cache_fold_intermediate(ctx, *cs, npix, st.intermediate.as<float4>());
+        if (!inferring) start_infer();  // the cache vertex lies beyond the path's depth
+        KRT_HIP(hipStreamWaitEvent(side, ev[nev - 1], 0));
+        cache_fold_intermediate(*cs, npix, st.intermediate.as<float4>(), side);
+        KRT_HIP(hipEventRecord(folded, side));
     }
-    mark();
     B.list.seg = kSpatialBlock * 2u;
     temporal_rays_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
     KRT_HIP(hipGetLastError());
@@ -983,26 +1020,34 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
             spatial_merge_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             walk_shadow_list(ctx, s.instanced, A.sv, B.list, B.occ, lean_spill, ctr, 2);
+            if (cached && pass == 0) KRT_HIP(hipStreamWaitEvent(ctx.stream, folded, 0));
             spatial_resolve_kernel<<<gb, kSpatialBlock, 0, ctx.stream>>>(A, B);
             KRT_HIP(hipGetLastError());
             st.cur = 1 - st.cur;  // swap(current, previous), restir.cuh:294
         }
     }
     mark();
-    if (cached) cache_train(ctx, *cs, A.cv);
+    if (cached) {
+        if (opt.spatial_passes == 0) KRT_HIP(hipStreamWaitEvent(ctx.stream, folded, 0));
+        cache_train(ctx, *cs, A.cv);
+    }
     mark();
     st.cur = 1 - st.cur;  // final swap, restir.cuh:300
     for (int k = 0; k < 16; k++) st.prev_pv[k] = li.pv[k];
     KRT_HIP(hipEventSynchronize(ev[nev - 1]));
-    // segments: initial, cache inference + fold, temporal, spatial, training
-    float seg[5] = {0, 0, 0, 0, 0};
-    for (int k = 0; k + 1 < nev && k < 5; k++) KRT_HIP(hipEventElapsedTime(&seg[k], ev[k], ev[k + 1]));
+    // segments: initial, temporal, spatial, training; the inference is timed
+    // on the side stream
+    float seg[4] = {0, 0, 0, 0};
+    for (int k = 0; k + 1 < nev && k < 4; k++) KRT_HIP(hipEventElapsedTime(&seg[k], ev[k], ev[k + 1]));
     times.ms[0] = seg[0];
-    times.ms[1] = seg[2];
-    times.ms[2] = seg[3];
-    times.ms[3] = seg[1];
-    times.ms[4] = seg[4];
+    times.ms[1] = seg[1];
+    times.ms[2] = seg[2];
+    times.ms[3] = 0.0f;
+    times.ms[4] = seg[3];
+    if (cached) KRT_HIP(hipEventElapsedTime(&times.ms[3], inf0, inf1));
     for (int k = 0; k < nev; k++) (void)hipEventDestroy(ev[k]);
+    for (hipEvent_t e : {queried, inf0, inf1, folded})
+        if (e) (void)hipEventDestroy(e);
 }
 
 }  // namespace krt

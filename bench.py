@@ -69,7 +69,8 @@ PASS_KERNELS = {
     "restir_di:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
     "restir_di:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
                           "spatial_resolve_kernel"] + LIST,
-    "restir_cache:initial": ["initial_kernel"],
+    "restir_cache:initial": ["init_camera_kernel", "closest_list_kernel", "init_shade_kernel", "walk_pair_kernel",
+                             "shadow_list_kernel", "init_compose_kernel"] + LIST,
     "restir_cache:temporal": ["temporal_rays_kernel", "shadow_list_kernel", "temporal_resolve_kernel"] + LIST,
     "restir_cache:spatial": ["spatial_rays_kernel", "shadow_list_kernel", "spatial_merge_kernel",
                              "spatial_resolve_kernel"] + LIST,
@@ -80,7 +81,8 @@ PASS_KERNELS = {
     "restir_gi:temporal": ["gi_temporal_kernel"],
     "restir_gi:spatial": ["gi_spatial_rays_kernel", "shadow_list_kernel", "gi_spatial_merge_kernel",
                           "gi_spatial_resolve_kernel"] + LIST,
-    "pt_cache:path": ["cache_path_kernel"],
+    "pt_cache:path": ["pt_camera_kernel", "closest_list_kernel", "pt_shade_kernel", "walk_pair_kernel",
+                      "shadow_list_kernel", "pt_cache_compose_kernel"] + LIST,
 }
 PT_PIXEL_BYTES = 112  # color/normal/albedo float4 read+write + position write
 
