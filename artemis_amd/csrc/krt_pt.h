@@ -1,7 +1,8 @@
 // One path-tracer sample of one pixel: the estimator of
 // source/optix/amadeus_path_tracer.cu:73-228 (raygen + closest-hit + miss),
-// shared by the attachment kernel (krt_pt.hip) and the batched serving kernel
-// (krt_serve.hip), so a pixel's value does not depend on which one ran it.
+// run by the batched serving kernel (krt_serve.hip).  The attachment's split
+// passes (krt_pt.hip) evaluate the same expressions in the same order, so a
+// pixel's value does not depend on which one ran it.
 //
 //   value_k = Le_k + Ld_k (+ T_k * value_{k+1} when pdf_k > 0),
 //   T_k = f_k |wi.n| / pdf_k,
