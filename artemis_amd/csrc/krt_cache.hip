@@ -255,6 +255,7 @@ void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOp
     KRT_HIP(hipEventElapsedTime(&times.ms[1], inf0, inf1));
     KRT_HIP(hipEventElapsedTime(&times.ms[2], ev[2], ev[3]));
     KRT_HIP(hipEventElapsedTime(&times.ms[3], ev[3], ev[4]));
+    KRT_HIP(hipEventElapsedTime(&times.frame, ev[0], ev[4]));
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(queried);
     (void)hipEventDestroy(inf0);

@@ -798,7 +798,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
             a->stats.kernel_ms[k] = times.ms[k];
             tot += times.ms[k];
         }
-        a->stats.gpu_ms = tot;
+        a->stats.gpu_ms = times.frame >= 0.0f ? times.frame : tot;
         a->samples++;  // armada.cu:779
     });
 }

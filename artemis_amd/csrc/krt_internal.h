@@ -230,6 +230,7 @@ struct GiState {
 
 struct PassTimes {
     float ms[8] = {0};
+    float frame = -1.0f;  // the frame's GPU span when passes overlap (else the sum of ms)
 };
 
 void pt_render(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, PassTimes &times,

@@ -1045,6 +1045,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     times.ms[3] = 0.0f;
     times.ms[4] = seg[3];
     if (cached) KRT_HIP(hipEventElapsedTime(&times.ms[3], inf0, inf1));
+    if (nev > 1) KRT_HIP(hipEventElapsedTime(&times.frame, ev[0], ev[nev - 1]));
     for (int k = 0; k < nev; k++) (void)hipEventDestroy(ev[k]);
     for (hipEvent_t e : {queried, inf0, inf1, folded})
         if (e) (void)hipEventDestroy(e);
