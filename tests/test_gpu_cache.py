@@ -63,6 +63,41 @@ def test_cache_vertex_beyond_depth_is_the_path_tracer(krt_ctx, cornell):
     assert np.array_equal(plain.color_buffer().view(np.uint32), arm.color_buffer().view(np.uint32))
 
 
+@pytest.mark.parametrize("kind", ["pt", "restir"])
+@pytest.mark.parametrize("cache_depth", [1, 2])
+def test_all_training_paths_are_the_uncached_renderer(krt_ctx, cornell, kind, cache_depth):
+    # With a training path at every pixel (train_samples = pixel count) every
+    # path runs to full depth and shows its own value, so the cached renderer
+    # is the uncached one bit for bit, while it still records every query,
+    # target and optimiser step; the inference on the side stream, the
+    # training-path levels past the cache vertex and the slot compaction
+    # (16 chunks of 256 slots) all run.
+    _, sc, cam = cornell
+
+    def make(cached):
+        arm = ArmadaRTX(krt_ctx, W, H)
+        arm.attach(kind, PathTracer(krt_ctx) if kind == "pt" else ReSTIR(krt_ctx))
+        arm.set_depth(DEPTH)
+        if cached:
+            arm.set_option("radiance_cache", True)
+            arm.set_option("cache_depth", cache_depth)
+            arm.set_option("train_samples", W * H)
+        return arm
+
+    plain, cached = make(False), make(True)
+    for f in range(3):
+        plain.render(sc, cam)
+        cached.render(sc, cam)
+        a, b = plain.color_buffer(), cached.color_buffer()
+        same = np.all(a.view(np.uint32) == b.view(np.uint32), axis=-1)
+        print(f"  {kind} cache_depth {cache_depth} frame {f}: {int(same.sum())} of {same.size} pixels equal",
+              flush=True)
+        assert same.all()
+    if kind == "restir":
+        for which in (0, 1):
+            assert np.array_equal(plain.reservoirs(which).view(np.uint32), cached.reservoirs(which).view(np.uint32))
+
+
 def one_sample_error(arm, sc, cam, ref, frames=16):
     # a non-accumulating frame restarts the sample counter (armada.cu:771-772),
     # so independent one-sample frames take a new user seed each
