@@ -56,11 +56,14 @@ struct Resv {
     }
 };
 
+// A frame's reservoirs: one 64 B record of four float4 per pixel, so a
+// neighbour's reservoir is one cache line to fetch.
 struct Planes {
     float4 *p;
     uint32_t n;
+    KRT_DEV float4 &part(uint32_t k, uint32_t i) const { return p[4 * (size_t)i + k]; }
     KRT_DEV Resv load(uint32_t i) const {
-        float4 a = p[i], b = p[n + i], c = p[2 * n + i], d = p[3 * n + i];
+        float4 a = part(0, i), b = part(1, i), c = part(2, i), d = part(3, i);
         Resv r;
         r.data.Le = mk(a.x, a.y, a.z);
         r.data.type = __float_as_int(a.w);
@@ -73,19 +76,19 @@ struct Planes {
         return r;
     }
     KRT_DEV void store(uint32_t i, const Resv &r) const {
-        p[i] = make_float4(r.data.Le.x, r.data.Le.y, r.data.Le.z, __int_as_float(r.data.type));
-        p[n + i] = make_float4(r.data.normal.x, r.data.normal.y, r.data.normal.z, r.w);
-        p[2 * n + i] = make_float4(r.data.point.x, r.data.point.y, r.data.point.z, r.W);
-        p[3 * n + i] = make_float4(r.seed.x, r.seed.y, r.seed.z, __int_as_float(r.M));
+        part(0, i) = make_float4(r.data.Le.x, r.data.Le.y, r.data.Le.z, __int_as_float(r.data.type));
+        part(1, i) = make_float4(r.data.normal.x, r.data.normal.y, r.data.normal.z, r.w);
+        part(2, i) = make_float4(r.data.point.x, r.data.point.y, r.data.point.z, r.W);
+        part(3, i) = make_float4(r.seed.x, r.seed.y, r.seed.z, __int_as_float(r.M));
     }
     // Reservoir::reset (reservoir.cuh:64-69): w = M = W = 0, data and seed kept
     KRT_DEV void reset(uint32_t i) const {
-        p[n + i].w = 0.0f;
-        p[2 * n + i].w = 0.0f;
-        p[3 * n + i].w = __int_as_float(0);
+        part(1, i).w = 0.0f;
+        part(2, i).w = 0.0f;
+        part(3, i).w = __int_as_float(0);
     }
-    KRT_DEV void store_M(uint32_t i, int M) const { p[3 * n + i].w = __int_as_float(M); }
-    KRT_DEV int M(uint32_t i) const { return __float_as_int(p[3 * n + i].w); }
+    KRT_DEV void store_M(uint32_t i, int M) const { part(3, i).w = __int_as_float(M); }
+    KRT_DEV int M(uint32_t i) const { return __float_as_int(part(3, i).w); }
 };
 
 // direct_unoccluded (amadeus_common.cuh:380-397)
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs 
     const float count = (float)li.samples;
     if (nlev > 0) {
         // the reference traces this visibility ray a second time: W *= 1 - occluded
-        float4 &w = A.cur.p[2 * (size_t)A.cur.n + pix];
+        float4 &w = A.cur.part(2, pix);
         w.w = B.occ[pix] ? 0.0f : w.w * (1.0f - 0.0f);
     } else {  // no reservoir written: primary miss or no surface
         A.cur.reset(pix);
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs
             const int pm = A.prev.M((uint32_t)pindex);
             if (pm > 0) {
                 A.prev.store_M((uint32_t)pindex, min(pm, A.m_cap));
-                const float4 p = A.prev.p[2 * (size_t)A.prev.n + (uint32_t)pindex];
+                const float4 p = A.prev.part(2, (uint32_t)pindex);
                 D = mk(p.x, p.y, p.z) - x;
                 d = length(D);
                 d1 = D / d;
