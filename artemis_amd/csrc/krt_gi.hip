@@ -54,11 +54,16 @@ struct GRes {
     }
 };
 
+// A frame's GI reservoirs: per pixel one 64 B record {xs, M}, {ns, w},
+// {Lo, W}, {xo} (a neighbour's sample is one cache line to fetch), then the
+// seeds as a plane of their own after the n records.
 struct GPlanes {
     float4 *p;
     uint32_t n;
+    KRT_DEV float4 &part(uint32_t k, uint32_t i) const { return p[4 * (size_t)i + k]; }
+    KRT_DEV float4 &seed(uint32_t i) const { return p[4 * (size_t)n + i]; }
     KRT_DEV GRes load(uint32_t i) const {
-        float4 a = p[i], b = p[n + i], c = p[2 * n + i], d = p[3 * n + i], e = p[4 * n + i];
+        float4 a = part(0, i), b = part(1, i), c = part(2, i), d = part(3, i), e = seed(i);
         GRes r;
         r.s.xs = mk(a.x, a.y, a.z);
         r.M = __float_as_int(a.w);
@@ -71,19 +76,19 @@ struct GPlanes {
         return r;
     }
     KRT_DEV void store(uint32_t i, const GRes &r) const {
-        p[i] = make_float4(r.s.xs.x, r.s.xs.y, r.s.xs.z, __int_as_float(r.M));
-        p[n + i] = make_float4(r.s.ns.x, r.s.ns.y, r.s.ns.z, r.w);
-        p[2 * n + i] = make_float4(r.s.Lo.x, r.s.Lo.y, r.s.Lo.z, r.W);
-        p[3 * n + i] = make_float4(r.xo.x, r.xo.y, r.xo.z, 0.0f);
-        p[4 * n + i] = make_float4(r.seed.x, r.seed.y, r.seed.z, 0.0f);
+        part(0, i) = make_float4(r.s.xs.x, r.s.xs.y, r.s.xs.z, __int_as_float(r.M));
+        part(1, i) = make_float4(r.s.ns.x, r.s.ns.y, r.s.ns.z, r.w);
+        part(2, i) = make_float4(r.s.Lo.x, r.s.Lo.y, r.s.Lo.z, r.W);
+        part(3, i) = make_float4(r.xo.x, r.xo.y, r.xo.z, 0.0f);
+        seed(i) = make_float4(r.seed.x, r.seed.y, r.seed.z, 0.0f);
     }
     KRT_DEV void reset(uint32_t i) const {
-        p[i].w = __int_as_float(0);
-        p[n + i].w = 0.0f;
-        p[2 * n + i].w = 0.0f;
+        part(0, i).w = __int_as_float(0);
+        part(1, i).w = 0.0f;
+        part(2, i).w = 0.0f;
     }
-    KRT_DEV void store_M(uint32_t i, int M) const { p[i].w = __int_as_float(M); }
-    KRT_DEV int M(uint32_t i) const { return __float_as_int(p[i].w); }
+    KRT_DEV void store_M(uint32_t i, int M) const { part(0, i).w = __int_as_float(M); }
+    KRT_DEV int M(uint32_t i) const { return __float_as_int(part(0, i).w); }
 };
 
 // G-buffer: {x, depth}, {n, valid}, material id
