@@ -16,7 +16,7 @@ queries + one training step on 2^16 samples per step, MFMA roofline).
 The default workload also runs the equal-relMSE study (quality(); skip with
 --no-quality): ReSTIR DI, the path tracer with the radiance cache and ReSTIR
 DI with the cache against the path tracer's own accumulation curve, at
-1920x1080, judged by the path tracer's FP64 reference.
+1920x1080, measured against the path tracer's FP64 reference image.
 
 Prints one JSON line: Msamples/s (pixels x frames / s), MRays/s, p50/p99
 frame latency, the roofline of the dominant kernel (HIP-event kernel times)
@@ -435,7 +435,7 @@ def quality(args, ctx, scene, sc):
     lamps: the same spheres with 4,096 separate lamps).
 
     Every method converges to the same image (fixed -0.5 pixel offset, path
-    depth 5); the judge is the path tracer's FP64-accumulated image of
+    depth 5); the reference is the path tracer's FP64-accumulated image of
     `quality_ref_spp` samples per pixel.  The path tracer's curve is measured
     directly: its accumulated image after 1, 2, 4, ... `quality_pt_spp`
     samples per pixel against the cumulative GPU time.  Each other method is

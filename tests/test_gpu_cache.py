@@ -1,6 +1,6 @@
 """Path tracer with the online radiance cache (krt_cache.hip) on the GPU.
 
-The judge of the cache is the full path tracer: a converged FP64 image of the
+The yardstick of the cache is the full path tracer: a converged FP64 image of the
 same scene.  The tests show (1) with the cache vertex beyond the path depth the
 cached renderer is the path tracer bit for bit, (2) after online training a
 one-sample frame that stops at the first bounce and asks the cache has a lower
