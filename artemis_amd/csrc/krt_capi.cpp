@@ -97,7 +97,14 @@ struct krt_attachment {
         if (kind != "restir" && kind != "restir_gi") return true;  // unknown option names are ignored
         if (f == "candidates") return as_int(ropt.candidates);
         if (f == "m_cap") return as_int(ropt.m_cap);
-        if (f == "spatial_passes") return as_int(ropt.spatial_passes);
+        if (f == "spatial_passes") {
+            // ReSTIR DI shades and accumulates the colour in its last spatial
+            // pass (amadeus_restir.cu:420-433; the reference always runs two),
+            // so it needs at least one; ReSTIR GI also shades in its temporal pass.
+            if (!as_int(tmp) || tmp < (kind == "restir" ? 1 : 0)) return false;
+            ropt.spatial_passes = tmp;
+            return true;
+        }
         if (f == "neighbours") return as_int(ropt.neighbours);
         if (f == "radius") return as_int(ropt.radius);
         return true;
@@ -700,7 +707,7 @@ int krt_armada_set_option(krt_armada *a, const char *field, const krt_option *va
         KRT_REQUIRE(a && field && value, KRT_ERR_INVALID, "null argument");
         KRT_REQUIRE(!a->active.empty(), KRT_ERR_STATE, "no active attachment");
         KRT_REQUIRE(a->attachments.at(a->active)->set_option(field, *value), KRT_ERR_INVALID,
-                    std::string("bad value type for option ") + field);
+                    std::string("bad value or value type for option ") + field);
     });
 }
 

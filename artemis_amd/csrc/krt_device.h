@@ -339,6 +339,15 @@ KRT_DEV float4 accumulate4(float4 dst, float4 s, bool acc, float count) {
                        (dst.w * count + s.w) / c1);
 }
 
+// (int)((double)x + 0.5) of the reference's reprojection (amadeus_restir.cu:
+// 202-203, a double literal) for 0 <= x < 2^23 without FP64: the double sum
+// is exact, so it equals floor(x) + (frac(x) >= 0.5), and x - floor(x) is
+// exact in float for such x.
+KRT_DEV int round_half_up(float x) {
+    const int i = (int)x;
+    return i + ((x - (float)i) >= 0.5f ? 1 : 0);
+}
+
 // Wave-level counter add (one atomic per wavefront).
 KRT_DEV void wave_count(unsigned long long *ctr, unsigned v) {
     if (!ctr) return;

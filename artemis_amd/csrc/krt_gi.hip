@@ -400,8 +400,8 @@ KRT_DEV int gi_reproject(const LaunchInfo &li, uint32_t pix, f3 x) {
     for (int k = 0; k < 4; k++) v[k] = (m[0 + k] * x.x + m[4 + k] * x.y) + (m[8 + k] * x.z + m[12 + k] * 1.0f);
     float u = v[0] / v[3], vv = v[1] / v[3];
     if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
-    int ix = (int)((double)((u + 1.0f) * 0.5f * (float)li.width) + 0.5);
-    int iy = (int)((double)((vv + 1.0f) * 0.5f * (float)li.height) + 0.5);
+    int ix = round_half_up((u + 1.0f) * 0.5f * (float)li.width);
+    int iy = round_half_up((vv + 1.0f) * 0.5f * (float)li.height);
     ix = min(max(ix, 0), (int)li.width - 1);
     iy = min(max(iy, 0), (int)li.height - 1);
     return iy * (int)li.width + ix;

@@ -174,8 +174,8 @@ KRT_DEV int reproject(const LaunchInfo &li, int index, f3 pos) {
     if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
     u = (u + 1.0f) * 0.5f;
     vv = (vv + 1.0f) * 0.5f;
-    int ix = (int)((double)(u * (float)li.width) + 0.5);
-    int iy = (int)((double)(vv * (float)li.height) + 0.5);
+    int ix = round_half_up(u * (float)li.width);
+    int iy = round_half_up(vv * (float)li.height);
     ix = min(max(ix, 0), (int)li.width - 1);
     iy = min(max(iy, 0), (int)li.height - 1);
     return iy * (int)li.width + ix;
@@ -859,8 +859,8 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR buffers not loaded");
     KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kMaxNeighbours, KRT_ERR_INVALID,
                 "restir neighbours must be in [0, 8]");
-    KRT_REQUIRE(opt.candidates >= 0 && opt.spatial_passes >= 0 && opt.m_cap >= 0 && opt.radius >= 0,
-                KRT_ERR_INVALID, "restir options must be non-negative");
+    KRT_REQUIRE(opt.candidates >= 0 && opt.spatial_passes >= 1 && opt.m_cap >= 0 && opt.radius >= 0,
+                KRT_ERR_INVALID, "restir options must be non-negative and spatial_passes >= 1");
     LaunchInfo li = li_in;
     for (int k = 0; k < 16; k++) li.prev_pv[k] = st.prev_pv[k];
     const uint32_t npix = li.width * li.height;

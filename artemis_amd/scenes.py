@@ -238,6 +238,24 @@ def cornell(seed: int = 1, light_half: float = 0.25, light_emission: float = 15.
     return b.build(mats, camera=cam, name="cornell")
 
 
+def cornell_glossy(seed: int = 5) -> SceneData:
+    """The Cornell box with every branch of the BSDF (brdf.cuh) in use: walls
+    and the tall box with specular U[0.05, 0.9]^3 (the GGX lobe is sampled
+    with probability Ks / (Kd + Ks)), the short box a near-mirror (Kd 0.05,
+    Ks 0.95, roughness 0.05), the icosphere glass (type eTransmission,
+    refraction 1.5, specular 0.9: Glass::sample, brdf.cuh:125-176)."""
+    base = cornell(seed=seed)
+    rng = SplitMix64(seed + 7)
+    mats = np.array(base.materials, np.float32)
+    for m in range(len(mats) - 1):  # the last material is the light
+        mats[m, 3:6] = [rng.uniform(0.05, 0.9) for _ in range(3)]
+    short_box, sphere = 6, 7
+    mats[short_box] = material((0.05, 0.05, 0.05), 0.05, specular=(0.95, 0.95, 0.95))
+    mats[sphere] = material((0.0, 0.0, 0.0), 0.1, specular=(0.9, 0.9, 0.9), refraction=1.5, type_=2.0)
+    return SceneData(base.positions, base.normals, base.indices, base.tri_material, mats, camera=base.camera,
+                     name="cornell_glossy")
+
+
 def sphere_mesh(slices: int, stacks: int):
     """Submesh::sphere (source/mesh.cpp:189-290): unit sphere, 2*slices*(stacks-1) triangles."""
     pos = [(0.0, 1.0, 0.0)]

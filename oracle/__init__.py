@@ -53,6 +53,8 @@ _SIGS = {
     "orc_ggx_brdf": (None, [fp, fp, fp, fp, fp, C.c_float, fp]),
     "orc_ggx_sample": (None, [fp, fp, fp, fp, C.c_float, fp, fp]),
     "orc_light_sample": (None, [fp, fp, fp]),
+    "orc_ggx_sample_batch": (None, [fp, fp, fp, fp, C.c_float, C.c_int, C.c_uint32, fp]),
+    "orc_glass_sample": (None, [fp, fp, C.c_float, C.c_int, fp]),
     "orc_render_pt": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), fp, fp, fp, fp, C.c_int, u64p]),
     "orc_render_pt_pixels": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), i32p, C.c_int, fp]),
     "orc_accumulate_pt_f64": (None, [vp, C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int, C.c_int, dp,

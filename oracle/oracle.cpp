@@ -861,6 +861,31 @@ void orc_ggx_sample(const float *n, const float *wo, const float *kd, const floa
     wi_out[2] = wi.z;
 }
 
+void orc_ggx_sample_batch(const float *n, const float *wo, const float *kd, const float *ks, float roughness,
+                          int count, uint32_t word, float *wi_out) {
+    const Material m = probe_mat(kd, ks, roughness);
+    for (int i = 0; i < count; i++) {
+        f3 sd = seed_from_words((uint32_t)i, word, 0x51u);
+        f3 wi = ggx_sample(m, load3(n), load3(wo), sd);
+        wi_out[3 * i] = wi.x;
+        wi_out[3 * i + 1] = wi.y;
+        wi_out[3 * i + 2] = wi.z;
+    }
+}
+
+void orc_glass_sample(const float *n, const float *wi, float eta, int entering, float *out) {
+    SurfaceHit sh;
+    sh.mat = probe_mat(n, n, 1.0f);
+    sh.mat.refraction = eta;
+    sh.mat.type = eTransmission;
+    sh.n = load3(n);
+    sh.entering = entering != 0;
+    f3 d = glass_sample(sh, load3(wi));
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
 void orc_light_sample(const float *l, float *seed, float *point_out) {
     TriLight L{load3(l), load3(l + 3), load3(l + 6), load3(l + 9)};
     f3 sd = load3(seed);

@@ -97,6 +97,11 @@ void orc_ggx_brdf(const float *n, const float *wi, const float *wo, const float 
 void orc_ggx_sample(const float *n, const float *wo, const float *kd, const float *ks,
                     float roughness, float *seed, float *wi_out);
 void orc_light_sample(const float *light12, float *seed, float *point_out);
+/* count GGX/Lambert samples (brdf.cuh:386-433) from the seeds (i, word, 0x51) */
+void orc_ggx_sample_batch(const float *n, const float *wo, const float *kd, const float *ks, float roughness,
+                          int count, uint32_t word, float *wi_out);
+/* Glass::sample direction (brdf.cuh:144-175) for incident direction wi */
+void orc_glass_sample(const float *n, const float *wi, float eta, int entering, float *out);
 
 typedef struct {
     float center[3], U[3], V[3], W[3];

@@ -35,7 +35,7 @@ def test_reservoirs_per_frame(krt_ctx, passes, mcap, frames):
     col = arm.color_buffer()
     ex = np.mean(np.all(col.view(np.uint32) == ors.buffers[0].view(np.uint32), axis=-1))
     print(f"  color exact {ex:.4f}")
-    assert ex >= 0.99
+    assert ex == 1.0
     sc.free()
 
 
