@@ -183,12 +183,13 @@ int krt_create(int device, krt_context **out) {
         KRT_HIP(hipGetDeviceCount(&n));
         KRT_REQUIRE(device >= 0 && device < n, KRT_ERR_INVALID, "no such HIP device");
         KRT_HIP(hipSetDevice(device));
-        auto *c = new krt_context();
+        auto c = std::make_unique<krt_context>();
         c->impl.device = device;
+        c->impl.noalloc_spill = noalloc_scratch();  // once per device, with the device current
         KRT_HIP(hipStreamCreate(&c->impl.stream));
         KRT_HIP(hipStreamCreate(&c->impl.side));
         c->impl.counters.alloc(sizeof(unsigned long long) * 16);
-        *out = c;
+        *out = c.release();
     });
 }
 
@@ -528,6 +529,14 @@ int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *ray
     return guard([&] {
         KRT_REQUIRE(ctx && rays_dev && hits_dev, KRT_ERR_INVALID, "null argument");
         trace_closest_dev(ctx->impl, ctx->impl.scene(scene), (const float *)rays_dev, n, (float *)hits_dev);
+    });
+}
+
+int krt_trace_closest_device_noalloc(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n,
+                                     void *hits_dev) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && rays_dev && hits_dev, KRT_ERR_INVALID, "null argument");
+        trace_closest_dev_noalloc(ctx->impl, ctx->impl.scene(scene), (const float *)rays_dev, n, (float *)hits_dev);
     });
 }
 

@@ -160,6 +160,7 @@ struct Context {
     DevBuf spill;     // traversal stack spill slab (grown on demand)
     WaveScratch wave;
     hipEvent_t timer[2] = {nullptr, nullptr};  // krt_timer_begin / krt_timer_end
+    int *noalloc_spill = nullptr;  // this device's address of the file-scope slab of krt_trace.hip
     Scene &scene(krt_handle h) {
         auto it = scenes.find(h);
         KRT_REQUIRE(it != scenes.end(), KRT_ERR_INVALID, "unknown scene handle");
@@ -172,6 +173,14 @@ void accel_build(Context &ctx, Scene &s, int leaf_max, int layout);
 void upload_scene_derived(Context &ctx, Scene &s, const float *mats12, uint32_t nm);
 void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits);
 void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ);
+// The closest-hit query without allocation: the traversal-stack overflow slab
+// is a file-scope __device__ array of kNoAllocSpillInts ints (16 MiB) instead
+// of the grown-on-demand Context::spill.  noalloc_scratch() returns the
+// calling device's address of that array (hipGetSymbolAddress, resolved once
+// per device; call it with the device current).
+constexpr size_t kNoAllocSpillInts = size_t(4) << 20;
+int *noalloc_scratch();
+void trace_closest_dev_noalloc(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits);
 
 // Launch parameters of the rendering kernels (ArmadaLaunchInfo, armada.cuh:43-83)
 struct LaunchInfo {

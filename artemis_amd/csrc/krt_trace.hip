@@ -1,6 +1,8 @@
 // Ray-query kernels behind krt_trace_closest / krt_trace_occluded.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "krt_device.h"
 #include "krt_internal.h"
 #include "krt_launch.h"
@@ -124,6 +126,61 @@ static void trace_common(Context &ctx, Scene &s, const float *rays_dev, uint32_t
 
 void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits) {
     trace_common(ctx, s, rays, n, hits, nullptr, false);
+}
+
+// Traversal-stack overflow slab of trace_closest_dev_noalloc.  In host code
+// `scratch` names a host-side stand-in, not device memory: kernels get the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) int scratch[kNoAllocSpillInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int *noalloc_scratch() {
+    int dev = 0;
+    KRT_HIP(hipGetDevice(&dev));
+    KRT_REQUIRE(dev >= 0 && dev < kMaxDevices, KRT_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        KRT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        KRT_REQUIRE(p != nullptr, KRT_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch slab");
+        g_scratch_ptr[dev] = (int *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Same kernel and results as trace_closest_dev; the persistent grid is cut to
+// what the slab holds (depth x 128 lanes per workgroup).  Nothing is
+// allocated and nothing needs clearing: a stack entry is written before it is
+// read.  Calls that overlap on one device share the slab and race.
+void trace_closest_dev_noalloc(Context &ctx, Scene &s, const float *rays_dev, uint32_t n, float *hits_dev) {
+    KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure (call krt_accel_build)");
+    KRT_REQUIRE(ctx.noalloc_spill != nullptr, KRT_ERR_STATE, "context has no scratch slab address");
+    const size_t depth = s.accel.depth_bound > (uint32_t)kLdsDepth ? s.accel.depth_bound - kLdsDepth + 1 : 1;
+    const size_t per_block = depth * (size_t)kTraceBlock;
+    KRT_REQUIRE(per_block <= kNoAllocSpillInts, KRT_ERR_BUDGET,
+                "traversal stack of this tree (depth bound " + std::to_string(s.accel.depth_bound) +
+                    ") exceeds the no-allocation slab of " + std::to_string(kNoAllocSpillInts) + " ints");
+    if (n == 0) return;
+    const uint32_t cap_blocks = (uint32_t)(kNoAllocSpillInts / per_block);
+    uint32_t nb = persistent_blocks(n, kTraceBlock, kMaxResidentBlocks);
+    if (nb > cap_blocks) nb = cap_blocks;
+    const float4 *nodes = s.accel.nodes.as<float4>(), *wide = s.accel.wide.as<float4>(),
+                 *tri = s.accel.tri.as<float4>(), *tlas = s.tlas.wide.as<float4>(), *inst = s.inst_rec.as<float4>();
+    const int dbound = (int)s.accel.depth_bound;
+    if (s.instanced)
+        trace_kernel<false, true><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst,
+                                                                     (const float4 *)rays_dev, n, (float4 *)hits_dev,
+                                                                     nullptr, ctx.noalloc_spill, dbound);
+    else
+        trace_kernel<false, false><<<nb, kTraceBlock, 0, ctx.stream>>>(nodes, wide, tri, tlas, inst,
+                                                                      (const float4 *)rays_dev, n, (float4 *)hits_dev,
+                                                                      nullptr, ctx.noalloc_spill, dbound);
+    KRT_HIP(hipGetLastError());
 }
 
 void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ) {

@@ -156,6 +156,17 @@ int krt_trace_occluded(krt_context *ctx, krt_handle scene, const float *rays, ui
 /* Same on device buffers (rays/hits already in HBM); enqueued on the
  * context stream, no synchronisation. */
 int krt_trace_closest_device(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n, void *hits_dev);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same query
+ * and the same hits as krt_trace_closest_device, but the traversal stack's
+ * overflow slab is a fixed 16 MiB device array of the library (one per GPU,
+ * held until the process ends) instead of memory allocated on demand, so
+ * the call makes no hipMalloc / hipFree.  The workgroup count is cut to what
+ * the slab holds (tree depth bound x 128 lanes per workgroup);
+ * KRT_ERR_BUDGET if the tree is deeper than the slab allows.  Calls that
+ * overlap on one GPU (several contexts or threads) share the slab and race. */
+int krt_trace_closest_device_noalloc(krt_context *ctx, krt_handle scene, const void *rays_dev, uint32_t n,
+                                     void *hits_dev);
 
 /* Streaming copy of `bytes` between two device buffers, `reps` times: average
  * milliseconds per copy (HBM bandwidth and profiler byte-counter calibration;
