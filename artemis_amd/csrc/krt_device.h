@@ -108,6 +108,19 @@ struct TileOrder {
     }
 };
 
+// Workgroups go to the chip's 8 XCDs round-robin (workgroup b to XCD b % 8),
+// and each XCD has its own L2.  Per-pixel kernels take their tiles through
+// this bijection of [0, gridDim.x), which hands XCD x one contiguous run of
+// workgroups (a band of image rows), so that the neighbourhood reads of a
+// tile (spatial reuse, reprojection) mostly hit lines its own L2 already
+// holds instead of pulling them into all eight.
+constexpr uint32_t kXcds = 8;
+KRT_DEV uint32_t xcd_block() {
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t q = G / kXcds, r = G % kXcds, x = b % kXcds, j = b / kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // Seeds: three u32 words bit-cast into the float3 state.
 KRT_DEV f3 pixel_seed(uint32_t px, uint32_t py, uint32_t sample, uint32_t user) {
     return mk(__uint_as_float(px), __uint_as_float(py), __uint_as_float(sample + 0x9E3779B9u * user));

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_camera_kernel(LaunchInfo li, 
     if (threadIdx.x == 0) nray = 0;
     __syncthreads();
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px = 0, py = 0, pix = 0;
     const bool live = item < tiles.items && tiles.pixel(item, px, py);
@@ -314,15 +314,15 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_shade_kernel(LaunchInfo li, S
     emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
     __syncthreads();
     if (threadIdx.x == 0) {
-        B.sl.counts[blockIdx.x] = nsl;
-        B.cl.counts[blockIdx.x] = ncl;
+        B.sl.counts[xcd_block()] = nsl;
+        B.cl.counts[xcd_block()] = ncl;
     }
 }
 
 __global__ __launch_bounds__(kGiWaveBlock) void gi_compose_kernel(GiArgs A, GiWave B) {
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_rays_kernel(GiArgs
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px = 0, py = 0, pix = 0;
     bool live = item < tiles.items && tiles.pixel(item, px, py);
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_merge_kernel(GiArg
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     const int S = A.neighbours;
     uint32_t px = 0, py = 0, pix = 0;
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_merge_kernel(GiArg
 __global__ __launch_bounds__(kGiSpatialBlock) void gi_spatial_resolve_kernel(GiArgs A, GiSpatialBufs B) {
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_camera_kernel(LaunchInfo li, Pt
     if (threadIdx.x == 0) nray = 0;
     __syncthreads();
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t px = 0, py = 0, pix = 0;
     const bool live = item < tiles.items && tiles.pixel(item, px, py);
     f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
@@ -170,8 +170,8 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
     emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
     __syncthreads();
     if (threadIdx.x == 0) {
-        B.sl.counts[blockIdx.x] = nsl;
-        B.cl.counts[blockIdx.x] = ncl;
+        B.sl.counts[xcd_block()] = nsl;
+        B.cl.counts[xcd_block()] = ncl;
     }
 }
 
@@ -197,7 +197,7 @@ KRT_DEV f3 level_direct(const PtWave &B, uint32_t npix, uint32_t pix, int k, flo
 __global__ __launch_bounds__(kWaveBlock) void pt_cache_compose_kernel(LaunchInfo li, PtWave B, CacheView cv,
                                                                       float4 *prefix) {
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_cache_compose_kernel(LaunchInfo
 
 __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, PtWave B, double *acc64) {
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_camera_kernel(RestirArgs A
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t px = 0, py = 0, pix = 0;
     const bool live = item < tiles.items && tiles.pixel(item, px, py);
     f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
@@ -419,15 +419,15 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
     emit_ray(B.cl, &ncl, next, no, nd, kTraceTmax, pix);
     __syncthreads();
     if (threadIdx.x == 0) {
-        B.sl.counts[blockIdx.x] = nsl;
-        B.cl.counts[blockIdx.x] = ncl;
+        B.sl.counts[xcd_block()] = nsl;
+        B.cl.counts[xcd_block()] = ncl;
     }
 }
 
 __global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs A, InitWave B) {
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px = 0, py = 0, pix = 0;
     bool live = item < tiles.items && tiles.pixel(item, px, py);
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kSpatialBlock) void temporal_rays_kernel(RestirArgs
 __global__ __launch_bounds__(kSpatialBlock) void temporal_resolve_kernel(RestirArgs A, SpatialBufs B) {
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_rays_kernel(RestirArgs 
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px = 0, py = 0, pix = 0;
     bool live = item < tiles.items && tiles.pixel(item, px, py);
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
     __syncthreads();
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     const int S = A.neighbours;
     uint32_t px = 0, py = 0, pix = 0;
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
 __global__ __launch_bounds__(kSpatialBlock) void spatial_resolve_kernel(RestirArgs A, SpatialBufs B) {
     const LaunchInfo &li = A.li;
     const TileOrder tiles(li.width, li.height);
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t npix = li.width * li.height;
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;

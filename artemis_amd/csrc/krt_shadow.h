@@ -19,6 +19,7 @@ namespace krt {
 namespace {
 
 // Producers append with no global atomics: workgroup b of a producing launch
+// (numbered by xcd_block(), the order in which its tiles lie in the image)
 // owns the segment [b * seg, (b + 1) * seg) of `rays` and counts its rays in
 // LDS; counts[b] is written by every producing workgroup, so nothing needs
 // clearing between launches.  Before the walk, the segments are packed into
@@ -54,7 +55,7 @@ KRT_DEV void emit_ray(const RayList &L, uint32_t *nray, bool valid, f3 o, f3 d, 
     if ((int)lane == leader) base = atomicAdd(nray, (uint32_t)__popcll(want));
     base = __shfl(base, leader, 64);
     if (valid) {
-        const size_t k = (size_t)blockIdx.x * L.seg + base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        const size_t k = (size_t)xcd_block() * L.seg + base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
         L.rays[2 * k] = make_float4(o.x, o.y, o.z, tmax);
         L.rays[2 * k + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
     }
@@ -62,7 +63,7 @@ KRT_DEV void emit_ray(const RayList &L, uint32_t *nray, bool valid, f3 o, f3 d, 
 
 KRT_DEV void publish_rays(const RayList &L, const uint32_t *nray) {
     __syncthreads();
-    if (threadIdx.x == 0) L.counts[blockIdx.x] = *nray;
+    if (threadIdx.x == 0) L.counts[xcd_block()] = *nray;
 }
 
 constexpr int kScanBlock = 1024;
