@@ -421,7 +421,8 @@ class RadianceField:
 
 class Server:
     """Render-request server over krt_server_* (continuous batching of
-    path-tracer requests on resident scenes, admission by HBM budget)."""
+    path-tracer and ReSTIR DI requests on resident scenes, reservoirs
+    resident per request, admission by HBM budget)."""
 
     def __init__(self, ctx: Context, hbm_budget: int | None = None, max_active: int | None = None,
                  use_graph: bool = True, max_queue_ms: float = 0.0, step_pixels: int | None = None):
@@ -456,11 +457,19 @@ class Server:
             pass
 
     def submit(self, scene: Scene, camera: CameraFrame, width: int, height: int, spp: int = 1,
-               max_depth: int = 5, seed: int = 0) -> int:
+               max_depth: int = 5, seed: int = 0, kind: str = "path_tracer", candidates: int = 32,
+               m_cap: int = 200, spatial_passes: int = 2, neighbours: int = 5, radius: int = 30) -> int:
+        """Queue a request: `spp` accumulated frames of the path tracer
+        (kind "path_tracer") or of ReSTIR DI (kind "restir", with the restir
+        attachment's options)."""
         r = L.Request()
         r.scene = scene.h
         r.camera = camera.to_ctypes()
         r.width, r.height, r.spp, r.max_depth, r.seed = width, height, spp, max_depth, seed
+        if kind not in ("path_tracer", "restir"):
+            raise ValueError(f"unknown request kind {kind!r}")
+        r.kind = L.KRT_REQUEST_RESTIR if kind == "restir" else L.KRT_REQUEST_PATH_TRACER
+        r.restir = L.RestirOptions(candidates, m_cap, spatial_passes, neighbours, radius)
         rid = C.c_uint64()
         L.check(self.ctx._lib.krt_server_submit(self.h, C.byref(r), C.byref(rid)))
         self._shape[rid.value] = (height, width)

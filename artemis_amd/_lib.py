@@ -81,9 +81,19 @@ class ServerConfig(C.Structure):
                 ("max_queue_ms", C.c_double), ("step_pixels", C.c_uint64)]
 
 
+class RestirOptions(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("m_cap", C.c_int32), ("spatial_passes", C.c_int32),
+                ("neighbours", C.c_int32), ("radius", C.c_int32)]
+
+
+KRT_REQUEST_PATH_TRACER = 0
+KRT_REQUEST_RESTIR = 1
+
+
 class Request(C.Structure):
     _fields_ = [("scene", C.c_uint64), ("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
-                ("spp", C.c_uint32), ("max_depth", C.c_int32), ("seed", C.c_uint32)]
+                ("spp", C.c_uint32), ("max_depth", C.c_int32), ("seed", C.c_uint32), ("kind", C.c_int32),
+                ("restir", RestirOptions)]
 
 
 class ServerStats(C.Structure):

@@ -420,6 +420,8 @@ int krt_scene_free(krt_context *ctx, krt_handle scene) {
     return guard([&] {
         KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
         KRT_HIP(hipStreamSynchronize(ctx->impl.stream));
+        KRT_REQUIRE(ctx->impl.scene(scene).serve_refs.load() == 0, KRT_ERR_STATE,
+                    "scene is used by server requests in flight");
         KRT_REQUIRE(ctx->impl.scenes.erase(scene) == 1, KRT_ERR_INVALID, "unknown scene handle");
     });
 }
@@ -451,7 +453,9 @@ int krt_accel_build_layout(krt_context *ctx, krt_handle scene, int leaf_max, int
     return guard([&] {
         KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
         KRT_HIP(hipSetDevice(ctx->impl.device));
-        accel_build(ctx->impl, ctx->impl.scene(scene), leaf_max, layout);
+        Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(s.serve_refs.load() == 0, KRT_ERR_STATE, "scene is used by server requests in flight");
+        accel_build(ctx->impl, s, leaf_max, layout);
     });
 }
 
@@ -1177,7 +1181,16 @@ int krt_server_submit(krt_server *srv, const krt_request *req, uint64_t *id) {
         Scene &s = srv->ctx->impl.scene(req->scene);
         KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "the server renders flat scenes (krt_scene_upload)");
         KRT_HIP(hipStreamSynchronize(srv->ctx->impl.stream));  // scene uploads and builds are complete
-        *id = serve_submit(srv->impl, s, req->camera, req->width, req->height, req->spp, req->max_depth, req->seed);
+        RestirOptions ro;
+        if (req->kind == KRT_REQUEST_RESTIR) {
+            ro.candidates = req->restir.candidates;
+            ro.m_cap = req->restir.m_cap;
+            ro.spatial_passes = req->restir.spatial_passes;
+            ro.neighbours = req->restir.neighbours;
+            ro.radius = req->restir.radius;
+        }
+        *id = serve_submit(srv->impl, s, req->camera, req->width, req->height, req->spp, req->max_depth, req->seed,
+                           req->kind, ro);
     });
 }
 

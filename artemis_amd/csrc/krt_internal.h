@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -101,6 +102,9 @@ struct Scene {
     Accel accel;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // vertex bounding box
     uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
+    // server requests that hold this scene (queued or rendering): while any
+    // do, the scene may be neither freed nor rebuilt
+    std::atomic<uint32_t> serve_refs{0};
     // Two-level scenes (krt_scene_upload_instanced).  pos / nrm / idx then hold
     // the base meshes back to back in their own space (indices local to their
     // mesh); tri_mat is per global triangle (instance i's triangles follow
@@ -340,8 +344,10 @@ struct CacheState {
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
                      unsigned long long *ctr, PassTimes &times);
 // ReSTIR DI; with a cache (copt->enabled) its indirect path stops at the cache vertex.
+// sync = false: everything is queued on ctx.stream and the call returns
+// without waiting (no pass times); the radiance cache needs sync = true.
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,
                    unsigned long long *ctr, PassTimes &times, const CacheOptions *copt = nullptr,
-                   CacheState *cs = nullptr);
+                   CacheState *cs = nullptr, bool sync = true);
 
 }  // namespace krt

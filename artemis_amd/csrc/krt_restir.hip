@@ -854,7 +854,7 @@ void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_
 }
 
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOptions &opt, RestirState &st,
-                   unsigned long long *ctr, PassTimes &times, const CacheOptions *copt, CacheState *cs) {
+                   unsigned long long *ctr, PassTimes &times, const CacheOptions *copt, CacheState *cs, bool sync) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR buffers not loaded");
     KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kMaxNeighbours, KRT_ERR_INVALID,
@@ -881,6 +881,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.neighbours = opt.neighbours;
     A.radius = opt.radius;
     const bool cached = copt && cs && copt->enabled;
+    KRT_REQUIRE(sync || !cached, KRT_ERR_INVALID, "the radiance cache needs a synchronous ReSTIR frame");
     if (cached) {
         KRT_REQUIRE(copt->depth >= 1, KRT_ERR_INVALID, "restir radiance cache needs cache_depth >= 1");
         A.cv = cache_begin(ctx, s, *copt, *cs, npix, li.user_seed);
@@ -889,6 +890,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     hipEvent_t ev[16];
     int nev = 0;
     auto mark = [&]() {
+        if (!sync) return;  // untimed: queued on ctx.stream, no host wait
         KRT_HIP(hipEventCreate(&ev[nev]));
         KRT_HIP(hipEventRecord(ev[nev], ctx.stream));
         nev++;
@@ -1037,6 +1039,11 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     mark();
     st.cur = 1 - st.cur;  // final swap, restir.cuh:300
     for (int k = 0; k < 16; k++) st.prev_pv[k] = li.pv[k];
+    if (!sync) {
+        for (hipEvent_t e : {queried, inf0, inf1, folded})
+            if (e) (void)hipEventDestroy(e);
+        return;
+    }
     KRT_HIP(hipEventSynchronize(ev[nev - 1]));
     // segments: initial, temporal, spatial, training; the inference is timed
     // on the side stream

@@ -1,5 +1,6 @@
 // Render-request server (krt_serve.hip): continuous batching of path-tracer
-// requests over resident scenes, admission by HBM budget.
+// and ReSTIR DI requests over resident scenes, reservoirs resident per
+// request, admission by HBM budget.
 #pragma once
 
 #include "krt_internal.h"
@@ -20,11 +21,15 @@ struct ServeConfig {
     uint64_t step_pixels = 1ull << 22;  // pixel-frames one step aims at, shared fairly
 };
 
+constexpr int kServeLanes = 3;  // streams that run ReSTIR requests' frames side by side
+
 struct Server;
 Server *serve_create(Context &ctx, const ServeConfig &cfg);
 void serve_destroy(Server *s);
+// kind KRT_REQUEST_PATH_TRACER or KRT_REQUEST_RESTIR (ropt: the restir
+// attachment's options)
 uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w, uint32_t h, uint32_t spp,
-                      int max_depth, uint32_t seed);
+                      int max_depth, uint32_t seed, int kind, const RestirOptions &ropt);
 // KRT_OK with the colour buffer (W x H x float4) or KRT_ERR_BUDGET when the
 // request was rejected; the request is forgotten afterwards.
 int serve_wait(Server *s, uint64_t id, float *color, double *latency_ms, uint32_t *w, uint32_t *h);

@@ -1,25 +1,33 @@
-// Render-request server: continuous batching of path-tracer requests.
+// Render-request server: continuous batching of path-tracer and ReSTIR DI
+// requests.
 //
-// A request is {scene, camera, resolution, spp, depth, seed}; its answer is
-// the colour buffer the path-tracer attachment would hold after `spp`
-// accumulated frames (ArmadaRTX::render, source/amadeus/armada.cu:733-780,
-// with the estimator of source/optix/amadeus_path_tracer.cu:73-228).
+// A request is {scene, camera, resolution, spp, depth, seed, kind}; its
+// answer is the colour buffer the path-tracer (or ReSTIR DI) attachment would
+// hold after `spp` accumulated frames (ArmadaRTX::render,
+// source/amadeus/armada.cu:733-780, with the estimators of
+// source/optix/amadeus_path_tracer.cu:73-228 and amadeus_restir.cu).
 //
-// One host thread runs the loop.  Each step advances every active request with
-// a single persistent launch: the grid strides over the concatenated pixels of
-// all active requests and finds each pixel's request by binary search in a
-// device table {LaunchInfo, SceneView, first pixel, frames}.  A request's share
-// of a step is a fair share of `step_pixels` pixel-frames: small requests run
-// several consecutive frames of a pixel in one thread (the accumulation is per
-// pixel, so the result is the one a launch per frame gives) and finish in one
-// step; large ones advance a frame per step.  The
-// step (table upload + kernel) is a hipGraph captured once; only the table's
-// contents change between steps.  Requests arriving while others run join at
-// the next step as soon as a slot and HBM budget are free (FIFO admission,
-// optional queueing deadline); finished requests leave after their last frame
-// and their buffers are released.  Scenes (vertex data and BVHs) stay
-// resident; every request keeps its own AOV buffers, so a pixel's value never
-// depends on what else shares the step.
+// One host thread runs the loop.  Each step advances every active request:
+//  * path-tracer requests in a single persistent launch: the grid strides
+//    over the concatenated pixels of all of them and finds each pixel's
+//    request by binary search in a device table {LaunchInfo, SceneView, first
+//    pixel, frames}.  A request's share of a step is a fair share of
+//    `step_pixels` pixel-frames: small requests run several consecutive
+//    frames of a pixel in one thread (the accumulation is per pixel, so the
+//    result is the one a launch per frame gives) and finish in one step; large
+//    ones advance a frame per step.  The step (table upload + kernel) is a
+//    hipGraph captured once; only the table's contents change between steps.
+//  * ReSTIR DI requests frame by frame through the attachment's own passes
+//    (restir_render), each request on one of kServeLanes streams for its whole
+//    life (its frames stay in order), the lanes running side by side with the
+//    path-tracer launch.  A request's reservoirs, G-buffer and ray lists are
+//    allocated when it is admitted and stay resident until its last frame.
+// Requests arriving while others run join at the next step as soon as a slot
+// and HBM budget are free (FIFO admission, optional queueing deadline);
+// finished requests leave after their last frame and their buffers are
+// released.  Scenes (vertex data and BVHs) stay resident and cannot be freed
+// or rebuilt while a request holds them; every request keeps its own
+// buffers, so a pixel's value never depends on what else shares the step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -91,6 +99,10 @@ struct ServeRequest {
     Scene *scene = nullptr;
     uint32_t w = 0, h = 0, spp = 0, seed = 0;
     int max_depth = 5;
+    int kind = KRT_REQUEST_PATH_TRACER;
+    RestirOptions ropt;
+    RestirState rst;  // ReSTIR: reservoirs, G-buffer and ray lists, resident while active
+    int lane = 0;     // ReSTIR: the stream its frames run on
     krt_camera cam{};
     int state = 0;  // 0 queued, 1 active, 2 done, 3 rejected
     uint32_t frames = 0, step_frames = 1;
@@ -98,12 +110,25 @@ struct ServeRequest {
     size_t bytes = 0;
     DevBuf aov;  // colour, normal, albedo, position planes (float4 x W x H each)
     std::vector<float> result;
+    ~ServeRequest() {
+        if (scene && state < 2) scene->serve_refs--;
+    }
+    void finish(int st) {  // done (2) or rejected (3): the scene is free again
+        if (state < 2 && scene) scene->serve_refs--;
+        state = st;
+    }
 };
 
 struct Server {
     Context *ctx = nullptr;
     ServeConfig cfg;
     hipStream_t stream = nullptr;
+    // ReSTIR lanes: a context shell per stream (its stream, traversal-stack
+    // spill slab and counters), so that requests on different lanes share no
+    // scratch
+    std::unique_ptr<Context> lanes[kServeLanes];
+    hipEvent_t lane_done[kServeLanes] = {};
+    int next_lane = 0;
     std::thread th;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -131,7 +156,63 @@ struct Server {
 
 size_t serve_request_bytes(uint32_t w, uint32_t h) { return sizeof(float4) * 4 * (size_t)w * h; }
 
-void Server::step(std::vector<ServeRequest *> &batch) {
+// Device memory of a ReSTIR DI request: AOVs, two reservoir buffers, the
+// G-buffer (material id, intermediate, auxiliary), the shadow-ray list (rays
+// and packed copy, S + 1 slots per pixel) with its occlusion bytes, 1/Z
+// weights and flags, and the initial pass's wavefront state and closest-ray
+// list (pixels rounded up to whole 256-lane workgroups of 8x8 tiles).
+size_t serve_restir_bytes(uint32_t w, uint32_t h, const RestirOptions &o, int depth) {
+    const size_t npix = (size_t)w * h;
+    const size_t items = (size_t)((w + 7) / 8) * ((h + 7) / 8) * 64;
+    const size_t slots = (items + 255) / 256 * 256;
+    const size_t per_lane = (size_t)std::max(o.neighbours + 1, 2);
+    const size_t levels = (size_t)depth + 1;
+    size_t b = serve_request_bytes(w, h);
+    b += 2 * 64 * npix + 4 * npix + 2 * 16 * npix;
+    b += 2 * 32 * per_lane * slots + per_lane * npix + 4 * (size_t)o.neighbours * npix + npix;
+    b += 16 * npix + 16 * npix + 48 * levels * npix + levels * npix + 2 * 32 * slots;
+    return b + (1u << 16);
+}
+
+void Server::step(std::vector<ServeRequest *> &all_batch) {
+    std::vector<ServeRequest *> batch, restir;
+    for (ServeRequest *r : all_batch) (r->kind == KRT_REQUEST_RESTIR ? restir : batch).push_back(r);
+    KRT_HIP(hipEventRecord(ev[0], stream));
+    // ReSTIR requests: the frames of this step, queued on their lanes
+    bool used[kServeLanes] = {};
+    for (ServeRequest *r : restir) {
+        Context &lane = *lanes[r->lane];
+        if (!used[r->lane]) KRT_HIP(hipStreamWaitEvent(lane.stream, ev[0], 0));
+        used[r->lane] = true;
+        for (uint32_t f = 0; f < r->step_frames; f++) {
+            LaunchInfo li{};
+            li.width = r->w;
+            li.height = r->h;
+            std::memcpy(li.center, r->cam.center, sizeof(li.center));
+            std::memcpy(li.U, r->cam.U, sizeof(li.U));
+            std::memcpy(li.V, r->cam.V, sizeof(li.V));
+            std::memcpy(li.W, r->cam.W, sizeof(li.W));
+            std::memcpy(li.pv, r->cam.pv, sizeof(li.pv));
+            li.max_depth = r->max_depth;
+            li.samples = (int)(r->frames + f);
+            li.accumulate = 1;
+            li.user_seed = r->seed;
+            li.jitter = 1;
+            float4 *planes = r->aov.as<float4>();
+            const size_t n = (size_t)r->w * r->h;
+            li.color = planes;
+            li.normal = planes + n;
+            li.albedo = planes + 2 * n;
+            li.position = planes + 3 * n;
+            PassTimes times;
+            restir_render(lane, *r->scene, li, r->ropt, r->rst, nullptr, times, nullptr, nullptr, false);
+        }
+    }
+    for (int l = 0; l < kServeLanes; l++)
+        if (used[l]) {
+            KRT_HIP(hipEventRecord(lane_done[l], lanes[l]->stream));
+            KRT_HIP(hipStreamWaitEvent(stream, lane_done[l], 0));
+        }
     uint32_t total = 0;
     table_host->count = (uint32_t)batch.size();
     for (size_t i = 0; i < batch.size(); i++) {
@@ -163,14 +244,15 @@ void Server::step(std::vector<ServeRequest *> &batch) {
         total += r.w * r.h;
     }
     table_host->total = total;
-    KRT_HIP(hipEventRecord(ev[0], stream));
-    if (exec) {
-        KRT_HIP(hipGraphLaunch(exec, stream));
-    } else {
-        KRT_HIP(hipMemcpyAsync(table_dev.p, table_host, sizeof(ServeTable), hipMemcpyHostToDevice, stream));
-        serve_kernel<<<nblocks, kTraceBlock, 0, stream>>>(table_dev.as<ServeTable>(), hist.as<float4>(),
-                                                         spill.as<int>());
-        KRT_HIP(hipGetLastError());
+    if (!batch.empty()) {
+        if (exec) {
+            KRT_HIP(hipGraphLaunch(exec, stream));
+        } else {
+            KRT_HIP(hipMemcpyAsync(table_dev.p, table_host, sizeof(ServeTable), hipMemcpyHostToDevice, stream));
+            serve_kernel<<<nblocks, kTraceBlock, 0, stream>>>(table_dev.as<ServeTable>(), hist.as<float4>(),
+                                                             spill.as<int>());
+            KRT_HIP(hipGetLastError());
+        }
     }
     KRT_HIP(hipEventRecord(ev[1], stream));
     KRT_HIP(hipEventSynchronize(ev[1]));
@@ -193,7 +275,7 @@ void Server::run() {
             if (cfg.max_queue_ms > 0)
                 for (auto it = queue.begin(); it != queue.end();) {
                     if (t - (*it)->t_submit > cfg.max_queue_ms) {
-                        (*it)->state = 3;
+                        (*it)->finish(3);
                         (*it)->t_done = t;
                         rejected++;
                         it = queue.erase(it);
@@ -205,11 +287,19 @@ void Server::run() {
             while (!queue.empty() && active.size() < cfg.max_active) {
                 ServeRequest *r = queue.front();
                 if (scratch + in_use + r->bytes > cfg.hbm_budget) break;
-                r->aov.alloc(r->bytes);
+                r->aov.alloc(serve_request_bytes(r->w, r->h));
                 // frame 0 accumulates dst * 0 + sample: recycled memory holding
                 // a non-finite value would poison it (stream-ordered, outside the
                 // captured step)
-                KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->bytes, stream));
+                if (r->kind == KRT_REQUEST_RESTIR) {
+                    r->lane = next_lane;
+                    next_lane = (next_lane + 1) % kServeLanes;
+                    Context &lane = *lanes[r->lane];
+                    KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->aov.bytes, lane.stream));
+                    restir_alloc(lane, r->rst, r->w, r->h, r->seed);  // reservoirs resident from here on
+                } else {
+                    KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->aov.bytes, stream));
+                }
                 in_use += r->bytes;
                 peak = std::max(peak, in_use);
                 r->state = 1;
@@ -235,16 +325,19 @@ void Server::run() {
                 if (r->frames >= r->spp) {
                     r->result.resize(4 * (size_t)r->w * r->h);
                     KRT_HIP(hipMemcpyAsync(r->result.data(), r->aov.p, sizeof(float) * r->result.size(),
-                                           hipMemcpyDeviceToHost, stream));
+                                           hipMemcpyDeviceToHost, stream));  // after the lanes (step waits)
                     finished.push_back(r);
                 }
             }
             if (!finished.empty()) KRT_HIP(hipStreamSynchronize(stream));
-            for (ServeRequest *r : finished) r->aov.release();
+            for (ServeRequest *r : finished) {
+                r->aov.release();
+                r->rst = RestirState();
+            }
             lk.lock();
             const double td = now_ms();
             for (ServeRequest *r : finished) {
-                r->state = 2;
+                r->finish(2);
                 r->t_done = td;
                 in_use -= r->bytes;
                 latency.push_back(td - r->t_submit);
@@ -281,6 +374,13 @@ Server *serve_create(Context &ctx, const ServeConfig &cfg) {
     s->scratch = s->table_dev.bytes + s->hist.bytes + s->spill.bytes;
     KRT_REQUIRE(s->scratch < cfg.hbm_budget, KRT_ERR_BUDGET, "HBM budget below the server's fixed scratch");
     for (auto &e : s->ev) KRT_HIP(hipEventCreate(&e));
+    for (int l = 0; l < kServeLanes; l++) {
+        s->lanes[l] = std::make_unique<Context>();
+        s->lanes[l]->device = ctx.device;
+        KRT_HIP(hipStreamCreateWithFlags(&s->lanes[l]->stream, hipStreamNonBlocking));
+        s->lanes[l]->side = s->lanes[l]->stream;
+        KRT_HIP(hipEventCreate(&s->lane_done[l]));
+    }
     if (cfg.use_graph) {
         // one step = table upload + batched kernel, captured once
         KRT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -300,18 +400,28 @@ void serve_destroy(Server *s) {
     {
         std::lock_guard<std::mutex> g(s->mu);
         s->stop = true;
-        for (ServeRequest *r : s->queue) r->state = 3;
+        for (ServeRequest *r : s->queue) r->finish(3);
         s->queue.clear();
     }
     s->cv.notify_all();
     s->done_cv.notify_all();
     if (s->th.joinable()) s->th.join();
     (void)hipStreamSynchronize(s->stream);
+    for (int l = 0; l < kServeLanes; l++)
+        if (s->lanes[l]) (void)hipStreamSynchronize(s->lanes[l]->stream);
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
     s->all.clear();
+    for (int l = 0; l < kServeLanes; l++) {
+        if (!s->lanes[l]) continue;
+        s->lanes[l]->spill.release();
+        s->lanes[l]->counters.release();
+        (void)hipStreamDestroy(s->lanes[l]->stream);
+        if (s->lane_done[l]) (void)hipEventDestroy(s->lane_done[l]);
+        s->lanes[l].reset();
+    }
     s->table_dev.release();
     s->hist.release();
     s->spill.release();
@@ -321,7 +431,13 @@ void serve_destroy(Server *s) {
 }
 
 uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w, uint32_t h, uint32_t spp,
-                      int max_depth, uint32_t seed) {
+                      int max_depth, uint32_t seed, int kind, const RestirOptions &ropt) {
+    KRT_REQUIRE(kind == KRT_REQUEST_PATH_TRACER || kind == KRT_REQUEST_RESTIR, KRT_ERR_INVALID,
+                "unknown request kind");
+    if (kind == KRT_REQUEST_RESTIR)
+        KRT_REQUIRE(ropt.neighbours >= 0 && ropt.neighbours <= 8 && ropt.candidates >= 0 && ropt.m_cap >= 0 &&
+                        ropt.radius >= 0 && ropt.spatial_passes >= 1,
+                    KRT_ERR_INVALID, "restir options out of range");
     KRT_REQUIRE(scene.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(w > 0 && h > 0 && (uint64_t)w * h <= (1ull << 26), KRT_ERR_INVALID, "resolution out of range");
     KRT_REQUIRE(spp >= 1, KRT_ERR_INVALID, "spp must be >= 1");
@@ -340,12 +456,15 @@ uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w
     r->spp = spp;
     r->max_depth = max_depth;
     r->seed = seed;
-    r->bytes = serve_request_bytes(w, h);
+    r->kind = kind;
+    r->ropt = ropt;
+    r->bytes = kind == KRT_REQUEST_RESTIR ? serve_restir_bytes(w, h, ropt, max_depth) : serve_request_bytes(w, h);
     std::lock_guard<std::mutex> g(s->mu);
     KRT_REQUIRE(s->fault.empty(), KRT_ERR_HIP, "server stopped: " + s->fault);
     KRT_REQUIRE(s->scratch + r->bytes <= s->cfg.hbm_budget, KRT_ERR_BUDGET,
                 "request needs more HBM than the server's budget");
     r->id = s->next_id++;
+    scene.serve_refs++;  // held until the request is done or rejected
     r->t_submit = now_ms();
     if (s->t_first < 0) s->t_first = r->t_submit;
     s->submitted++;

@@ -354,30 +354,46 @@ int krt_attachment_cache(krt_attachment *att, krt_mlp **out);
 
 /* ------------------------------------------------------------- serving --
  * Render-request server (no reference counterpart): continuous batching of
- * path-tracer requests.  A request's answer is the colour buffer a
- * path_tracer attachment holds after `spp` accumulated frames of the same
- * scene, camera, depth and seed (ArmadaRTX::render, armada.cu:733-780), bit
- * for bit, whatever else runs beside it.  Every step advances each active
- * request by its share of `step_pixels` pixel-frames (at least one frame) in
- * one persistent launch (a captured hipGraph); requests are admitted FIFO as
- * slots and HBM budget free up.  Scenes must
- * outlive the requests that use them. */
+ * path-tracer and ReSTIR DI requests.  A request's answer is the colour
+ * buffer a path_tracer (or restir) attachment holds after `spp` accumulated
+ * frames of the same scene, camera, depth, seed and options
+ * (ArmadaRTX::render, armada.cu:733-780), bit for bit, whatever else runs
+ * beside it.  Every step advances each active request by its share of
+ * `step_pixels` pixel-frames (at least one frame): the path-tracer requests
+ * in one persistent launch (a captured hipGraph), each ReSTIR request's
+ * frames on one of a few streams so that they overlap on the device, its
+ * reservoirs resident from admission to its last frame.  Requests are
+ * admitted FIFO as slots and HBM budget free up.  While a request holds a
+ * scene, krt_scene_free and krt_accel_build of that scene fail with
+ * KRT_ERR_STATE. */
 typedef struct krt_server krt_server;
 
 typedef struct {
-    uint64_t hbm_budget;   /* bytes: fixed scratch + 64 B per pixel of each active request */
+    uint64_t hbm_budget;   /* bytes: fixed scratch + each active request's buffers (64 B per pixel for the
+                              path tracer, about 0.9 KB per pixel for ReSTIR) */
     uint32_t max_active;   /* requests per step, 1..64 */
     int32_t use_graph;     /* 1: replay a captured hipGraph per step */
     double max_queue_ms;   /* > 0: reject requests queued longer than this */
     uint64_t step_pixels;  /* pixel-frames per step, shared fairly among active requests */
 } krt_server_config;
 
+enum { KRT_REQUEST_PATH_TRACER = 0, KRT_REQUEST_RESTIR = 1 };
+
+/* The restir attachment's options ("candidates", "m_cap", "spatial_passes",
+ * "neighbours", "radius"; include/amadeus/restir.cuh:223-304 runs 32, 200, 2,
+ * 5, 30). */
+typedef struct {
+    int32_t candidates, m_cap, spatial_passes, neighbours, radius;
+} krt_restir_options;
+
 typedef struct {
     krt_handle scene;
     krt_camera camera;
-    uint32_t width, height, spp;
+    uint32_t width, height, spp; /* spp: accumulated frames */
     int32_t max_depth;     /* 0..8 */
     uint32_t seed;         /* the attachment's "seed" option */
+    int32_t kind;          /* KRT_REQUEST_PATH_TRACER or KRT_REQUEST_RESTIR */
+    krt_restir_options restir; /* KRT_REQUEST_RESTIR only */
 } krt_request;
 
 typedef struct {
