@@ -11,8 +11,8 @@ BVH, reservoirs and AOV buffers are resident in HBM before timing starts.
 The default run also carries, as extra keys of the same line: the
 equal-relMSE study (quality(); skip with --no-quality), short runs of the
 path tracer (depth 5), the path tracer with the radiance cache, ReSTIR GI
-(2M triangles, panning camera) and the cache network alone (`workloads`;
-skip with --no-extra), the CPU oracle on a bounded sample of the headline
+(2M triangles, panning camera), the cache network alone and the serving
+loop at three request rates (`workloads`; skip with --no-extra), the CPU oracle on a bounded sample of the headline
 workload (`cpu_baseline`) and the planned CPU baseline of the Cornell
 configuration (`cpu_baseline_config1`).
 
@@ -286,15 +286,11 @@ SERVE_RES = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
 SERVE_SPP = [1, 4, 16, 64]
 
 
-def serve_workload(args, ctx):
-    """Render-request serving: Poisson arrivals (mean --serve-rate per s) for
-    --serve-seconds, resolution uniform over SERVE_RES, spp over SERVE_SPP, a
-    uniformly chosen one of 8 resident scenes, path depth 5.  The server
-    batches every active request into one launch per step, admits by HBM
-    budget and sheds requests queued longer than --serve-deadline-ms."""
-    import threading
+SERVE_RATES = [5.0, 10.0, 25.0, 50.0, 100.0, 200.0]
 
-    from artemis_amd import Server, look_at, scenes
+
+def serve_scenes(ctx):
+    from artemis_amd import scenes
 
     resident = []
     for k, (inst, sl, st) in enumerate(SERVE_SCENES):
@@ -302,13 +298,29 @@ def serve_workload(args, ctx):
         data = scenes.spheres(instances=inst, slices=sl, stacks=st, emissive=max(64, nt // 256), seed=10 + k)
         resident.append((ctx.upload(data), data))
         log(f"[serve] scene {k}: {data.triangles} triangles resident")
+    return resident
+
+
+def serve_run(args, ctx, resident, rate, seconds):
+    """One serving run: Poisson arrivals at `rate` per second for `seconds`,
+    resolution uniform over SERVE_RES, spp over SERVE_SPP, a uniformly
+    chosen resident scene; half the requests are path-tracer requests (depth
+    5), half ReSTIR DI requests (32 candidates, M-cap 20, one spatial pass of
+    5 neighbours in 30 px, depth 2; reservoirs resident for the request's
+    frames).  The server batches every active request into each step, admits
+    by HBM budget and sheds requests queued longer than the deadline."""
+    import threading
+
+    from artemis_amd import Server, look_at
+
     srv = Server(ctx, hbm_budget=args.serve_budget_gb << 30, max_active=args.serve_active,
                  max_queue_ms=args.serve_deadline_ms)
-    rng = np.random.default_rng(5)
-    n = int(rng.poisson(args.serve_rate * args.serve_seconds))
-    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    rng = np.random.default_rng(5 + int(rate))
+    n = int(rng.poisson(rate * seconds))
+    gaps = rng.exponential(1.0 / rate, n)
     plan = [(float(t), int(rng.integers(len(resident))), SERVE_RES[int(rng.integers(4))],
-             SERVE_SPP[int(rng.integers(4))]) for t in np.cumsum(gaps)]
+             SERVE_SPP[int(rng.integers(4))], "restir" if rng.random() < 0.5 else "path_tracer")
+            for t in np.cumsum(gaps)]
     ids, done = [], []
     lock = threading.Lock()
 
@@ -325,46 +337,71 @@ def serve_workload(args, ctx):
             if rid is None:
                 time.sleep(0.001)
                 continue
-            img, lat = srv.wait(rid)
-            done.append((img is not None, lat))
+            img, lat = srv.wait(rid[0])
+            done.append((img is not None, lat, rid[1]))
             k += 1
 
     th = threading.Thread(target=consumer)
     th.start()
     t0 = time.perf_counter()
     offered = 0.0
-    for i, (t, si, (w, h), spp) in enumerate(plan):
+    for i, (t, si, (w, h), spp, kind) in enumerate(plan):
         delay = t - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
         sc, data = resident[si]
-        rid = srv.submit(sc, look_at(**data.camera, aspect=w / h), w, h, spp=spp, max_depth=5, seed=i)
+        kw = dict(candidates=32, m_cap=20, spatial_passes=1, neighbours=5, radius=30) if kind == "restir" else {}
+        rid = srv.submit(sc, look_at(**data.camera, aspect=w / h), w, h, spp=spp,
+                         max_depth=2 if kind == "restir" else 5, seed=i, kind=kind, **kw)
         offered += w * h * spp
         with lock:
-            ids.append(rid)
+            ids.append((rid, kind))
     th.join()
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat_ok = [l for ok, l in done if ok]
+    lat_ok = [l for ok, l, _ in done if ok]
+    lat_kind = {k: [l for ok, l, kk in done if ok and kk == k] for k in ("path_tracer", "restir")}
     return {
-        "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
-        "value": st["samples"] / wall / 1e6 if wall > 0 else 0.0,
-        "unit": "Msamples/s", "n_gpus": 1, "steps": int(st["steps"]), "warmup": 0,
-        "ms_per_step": st["busy_ms"] / max(st["steps"], 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes, seeded arrivals)",
-        "config": {"workload": f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s, "
-                               "resolution uniform over 256^2/512^2/1280x720/1920x1080, spp over 1/4/16/64, "
-                               "8 resident scenes of 10k-2M triangles, path depth 5",
-                   "hbm_budget_gb": args.serve_budget_gb, "max_active": args.serve_active,
-                   "queue_deadline_ms": args.serve_deadline_ms, "parallelism": "single GPU"},
-        "requests": n, "completed": int(st["completed"]), "rejected": int(st["rejected"]),
+        "rate_per_s": rate, "seconds": seconds, "requests": n, "completed": int(st["completed"]),
+        "rejected": int(st["rejected"]), "msamples_per_s": st["samples"] / wall / 1e6 if wall > 0 else 0.0,
         "offered_msamples_per_s": offered / wall / 1e6,
         "p50_latency_ms": float(np.percentile(lat_ok, 50)) if lat_ok else None,
         "p99_latency_ms": float(np.percentile(lat_ok, 99)) if lat_ok else None,
-        "p50_step_ms": st["step_p50_ms"], "p99_step_ms": st["step_p99_ms"],
+        "p99_latency_ms_by_kind": {k: (float(np.percentile(v, 99)) if v else None) for k, v in lat_kind.items()},
+        "p50_step_ms": st["step_p50_ms"], "p99_step_ms": st["step_p99_ms"], "steps": int(st["steps"]),
         "gpu_busy_frac": st["busy_ms"] / (wall * 1e3), "max_concurrent": int(st["max_concurrent"]),
         "hbm_peak_gb": st["hbm_peak"] / 2 ** 30, "wall_s": wall,
+    }
+
+
+def serve_workload(args, ctx, rates=None, seconds=None):
+    """Render-request serving (SERVE_SCENES resident): one run at
+    --serve-rate requests/s (the line's value) and a latency-vs-load sweep
+    over SERVE_RATES."""
+    resident = serve_scenes(ctx)
+    seconds = args.serve_seconds if seconds is None else seconds
+    rates = SERVE_RATES if rates is None else rates
+    sweep = []
+    for r in rates:
+        log(f"[serve] {r:.0f} requests/s for {seconds} s")
+        sweep.append(serve_run(args, ctx, resident, r, seconds))
+    head = next((x for x in sweep if x["rate_per_s"] == args.serve_rate), sweep[-1])
+    for sc, _ in resident:
+        sc.free()
+    return {
+        "metric": "Msamples/s + MRays/s at 1080p; p99 frame latency; all at equal relMSE",
+        "value": head["msamples_per_s"], "unit": "Msamples/s", "n_gpus": 1, "steps": head["steps"], "warmup": 0,
+        "ms_per_step": head["p50_step_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes, seeded arrivals)",
+        "config": {"workload": f"serving: Poisson arrivals {head['rate_per_s']}/s for {seconds} s, resolution "
+                               "uniform over 256^2/512^2/1280x720/1920x1080, spp over 1/4/16/64, half path "
+                               "tracer (depth 5), half ReSTIR DI (depth 2, reservoirs resident), 8 resident scenes "
+                               "of 10k-2M triangles",
+                   "hbm_budget_gb": args.serve_budget_gb, "max_active": args.serve_active,
+                   "queue_deadline_ms": args.serve_deadline_ms, "parallelism": "single GPU"},
+        "p50_latency_ms": head["p50_latency_ms"], "p99_latency_ms": head["p99_latency_ms"],
+        "sweep": sweep,
     }
 
 
@@ -861,6 +898,13 @@ def main():
         except Exception as e:
             log(f"[bench] mlp failed: {e!r}")
             extra["mlp"] = {"error": repr(e)}
+        try:
+            extra["serve"] = {k: v for k, v in serve_workload(args, ctx, rates=[10.0, 50.0, 200.0],
+                                                              seconds=3.0).items()
+                              if k not in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline")}
+        except Exception as e:
+            log(f"[bench] serve failed: {e!r}")
+            extra["serve"] = {"error": repr(e)}
         line["workloads"] = extra
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
