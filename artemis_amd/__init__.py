@@ -310,6 +310,14 @@ class ReSTIRGI(_Attachment):
     kind = "restir_gi"
 
 
+def write_png(path, rgba: np.ndarray) -> None:
+    """(H, W) uint32 packed RGBA8 pixels (hdr_to_ldr's output) -> PNG file,
+    rows in memory order."""
+    rgba = np.ascontiguousarray(rgba, np.uint32)
+    h, w = rgba.shape
+    L.check(L.lib().krt_write_png(str(path).encode(), rgba.ctypes.data_as(C.POINTER(C.c_uint32)), w, h))
+
+
 def load_attachment(ctx: Context, kind: str):
     """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
     return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI}[kind](ctx)
@@ -583,6 +591,11 @@ class ArmadaRTX:
         out = np.empty((self.height, self.width), np.uint32)
         L.check(self.ctx._lib.krt_armada_read_ldr(self.h, tonemapping, out.ctypes.data_as(C.POINTER(C.c_uint32))))
         return out
+
+    def write_png(self, path, tonemapping: int = 0) -> None:
+        """The colour buffer through hdr_to_ldr into a PNG file (the
+        reference's still-image capture, bench.cu:226-233)."""
+        L.check(self.ctx._lib.krt_armada_write_png(self.h, tonemapping, str(path).encode()))
 
     def upload_buffer(self, which: int, data: np.ndarray) -> None:
         data = np.ascontiguousarray(data, np.float32)

@@ -136,6 +136,8 @@ _SIGS = {
     "krt_trace_occluded": (C.c_int, [vp, handle, fp, C.c_uint32, i32p]),
     "krt_trace_closest_device": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
     "krt_trace_closest_device_noalloc": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
+    "krt_write_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]),
+    "krt_armada_write_png": (C.c_int, [vp, C.c_int, C.c_char_p]),
     "krt_camera_rays": (C.c_int, [vp, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
                                   fp]),
     "krt_copy_probe": (C.c_int, [vp, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),

@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libkrt.so"
 BUILD = HERE / "_build"
 
-SOURCES = ["krt_bvh.hip", "krt_trace.hip", "krt_pt.hip", "krt_restir.hip", "krt_gi.hip", "krt_mlp.hip", "krt_cache.hip", "krt_serve.hip", "krt_color.hip", "krt_capi.cpp"]
+SOURCES = ["krt_bvh.hip", "krt_trace.hip", "krt_pt.hip", "krt_restir.hip", "krt_gi.hip", "krt_mlp.hip", "krt_cache.hip", "krt_serve.hip", "krt_color.hip", "krt_image.cpp", "krt_capi.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -56,7 +56,7 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -933,6 +933,23 @@ int krt_armada_read_ldr(krt_armada *a, int tonemapping, uint32_t *out) {
     });
 }
 
+int krt_write_png(const char *path, const uint32_t *rgba, uint32_t width, uint32_t height) {
+    return guard([&] {
+        KRT_REQUIRE(path && rgba, KRT_ERR_INVALID, "null argument");
+        write_png(path, rgba, width, height);
+    });
+}
+
+int krt_armada_write_png(krt_armada *a, int tonemapping, const char *path) {
+    return guard([&] {
+        KRT_REQUIRE(a && path, KRT_ERR_INVALID, "null argument");
+        std::vector<uint32_t> px((size_t)a->w * a->h);
+        const int rc = krt_armada_read_ldr(a, tonemapping, px.data());
+        if (rc != KRT_OK) throw Error(rc, krt_last_error());
+        write_png(path, px.data(), a->w, a->h);
+    });
+}
+
 int krt_armada_stats(krt_armada *a, krt_frame_stats *out) {
     return guard([&] {
         KRT_REQUIRE(a && out, KRT_ERR_INVALID, "null argument");

@@ -172,6 +172,9 @@ struct Context {
     }
 };
 
+// PNG file of packed RGBA8 pixels (krt_image.cpp)
+void write_png(const std::string &path, const uint32_t *rgba, uint32_t width, uint32_t height);
+
 // Builders / kernels (defined in the .hip units)
 void accel_build(Context &ctx, Scene &s, int leaf_max, int layout);
 void upload_scene_derived(Context &ctx, Scene &s, const float *mats12, uint32_t nm);

@@ -285,6 +285,13 @@ int krt_hdr_to_ldr(krt_context *ctx, const void *pixels_dev, void *target_dev, u
 /* The armada's colour buffer through krt_hdr_to_ldr into a host array of
  * W*H uint32; returns when the copy is complete. */
 int krt_armada_read_ldr(krt_armada *a, int tonemapping, uint32_t *out);
+/* Image output (the still-image capture of bench.cu:226-233, stbi_write_png
+ * of the hdr_to_ldr frame: RGBA8, rows in memory order): a PNG file of
+ * width x height packed pixels, and of the armada's colour buffer through
+ * krt_hdr_to_ldr.  Video capture (include/capture.hpp, ffmpeg) is not
+ * provided. */
+int krt_write_png(const char *path, const uint32_t *rgba, uint32_t width, uint32_t height);
+int krt_armada_write_png(krt_armada *a, int tonemapping, const char *path);
 
 /* Plugin entry of source/amadeus/naive_path_tracer.cu:280-344
  * (`extern "C" ret load_attachment()`), by kind. */

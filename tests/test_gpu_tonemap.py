@@ -70,3 +70,20 @@ def test_armada_ldr_buffer(krt_ctx):
     img = arm.ldr_buffer(0)
     assert (img >> 24 == 255).all() and np.any(img & 255)
     sc.free()
+
+
+def test_armada_png_file(krt_ctx, tmp_path):
+    """krt_armada_write_png: the colour buffer through hdr_to_ldr into a PNG
+    file whose pixels are the LDR buffer's bytes."""
+    from .test_png_output import read_png
+
+    s = scenes.cornell()
+    sc = krt_ctx.upload(s)
+    arm = ArmadaRTX(krt_ctx, 40, 24)
+    arm.attach("pt", PathTracer(krt_ctx))
+    arm.render(sc, look_at(**s.camera, aspect=40 / 24))
+    path = tmp_path / "frame.png"
+    arm.write_png(path, 0)
+    ldr = arm.ldr_buffer(0)
+    np.testing.assert_array_equal(read_png(path), ldr.view(np.uint8).reshape(24, 40, 4))
+    sc.free()
