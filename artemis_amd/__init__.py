@@ -17,7 +17,7 @@ from . import _lib as L
 from .camera import CameraFrame, look_at
 from .scenes import SceneData
 
-__all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "CameraFrame", "look_at", "SceneData",
+__all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "Mamba", "CameraFrame", "look_at", "SceneData",
            "load_attachment", "RadianceField", "Server", "DeviceArray"]
 
 
@@ -310,6 +310,15 @@ class ReSTIRGI(_Attachment):
     kind = "restir_gi"
 
 
+class Mamba(_Attachment):
+    """The editor's G-buffer ReSTIR DI (editor/optix/mamba_shader.cu:35-505):
+    8 light candidates, temporal reuse (M-cap 200), spatial reuse of 2
+    neighbours in 20 px; options temporal, spatial (bool).  The colour buffer
+    holds the accumulated direct lighting; reservoirs() gives 16 floats per
+    pixel {light position, normal, emission, area, w, W, M, seed}."""
+    kind = "mamba"
+
+
 def write_png(path, rgba: np.ndarray) -> None:
     """(H, W) uint32 packed RGBA8 pixels (hdr_to_ldr's output) -> PNG file,
     rows in memory order."""
@@ -320,7 +329,7 @@ def write_png(path, rgba: np.ndarray) -> None:
 
 def load_attachment(ctx: Context, kind: str):
     """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
-    return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI}[kind](ctx)
+    return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI, "mamba": Mamba}[kind](ctx)
 
 
 class RadianceField:

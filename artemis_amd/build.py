@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libkrt.so"
 BUILD = HERE / "_build"
 
-SOURCES = ["krt_bvh.hip", "krt_trace.hip", "krt_pt.hip", "krt_restir.hip", "krt_gi.hip", "krt_mlp.hip", "krt_cache.hip", "krt_serve.hip", "krt_color.hip", "krt_image.cpp", "krt_capi.cpp"]
+SOURCES = ["krt_bvh.hip", "krt_trace.hip", "krt_pt.hip", "krt_restir.hip", "krt_gi.hip", "krt_mamba.hip", "krt_mlp.hip", "krt_cache.hip", "krt_serve.hip", "krt_color.hip", "krt_image.cpp", "krt_capi.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 
@@ -53,6 +54,8 @@ struct krt_attachment {
     RestirOptions ropt;
     RestirState rst;
     GiState gst;
+    MambaOptions mopt;
+    MambaState mst;
     CacheOptions copt;
     CacheState cst;
     std::unique_ptr<krt_mlp> cache_handle;  // borrowed view of cst.net for krt_attachment_cache
@@ -63,11 +66,13 @@ struct krt_attachment {
         h = height;
         if (kind == "restir") restir_alloc(*ctx, rst, width, height, user_seed);
         if (kind == "restir_gi") gi_alloc(*ctx, gst, width, height, user_seed);
+        if (kind == "mamba") mamba_alloc(*ctx, mst, width, height, user_seed);
         loaded = true;
     }
     void unload() {  // restir.cuh:213-220
         rst = RestirState();
         gst = GiState();
+        mst = MambaState();
         cache_handle.reset();
         cst = CacheState();
         loaded = false;
@@ -87,6 +92,11 @@ struct krt_attachment {
             return true;
         }
         if (kind == "path_tracer" && f == "pixel_jitter") return as_int(jitter);
+        if (kind == "mamba") {  // mamba.cu temporal_reuse / spatial_reuse
+            if (f == "temporal") return as_int(mopt.temporal);
+            if (f == "spatial") return as_int(mopt.spatial);
+            return true;
+        }
         if (kind == "path_tracer" || kind == "restir") {
             if (f == "radiance_cache") return as_int(copt.enabled);
             if (f == "cache_depth") return as_int(copt.depth);
@@ -114,6 +124,9 @@ struct krt_attachment {
         o.kind = KRT_OPT_INT;
         if (f == "seed") o.i = (int)user_seed;
         else if (kind == "path_tracer" && f == "pixel_jitter") o.i = jitter;
+        else if (kind == "mamba" && f == "temporal") o.kind = KRT_OPT_BOOL, o.b = mopt.temporal;
+        else if (kind == "mamba" && f == "spatial") o.kind = KRT_OPT_BOOL, o.b = mopt.spatial;
+        else if (kind == "mamba") o.kind = KRT_OPT_NONE;
         else if ((kind == "path_tracer" || kind == "restir") && f == "radiance_cache") o.i = copt.enabled;
         else if ((kind == "path_tracer" || kind == "restir") && f == "cache_depth") o.i = copt.depth;
         else if ((kind == "path_tracer" || kind == "restir") && f == "train_samples") o.i = copt.train_samples;
@@ -283,6 +296,44 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scen
         }
         s->nl = (uint32_t)(lt.size() / 16);
         upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
+        // area lights of the editor's ReSTIR (one per emissive material, its
+        // triangles in triangle order)
+        {
+            std::map<uint32_t, uint32_t> of_mat;
+            std::vector<int32_t> count;
+            std::vector<float> em;
+            for (uint32_t t = 0; t < d->nt; t++) {
+                const uint32_t m = d->tri_material[t];
+                const float *e = d->materials + 12 * m + 6;
+                if (!emissive(e)) continue;
+                auto it = of_mat.find(m);
+                if (it == of_mat.end()) {
+                    it = of_mat.emplace(m, (uint32_t)count.size()).first;
+                    count.push_back(0);
+                    em.insert(em.end(), {e[0], e[1], e[2], 0.0f});
+                }
+                count[it->second]++;
+            }
+            std::vector<int32_t> range(2 * count.size()), fill(count.size(), 0), tris;
+            int32_t run = 0;
+            for (size_t l = 0; l < count.size(); l++) {
+                range[2 * l] = run;
+                range[2 * l + 1] = count[l];
+                run += count[l];
+            }
+            tris.assign((size_t)run, 0);
+            for (uint32_t t = 0; t < d->nt; t++) {
+                auto it = of_mat.find(d->tri_material[t]);
+                if (it == of_mat.end()) continue;
+                tris[(size_t)range[2 * it->second] + fill[it->second]++] = (int32_t)t;
+            }
+            s->nml = (uint32_t)count.size();
+            if (s->nml) {
+                upload(C, s->ml_range, range.data(), sizeof(int32_t) * range.size());
+                upload(C, s->ml_tris, tris.data(), sizeof(int32_t) * tris.size());
+                upload(C, s->ml_emission, em.data(), sizeof(float) * em.size());
+            }
+        }
         KRT_HIP(hipStreamSynchronize(C.stream));
         krt_handle h = C.next_handle++;
         C.scenes[h] = std::move(s);
@@ -676,7 +727,7 @@ int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **o
     return guard([&] {
         KRT_REQUIRE(ctx && kind && out, KRT_ERR_INVALID, "null argument");
         std::string k(kind);
-        KRT_REQUIRE(k == "path_tracer" || k == "restir" || k == "restir_gi", KRT_ERR_INVALID,
+        KRT_REQUIRE(k == "path_tracer" || k == "restir" || k == "restir_gi" || k == "mamba", KRT_ERR_INVALID,
                     "unknown attachment kind: " + k);
         auto *a = new krt_attachment();
         a->ctx = &ctx->impl;
@@ -792,6 +843,8 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
                       a->f64 ? a->acc64.as<double>() : nullptr);
             if (a->f64) a->f64_frames++;
         }
+        else if (att->kind == "mamba")
+            mamba_render(C, s, li, att->mopt, att->mst, times);
         else if (att->kind == "restir")
             restir_render(C, s, li, att->ropt, att->rst, a->ctr.as<unsigned long long>(), times, &att->copt,
                           &att->cst);
@@ -868,6 +921,7 @@ int krt_armada_read_reservoirs(krt_armada *a, int which, float *out) {
         krt_attachment *att = a->attachments.at(a->previous);
         KRT_REQUIRE(att->kind != "path_tracer" && att->loaded, KRT_ERR_STATE, "active attachment has no reservoirs");
         if (att->kind == "restir") restir_read(*a->ctx, att->rst, which, out);
+        else if (att->kind == "mamba") mamba_read(*a->ctx, att->mst, which, out);
         else gi_read(*a->ctx, att->gst, which, out);
     });
 }

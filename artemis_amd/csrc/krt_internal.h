@@ -99,6 +99,11 @@ struct Scene {
     DevBuf tri_mat;    // uint per triangle
     DevBuf mats;       // float4 x 4 per material
     DevBuf lights;     // float4 x 4 per light
+    // the editor's area lights (Mamba): emissive triangles grouped by
+    // material in order of their first triangle; per light {first, count}
+    // into ml_tris, and its emission
+    uint32_t nml = 0;
+    DevBuf ml_range, ml_tris, ml_emission;
     Accel accel;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // vertex bounding box
     uint64_t version = 0;  // bumped on every rebuild (resets accumulation)
@@ -231,6 +236,23 @@ struct RestirState {
     float prev_pv[16] = {0};
 };
 
+// The editor's G-buffer ReSTIR DI (krt_mamba.hip)
+struct MambaOptions {
+    int temporal = 1;  // mamba.cu temporal_reuse
+    int spatial = 1;   // mamba.cu spatial_reuse
+};
+
+struct MambaState {
+    uint32_t w = 0, h = 0, user_seed = 0, frame = 0;
+    int samples = 0;
+    DevBuf R, S, P[2];  // 64 B reservoir records: current, spatial output, previous (double-buffered)
+    int pin = 0;
+    DevBuf gx, gn, gpm;  // ray-cast G-buffer
+    float prev_pv[16] = {0};
+    float prev_cam[12] = {0};
+    bool have_prev = false;
+};
+
 struct GiState {
     uint32_t w = 0, h = 0;
     DevBuf res[2];  // 5 float4 planes per buffer
@@ -261,6 +283,8 @@ void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *c
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
 void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed);
+void mamba_alloc(Context &ctx, MambaState &st, uint32_t w, uint32_t h, uint32_t user_seed);
+void mamba_read(Context &ctx, MambaState &st, int which, float *host_out);
 void gi_read(Context &ctx, GiState &st, int which, float *host_out);
 void gi_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, GiState &st,
                unsigned long long *ctr, PassTimes &times);
@@ -347,6 +371,9 @@ struct CacheState {
 void cache_pt_render(Context &ctx, Scene &s, const LaunchInfo &li, const CacheOptions &opt, CacheState &cs,
                      unsigned long long *ctr, PassTimes &times);
 // ReSTIR DI; with a cache (copt->enabled) its indirect path stops at the cache vertex.
+void mamba_render(Context &ctx, Scene &s, const LaunchInfo &li, const MambaOptions &opt, MambaState &st,
+                  PassTimes &times);
+
 // sync = false: everything is queued on ctx.stream and the call returns
 // without waiting (no pass times); the radiance cache needs sync = true.
 void restir_render(Context &ctx, Scene &s, const LaunchInfo &li, const RestirOptions &opt, RestirState &st,

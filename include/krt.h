@@ -231,7 +231,8 @@ typedef struct {
 /* ArmadaRTX(context, accelerator, mesh daemon, extent) (armada.cu:28-83) */
 int krt_armada_create(krt_context *ctx, uint32_t width, uint32_t height, krt_armada **out);
 void krt_armada_destroy(krt_armada *a);
-/* Attachment factory: kind "path_tracer" | "restir" | "restir_gi".
+/* Attachment factory: kind "path_tracer" | "restir" | "restir_gi" |
+ * "mamba" (the editor's G-buffer ReSTIR DI, editor/optix/mamba_shader.cu).
  * Lifetimes: destroy armadas before the attachments attached to them, and
  * everything created on a context before krt_destroy. */
 int krt_attachment_create(krt_context *ctx, const char *kind, krt_attachment **out);
@@ -264,7 +265,9 @@ int krt_armada_set_f64_accumulation(krt_armada *a, int enable);
  * pixel {Le.xyz, normal.xyz, point.xyz, type, w, W, M, seed.xyz}; for
  * "restir_gi" 20 floats {x_s, n_s, L_o, x_owner, w, W, M, seed.xyz, 0, 0}.
  * which 0: the buffer the next frame's temporal pass merges with (last
- * spatial output), 1: the other one. */
+ * spatial output), 1: the other one.  For "mamba" 16 floats {light position,
+ * normal, emission, area, w, W, M, seed}; which 0: the spatial output of the
+ * last frame, 1: the temporal output the next frame reads. */
 int krt_armada_read_reservoirs(krt_armada *a, int which, float *out);
 int krt_armada_read_f64(krt_armada *a, double *out, uint64_t *frames);
 int krt_armada_read_f64_squares(krt_armada *a, double *out);

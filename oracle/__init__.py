@@ -77,6 +77,11 @@ _SIGS = {
     "orc_mlp_grad": (C.c_double, [dp, fp, fp, C.c_int, fp, fp, C.c_int, C.c_float, dp]),
     "orc_mlp_adam": (None, [fp, fp, fp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp]),
     "orc_hdr_to_ldr": (None, [fp, u32p, C.c_int, C.c_int, C.c_int]),
+    "orc_mamba_create": (vp, [vp, C.c_int, C.c_int, C.c_uint32]),
+    "orc_mamba_destroy": (None, [vp]),
+    "orc_mamba_light_count": (C.c_int, [vp]),
+    "orc_mamba_frame": (None, [vp, vp, C.POINTER(OrcCamera), C.c_int, C.c_int, C.c_int, fp, C.c_int]),
+    "orc_mamba_reservoirs": (None, [vp, C.c_int, fp]),
     "orc_scene_create_instanced": (vp, [C.c_int, i32p, i32p, fp, fp, u32p, C.c_int, i32p, fp, u32p, fp, C.c_int]),
     "orc_scene_tlas": (C.c_int, [vp, fp]),
     "orc_affine_inverse": (None, [fp, fp]),
@@ -369,6 +374,37 @@ class OracleGI:
     def reservoirs(self, which=0) -> np.ndarray:
         out = np.empty((self.h_, self.w, 20), np.float32)
         lib().orc_gi_reservoirs(self.h, which, _f(out))
+        return out
+
+
+class OracleMamba:
+    """The editor's G-buffer ReSTIR DI (editor/optix/mamba_shader.cu), our restatement."""
+
+    def __init__(self, scene: OracleScene, width, height, user_seed=0):
+        self.scene = scene
+        self.w, self.h_ = width, height
+        self.h = lib().orc_mamba_create(scene.h, width, height, user_seed)
+        self.color = np.zeros((height, width, 4), np.float32)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_mamba_destroy(self.h)
+        except Exception:
+            pass
+
+    def light_count(self) -> int:
+        return int(lib().orc_mamba_light_count(self.h))
+
+    def frame(self, frame, temporal=True, spatial=True, accumulate=True, nthreads=8):
+        cam = camera_struct(frame)
+        lib().orc_mamba_frame(self.h, self.scene.h, C.byref(cam), int(temporal), int(spatial), int(accumulate),
+                              _f(self.color), nthreads)
+        return self.color
+
+    def reservoirs(self, which=0) -> np.ndarray:
+        out = np.empty((self.h_, self.w, 16), np.float32)
+        lib().orc_mamba_reservoirs(self.h, which, _f(out))
         return out
 
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -1975,4 +1976,441 @@ void orc_hdr_to_ldr(const float *pixels, uint32_t *target, int width, int height
             for (int k = 0; k < 3; k++) ch[k] = ldr_quantize(tonemapping == 0 ? ldr_aces(p[k]) : ldr_srgb(p[k]));
             target[(size_t)(height - y - 1) * width + x] = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (255u << 24);
         }
+}
+
+// ---------------------------------------------------------------------------
+// Mamba: the editor's G-buffer ReSTIR DI (editor/optix/mamba_shader.cu:35-505,
+// host sequence editor/mamba.cu:1055-1150).  A second ReSTIR semantics beside
+// amadeus_restir.cu: 8 light candidates per pixel drawn by sample_light
+// (:99-162: an area light uniformly, then one of its triangles uniformly, pdf
+// 1 / (triangle area x lights), the triangle choice left out as in the
+// reference), visibility in the initial resampling target, temporal reuse
+// with the previous reservoir capped at M = 200, spatial reuse of 2
+// neighbours in a 20 px disk with the 1/Z correction, shading ray epsilon
+// 1e-3 (:38-63).  Deviations, each deterministic: the raster G-buffer is a
+// ray-cast one (pixel-centre camera rays in the armada's orientation); seeds
+// (x, y, time) use time = frame number + 4096 x user seed instead of the wall
+// clock; reservoir seeds start from reservoir_seed(); the spatial pass reads
+// the reservoirs as the temporal pass left them and writes a second buffer
+// (the reference overwrites them in place while neighbours read them), and
+// the previous-frame reservoirs are double-buffered for the same reason; the
+// in-place M cap of a reprojected reservoir is applied to the reader's copy
+// only (every reader caps it to the same value); reprojection indices are
+// clamped to the image.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct MLight {  // LightInfo (mamba_shader.cuh:19-25)
+    f3 position{0, 0, 0}, normal{0, 0, 0}, emission{0, 0, 0};
+    float area = 0.0f;
+    bool sky = false;
+};
+
+struct MRes {  // Reservoir<LightInfo> (include/amadeus/reservoir.cuh:19-76)
+    MLight data;
+    float w = 0.0f, W = 0.0f;
+    int M = 0;
+    f3 seed{0, 0, 0};
+    bool update(const MLight &s, float weight) {
+        w += weight;
+        float eta = rand_uniform(seed);
+        bool selected = (eta * w) < weight;
+        if (selected) data = s;
+        M++;
+        return selected;
+    }
+    void resample(float target) {
+        float d = target * (float)M;
+        W = (d > 0.0f) ? w / d : 0.0f;
+    }
+    void reset() {
+        w = 0.0f;
+        M = 0;
+        W = 0.0f;
+    }
+};
+
+struct MGb {  // G-buffer texel: position, normal (w: entering), triangle, material (-1: no surface)
+    f3 x{0, 0, 0}, n{0, 0, 0};
+    float nw = 0.0f;
+    int prim = -1, mat = -1;
+};
+
+const float kMambaEps = 1e-3f;   // mamba_shader.cu:38
+const int kMambaCandidates = 8;  // :248
+const int kMambaMCap = 200;      // :352, :457
+const int kMambaNeighbours = 2;  // :443
+const int kMambaRadius = 20;     // :449
+
+inline float mamba_lum(f3 v) { return v.x + v.y + v.z; }  // luminance (:32-37)
+inline f3 cleanse(f3 v) { return isnan3(v) ? mk(0, 0, 0) : v; }
+
+}  // namespace
+
+struct orc_mamba {
+    int w = 0, h = 0;
+    uint32_t user = 0, frame = 0;
+    int samples = 0;
+    std::vector<MRes> R, S, P[2];
+    int pin = 0;  // P[pin] is the previous frame's
+    std::vector<MGb> g;
+    std::vector<float> Le;  // 3 per pixel
+    // area lights: the emissive triangles grouped by material, in order of
+    // their first triangle (an entity submesh per material)
+    std::vector<int> first, count, tris;
+    std::vector<f3> emission;
+    float prev_pv[16];
+    float prev_cam[12];
+    bool have_prev = false;
+};
+
+namespace {
+
+float mamba_sample_light(const orc_mamba *m, const orc_scene *s, MLight &li, f3 &seed) {
+    const unsigned total = (unsigned)m->first.size();
+    if (total == 0) return -1.0f;
+    unsigned light_index = (unsigned)(rand_uniform(seed) * (float)total);
+    light_index = light_index < total ? light_index : total - 1;  // rand * total may round up to total
+    const unsigned cnt = (unsigned)m->count[light_index];
+    unsigned ti = (unsigned)(rand_uniform(seed) * (float)cnt);
+    ti = ti < cnt ? ti : cnt - 1;
+    f3 v0, v1, v2;
+    tri_verts(s, m->tris[m->first[light_index] + ti], v0, v1, v2);
+    const f3 gnormal = normalize(cross(v1 - v0, v2 - v0));
+    const f3 bary = pcg3f(seed);
+    float u = bary.x, v = bary.y;
+    if (u + v > 1.0f) {
+        u = 1.0f - u;
+        v = 1.0f - v;
+    }
+    li.sky = false;
+    li.position = (v0 * (1.0f - u - v) + v1 * u) + v2 * v;
+    li.normal = gnormal;
+    li.emission = m->emission[light_index];
+    li.area = 0.5f * length(cross(v1 - v0, v2 - v0));
+    return 1.0f / (li.area * (float)total);
+}
+
+f3 mamba_direct(const SurfaceHit &sh, const MLight &li) {  // direct_unoccluded (:70-83)
+    const f3 wi = normalize(li.position - sh.x);
+    const float R = length(li.position - sh.x);
+    const f3 rho = cleanse(brdf(sh, wi, sh.mat.type));
+    const float ldotn = li.sky ? 1.0f : fabsf(dot(li.normal, wi));
+    const float ndotwi = fabsf(dot(sh.n, wi));
+    const float falloff = li.sky ? 1.0f : 1.0f / (R * R);
+    return (((rho * li.emission) * ldotn) * ndotwi) * falloff;
+}
+
+bool mamba_occluded(Ctx &cx, f3 origin, f3 target) {  // occluded (:38-63)
+    cx.c.shadow++;
+    const f3 d = normalize(target - origin);
+    return occluded(cx.s, origin, d, length(target - origin) - kMambaEps, cx.c);
+}
+
+float mamba_target(const SurfaceHit &sh, const MRes &r) { return mamba_lum(mamba_direct(sh, r.data)); }
+
+float mamba_occluded_target(Ctx &cx, const SurfaceHit &sh, const MRes &r) {
+    const f3 Li = mamba_direct(sh, r.data);
+    return mamba_lum(Li * (1.0f - (mamba_occluded(cx, sh.x, r.data.position) ? 1.0f : 0.0f)));
+}
+
+// load_surface_hit (:165-216) from the ray-cast G-buffer
+bool mamba_surface(const orc_mamba *m, const orc_scene *s, const orc_camera *cam, int index, SurfaceHit &sh) {
+    const MGb &t = m->g[index];
+    if (t.mat < 0) return false;
+    const f3 origin = load3(cam->center);
+    f3 n = t.n;
+    if (dot(t.x - origin, n) > 0.0f) n = -n;
+    sh.x = t.x;
+    sh.wo = normalize(origin - t.x);
+    sh.n = normalize(n);
+    sh.entering = t.nw > 0.0f;
+    sh.mat = s->mats[t.mat];
+    const float sign = (sh.mat.type == eTransmission) ? -1.0f : 1.0f;
+    sh.x = sh.x + (sign * sh.n) * kMambaEps;
+    return true;
+}
+
+int mamba_reproject(const orc_mamba *m, bool dirty, int index, f3 pos) {  // reproject (:301-330)
+    if (!dirty) return index;
+    const float *M = m->prev_pv;
+    float v[4];
+    for (int k = 0; k < 4; k++) v[k] = (M[0 + k] * pos.x + M[4 + k] * pos.y) + (M[8 + k] * pos.z + M[12 + k] * 1.0f);
+    float u = v[0] / v[3], vv = v[1] / v[3];
+    if (!(u >= -1.0f && u <= 1.0f && vv >= -1.0f && vv <= 1.0f)) return -1;
+    u = (u + 1.0f) * 0.5f;
+    vv = (vv + 1.0f) * 0.5f;
+    int ix = (int)((double)(u * (float)m->w) + 0.5);
+    int iy = (int)((double)(vv * (float)m->h) + 0.5);
+    ix = std::min(std::max(ix, 0), m->w - 1);
+    iy = std::min(std::max(iy, 0), m->h - 1);
+    return iy * m->w + ix;
+}
+
+int mamba_neighbour(const orc_mamba *m, int index, f3 &seed) {  // sample_spatial_neighborhood (:411-432)
+    const int x = index % m->w, y = index / m->w;
+    const f3 off = rand_uniform_3f(seed);
+    float c, sn;
+    sincos_turn(off.x, c, sn);
+    const float rr = (float)kMambaRadius * sqrtf(off.y);
+    int x0 = (int)((float)x + rr * c);
+    int y0 = (int)((float)y + rr * sn);
+    x0 = std::min(std::max(x0, 0), m->w - 1);
+    y0 = std::min(std::max(y0, 0), m->h - 1);
+    return x0 + y0 * m->w;
+}
+
+void mamba_shade(orc_mamba *m, Ctx &cx, const SurfaceHit &sh, const MRes &r, int index) {
+    const f3 Ld = mamba_direct(sh, r.data);
+    const bool occ = mamba_occluded(cx, sh.x, r.data.position);
+    const f3 nd = cleanse((Ld * (1.0f - (occ ? 1.0f : 0.0f))) * r.W);
+    const float s = (float)m->samples;
+    const f3 prev = load3(&m->Le[3 * (size_t)index]);
+    const f3 v = ((prev * s + sh.mat.emission) + nd) / (s + 1.0f);
+    m->Le[3 * (size_t)index] = v.x;
+    m->Le[3 * (size_t)index + 1] = v.y;
+    m->Le[3 * (size_t)index + 2] = v.z;
+}
+
+}  // namespace
+
+orc_mamba *orc_mamba_create(const orc_scene *s, int width, int height, uint32_t user_seed) {
+    auto *m = new orc_mamba();
+    m->w = width;
+    m->h = height;
+    m->user = user_seed;
+    const size_t n = (size_t)width * height;
+    m->R.resize(n);
+    m->S.resize(n);
+    m->P[0].resize(n);
+    m->P[1].resize(n);
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            const size_t i = (size_t)y * width + x;
+            const f3 sd = reservoir_seed((uint32_t)x, (uint32_t)y, user_seed);
+            m->R[i].seed = m->S[i].seed = m->P[0][i].seed = m->P[1][i].seed = sd;
+        }
+    m->g.resize(n);
+    m->Le.assign(3 * n, 0.0f);
+    std::map<uint32_t, int> of_mat;
+    for (size_t t = 0; t < s->tri_mat.size(); t++) {
+        const Material &mt = s->mats[s->tri_mat[t]];
+        if (!(length(mt.emission) > 0.0f)) continue;  // the armada's emissive test (armada.cu:240)
+        auto it = of_mat.find(s->tri_mat[t]);
+        if (it == of_mat.end()) {
+            of_mat[s->tri_mat[t]] = (int)m->emission.size();
+            m->emission.push_back(mt.emission);
+            m->count.push_back(0);
+        }
+        m->count[of_mat[s->tri_mat[t]]]++;
+    }
+    int run = 0;
+    for (int c : m->count) {
+        m->first.push_back(run);
+        run += c;
+    }
+    m->tris.assign(run, 0);
+    std::vector<int> fill(m->count.size(), 0);
+    for (size_t t = 0; t < s->tri_mat.size(); t++) {
+        auto it = of_mat.find(s->tri_mat[t]);
+        if (it == of_mat.end()) continue;
+        m->tris[m->first[it->second] + fill[it->second]++] = (int)t;
+    }
+    std::fill(m->prev_pv, m->prev_pv + 16, 0.0f);
+    std::fill(m->prev_cam, m->prev_cam + 12, 0.0f);
+    return m;
+}
+
+void orc_mamba_destroy(orc_mamba *m) { delete m; }
+
+int orc_mamba_light_count(const orc_mamba *m) { return (int)m->first.size(); }
+
+void orc_mamba_frame(orc_mamba *m, const orc_scene *s, const orc_camera *cam, int temporal, int spatial,
+                     int accumulate, float *color, int nthreads) {
+    const int W = m->w, H = m->h;
+    float camv[12];
+    std::memcpy(camv, cam->center, sizeof(float) * 3);
+    std::memcpy(camv + 3, cam->U, sizeof(float) * 3);
+    std::memcpy(camv + 6, cam->V, sizeof(float) * 3);
+    std::memcpy(camv + 9, cam->W, sizeof(float) * 3);
+    const bool dirty = m->have_prev && std::memcmp(camv, m->prev_cam, sizeof(camv)) != 0;
+    if (!accumulate || dirty) m->samples = 0;
+    const float time = (float)m->frame + 4096.0f * (float)m->user;
+    std::vector<Counters> per;
+    // G-buffer: the surface of the pixel-centre camera ray
+    parallel_rows(H, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        for (int x = 0; x < W; x++) {
+            const float dx = 2.0f * (((float)x + 0.5f) / (float)W) - 1.0f;
+            const float dy = 2.0f * (((float)y + 0.5f) / (float)H) - 1.0f;
+            const f3 o = load3(cam->center);
+            const f3 d = normalize((dx * load3(cam->U) + dy * load3(cam->V)) + load3(cam->W));
+            cx.c.closest++;
+            const Hit h = closest(s, o, d, kTraceTmax, 2, cx.c);
+            MGb &t = m->g[(size_t)y * W + x];
+            t = MGb();
+            Surface sf;
+            if (h.prim >= 0 && load_surface(s, h, d, sf)) {
+                t.x = sf.x;
+                t.n = sf.n;
+                t.nw = sf.entering ? 1.0f : -1.0f;
+                t.prim = h.prim;
+                t.mat = (int)s->tri_mat[h.prim];
+            }
+        }
+        c.add(cx.c);
+    });
+    const bool end_initial = !temporal && !spatial;
+    // initial pass (__raygen__direct_primary, :219-266)
+    parallel_rows(H, nthreads, per, [&](int y, Counters &c) {
+        Ctx cx{s, Counters()};
+        for (int x = 0; x < W; x++) {
+            const int index = y * W + x;
+            MRes &r = m->R[index];
+            r.reset();
+            SurfaceHit sh;
+            if (!mamba_surface(m, s, cam, index, sh)) continue;
+            f3 seed = mk((float)x, (float)y, time);
+            for (int i = 0; i < kMambaCandidates; i++) {
+                MLight li;
+                const float pdf = mamba_sample_light(m, s, li, seed);
+                const f3 Ld = mamba_direct(sh, li);
+                const float t = Ld.x + Ld.y + Ld.z;
+                r.update(li, t / pdf > 0.0f ? t / pdf : 0.0f);
+            }
+            const f3 Ld = mamba_direct(sh, r.data);
+            const bool occ = mamba_occluded(cx, sh.x, r.data.position);
+            r.resample(mamba_lum(Ld * (1.0f - (occ ? 1.0f : 0.0f))));
+            if (end_initial) mamba_shade(m, cx, sh, r, index);
+        }
+        c.add(cx.c);
+    });
+    std::vector<MRes> &Pin = m->P[m->pin], &Pout = m->P[1 - m->pin];
+    if (temporal) {  // __raygen__temporal_reuse (:333-394)
+        const bool end = !spatial;
+        parallel_rows(H, nthreads, per, [&](int y, Counters &c) {
+            Ctx cx{s, Counters()};
+            for (int x = 0; x < W; x++) {
+                const int index = y * W + x;
+                SurfaceHit sh;
+                if (!mamba_surface(m, s, cam, index, sh)) {
+                    if (end) {
+                        Pout[index] = Pin[index];
+                        Pout[index].reset();
+                    }
+                    continue;
+                }
+                const int pindex = mamba_reproject(m, dirty, index, sh.x);
+                MRes &cur = m->R[index];
+                if (cur.M <= 0) {
+                    if (end) Pout[index] = Pin[index];
+                    continue;
+                }
+                MRes merged;
+                merged.seed = 2.0f * mk((float)x, (float)y, time);
+                const float t_cur = mamba_occluded_target(cx, sh, cur);
+                merged.update(cur.data, (t_cur * cur.W) * (float)cur.M);
+                int N = cur.M;
+                if (pindex >= 0 && Pin[pindex].M > 0) {
+                    MRes prev = Pin[pindex];
+                    prev.M = std::min(prev.M, kMambaMCap);
+                    const float t_prev = mamba_occluded_target(cx, sh, prev);
+                    merged.update(prev.data, (t_prev * prev.W) * (float)prev.M);
+                    N += (t_prev > 0.0f ? 1 : 0) * prev.M;
+                }
+                merged.M = N;
+                merged.resample(mamba_target(sh, merged));
+                cur = merged;
+                if (end) {
+                    mamba_shade(m, cx, sh, cur, index);
+                    Pout[index] = cur;
+                }
+            }
+            c.add(cx.c);
+        });
+    }
+    if (spatial) {  // __raygen__spatial_reuse (:435-505)
+        parallel_rows(H, nthreads, per, [&](int y, Counters &c) {
+            Ctx cx{s, Counters()};
+            for (int x = 0; x < W; x++) {
+                const int index = y * W + x;
+                SurfaceHit sh;
+                if (!mamba_surface(m, s, cam, index, sh)) {
+                    Pout[index] = Pin[index];
+                    Pout[index].reset();
+                    m->S[index] = m->R[index];
+                    continue;
+                }
+                const MRes &cur = m->R[index];
+                if (cur.M <= 0) {  // the reference's debug colour
+                    m->Le[3 * (size_t)index] = 0.0f;
+                    m->Le[3 * (size_t)index + 1] = 0.0f;
+                    m->Le[3 * (size_t)index + 2] = 1.0f;
+                    Pout[index] = Pin[index];
+                    Pout[index].reset();
+                    m->S[index] = cur;
+                    continue;
+                }
+                MRes merged;
+                merged.seed = 2.0f * mk((float)x, (float)y, time);
+                const float t_cur = mamba_target(sh, cur);
+                merged.update(cur.data, (t_cur * cur.W) * (float)cur.M);
+                SurfaceHit hits[kMambaNeighbours];
+                int sizes[kMambaNeighbours];
+                int count = 0;
+                for (int i = 0; i < kMambaNeighbours; i++) {
+                    const int index0 = mamba_neighbour(m, index, merged.seed);
+                    MRes nb = m->R[index0];
+                    nb.M = std::min(nb.M, kMambaMCap);
+                    const float t_nb = mamba_occluded_target(cx, sh, nb);
+                    if (nb.M > 0) merged.update(nb.data, (t_nb * nb.W) * (float)nb.M);
+                    hits[i] = SurfaceHit();
+                    mamba_surface(m, s, cam, index0, hits[i]);
+                    hits[i].wo = sh.wo;
+                    hits[i].entering = nb.M > 0;
+                    sizes[i] = nb.M;
+                    count += nb.M;
+                }
+                merged.M = cur.M + count;
+                int Z = cur.M;
+                for (int i = 0; i < kMambaNeighbours; i++) {
+                    const float t = hits[i].entering ? mamba_occluded_target(cx, hits[i], merged) : 0.0f;
+                    if (t > 0.0f) Z += sizes[i];
+                }
+                const float den = mamba_target(sh, merged) * (float)Z;
+                merged.W = den > 0.0f ? merged.w / den : 0.0f;
+                m->S[index] = merged;
+                mamba_shade(m, cx, sh, merged, index);
+                Pout[index] = merged;
+            }
+            c.add(cx.c);
+        });
+        std::swap(m->R, m->S);
+    }
+    if (temporal || spatial) m->pin = 1 - m->pin;
+    for (size_t i = 0; i < (size_t)W * H; i++) {
+        color[4 * i] = m->Le[3 * i];
+        color[4 * i + 1] = m->Le[3 * i + 1];
+        color[4 * i + 2] = m->Le[3 * i + 2];
+        color[4 * i + 3] = 1.0f;
+    }
+    std::memcpy(m->prev_pv, cam->pv, sizeof(m->prev_pv));
+    std::memcpy(m->prev_cam, camv, sizeof(camv));
+    m->have_prev = true;
+    m->samples++;
+    m->frame++;
+}
+
+void orc_mamba_reservoirs(const orc_mamba *m, int which, float *out) {
+    const std::vector<MRes> &b = which == 0 ? m->R : m->P[m->pin];
+    for (size_t i = 0; i < b.size(); i++) {
+        const MRes &r = b[i];
+        float *o = out + 16 * i;
+        o[0] = r.data.position.x, o[1] = r.data.position.y, o[2] = r.data.position.z;
+        o[3] = r.data.normal.x, o[4] = r.data.normal.y, o[5] = r.data.normal.z;
+        o[6] = r.data.emission.x, o[7] = r.data.emission.y, o[8] = r.data.emission.z;
+        o[9] = r.data.area;
+        o[10] = r.w, o[11] = r.W, o[12] = (float)r.M;
+        o[13] = r.seed.x, o[14] = r.seed.y, o[15] = r.seed.z;
+    }
 }

@@ -176,6 +176,20 @@ void orc_mlp_adam(float *params, float *m, float *v, const float *grad, int n, f
 /* reconnection-shift Jacobian d(omega_r)/d(omega_q) (0 when rejected) */
 float orc_gi_jacobian(const float *xq, const float *xr, const float *xs, const float *ns);
 
+/* Mamba, the editor's G-buffer ReSTIR DI (editor/optix/mamba_shader.cu): one
+ * frame = ray-cast G-buffer, initial pass (8 candidates), temporal reuse if
+ * `temporal`, spatial reuse (2 neighbours, 20 px) if `spatial`; color:
+ * float4 per pixel, the accumulated direct lighting.  Reservoirs: 16 floats
+ * per pixel {position, normal, emission, area, w, W, M, seed}; which 0 = the
+ * current reservoirs, 1 = the previous frame's. */
+typedef struct orc_mamba orc_mamba;
+orc_mamba *orc_mamba_create(const orc_scene *s, int width, int height, uint32_t user_seed);
+void orc_mamba_destroy(orc_mamba *m);
+int orc_mamba_light_count(const orc_mamba *m);
+void orc_mamba_frame(orc_mamba *m, const orc_scene *s, const orc_camera *cam, int temporal, int spatial,
+                     int accumulate, float *color, int nthreads);
+void orc_mamba_reservoirs(const orc_mamba *m, int which, float *out);
+
 /* hdr_to_ldr (include/cuda/color.cuh:82-121): float4 pixels -> packed RGBA8,
  * rows flipped; tonemapping 0 = ACES (:35-41), 1 = sRGB (:14-31). */
 void orc_hdr_to_ldr(const float *pixels, uint32_t *target, int width, int height, int tonemapping);
