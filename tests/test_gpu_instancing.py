@@ -7,7 +7,7 @@ import pytest
 
 import oracle as O
 from artemis_amd import ArmadaRTX, PathTracer, ReSTIR, ReSTIRGI, look_at, scenes
-from artemis_amd._lib import KrtError
+from artemis_amd._lib import KRT_ERR_STATE, KrtError
 
 pytestmark = [
     pytest.mark.gpu,
@@ -121,10 +121,12 @@ def test_flat_only_paths_refuse_instanced_scenes(krt_ctx, inst_pair):
     cam = look_at(**data.camera)
     arm = ArmadaRTX(krt_ctx, 16, 16)
     arm.attach("gi", ReSTIRGI(krt_ctx))
-    with pytest.raises(KrtError):
+    with pytest.raises(KrtError) as e:
         arm.render(sc, cam)
+    assert e.value.code == KRT_ERR_STATE  # a refusal, not a device error
     arm2 = ArmadaRTX(krt_ctx, 16, 16)
     arm2.attach("pt", PathTracer(krt_ctx))
     arm2.set_option("radiance_cache", True)
-    with pytest.raises(KrtError):
+    with pytest.raises(KrtError) as e:
         arm2.render(sc, cam)
+    assert e.value.code == KRT_ERR_STATE
