@@ -573,6 +573,21 @@ class ArmadaRTX:
     def set_depth(self, depth: int) -> None:
         L.check(self.ctx._lib.krt_armada_set_depth(self.h, depth))
 
+    def set_envmap(self, rgba: np.ndarray | None) -> None:
+        """ArmadaRTX::set_envmap: an (H, W, 4) float32 environment map (row 0
+        is v = 0), or None to remove it."""
+        if rgba is None:
+            L.check(self.ctx._lib.krt_armada_set_envmap(self.h, None, 0, 0))
+            return
+        a = np.ascontiguousarray(rgba, np.float32)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("environment map must be (H, W, 4)")
+        self._env = a
+        L.check(self.ctx._lib.krt_armada_set_envmap(self.h, _f(a), a.shape[1], a.shape[0]))
+
+    def set_envmap_enabled(self, enabled: bool) -> None:
+        L.check(self.ctx._lib.krt_armada_set_envmap_enabled(self.h, int(bool(enabled))))
+
     def render(self, scene: Scene, camera: CameraFrame, accumulate: bool = True) -> None:
         cam = camera.to_ctypes()
         L.check(self.ctx._lib.krt_armada_render(self.h, scene.h, C.byref(cam), int(accumulate)))

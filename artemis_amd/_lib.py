@@ -157,6 +157,8 @@ _SIGS = {
     "krt_armada_set_option": (C.c_int, [vp, C.c_char_p, C.POINTER(Option)]),
     "krt_armada_get_option": (C.c_int, [vp, C.c_char_p, C.POINTER(Option)]),
     "krt_armada_set_depth": (C.c_int, [vp, C.c_int]),
+    "krt_armada_set_envmap": (C.c_int, [vp, fp, C.c_uint32, C.c_uint32]),
+    "krt_armada_set_envmap_enabled": (C.c_int, [vp, C.c_int]),
     "krt_armada_render": (C.c_int, [vp, handle, C.POINTER(Camera), C.c_int]),
     "krt_armada_buffers": (C.c_int, [vp, C.POINTER(Buffers)]),
     "krt_armada_readback": (C.c_int, [vp, C.c_int, fp]),

@@ -158,6 +158,9 @@ struct krt_armada {
     DevBuf acc64;  // FP64 sample sums (path tracer), enabled on request
     bool f64 = false;
     uint64_t f64_frames = 0;
+    DevBuf env;  // environment map texels (float4, row-major)
+    uint32_t env_w = 0, env_h = 0;
+    bool env_on = false;
 };
 
 struct krt_context {
@@ -783,6 +786,38 @@ int krt_armada_get_option(krt_armada *a, const char *field, krt_option *out) {
     });
 }
 
+int krt_armada_set_envmap(krt_armada *a, const float *rgba, uint32_t width, uint32_t height) {
+    return guard([&] {
+        KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipSetDevice(a->ctx->device));
+        if (!rgba || width == 0 || height == 0) {
+            KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+            a->env.release();
+            a->env_w = a->env_h = 0;
+            a->env_on = false;
+            return;
+        }
+        KRT_REQUIRE((uint64_t)width * height < (1ull << 28), KRT_ERR_INVALID, "environment map too large");
+        for (size_t i = 0; i < 4 * (size_t)width * height; i++)
+            KRT_REQUIRE(std::isfinite(rgba[i]), KRT_ERR_INVALID, "environment map has non-finite texels");
+        const size_t bytes = sizeof(float) * 4 * (size_t)width * height;
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));  // frames in flight may still read the old map
+        if (a->env.bytes < bytes) a->env.alloc(bytes);
+        KRT_HIP(hipMemcpyAsync(a->env.p, rgba, bytes, hipMemcpyHostToDevice, a->ctx->stream));
+        KRT_HIP(hipStreamSynchronize(a->ctx->stream));
+        a->env_w = width;
+        a->env_h = height;
+        a->env_on = true;
+    });
+}
+
+int krt_armada_set_envmap_enabled(krt_armada *a, int enabled) {
+    return guard([&] {
+        KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
+        a->env_on = enabled != 0;
+    });
+}
+
 int krt_armada_set_depth(krt_armada *a, int depth) {
     return guard([&] {
         KRT_REQUIRE(a, KRT_ERR_INVALID, "null argument");
@@ -834,6 +869,11 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         li.normal = a->normal.as<float4>();
         li.albedo = a->albedo.as<float4>();
         li.position = a->position.as<float4>();
+        if (a->env_on && a->env_w > 0) {  // armada.cuh:76-77 (has_environment_map)
+            li.env = a->env.as<float4>();
+            li.env_w = a->env_w;
+            li.env_h = a->env_h;
+        }
         KRT_HIP(hipMemsetAsync(a->ctr.p, 0, a->ctr.bytes, C.stream));
         PassTimes times;
         if (att->kind == "path_tracer" && att->copt.enabled) {

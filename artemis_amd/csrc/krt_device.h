@@ -344,6 +344,135 @@ KRT_DEV unsigned pick_light(f3 &seed, unsigned total) {
     return i < total ? i : total - 1;
 }
 
+// ---------------------------------------------------------------------------
+// Environment map (ArmadaRTX::set_envmap, source/amadeus/armada.cu:86-96; the
+// lookups of amadeus_path_tracer.cu:204-220, amadeus_restir.cu:566-577 and
+// Ld_Environment / sample_direct, amadeus_common.cuh:215-252, 341-370).
+// atan and asin are their Maclaurin series after argument reduction (the
+// same expressions in the oracle, so both give the same bits):
+//   atan t = t - t^3/3 + t^5/5 - ... for |t| <= tan(pi/8); above it
+//   atan t = pi/4 + atan((t - 1)/(t + 1)); above 1, atan t = pi/2 - atan(1/t);
+//   asin x = sum (2k)!/(4^k (k!)^2 (2k+1)) x^(2k+1) for |x| <= 1/2; above it
+//   asin x = pi/2 - 2 asin(sqrt((1 - x)/2)).
+// ---------------------------------------------------------------------------
+constexpr float kPiF = 0x1.921fb6p+1f;
+constexpr float kHalfPiF = 0x1.921fb6p+0f;
+constexpr float kQuarterPiF = 0x1.921fb6p-1f;
+constexpr float kTwoPiF = 0x1.921fb6p+2f;
+constexpr float kFourPiF = 0x1.921fb6p+3f;
+constexpr float kTanPi8F = 0x1.a8279ap-2f;
+constexpr float kEnvDistance = 10000.0f;  // WORLD_RADIUS, amadeus_common.cuh:219
+
+KRT_DEV float atan_series(float t) {  // |t| <= tan(pi/8)
+    const float t2 = t * t;
+    float p = 0x1.47ae14p-5f;  // 1/25
+    p = p * t2 + -0x1.642c86p-5f;
+    p = p * t2 + 0x1.861862p-5f;
+    p = p * t2 + -0x1.af286cp-5f;
+    p = p * t2 + 0x1.e1e1e2p-5f;
+    p = p * t2 + -0x1.111112p-4f;
+    p = p * t2 + 0x1.3b13b2p-4f;
+    p = p * t2 + -0x1.745d18p-4f;
+    p = p * t2 + 0x1.c71c72p-4f;
+    p = p * t2 + -0x1.24924ap-3f;
+    p = p * t2 + 0x1.99999ap-3f;
+    p = p * t2 + -0x1.555556p-2f;
+    return t + (t * t2) * p;
+}
+
+KRT_DEV float atan_unit(float t) {  // 0 <= t <= 1
+    return t > kTanPi8F ? kQuarterPiF + atan_series((t - 1.0f) / (t + 1.0f)) : atan_series(t);
+}
+
+KRT_DEV float krt_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    float r;
+    if (ax == 0.0f && ay == 0.0f) r = signbit(x) ? kPiF : 0.0f;
+    else {
+        r = ay <= ax ? atan_unit(ay / ax) : kHalfPiF - atan_unit(ax / ay);
+        if (signbit(x)) r = kPiF - r;
+    }
+    return signbit(y) ? -r : r;
+}
+
+KRT_DEV float asin_series(float x) {  // |x| <= 1/2
+    const float x2 = x * x;
+    float p = 0x1.df3bd4p-8f;  // 88179/12058624
+    p = p * x2 + 0x1.12ef3cp-7f;
+    p = p * x2 + 0x1.3fde50p-7f;
+    p = p * x2 + 0x1.7a8788p-7f;
+    p = p * x2 + 0x1.c9999ap-7f;
+    p = p * x2 + 0x1.1c4ec4p-6f;
+    p = p * x2 + 0x1.6e8ba2p-6f;
+    p = p * x2 + 0x1.f1c71cp-6f;
+    p = p * x2 + 0x1.6db6dcp-5f;
+    p = p * x2 + 0x1.333334p-4f;
+    p = p * x2 + 0x1.555556p-3f;
+    return x + (x * x2) * p;
+}
+
+KRT_DEV float krt_asin(float x) {  // x clamped to [-1, 1]
+    const float a = fminf(fabsf(x), 1.0f);
+    const float r = a <= 0.5f ? asin_series(a) : kHalfPiF - 2.0f * asin_series(sqrtf((1.0f - a) * 0.5f));
+    return signbit(x) ? -r : r;
+}
+
+struct EnvMap {  // RGBA float texels, row-major, width x height; tex == nullptr: none
+    const float4 *tex;
+    uint32_t w, h;
+};
+
+KRT_DEV int env_wrap(int i, int n) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// tex2D <float4> with normalised coordinates, bilinear filtering and wrap
+// addressing (the texture descriptors of include/cuda/interop.cuh:96-135),
+// filtered in FP32 (the texture unit's 8-bit weights are not restated).
+KRT_DEV f3 env_fetch(const EnvMap &env, float u, float v) {
+    if (!(u >= -1.0f && u <= 2.0f && v >= -1.0f && v <= 2.0f)) return mk(0.0f, 0.0f, 0.0f);  // NaN direction
+    const float x = u * (float)env.w - 0.5f, y = v * (float)env.h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy, W = (int)env.w, H = (int)env.h;
+    const int i0 = env_wrap(ix, W), i1 = env_wrap(ix + 1, W), j0 = env_wrap(iy, H), j1 = env_wrap(iy + 1, H);
+    const f3 t00 = ld3(env.tex[(size_t)j0 * env.w + i0]), t10 = ld3(env.tex[(size_t)j0 * env.w + i1]);
+    const f3 t01 = ld3(env.tex[(size_t)j1 * env.w + i0]), t11 = ld3(env.tex[(size_t)j1 * env.w + i1]);
+    const float a0 = 1.0f - a, b0 = 1.0f - b;
+    return (t00 * (a0 * b0) + t10 * (a * b0)) + (t01 * (a0 * b) + t11 * (a * b));
+}
+
+// Texture coordinates of a direction: u = atan2(x, z) / 2 pi + 1/2,
+// v = asin(y) / pi + 1/2.
+KRT_DEV void env_uv(f3 d, float &u, float &v) {
+    u = krt_atan2(d.x, d.z) / kTwoPiF + 0.5f;
+    v = krt_asin(d.y) / kPiF + 0.5f;
+}
+
+// Radiance of the environment seen along d: the path tracer's miss program
+// flips v (amadeus_path_tracer.cu:215-216), ReSTIR's and the light sampling do
+// not (amadeus_restir.cu:573-574, amadeus_common.cuh:237-240).
+KRT_DEV f3 env_radiance(const EnvMap &env, f3 d, bool flip_v) {
+    float u, v;
+    env_uv(d, u, v);
+    return env_fetch(env, u, flip_v ? 1.0f - v : v);
+}
+
+// The environment light's direction sample (Ld_Environment /
+// sample_direct): one pcg3f step, theta = acos(sqrt(1 - u)) about the world
+// +z axis, phi = 2 pi u'; with cos(theta) = sqrt(1 - u) and sin(theta) =
+// sqrt(1 - cos^2(theta)).  Its pdf is taken as 1 / (4 pi) by the reference.
+KRT_DEV f3 env_sample_dir(f3 &seed) {
+    seed = rand_uniform_3f(seed);
+    const float ct = sqrtf(1.0f - fract(seed.x));
+    const float st = sqrtf(1.0f - ct * ct);
+    float cp, sp;
+    sincos_turn(fract(seed.y), cp, sp);
+    return mk(st * cp, st * sp, ct);
+}
+constexpr float kEnvPdf = 0x1.45f306p-4f;  // 1 / (4 pi)
+
 // Accumulation on a glm::vec4 (amadeus_path_tracer.cu:59-70): true division.
 KRT_DEV float4 accumulate4(float4 dst, float4 s, bool acc, float count) {
     if (!acc) return s;

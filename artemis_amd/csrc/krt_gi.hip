@@ -724,6 +724,8 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
                unsigned long long *ctr, PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "restir_gi renders flat scenes (krt_scene_upload)");
+    KRT_REQUIRE(!li_in.env, KRT_ERR_STATE,
+                "restir_gi has no environment lighting (its samples are surface points); disable the envmap");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR GI buffers not loaded");
     KRT_REQUIRE(opt.neighbours >= 0 && opt.neighbours <= kGiMaxNeighbours, KRT_ERR_INVALID,
                 "restir_gi neighbours must be in [0, 8]");

@@ -206,6 +206,8 @@ struct LaunchInfo {
     uint32_t user_seed;
     int jitter;  // path tracer: sub-pixel jitter (1, the reference's) or the fixed -0.5 offset (0)
     float4 *color, *normal, *albedo, *position;
+    const float4 *env;  // environment map texels (RGBA, row-major), nullptr when off
+    uint32_t env_w, env_h;
 };
 
 struct RestirOptions {

@@ -30,19 +30,26 @@ KRT_DEV void pt_pixel(const LaunchInfo &li, const SceneView &sv, uint32_t pix, i
     f3 o, d;
     make_ray(li, px, py, seed, li.jitter != 0, o, d);
     f3 pos0 = mk(0, 0, 0), n0 = mk(0, 0, 0), alb0 = mk(0, 0, 0);
+    const EnvMap env = env_of(li);
     int nlev = 0;
     for (int level = 0; level <= li.max_depth; level++) {
         Hit h;
         rc.closest++;
         traverse<false>(sv.nodes, sv.wide, sv.tri, o, d, kTraceTmax, stk, h, ts);
-        if (h.prim < 0) break;  // __miss__: no environment map, value 0
+        if (h.prim < 0) {  // __miss__: the environment's radiance, or 0
+            if (env.tex) {
+                hist.put(level, env_radiance(env, d, true), false, mk(0.0f, 0.0f, 0.0f));
+                nlev = level + 1;
+            }
+            break;
+        }
         Surface sf;
         if (!load_surface(sv, h, d, sf)) break;
         Material mat = load_material(sv.mats, sf.mat_id);
         f3 n = sf.n;
         f3 x = offset_point(sf.x, n, mat.type);
         SurfaceHit sh{mat, sf.entering, n, -d, x};
-        f3 direct = mat.emission + Ld(sv, sh, seed, stk, ts, rc);
+        f3 direct = mat.emission + Ld(sv, env, sh, seed, stk, ts, rc);
         int out;
         f3 wi;
         float pdf;

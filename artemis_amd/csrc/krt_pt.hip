@@ -42,6 +42,8 @@ constexpr int kWaveBlock = 256;
 // state.w: levels with history in the low 16 bits, bit 16 set when the path
 // recorded a radiance-cache query
 constexpr int kQueried = 1 << 16;
+// state.w bit: the path's last level is a miss that took the environment's radiance
+constexpr int kEnvMiss = 1 << 17;
 
 __global__ __launch_bounds__(kWaveBlock) void pt_camera_kernel(LaunchInfo li, PtWave B) {
     __shared__ uint32_t nray;
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const float count = (float)li.samples;
     const bool acc = li.accumulate != 0;
-    const unsigned total = sv.nl;
+    const EnvMap env = env_of(li);
     const bool live = k < B.cl.prefix[B.cl.nseg];
     bool shadow = false, next = false;
     f3 so = mk(0, 0, 0), sd = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
@@ -95,6 +97,14 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
         Surface sf;
         const bool surf = h.prim >= 0 && load_surface<INST>(sv, h, d, sf);  // a miss or a NaN normal ends the path
         int queried = __float_as_int(st.w) & kQueried;
+        if (h.prim < 0 && env.tex) {  // __miss__ (amadeus_path_tracer.cu:204-220): value = the environment's
+            const f3 Le = env_radiance(env, d, true);
+            float4 *hl = B.hist + 3 * (size_t)level * npix;
+            hl[pix] = make_float4(Le.x, Le.y, Le.z, 0.0f);
+            hl[npix + pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            hl[2 * (size_t)npix + pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            B.state[pix] = make_float4(st.x, st.y, st.z, __int_as_float((level + 1) | queried | kEnvMiss));
+        }
         bool cache_stop = false;
         if (surf && cv.enabled && level == cv.depth) {
             // the cache vertex (krt_cache.hip): record the query; the path stops
@@ -118,27 +128,14 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
             // Ld (krt_shade.h) with its shadow ray deferred
             f3 contr = mk(0, 0, 0);
             float lpdf = 0.0f;
-            bool ld = false;
-            if (total == 0) {
-                rand_uniform(seed);
-            } else {
-                unsigned i = pick_light(seed, total);
-                float pdf = 1.0f / (float)total;
-                TriLight L = load_light(sv.lights, i);
-                float light_pdf = 1.0f / L.area;
-                f3 lpos = sample_area_light(L, seed);
-                f3 wi = normalize(lpos - sh.x);
-                float R = length(lpos - sh.x);
-                f3 f = brdf(sh, wi, eDiffuse);
-                float ldot = fabsf(dot(L.normal, wi));
-                float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
-                contr = f * L.intensity * geometric;
-                pdf *= light_pdf;
-                lpdf = pdf;
-                ld = true;
+            LdSample ls;
+            const bool ld = ld_sample(sv, env, sh, seed, ls);
+            if (ld) {
+                contr = ls.contr;
+                lpdf = ls.pdf;
                 so = sh.x;
-                sd = wi;
-                stmax = R - kShadowEps;
+                sd = ls.wi;
+                stmax = ls.tmax;
                 shadow = true;
             }
             int out;
@@ -245,7 +242,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_cache_compose_kernel(LaunchInfo
     }
     prefix[pix] = pre;
     cv.theta[pix] = th;
-    if (nlev == 0) {  // the primary ray missed: the AOVs take zeros
+    if (nlev == 0 || (nlev == 1 && (sw & kEnvMiss))) {  // the primary ray missed: the AOVs take zeros
         const bool acc = li.accumulate != 0;
         const float count = (float)li.samples;
         const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -262,7 +259,8 @@ __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, P
     uint32_t px, py;
     if (item >= tiles.items || !tiles.pixel(item, px, py)) return;
     const uint32_t pix = py * li.width + px;
-    const int nlev = __float_as_int(B.state[pix].w) & (kQueried - 1);
+    const int sw = __float_as_int(B.state[pix].w);
+    const int nlev = sw & (kQueried - 1);
     f3 value = mk(0, 0, 0);
     for (int k = nlev - 1; k >= 0; k--) {
         float4 e, t;
@@ -285,7 +283,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, P
     const bool acc = li.accumulate != 0;
     const float count = (float)li.samples;
     li.color[pix] = accumulate4(li.color[pix], col, acc, count);
-    if (nlev == 0) {  // the primary ray missed: the AOVs take zeros, as pt_pixel's
+    if (nlev == 0 || (nlev == 1 && (sw & kEnvMiss))) {  // the primary ray missed: the AOVs take zeros, as pt_pixel's
         const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         li.normal[pix] = accumulate4(li.normal[pix], zero, acc, count);
         li.albedo[pix] = accumulate4(li.albedo[pix], zero, acc, count);

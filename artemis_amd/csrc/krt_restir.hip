@@ -245,6 +245,8 @@ struct InitWave {
 // state.w of the wavefront initial pass: levels with history in the low 16
 // bits, bit 16 set when the path recorded a radiance-cache query
 constexpr int kQueried = 1 << 16;
+// state.w bit: the path's last level is a miss that took the environment's radiance
+constexpr int kEnvMiss = 1 << 17;
 
 __global__ __launch_bounds__(kSpatialBlock) void init_camera_kernel(RestirArgs A, InitWave B) {
     __shared__ uint32_t nray;
@@ -277,7 +279,8 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const float count = (float)li.samples;
     const bool acc = li.accumulate != 0;
-    const unsigned total = sv.nl;
+    const EnvMap env = env_of(li);
+    const unsigned total = sv.nl + (env.tex ? 1u : 0u);
     const bool live = k < B.cl.prefix[B.cl.nseg];
     bool shadow = false, next = false;
     f3 so = mk(0, 0, 0), sd = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
@@ -302,6 +305,14 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
         const CacheView &cv = A.cv;
         const bool surf = h.prim >= 0 && load_surface<INST>(sv, h, d, sf);
         int queried = __float_as_int(st.w) & kQueried;
+        if (h.prim < 0 && env.tex) {  // __miss__ (amadeus_restir.cu:566-577): the environment's radiance
+            const f3 Le = env_radiance(env, d, false);
+            float4 *hl = B.hist + 3 * (size_t)level * npix;
+            hl[pix] = make_float4(Le.x, Le.y, Le.z, 0.0f);
+            hl[npix + pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            hl[2 * (size_t)npix + pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            B.state[pix] = make_float4(st.x, st.y, st.z, __int_as_float((level + 1) | queried | kEnvMiss));
+        }
         bool cache_stop = false;
         if (surf && cv.enabled && level == cv.depth) {
             // the cache vertex: record the query; the path stops here unless
@@ -331,19 +342,28 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
                 res.M = 0;
                 res.W = 0.0f;
                 for (int i = 0; i < A.candidates; i++) {
-                    // sample_direct (amadeus_common.cuh:301-377), triangle lights
+                    // sample_direct (amadeus_common.cuh:301-377): triangle lights and the environment
                     RSample smp;
                     smp.Le = smp.normal = smp.point = mk(0, 0, 0);
                     smp.type = 0;
                     float pdf = 0.0f;
                     if (total > 0) {
                         unsigned li_ = pick_light(seed, total);
-                        TriLight L = load_light(sv.lights, li_);
-                        smp.point = sample_area_light(L, seed);
-                        smp.Le = L.intensity;
-                        smp.normal = L.normal;
-                        smp.type = 1;
-                        pdf = 1.0f / (L.area * (float)total);
+                        if (li_ < sv.nl) {
+                            TriLight L = load_light(sv.lights, li_);
+                            smp.point = sample_area_light(L, seed);
+                            smp.Le = L.intensity;
+                            smp.normal = L.normal;
+                            smp.type = 1;
+                            pdf = 1.0f / (L.area * (float)total);
+                        } else {  // a direction toward the environment, WORLD_RADIUS away
+                            const f3 wi = env_sample_dir(seed);
+                            smp.point = sh.x + wi * kEnvDistance;
+                            smp.Le = env_radiance(env, wi, false);
+                            smp.normal = -wi;
+                            smp.type = 2;
+                            pdf = 1.0f / (kFourPiF * (float)total);
+                        }
                     } else {
                         rand_uniform(seed);
                     }
@@ -367,26 +387,14 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
             } else {
                 // Ld (krt_shade.h) with its shadow ray deferred
                 emission = mat.emission;
-                if (total == 0) {
-                    rand_uniform(seed);
-                } else {
-                    unsigned i = pick_light(seed, total);
-                    float pdf = 1.0f / (float)total;
-                    TriLight L = load_light(sv.lights, i);
-                    float light_pdf = 1.0f / L.area;
-                    f3 lpos = sample_area_light(L, seed);
-                    f3 wi = normalize(lpos - sh.x);
-                    float R = length(lpos - sh.x);
-                    f3 f = brdf(sh, wi, eDiffuse);
-                    float ldot = fabsf(dot(L.normal, wi));
-                    float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
-                    contr = f * L.intensity * geometric;
-                    pdf *= light_pdf;
-                    lpdf = pdf;
-                    ld = true;
+                LdSample ls;
+                ld = ld_sample(sv, env, sh, seed, ls);
+                if (ld) {
+                    contr = ls.contr;
+                    lpdf = ls.pdf;
                     so = sh.x;
-                    sd = wi;
-                    stmax = R - kShadowEps;
+                    sd = ls.wi;
+                    stmax = ls.tmax;
                     shadow = true;
                 }
             }
@@ -479,7 +487,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_compose_kernel(RestirArgs 
     A.auxiliary[pix] = make_float4(st.x, st.y, st.z, 1.0f);
     const bool acc = li.accumulate != 0;
     const float count = (float)li.samples;
-    if (nlev > 0) {
+    if (nlev > 0 && !(nlev == 1 && (sw & kEnvMiss))) {
         // the reference traces this visibility ray a second time: W *= 1 - occluded
         float4 &w = A.cur.part(2, pix);
         w.w = B.occ[pix] ? 0.0f : w.w * (1.0f - 0.0f);

@@ -103,34 +103,64 @@ KRT_DEV bool occluded(const SceneView &sv, f3 o, f3 d, float tmax, const TraceSt
     return traverse<true, WIDE>(sv.nodes, sv.wide, sv.tri, o, d, tmax, stk, h, ts);
 }
 
-// Ld with one uniformly chosen triangle light (no quad lights, no envmap).
-template <bool WIDE = true, bool INST = false>
-KRT_DEV f3 Ld(const SceneView &sv, const SurfaceHit &sh, f3 &seed, const TraceStack &stk, TravStats &ts,
-              RayCount &rc) {
-    const unsigned total = sv.nl;
+// The environment map of a launch (ArmadaLaunchInfo::environment_map,
+// include/amadeus/armada.cuh:76-77); tex == nullptr when it is off.
+KRT_DEV EnvMap env_of(const LaunchInfo &li) { return EnvMap{li.env, li.env_w, li.env_h}; }
+
+// Ld's light sample with its shadow ray left to the caller (Ld,
+// amadeus_common.cuh:255-286): one light of the triangle lights and the
+// environment map chosen uniformly; a triangle light through Ld_light
+// (lighting.cuh:149-174), the environment through Ld_Environment
+// (amadeus_common.cuh:215-252: a direction about world +z, pdf 1/(4 pi), the
+// shadow ray to WORLD_RADIUS).  contr is the unshadowed contribution, pdf the
+// probability it is divided by; false when there is no light at all.
+struct LdSample {
+    f3 contr, wi;
+    float pdf, tmax;
+};
+KRT_DEV bool ld_sample(const SceneView &sv, const EnvMap &env, const SurfaceHit &sh, f3 &seed, LdSample &s) {
+    const unsigned total = sv.nl + (env.tex ? 1u : 0u);
     if (total == 0) {
         rand_uniform(seed);
-        return mk(0.0f, 0.0f, 0.0f);
+        return false;
     }
-    unsigned i = pick_light(seed, total);
+    const unsigned i = pick_light(seed, total);
     float pdf = 1.0f / (float)total;
-    TriLight L = load_light(sv.lights, i);
-    float light_pdf = 1.0f / L.area;
-    f3 lpos = sample_area_light(L, seed);
-    f3 wi = normalize(lpos - sh.x);
-    float R = length(lpos - sh.x);
-    rc.shadow++;
-    f3 contr;
-    if (occluded<WIDE, INST>(sv, sh.x, wi, R - kShadowEps, stk, ts)) {
-        contr = mk(0.0f, 0.0f, 0.0f);
-    } else {
+    if (i < sv.nl) {
+        TriLight L = load_light(sv.lights, i);
+        float light_pdf = 1.0f / L.area;
+        f3 lpos = sample_area_light(L, seed);
+        f3 wi = normalize(lpos - sh.x);
+        float R = length(lpos - sh.x);
         f3 f = brdf(sh, wi, eDiffuse);
         float ldot = fabsf(dot(L.normal, wi));
         float geometric = ldot * fabsf(dot(sh.n, wi)) / (R * R);
-        contr = f * L.intensity * geometric;
+        s.contr = f * L.intensity * geometric;
+        s.wi = wi;
+        s.tmax = R - kShadowEps;
+        pdf *= light_pdf;
+    } else {
+        const f3 wi = env_sample_dir(seed);
+        const f3 Li = env_radiance(env, wi, false);
+        const f3 f = brdf(sh, wi, eDiffuse) * fabsf(dot(sh.n, wi));
+        s.contr = f * Li;
+        s.wi = wi;
+        s.tmax = kEnvDistance - kShadowEps;
+        pdf *= kEnvPdf;
     }
-    pdf *= light_pdf;
-    return contr / pdf;
+    s.pdf = pdf;
+    return true;
+}
+
+// Ld with its shadow ray traced here (the fused kernels).
+template <bool WIDE = true, bool INST = false>
+KRT_DEV f3 Ld(const SceneView &sv, const EnvMap &env, const SurfaceHit &sh, f3 &seed, const TraceStack &stk,
+              TravStats &ts, RayCount &rc) {
+    LdSample s;
+    if (!ld_sample(sv, env, sh, seed, s)) return mk(0.0f, 0.0f, 0.0f);
+    rc.shadow++;
+    const f3 contr = occluded<WIDE, INST>(sv, sh.x, s.wi, s.tmax, stk, ts) ? mk(0.0f, 0.0f, 0.0f) : s.contr;
+    return contr / s.pdf;
 }
 
 // Camera ray; the seed is advanced once by pcg3f (jittered or fixed -0.5 offset).

@@ -244,6 +244,17 @@ int krt_armada_activate(krt_armada *a, const char *name);
 int krt_armada_set_option(krt_armada *a, const char *field, const krt_option *value);
 int krt_armada_get_option(krt_armada *a, const char *field, krt_option *out);
 int krt_armada_set_depth(krt_armada *a, int depth);
+/* ArmadaRTX::set_envmap / set_envmap_enabled (armada.cu:86-96,
+ * armada.cuh:323-328): the environment map as width x height RGBA float
+ * texels, row-major (row 0 is v = 0), copied to the device; rgba == NULL or a
+ * zero extent removes it.  Sampled like the reference's texture (normalised
+ * coordinates, bilinear, wrap): rays that miss take its radiance (the path
+ * tracer at (u, 1 - v), ReSTIR at (u, v)), and next-event estimation and
+ * ReSTIR's candidates pick it as one more light.  restir_gi refuses a frame
+ * while it is enabled (KRT_ERR_STATE); the editor's mamba attachment does not
+ * read it.  Setting it does not restart accumulation, as in the reference. */
+int krt_armada_set_envmap(krt_armada *a, const float *rgba, uint32_t width, uint32_t height);
+int krt_armada_set_envmap_enabled(krt_armada *a, int enabled);
 /* ArmadaRTX::render(system, entities, transforms, camera, transform,
  * accumulate) (armada.cu:733-780): one frame of the active attachment. */
 int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *camera, int accumulate);

@@ -355,6 +355,12 @@ public:
         if (dynamic_cast<BuiltinAttachment *>(m_attachments[name].get()))
             check(krt_armada_activate(m_armada, name.c_str()));
     }
+    // ArmadaRTX::set_envmap / set_envmap_enabled (armada.cu:86-96,
+    // armada.cuh:323-328), from texels instead of an image path
+    void set_envmap(const float *rgba, uint32_t width, uint32_t height) {
+        check(krt_armada_set_envmap(m_armada, rgba, width, height));
+    }
+    void set_envmap_enabled(bool enabled) { check(krt_armada_set_envmap_enabled(m_armada, enabled ? 1 : 0)); }
     void set_depth(int depth) {
         check(krt_armada_set_depth(m_armada, depth));
         m_depth = depth;

@@ -38,6 +38,10 @@ class OrcParams(C.Structure):
 _SIGS = {
     "orc_scene_create": (vp, [fp, fp, C.c_int, u32p, u32p, C.c_int, fp, C.c_int, fp, C.c_int]),
     "orc_scene_destroy": (None, [vp]),
+    "orc_scene_set_envmap": (None, [vp, fp, C.c_uint32, C.c_uint32]),
+    "orc_env_math": (None, [fp, C.c_int, C.c_int, fp]),
+    "orc_env_lookup": (None, [vp, fp, C.c_int, C.c_int, fp]),
+    "orc_env_sample_dirs": (None, [C.c_uint32, C.c_int, fp]),
     "orc_scene_light_count": (C.c_int, [vp]),
     "orc_scene_lights": (None, [vp, fp]),
     "orc_pcg3d": (None, [u32p, u32p, C.c_int]),
@@ -253,6 +257,20 @@ class OracleScene:
         except Exception:
             pass
 
+    def set_envmap(self, rgba) -> None:
+        """The armada's environment map: (H, W, 4) float32, or None."""
+        if rgba is None:
+            lib().orc_scene_set_envmap(self.h, None, 0, 0)
+            return
+        a = np.ascontiguousarray(rgba, np.float32)
+        lib().orc_scene_set_envmap(self.h, _f(a), a.shape[1], a.shape[0])
+
+    def env_lookup(self, dirs, flip_v=False) -> np.ndarray:
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.empty((len(d), 3), np.float32)
+        lib().orc_env_lookup(self.h, _f(d), len(d), int(flip_v), _f(out))
+        return out
+
     def lights(self) -> np.ndarray:
         n = lib().orc_scene_light_count(self.h)
         out = np.empty((n, 12), np.float32)
@@ -406,6 +424,27 @@ class OracleMamba:
         out = np.empty((self.h_, self.w, 16), np.float32)
         lib().orc_mamba_reservoirs(self.h, which, _f(out))
         return out
+
+
+def env_atan2(y, x) -> np.ndarray:
+    a = np.ascontiguousarray(np.stack([np.asarray(y, np.float32), np.asarray(x, np.float32)], -1), np.float32)
+    out = np.empty(len(a), np.float32)
+    lib().orc_env_math(_f(a), len(a), 0, _f(out))
+    return out
+
+
+def env_asin(x) -> np.ndarray:
+    a = np.zeros((len(x), 2), np.float32)
+    a[:, 0] = x
+    out = np.empty(len(a), np.float32)
+    lib().orc_env_math(_f(a), len(a), 1, _f(out))
+    return out
+
+
+def env_sample_dirs(seed_word: int, n: int) -> np.ndarray:
+    out = np.empty((n, 3), np.float32)
+    lib().orc_env_sample_dirs(seed_word, n, _f(out))
+    return out
 
 
 def gi_jacobian(xq, xr, xs, ns) -> float:

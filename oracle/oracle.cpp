@@ -72,6 +72,18 @@ struct orc_scene {
     std::vector<float> tlas;  // bvh.cpp tree over the instances' world boxes
     int ntlas = 0;
     std::vector<int32_t> inst_order;  // instances in preorder of the instance tree
+    // environment map (the armada's, ArmadaRTX::set_envmap): RGBA texels
+    std::vector<float> env;
+    uint32_t env_w = 0, env_h = 0;
+    EnvMap envmap() const {
+        EnvMap e;
+        if (env_w > 0) {
+            e.tex = env.data();
+            e.w = env_w;
+            e.h = env_h;
+        }
+        return e;
+    }
 };
 
 static inline f3 load3(const float *p) { return mk(p[0], p[1], p[2]); }
@@ -251,6 +263,42 @@ int orc_scene_tlas(const orc_scene *s, float *nodes) {
 }
 
 void orc_scene_destroy(orc_scene *s) { delete s; }
+
+void orc_env_math(const float *in, int n, int which, float *out) {
+    for (int i = 0; i < n; i++)
+        out[i] = which == 0 ? orc_atan2(in[2 * i], in[2 * i + 1]) : orc_asin(in[2 * i]);
+}
+
+void orc_env_lookup(const orc_scene *s, const float *dirs, int n, int flip_v, float *out) {
+    const EnvMap env = s->envmap();
+    for (int i = 0; i < n; i++) {
+        const f3 L = env.tex ? env_radiance(env, load3(dirs + 3 * (size_t)i), flip_v != 0) : mk(0, 0, 0);
+        out[3 * (size_t)i] = L.x;
+        out[3 * (size_t)i + 1] = L.y;
+        out[3 * (size_t)i + 2] = L.z;
+    }
+}
+
+void orc_env_sample_dirs(uint32_t seed_word, int n, float *out) {
+    f3 seed = seed_from_words(seed_word, 0x51u, 0x7u);
+    for (int i = 0; i < n; i++) {
+        const f3 d = env_sample_dir(seed);
+        out[3 * (size_t)i] = d.x;
+        out[3 * (size_t)i + 1] = d.y;
+        out[3 * (size_t)i + 2] = d.z;
+    }
+}
+
+void orc_scene_set_envmap(orc_scene *s, const float *rgba, uint32_t w, uint32_t h) {
+    if (!rgba || w == 0 || h == 0) {
+        s->env.clear();
+        s->env_w = s->env_h = 0;
+        return;
+    }
+    s->env.assign(rgba, rgba + 4 * (size_t)w * h);
+    s->env_w = w;
+    s->env_h = h;
+}
 int orc_scene_light_count(const orc_scene *s) { return (int)s->lights.size(); }
 void orc_scene_lights(const orc_scene *s, float *out) {
     for (size_t i = 0; i < s->lights.size(); i++) {
@@ -979,16 +1027,36 @@ inline unsigned pick_light(f3 &seed, unsigned total) {
     return i < total ? i : total - 1;  // rand*total may round up to total
 }
 
-// Ld (amadeus_common.cuh:255-286) with Ld_light (lighting.cuh:149-174)
+// Ld (amadeus_common.cuh:255-286) with Ld_light (lighting.cuh:149-174) and
+// Ld_Environment (amadeus_common.cuh:215-252) when the environment map is on
 f3 Ld(Ctx &cx, const SurfaceHit &sh, f3 &seed) {
     const orc_scene *s = cx.s;
-    int total = (int)s->lights.size();
+    const EnvMap env = s->envmap();
+    const int nl = (int)s->lights.size();
+    int total = nl + (env.tex ? 1 : 0);
     if (total == 0) {
         rand_uniform(seed);
         return mk(0, 0, 0);
     }
     unsigned i = pick_light(seed, (unsigned)total);
     float pdf = 1.0f / (float)total;
+    if ((int)i >= nl) {
+        // Ld_Environment: a direction about world +z, pdf 1/(4 pi), the
+        // shadow ray to WORLD_RADIUS
+        f3 wi = env_sample_dir(seed);
+        f3 Li = env_radiance(env, wi, false);
+        float light_pdf = kEnvPdf;
+        cx.c.shadow++;
+        f3 contr;
+        if (occluded(s, sh.x, wi, kEnvDistance - kShadowEps, cx.c)) {
+            contr = mk(0, 0, 0);
+        } else {
+            f3 f = brdf(sh, wi, eDiffuse) * fabsf(dot(sh.n, wi));
+            contr = f * Li;
+        }
+        pdf *= light_pdf;
+        return contr / pdf;
+    }
     const TriLight &light = s->lights[i];
     float light_pdf = 1.0f / light_area(light);
     f3 lpos = sample_area_light(light, seed);
@@ -1029,8 +1097,9 @@ void pt_trace(Ctx &cx, int max_depth, f3 o, f3 d, RayPacket &rp) {
     }
     cx.c.closest++;
     Hit h = closest(cx.s, o, d, kTraceTmax, 2, cx.c);
-    if (h.prim < 0) {
-        rp.value = mk(0, 0, 0);  // no environment map
+    if (h.prim < 0) {  // __miss__: the environment map at (u, 1 - v), or 0
+        const EnvMap env = cx.s->envmap();
+        rp.value = env.tex ? env_radiance(env, d, true) : mk(0, 0, 0);
         return;
     }
     Surface sf;
@@ -1311,8 +1380,9 @@ void restir_trace(Ctx &cx, orc_restir *r, const orc_params *p, int index, f3 o, 
     }
     cx.c.closest++;
     Hit h = closest(cx.s, o, d, kTraceTmax, 2, cx.c);
-    if (h.prim < 0) {
-        rp.value = mk(0, 0, 0);
+    if (h.prim < 0) {  // __miss__ (amadeus_restir.cu:566-577): the environment map at (u, v), or 0
+        const EnvMap env = cx.s->envmap();
+        rp.value = env.tex ? env_radiance(env, d, false) : mk(0, 0, 0);
         return;
     }
     Surface sf;
@@ -1330,19 +1400,30 @@ void restir_trace(Ctx &cx, orc_restir *r, const orc_params *p, int index, f3 o, 
     if (primary) {
         Reservoir &res = r->buf[r->cur][index];
         res.reset();
-        unsigned total = (unsigned)s->lights.size();
+        const EnvMap env = s->envmap();
+        const unsigned nl = (unsigned)s->lights.size();
+        unsigned total = nl + (env.tex ? 1u : 0u);
         for (int i = 0; i < p->candidates; i++) {
-            // sample_direct, amadeus_common.cuh:301-377 (triangle lights)
+            // sample_direct, amadeus_common.cuh:301-377 (triangle lights, the environment)
             Sample smp;
             float pdf = 0.0f;
             if (total > 0) {
                 unsigned li = pick_light(rp.seed, total);
-                const TriLight &L = s->lights[li];
-                smp.point = sample_area_light(L, rp.seed);
-                smp.Le = L.intensity;
-                smp.normal = light_normal(L);
-                smp.type = 1;
-                pdf = 1.0f / (light_area(L) * (float)total);
+                if (li < nl) {
+                    const TriLight &L = s->lights[li];
+                    smp.point = sample_area_light(L, rp.seed);
+                    smp.Le = L.intensity;
+                    smp.normal = light_normal(L);
+                    smp.type = 1;
+                    pdf = 1.0f / (light_area(L) * (float)total);
+                } else {  // amadeus_common.cuh:341-370: a point WORLD_RADIUS away
+                    f3 wi = env_sample_dir(rp.seed);
+                    smp.point = sh.x + wi * kEnvDistance;
+                    smp.Le = env_radiance(env, wi, false);
+                    smp.normal = -wi;
+                    smp.type = 2;
+                    pdf = 1.0f / (kFourPiF * (float)total);
+                }
             } else {
                 rand_uniform(rp.seed);
             }

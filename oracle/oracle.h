@@ -64,6 +64,15 @@ int orc_scene_tlas(const orc_scene *s, float *nodes);
 void orc_affine_inverse(const float *M, float *Minv);
 void orc_instance_box(const float *root_min, const float *root_max, const float *M, float *box6);
 void orc_scene_destroy(orc_scene *s);
+/* The armada's environment map (ArmadaRTX::set_envmap): w x h RGBA float
+ * texels, row-major; rgba == NULL removes it. */
+void orc_scene_set_envmap(orc_scene *s, const float *rgba, uint32_t w, uint32_t h);
+/* The environment map's math, for its pins: which 0 = atan2(in[2i], in[2i+1]),
+ * 1 = asin(in[2i]); the radiance along n directions (3 floats each); n
+ * directions of Ld_Environment's sampling from one seed. */
+void orc_env_math(const float *in, int n, int which, float *out);
+void orc_env_lookup(const orc_scene *s, const float *dirs, int n, int flip_v, float *out);
+void orc_env_sample_dirs(uint32_t seed_word, int n, float *out);
 int orc_scene_light_count(const orc_scene *s);
 /* 12 floats per light: a, ab, ac, intensity (armada.cu:98-150 rule). */
 void orc_scene_lights(const orc_scene *s, float *out);

@@ -330,6 +330,123 @@ static inline f3 sample_area_light(const TriLight &l, f3 &seed) {
     return (l.a + u * l.ab) + v * l.ac;
 }
 
+// ---------------------------------------------------------------------------
+// Environment map (ArmadaRTX::set_envmap, source/amadeus/armada.cu:86-96; the
+// miss programs amadeus_path_tracer.cu:204-220 and amadeus_restir.cu:566-577;
+// Ld_Environment and sample_direct, amadeus_common.cuh:215-252, 341-370).
+// CUDA's atan2f / asinf are not restated: atan and asin here are their
+// Maclaurin series after argument reduction,
+//   atan t = t - t^3/3 + ... - t^23/23 + t^25/25 for |t| <= tan(pi/8);
+//   atan t = pi/4 + atan((t - 1)/(t + 1)) above it; pi/2 - atan(1/t) above 1;
+//   asin x = sum_k (2k)!/(4^k (k!)^2 (2k+1)) x^(2k+1), k <= 11, for |x| <= 1/2;
+//   asin x = pi/2 - 2 asin(sqrt((1 - x)/2)) above it.
+// The texture fetch is tex2D with normalised coordinates, bilinear filtering
+// and wrap addressing (include/cuda/interop.cuh:96-135), in FP32.
+// ---------------------------------------------------------------------------
+static const float kPiF = 0x1.921fb6p+1f;
+static const float kHalfPiF = 0x1.921fb6p+0f;
+static const float kQuarterPiF = 0x1.921fb6p-1f;
+static const float kTwoPiF = 0x1.921fb6p+2f;
+static const float kFourPiF = 0x1.921fb6p+3f;
+static const float kTanPi8F = 0x1.a8279ap-2f;
+static const float kEnvDistance = 10000.0f;        // WORLD_RADIUS, amadeus_common.cuh:219
+static const float kEnvPdf = 0x1.45f306p-4f;       // 1 / (4 pi), amadeus_common.cuh:245
+
+static inline float atan_series(float t) {
+    static const float c[12] = {0x1.47ae14p-5f, -0x1.642c86p-5f, 0x1.861862p-5f, -0x1.af286cp-5f,
+                                0x1.e1e1e2p-5f, -0x1.111112p-4f, 0x1.3b13b2p-4f, -0x1.745d18p-4f,
+                                0x1.c71c72p-4f, -0x1.24924ap-3f, 0x1.99999ap-3f, -0x1.555556p-2f};
+    const float t2 = t * t;
+    float p = c[0];
+    for (int k = 1; k < 12; k++) p = p * t2 + c[k];
+    return t + (t * t2) * p;
+}
+
+static inline float atan_unit(float t) {
+    return t > kTanPi8F ? kQuarterPiF + atan_series((t - 1.0f) / (t + 1.0f)) : atan_series(t);
+}
+
+static inline float orc_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    float r;
+    if (ax == 0.0f && ay == 0.0f) {
+        r = std::signbit(x) ? kPiF : 0.0f;
+    } else {
+        r = ay <= ax ? atan_unit(ay / ax) : kHalfPiF - atan_unit(ax / ay);
+        if (std::signbit(x)) r = kPiF - r;
+    }
+    return std::signbit(y) ? -r : r;
+}
+
+static inline float asin_series(float x) {
+    static const float c[11] = {0x1.df3bd4p-8f, 0x1.12ef3cp-7f, 0x1.3fde50p-7f, 0x1.7a8788p-7f,
+                                0x1.c9999ap-7f, 0x1.1c4ec4p-6f, 0x1.6e8ba2p-6f, 0x1.f1c71cp-6f,
+                                0x1.6db6dcp-5f, 0x1.333334p-4f, 0x1.555556p-3f};
+    const float x2 = x * x;
+    float p = c[0];
+    for (int k = 1; k < 11; k++) p = p * x2 + c[k];
+    return x + (x * x2) * p;
+}
+
+static inline float orc_asin(float x) {
+    const float a = fminf(fabsf(x), 1.0f);
+    const float r = a <= 0.5f ? asin_series(a) : kHalfPiF - 2.0f * asin_series(sqrtf((1.0f - a) * 0.5f));
+    return std::signbit(x) ? -r : r;
+}
+
+struct EnvMap {  // RGBA float texels, row-major, w x h; tex == nullptr: none
+    const float *tex = nullptr;
+    uint32_t w = 0, h = 0;
+};
+
+static inline int env_wrap(int i, int n) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+static inline f3 env_texel(const EnvMap &env, int i, int j) {
+    const float *t = env.tex + 4 * ((size_t)j * env.w + (size_t)i);
+    return mk(t[0], t[1], t[2]);
+}
+
+static inline f3 env_fetch(const EnvMap &env, float u, float v) {
+    if (!(u >= -1.0f && u <= 2.0f && v >= -1.0f && v <= 2.0f)) return mk(0, 0, 0);  // NaN direction
+    const float x = u * (float)env.w - 0.5f, y = v * (float)env.h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy, W = (int)env.w, H = (int)env.h;
+    const int i0 = env_wrap(ix, W), i1 = env_wrap(ix + 1, W), j0 = env_wrap(iy, H), j1 = env_wrap(iy + 1, H);
+    const f3 t00 = env_texel(env, i0, j0), t10 = env_texel(env, i1, j0);
+    const f3 t01 = env_texel(env, i0, j1), t11 = env_texel(env, i1, j1);
+    const float a0 = 1.0f - a, b0 = 1.0f - b;
+    return (t00 * (a0 * b0) + t10 * (a * b0)) + (t01 * (a0 * b) + t11 * (a * b));
+}
+
+// u = atan2(x, z) / 2 pi + 1/2, v = asin(y) / pi + 1/2
+static inline void env_uv(f3 d, float &u, float &v) {
+    u = orc_atan2(d.x, d.z) / kTwoPiF + 0.5f;
+    v = orc_asin(d.y) / kPiF + 0.5f;
+}
+
+// the path tracer's miss program reads (u, 1 - v); ReSTIR's and the light
+// sampling read (u, v)
+static inline f3 env_radiance(const EnvMap &env, f3 d, bool flip_v) {
+    float u, v;
+    env_uv(d, u, v);
+    return env_fetch(env, u, flip_v ? 1.0f - v : v);
+}
+
+// Ld_Environment's direction: one pcg3f step; theta = acos(sqrt(1 - u)) about
+// world +z (cos = sqrt(1 - u), sin = sqrt(1 - cos^2)), phi = 2 pi u'
+static inline f3 env_sample_dir(f3 &seed) {
+    seed = rand_uniform_3f(seed);
+    const float ct = sqrtf(1.0f - fract(seed.x));
+    const float st = sqrtf(1.0f - ct * ct);
+    float cp, sp;
+    sincos_turn(fract(seed.y), cp, sp);
+    return mk(st * cp, st * sp, ct);
+}
+
 // Möller–Trumbore: shaders/rt/modules/intersect.glsl:14-33 ==
 // include/optix/lighting.cuh:34-55.  Returns t, or -1 on a miss.
 static inline float intersect_tri(f3 v1, f3 v2, f3 v3, f3 o, f3 d, float &b1, float &b2) {

@@ -44,6 +44,7 @@
 using std::isnan;
 using std::isinf;
 using std::isfinite;
+using std::signbit;
 
 // ------------------------------------------------------------- vector types
 struct float2 { float x, y; };

@@ -209,7 +209,54 @@ def case_attachments(ctx):
     sc.free()
 
 
-CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "instancing": case_instancing,
+def case_envmap(ctx):
+    """The armada's environment map: misses, next-event estimation and
+    ReSTIR's environment candidates, bit for bit vs the oracle; ReSTIR GI
+    refuses to render while it is on."""
+    s = scenes.sky_room()
+    sc = ctx.upload(s)
+    osc = O.OracleScene(s, sc.serialized())
+    rng = np.random.default_rng(5)
+    env = rng.uniform(0.0, 3.0, (8, 16, 4)).astype(np.float32)
+    osc.set_envmap(env)
+    W, H = 24, 16
+    cam = look_at(**s.camera, aspect=W / H)
+    arm = ArmadaRTX(ctx, W, H)
+    arm.set_envmap(env)
+    arm.attach("pt", PathTracer(ctx))
+    arm.set_depth(4)
+    for _ in range(2):
+        arm.render(sc, cam)
+    ref, _ = osc.render_pt(cam, W, H, frames=2, max_depth=4)
+    for k in range(4):
+        assert bits_equal(arm._read(k), ref[k]), f"path tracer buffer {k}"
+    say("envmap: path tracer bit-exact")
+    del arm
+    arm = ArmadaRTX(ctx, W, H)
+    arm.set_envmap(env)
+    arm.attach("restir", ReSTIR(ctx))
+    ors = O.OracleReSTIR(W, H)
+    for _ in range(2):
+        arm.render(sc, cam)
+        ors.frame(osc, cam, max_depth=2)
+    for which in (0, 1):
+        assert bits_equal(arm.reservoirs(which), ors.reservoirs(which)), f"restir reservoirs {which}"
+    assert bits_equal(arm.color_buffer(), ors.buffers[0]), "restir colour"
+    say("envmap: ReSTIR DI bit-exact")
+    arm.attach("gi", ReSTIRGI(ctx))
+    try:
+        arm.render(sc, cam)
+        raise AssertionError("restir_gi rendered with an environment map")
+    except KrtError as e:
+        assert e.code == KRT_ERR_STATE, e
+    arm.set_envmap_enabled(False)
+    arm.render(sc, cam)
+    say("envmap: restir_gi refuses it (KRT_ERR_STATE) and renders once it is off")
+    del arm
+    sc.free()
+
+
+CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "instancing": case_instancing,
          "instancing_full": lambda ctx: case_instancing(ctx, full=True)}
 
 if __name__ == "__main__":
