@@ -213,6 +213,21 @@ class Scene:
         code = {"sah": L.LAYOUT_SAH, "reference": L.LAYOUT_REFERENCE}[layout]
         L.check(self.ctx._lib.krt_accel_build_layout(self.ctx.h, self.h, leaf_max, code))
 
+    def update_vertices(self, first: int, positions: np.ndarray, normals: np.ndarray | None = None) -> None:
+        """Moves vertices [first, first + len(positions)) (world space); the
+        scene then needs refit() or build() before it is rendered again."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        if n is not None and len(n) != len(p):
+            raise ValueError("positions and normals differ in length")
+        L.check(self.ctx._lib.krt_scene_update_vertices(self.ctx.h, self.h, first, len(p), _f(p),
+                                                        None if n is None else _f(n)))
+
+    def refit(self) -> None:
+        """Recomputes the trees' boxes, triangle records and lights for moved
+        vertices, keeping the trees' shapes."""
+        L.check(self.ctx._lib.krt_accel_refit(self.ctx.h, self.h))
+
     def stats(self) -> dict:
         s = L.BvhStats()
         L.check(self.ctx._lib.krt_accel_stats(self.ctx.h, self.h, C.byref(s)))

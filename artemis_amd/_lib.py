@@ -45,6 +45,7 @@ class BvhStats(C.Structure):
         ("triangles", C.c_uint32), ("serialized_nodes", C.c_uint32), ("traversal_nodes", C.c_uint32),
         ("depth_bound", C.c_uint32), ("build_levels", C.c_uint32), ("build_ms", C.c_float),
         ("layout", C.c_int), ("binary_nodes", C.c_uint32), ("node_bytes", C.c_uint32),
+        ("refit_ms", C.c_float),
     ]
 
 
@@ -127,6 +128,8 @@ _SIGS = {
     "krt_accel_build": (C.c_int, [vp, handle, C.c_int]),
     "krt_accel_build_layout": (C.c_int, [vp, handle, C.c_int, C.c_int]),
     "krt_accel_stats": (C.c_int, [vp, handle, C.POINTER(BvhStats)]),
+    "krt_scene_update_vertices": (C.c_int, [vp, handle, C.c_uint32, C.c_uint32, fp, fp]),
+    "krt_accel_refit": (C.c_int, [vp, handle]),
     "krt_accel_serialized": (C.c_int, [vp, handle, fp, C.c_size_t]),
     "krt_scene_upload_instanced": (C.c_int, [vp, C.POINTER(InstancedDesc), C.POINTER(handle)]),
     "krt_accel_mesh_serialized": (C.c_int, [vp, handle, C.c_uint32, fp, C.c_size_t, u32p]),

@@ -927,6 +927,7 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
     A.nnodes = (uint32_t)nnodes;
     A.levels = levels;
     A.depth_bound = std::max(depth, 3 * wide_levels + 1);
+    A.bin_depth = depth;
     A.layout = layout;
 }
 
@@ -1094,8 +1095,204 @@ static void accel_build_instanced(Context &ctx, Scene &s, int leaf_max, int layo
     s.version++;
 }
 
+// ---------------------------------------------------------------------------
+// Refit (moved geometry, same topology): Accelerator::update rebuilds the
+// acceleration structure when an entity moves (accelerator.cuh:145-196,
+// armada.cu:575-602, 716-727); here the trees keep their shape and their
+// boxes are recomputed bottom-up from the new vertex positions.  A node is
+// computed by the last of its children to finish (an arrival counter per
+// node): leaves start, each finished node counts itself at its parent, and
+// the thread whose arrival completes a parent goes on with it.  Boxes are
+// unions (min / max) of triangle boxes, so a refit node's box is exactly the
+// one a build of the same shape would give.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 vload(const float4 *p) {  // bypasses the (non-coherent) vector L1
+    const volatile float *v = reinterpret_cast<const volatile float *>(p);
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+__global__ void ser_parent_kernel(const float4 *ser, uint32_t nser, int *parent) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nser) return;
+    if (p == 0) parent[0] = -1;
+    if (ser[3 * p].x != 0.0f) return;  // leaf
+    const int c0 = (int)p + 1, c1 = __float_as_int(ser[3 * c0].w) / 3;  // left child's miss link = right child
+    parent[c0] = (int)p;
+    parent[c1] = (int)p;
+}
+
+// bvh.cpp layout: leaf box = its primitive's box, internal = union of both children
+__global__ void ser_refit_kernel(float4 *ser, uint32_t nser, const int *parent, int *arrivals, const float4 *bmin,
+                                 const float4 *bmax) {
+    const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= nser || ser[3 * leaf].x == 0.0f) return;
+    const int prim = __float_as_int(ser[3 * leaf].y);
+    ser[3 * leaf + 1] = make_float4(bmin[prim].x, bmin[prim].y, bmin[prim].z, 0.0f);
+    ser[3 * leaf + 2] = make_float4(bmax[prim].x, bmax[prim].y, bmax[prim].z, 0.0f);
+    int p = parent[leaf];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&arrivals[p], 1) == 0) return;  // the other child finishes this node
+        const int c0 = p + 1, c1 = __float_as_int(ser[3 * c0].w) / 3;
+        const float4 a0 = vload(ser + 3 * c0 + 1), a1 = vload(ser + 3 * c0 + 2);
+        const float4 b0 = vload(ser + 3 * c1 + 1), b1 = vload(ser + 3 * c1 + 2);
+        ser[3 * p + 1] = make_float4(gmin(a0.x, b0.x), gmin(a0.y, b0.y), gmin(a0.z, b0.z), 0.0f);
+        ser[3 * p + 2] = make_float4(gmax(a1.x, b1.x), gmax(a1.y, b1.y), gmax(a1.z, b1.z), 0.0f);
+        p = parent[p];
+    }
+}
+
+// Binary traversal tree (either layout): links are node ids (> 0) or leaf
+// ranges of traversal positions; a node's arrival target is its number of
+// internal children.
+__global__ void bin_parent_kernel(const float4 *nodes, uint32_t nn, int *parent, int *need) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    if (i == 0) parent[0] = -1;
+    const float4 L = nodes[4 * i + 3];
+    const int l0 = __float_as_int(L.x), l1 = __float_as_int(L.y);
+    need[i] = (l0 > 0 ? 1 : 0) + (l1 > 0 ? 1 : 0);
+    if (l0 > 0) parent[l0] = (int)i;
+    if (l1 > 0) parent[l1] = (int)i;
+}
+
+__device__ void range_box(const float4 *tri, const float4 *bmin, const float4 *bmax, int link, float4 &mn,
+                          float4 &mx) {
+    const int first = link & 0x03FFFFFF, cnt = ((link >> 26) & 31) + 1;
+    mn = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f);
+    mx = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
+    for (int r = first; r < first + cnt; r++) {
+        const int prim = __float_as_int(tri[3 * r].w);
+        const float4 a = bmin[prim], b = bmax[prim];
+        mn = make_float4(fminf(mn.x, a.x), fminf(mn.y, a.y), fminf(mn.z, a.z), 0.0f);
+        mx = make_float4(fmaxf(mx.x, b.x), fmaxf(mx.y, b.y), fmaxf(mx.z, b.z), 0.0f);
+    }
+}
+
+__device__ void bin_node_refit(float4 *nodes, int i, const float4 *tri, const float4 *bmin, const float4 *bmax) {
+    const float4 L = nodes[4 * i + 3];
+    const int link[2] = {__float_as_int(L.x), __float_as_int(L.y)};
+    float4 mn[2], mx[2];
+    for (int k = 0; k < 2; k++) {
+        if (link[k] > 0) {  // union of the child node's two slots
+            const int c = link[k];
+            const float4 X = vload(nodes + 4 * c), Y = vload(nodes + 4 * c + 1), Z = vload(nodes + 4 * c + 2);
+            mn[k] = make_float4(fminf(X.x, X.z), fminf(Y.x, Y.z), fminf(Z.x, Z.z), 0.0f);
+            mx[k] = make_float4(fmaxf(X.y, X.w), fmaxf(Y.y, Y.w), fmaxf(Z.y, Z.w), 0.0f);
+        } else {
+            range_box(tri, bmin, bmax, link[k], mn[k], mx[k]);
+        }
+    }
+    nodes[4 * i + 0] = make_float4(mn[0].x, mx[0].x, mn[1].x, mx[1].x);
+    nodes[4 * i + 1] = make_float4(mn[0].y, mx[0].y, mn[1].y, mx[1].y);
+    nodes[4 * i + 2] = make_float4(mn[0].z, mx[0].z, mn[1].z, mx[1].z);
+}
+
+__global__ void bin_refit_kernel(float4 *nodes, uint32_t nn, const int *parent, const int *need, int *arrivals,
+                                 const float4 *tri, const float4 *bmin, const float4 *bmax) {
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 >= nn || need[i0] != 0) return;  // start at nodes whose children are all leaf ranges
+    int i = (int)i0;
+    for (;;) {
+        bin_node_refit(nodes, i, tri, bmin, bmax);
+        const int p = parent[i];
+        if (p < 0) return;
+        __threadfence();
+        if (atomicAdd(&arrivals[p], 1) + 1 < need[p]) return;  // another child finishes the parent
+        i = p;
+    }
+}
+
+// The triangle records in traversal order keep their primitive and rank
+__global__ void tri_repack_kernel(const float *pos, const uint32_t *idx, uint32_t nt, float4 *tri) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt) return;
+    const int prim = __float_as_int(tri[3 * r].w);
+    const f3 v1 = ld3(pos + 3 * idx[3 * prim]), v2 = ld3(pos + 3 * idx[3 * prim + 1]),
+             v3 = ld3(pos + 3 * idx[3 * prim + 2]);
+    const f3 e1 = v2 - v1, e2 = v3 - v1;
+    tri[3 * r] = make_float4(v1.x, v1.y, v1.z, __int_as_float(prim));
+    tri[3 * r + 1] = make_float4(e1.x, e1.y, e1.z, tri[3 * r + 1].w);
+    tri[3 * r + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+}
+
+// Triangle light rows of the moved geometry (the expressions of the upload's
+// light_row: armada.cu:117-140 with lighting.cuh:102-110)
+__global__ void light_rows_kernel(const float *pos, const uint32_t *idx, const uint32_t *tri_mat, const float4 *mats,
+                                  const int *light_tri, uint32_t nl, float4 *lights) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    const int t = light_tri[l];
+    const float *a = pos + 3 * idx[3 * t], *b = pos + 3 * idx[3 * t + 1], *c = pos + 3 * idx[3 * t + 2];
+    const float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+    const float dd = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+    const float area = sqrtf(dd) * 0.5f;
+    const float inv = 1.0f / sqrtf(dd);
+    const float4 e = mats[4 * tri_mat[t] + 2];
+    lights[4 * l + 0] = make_float4(a[0], a[1], a[2], area);
+    lights[4 * l + 1] = make_float4(ab[0], ab[1], ab[2], cr[0] * inv);
+    lights[4 * l + 2] = make_float4(ac[0], ac[1], ac[2], cr[1] * inv);
+    lights[4 * l + 3] = make_float4(e.x, e.y, e.z, cr[2] * inv);
+}
+
+void accel_refit(Context &ctx, Scene &s) {
+    KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "refit takes flat scenes (krt_scene_upload)");
+    KRT_REQUIRE(s.accel.built && s.accel.nser > 0, KRT_ERR_STATE, "scene has no acceleration structure to refit");
+    hipStream_t st = ctx.stream;
+    Accel &A = s.accel;
+    const uint32_t n = s.nt;
+    hipEvent_t e0, e1;
+    KRT_HIP(hipEventCreate(&e0));
+    KRT_HIP(hipEventCreate(&e1));
+    KRT_HIP(hipEventRecord(e0, st));
+    DevBuf bmin(sizeof(float4) * n), bmax(sizeof(float4) * n);
+    tri_boxes_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), n, bmin.as<float4>(),
+                                                     bmax.as<float4>());
+    KRT_HIP(hipGetLastError());
+    // bvh.cpp tree
+    DevBuf parent(sizeof(int) * A.nser), arrivals(sizeof(int) * A.nser);
+    KRT_HIP(hipMemsetAsync(arrivals.p, 0, arrivals.bytes, st));
+    ser_parent_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, parent.as<int>());
+    KRT_HIP(hipGetLastError());
+    ser_refit_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, parent.as<int>(),
+                                                          arrivals.as<int>(), bmin.as<float4>(), bmax.as<float4>());
+    KRT_HIP(hipGetLastError());
+    // triangle records, then the binary traversal tree over them, then its 4-wide collapse
+    tri_repack_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), n, A.tri.as<float4>());
+    KRT_HIP(hipGetLastError());
+    DevBuf bparent(sizeof(int) * A.nnodes), need(sizeof(int) * A.nnodes), barr(sizeof(int) * A.nnodes);
+    KRT_HIP(hipMemsetAsync(barr.p, 0, barr.bytes, st));
+    bin_parent_kernel<<<blocks(A.nnodes, 256), 256, 0, st>>>(A.nodes.as<float4>(), A.nnodes, bparent.as<int>(),
+                                                            need.as<int>());
+    KRT_HIP(hipGetLastError());
+    bin_refit_kernel<<<blocks(A.nnodes, 256), 256, 0, st>>>(A.nodes.as<float4>(), A.nnodes, bparent.as<int>(),
+                                                           need.as<int>(), barr.as<int>(), A.tri.as<float4>(),
+                                                           bmin.as<float4>(), bmax.as<float4>());
+    KRT_HIP(hipGetLastError());
+    const uint32_t wide_levels = collapse_wide(ctx, A, (int)A.nnodes);
+    A.depth_bound = std::max(A.bin_depth, 3 * wide_levels + 1);
+    if (s.nl > 0) {
+        light_rows_kernel<<<blocks(s.nl, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(),
+                                                            s.tri_mat.as<uint32_t>(), s.mats.as<float4>(),
+                                                            s.light_tri.as<int>(), s.nl, s.lights.as<float4>());
+        KRT_HIP(hipGetLastError());
+    }
+    KRT_HIP(hipEventRecord(e1, st));
+    KRT_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    KRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A.refit_ms = ms;
+    s.stale = false;
+    s.version++;
+}
+
 void accel_build(Context &ctx, Scene &s, int leaf_max, int layout) {
     KRT_REQUIRE(leaf_max >= 1 && leaf_max <= 32, KRT_ERR_INVALID, "leaf_max must be in [1, 32]");
+    s.stale = false;
     KRT_REQUIRE(layout == KRT_LAYOUT_SAH || layout == KRT_LAYOUT_REFERENCE, KRT_ERR_INVALID, "unknown BVH layout");
     KRT_REQUIRE(s.nt >= 1, KRT_ERR_STATE, "scene has no triangles");
     KRT_REQUIRE(s.nt < (1u << 26), KRT_ERR_INVALID, "too many triangles for the leaf link encoding");

@@ -299,6 +299,12 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scen
         }
         s->nl = (uint32_t)(lt.size() / 16);
         upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
+        {
+            std::vector<int32_t> ltri;
+            for (uint32_t t = 0; t < d->nt; t++)
+                if (emissive(d->materials + 12 * d->tri_material[t] + 6)) ltri.push_back((int32_t)t);
+            upload(C, s->light_tri, ltri.data(), sizeof(int32_t) * ltri.size());
+        }
         // area lights of the editor's ReSTIR (one per emissive material, its
         // triangles in triangle order)
         {
@@ -513,6 +519,46 @@ int krt_accel_build_layout(krt_context *ctx, krt_handle scene, int leaf_max, int
     });
 }
 
+int krt_scene_update_vertices(krt_context *ctx, krt_handle scene, uint32_t first, uint32_t count,
+                              const float *positions, const float *normals) {
+    return guard([&] {
+        KRT_REQUIRE(ctx && positions, KRT_ERR_INVALID, "null argument");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "vertex updates take flat scenes (krt_scene_upload)");
+        KRT_REQUIRE(s.serve_refs.load() == 0, KRT_ERR_STATE, "scene is used by server requests in flight");
+        KRT_REQUIRE((uint64_t)first + count <= s.nv, KRT_ERR_INVALID, "vertex range out of the scene");
+        for (size_t i = 0; i < 3 * (size_t)count; i++)
+            KRT_REQUIRE(std::isfinite(positions[i]) && (!normals || std::isfinite(normals[i])), KRT_ERR_INVALID,
+                        "non-finite vertex data");
+        if (count == 0) return;
+        hipStream_t st = ctx->impl.stream;
+        KRT_HIP(hipStreamSynchronize(st));  // frames in flight may still read the old vertices
+        KRT_HIP(hipMemcpyAsync(s.pos.as<float>() + 3 * (size_t)first, positions, sizeof(float) * 3 * count,
+                               hipMemcpyHostToDevice, st));
+        if (normals)
+            KRT_HIP(hipMemcpyAsync(s.nrm.as<float>() + 3 * (size_t)first, normals, sizeof(float) * 3 * count,
+                                   hipMemcpyHostToDevice, st));
+        KRT_HIP(hipStreamSynchronize(st));
+        for (uint32_t v = 0; v < count; v++)  // the box grows to hold the moved vertices
+            for (int a = 0; a < 3; a++) {
+                s.bmin[a] = std::min(s.bmin[a], positions[3 * v + a]);
+                s.bmax[a] = std::max(s.bmax[a], positions[3 * v + a]);
+            }
+        s.stale = true;
+    });
+}
+
+int krt_accel_refit(krt_context *ctx, krt_handle scene) {
+    return guard([&] {
+        KRT_REQUIRE(ctx, KRT_ERR_INVALID, "null context");
+        KRT_HIP(hipSetDevice(ctx->impl.device));
+        Scene &s = ctx->impl.scene(scene);
+        KRT_REQUIRE(s.serve_refs.load() == 0, KRT_ERR_STATE, "scene is used by server requests in flight");
+        accel_refit(ctx->impl, s);
+    });
+}
+
 int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max) {
     return krt_accel_build_layout(ctx, scene, leaf_max, KRT_LAYOUT_SAH);
 }
@@ -528,6 +574,7 @@ int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out) {
         out->depth_bound = A.depth_bound;
         out->build_levels = A.levels;
         out->build_ms = A.build_ms;
+        out->refit_ms = A.refit_ms;
         out->layout = A.layout;
         out->binary_nodes = A.nnodes;
         out->node_bytes = 128;
@@ -838,6 +885,7 @@ int krt_armada_render(krt_armada *a, krt_handle scene, const krt_camera *cam, in
         KRT_HIP(hipSetDevice(C.device));
         Scene &s = C.scene(scene);
         KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+        KRT_REQUIRE(!s.stale, KRT_ERR_STATE, "vertices moved: call krt_accel_refit or krt_accel_build first");
         // attachment switch (armada.cu:747-753)
         if (a->active != a->previous) {
             if (!a->previous.empty() && a->attachments.count(a->previous)) {

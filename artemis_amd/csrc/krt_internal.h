@@ -83,6 +83,8 @@ struct Accel {
     uint32_t depth_bound = 0;
     uint32_t levels = 0;
     float build_ms = 0.0f;
+    float refit_ms = 0.0f;     // last krt_accel_refit
+    uint32_t bin_depth = 0;    // stack bound of the binary traversal tree
     int layout = 0;
     DevBuf ser;        // float4 x 3 per node, bvh.cpp layout
     DevBuf leaf_order; // int per rank
@@ -99,6 +101,8 @@ struct Scene {
     DevBuf tri_mat;    // uint per triangle
     DevBuf mats;       // float4 x 4 per material
     DevBuf lights;     // float4 x 4 per light
+    DevBuf light_tri;  // int per light: its triangle (flat scenes)
+    bool stale = false;  // vertices moved since the last build / refit
     // the editor's area lights (Mamba): emissive triangles grouped by
     // material in order of their first triangle; per light {first, count}
     // into ml_tris, and its emission
@@ -182,6 +186,7 @@ void write_png(const std::string &path, const uint32_t *rgba, uint32_t width, ui
 
 // Builders / kernels (defined in the .hip units)
 void accel_build(Context &ctx, Scene &s, int leaf_max, int layout);
+void accel_refit(Context &ctx, Scene &s);
 void upload_scene_derived(Context &ctx, Scene &s, const float *mats12, uint32_t nm);
 void trace_closest_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, float *hits);
 void trace_occluded_dev(Context &ctx, Scene &s, const float *rays, uint32_t n, int32_t *occ);

@@ -439,6 +439,7 @@ uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w
                         ropt.radius >= 0 && ropt.spatial_passes >= 1,
                     KRT_ERR_INVALID, "restir options out of range");
     KRT_REQUIRE(scene.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
+    KRT_REQUIRE(!scene.stale, KRT_ERR_STATE, "vertices moved: call krt_accel_refit or krt_accel_build first");
     KRT_REQUIRE(w > 0 && h > 0 && (uint64_t)w * h <= (1ull << 26), KRT_ERR_INVALID, "resolution out of range");
     KRT_REQUIRE(spp >= 1, KRT_ERR_INVALID, "spp must be >= 1");
     KRT_REQUIRE(max_depth >= 0 && max_depth <= kServeMaxDepth, KRT_ERR_INVALID, "max_depth must be in [0, 8]");

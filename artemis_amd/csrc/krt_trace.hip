@@ -102,6 +102,7 @@ uint32_t persistent_blocks(uint32_t work, uint32_t block, uint32_t max_blocks) {
 static void trace_common(Context &ctx, Scene &s, const float *rays_dev, uint32_t n, float *hits_dev,
                          int32_t *occ_dev, bool any) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure (call krt_accel_build)");
+    KRT_REQUIRE(!s.stale, KRT_ERR_STATE, "vertices moved: call krt_accel_refit or krt_accel_build first");
     if (n == 0) return;
     uint32_t nb = persistent_blocks(n, kTraceBlock, kMaxResidentBlocks);
     int *spill = spill_buffer(ctx, s, nb * kTraceBlock);
@@ -159,6 +160,7 @@ int *noalloc_scratch() {
 // read.  Calls that overlap on one device share the slab and race.
 void trace_closest_dev_noalloc(Context &ctx, Scene &s, const float *rays_dev, uint32_t n, float *hits_dev) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure (call krt_accel_build)");
+    KRT_REQUIRE(!s.stale, KRT_ERR_STATE, "vertices moved: call krt_accel_refit or krt_accel_build first");
     KRT_REQUIRE(ctx.noalloc_spill != nullptr, KRT_ERR_STATE, "context has no scratch slab address");
     const size_t depth = s.accel.depth_bound > (uint32_t)kLdsDepth ? s.accel.depth_bound - kLdsDepth + 1 : 1;
     const size_t per_block = depth * (size_t)kTraceBlock;

@@ -120,6 +120,7 @@ typedef struct {
     int layout;            /* KRT_LAYOUT_* of the binary tree the wide tree collapses */
     uint32_t binary_nodes; /* binary traversal tree before the 4-wide collapse */
     uint32_t node_bytes;   /* bytes fetched per traversal-node visit */
+    float refit_ms;        /* GPU time of the last krt_accel_refit (0 if none) */
 } krt_bvh_stats;
 
 enum {
@@ -132,6 +133,19 @@ enum {
 int krt_accel_build_layout(krt_context *ctx, krt_handle scene, int leaf_max, int layout);
 int krt_accel_build(krt_context *ctx, krt_handle scene, int leaf_max); /* layout SAH */
 int krt_accel_stats(krt_context *ctx, krt_handle scene, krt_bvh_stats *out);
+/* Moved geometry (Accelerator::update when an entity's transform changes,
+ * accelerator.cuh:145-196; armada.cu:575-602 and :716-727 push the new
+ * transform and rebuild).  krt_scene_update_vertices overwrites the world-space
+ * positions (and normals, when not NULL) of vertices [first, first + count)
+ * of a flat scene; the acceleration structure is then out of date and every
+ * render or query of the scene returns KRT_ERR_STATE until krt_accel_refit
+ * (same tree shapes, boxes recomputed bottom-up from the new vertices, the
+ * triangle records and the triangle lights rewritten) or krt_accel_build.
+ * Either bumps the scene's version, so an armada restarts accumulation
+ * (armada.cu:771-772).  Both refuse a scene held by server requests. */
+int krt_scene_update_vertices(krt_context *ctx, krt_handle scene, uint32_t first, uint32_t count,
+                              const float *positions, const float *normals);
+int krt_accel_refit(krt_context *ctx, krt_handle scene);
 /* bvh.cpp serialised layout, 12 floats (3 x vec4) per node */
 int krt_accel_serialized(krt_context *ctx, krt_handle scene, float *out, size_t nfloats);
 /* primitive id of each leaf in preorder (the closest-hit tie-break rank) */

@@ -205,6 +205,15 @@ public:
         check(krt_accel_build(m_ctx.get(), h, leaf_max));
         return h;
     }
+    // A moved entity: new world-space vertices for [first, first + count),
+    // then the trees refit to them (same shapes) or rebuilt (after a large
+    // motion, where refit boxes grow loose).
+    void update_vertices(krt_handle scene, uint32_t first, uint32_t count, const float *positions,
+                         const float *normals = nullptr) {
+        check(krt_scene_update_vertices(m_ctx.get(), scene, first, count, positions, normals));
+    }
+    void refit(krt_handle scene) { check(krt_accel_refit(m_ctx.get(), scene)); }
+    void rebuild(krt_handle scene, int leaf_max = 4) { check(krt_accel_build(m_ctx.get(), scene, leaf_max)); }
     krt_context *context() const { return m_ctx.get(); }
     Context &ctx() const { return m_ctx; }
 };

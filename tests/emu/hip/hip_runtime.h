@@ -100,6 +100,7 @@ uint64_t wave_exchange(uint64_t mine, int src_lane);  // value of src_lane (0 if
 }  // namespace krt_emu
 
 inline void __syncthreads() { krt_emu::syncthreads(); }
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __syncthreads_count(int pred) { return krt_emu::syncthreads_count(pred); }
 inline unsigned long long __ballot(int pred) { return krt_emu::wave_ballot(pred); }
 

@@ -256,7 +256,54 @@ def case_envmap(ctx):
     sc.free()
 
 
-CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "instancing": case_instancing,
+def case_refit(ctx):
+    """Moved geometry: update_vertices makes the scene stale (renders refuse),
+    refit gives the numpy restatement's tree bit for bit and frames equal the
+    oracle's over the moved geometry and the refit tree; a light moves too."""
+    from tests.test_oracle_refit import moved, refit_serialized
+
+    s = scenes.sky_room()
+    sc = ctx.upload(s)
+    old = sc.serialized()
+    new, first, count = moved(s, 2, (0.3, 0.1, -0.2))
+    new, lfirst, lcount = moved(new, 3, (0.0, 0.5, 0.1))  # the ceiling light
+    sc.update_vertices(first, new.positions[first:first + count])
+    sc.update_vertices(lfirst, new.positions[lfirst:lfirst + lcount])
+    W, H = 24, 16
+    cam = look_at(**s.camera, aspect=W / H)
+    arm = ArmadaRTX(ctx, W, H)
+    arm.attach("pt", PathTracer(ctx))
+    arm.set_depth(4)
+    try:
+        arm.render(sc, cam)
+        raise AssertionError("a stale scene rendered")
+    except KrtError as e:
+        assert e.code == KRT_ERR_STATE, e
+    sc.refit()
+    b = O.tri_boxes(new.positions, new.indices)
+    want = refit_serialized(old, b[:, 0:3], b[:, 3:6])
+    assert bits_equal(sc.serialized(), want), "refit tree"
+    osc = O.OracleScene(new, want)
+    assert bits_equal(sc.lights(), osc.lights()), "moved light"
+    for _ in range(2):
+        arm.render(sc, cam)
+    ref, _ = osc.render_pt(cam, W, H, frames=2, max_depth=4)
+    assert bits_equal(arm.color_buffer(), ref[0]), "path tracer over the refit tree"
+    say("refit: tree = numpy refit, lights moved, path tracer bit-exact")
+    rays = np.zeros((2000, 8), np.float32)
+    rng = np.random.default_rng(2)
+    rays[:, :3] = rng.uniform(-3, 3, (2000, 3)) + np.array([0, 2, 0])
+    d = rng.normal(size=(2000, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3] = 1e16
+    want_h, _ = osc.trace(rays, mode=0)
+    assert bits_equal(sc.trace_closest(rays), want_h), "closest hits"
+    say("refit: 2,000 closest hits bit-exact")
+    del arm
+    sc.free()
+
+
+CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "refit": case_refit, "instancing": case_instancing,
          "instancing_full": lambda ctx: case_instancing(ctx, full=True)}
 
 if __name__ == "__main__":
