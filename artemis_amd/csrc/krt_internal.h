@@ -340,6 +340,7 @@ struct Mlp {
     DevBuf frag_f, frag_b;      // bf16 MFMA fragments of W and W^T
     DevBuf state;               // MlpDevState
     DevBuf partial;             // per-workgroup gradient partials
+    DevBuf mid;                 // sums of groups of partials
 };
 
 void mlp_init(Context &ctx, Mlp &net, const MlpConfig &cfg, uint64_t seed);

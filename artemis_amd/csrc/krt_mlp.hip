@@ -61,59 +61,75 @@ static_assert(w_off(kMlpLayers) == kMlpParams, "parameter count");
 
 KRT_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
-KRT_DEV float pick9(const float v[9], int i) {
-    float r = v[0];
-#pragma unroll
-    for (int a = 1; a < 9; a++) r = (i == a) ? v[a] : r;
-    return r;
+// Layer 0's inputs in fragment order.  Fragment input k = 32c + 8g + j
+// (chunk c, lane group g, element j) holds encoded feature enc_feature(k):
+// every lane group computes six of the 24 sin/cos pairs (features 2p, 2p + 1
+// of pair p) and four raw inputs, so the sine work is spread evenly:
+//   chunk 0: pairs 6g .. 6g + 3      (features 12g .. 12g + 7)
+//   chunk 1: pairs 6g + 4, 6g + 5    (features 12g + 8 .. 12g + 11),
+//            raw features 48 + 4g .. 48 + 4g + 3
+// Features 0..47 are the sin/cos pairs, 48..60 the raw inputs (pos mapped to
+// [0, 1]), 61 the constant 1, 62..63 zero.  The weights are packed, and the
+// layer-0 gradient written, through the same map.
+__host__ __device__ constexpr int enc_feature(int k) {
+    const int c = k >> 5, g = (k >> 3) & 3, j = k & 7;
+    return c == 0 ? 12 * g + j : (j < 4 ? 12 * g + 8 + j : 48 + 4 * g + (j - 4));
 }
 
-// Turns of frequency pair p (0..23): pos x 4 octaves, dir x 2, normal x 2.
-KRT_DEV float pair_turns(int p, const float pn[9]) {
-    int vec = p < 12 ? 0 : (p < 18 ? 1 : 2);
-    int base = vec == 0 ? 0 : (vec == 1 ? 12 : 18);
-    int comp = (p - base) % 3, k = (p - base) / 3;
-    return pick9(pn, 3 * vec + comp) * (0.5f * (float)(1 << k));
+// Turns of frequency pair p (0..23): pos x 4 octaves, dir x 2, normal x 2,
+// from the query's 13 mapped inputs q (pos already in [0, 1]).
+KRT_DEV float pair_turns(int p, const float *q) {
+    const int vec = p < 12 ? 0 : (p < 18 ? 1 : 2);
+    const int base = vec == 0 ? 0 : (vec == 1 ? 12 : 18);
+    const int comp = (p - base) % 3, k = (p - base) / 3;
+    return q[3 * vec + comp] * (0.5f * (float)(1 << k));
 }
 
-// The 16 features lane group g owns for one query: chunk c holds features
-// 32c + 8g + j.  Features 0..47 are sin/cos pairs, 48..60 the raw inputs with
-// pos mapped to [0, 1], 61 the constant 1, 62..63 zero.
-KRT_DEV void encode(const float *__restrict__ raw, bool valid, const MlpEncoding &enc, int g, bf16x8 &b0,
-                    bf16x8 &b1) {
-    float x[kMlpRaw];
+// The 13 raw floats of 16 consecutive queries (one MFMA tile) are 208
+// contiguous floats: the wave's 64 lanes load them coalesced (lane l takes
+// floats l, l + 64, l + 128, l + 192), then stage them into the wave's LDS
+// slice with pos mapped to [0, 1].
+KRT_DEV void load_tile(const float *__restrict__ raw, uint32_t q0, uint32_t n, int lane, float (&v)[4]) {
 #pragma unroll
-    for (int a = 0; a < kMlpRaw; a++) x[a] = valid ? raw[a] : 0.0f;
-    float pn[9];
+    for (int i = 0; i < 4; i++) {
+        const int e = lane + 64 * i;
+        const uint32_t q = q0 + (uint32_t)(e / kMlpRaw);
+        v[i] = (e < 16 * kMlpRaw && q < n) ? raw[(size_t)kMlpRaw * q0 + e] : 0.0f;
+    }
+}
+
+KRT_DEV void stage_tile(const float (&v)[4], const MlpEncoding &enc, int lane, float *tile) {
 #pragma unroll
-    for (int a = 0; a < 3; a++) pn[a] = (x[a] - enc.pmin[a]) * enc.pscale[a];
-#pragma unroll
-    for (int a = 3; a < 9; a++) pn[a] = x[a];
+    for (int i = 0; i < 4; i++) {
+        const int e = lane + 64 * i;
+        if (e < 16 * kMlpRaw) {
+            const int a = e % kMlpRaw;
+            tile[e] = a < 3 ? (v[i] - enc.pmin[a]) * enc.pscale[a] : v[i];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The 16 features lane group g owns for one query (q: its 13 mapped inputs).
+KRT_DEV void encode(const float *q, int g, bf16x8 &b0, bf16x8 &b1) {
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
         float c, s;
-        sincos_turn(pair_turns(4 * g + jj, pn), c, s);
+        sincos_turn(pair_turns(6 * g + jj, q), c, s);
         b0[2 * jj] = (__bf16)s;
         b0[2 * jj + 1] = (__bf16)c;
     }
-    if (g < 2) {
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            float c, s;
-            sincos_turn(pair_turns(16 + 4 * g + jj, pn), c, s);
-            b1[2 * jj] = (__bf16)s;
-            b1[2 * jj + 1] = (__bf16)c;
-        }
-    } else {
-        float rv[13] = {pn[0], pn[1], pn[2], x[3], x[4], x[5], x[6], x[7], x[8], x[9], x[10], x[11], x[12]};
+    for (int jj = 0; jj < 2; jj++) {
+        float c, s;
+        sincos_turn(pair_turns(6 * g + 4 + jj, q), c, s);
+        b1[2 * jj] = (__bf16)s;
+        b1[2 * jj + 1] = (__bf16)c;
+    }
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            int r = 8 * (g - 2) + j;  // feature 48 + r
-            float f = (r == 13) ? 1.0f : 0.0f;
-#pragma unroll
-            for (int a = 0; a < 13; a++) f = (r == a) ? rv[a] : f;
-            b1[j] = (__bf16)f;
-        }
+    for (int jj = 0; jj < 4; jj++) {
+        const int r = 4 * g + jj;  // feature 48 + r
+        b1[4 + jj] = (__bf16)(r < kMlpRaw ? q[r] : (r == kMlpRaw ? 1.0f : 0.0f));
     }
 }
 
@@ -136,7 +152,7 @@ __global__ void pack_kernel(const float *__restrict__ params, __bf16 *__restrict
     if (frag < kFragF) {
         const int L = frag < 40 ? frag / 8 : 5, loc = frag - 8 * L, m = loc / 2, c = loc % 2;
         const int o = 16 * m + r;
-        const int k = (L == 0) ? 32 * c + 8 * g + j : 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3);
+        const int k = (L == 0) ? enc_feature(32 * c + 8 * g + j) : 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3);
         if (o < out_dim(L) && k < in_dim(L)) val = params[w_off(L) + o * in_dim(L) + k];
         fragF[e] = (__bf16)val;
     } else {
@@ -159,22 +175,36 @@ __global__ void pack_kernel(const float *__restrict__ params, __bf16 *__restrict
 // Inference: persistent grid, forward fragments resident in LDS, 64 queries
 // (4 tiles) per wave iteration.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kInferBlock) void infer_kernel(const bf16x8 *__restrict__ fragF,
+__global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(3))) void infer_kernel(const bf16x8 *__restrict__ fragF,
                                                              const float *__restrict__ params,
                                                              const float *__restrict__ raw, uint32_t n,
                                                              float *__restrict__ out, MlpEncoding enc) {
     __shared__ bf16x8 sF[kFragF * 64];
+    __shared__ float sBias[5 * kMlpHidden + 4];  // hidden layers' biases, then the output's
+    __shared__ float sQ[kInferBlock / 64][16 * kMlpRaw];  // each wave's staged query tile
     for (int i = threadIdx.x; i < kFragF * 64; i += kInferBlock) sF[i] = fragF[i];
+    for (int i = threadIdx.x; i < 5 * kMlpHidden + 4; i += kInferBlock)
+        sBias[i] = i < 5 * kMlpHidden ? params[b_off(i / kMlpHidden) + i % kMlpHidden]
+                                      : (i - 5 * kMlpHidden < kMlpOut ? params[b_off(5) + i - 5 * kMlpHidden] : 0.0f);
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     const uint32_t wave = blockIdx.x * (kInferBlock / 64) + (threadIdx.x >> 6);
     const uint32_t nwaves = gridDim.x * (kInferBlock / 64);
+    // the next iteration's queries are loaded while this one computes
+    float cur[4][4], nxt[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) load_tile(raw, wave * 64 + 16 * t, n, lane, cur[t]);
     for (uint32_t base = wave * 64; base < n; base += nwaves * 64) {
+        const uint32_t next = base + nwaves * 64;
+#pragma unroll
+        for (int t = 0; t < 4; t++) load_tile(raw, next + 16 * t, n, lane, nxt[t]);
         bf16x8 B[2][4];
+        float *tile = sQ[threadIdx.x >> 6];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const uint32_t q = base + 16 * t + r;
-            encode(raw + (size_t)kMlpRaw * (q < n ? q : 0), q < n, enc, g, B[0][t], B[1][t]);
+            stage_tile(cur[t], enc, lane, tile);
+            encode(tile + kMlpRaw * r, g, B[0][t], B[1][t]);
+            __builtin_amdgcn_wave_barrier();  // the next tile overwrites the slice
         }
         f32x4 acc[4][4];
 #pragma unroll
@@ -183,7 +213,7 @@ __global__ __launch_bounds__(kInferBlock) void infer_kernel(const bf16x8 *__rest
             for (int m = 0; m < 4; m++) {
                 f32x4 bias;
 #pragma unroll
-                for (int v = 0; v < 4; v++) bias[v] = params[b_off(L) + 16 * m + 4 * g + v];
+                for (int v = 0; v < 4; v++) bias[v] = sBias[kMlpHidden * L + 16 * m + 4 * g + v];
 #pragma unroll
                 for (int t = 0; t < 4; t++) acc[m][t] = bias;
             }
@@ -204,7 +234,7 @@ __global__ __launch_bounds__(kInferBlock) void infer_kernel(const bf16x8 *__rest
         }
         f32x4 bias5;
 #pragma unroll
-        for (int v = 0; v < 4; v++) bias5[v] = (g == 0 && v < kMlpOut) ? params[b_off(5) + v] : 0.0f;
+        for (int v = 0; v < 4; v++) bias5[v] = g == 0 ? sBias[5 * kMlpHidden + v] : 0.0f;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             f32x4 y = bias5;
@@ -216,6 +246,10 @@ __global__ __launch_bounds__(kInferBlock) void infer_kernel(const bf16x8 *__rest
                 for (int v = 0; v < kMlpOut; v++) out[(size_t)kMlpOut * q + v] = y[v];
             }
         }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) cur[t][i] = nxt[t][i];
     }
 }
 
@@ -239,6 +273,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 sH[6 * 64 * kStride];
     __shared__ __attribute__((aligned(16))) __bf16 sD[2 * 64 * kStride];
     __shared__ float sLoss[kTrainBlock / 64];
+    __shared__ float sQ[kTrainBlock / 64][16 * kMlpRaw];  // each wave's staged sample tile
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
     const uint32_t n = a.n_dev ? *a.n_dev : a.n;
     if (blockIdx.x * kTrainSamples >= n) return;  // beyond the batch: no partial
@@ -252,7 +287,11 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
         const uint32_t q = s0 + 16 * t + r;
         valid[t] = q < n;
         col[t] = 32 * w + 16 * t + r;
-        encode(a.raw + (size_t)kMlpRaw * (valid[t] ? q : 0), valid[t], a.enc, g, B[0][t], B[1][t]);
+        float v[4];
+        load_tile(a.raw, s0 + 16 * t, n, lane, v);
+        stage_tile(v, a.enc, lane, sQ[w]);
+        encode(sQ[w] + kMlpRaw * r, g, B[0][t], B[1][t]);
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -341,7 +380,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
                 const bf16x8 Bh = *reinterpret_cast<const bf16x8 *>(H + (16 * w + r) * kStride + 32 * qq + 8 * g);
                 gw = mfma(A, Bh, gw);
             }
-            const int i = 16 * w + r;
+            const int i = L == 0 ? enc_feature(16 * w + r) : 16 * w + r;  // layer 0: fragment input -> feature
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int o = 16 * m + 4 * g + v;
@@ -390,10 +429,26 @@ __global__ __launch_bounds__(kTrainBlock) void train_kernel(TrainArgs a) {
     if (tid == 0) part[kMlpParams] = ((sLoss[0] + sLoss[1]) + sLoss[2]) + sLoss[3];
 }
 
-// Sum of the workgroup partials in workgroup order, unscaled; flags
-// non-finite gradients (bad = 1) and an empty batch (bad = 2: no step, no
-// scale change).
-__global__ void reduce_kernel(const float *__restrict__ partial, uint32_t n_host, const uint32_t *n_dev,
+// Sum of the workgroup partials in a fixed order (deterministic): groups of
+// kReduceGroup consecutive partials are summed in workgroup order
+// (reduce_groups_kernel, one grid row per group), then the group sums in
+// group order (reduce_kernel), unscaled; flags non-finite gradients (bad = 1)
+// and an empty batch (bad = 2: no step, no scale change).
+constexpr int kReduceGroup = 32;
+
+__global__ void reduce_groups_kernel(const float *__restrict__ partial, uint32_t n_host, const uint32_t *n_dev,
+                                     float *__restrict__ mid) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    const uint32_t n = n_dev ? *n_dev : n_host;
+    const int nparts = (int)((n + kTrainSamples - 1) / kTrainSamples);
+    const int k0 = (int)blockIdx.y * kReduceGroup, k1 = min(k0 + kReduceGroup, nparts);
+    float s = 0.0f;
+    for (int k = k0; k < k1; k++) s += partial[(size_t)k * (kMlpParams + 1) + p];
+    mid[(size_t)blockIdx.y * (kMlpParams + 1) + p] = s;
+}
+
+__global__ void reduce_kernel(const float *__restrict__ mid, int ngroups, uint32_t n_host, const uint32_t *n_dev,
                               MlpDevState *st, float *__restrict__ grad) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > kMlpParams) return;
@@ -405,9 +460,8 @@ __global__ void reduce_kernel(const float *__restrict__ partial, uint32_t n_host
         }
         return;
     }
-    const int nparts = (int)((n + kTrainSamples - 1) / kTrainSamples);
     float s = 0.0f;
-    for (int k = 0; k < nparts; k++) s += partial[(size_t)k * (kMlpParams + 1) + p];
+    for (int k = 0; k < ngroups; k++) s += mid[(size_t)k * (kMlpParams + 1) + p];
     if (p == kMlpParams) {
         st->loss = s / (float)(3u * n);
         return;
@@ -560,7 +614,12 @@ void mlp_gradient_dev(Context &ctx, Mlp &net, const float *raw, const float *tar
     TrainArgs a{net.frag_f.as<bf16x8>(), net.frag_b.as<bf16x8>(), net.params.as<float>(), raw, target, n, net.enc,
                 net.state.as<MlpDevState>(), net.partial.as<float>(), n_dev};
     train_kernel<<<nb, kTrainBlock, 0, ctx.stream>>>(a);
-    reduce_kernel<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx.stream>>>(net.partial.as<float>(), n, n_dev,
+    const int ngroups = (int)((nb + kReduceGroup - 1) / kReduceGroup);
+    const size_t mid_bytes = sizeof(float) * (size_t)ngroups * (kMlpParams + 1);
+    if (net.mid.bytes < mid_bytes) net.mid.alloc(mid_bytes);
+    reduce_groups_kernel<<<dim3((kMlpParams + 1 + 255) / 256, ngroups), 256, 0, ctx.stream>>>(
+        net.partial.as<float>(), n, n_dev, net.mid.as<float>());
+    reduce_kernel<<<(kMlpParams + 1 + 255) / 256, 256, 0, ctx.stream>>>(net.mid.as<float>(), ngroups, n, n_dev,
                                                                         net.state.as<MlpDevState>(),
                                                                         net.grad.as<float>());
     KRT_HIP(hipGetLastError());

@@ -303,7 +303,20 @@ def case_refit(ctx):
     sc.free()
 
 
-CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "refit": case_refit, "instancing": case_instancing,
+def case_mlp(ctx):
+    """The radiance-cache network's inference and gradient against the
+    oracle's BF16 mode (the GPU tests' checks, on the emulated MFMA)."""
+    from tests import test_gpu_mlp as T
+
+    for n in (1, 65, 1000):
+        T.test_infer_matches_oracle(ctx, n)
+    for n in (1, 300):
+        T.test_gradient_matches_oracle(ctx, n)
+    say("mlp: inference and gradients within the BF16 bounds of the oracle")
+
+
+CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "refit": case_refit,
+         "mlp": case_mlp, "instancing": case_instancing,
          "instancing_full": lambda ctx: case_instancing(ctx, full=True)}
 
 if __name__ == "__main__":

@@ -10,6 +10,7 @@ behaviour, or a result that depends on memory nothing wrote, fails it.
   the editor's G-buffer ReSTIR, a served request, the cached path tracer;
 * envmap: the path tracer and ReSTIR DI with the armada's environment map;
 * refit: moved vertices, the bottom-up refit of both trees and the lights;
+* mlp: the radiance-cache network's inference and gradients (emulated MFMA);
 * instancing: the sequence of tests/test_gpu_instancing.py on the two-level
   path (which the GPU build keeps switched off): per-mesh and instance trees,
   closest hits and occlusion, path-tracer frames on both tree layouts,
@@ -44,7 +45,7 @@ def emu_env():
     return env
 
 
-@pytest.mark.parametrize("case", ["attachments", "envmap", "refit", "instancing"])
+@pytest.mark.parametrize("case", ["attachments", "envmap", "refit", "mlp", "instancing"])
 def test_sanitized_case(emu_env, case):
     r = subprocess.run([sys.executable, str(EMU / "run_case.py"), case], env=emu_env, capture_output=True,
                        text=True, timeout=900)
