@@ -457,7 +457,8 @@ class Server:
     resident per request, admission by HBM budget)."""
 
     def __init__(self, ctx: Context, hbm_budget: int | None = None, max_active: int | None = None,
-                 use_graph: bool = True, max_queue_ms: float = 0.0, step_pixels: int | None = None):
+                 use_graph: bool = True, max_queue_ms: float = 0.0, step_pixels: int | None = None,
+                 wave_pixels: int | None = None):
         self.ctx = ctx
         cfg = L.ServerConfig()
         ctx._lib.krt_server_default_config(C.byref(cfg))
@@ -469,6 +470,8 @@ class Server:
         cfg.max_queue_ms = max_queue_ms
         if step_pixels is not None:
             cfg.step_pixels = step_pixels
+        if wave_pixels is not None:
+            cfg.wave_pixels = wave_pixels
         h = C.c_void_p()
         L.check(ctx._lib.krt_server_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

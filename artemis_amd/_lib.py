@@ -79,7 +79,8 @@ MLP_RAW, MLP_OUT, MLP_PARAMS = 13, 3, 20867
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_uint32), ("use_graph", C.c_int32),
-                ("max_queue_ms", C.c_double), ("step_pixels", C.c_uint64)]
+                ("max_queue_ms", C.c_double), ("step_pixels", C.c_uint64),
+                ("wave_pixels", C.c_uint64)]
 
 
 class RestirOptions(C.Structure):

@@ -1306,6 +1306,7 @@ void krt_server_default_config(krt_server_config *cfg) {
     cfg->use_graph = d.use_graph;
     cfg->max_queue_ms = d.max_queue_ms;
     cfg->step_pixels = d.step_pixels;
+    cfg->wave_pixels = d.wave_pixels;
 }
 
 int krt_server_create(krt_context *ctx, const krt_server_config *cfg, krt_server **out) {
@@ -1321,6 +1322,7 @@ int krt_server_create(krt_context *ctx, const krt_server_config *cfg, krt_server
         sc.max_queue_ms = c.max_queue_ms;
         KRT_REQUIRE(c.step_pixels >= 1, KRT_ERR_INVALID, "step_pixels must be >= 1");
         sc.step_pixels = c.step_pixels;
+        sc.wave_pixels = c.wave_pixels;
         auto srv = std::make_unique<krt_server>();
         srv->ctx = ctx;
         srv->impl = serve_create(ctx->impl, sc);

@@ -287,6 +287,11 @@ struct CacheView;
 // `queried` (when given) on ctx.stream once every query record is written.
 void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, double *acc64,
              const CacheView *cv, float4 *prefix, hipEvent_t queried = nullptr);
+// Grows each buffer of `ws` to what a path-tracer frame of w x h pixels at
+// max_depth needs (dry_run: only counts) and returns the bytes added.
+size_t wave_scratch_grow(WaveScratch &ws, uint32_t w, uint32_t h, int max_depth, bool instanced, bool cache,
+                         bool dry_run = false);
+size_t wave_scratch_bytes(const WaveScratch &ws);
 void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_t user_seed);
 void restir_read(Context &ctx, RestirState &st, int which, float *host_out);
 void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_seed);

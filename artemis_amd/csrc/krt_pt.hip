@@ -293,6 +293,74 @@ __global__ __launch_bounds__(kWaveBlock) void pt_compose_kernel(LaunchInfo li, P
 
 }  // namespace
 
+// Workgroups of the wavefront passes: whole 8x8 tiles, 256 lanes a group.
+static uint32_t wave_blocks(uint32_t w, uint32_t h) {
+    const uint32_t items = ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
+    return (items + kWaveBlock - 1) / kWaveBlock;
+}
+
+// The size of each wavefront-scratch buffer for a frame of w x h pixels:
+// hits, instance ids, path state, per-level history (3 float4) and occlusion
+// bytes, cache-vertex emission, and per ray list (closest, shadow) the rays,
+// segment counts, offsets and the packed copy.  `b` is filled in the order of
+// the buffers of WaveScratch; zero means the buffer is not needed.
+static void wave_sizes(uint32_t w, uint32_t h, int max_depth, bool instanced, bool cache, size_t b[14]) {
+    const size_t npix = (size_t)w * h;
+    const uint32_t gb = wave_blocks(w, h);
+    const size_t levels = (size_t)max_depth + 1;
+    const size_t cap = (size_t)gb * kWaveBlock;
+    b[0] = sizeof(float4) * npix;
+    b[1] = instanced ? sizeof(int) * npix : 0;
+    b[2] = sizeof(float4) * npix;
+    b[3] = sizeof(float4) * 3 * levels * npix;
+    b[4] = levels * npix;
+    b[5] = cache ? sizeof(float4) * npix : 0;
+    for (int l = 0; l < 2; l++) {
+        b[6 + 4 * l] = sizeof(float4) * 2 * cap;
+        b[7 + 4 * l] = sizeof(uint32_t) * gb;
+        b[8 + 4 * l] = sizeof(uint32_t) * (gb + 1);
+        b[9 + 4 * l] = sizeof(float4) * 2 * cap;
+    }
+}
+
+static void wave_bufs(WaveScratch &ws, DevBuf *b[14]) {
+    b[0] = &ws.hits;
+    b[1] = &ws.inst;
+    b[2] = &ws.state;
+    b[3] = &ws.hist;
+    b[4] = &ws.occ;
+    b[5] = &ws.lec;
+    for (int l = 0; l < 2; l++) {
+        b[6 + 4 * l] = &ws.rays[l];
+        b[7 + 4 * l] = &ws.counts[l];
+        b[8 + 4 * l] = &ws.prefix[l];
+        b[9 + 4 * l] = &ws.dense[l];
+    }
+}
+
+size_t wave_scratch_grow(WaveScratch &ws, uint32_t w, uint32_t h, int max_depth, bool instanced, bool cache,
+                         bool dry_run) {
+    size_t need[14];
+    DevBuf *bufs[14];
+    wave_sizes(w, h, max_depth, instanced, cache, need);
+    wave_bufs(ws, bufs);
+    size_t grown = 0;
+    for (int i = 0; i < 14; i++)
+        if (need[i] && bufs[i]->bytes < need[i]) {
+            grown += need[i] - bufs[i]->bytes;
+            if (!dry_run) bufs[i]->alloc(need[i]);
+        }
+    return grown;
+}
+
+size_t wave_scratch_bytes(const WaveScratch &ws) {
+    DevBuf *bufs[14];
+    wave_bufs(const_cast<WaveScratch &>(ws), bufs);
+    size_t b = 0;
+    for (DevBuf *d : bufs) b += d->bytes;
+    return b;
+}
+
 void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *ctr, double *acc64,
              const CacheView *cv, float4 *prefix, hipEvent_t queried) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
@@ -301,25 +369,8 @@ void pt_wave(Context &ctx, Scene &s, const LaunchInfo &li, unsigned long long *c
     const CacheView cview = cv ? *cv : CacheView{};
     const SceneView sv = make_view(s);
     WaveScratch &ws = ctx.wave;
-    const uint32_t items = ((li.width + 7u) / 8u) * ((li.height + 7u) / 8u) * 64u;
-    const uint32_t gb = (items + kWaveBlock - 1) / kWaveBlock;
-    const size_t levels = (size_t)li.max_depth + 1;
-    const size_t cap = (size_t)gb * kWaveBlock;
-    auto grow = [](DevBuf &b, size_t bytes) {
-        if (b.bytes < bytes) b.alloc(bytes);
-    };
-    grow(ws.hits, sizeof(float4) * npix);
-    if (s.instanced) grow(ws.inst, sizeof(int) * npix);
-    grow(ws.state, sizeof(float4) * npix);
-    grow(ws.hist, sizeof(float4) * 3 * levels * npix);
-    grow(ws.occ, levels * npix);
-    if (cview.enabled) grow(ws.lec, sizeof(float4) * npix);
-    for (int l = 0; l < 2; l++) {
-        grow(ws.rays[l], sizeof(float4) * 2 * cap);
-        grow(ws.counts[l], sizeof(uint32_t) * gb);
-        grow(ws.prefix[l], sizeof(uint32_t) * (gb + 1));
-        grow(ws.dense[l], sizeof(float4) * 2 * cap);
-    }
+    const uint32_t gb = wave_blocks(li.width, li.height);
+    wave_scratch_grow(ws, li.width, li.height, li.max_depth, s.instanced, cview.enabled);
     spill_buffer(ctx, s, 2 * kMaxResidentBlocks * kTraceBlock, kLdsDepthLean);
     int *spill = ctx.spill.as<int>();
     PtWave B;

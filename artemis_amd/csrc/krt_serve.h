@@ -19,9 +19,12 @@ struct ServeConfig {
     int use_graph = 1;
     double max_queue_ms = 0.0;  // > 0: queued requests older than this are rejected
     uint64_t step_pixels = 1ull << 22;  // pixel-frames one step aims at, shared fairly
+    // path-tracer requests of at least this many pixels run the wavefront
+    // passes on a lane; smaller ones share the persistent batched launch
+    uint64_t wave_pixels = 1ull << 18;
 };
 
-constexpr int kServeLanes = 3;  // streams that run ReSTIR requests' frames side by side
+constexpr int kServeLanes = 3;  // streams that run lane requests' frames side by side
 
 struct Server;
 Server *serve_create(Context &ctx, const ServeConfig &cfg);

@@ -7,21 +7,27 @@
 // source/amadeus/armada.cu:733-780, with the estimators of
 // source/optix/amadeus_path_tracer.cu:73-228 and amadeus_restir.cu).
 //
-// One host thread runs the loop.  Each step advances every active request:
-//  * path-tracer requests in a single persistent launch: the grid strides
-//    over the concatenated pixels of all of them and finds each pixel's
-//    request by binary search in a device table {LaunchInfo, SceneView, first
-//    pixel, frames}.  A request's share of a step is a fair share of
-//    `step_pixels` pixel-frames: small requests run several consecutive
-//    frames of a pixel in one thread (the accumulation is per pixel, so the
-//    result is the one a launch per frame gives) and finish in one step; large
-//    ones advance a frame per step.  The step (table upload + kernel) is a
-//    hipGraph captured once; only the table's contents change between steps.
-//  * ReSTIR DI requests frame by frame through the attachment's own passes
-//    (restir_render), each request on one of kServeLanes streams for its whole
-//    life (its frames stay in order), the lanes running side by side with the
-//    path-tracer launch.  A request's reservoirs, G-buffer and ray lists are
-//    allocated when it is admitted and stay resident until its last frame.
+// One host thread runs the loop.  Each step advances every active request by
+// its fair share of `step_pixels` pixel-frames (at least one frame):
+//  * small path-tracer requests (fewer than `wave_pixels` pixels) in a single
+//    persistent launch: the grid strides over the concatenated pixels of all
+//    of them and finds each pixel's request by binary search in a device
+//    table {LaunchInfo, SceneView, first pixel, frames}.  Small requests run
+//    several consecutive frames of a pixel in one thread (the accumulation is
+//    per pixel, so the result is the one a launch per frame gives) and finish
+//    in one step.  The step (table upload + kernel) is a hipGraph captured
+//    once; only the table's contents change between steps.  A request this
+//    small cannot fill the device on its own, so batching them is what keeps
+//    it busy.
+//  * large path-tracer requests frame by frame through the attachment's own
+//    wavefront passes (pt_wave: ray lists sorted by 8x8 tile, one list walk
+//    per level), and ReSTIR DI requests through theirs (restir_render).  Each
+//    such request runs on one of kServeLanes streams for its whole life (its
+//    frames stay in order), the lanes side by side with the batched launch.
+//    Its wavefront buffers (path tracer) or reservoirs, G-buffer and ray lists
+//    (ReSTIR) are allocated when it is admitted and stay resident until its
+//    last frame; the path tracer's are swapped into the lane's context for
+//    the frames it enqueues.
 // Requests arriving while others run join at the next step as soon as a slot
 // and HBM budget are free (FIFO admission, optional queueing deadline);
 // finished requests leave after their last frame and their buffers are
@@ -102,7 +108,9 @@ struct ServeRequest {
     int kind = KRT_REQUEST_PATH_TRACER;
     RestirOptions ropt;
     RestirState rst;  // ReSTIR: reservoirs, G-buffer and ray lists, resident while active
-    int lane = 0;     // ReSTIR: the stream its frames run on
+    bool on_lane = false;  // ReSTIR, or a path-tracer request of at least wave_pixels pixels
+    int lane = 0;          // the stream its frames run on
+    WaveScratch wave;      // path tracer on a lane: its wavefront buffers, resident while active
     krt_camera cam{};
     int state = 0;  // 0 queued, 1 active, 2 done, 3 rejected
     uint32_t frames = 0, step_frames = 1;
@@ -123,8 +131,8 @@ struct Server {
     Context *ctx = nullptr;
     ServeConfig cfg;
     hipStream_t stream = nullptr;
-    // ReSTIR lanes: a context shell per stream (its stream, traversal-stack
-    // spill slab and counters), so that requests on different lanes share no
+    // lanes: a context shell per stream (its stream, traversal-stack spill
+    // slab and counters), so that requests on different lanes share no
     // scratch
     std::unique_ptr<Context> lanes[kServeLanes];
     hipEvent_t lane_done[kServeLanes] = {};
@@ -145,6 +153,7 @@ struct Server {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     size_t scratch = 0, in_use = 0, peak = 0;
+    size_t fixed_scratch = 0;  // the batched launch's table, history and spill slabs
     // statistics
     std::vector<double> latency, step_ms;
     uint64_t submitted = 0, completed = 0, rejected = 0, steps = 0, max_concurrent = 0;
@@ -175,15 +184,16 @@ size_t serve_restir_bytes(uint32_t w, uint32_t h, const RestirOptions &o, int de
 }
 
 void Server::step(std::vector<ServeRequest *> &all_batch) {
-    std::vector<ServeRequest *> batch, restir;
-    for (ServeRequest *r : all_batch) (r->kind == KRT_REQUEST_RESTIR ? restir : batch).push_back(r);
+    std::vector<ServeRequest *> batch, laned;
+    for (ServeRequest *r : all_batch) (r->on_lane ? laned : batch).push_back(r);
     KRT_HIP(hipEventRecord(ev[0], stream));
-    // ReSTIR requests: the frames of this step, queued on their lanes
+    // requests on lanes: the frames of this step, queued on their streams
     bool used[kServeLanes] = {};
-    for (ServeRequest *r : restir) {
+    for (ServeRequest *r : laned) {
         Context &lane = *lanes[r->lane];
         if (!used[r->lane]) KRT_HIP(hipStreamWaitEvent(lane.stream, ev[0], 0));
         used[r->lane] = true;
+        if (r->kind == KRT_REQUEST_PATH_TRACER) std::swap(lane.wave, r->wave);
         for (uint32_t f = 0; f < r->step_frames; f++) {
             LaunchInfo li{};
             li.width = r->w;
@@ -204,9 +214,14 @@ void Server::step(std::vector<ServeRequest *> &all_batch) {
             li.normal = planes + n;
             li.albedo = planes + 2 * n;
             li.position = planes + 3 * n;
-            PassTimes times;
-            restir_render(lane, *r->scene, li, r->ropt, r->rst, nullptr, times, nullptr, nullptr, false);
+            if (r->kind == KRT_REQUEST_RESTIR) {
+                PassTimes times;
+                restir_render(lane, *r->scene, li, r->ropt, r->rst, nullptr, times, nullptr, nullptr, false);
+            } else {
+                pt_wave(lane, *r->scene, li, nullptr, nullptr, nullptr, nullptr);
+            }
         }
+        if (r->kind == KRT_REQUEST_PATH_TRACER) std::swap(lane.wave, r->wave);
     }
     for (int l = 0; l < kServeLanes; l++)
         if (used[l]) {
@@ -291,12 +306,15 @@ void Server::run() {
                 // frame 0 accumulates dst * 0 + sample: recycled memory holding
                 // a non-finite value would poison it (stream-ordered, outside the
                 // captured step)
-                if (r->kind == KRT_REQUEST_RESTIR) {
+                if (r->on_lane) {
                     r->lane = next_lane;
                     next_lane = (next_lane + 1) % kServeLanes;
                     Context &lane = *lanes[r->lane];
                     KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->aov.bytes, lane.stream));
-                    restir_alloc(lane, r->rst, r->w, r->h, r->seed);  // reservoirs resident from here on
+                    if (r->kind == KRT_REQUEST_RESTIR)
+                        restir_alloc(lane, r->rst, r->w, r->h, r->seed);  // reservoirs resident from here on
+                    else
+                        wave_scratch_grow(r->wave, r->w, r->h, r->max_depth, r->scene->instanced, false);
                 } else {
                     KRT_HIP(hipMemsetAsync(r->aov.p, 0, r->aov.bytes, stream));
                 }
@@ -333,8 +351,14 @@ void Server::run() {
             for (ServeRequest *r : finished) {
                 r->aov.release();
                 r->rst = RestirState();
+                r->wave = WaveScratch();
             }
+            // the lanes' traversal-stack slabs grow with the deepest scene
+            // they traversed: counted as fixed scratch from then on
+            size_t lane_bytes = 0;
+            for (auto &l : lanes) lane_bytes += l->spill.bytes + l->counters.bytes;
             lk.lock();
+            scratch = fixed_scratch + lane_bytes;
             const double td = now_ms();
             for (ServeRequest *r : finished) {
                 r->finish(2);
@@ -371,7 +395,7 @@ Server *serve_create(Context &ctx, const ServeConfig &cfg) {
     const size_t threads = (size_t)s->nblocks * kTraceBlock;
     s->hist.alloc(sizeof(float4) * 2 * (size_t)(kServeMaxDepth + 1) * threads);
     s->spill.alloc(sizeof(int) * (size_t)kServeSpillDepth * threads);
-    s->scratch = s->table_dev.bytes + s->hist.bytes + s->spill.bytes;
+    s->scratch = s->fixed_scratch = s->table_dev.bytes + s->hist.bytes + s->spill.bytes;
     KRT_REQUIRE(s->scratch < cfg.hbm_budget, KRT_ERR_BUDGET, "HBM budget below the server's fixed scratch");
     for (auto &e : s->ev) KRT_HIP(hipEventCreate(&e));
     for (int l = 0; l < kServeLanes; l++) {
@@ -459,7 +483,13 @@ uint64_t serve_submit(Server *s, Scene &scene, const krt_camera &cam, uint32_t w
     r->seed = seed;
     r->kind = kind;
     r->ropt = ropt;
-    r->bytes = kind == KRT_REQUEST_RESTIR ? serve_restir_bytes(w, h, ropt, max_depth) : serve_request_bytes(w, h);
+    r->on_lane = kind == KRT_REQUEST_RESTIR || (uint64_t)w * h >= s->cfg.wave_pixels;
+    if (kind == KRT_REQUEST_RESTIR) {
+        r->bytes = serve_restir_bytes(w, h, ropt, max_depth);
+    } else {
+        r->bytes = serve_request_bytes(w, h);
+        if (r->on_lane) r->bytes += wave_scratch_grow(r->wave, w, h, max_depth, scene.instanced, false, true);
+    }
     std::lock_guard<std::mutex> g(s->mu);
     KRT_REQUIRE(s->fault.empty(), KRT_ERR_HIP, "server stopped: " + s->fault);
     KRT_REQUIRE(s->scratch + r->bytes <= s->cfg.hbm_budget, KRT_ERR_BUDGET,

@@ -314,7 +314,7 @@ def serve_run(args, ctx, resident, rate, seconds):
     from artemis_amd import Server, look_at
 
     srv = Server(ctx, hbm_budget=args.serve_budget_gb << 30, max_active=args.serve_active,
-                 max_queue_ms=args.serve_deadline_ms)
+                 max_queue_ms=args.serve_deadline_ms, wave_pixels=args.serve_wave_pixels)
     rng = np.random.default_rng(5 + int(rate))
     n = int(rng.poisson(rate * seconds))
     gaps = rng.exponential(1.0 / rate, n)
@@ -399,6 +399,7 @@ def serve_workload(args, ctx, rates=None, seconds=None):
                                "tracer (depth 5), half ReSTIR DI (depth 2, reservoirs resident), 8 resident scenes "
                                "of 10k-2M triangles",
                    "hbm_budget_gb": args.serve_budget_gb, "max_active": args.serve_active,
+                   "wave_pixels": args.serve_wave_pixels if args.serve_wave_pixels is not None else 1 << 18,
                    "queue_deadline_ms": args.serve_deadline_ms, "parallelism": "single GPU"},
         "p50_latency_ms": head["p50_latency_ms"], "p99_latency_ms": head["p99_latency_ms"],
         "sweep": sweep,
@@ -785,6 +786,9 @@ def main():
     ap.add_argument("--serve-deadline-ms", type=float, default=2000.0)
     ap.add_argument("--serve-budget-gb", type=int, default=32)
     ap.add_argument("--serve-active", type=int, default=32)
+    ap.add_argument("--serve-wave-pixels", type=int, default=None,
+                    help="path-tracer requests of at least this many pixels run the wavefront passes on a "
+                         "lane (default: the server's, 512x512)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -394,22 +394,27 @@ int krt_attachment_cache(krt_attachment *att, krt_mlp **out);
  * frames of the same scene, camera, depth, seed and options
  * (ArmadaRTX::render, armada.cu:733-780), bit for bit, whatever else runs
  * beside it.  Every step advances each active request by its share of
- * `step_pixels` pixel-frames (at least one frame): the path-tracer requests
- * in one persistent launch (a captured hipGraph), each ReSTIR request's
- * frames on one of a few streams so that they overlap on the device, its
- * reservoirs resident from admission to its last frame.  Requests are
+ * `step_pixels` pixel-frames (at least one frame): small path-tracer
+ * requests in one persistent launch (a captured hipGraph); large ones
+ * (`wave_pixels`) through the wavefront passes, and each ReSTIR request's
+ * frames, on one of a few streams so that they overlap on the device, each
+ * such request's reservoirs or wavefront buffers resident from admission to
+ * its last frame.  Requests are
  * admitted FIFO as slots and HBM budget free up.  While a request holds a
  * scene, krt_scene_free and krt_accel_build of that scene fail with
  * KRT_ERR_STATE. */
 typedef struct krt_server krt_server;
 
 typedef struct {
-    uint64_t hbm_budget;   /* bytes: fixed scratch + each active request's buffers (64 B per pixel for the
-                              path tracer, about 0.9 KB per pixel for ReSTIR) */
+    uint64_t hbm_budget;   /* bytes: fixed scratch + each active request's buffers (64 B per pixel for a
+                              batched path-tracer request, plus 160 B + 49 B per path vertex for one on a
+                              lane, about 0.9 KB per pixel for ReSTIR) */
     uint32_t max_active;   /* requests per step, 1..64 */
     int32_t use_graph;     /* 1: replay a captured hipGraph per step */
     double max_queue_ms;   /* > 0: reject requests queued longer than this */
     uint64_t step_pixels;  /* pixel-frames per step, shared fairly among active requests */
+    uint64_t wave_pixels;  /* path-tracer requests of at least this many pixels run the wavefront passes on a
+                              lane, smaller ones the persistent batched launch (0: every request on lanes) */
 } krt_server_config;
 
 enum { KRT_REQUEST_PATH_TRACER = 0, KRT_REQUEST_RESTIR = 1 };

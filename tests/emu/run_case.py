@@ -139,7 +139,7 @@ def case_instancing(ctx, full=False):
 def case_attachments(ctx):
     """Every renderer on a small flat scene, bit for bit vs the oracle: path
     tracer, ReSTIR DI and ReSTIR GI under a moving camera, the editor's
-    G-buffer ReSTIR, a served request; the cached path tracer (MLP inference
+    G-buffer ReSTIR, served requests (batched and on lanes); the cached path tracer (MLP inference
     and training on the side stream) runs for coverage."""
     from artemis_amd import Mamba, Server
 
@@ -197,6 +197,21 @@ def case_attachments(ctx):
     ref, _ = osc.render_pt(cams[1], W, H, frames=2, max_depth=5, user_seed=3)
     assert bits_equal(img, ref[0]), "served request"
     say("attachments: served path-tracer request bit-exact")
+    # every request on a lane: the path tracer's wavefront passes with the
+    # request's own buffers swapped in, beside a ReSTIR request
+    srv = Server(ctx, wave_pixels=0)
+    ropt = dict(candidates=8, m_cap=20, spatial_passes=1, neighbours=3, radius=10)
+    rid = srv.submit(sc, cams[1], W, H, spp=2, max_depth=5, seed=3)
+    rid2 = srv.submit(sc, cams[0], W, H, spp=2, max_depth=2, seed=4, kind="restir", **ropt)
+    img, _ = srv.wait(rid)
+    img2, _ = srv.wait(rid2)
+    srv.close()
+    assert bits_equal(img, ref[0]), "served request on a lane"
+    ors = O.OracleReSTIR(W, H, user_seed=4)
+    for _ in range(2):
+        ors.frame(osc, cams[0], max_depth=2, **ropt)
+    assert bits_equal(img2, ors.buffers[0]), "served ReSTIR request"
+    say("attachments: served requests on lanes bit-exact")
     arm = ArmadaRTX(ctx, W, H)
     arm.attach("pt", PathTracer(ctx))
     arm.set_option("radiance_cache", True)

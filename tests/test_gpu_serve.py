@@ -131,3 +131,68 @@ def test_queueing_deadline_sheds_load(krt_ctx, two_scenes):
     assert res[0] is not None
     assert st["rejected"] >= 1 and st["rejected"] + st["completed"] == 12
     assert sum(r is None for r in res) == st["rejected"]
+
+
+def test_wavefront_lane_equals_batched_launch(krt_ctx, two_scenes):
+    """The same request through the wavefront passes on a lane (wave_pixels
+    = 0) and through the batched launch (wave_pixels above its size): the
+    same bits, the attachment's."""
+    sc, data = two_scenes[1]
+    w, h, spp, depth, seed = 96, 64, 5, 4, 21
+    cam = look_at(**data.camera, aspect=w / h)
+    out = {}
+    for wave_pixels in (0, 1 << 40):
+        srv = Server(krt_ctx, wave_pixels=wave_pixels, step_pixels=2 * w * h)  # two frames per step
+        out[wave_pixels], _ = srv.wait(srv.submit(sc, cam, w, h, spp=spp, max_depth=depth, seed=seed))
+        srv.close()
+    assert np.array_equal(bits(out[0]), bits(out[1 << 40]))
+    assert np.array_equal(bits(out[0]), bits(armada_pt(krt_ctx, sc, data, w, h, spp, depth, seed)))
+
+
+def test_lanes_shared_by_several_requests(krt_ctx, two_scenes):
+    """Every request on a lane (wave_pixels = 0): path-tracer requests of
+    different sizes and depths share the three lanes with ReSTIR requests,
+    each with its own wavefront buffers swapped in for its frames; each
+    result equals the batched launch's for the same request."""
+    (sa, da), (sb, db) = two_scenes
+    reqs = []
+    for k in range(9):
+        sc, d = (sa, da) if k % 3 else (sb, db)
+        w, h = [(40, 30), (72, 72), (100, 36)][k % 3]
+        reqs.append((sc, d, w, h, 2 + k % 4, 1 + k % 5, 30 + k, "restir" if k % 4 == 3 else "path_tracer"))
+
+    def run(wave_pixels):
+        srv = Server(krt_ctx, max_active=16, wave_pixels=wave_pixels, step_pixels=1 << 14)
+        ids = [srv.submit(sc, look_at(**d.camera, aspect=w / h), w, h, spp=spp, max_depth=depth, seed=seed,
+                          kind=kind, **({"candidates": 8, "m_cap": 20, "spatial_passes": 1, "neighbours": 3,
+                                         "radius": 10} if kind == "restir" else {}))
+               for sc, d, w, h, spp, depth, seed, kind in reqs]
+        imgs = [srv.wait(i)[0] for i in ids]
+        st = srv.stats()
+        srv.close()
+        return imgs, st
+
+    laned, st = run(0)
+    batched, _ = run(1 << 40)
+    assert st["max_concurrent"] > 3 and st["completed"] == len(reqs)
+    for r, a, b in zip(reqs, laned, batched):
+        assert np.array_equal(bits(a), bits(b)), r[2:]
+
+
+def test_lane_request_budget_counts_its_wavefront_buffers(krt_ctx, two_scenes):
+    """A path-tracer request on a lane holds its colour/AOV planes (64 B a
+    pixel) and its wavefront buffers: per pixel 16 B hits, 16 B path state,
+    48 B history and 1 B occlusion per path vertex, and two ray lists of 2
+    float4 per lane slot (rays and packed copy) with per-workgroup counts and
+    offsets.  128 x 128 pixels at depth 2 (3 vertices): 64 workgroups of
+    256 lane slots."""
+    sc, d = two_scenes[1]
+    w = h = 128
+    npix, gb = w * h, 64
+    expect = npix * (64 + 16 + 16 + 3 * 48 + 3) + 2 * (2 * 32 * gb * 256 + 4 * gb + 4 * (gb + 1))
+    srv = Server(krt_ctx, wave_pixels=0)
+    img, _ = srv.wait(srv.submit(sc, look_at(**d.camera, aspect=1.0), w, h, spp=2, max_depth=2, seed=1))
+    st = srv.stats()
+    srv.close()
+    assert img is not None
+    assert st["hbm_peak"] == expect and st["hbm_in_use"] == 0
