@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib as L
 from .camera import CameraFrame, look_at
-from .scenes import SceneData
+from .scenes import SceneData, pack_textures
 
 __all__ = ["Context", "Scene", "ArmadaRTX", "PathTracer", "ReSTIR", "ReSTIRGI", "Mamba", "CameraFrame", "look_at", "SceneData",
            "load_attachment", "RadianceField", "Server", "DeviceArray"]
@@ -166,7 +166,19 @@ class Scene:
         d = L.SceneDesc(_f(p), _f(n), len(p), i.ctypes.data_as(L.u32p), m.ctypes.data_as(L.u32p), len(i),
                         _f(mats), len(mats))
         h = L.handle()
-        L.check(lib.krt_scene_upload(ctx.h, C.byref(d), C.byref(h)))
+        if getattr(data, "textures", None) is not None:
+            # krt_scene_upload_textured: tex_coords, tangent frames, RGBA8 images
+            uv = np.ascontiguousarray(data.uvs, np.float32)
+            tan = None if data.tangents is None else np.ascontiguousarray(data.tangents, np.float32)
+            texels, dims = pack_textures(data.textures)
+            mt = np.ascontiguousarray(data.material_textures, np.int32)
+            self._keep += [uv, tan, texels, dims, mt]
+            t = L.SceneTextures(_f(uv), None if tan is None else _f(tan), len(uv), texels.ctypes.data_as(L.u32p),
+                                dims.ctypes.data_as(L.u32p), len(dims), mt.ctypes.data_as(C.POINTER(C.c_int32)),
+                                len(mats))
+            L.check(lib.krt_scene_upload_textured(ctx.h, C.byref(d), C.byref(t), C.byref(h)))
+        else:
+            L.check(lib.krt_scene_upload(ctx.h, C.byref(d), C.byref(h)))
         self.h = h.value
         ctx._adopt(self)
         if build:

@@ -40,6 +40,14 @@ class SceneDesc(C.Structure):
     ]
 
 
+class SceneTextures(C.Structure):
+    _fields_ = [
+        ("uvs", fp), ("tangents", fp), ("nv", C.c_uint32),
+        ("texels", u32p), ("dims", u32p), ("ntex", C.c_uint32),
+        ("material_textures", C.POINTER(C.c_int32)), ("nm", C.c_uint32),
+    ]
+
+
 class BvhStats(C.Structure):
     _fields_ = [
         ("triangles", C.c_uint32), ("serialized_nodes", C.c_uint32), ("traversal_nodes", C.c_uint32),
@@ -123,6 +131,7 @@ _SIGS = {
     "krt_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "krt_destroy": (None, [vp]),
     "krt_scene_upload": (C.c_int, [vp, C.POINTER(SceneDesc), C.POINTER(handle)]),
+    "krt_scene_upload_textured": (C.c_int, [vp, C.POINTER(SceneDesc), C.POINTER(SceneTextures), C.POINTER(handle)]),
     "krt_scene_free": (C.c_int, [vp, handle]),
     "krt_scene_light_count": (C.c_int, [vp, handle, u32p]),
     "krt_scene_lights": (C.c_int, [vp, handle, fp, C.c_size_t]),

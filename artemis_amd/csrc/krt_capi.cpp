@@ -29,7 +29,10 @@ SceneView make_view(const Scene &s) {
     return SceneView{s.pos.as<float>(),   s.nrm.as<float>(),    s.idx.as<uint32_t>(),
                      s.tri_mat.as<uint32_t>(), s.mats.as<float4>(), s.lights.as<float4>(),
                      s.nl,                s.accel.tri.as<float4>(), s.accel.nodes.as<float4>(),
-                     s.accel.wide.as<float4>(), s.tlas.wide.as<float4>(), s.inst_rec.as<float4>()};
+                     s.accel.wide.as<float4>(), s.tlas.wide.as<float4>(), s.inst_rec.as<float4>(),
+                     s.textured ? s.uv.as<float2>() : nullptr,
+                     s.textured && s.tan.p ? s.tan.as<float>() : nullptr,
+                     TexSet{s.texels.as<uint32_t>(), s.tex_desc.as<uint4>()}};
 }
 
 }  // namespace krt
@@ -229,8 +232,11 @@ static void upload(Context &C, DevBuf &b, const void *src, size_t bytes) {
     if (bytes) KRT_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, C.stream));
 }
 
-// Material rows: {diffuse, rough}, {specular, refr}, {emission, type}, {0}.
-static std::vector<float> material_rows(const float *materials, uint32_t nm) {
+// Material rows: {diffuse, rough}, {specular, refr}, {emission, type}, and
+// the texture ids {diffuse | normal << 16, specular | emission << 16,
+// roughness | 0xFFFF << 16, any} (0xFFFF: none; mat_tex: 5 per material
+// {diffuse, normal, specular, emission, roughness}, -1 none, or NULL).
+static std::vector<float> material_rows(const float *materials, uint32_t nm, const int32_t *mat_tex = nullptr) {
     std::vector<float> mt(16 * (size_t)nm, 0.0f);
     for (uint32_t m = 0; m < nm; m++) {
         const float *q = materials + 12 * m;
@@ -240,6 +246,15 @@ static std::vector<float> material_rows(const float *materials, uint32_t nm) {
         o[8] = q[6]; o[9] = q[7]; o[10] = q[8];
         int type = (int)q[11];
         std::memcpy(&o[11], &type, 4);
+        uint32_t id[5];
+        bool any = false;
+        for (int k = 0; k < 5; k++) {
+            const int32_t t = mat_tex ? mat_tex[5 * m + k] : -1;
+            id[k] = t < 0 ? 0xFFFFu : (uint32_t)t;
+            any = any || t >= 0;
+        }
+        const uint32_t row[4] = {id[0] | id[1] << 16, id[2] | id[3] << 16, id[4] | 0xFFFF0000u, any ? 1u : 0u};
+        std::memcpy(&o[12], row, sizeof(row));
     }
     return mt;
 }
@@ -261,92 +276,140 @@ static void light_row(const float *a, const float *b, const float *c, const floa
     lt.insert(lt.end(), row, row + 16);
 }
 
-int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scene) {
-    return guard([&] {
-        KRT_REQUIRE(ctx && d && scene, KRT_ERR_INVALID, "null argument");
-        KRT_REQUIRE(d->nt > 0 && d->nv > 0 && d->nm > 0, KRT_ERR_INVALID, "empty scene");
-        KRT_REQUIRE(d->positions && d->normals && d->indices && d->tri_material && d->materials, KRT_ERR_INVALID,
-                    "null scene array");
-        Context &C = ctx->impl;
-        KRT_HIP(hipSetDevice(C.device));
-        for (uint32_t i = 0; i < 3 * d->nt; i++)
-            KRT_REQUIRE(d->indices[i] < d->nv, KRT_ERR_INVALID, "vertex index out of range");
-        for (uint32_t t = 0; t < d->nt; t++)
-            KRT_REQUIRE(d->tri_material[t] < d->nm, KRT_ERR_INVALID, "material index out of range");
-        auto s = std::make_unique<Scene>();
-        s->nv = d->nv;
-        s->nt = d->nt;
-        s->nm = d->nm;
-        upload(C, s->pos, d->positions, sizeof(float) * 3 * d->nv);
-        for (int a = 0; a < 3; a++) s->bmin[a] = s->bmax[a] = d->positions[a];
-        for (uint32_t v = 0; v < d->nv; v++)
-            for (int a = 0; a < 3; a++) {
-                s->bmin[a] = std::min(s->bmin[a], d->positions[3 * v + a]);
-                s->bmax[a] = std::max(s->bmax[a], d->positions[3 * v + a]);
-            }
-        upload(C, s->nrm, d->normals, sizeof(float) * 3 * d->nv);
-        upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * d->nt);
-        upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * d->nt);
-        std::vector<float> mt = material_rows(d->materials, d->nm);
-        upload(C, s->mats, mt.data(), sizeof(float) * mt.size());
-        // triangle lights in triangle order (armada.cu:117-140, emissive test :240)
-        std::vector<float> lt;
+static void check_textures(const krt_scene_desc *d, const krt_scene_textures *x) {
+    KRT_REQUIRE(x->nv == d->nv && x->nm == d->nm, KRT_ERR_INVALID, "texture arrays do not match the scene");
+    KRT_REQUIRE(x->uvs && x->material_textures, KRT_ERR_INVALID, "null texture array");
+    KRT_REQUIRE(x->ntex == 0 || (x->texels && x->dims), KRT_ERR_INVALID, "null texel array");
+    KRT_REQUIRE(x->ntex < 0xFFFFu, KRT_ERR_INVALID, "at most 65534 textures");
+    for (uint32_t t = 0; t < x->ntex; t++)
+        KRT_REQUIRE(x->dims[2 * t] >= 1 && x->dims[2 * t + 1] >= 1 && x->dims[2 * t] <= 16384 &&
+                        x->dims[2 * t + 1] <= 16384,
+                    KRT_ERR_INVALID, "texture sides must be in [1, 16384]");
+    for (uint32_t m = 0; m < d->nm; m++)
+        for (int k = 0; k < 5; k++) {
+            const int32_t t = x->material_textures[5 * m + k];
+            KRT_REQUIRE(t >= -1 && t < (int32_t)x->ntex, KRT_ERR_INVALID, "material texture index out of range");
+            KRT_REQUIRE(!(k == 1 && t >= 0 && !x->tangents), KRT_ERR_INVALID, "a normal map needs tangents");
+        }
+}
+
+// Emissive test of the triangle-light list (armada.cu:264-265): a non-zero
+// emission or an emission texture.
+static bool emissive_mat(const krt_scene_desc *d, const krt_scene_textures *x, uint32_t m) {
+    return emissive(d->materials + 12 * m + 6) || (x && x->material_textures[5 * m + 3] >= 0);
+}
+
+static void scene_upload(krt_context *ctx, const krt_scene_desc *d, const krt_scene_textures *x,
+                     krt_handle *scene) {
+    KRT_REQUIRE(ctx && d && scene, KRT_ERR_INVALID, "null argument");
+    KRT_REQUIRE(d->nt > 0 && d->nv > 0 && d->nm > 0, KRT_ERR_INVALID, "empty scene");
+    KRT_REQUIRE(d->positions && d->normals && d->indices && d->tri_material && d->materials, KRT_ERR_INVALID,
+                "null scene array");
+    Context &C = ctx->impl;
+    KRT_HIP(hipSetDevice(C.device));
+    for (uint32_t i = 0; i < 3 * d->nt; i++)
+        KRT_REQUIRE(d->indices[i] < d->nv, KRT_ERR_INVALID, "vertex index out of range");
+    for (uint32_t t = 0; t < d->nt; t++)
+        KRT_REQUIRE(d->tri_material[t] < d->nm, KRT_ERR_INVALID, "material index out of range");
+    if (x) check_textures(d, x);
+    auto s = std::make_unique<Scene>();
+    s->nv = d->nv;
+    s->nt = d->nt;
+    s->nm = d->nm;
+    upload(C, s->pos, d->positions, sizeof(float) * 3 * d->nv);
+    for (int a = 0; a < 3; a++) s->bmin[a] = s->bmax[a] = d->positions[a];
+    for (uint32_t v = 0; v < d->nv; v++)
+        for (int a = 0; a < 3; a++) {
+            s->bmin[a] = std::min(s->bmin[a], d->positions[3 * v + a]);
+            s->bmax[a] = std::max(s->bmax[a], d->positions[3 * v + a]);
+        }
+    upload(C, s->nrm, d->normals, sizeof(float) * 3 * d->nv);
+    upload(C, s->idx, d->indices, sizeof(uint32_t) * 3 * d->nt);
+    upload(C, s->tri_mat, d->tri_material, sizeof(uint32_t) * d->nt);
+    std::vector<float> mt = material_rows(d->materials, d->nm, x ? x->material_textures : nullptr);
+    upload(C, s->mats, mt.data(), sizeof(float) * mt.size());
+    if (x) {
+        s->textured = true;
+        upload(C, s->uv, x->uvs, sizeof(float) * 2 * d->nv);
+        if (x->tangents) upload(C, s->tan, x->tangents, sizeof(float) * 6 * d->nv);
+        std::vector<uint32_t> desc(4 * (size_t)std::max(x->ntex, 1u), 0u);
+        size_t texels = 0;
+        for (uint32_t t = 0; t < x->ntex; t++) {
+            desc[4 * t] = (uint32_t)texels;
+            desc[4 * t + 1] = x->dims[2 * t];
+            desc[4 * t + 2] = x->dims[2 * t + 1];
+            texels += (size_t)x->dims[2 * t] * x->dims[2 * t + 1];
+        }
+        KRT_REQUIRE(texels < (1ull << 32), KRT_ERR_INVALID, "more than 2^32 texels");
+        upload(C, s->texels, x->texels, sizeof(uint32_t) * texels);
+        upload(C, s->tex_desc, desc.data(), sizeof(uint32_t) * desc.size());
+    }
+    // triangle lights in triangle order (armada.cu:117-140, emissive test
+    // :264-265), with the material's constant emission (armada.cu:143)
+    std::vector<float> lt;
+    std::vector<int32_t> ltri;
+    for (uint32_t t = 0; t < d->nt; t++) {
+        if (!emissive_mat(d, x, d->tri_material[t])) continue;
+        light_row(d->positions + 3 * d->indices[3 * t], d->positions + 3 * d->indices[3 * t + 1],
+                  d->positions + 3 * d->indices[3 * t + 2], d->materials + 12 * d->tri_material[t] + 6, lt);
+        ltri.push_back((int32_t)t);
+    }
+    s->nl = (uint32_t)(lt.size() / 16);
+    upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
+    upload(C, s->light_tri, ltri.data(), sizeof(int32_t) * ltri.size());
+    // area lights of the editor's ReSTIR (one per emissive material, its
+    // triangles in triangle order)
+    {
+        std::map<uint32_t, uint32_t> of_mat;
+        std::vector<int32_t> count;
+        std::vector<float> em;
         for (uint32_t t = 0; t < d->nt; t++) {
-            const float *e = d->materials + 12 * d->tri_material[t] + 6;
+            const uint32_t m = d->tri_material[t];
+            const float *e = d->materials + 12 * m + 6;
             if (!emissive(e)) continue;
-            light_row(d->positions + 3 * d->indices[3 * t], d->positions + 3 * d->indices[3 * t + 1],
-                      d->positions + 3 * d->indices[3 * t + 2], e, lt);
+            auto it = of_mat.find(m);
+            if (it == of_mat.end()) {
+                it = of_mat.emplace(m, (uint32_t)count.size()).first;
+                count.push_back(0);
+                em.insert(em.end(), {e[0], e[1], e[2], 0.0f});
+            }
+            count[it->second]++;
         }
-        s->nl = (uint32_t)(lt.size() / 16);
-        upload(C, s->lights, lt.data(), sizeof(float) * lt.size());
-        {
-            std::vector<int32_t> ltri;
-            for (uint32_t t = 0; t < d->nt; t++)
-                if (emissive(d->materials + 12 * d->tri_material[t] + 6)) ltri.push_back((int32_t)t);
-            upload(C, s->light_tri, ltri.data(), sizeof(int32_t) * ltri.size());
+        std::vector<int32_t> range(2 * count.size()), fill(count.size(), 0), tris;
+        int32_t run = 0;
+        for (size_t l = 0; l < count.size(); l++) {
+            range[2 * l] = run;
+            range[2 * l + 1] = count[l];
+            run += count[l];
         }
-        // area lights of the editor's ReSTIR (one per emissive material, its
-        // triangles in triangle order)
-        {
-            std::map<uint32_t, uint32_t> of_mat;
-            std::vector<int32_t> count;
-            std::vector<float> em;
-            for (uint32_t t = 0; t < d->nt; t++) {
-                const uint32_t m = d->tri_material[t];
-                const float *e = d->materials + 12 * m + 6;
-                if (!emissive(e)) continue;
-                auto it = of_mat.find(m);
-                if (it == of_mat.end()) {
-                    it = of_mat.emplace(m, (uint32_t)count.size()).first;
-                    count.push_back(0);
-                    em.insert(em.end(), {e[0], e[1], e[2], 0.0f});
-                }
-                count[it->second]++;
-            }
-            std::vector<int32_t> range(2 * count.size()), fill(count.size(), 0), tris;
-            int32_t run = 0;
-            for (size_t l = 0; l < count.size(); l++) {
-                range[2 * l] = run;
-                range[2 * l + 1] = count[l];
-                run += count[l];
-            }
-            tris.assign((size_t)run, 0);
-            for (uint32_t t = 0; t < d->nt; t++) {
-                auto it = of_mat.find(d->tri_material[t]);
-                if (it == of_mat.end()) continue;
-                tris[(size_t)range[2 * it->second] + fill[it->second]++] = (int32_t)t;
-            }
-            s->nml = (uint32_t)count.size();
-            if (s->nml) {
-                upload(C, s->ml_range, range.data(), sizeof(int32_t) * range.size());
-                upload(C, s->ml_tris, tris.data(), sizeof(int32_t) * tris.size());
-                upload(C, s->ml_emission, em.data(), sizeof(float) * em.size());
-            }
+        tris.assign((size_t)run, 0);
+        for (uint32_t t = 0; t < d->nt; t++) {
+            auto it = of_mat.find(d->tri_material[t]);
+            if (it == of_mat.end()) continue;
+            tris[(size_t)range[2 * it->second] + fill[it->second]++] = (int32_t)t;
         }
-        KRT_HIP(hipStreamSynchronize(C.stream));
-        krt_handle h = C.next_handle++;
-        C.scenes[h] = std::move(s);
-        *scene = h;
+        s->nml = (uint32_t)count.size();
+        if (s->nml) {
+            upload(C, s->ml_range, range.data(), sizeof(int32_t) * range.size());
+            upload(C, s->ml_tris, tris.data(), sizeof(int32_t) * tris.size());
+            upload(C, s->ml_emission, em.data(), sizeof(float) * em.size());
+        }
+    }
+    KRT_HIP(hipStreamSynchronize(C.stream));
+    krt_handle h = C.next_handle++;
+    C.scenes[h] = std::move(s);
+    *scene = h;
+}
+
+int krt_scene_upload(krt_context *ctx, const krt_scene_desc *d, krt_handle *scene) {
+    return guard([&] { scene_upload(ctx, d, nullptr, scene); });
+}
+
+int krt_scene_upload_textured(krt_context *ctx, const krt_scene_desc *d, const krt_scene_textures *tex,
+                              krt_handle *scene) {
+    return guard([&] {
+        KRT_REQUIRE(tex, KRT_ERR_INVALID, "null argument");
+        scene_upload(ctx, d, tex, scene);
     });
 }
 

@@ -169,7 +169,9 @@ struct Material {
 };
 
 // Device material table row: 4 x float4 = {diffuse, rough}, {specular, refr},
-// {emission, type-as-int-bits}, {1/PI-scaled diffuse, avg_Kd}.
+// {emission, type-as-int-bits}, {texture ids as uint bits: diffuse | normal
+// << 16, specular | emission << 16, roughness | 0xFFFF << 16, any-texture
+// flag}; a 16-bit id of kNoTex means no texture.
 KRT_DEV Material load_material(const float4 *tab, uint32_t m) {
     float4 a = tab[4 * m], b = tab[4 * m + 1], c = tab[4 * m + 2];
     Material M;
@@ -180,6 +182,19 @@ KRT_DEV Material load_material(const float4 *tab, uint32_t m) {
     M.emission = mk(c.x, c.y, c.z);
     M.type = __float_as_int(c.w);
     return M;
+}
+
+constexpr uint32_t kNoTex = 0xFFFFu;
+KRT_DEV uint4 material_tex(const float4 *tab, uint32_t m) {
+    const float4 r = tab[4 * m + 3];
+    return make_uint4(__float_as_uint(r.x), __float_as_uint(r.y), __float_as_uint(r.z), __float_as_uint(r.w));
+}
+
+// An RGBA8 texel (r in the low byte) read as byte / 255 per channel
+// (cudaReadModeNormalizedFloat).
+KRT_DEV float4 texel8(uint32_t t) {
+    return make_float4((float)(t & 255u) / 255.0f, (float)((t >> 8) & 255u) / 255.0f,
+                       (float)((t >> 16) & 255u) / 255.0f, (float)(t >> 24) / 255.0f);
 }
 
 struct SurfaceHit {

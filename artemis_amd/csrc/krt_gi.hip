@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_shade_kernel(LaunchInfo li, S
         h.inst = -1;
         Surface sf;
         if (h.prim >= 0 && load_surface(sv, h, d, sf)) {
-            const Material mat = load_material(sv.mats, sf.mat_id);
+            const Material mat = surface_material(sv, sf);
             const f3 n = sf.n;
             const f3 x = offset_point(sf.x, n, mat.type);
             const SurfaceHit sh{mat, sf.entering, n, -d, x};
@@ -724,6 +724,7 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
                unsigned long long *ctr, PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "restir_gi renders flat scenes (krt_scene_upload)");
+    KRT_REQUIRE(!s.textured, KRT_ERR_STATE, "restir_gi renders untextured scenes");
     KRT_REQUIRE(!li_in.env, KRT_ERR_STATE,
                 "restir_gi has no environment lighting (its samples are surface points); disable the envmap");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR GI buffers not loaded");

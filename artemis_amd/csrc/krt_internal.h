@@ -102,6 +102,12 @@ struct Scene {
     DevBuf mats;       // float4 x 4 per material
     DevBuf lights;     // float4 x 4 per light
     DevBuf light_tri;  // int per light: its triangle (flat scenes)
+    // material textures (krt_scene_upload_textured): float2 texture
+    // coordinates and 6 floats {tangent, bitangent} per vertex, RGBA8 texels
+    // of every texture back to back and a {offset, width, height, 0} uint4
+    // per texture
+    bool textured = false;
+    DevBuf uv, tan, texels, tex_desc;
     bool stale = false;  // vertices moved since the last build / refit
     // the editor's area lights (Mamba): emissive triangles grouped by
     // material in order of their first triangle; per light {first, count}
@@ -130,10 +136,18 @@ struct Scene {
     Accel tlas;
     DevBuf inst_rec;
     size_t bytes() const {
-        return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + accel.ser.bytes +
+        return pos.bytes + nrm.bytes + idx.bytes + tri_mat.bytes + mats.bytes + lights.bytes + uv.bytes + tan.bytes +
+               texels.bytes + tex_desc.bytes + accel.ser.bytes +
                accel.leaf_order.bytes + accel.tri.bytes + accel.nodes.bytes + accel.wide.bytes + mesh_ser.bytes +
                tlas.ser.bytes + tlas.leaf_order.bytes + tlas.nodes.bytes + tlas.wide.bytes + inst_rec.bytes;
     }
+};
+
+// Material textures on the device: RGBA8 texels of every texture back to
+// back and a {first texel, width, height, 0} descriptor per texture.
+struct TexSet {
+    const uint32_t *texels;
+    const uint4 *desc;
 };
 
 // Device view of a scene passed to kernels by value.
@@ -150,6 +164,11 @@ struct SceneView {
     const float4 *wide;   // its 4-wide collapse (instanced scenes: every mesh's, concatenated)
     const float4 *tlas;   // instanced scenes: 4-wide instance tree
     const float4 *inst;   // instanced scenes: instance records (krt_traverse.h)
+    // material textures (flat scenes): per-vertex texture coordinates and
+    // {tangent, bitangent} (nullptr when absent), the texels and descriptors
+    const float2 *uv;
+    const float *tan;
+    TexSet tex;
 };
 
 SceneView make_view(const Scene &s);
@@ -228,6 +247,7 @@ struct RestirState {
     DevBuf res[2];  // 4 float4 planes per buffer (SoA of float4)
     int cur = 0;
     DevBuf gmat;          // uint material id per pixel
+    DevBuf guv;           // float2 texture coordinates per pixel (textured scenes)
     DevBuf intermediate;  // float4 per pixel
     DevBuf auxiliary;     // float4 per pixel
     // spatial pass, split around its shadow rays: segmented ray list (2 float4

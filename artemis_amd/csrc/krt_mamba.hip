@@ -439,6 +439,7 @@ void mamba_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const MambaOp
                   PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "mamba renders flat scenes (krt_scene_upload)");
+    KRT_REQUIRE(!s.textured, KRT_ERR_STATE, "mamba renders untextured scenes");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "mamba buffers not loaded");
     MambaArgs A;
     A.li = li_in;

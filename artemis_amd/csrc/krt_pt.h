@@ -45,7 +45,7 @@ KRT_DEV void pt_pixel(const LaunchInfo &li, const SceneView &sv, uint32_t pix, i
         }
         Surface sf;
         if (!load_surface(sv, h, d, sf)) break;
-        Material mat = load_material(sv.mats, sf.mat_id);
+        Material mat = surface_material(sv, sf);
         f3 n = sf.n;
         f3 x = offset_point(sf.x, n, mat.type);
         SurfaceHit sh{mat, sf.entering, n, -d, x};

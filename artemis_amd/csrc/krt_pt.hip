@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
         if (surf && cv.enabled && level == cv.depth) {
             // the cache vertex (krt_cache.hip): record the query; the path stops
             // here unless the pixel is a training path
-            const Material mat = load_material(sv.mats, sf.mat_id);
+            const Material mat = surface_material(sv, sf);
             uint32_t slot = 0;
             const bool train = cv.training(pix, slot);
             put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d, sf.n,
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kWaveBlock) void pt_shade_kernel(LaunchInfo li, Sce
             cache_stop = !train;
         }
         if (surf && !cache_stop) {
-            const Material mat = load_material(sv.mats, sf.mat_id);
+            const Material mat = surface_material(sv, sf);
             const f3 n = sf.n;
             const f3 x = offset_point(sf.x, n, mat.type);
             const SurfaceHit sh{mat, sf.entering, n, -d, x};

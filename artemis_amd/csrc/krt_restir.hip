@@ -143,6 +143,7 @@ struct RestirArgs {
     SceneView sv;
     Planes cur, prev;  // prev: temporal source / spatial destination
     uint32_t *gmat;
+    float2 *guv;  // textured scenes: the G-buffer's texture coordinates
     float4 *intermediate, *auxiliary;
     int *spill;
     unsigned long long *ctr;
@@ -150,11 +151,25 @@ struct RestirArgs {
     CacheView cv;  // radiance cache for the indirect path (cv.enabled = 0: off)
 };
 
+// The G-buffer's material of a pixel (parameters.materials_buffer,
+// amadeus_restir.cu:231): its id and, for a textured material, its texture
+// coordinates.
+KRT_DEV Material gbuffer_material(const RestirArgs &A, uint32_t i) {
+    const uint32_t m = A.gmat[i];
+    Material M = load_material(A.sv.mats, m);
+    const uint4 t = material_tex(A.sv.mats, m);
+    if (t.w) {
+        const float2 uv = A.guv[i];
+        apply_textures(A.sv.tex, t, uv.x, uv.y, M);
+    }
+    return M;
+}
+
 // G-buffer surface of a pixel (amadeus_restir.cu:230-240, :323-336); the
 // normal is the accumulated normal AOV, as in the reference.
 KRT_DEV SurfaceHit gbuffer_hit(const RestirArgs &A, uint32_t i) {
     SurfaceHit h;
-    h.mat = load_material(A.sv.mats, A.gmat[i]);
+    h.mat = gbuffer_material(A, i);
     h.entering = false;
     float4 n = A.li.normal[i], x = A.li.position[i];
     h.n = mk(n.x, n.y, n.z);
@@ -317,7 +332,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
         if (surf && cv.enabled && level == cv.depth) {
             // the cache vertex: record the query; the path stops here unless
             // the pixel is a training path
-            const Material mat = load_material(sv.mats, sf.mat_id);
+            const Material mat = surface_material(sv, sf);
             uint32_t slot = 0;
             const bool train = cv.training(pix, slot);
             put_query(train ? cv.slot_raw + (size_t)kMlpRaw * slot : cv.query + (size_t)kMlpRaw * pix, sf.x, -d, sf.n,
@@ -329,7 +344,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
             cache_stop = !train;
         }
         if (surf && !cache_stop) {
-            const Material mat = load_material(sv.mats, sf.mat_id);
+            const Material mat = surface_material(sv, sf);
             const f3 n = sf.n;
             const f3 x = offset_point(sf.x, n, mat.type);
             const SurfaceHit sh{mat, sf.entering, n, -d, x};
@@ -380,6 +395,7 @@ __global__ __launch_bounds__(kSpatialBlock) void init_shade_kernel(RestirArgs A,
                 res.resample(Li.x + Li.y + Li.z);  // W for an unblocked sample; the compose kernel applies the ray
                 A.cur.store(pix, res);
                 A.gmat[pix] = sf.mat_id;
+                if (sv.uv) A.guv[pix] = make_float2(sf.u, sf.v);
                 so = sh.x;
                 sd = D / dd;
                 stmax = dd - kShadowEps;
@@ -728,7 +744,7 @@ __global__ __launch_bounds__(kSpatialBlock) void spatial_merge_kernel(RestirArgs
             if (m > 0) {  // `entering` carries neighbour validity
                 rand_uniform(rs);
                 SurfaceHit nh;
-                nh.mat = load_material(A.sv.mats, A.gmat[i0]);
+                nh.mat = gbuffer_material(A, i0);
                 nh.entering = true;
                 const float4 nn = li.normal[i0], xx = li.position[i0];
                 nh.n = mk(nn.x, nn.y, nn.z);
@@ -850,9 +866,11 @@ void restir_alloc(Context &ctx, RestirState &st, uint32_t w, uint32_t h, uint32_
     st.res[1].alloc(sizeof(float4) * 4 * n);
     st.cur = 0;
     st.gmat.alloc(sizeof(uint32_t) * n);
+    st.guv.alloc(sizeof(float2) * n);
     st.intermediate.alloc(sizeof(float4) * n);
     st.auxiliary.alloc(sizeof(float4) * n);
     KRT_HIP(hipMemsetAsync(st.gmat.p, 0, st.gmat.bytes, ctx.stream));
+    KRT_HIP(hipMemsetAsync(st.guv.p, 0, st.guv.bytes, ctx.stream));
     KRT_HIP(hipMemsetAsync(st.intermediate.p, 0, st.intermediate.bytes, ctx.stream));
     KRT_HIP(hipMemsetAsync(st.auxiliary.p, 0, st.auxiliary.bytes, ctx.stream));
     Planes a{st.res[0].as<float4>(), (uint32_t)n}, b{st.res[1].as<float4>(), (uint32_t)n};
@@ -876,6 +894,7 @@ void restir_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const Restir
     A.li = li;
     A.sv = make_view(s);
     A.gmat = st.gmat.as<uint32_t>();
+    A.guv = st.guv.as<float2>();
     A.intermediate = st.intermediate.as<float4>();
     A.auxiliary = st.auxiliary.as<float4>();
     // producer / consumer workgroups (= list segments) and the walks' spill

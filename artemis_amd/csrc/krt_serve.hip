@@ -166,7 +166,7 @@ struct Server {
 size_t serve_request_bytes(uint32_t w, uint32_t h) { return sizeof(float4) * 4 * (size_t)w * h; }
 
 // Device memory of a ReSTIR DI request: AOVs, two reservoir buffers, the
-// G-buffer (material id, intermediate, auxiliary), the shadow-ray list (rays
+// G-buffer (material id, texture coordinates, intermediate, auxiliary), the shadow-ray list (rays
 // and packed copy, S + 1 slots per pixel) with its occlusion bytes, 1/Z
 // weights and flags, and the initial pass's wavefront state and closest-ray
 // list (pixels rounded up to whole 256-lane workgroups of 8x8 tiles).
@@ -177,7 +177,7 @@ size_t serve_restir_bytes(uint32_t w, uint32_t h, const RestirOptions &o, int de
     const size_t per_lane = (size_t)std::max(o.neighbours + 1, 2);
     const size_t levels = (size_t)depth + 1;
     size_t b = serve_request_bytes(w, h);
-    b += 2 * 64 * npix + 4 * npix + 2 * 16 * npix;
+    b += 2 * 64 * npix + 4 * npix + 8 * npix + 2 * 16 * npix;
     b += 2 * 32 * per_lane * slots + per_lane * npix + 4 * (size_t)o.neighbours * npix + npix;
     b += 16 * npix + 16 * npix + 48 * levels * npix + levels * npix + 2 * 32 * slots;
     return b + (1u << 16);

@@ -20,10 +20,60 @@ struct RayCount {
     unsigned closest = 0, shadow = 0;
 };
 
+// Material textures (calculate_material / calculate_normal,
+// source/optix/amadeus_common.cuh:74-187): RGBA8 texels (r in the low byte)
+// read as byte / 255, with tex2D <float4>'s normalised coordinates, bilinear
+// filtering and wrap addressing (the descriptor of import_vulkan_texture,
+// include/cuda/interop.cuh:11-51), filtered in FP32 (the texture unit's 8-bit
+// weights are not restated).  A texture is {first texel, width, height} in
+// one texel array (TexSet); coordinates beyond +-2^16 (or NaN) read zero.
+KRT_DEV float4 tex_fetch(const TexSet &ts, uint32_t id, float u, float v) {
+    if (!(fabsf(u) <= 65536.0f && fabsf(v) <= 65536.0f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint4 d = ts.desc[id];
+    const uint32_t *tx = ts.texels + d.x;
+    const float x = u * (float)d.y - 0.5f, y = v * (float)d.z - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy, W = (int)d.y, H = (int)d.z;
+    int i0 = ix % W, i1 = (ix + 1) % W, j0 = iy % H, j1 = (iy + 1) % H;
+    i0 += i0 < 0 ? W : 0;
+    i1 += i1 < 0 ? W : 0;
+    j0 += j0 < 0 ? H : 0;
+    j1 += j1 < 0 ? H : 0;
+    const float4 t00 = texel8(tx[(size_t)j0 * d.y + i0]), t10 = texel8(tx[(size_t)j0 * d.y + i1]);
+    const float4 t01 = texel8(tx[(size_t)j1 * d.y + i0]), t11 = texel8(tx[(size_t)j1 * d.y + i1]);
+    const float a0 = 1.0f - a, b0 = 1.0f - b;
+    const float w00 = a0 * b0, w10 = a * b0, w01 = a0 * b, w11 = a * b;
+    return make_float4((t00.x * w00 + t10.x * w10) + (t01.x * w01 + t11.x * w11),
+                       (t00.y * w00 + t10.y * w10) + (t01.y * w01 + t11.y * w11),
+                       (t00.z * w00 + t10.z * w10) + (t01.z * w01 + t11.z * w11),
+                       (t00.w * w00 + t10.w * w10) + (t01.w * w01 + t11.w * w11));
+}
+
+// calculate_material's texture reads (amadeus_common.cuh:154-187): diffuse,
+// specular and emission take the texel's rgb, roughness its r.
+KRT_DEV void apply_textures(const TexSet &ts, uint4 t, float u, float v, Material &M) {
+    const uint32_t dif = t.x & 0xFFFFu, spe = t.y & 0xFFFFu, emi = t.y >> 16, rou = t.z & 0xFFFFu;
+    if (dif != kNoTex) {
+        const float4 c = tex_fetch(ts, dif, u, v);
+        M.diffuse = mk(c.x, c.y, c.z);
+    }
+    if (spe != kNoTex) {
+        const float4 c = tex_fetch(ts, spe, u, v);
+        M.specular = mk(c.x, c.y, c.z);
+    }
+    if (emi != kNoTex) {
+        const float4 c = tex_fetch(ts, emi, u, v);
+        M.emission = mk(c.x, c.y, c.z);
+    }
+    if (rou != kNoTex) M.roughness = tex_fetch(ts, rou, u, v).x;
+}
+
 struct Surface {
     f3 x, n;
     bool entering;
     uint32_t mat_id;
+    float u, v;  // texture coordinates (calculate_material's uv; 0 without textures)
 };
 
 KRT_DEV f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
@@ -58,7 +108,30 @@ KRT_DEV bool load_surface_inst(const SceneView &sv, const Hit &h, f3 dir, Surfac
     sf.n = normalize(nn);
     sf.x = xform(m0, m1, m2, interp(a, b, c, h.b1, h.b2), 1.0f);
     sf.mat_id = sv.tri_mat[h.prim];
+    sf.u = sf.v = 0.0f;
     return !isnan3(sf.n);
+}
+
+// The texture coordinates of a hit (LOAD_INTERSECTION_DATA,
+// amadeus_common.cuh:498-502: interpolated, v flipped) and calculate_normal's
+// normal map (:114-146): n = 2 rgb - 1 in the frame of the interpolated
+// tangent and bitangent (each normalised twice, as the reference does; the
+// model matrix is the identity for world-space triangles) and the shading
+// normal.
+KRT_DEV void surface_uv(const SceneView &sv, uint32_t i0, uint32_t i1, uint32_t i2, const Hit &h, Surface &sf) {
+    const float2 ta = sv.uv[i0], tb = sv.uv[i1], tc = sv.uv[i2];
+    const float w = 1.0f - h.b1 - h.b2;
+    sf.u = (w * ta.x + h.b1 * tb.x) + h.b2 * tc.x;
+    sf.v = 1.0f - ((w * ta.y + h.b1 * tb.y) + h.b2 * tc.y);
+    const uint32_t nmap = material_tex(sv.mats, sf.mat_id).x >> 16;
+    if (nmap == kNoTex || !sv.tan) return;
+    const float4 t4 = tex_fetch(sv.tex, nmap, sf.u, sf.v);
+    const f3 tn = mk(2.0f * t4.x - 1.0f, 2.0f * t4.y - 1.0f, 2.0f * t4.z - 1.0f);
+    const float *ta6 = sv.tan + 6 * (size_t)i0, *tb6 = sv.tan + 6 * (size_t)i1, *tc6 = sv.tan + 6 * (size_t)i2;
+    const f3 T = normalize(normalize(interp(ld3(ta6), ld3(tb6), ld3(tc6), h.b1, h.b2)));
+    const f3 B = normalize(normalize(interp(ld3(ta6 + 3), ld3(tb6 + 3), ld3(tc6 + 3), h.b1, h.b2)));
+    const f3 N = normalize(sf.n);
+    sf.n = normalize((T * tn.x + B * tn.y) + N * tn.z);
 }
 
 // Returns false when the shading normal is NaN (the reference then returns 0).
@@ -80,7 +153,20 @@ KRT_DEV bool load_surface(const SceneView &sv, const Hit &h, f3 dir, Surface &sf
     sf.n = normalize(nn);
     sf.x = interp(a, b, c, h.b1, h.b2);
     sf.mat_id = sv.tri_mat[h.prim];
+    sf.u = sf.v = 0.0f;
+    if (sv.uv) surface_uv(sv, i0, i1, i2, h, sf);
     return !isnan3(sf.n);
+}
+
+// calculate_material (amadeus_common.cuh:154-187) of material m at (u, v).
+KRT_DEV Material textured_material(const SceneView &sv, uint32_t m, float u, float v) {
+    Material M = load_material(sv.mats, m);
+    const uint4 t = material_tex(sv.mats, m);
+    if (t.w) apply_textures(sv.tex, t, u, v, M);
+    return M;
+}
+KRT_DEV Material surface_material(const SceneView &sv, const Surface &sf) {
+    return textured_material(sv, sf.mat_id, sf.u, sf.v);
 }
 
 KRT_DEV f3 offset_point(f3 x, f3 n, int type) {

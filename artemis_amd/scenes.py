@@ -62,10 +62,31 @@ class SceneData:
     materials: np.ndarray     # (nm, 12) float32
     camera: dict = field(default_factory=dict)
     name: str = ""
+    # material textures (krt_scene_upload_textured): Vertex::tex_coords,
+    # {tangent, bitangent}, RGBA8 images and per material the texture of
+    # {diffuse, normal, specular, emission, roughness} (-1: none)
+    uvs: np.ndarray | None = None                # (nv, 2) float32
+    tangents: np.ndarray | None = None           # (nv, 6) float32
+    textures: list | None = None                 # [(H, W, 4) uint8]
+    material_textures: np.ndarray | None = None  # (nm, 5) int32
 
     @property
     def triangles(self) -> int:
         return len(self.indices)
+
+
+def pack_textures(textures):
+    """RGBA8 images ((H, W, 4) uint8 each) as one uint32 texel array (r in
+    the low byte, row-major, back to back) and their (width, height) pairs."""
+    parts, dims = [], []
+    for t in textures:
+        a = np.ascontiguousarray(t, np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("textures are (H, W, 4) uint8 images")
+        parts.append(a.reshape(-1, 4).view("<u4").reshape(-1))
+        dims.append((a.shape[1], a.shape[0]))
+    texels = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1, np.uint32), np.uint32)
+    return texels, np.ascontiguousarray(np.array(dims, np.uint32).reshape(-1, 2))
 
 
 def xform_points(M: np.ndarray, p: np.ndarray, w: float = 1.0) -> np.ndarray:
@@ -121,10 +142,10 @@ def material(diffuse, roughness, emission=(0.0, 0.0, 0.0), specular=(0.0, 0.0, 0
 
 class _Builder:
     def __init__(self):
-        self.p, self.n, self.i, self.m = [], [], [], []
+        self.p, self.n, self.i, self.m, self.uv, self.tan = [], [], [], [], [], []
         self.nv = 0
 
-    def add(self, pos, nrm, idx, mat_id):
+    def add(self, pos, nrm, idx, mat_id, uv=None, tan=None):
         pos = np.asarray(pos, np.float64)
         nrm = np.asarray(nrm, np.float64)
         idx = np.asarray(idx, np.int64).reshape(-1, 3)
@@ -132,12 +153,20 @@ class _Builder:
         self.n.append(nrm)
         self.i.append(idx + self.nv)
         self.m.append(np.full(len(idx), mat_id, np.uint32) if np.isscalar(mat_id) else np.asarray(mat_id, np.uint32))
+        self.uv.append(np.zeros((len(pos), 2)) if uv is None else np.asarray(uv, np.float64))
+        self.tan.append(np.zeros((len(pos), 6)) if tan is None else np.asarray(tan, np.float64))
         self.nv += len(pos)
 
-    def build(self, mats, **kw) -> SceneData:
-        return SceneData(np.concatenate(self.p).astype(np.float32), np.concatenate(self.n).astype(np.float32),
-                         np.concatenate(self.i).astype(np.uint32), np.concatenate(self.m),
-                         np.asarray(mats, np.float32), **kw)
+    def build(self, mats, textures=None, material_textures=None, **kw) -> SceneData:
+        d = SceneData(np.concatenate(self.p).astype(np.float32), np.concatenate(self.n).astype(np.float32),
+                      np.concatenate(self.i).astype(np.uint32), np.concatenate(self.m),
+                      np.asarray(mats, np.float32), **kw)
+        if textures is not None:
+            d.uvs = np.concatenate(self.uv).astype(np.float32)
+            d.tangents = np.concatenate(self.tan).astype(np.float32)
+            d.textures = list(textures)
+            d.material_textures = np.asarray(material_textures, np.int32).reshape(-1, 5)
+        return d
 
 
 def grid_quad(origin, du, dv, n: int, normal):
@@ -156,6 +185,17 @@ def grid_quad(origin, du, dv, n: int, normal):
             v11 = (a + 1) * (n + 1) + b + 1
             idx += [(v00, v10, v11), (v00, v11, v01)]
     return pos, nrm, np.array(idx)
+
+
+def grid_frame(du, dv, n: int, scale=(1.0, 1.0), offset=(0.0, 0.0)):
+    """Texture coordinates of grid_quad's vertices ((s, t) * scale + offset)
+    and its tangent frame {du / |du|, dv / |dv|}."""
+    s = np.linspace(0.0, 1.0, n + 1)
+    S, T = np.meshgrid(s, s, indexing="ij")
+    uv = np.stack([S.reshape(-1) * scale[0] + offset[0], T.reshape(-1) * scale[1] + offset[1]], axis=1)
+    du, dv = np.asarray(du, np.float64), np.asarray(dv, np.float64)
+    tan = np.concatenate([du / np.linalg.norm(du), dv / np.linalg.norm(dv)])
+    return uv, np.tile(tan, (len(uv), 1))
 
 
 def box_mesh(center, half, n: int):
@@ -420,3 +460,91 @@ def sky_room(seed: int = 4) -> SceneData:
           np.array([(0, 2, 1), (0, 3, 2)]), len(mats) - 1)
     cam = dict(eye=(0.0, 3.0, 4.0), lookat=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=50.0)
     return b.build(mats, camera=cam, name="sky_room")
+
+
+def _texture(rng: SplitMix64, w: int, h: int, fn) -> np.ndarray:
+    img = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        for x in range(w):
+            img[y, x] = np.clip(np.round(np.asarray(fn(x, y, rng), np.float64) * 255.0), 0, 255)
+    return img
+
+
+def textured_room(seed: int = 6) -> SceneData:
+    """The Cornell room with every texture kind of calculate_material /
+    calculate_normal in use (RGBA8 images, 5 per material: diffuse, normal,
+    specular, emission, roughness):
+    * floor: checker diffuse texture tiled 3 x 3 (wrap addressing) and a
+      roughness texture;
+    * back wall: noise diffuse, normal map, specular texture (the GGX lobe);
+    * left wall: the normal map alone; the tall box: the noise diffuse at
+      negative texture coordinates; the icosphere: checker diffuse and the
+      normal map over a spherical (u, v) and tangent frame;
+    * the ceiling light: an emission texture beside its constant emission
+      15; a small wall quad with an emission texture and zero constant
+      emission (its triangles are lights of zero intensity);
+    * right wall, ceiling and the short box untextured."""
+    rng = SplitMix64(seed)
+    checker = _texture(rng, 16, 16, lambda x, y, r: (0.85, 0.8, 0.7, 1.0) if (x // 4 + y // 4) % 2 else
+                       (0.15, 0.25, 0.45, 1.0))
+    noise = _texture(rng, 32, 32, lambda x, y, r: (r.uniform(0.1, 0.9), r.uniform(0.1, 0.9), r.uniform(0.1, 0.9), 1.0))
+
+    def bump(x, y, r):
+        t = np.array([r.uniform(-0.5, 0.5), r.uniform(-0.5, 0.5), 1.0])
+        t /= np.linalg.norm(t)
+        return (*(0.5 * t + 0.5), 1.0)
+
+    normal = _texture(rng, 32, 32, bump)
+    spec = _texture(rng, 8, 8, lambda x, y, r: (r.uniform(0.0, 0.6),) * 3 + (1.0,))
+    rough = _texture(rng, 16, 16, lambda x, y, r: (r.uniform(0.05, 1.0), 0.0, 0.0, 1.0))
+    glow = _texture(rng, 4, 4, lambda x, y, r: (r.uniform(0.5, 1.0), r.uniform(0.5, 1.0), r.uniform(0.5, 1.0), 1.0))
+    CHECKER, NOISE, NORMAL, SPEC, ROUGH, GLOW = range(6)
+    mats, mtex, b = [], [], _Builder()
+
+    def mat(m, diffuse=-1, nmap=-1, specular=-1, emission=-1, roughness=-1):
+        mats.append(m)
+        mtex.append((diffuse, nmap, specular, emission, roughness))
+        return len(mats) - 1
+
+    walls = [((-1, 0, -1), (2, 0, 0), (0, 0, 2), (0, 1, 0), dict(diffuse=CHECKER, roughness=ROUGH), (3.0, 3.0)),
+             ((-1, 2, -1), (0, 0, 2), (2, 0, 0), (0, -1, 0), {}, (1.0, 1.0)),
+             ((-1, 0, -1), (0, 2, 0), (2, 0, 0), (0, 0, 1), dict(diffuse=NOISE, nmap=NORMAL, specular=SPEC),
+              (1.0, 1.0)),
+             ((-1, 0, -1), (0, 0, 2), (0, 2, 0), (1, 0, 0), dict(nmap=NORMAL), (2.0, 2.0)),
+             ((1, 0, -1), (0, 2, 0), (0, 0, 2), (-1, 0, 0), {}, (1.0, 1.0))]
+    for o, du, dv, nrm, tex, scale in walls:
+        m = mat(material([rng.uniform(0.2, 0.8) for _ in range(3)], rng.uniform(0.2, 1.0),
+                         specular=(0.2, 0.2, 0.2) if "specular" in tex else (0.0, 0.0, 0.0)), **tex)
+        uv, tan = grid_frame(du, dv, 16, scale)
+        b.add(*grid_quad(o, du, dv, 16, nrm), m, uv, tan)
+    for center, half, tex in (((-0.4, 0.6, -0.35), (0.3, 0.6, 0.3), dict(diffuse=NOISE)),
+                              ((0.45, 0.3, 0.3), (0.3, 0.3, 0.3), {})):
+        m = mat(material([rng.uniform(0.2, 0.8) for _ in range(3)], rng.uniform(0.05, 1.0)), **tex)
+        for face in box_mesh(center, half, 10):
+            pos = face[0]
+            du = pos[11] - pos[0]  # grid_quad: vertex (a=1, b=0) minus (0, 0), times n
+            dv = pos[1] - pos[0]
+            uv, tan = grid_frame(du * 10, dv * 10, 10, (2.0, 2.0), (-1.0, -1.5))
+            b.add(*face, m, uv, tan)
+    m = mat(material((0.7, 0.7, 0.7), 0.4), diffuse=CHECKER, nmap=NORMAL)
+    v, f = icosphere(4)
+    r, c = 0.28, np.array([0.35, 0.88, 0.25])
+    uv = np.stack([np.arctan2(v[:, 0], v[:, 2]) / (2 * np.pi) + 0.5, np.arccos(np.clip(v[:, 1], -1, 1)) / np.pi],
+                  axis=1) * 2.0
+    t = np.stack([v[:, 2], np.zeros(len(v)), -v[:, 0]], axis=1)
+    t[np.linalg.norm(t, axis=1) < 1e-6] = (1.0, 0.0, 0.0)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    bt = np.cross(v, t)
+    b.add(c + r * v, v, f, m, uv, np.concatenate([t, bt], axis=1))
+    m = mat(material((0.8, 0.8, 0.8), 1.0, emission=(15.0,) * 3), emission=GLOW)
+    h = 0.25
+    quad_uv = np.array([(0, 0), (1, 0), (1, 1), (0, 1)], np.float64)
+    quad_tan = np.tile((1.0, 0.0, 0.0, 0.0, 0.0, 1.0), (4, 1))
+    b.add(np.array([(-h, 1.99, -h), (h, 1.99, -h), (h, 1.99, h), (-h, 1.99, h)]),
+          np.tile((0.0, -1.0, 0.0), (4, 1)), np.array([(0, 2, 1), (0, 3, 2)]), m, quad_uv, quad_tan)
+    m = mat(material((0.5, 0.5, 0.5), 1.0), emission=GLOW)
+    b.add(np.array([(0.99, 0.9, -0.2), (0.99, 0.9, 0.2), (0.99, 1.3, 0.2), (0.99, 1.3, -0.2)]),
+          np.tile((-1.0, 0.0, 0.0), (4, 1)), np.array([(0, 2, 1), (0, 3, 2)]), m, quad_uv, quad_tan)
+    cam = dict(eye=(0.0, 1.0, 3.5), lookat=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+    return b.build(mats, textures=[checker, noise, normal, spec, rough, glow], material_textures=mtex, camera=cam,
+                   name="textured_room")

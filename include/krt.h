@@ -69,6 +69,34 @@ typedef struct {
 
 int krt_scene_upload(krt_context *ctx, const krt_scene_desc *desc, krt_handle *scene);
 
+/* Material textures and normal maps of a flat scene: calculate_material and
+ * calculate_normal (source/optix/amadeus_common.cuh:74-187) with the textures
+ * of convert_material (source/amadeus/armada.cu:333-398) and the vertex
+ * attributes tex_coords / tangent / bitangent (include/vertex.hpp:15-20).
+ * A hit's (u, v) is the interpolated tex_coords with v flipped
+ * (amadeus_common.cuh:498-502); a texture is read as tex2D <float4> of an
+ * RGBA8 image with normalised coordinates, bilinear filtering and wrap
+ * addressing (include/cuda/interop.cuh:11-51), filtered in FP32.  Diffuse,
+ * specular and emission textures replace the material's values by the
+ * texel's rgb, the roughness texture by its r; a normal map tilts the shading
+ * normal into the tangent frame.  Triangles of a material with an emission
+ * texture are triangle lights too, with the material's constant emission
+ * (armada.cu:117-150, :264-265).  ReSTIR GI and the editor's ReSTIR refuse
+ * textured scenes (KRT_ERR_STATE). */
+typedef struct {
+    const float *uvs;       /* nv x 2: Vertex::tex_coords */
+    const float *tangents;  /* nv x 6: Vertex::tangent, Vertex::bitangent; may be NULL without normal maps */
+    uint32_t nv;            /* = the scene's nv */
+    const uint32_t *texels; /* RGBA8 texels (r in the low byte), row-major, every texture back to back */
+    const uint32_t *dims;   /* ntex x 2: width, height (each 1..16384) */
+    uint32_t ntex;          /* < 65535 */
+    const int32_t *material_textures; /* nm x 5: diffuse, normal, specular, emission, roughness texture or -1 */
+    uint32_t nm;            /* = the scene's nm */
+} krt_scene_textures;
+
+int krt_scene_upload_textured(krt_context *ctx, const krt_scene_desc *desc, const krt_scene_textures *textures,
+                              krt_handle *scene);
+
 /* Two-level scene: base meshes in their own space and instances of them with
  * a row-major 3x4 transform (OptixInstance::transform).  Replaces
  * Accelerator::update (one GAS per entity submesh,

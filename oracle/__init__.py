@@ -39,6 +39,8 @@ _SIGS = {
     "orc_scene_create": (vp, [fp, fp, C.c_int, u32p, u32p, C.c_int, fp, C.c_int, fp, C.c_int]),
     "orc_scene_destroy": (None, [vp]),
     "orc_scene_set_envmap": (None, [vp, fp, C.c_uint32, C.c_uint32]),
+    "orc_scene_set_textures": (None, [vp, fp, fp, u32p, u32p, C.c_int, i32p]),
+    "orc_tex_lookup": (None, [u32p, C.c_uint32, C.c_uint32, fp, C.c_int, fp]),
     "orc_env_math": (None, [fp, C.c_int, C.c_int, fp]),
     "orc_env_lookup": (None, [vp, fp, C.c_int, C.c_int, fp]),
     "orc_env_sample_dirs": (None, [C.c_uint32, C.c_int, fp]),
@@ -207,6 +209,28 @@ def params(width, height, max_depth=5, samples=0, accumulate=1, user_seed=0, can
                      neighbours, radius, fixed_offset)
 
 
+def pack_textures(textures):
+    """RGBA8 images ((H, W, 4) uint8 each) as one uint32 texel array (r in
+    the low byte, row-major, back to back) and their (width, height) pairs."""
+    parts, dims = [], []
+    for t in textures:
+        a = np.ascontiguousarray(t, np.uint8)
+        assert a.ndim == 3 and a.shape[2] == 4, "textures are (H, W, 4) uint8"
+        parts.append(a.reshape(-1, 4).view("<u4").reshape(-1))
+        dims.append((a.shape[1], a.shape[0]))
+    texels = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1, np.uint32), np.uint32)
+    return texels, np.ascontiguousarray(np.array(dims, np.uint32).reshape(-1, 2))
+
+
+def tex_lookup(texture, uv) -> np.ndarray:
+    """The texture fetch of one (H, W, 4) uint8 image at (n, 2) coordinates."""
+    texels, dims = pack_textures([texture])
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.empty((len(uv), 4), np.float32)
+    lib().orc_tex_lookup(_u(texels), int(dims[0, 0]), int(dims[0, 1]), _f(uv), len(uv), _f(out))
+    return out
+
+
 class OracleScene:
     """Oracle view of a scene (positions, normals, indices, materials, BVH)."""
 
@@ -226,6 +250,19 @@ class OracleScene:
         p, n, i, m, mats = self._keep
         self.h = lib().orc_scene_create(_f(p), _f(n), len(p), _u(i), _u(m), len(i), _f(mats), len(mats),
                                         _f(self.nodes), len(self.nodes))
+        if getattr(data, "textures", None) is not None:
+            self._set_textures(data)
+
+    def _set_textures(self, d):
+        """Material textures (SceneData.uvs / tangents / textures /
+        material_textures): calculate_material / calculate_normal."""
+        uv = np.ascontiguousarray(d.uvs, np.float32)
+        tan = None if d.tangents is None else np.ascontiguousarray(d.tangents, np.float32)
+        texels, dims = pack_textures(d.textures)
+        mt = np.ascontiguousarray(d.material_textures, np.int32)
+        self._keep += [uv, tan, texels, dims, mt]
+        lib().orc_scene_set_textures(self.h, _f(uv), None if tan is None else _f(tan), _u(texels), _u(dims),
+                                     len(dims), mt.ctypes.data_as(i32p))
 
     def _init_instanced(self, d):
         pos = np.ascontiguousarray(np.concatenate([m[0] for m in d.meshes]), np.float32)

@@ -72,6 +72,21 @@ struct orc_scene {
     std::vector<float> tlas;  // bvh.cpp tree over the instances' world boxes
     int ntlas = 0;
     std::vector<int32_t> inst_order;  // instances in preorder of the instance tree
+    // material textures (orc_scene_set_textures): per-vertex tex_coords and
+    // {tangent, bitangent}, RGBA8 texels and {offset, w, h} per texture, 5
+    // texture ids per material {diffuse, normal, specular, emission,
+    // roughness} (-1: none)
+    bool textured = false;
+    std::vector<float> uv, tan;
+    std::vector<uint32_t> texels, tex_desc;
+    std::vector<int32_t> mat_tex;
+    Texture texture(int32_t id) const {
+        Texture t;
+        t.texels = texels.data() + tex_desc[3 * id];
+        t.w = tex_desc[3 * id + 1];
+        t.h = tex_desc[3 * id + 2];
+        return t;
+    }
     // environment map (the armada's, ArmadaRTX::set_envmap): RGBA texels
     std::vector<float> env;
     uint32_t env_w = 0, env_h = 0;
@@ -140,6 +155,38 @@ orc_scene *orc_scene_create(const float *pos, const float *nrm, int nv, const ui
         }
     }
     return s;
+}
+
+// Material textures of a flat scene; the triangle lights are rebuilt with the
+// emissive test of armada.cu:264-265 (|emission| > 0 or an emission texture),
+// each with its material's constant emission (armada.cu:143).
+void orc_scene_set_textures(orc_scene *s, const float *uvs, const float *tangents, const uint32_t *texels,
+                            const uint32_t *dims, int ntex, const int32_t *mat_tex) {
+    const size_t nv = s->pos.size(), nm = s->mats.size();
+    s->textured = true;
+    s->uv.assign(uvs, uvs + 2 * nv);
+    if (tangents) s->tan.assign(tangents, tangents + 6 * nv);
+    else s->tan.assign(6 * nv, 0.0f);
+    s->mat_tex.assign(mat_tex, mat_tex + 5 * nm);
+    s->tex_desc.clear();
+    size_t total = 0;
+    for (int t = 0; t < ntex; t++) {
+        s->tex_desc.push_back((uint32_t)total);
+        s->tex_desc.push_back(dims[2 * t]);
+        s->tex_desc.push_back(dims[2 * t + 1]);
+        total += (size_t)dims[2 * t] * dims[2 * t + 1];
+    }
+    s->texels.assign(texels, texels + total);
+    s->lights.clear();
+    const int nt = (int)s->tri_mat.size();
+    for (int t = 0; t < nt; t++) {
+        const uint32_t m = s->tri_mat[t];
+        const Material &M = s->mats[m];
+        if (length(M.emission) > 0.0f || s->mat_tex[5 * m + 3] >= 0) {
+            f3 a = s->pos[s->idx[3 * t]], b = s->pos[s->idx[3 * t + 1]], c = s->pos[s->idx[3 * t + 2]];
+            s->lights.push_back(TriLight{a, b - a, c - a, M.emission});
+        }
+    }
 }
 
 // Inverse of a row-major 3x4 affine transform: cofactors of the 3x3 part in
@@ -277,6 +324,14 @@ void orc_env_lookup(const orc_scene *s, const float *dirs, int n, int flip_v, fl
         out[3 * (size_t)i + 1] = L.y;
         out[3 * (size_t)i + 2] = L.z;
     }
+}
+
+void orc_tex_lookup(const uint32_t *texels, uint32_t w, uint32_t h, const float *uv, int n, float *out) {
+    Texture t;
+    t.texels = texels;
+    t.w = w;
+    t.h = h;
+    for (int i = 0; i < n; i++) tex_fetch(t, uv[2 * (size_t)i], uv[2 * (size_t)i + 1], out + 4 * (size_t)i);
 }
 
 void orc_env_sample_dirs(uint32_t seed_word, int n, float *out) {
@@ -1018,7 +1073,47 @@ bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
     if (dot(nn, ng) < 0.0f) nn = -nn;
     sf.n = normalize(nn);
     sf.x = interp(a, b, c, h.b1, h.b2);
-    sf.mat = s->mats[s->tri_mat[h.prim]];
+    const uint32_t m = s->tri_mat[h.prim];
+    sf.mat = s->mats[m];
+    if (s->textured) {
+        // uv: interpolated tex_coords, v flipped (amadeus_common.cuh:498-502)
+        const float *ta = &s->uv[2 * (size_t)ix[0]], *tb = &s->uv[2 * (size_t)ix[1]], *tc = &s->uv[2 * (size_t)ix[2]];
+        const float w = 1.0f - h.b1 - h.b2;
+        const float u = (w * ta[0] + h.b1 * tb[0]) + h.b2 * tc[0];
+        const float v = 1.0f - ((w * ta[1] + h.b1 * tb[1]) + h.b2 * tc[1]);
+        const int32_t *mt = &s->mat_tex[5 * (size_t)m];
+        float t4[4];
+        // calculate_material (:154-187)
+        if (mt[0] >= 0) {
+            tex_fetch(s->texture(mt[0]), u, v, t4);
+            sf.mat.diffuse = mk(t4[0], t4[1], t4[2]);
+        }
+        if (mt[2] >= 0) {
+            tex_fetch(s->texture(mt[2]), u, v, t4);
+            sf.mat.specular = mk(t4[0], t4[1], t4[2]);
+        }
+        if (mt[3] >= 0) {
+            tex_fetch(s->texture(mt[3]), u, v, t4);
+            sf.mat.emission = mk(t4[0], t4[1], t4[2]);
+        }
+        if (mt[4] >= 0) {
+            tex_fetch(s->texture(mt[4]), u, v, t4);
+            sf.mat.roughness = t4[0];
+        }
+        // calculate_normal's normal map (:114-146): tbn = mat3(normalize(T),
+        // normalize(B), normalize(n)) with T and B normalised once before
+        // (glm::normalize(model * ...)); n = normalize(tbn * (2 rgb - 1))
+        if (mt[1] >= 0) {
+            tex_fetch(s->texture(mt[1]), u, v, t4);
+            const f3 tn = mk(2.0f * t4[0] - 1.0f, 2.0f * t4[1] - 1.0f, 2.0f * t4[2] - 1.0f);
+            const float *A = &s->tan[6 * (size_t)ix[0]], *B = &s->tan[6 * (size_t)ix[1]], *C = &s->tan[6 * (size_t)ix[2]];
+            const f3 T = normalize(normalize(interp(load3(A), load3(B), load3(C), h.b1, h.b2)));
+            const f3 Bt = normalize(normalize(interp(load3(A + 3), load3(B + 3), load3(C + 3), h.b1, h.b2)));
+            const f3 N = normalize(sf.n);
+            const f3 col0 = T * tn.x, col1 = Bt * tn.y, col2 = N * tn.z;
+            sf.n = normalize((col0 + col1) + col2);
+        }
+    }
     return !isnan3(sf.n);
 }
 

@@ -64,6 +64,16 @@ int orc_scene_tlas(const orc_scene *s, float *nodes);
 void orc_affine_inverse(const float *M, float *Minv);
 void orc_instance_box(const float *root_min, const float *root_max, const float *M, float *box6);
 void orc_scene_destroy(orc_scene *s);
+/* Material textures of a flat scene (calculate_material / calculate_normal,
+ * amadeus_common.cuh:74-187): nv x 2 tex_coords, nv x 6 {tangent, bitangent}
+ * (may be NULL), RGBA8 texels of ntex textures back to back (dims: w, h
+ * each), 5 texture ids per material {diffuse, normal, specular, emission,
+ * roughness} (-1: none).  Rebuilds the triangle lights. */
+void orc_scene_set_textures(orc_scene *s, const float *uvs, const float *tangents, const uint32_t *texels,
+                            const uint32_t *dims, int ntex, const int32_t *mat_tex);
+/* The texture fetch, for its pins: n lookups (u, v) of one w x h RGBA8
+ * texture, 4 floats each. */
+void orc_tex_lookup(const uint32_t *texels, uint32_t w, uint32_t h, const float *uv, int n, float *out);
 /* The armada's environment map (ArmadaRTX::set_envmap): w x h RGBA float
  * texels, row-major; rgba == NULL removes it. */
 void orc_scene_set_envmap(orc_scene *s, const float *rgba, uint32_t w, uint32_t h);

@@ -436,6 +436,40 @@ static inline f3 env_radiance(const EnvMap &env, f3 d, bool flip_v) {
     return env_fetch(env, u, flip_v ? 1.0f - v : v);
 }
 
+// Material textures (calculate_material / calculate_normal,
+// amadeus_common.cuh:74-187): tex2D <float4> of an RGBA8 image
+// (import_vulkan_texture, include/cuda/interop.cuh:11-51: read as byte / 255,
+// normalised coordinates, bilinear, wrap addressing), filtered in FP32;
+// coordinates beyond +-2^16 (or NaN) read zero.
+struct Texture {
+    const uint32_t *texels = nullptr;  // r in the low byte, row-major
+    uint32_t w = 0, h = 0;
+};
+
+static inline void texel_rgba(uint32_t t, float out[4]) {
+    for (int k = 0; k < 4; k++) out[k] = (float)((t >> (8 * k)) & 255u) / 255.0f;
+}
+
+static inline void tex_fetch(const Texture &tx, float u, float v, float out[4]) {
+    if (!(std::fabs(u) <= 65536.0f && std::fabs(v) <= 65536.0f)) {
+        for (int k = 0; k < 4; k++) out[k] = 0.0f;
+        return;
+    }
+    const float x = u * (float)tx.w - 0.5f, y = v * (float)tx.h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy, W = (int)tx.w, H = (int)tx.h;
+    const int i0 = env_wrap(ix, W), i1 = env_wrap(ix + 1, W), j0 = env_wrap(iy, H), j1 = env_wrap(iy + 1, H);
+    float t00[4], t10[4], t01[4], t11[4];
+    texel_rgba(tx.texels[(size_t)j0 * tx.w + i0], t00);
+    texel_rgba(tx.texels[(size_t)j0 * tx.w + i1], t10);
+    texel_rgba(tx.texels[(size_t)j1 * tx.w + i0], t01);
+    texel_rgba(tx.texels[(size_t)j1 * tx.w + i1], t11);
+    const float a0 = 1.0f - a, b0 = 1.0f - b;
+    const float w00 = a0 * b0, w10 = a * b0, w01 = a0 * b, w11 = a * b;
+    for (int k = 0; k < 4; k++) out[k] = (t00[k] * w00 + t10[k] * w10) + (t01[k] * w01 + t11[k] * w11);
+}
+
 // Ld_Environment's direction: one pcg3f step; theta = acos(sqrt(1 - u)) about
 // world +z (cos = sqrt(1 - u), sin = sqrt(1 - cos^2)), phi = 2 pi u'
 static inline f3 env_sample_dir(f3 &seed) {

@@ -271,6 +271,48 @@ def case_envmap(ctx):
     sc.free()
 
 
+def case_textures(ctx):
+    """Material textures and normal maps: the path tracer, ReSTIR DI under a
+    moving camera and a served request on a lane, bit for bit vs the
+    oracle."""
+    from artemis_amd import Server
+
+    s = scenes.textured_room()
+    sc = ctx.upload(s)
+    osc = O.OracleScene(s, sc.serialized())
+    assert bits_equal(sc.lights(), osc.lights()), "lights"
+    W, H = 24, 16
+    cam = look_at(**s.camera, aspect=W / H)
+    arm = ArmadaRTX(ctx, W, H)
+    arm.attach("pt", PathTracer(ctx))
+    arm.set_depth(4)
+    for _ in range(2):
+        arm.render(sc, cam)
+    ref, _ = osc.render_pt(cam, W, H, frames=2, max_depth=4)
+    for k in range(4):
+        assert bits_equal(arm._read(k), ref[k]), f"path tracer buffer {k}"
+    say("textures: path tracer bit-exact")
+    del arm
+    arm = ArmadaRTX(ctx, W, H)
+    arm.attach("restir", ReSTIR(ctx))
+    ors = O.OracleReSTIR(W, H)
+    for f in range(3):
+        c = look_at((0.2 * f - 0.2, 1.0, 3.5), (0.0, 0.9, 0.0), aspect=W / H)
+        arm.render(sc, c, accumulate=False)
+        ors.frame(osc, c, max_depth=2, reset=True)
+        for which in (0, 1):
+            assert bits_equal(arm.reservoirs(which), ors.reservoirs(which)), f"restir reservoirs {which}"
+        assert bits_equal(arm.color_buffer(), ors.buffers[0]), "restir colour"
+    say("textures: ReSTIR DI (moving camera) bit-exact")
+    del arm
+    srv = Server(ctx, wave_pixels=0)
+    img, _ = srv.wait(srv.submit(sc, cam, W, H, spp=2, max_depth=4, seed=0))
+    srv.close()
+    assert bits_equal(img, ref[0]), "served request"
+    say("textures: served request on a lane bit-exact")
+    sc.free()
+
+
 def case_refit(ctx):
     """Moved geometry: update_vertices makes the scene stale (renders refuse),
     refit gives the numpy restatement's tree bit for bit and frames equal the
@@ -331,6 +373,7 @@ def case_mlp(ctx):
 
 
 CASES = {"flat_pt": case_flat_pt, "attachments": case_attachments, "envmap": case_envmap, "refit": case_refit,
+         "textures": case_textures,
          "mlp": case_mlp, "instancing": case_instancing,
          "instancing_full": lambda ctx: case_instancing(ctx, full=True)}
 

@@ -9,6 +9,8 @@ behaviour, or a result that depends on memory nothing wrote, fails it.
 * attachments: path tracer, ReSTIR DI and ReSTIR GI under a moving camera,
   the editor's G-buffer ReSTIR, a served request, the cached path tracer;
 * envmap: the path tracer and ReSTIR DI with the armada's environment map;
+* textures: material textures and normal maps in the path tracer, ReSTIR DI
+  under a moving camera and a served request;
 * refit: moved vertices, the bottom-up refit of both trees and the lights;
 * mlp: the radiance-cache network's inference and gradients (emulated MFMA);
 * instancing: the sequence of tests/test_gpu_instancing.py on the two-level
@@ -45,7 +47,7 @@ def emu_env():
     return env
 
 
-@pytest.mark.parametrize("case", ["attachments", "envmap", "refit", "mlp", "instancing"])
+@pytest.mark.parametrize("case", ["attachments", "envmap", "textures", "refit", "mlp", "instancing"])
 def test_sanitized_case(emu_env, case):
     r = subprocess.run([sys.executable, str(EMU / "run_case.py"), case], env=emu_env, capture_output=True,
                        text=True, timeout=900)
