@@ -354,6 +354,17 @@ def write_png(path, rgba: np.ndarray) -> None:
     L.check(L.lib().krt_write_png(str(path).encode(), rgba.ctypes.data_as(C.POINTER(C.c_uint32)), w, h))
 
 
+def read_png(path) -> np.ndarray:
+    """PNG file -> (H, W, 4) uint8 RGBA, rows in file order (krt_read_png:
+    every colour type and bit depth, interlacing, tRNS)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    L.check(L.lib().krt_read_png(str(path).encode(), C.byref(w), C.byref(h), None, 0))
+    out = np.empty((h.value, w.value), np.uint32)
+    L.check(L.lib().krt_read_png(str(path).encode(), C.byref(w), C.byref(h),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size))
+    return out.view(np.uint8).reshape(h.value, w.value, 4)
+
+
 def load_attachment(ctx: Context, kind: str):
     """Plugin entry (source/amadeus/naive_path_tracer.cu:280-344)."""
     return {"path_tracer": PathTracer, "restir": ReSTIR, "restir_gi": ReSTIRGI, "mamba": Mamba}[kind](ctx)

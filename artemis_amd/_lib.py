@@ -150,6 +150,8 @@ _SIGS = {
     "krt_trace_closest_device": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
     "krt_trace_closest_device_noalloc": (C.c_int, [vp, handle, vp, C.c_uint32, vp]),
     "krt_write_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]),
+    "krt_read_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                               C.c_size_t]),
     "krt_armada_write_png": (C.c_int, [vp, C.c_int, C.c_char_p]),
     "krt_camera_rays": (C.c_int, [vp, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
                                   fp]),

@@ -1145,6 +1145,23 @@ int krt_write_png(const char *path, const uint32_t *rgba, uint32_t width, uint32
     });
 }
 
+int krt_read_png(const char *path, uint32_t *width, uint32_t *height, uint32_t *rgba, size_t capacity) {
+    return guard([&] {
+        KRT_REQUIRE(path && width && height, KRT_ERR_INVALID, "null argument");
+        if (!rgba) {
+            read_png_size(path, *width, *height);
+            return;
+        }
+        std::vector<uint32_t> px;
+        uint32_t w = 0, h = 0;
+        read_png(path, px, w, h);
+        *width = w;
+        *height = h;
+        KRT_REQUIRE(capacity >= px.size(), KRT_ERR_INVALID, "rgba holds fewer than width x height texels");
+        std::memcpy(rgba, px.data(), sizeof(uint32_t) * px.size());
+    });
+}
+
 int krt_armada_write_png(krt_armada *a, int tonemapping, const char *path) {
     return guard([&] {
         KRT_REQUIRE(a && path, KRT_ERR_INVALID, "null argument");

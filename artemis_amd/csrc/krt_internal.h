@@ -202,6 +202,9 @@ struct Context {
 
 // PNG file of packed RGBA8 pixels (krt_image.cpp)
 void write_png(const std::string &path, const uint32_t *rgba, uint32_t width, uint32_t height);
+// PNG file -> packed RGBA8 (r in the low byte), rows in file order
+void read_png(const std::string &path, std::vector<uint32_t> &rgba, uint32_t &width, uint32_t &height);
+void read_png_size(const std::string &path, uint32_t &width, uint32_t &height);
 
 // Builders / kernels (defined in the .hip units)
 void accel_build(Context &ctx, Scene &s, int leaf_max, int layout);

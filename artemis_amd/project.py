@@ -106,7 +106,9 @@ class Entity:
     position: tuple = (0.0, 0.0, 0.0)
     rotation: tuple = (0.0, 0.0, 0.0)
     scale: tuple = (1.0, 1.0, 1.0)
-    submeshes: list = field(default_factory=list)  # [(positions, normals, indices, material row)]
+    # [(positions, normals, indices, material row, tex_coords, {tangent,
+    # bitangent}, (albedo, normal, roughness) texture files or None)]
+    submeshes: list = field(default_factory=list)
     renderable: bool = False
     camera: tuple | None = None                    # (fov, aspect)
     light: tuple | None = None                     # (color, power, type) of the scene dialect
@@ -123,27 +125,43 @@ class KobraScene:
         instance per entity submesh (entity model matrix), materials
         deduplicated by value in first-use order."""
         meshes, mesh_of, inst_mesh, xforms, tri_mat, mats, mat_of = [], {}, [], [], [], [], {}
+        uvs, tans, mtex, images, image_of = [], [], [], [], {}
         for e in self.entities:
             if not e.submeshes or (renderable_only and not e.renderable):
                 continue
             M = transform_matrix(e.position, e.rotation, e.scale)
-            for pos, nrm, idx, mrow in e.submeshes:
+            for pos, nrm, idx, mrow, uv, tan, tex in e.submeshes:
                 key = id(pos)
                 if key not in mesh_of:
                     mesh_of[key] = len(meshes)
                     meshes.append((pos, nrm, idx))
-                mk = tuple(np.asarray(mrow, np.float32).tolist())
+                    uvs.append(np.zeros((len(pos), 2), np.float32) if uv is None else uv)
+                    tans.append(np.zeros((len(pos), 6), np.float32) if tan is None else tan)
+                slots = [-1] * 5  # diffuse, normal, specular, emission, roughness
+                for slot, f in zip((0, 1, 4), tex or (None, None, None)):
+                    if f is not None:
+                        if f not in image_of:
+                            image_of[f] = len(images)
+                            images.append(f)
+                        slots[slot] = image_of[f]
+                mk = tuple(np.asarray(mrow, np.float32).tolist()) + tuple(slots)
                 if mk not in mat_of:
                     mat_of[mk] = len(mats)
-                    mats.append(mk)
+                    mats.append(mk[:12])
+                    mtex.append(slots)
                 inst_mesh.append(mesh_of[key])
                 xforms.append(M)
                 tri_mat.append(np.full(len(idx), mat_of[mk], np.uint32))
         if not inst_mesh:
             raise ProjectError(f"scene {self.name!r} has no meshes")
-        return InstancedSceneData(meshes, np.array(inst_mesh, np.int32), np.stack(xforms).astype(np.float32),
-                                  np.concatenate(tri_mat), np.array(mats, np.float32), camera=self.camera(),
-                                  name=self.name)
+        d = InstancedSceneData(meshes, np.array(inst_mesh, np.int32), np.stack(xforms).astype(np.float32),
+                               np.concatenate(tri_mat), np.array(mats, np.float32), camera=self.camera(),
+                               name=self.name)
+        if images:
+            d.mesh_uvs, d.mesh_tangents = uvs, tans
+            d.textures = [load_texture(f) for f in images]
+            d.material_textures = np.array(mtex, np.int32)
+        return d
 
     def camera(self) -> dict:
         """look_at arguments of the first camera entity (eye, lookat, up,
@@ -165,8 +183,10 @@ class KobraScene:
 
 # --------------------------------------------------------------- file readers
 
-def read_submesh(path) -> tuple:
-    """A ``.submesh`` cache file -> (positions, normals, indices)."""
+def read_submesh(path, attributes: bool = False) -> tuple:
+    """A ``.submesh`` cache file -> (positions, normals, indices), with
+    attributes=True also tex_coords (nv, 2) and {tangent, bitangent}
+    (nv, 6)."""
     data = Path(path).read_bytes()
     if len(data) < 8:
         raise ProjectError(f"{path}: truncated submesh header")
@@ -178,17 +198,27 @@ def read_submesh(path) -> tuple:
     idx = np.frombuffer(data, "<i4", ni, 8 + 56 * nv).astype(np.int64)
     if ni and (idx.min() < 0 or idx.max() >= nv):
         raise ProjectError(f"{path}: index out of range")
-    return (np.ascontiguousarray(v["position"]), np.ascontiguousarray(v["normal"]),
-            idx.reshape(-1, 3).astype(np.uint32))
+    out = (np.ascontiguousarray(v["position"]), np.ascontiguousarray(v["normal"]),
+           idx.reshape(-1, 3).astype(np.uint32))
+    if attributes:
+        out += (np.ascontiguousarray(v["tex_coords"]),
+                np.ascontiguousarray(np.concatenate([v["tangent"], v["bitangent"]], axis=1)))
+    return out
 
 
-def write_submesh(path, positions, normals, indices) -> None:
+def write_submesh(path, positions, normals, indices, uvs=None, tangents=None) -> None:
     """The ``.submesh`` layout that read_submesh takes (transcribe_submesh,
-    source/project.cpp): tex_coords, tangent and bitangent are zero."""
+    source/project.cpp): tex_coords and {tangent, bitangent} as given, else
+    zero."""
     positions = np.asarray(positions, np.float32)
     v = np.zeros(len(positions), VERTEX_DTYPE)
     v["position"] = positions
     v["normal"] = np.asarray(normals, np.float32)
+    if uvs is not None:
+        v["tex_coords"] = np.asarray(uvs, np.float32)
+    if tangents is not None:
+        t = np.asarray(tangents, np.float32)
+        v["tangent"], v["bitangent"] = t[:, 0:3], t[:, 3:6]
     idx = np.asarray(indices, np.int32).reshape(-1)
     Path(path).write_bytes(struct.pack("<ii", len(v), len(idx)) + v.tobytes() + idx.tobytes())
 
@@ -211,8 +241,11 @@ def _fields(line: str, key: str, n: int, sep: str):
     return out
 
 
-def read_material(path) -> tuple:
-    """An ``assets/*.mat`` file -> (name, 12-float material row)."""
+def read_material(path, textures: bool = False) -> tuple:
+    """An ``assets/*.mat`` file -> (name, 12-float material row), with
+    textures=True also the albedo / normal / roughness texture names
+    (``""`` when absent: Material::has_albedo etc. treat "" and "0" as
+    none)."""
     lines = Path(path).read_text().split("\n")
     lines += [""] * (10 - len(lines))
     row = list(DEFAULT_MATERIAL)
@@ -224,16 +257,53 @@ def read_material(path) -> tuple:
     t = _fields(lines[6], "type: ", 1, ",")
     if t:
         row[11] = float(int(t[0]))
-    return name, row
+    if not textures:
+        return name, row
+    names = []
+    for k, key in enumerate(("albedo_texture: ", "normal_texture: ", "roughness_texture: ")):
+        ln = lines[7 + k] if 7 + k < len(lines) else ""
+        tok = ln[len(key):].split() if ln.startswith(key) else []
+        names.append("" if not tok or tok[0] == "0" else tok[0])
+    return name, row, tuple(names)
 
 
-def write_material(path, name: str, row) -> None:
-    """The text of transcribe_material (source/project.cpp) for a 12-float row."""
+def load_texture(path) -> np.ndarray:
+    """A texture file as (H, W, 4) uint8 in texture memory order: rows
+    flipped, as TextureLoader::load_texture loads them (flip = true,
+    include/backend.hpp:89, source/image.cpp:96).  PNG only."""
+    from . import read_png
+
+    return np.ascontiguousarray(read_png(path)[::-1])
+
+
+def resolve_texture(name: str, directory) -> Path | None:
+    """The file of a material's texture name: as given when absolute, else
+    under the project directory or its assets/; None (a warning) when it is
+    missing or not a PNG file, the reference's log-and-continue."""
+    import warnings
+
+    if not name:
+        return None
+    cands = [Path(name)] if Path(name).is_absolute() else [Path(directory) / name, Path(directory) / "assets" / name]
+    for c in cands:
+        if c.exists():
+            if c.suffix.lower() != ".png":
+                warnings.warn(f"texture {c}: only PNG textures are decoded; the material keeps its constants")
+                return None
+            return c.resolve()
+    warnings.warn(f"texture {name!r} not found under {directory}")
+    return None
+
+
+def write_material(path, name: str, row, textures=("", "", "")) -> None:
+    """The text of transcribe_material (source/project.cpp) for a 12-float
+    row and its albedo / normal / roughness texture names."""
     r = [float(x) for x in row]
+    a, n, ro = textures
     Path(path).write_text(
         f"name: {name}\ndiffuse: {r[0]:f},{r[1]:f},{r[2]:f}\nspecular: {r[3]:f},{r[4]:f},{r[5]:f}\n"
         f"emission: {r[6]:f},{r[7]:f},{r[8]:f}\nroughness: {r[9]:f}\nrefraction: {r[10]:f}\ntype: {int(r[11])}\n"
-        f"albedo_texture: \nnormal_texture: \nroughness_texture: ")
+        f"albedo_texture: {a}\nnormal_texture: {n}\nroughness_texture: {ro}")
 
 
 # ------------------------------------------------------ the project dialect
@@ -289,7 +359,7 @@ def parse_project_scene(text: str, directory, name: str = "") -> KobraScene:
             elif fld.startswith("camera"):
                 fov, aspect = (float(x) for x in fld[7:].split()[:2])
                 cur["camera"] = (fov, aspect)
-    materials = {}  # name -> row: the daemon's name lookup, first load wins
+    materials = {}  # name -> (row, textures): the daemon's name lookup, first load wins
     cache = {}
     entities = []
     for el in elements:
@@ -301,17 +371,18 @@ def parse_project_scene(text: str, directory, name: str = "") -> KobraScene:
             e.position, e.rotation, e.scale = tuple(t[0:3]), tuple(t[3:6]), tuple(t[6:9])
         for mesh, mat in zip(el.get("mesh", []), el.get("material", [])):
             mpath = directory / "assets" / mat
-            row = list(DEFAULT_MATERIAL)
+            row, tex = list(DEFAULT_MATERIAL), ("", "", "")
             if mpath.exists():
-                mname, row = read_material(mpath)
-                row = materials.setdefault(mname, row)
+                mname, row, tex = read_material(mpath, textures=True)
+                row, tex = materials.setdefault(mname, (row, tex))
             spath = directory / ".cache" / mesh
             if not spath.exists():
                 continue  # the reference logs and skips a missing mesh file
             if spath not in cache:
-                cache[spath] = read_submesh(spath)
-            pos, nrm, idx = cache[spath]
-            e.submeshes.append((pos, nrm, idx, row))
+                cache[spath] = read_submesh(spath, attributes=True)
+            pos, nrm, idx, uv, tan = cache[spath]
+            files = tuple(resolve_texture(t, directory) for t in tex)
+            e.submeshes.append((pos, nrm, idx, row, uv, tan, files if any(files) else None))
         e.renderable = bool(el.get("renderable", False))
         e.camera = el.get("camera")
         entities.append(e)
@@ -392,7 +463,7 @@ def parse_scene_file(text: str, name: str = "") -> KobraScene:
                     # raw vertices carry no normals: the face normals, as the reference's
                     # Vertex(position) constructor leaves them unset
                     n = _vertex_normals(p, idx)
-                    e.submeshes.append((p, n, idx, list(DEFAULT_MATERIAL)))
+                    e.submeshes.append((p, n, idx, list(DEFAULT_MATERIAL), None, None, None))
             elif h == "[RENDERABLE]":
                 e.renderable = True
             elif h == "[CAMERA]":
@@ -452,15 +523,35 @@ def save_project(directory, scenes: dict) -> None:
     ``.kobra`` per scene and ``project.den``.  `scenes` maps a name to
     (InstancedSceneData, [(entity name, position, rotation, scale, instance
     ids)], camera entity or None) — the transforms are the entities', so the
-    instance transforms of the data are not used."""
+    instance transforms of the data are not used.  Textured data (mesh_uvs,
+    mesh_tangents, textures) writes its tex_coords and tangent frames into
+    the submeshes and every texture a material's albedo / normal /
+    roughness slot uses as ``assets/<scene>-texture-<i>.png`` (rows flipped
+    back to file order); the .mat format has no specular or emission
+    texture, so those slots must be empty."""
     directory = Path(directory)
     (directory / ".cache").mkdir(parents=True, exist_ok=True)
     (directory / "assets").mkdir(parents=True, exist_ok=True)
     for sname, (data, ents, cam) in scenes.items():
+        textured = getattr(data, "textures", None) is not None
         for m, (pos, nrm, idx) in enumerate(data.meshes):
-            write_submesh(directory / ".cache" / f"{sname}-submesh-{m}.submesh", pos, nrm, idx)
+            write_submesh(directory / ".cache" / f"{sname}-submesh-{m}.submesh", pos, nrm, idx,
+                          data.mesh_uvs[m] if textured else None, data.mesh_tangents[m] if textured else None)
+        if textured:
+            from . import write_png
+
+            for t, img in enumerate(data.textures):
+                rgba = np.ascontiguousarray(np.asarray(img, np.uint8)[::-1])
+                write_png(directory / "assets" / f"{sname}-texture-{t}.png",
+                          rgba.view(np.uint32).reshape(rgba.shape[0], rgba.shape[1]))
         for k, row in enumerate(data.materials):
-            write_material(directory / "assets" / f"{sname}-material-{k}.mat", f"{sname}-material-{k}", row)
+            names = ("", "", "")
+            if textured:
+                slots = np.asarray(data.material_textures[k])
+                if slots[2] >= 0 or slots[3] >= 0:
+                    raise ProjectError("the .mat format holds no specular or emission texture")
+                names = tuple(f"{sname}-texture-{int(slots[j])}.png" if slots[j] >= 0 else "" for j in (0, 1, 4))
+            write_material(directory / "assets" / f"{sname}-material-{k}.mat", f"{sname}-material-{k}", row, names)
         starts = np.concatenate([[0], np.cumsum([len(data.meshes[m][2]) for m in data.inst_mesh])])
         out = []
         for ename, p, r, s, insts in ents:

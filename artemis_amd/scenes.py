@@ -115,6 +115,13 @@ class InstancedSceneData:
     materials: np.ndarray     # (nm, 12) float32
     camera: dict = field(default_factory=dict)
     name: str = ""
+    # material textures (as SceneData's): per mesh (nv, 2) tex_coords and
+    # (nv, 6) {tangent, bitangent} in its own space, RGBA8 images, 5 texture
+    # slots per material
+    mesh_uvs: list | None = None
+    mesh_tangents: list | None = None
+    textures: list | None = None
+    material_textures: np.ndarray | None = None
 
     @property
     def triangles(self) -> int:
@@ -122,17 +129,31 @@ class InstancedSceneData:
 
     def flatten(self) -> SceneData:
         """The same scene as one world-space soup (vertices through the
-        transforms in float32, normals through the 3x3 part)."""
-        P, N, I = [], [], []
+        transforms in float32, normals through the 3x3 part).  Tangent
+        frames go through the 3x3 part twice: calculate_normal applies the
+        model matrix to the interpolated tangent and bitangent and then again
+        (amadeus_common.cuh:125-136), which a world-space soup reproduces
+        with the product taken before the interpolation."""
+        P, N, I, UV, T = [], [], [], [], []
         nv = 0
         for i, m in enumerate(self.inst_mesh):
             pos, nrm, idx = self.meshes[m]
-            P.append(xform_points(self.transforms[i], pos, 1.0))
-            N.append(xform_points(self.transforms[i], nrm, 0.0))
+            M = self.transforms[i]
+            P.append(xform_points(M, pos, 1.0))
+            N.append(xform_points(M, nrm, 0.0))
             I.append(idx + nv)
+            if self.textures is not None:
+                UV.append(np.asarray(self.mesh_uvs[m], np.float32))
+                t = np.asarray(self.mesh_tangents[m], np.float32)
+                T.append(np.concatenate([xform_points(M, xform_points(M, t[:, 0:3], 0.0), 0.0),
+                                         xform_points(M, xform_points(M, t[:, 3:6], 0.0), 0.0)], axis=1))
             nv += len(pos)
-        return SceneData(np.concatenate(P), np.concatenate(N), np.concatenate(I).astype(np.uint32),
-                         self.tri_material, self.materials, camera=self.camera, name=self.name + "_flat")
+        d = SceneData(np.concatenate(P), np.concatenate(N), np.concatenate(I).astype(np.uint32),
+                      self.tri_material, self.materials, camera=self.camera, name=self.name + "_flat")
+        if self.textures is not None:
+            d.uvs, d.tangents = np.concatenate(UV), np.concatenate(T)
+            d.textures, d.material_textures = list(self.textures), np.asarray(self.material_textures, np.int32)
+        return d
 
 
 def material(diffuse, roughness, emission=(0.0, 0.0, 0.0), specular=(0.0, 0.0, 0.0), refraction=1.0,

@@ -347,6 +347,13 @@ int krt_armada_read_ldr(krt_armada *a, int tonemapping, uint32_t *out);
  * krt_hdr_to_ldr.  Video capture (include/capture.hpp, ffmpeg) is not
  * provided. */
 int krt_write_png(const char *path, const uint32_t *rgba, uint32_t width, uint32_t height);
+/* Image input for material textures (the loader behind
+ * TextureLoader::load_texture, source/image.cpp:96-118, which takes stbi_load
+ * with 4 channels): a PNG file of any colour type and bit depth (16-bit
+ * samples keep their high byte), Adam7 interlacing and tRNS transparency, as
+ * packed RGBA8 (r in the low byte), rows in file order.  With rgba == NULL
+ * only width and height are read; capacity counts uint32 texels. */
+int krt_read_png(const char *path, uint32_t *width, uint32_t *height, uint32_t *rgba, size_t capacity);
 int krt_armada_write_png(krt_armada *a, int tonemapping, const char *path);
 
 /* Plugin entry of source/amadeus/naive_path_tracer.cu:280-344

@@ -311,6 +311,22 @@ def case_textures(ctx):
     assert bits_equal(img, ref[0]), "served request"
     say("textures: served request on a lane bit-exact")
     sc.free()
+    # the PNG decoder: a written file reads back; a truncated one is refused
+    import tempfile
+    from artemis_amd import read_png, write_png
+
+    with tempfile.TemporaryDirectory() as tmp:
+        px = np.random.default_rng(1).integers(0, 2 ** 32, (9, 7), dtype=np.uint64).astype(np.uint32)
+        write_png(f"{tmp}/a.png", px)
+        assert np.array_equal(read_png(f"{tmp}/a.png").view(np.uint32)[..., 0], px), "png round trip"
+        data = open(f"{tmp}/a.png", "rb").read()
+        open(f"{tmp}/b.png", "wb").write(data[:len(data) - 20])
+        try:
+            read_png(f"{tmp}/b.png")
+            raise AssertionError("truncated PNG decoded")
+        except KrtError:
+            pass
+    say("textures: PNG decoder round trip and truncated file")
 
 
 def case_refit(ctx):

@@ -179,3 +179,98 @@ def test_loaded_project_renders_like_oracle(tmp_path, krt_ctx):
     print(f"  exact {ex:.4f}", flush=True)
     assert ex >= 0.99
     sc.free()
+
+
+def _textured_fixture(tmp_path):
+    """textured_room as a one-entity project (identity transform), one
+    submesh per material, its albedo / normal / roughness textures as PNG
+    files.  Returns the scene as the loader's flattening must give it back
+    (triangles grouped by material)."""
+    d = scenes.textured_room()
+    mt = d.material_textures.copy()
+    mt[:, 2] = mt[:, 3] = -1  # the .mat format has no specular / emission texture
+    meshes, uvs, tans, tri_mat = [], [], [], []
+    for m in range(len(d.materials)):
+        tris = d.indices[d.tri_material == m]
+        used, local = np.unique(tris, return_inverse=True)
+        meshes.append((d.positions[used], d.normals[used], local.reshape(-1, 3).astype(np.uint32)))
+        uvs.append(d.uvs[used])
+        tans.append(d.tangents[used])
+        tri_mat.append(np.full(len(tris), m, np.uint32))
+    n = len(meshes)
+    inst = scenes.InstancedSceneData(meshes, np.arange(n, dtype=np.int32),
+                                     np.repeat(np.eye(3, 4, dtype=np.float32)[None], n, 0),
+                                     np.concatenate(tri_mat), d.materials, camera=d.camera, name="room")
+    inst.mesh_uvs, inst.mesh_tangents, inst.textures, inst.material_textures = uvs, tans, d.textures, mt
+    P.save_project(tmp_path, {"room": (inst, [("room", (0, 0, 0), (0, 0, 0), (1, 1, 1), list(range(n)))], None)})
+    return inst.flatten(), mt
+
+
+def test_textured_project_round_trip(tmp_path):
+    d, mt = _textured_fixture(tmp_path)
+    [ks] = P.load_project(tmp_path)
+    flat = ks.instanced().flatten()
+    assert np.array_equal(flat.positions, d.positions) and np.array_equal(flat.uvs, d.uvs)
+    assert np.array_equal(flat.tangents, d.tangents)  # identity transform, applied twice
+    # the slots refer to the loader's own texture order: compare the images slot by slot
+    for m in range(len(d.materials)):
+        for j in (0, 1, 4):
+            a, b = mt[m, j], flat.material_textures[m, j]
+            assert (a < 0) == (b < 0)
+            if a >= 0:
+                assert np.array_equal(flat.textures[b], d.textures[a]), (m, j)
+    assert np.all(flat.material_textures[:, 2:4] == -1)
+    # the .mat files name PNG files whose rows run opposite to texture memory
+    img = P.read_material(tmp_path / "assets" / "room-material-0.mat", textures=True)[2]
+    assert img[0].endswith(".png") and img[1] == ""
+    from artemis_amd import read_png
+    first = read_png(tmp_path / "assets" / img[0])
+    assert np.array_equal(first[::-1], d.textures[mt[0, 0]])
+
+
+def test_textured_project_rotated_entity_tangents(tmp_path):
+    """An entity's tangent frame goes through its model matrix twice, as
+    calculate_normal applies it (amadeus_common.cuh:125-136)."""
+    d = scenes.textured_room()
+    inst = scenes.InstancedSceneData([(d.positions, d.normals, d.indices)], np.array([0], np.int32),
+                                     P.transform_matrix((0.5, 0, 0), (0, 30, 0), (1, 1, 1))[None],
+                                     d.tri_material, d.materials, camera=d.camera)
+    inst.mesh_uvs, inst.mesh_tangents, inst.textures = [d.uvs], [d.tangents], d.textures
+    inst.material_textures = d.material_textures
+    flat = inst.flatten()
+    R = Rotation.from_euler("xyz", (0, 60, 0), degrees=True).as_matrix()  # twice 30 degrees
+    assert np.allclose(flat.tangents[:, 0:3], d.tangents[:, 0:3] @ R.T, atol=1e-5)
+    assert np.allclose(flat.tangents[:, 3:6], d.tangents[:, 3:6] @ R.T, atol=1e-5)
+
+
+def test_missing_or_unsupported_textures_warn(tmp_path):
+    assert P.resolve_texture("", tmp_path) is None
+    with pytest.warns(UserWarning):
+        assert P.resolve_texture("nope.png", tmp_path) is None
+    (tmp_path / "a.jpg").write_bytes(b"\xff\xd8")
+    with pytest.warns(UserWarning):
+        assert P.resolve_texture("a.jpg", tmp_path) is None
+
+
+@pytest.mark.gpu
+def test_textured_project_renders_like_oracle(tmp_path, krt_ctx):
+    import oracle as O
+    from artemis_amd import ArmadaRTX, PathTracer, look_at
+
+    _textured_fixture(tmp_path)
+    [ks] = P.load_project(tmp_path)
+    flat = ks.instanced().flatten()
+    sc = krt_ctx.upload(flat)
+    osc = O.OracleScene(flat, sc.serialized())
+    W, H = 40, 32
+    cam = look_at(**scenes.textured_room().camera, aspect=W / H)  # the project has no camera entity
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("path_tracer", PathTracer(krt_ctx))
+    arm.set_depth(3)
+    print(f"[project] textured {flat.triangles} triangles {W}x{H}", flush=True)
+    for _ in range(2):
+        arm.render(sc, cam, accumulate=True)
+    ref, _ = osc.render_pt(cam, W, H, frames=2, max_depth=3)
+    for k in range(4):
+        assert np.array_equal(arm._read(k).view(np.uint32), ref[k].view(np.uint32)), k
+    sc.free()
