@@ -91,10 +91,11 @@ struct GPlanes {
     KRT_DEV int M(uint32_t i) const { return __float_as_int(part(0, i).w); }
 };
 
-// G-buffer: {x, depth}, {n, valid}, material id
+// G-buffer: {x, depth}, {n, valid}, material id, texture coordinates
 struct GBuf {
     float4 *xd, *nv;
     uint32_t *mat;
+    float2 *uv;
 };
 
 struct GPix {
@@ -162,6 +163,11 @@ KRT_DEV void gi_shade(const GiArgs &A, uint32_t pix, const GPix &gb, const GRes 
         f3 wi = normalize(R.s.xs - gb.x);
         SurfaceHit sh;
         sh.mat = load_material(A.sv.mats, gb.mat);
+        const uint4 tx = material_tex(A.sv.mats, gb.mat);
+        if (tx.w) {  // the primary surface's textured material (the oracle's G-buffer holds it resolved)
+            const float2 uv = A.g.uv[pix];
+            apply_textures(A.sv.tex, tx, uv.x, uv.y, sh.mat);
+        }
         sh.entering = true;
         sh.n = gb.n;
         sh.wo = normalize(ld3(li.center) - gb.x);
@@ -284,6 +290,7 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_shade_kernel(LaunchInfo li, S
                 hl[npix + pix] = make_float4(0.0f, 0.0f, 0.0f, ld ? 1.0f : 0.0f);
                 B.v0[pix] = make_float4(x.x, x.y, x.z, pdf);
                 B.v0[npix + pix] = make_float4(n.x, n.y, n.z, __uint_as_float(sf.mat_id));
+                if (sv.uv) B.v0[4 * (size_t)npix + pix] = make_float4(sf.u, sf.v, 0.0f, 0.0f);
                 B.v0[2 * (size_t)npix + pix] = make_float4(wi.x, wi.y, wi.z, __int_as_float(1));
                 B.state[pix] = make_float4(seed.x, seed.y, seed.z, __int_as_float(0));
                 if (pdf > 0.0f && li.max_depth >= 1) {
@@ -351,6 +358,7 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_compose_kernel(GiArgs A, GiWa
     R.W = 0.0f;
     float4 gxd = make_float4(0, 0, 0, 0), gnv = make_float4(0, 0, 0, 0);
     uint32_t gm = 0;
+    float2 guv = make_float2(0.0f, 0.0f);
     if (hit0) {
         const float4 a = B.v0[pix], b = B.v0[npix + pix];
         const f3 xv = mk(a.x, a.y, a.z), nv = mk(b.x, b.y, b.z), wi0 = mk(w0.x, w0.y, w0.z);
@@ -383,10 +391,15 @@ __global__ __launch_bounds__(kGiWaveBlock) void gi_compose_kernel(GiArgs A, GiWa
         gxd = make_float4(xv.x, xv.y, xv.z, length(xv - ld3(li.center)));
         gnv = make_float4(nv.x, nv.y, nv.z, 1.0f);
         gm = __float_as_uint(b.w);
+        if (A.sv.uv) {
+            const float4 q = B.v0[4 * (size_t)npix + pix];
+            guv = make_float2(q.x, q.y);
+        }
     }
     A.g.xd[pix] = gxd;
     A.g.nv[pix] = gnv;
     A.g.mat[pix] = gm;
+    if (A.sv.uv) A.g.uv[pix] = guv;
     A.direct[pix] = make_float4(direct.x, direct.y, direct.z, 0.0f);
     A.aux[pix] = make_float4(st.x, st.y, st.z, 0.0f);
     A.cur.store(pix, R);
@@ -695,6 +708,8 @@ void gi_alloc(Context &ctx, GiState &st, uint32_t w, uint32_t h, uint32_t user_s
         st.gxd[b].alloc(sizeof(float4) * n);
         st.gnv[b].alloc(sizeof(float4) * n);
         st.gmat[b].alloc(sizeof(uint32_t) * n);
+        st.guv[b].alloc(sizeof(float2) * n);
+        KRT_HIP(hipMemsetAsync(st.guv[b].p, 0, st.guv[b].bytes, ctx.stream));
         KRT_HIP(hipMemsetAsync(st.gxd[b].p, 0, st.gxd[b].bytes, ctx.stream));
         KRT_HIP(hipMemsetAsync(st.gnv[b].p, 0, st.gnv[b].bytes, ctx.stream));
         KRT_HIP(hipMemsetAsync(st.gmat[b].p, 0, st.gmat[b].bytes, ctx.stream));
@@ -724,7 +739,6 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
                unsigned long long *ctr, PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "restir_gi renders flat scenes (krt_scene_upload)");
-    KRT_REQUIRE(!s.textured, KRT_ERR_STATE, "restir_gi renders untextured scenes");
     KRT_REQUIRE(!li_in.env, KRT_ERR_STATE,
                 "restir_gi has no environment lighting (its samples are surface points); disable the envmap");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "ReSTIR GI buffers not loaded");
@@ -738,7 +752,9 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
     uint32_t nb = persistent_blocks(npix, kTraceBlock, kMaxResidentBlocks);
     uint32_t nthreads = nb * kTraceBlock;
     auto planes = [&](int b) { return GPlanes{st.res[b].as<float4>(), npix}; };
-    auto gbuf = [&](int b) { return GBuf{st.gxd[b].as<float4>(), st.gnv[b].as<float4>(), st.gmat[b].as<uint32_t>()}; };
+    auto gbuf = [&](int b) {
+        return GBuf{st.gxd[b].as<float4>(), st.gnv[b].as<float4>(), st.gmat[b].as<uint32_t>(), st.guv[b].as<float2>()};
+    };
     GiArgs A;
     A.li = li;
     A.sv = make_view(s);
@@ -774,7 +790,7 @@ void gi_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const RestirOpti
         grow(ws.state, sizeof(float4) * npix);
         grow(ws.hist, sizeof(float4) * 3 * levels * npix);
         grow(ws.occ, levels * npix);
-        grow(ws.v0, sizeof(float4) * 4 * npix);
+        grow(ws.v0, sizeof(float4) * (s.textured ? 5 : 4) * npix);
         for (int l = 0; l < 2; l++) {
             grow(ws.rays[l], sizeof(float4) * 2 * cap);
             grow(ws.counts[l], sizeof(uint32_t) * gb);

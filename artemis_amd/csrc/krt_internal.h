@@ -278,6 +278,7 @@ struct MambaState {
     DevBuf R, S, P[2];  // 64 B reservoir records: current, spatial output, previous (double-buffered)
     int pin = 0;
     DevBuf gx, gn, gpm;  // ray-cast G-buffer
+    DevBuf guv;          // its texture coordinates (float2 per pixel)
     float prev_pv[16] = {0};
     float prev_cam[12] = {0};
     bool have_prev = false;
@@ -288,6 +289,7 @@ struct GiState {
     DevBuf res[2];  // 5 float4 planes per buffer
     int cur = 0;
     DevBuf gxd[2], gnv[2], gmat[2];  // G-buffer of this and the last frame
+    DevBuf guv[2];                   // its texture coordinates (float2 per pixel; textured scenes)
     int gcur = 0;
     DevBuf direct, aux;
     // spatial pass: segmented shadow-ray list, segment counts and offsets,

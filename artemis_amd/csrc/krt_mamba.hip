@@ -112,6 +112,7 @@ struct MambaArgs {
     uint32_t nml;
     float4 *gx, *gn;  // G-buffer: {position, entering sign}, {normal, 0}
     int2 *gpm;        // {triangle, material}; material -1: no surface
+    float2 *guv;      // texture coordinates (u, 1 - v)
     MBuf R, S, Pin, Pout;
     float time;
     int dirty, end_initial, end_temporal;
@@ -181,6 +182,13 @@ KRT_DEV bool mamba_surface(const MambaArgs &A, uint32_t i, SurfaceHit &sh) {
     sh.n = normalize(n);
     sh.entering = gx.w > 0.0f;
     sh.mat = load_material(A.sv.mats, (uint32_t)pm.y);
+    // convert_material (editor/optix/common.cuh:34-47): the diffuse texture only
+    const uint32_t dtex = material_tex(A.sv.mats, (uint32_t)pm.y).x & 0xFFFFu;
+    if (A.sv.uv && dtex != kNoTex) {
+        const float2 uv = A.guv[i];
+        const float4 c = tex_fetch(A.sv.tex, dtex, uv.x, uv.y);
+        sh.mat.diffuse = mk(c.x, c.y, c.z);
+    }
     const float sign = (sh.mat.type == eTransmission) ? -1.0f : 1.0f;
     sh.x = sh.x + (sign * sh.n) * kMambaEps;
     return true;
@@ -252,14 +260,17 @@ __global__ __launch_bounds__(kMambaBlock) void mamba_gbuffer_kernel(MambaArgs A)
     Surface sf;
     float4 gx = make_float4(0.0f, 0.0f, 0.0f, 0.0f), gn = gx;
     int2 pm = make_int2(-1, -1);
-    if (h.prim >= 0 && load_surface<false>(A.sv, h, d, sf)) {
+    float2 uv = make_float2(0.0f, 0.0f);
+    if (h.prim >= 0 && load_surface<false, false>(A.sv, h, d, sf)) {
         gx = make_float4(sf.x.x, sf.x.y, sf.x.z, sf.entering ? 1.0f : -1.0f);
         gn = make_float4(sf.n.x, sf.n.y, sf.n.z, 0.0f);
         pm = make_int2(h.prim, (int)sf.mat_id);
+        uv = make_float2(sf.u, sf.v);
     }
     A.gx[index] = gx;
     A.gn[index] = gn;
     A.gpm[index] = pm;
+    if (A.sv.uv) A.guv[index] = uv;
     const float4 c = A.li.color[index];
     A.li.color[index] = make_float4(c.x, c.y, c.z, 1.0f);
 }
@@ -429,6 +440,7 @@ void mamba_alloc(Context &ctx, MambaState &st, uint32_t w, uint32_t h, uint32_t 
     st.gx.alloc(sizeof(float4) * n);
     st.gn.alloc(sizeof(float4) * n);
     st.gpm.alloc(sizeof(int2) * n);
+    st.guv.alloc(sizeof(float2) * n);
     mamba_seed_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx.stream>>>(
         MBuf{st.R.as<float4>()}, MBuf{st.S.as<float4>()}, MBuf{st.P[0].as<float4>()}, MBuf{st.P[1].as<float4>()}, w,
         h, user_seed);
@@ -439,7 +451,6 @@ void mamba_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const MambaOp
                   PassTimes &times) {
     KRT_REQUIRE(s.accel.built, KRT_ERR_STATE, "scene has no acceleration structure");
     KRT_REQUIRE(!s.instanced, KRT_ERR_STATE, "mamba renders flat scenes (krt_scene_upload)");
-    KRT_REQUIRE(!s.textured, KRT_ERR_STATE, "mamba renders untextured scenes");
     KRT_REQUIRE(st.w == li_in.width && st.h == li_in.height, KRT_ERR_STATE, "mamba buffers not loaded");
     MambaArgs A;
     A.li = li_in;
@@ -462,6 +473,7 @@ void mamba_render(Context &ctx, Scene &s, const LaunchInfo &li_in, const MambaOp
     A.gx = st.gx.as<float4>();
     A.gn = st.gn.as<float4>();
     A.gpm = st.gpm.as<int2>();
+    A.guv = st.guv.as<float2>();
     A.R = MBuf{st.R.as<float4>()};
     A.S = MBuf{st.S.as<float4>()};
     A.Pin = MBuf{st.P[st.pin].as<float4>()};

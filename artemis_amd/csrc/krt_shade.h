@@ -118,11 +118,13 @@ KRT_DEV bool load_surface_inst(const SceneView &sv, const Hit &h, f3 dir, Surfac
 // tangent and bitangent (each normalised twice, as the reference does; the
 // model matrix is the identity for world-space triangles) and the shading
 // normal.
-KRT_DEV void surface_uv(const SceneView &sv, uint32_t i0, uint32_t i1, uint32_t i2, const Hit &h, Surface &sf) {
+KRT_DEV void surface_uv(const SceneView &sv, uint32_t i0, uint32_t i1, uint32_t i2, const Hit &h, Surface &sf,
+                        bool normal_map) {
     const float2 ta = sv.uv[i0], tb = sv.uv[i1], tc = sv.uv[i2];
     const float w = 1.0f - h.b1 - h.b2;
     sf.u = (w * ta.x + h.b1 * tb.x) + h.b2 * tc.x;
     sf.v = 1.0f - ((w * ta.y + h.b1 * tb.y) + h.b2 * tc.y);
+    if (!normal_map) return;
     const uint32_t nmap = material_tex(sv.mats, sf.mat_id).x >> 16;
     if (nmap == kNoTex || !sv.tan) return;
     const float4 t4 = tex_fetch(sv.tex, nmap, sf.u, sf.v);
@@ -135,7 +137,9 @@ KRT_DEV void surface_uv(const SceneView &sv, uint32_t i0, uint32_t i1, uint32_t 
 }
 
 // Returns false when the shading normal is NaN (the reference then returns 0).
-template <bool INST = false>
+// NMAP = false: the editor's ray-cast G-buffer (editor/gbuffer_rtx_shader.cu:
+// 95-160), which records (u, 1 - v) but applies no normal map.
+template <bool INST = false, bool NMAP = true>
 KRT_DEV bool load_surface(const SceneView &sv, const Hit &h, f3 dir, Surface &sf) {
     if (INST) return load_surface_inst(sv, h, dir, sf);
     const uint32_t i0 = sv.idx[3 * h.prim], i1 = sv.idx[3 * h.prim + 1], i2 = sv.idx[3 * h.prim + 2];
@@ -154,7 +158,7 @@ KRT_DEV bool load_surface(const SceneView &sv, const Hit &h, f3 dir, Surface &sf
     sf.x = interp(a, b, c, h.b1, h.b2);
     sf.mat_id = sv.tri_mat[h.prim];
     sf.u = sf.v = 0.0f;
-    if (sv.uv) surface_uv(sv, i0, i1, i2, h, sf);
+    if (sv.uv) surface_uv(sv, i0, i1, i2, h, sf, NMAP);
     return !isnan3(sf.n);
 }
 

@@ -81,8 +81,9 @@ int krt_scene_upload(krt_context *ctx, const krt_scene_desc *desc, krt_handle *s
  * texel's rgb, the roughness texture by its r; a normal map tilts the shading
  * normal into the tangent frame.  Triangles of a material with an emission
  * texture are triangle lights too, with the material's constant emission
- * (armada.cu:117-150, :264-265).  ReSTIR GI and the editor's ReSTIR refuse
- * textured scenes (KRT_ERR_STATE). */
+ * (armada.cu:117-150, :264-265).  The editor's ReSTIR ("mamba") reads the
+ * diffuse texture only and no normal map (editor/optix/common.cuh:34-47,
+ * editor/gbuffer_rtx_shader.cu:95-160). */
 typedef struct {
     const float *uvs;       /* nv x 2: Vertex::tex_coords */
     const float *tangents;  /* nv x 6: Vertex::tangent, Vertex::bitangent; may be NULL without normal maps */

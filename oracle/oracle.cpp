@@ -1022,6 +1022,7 @@ struct Surface {
     f3 x, n;
     bool entering;
     Material mat;
+    float u = 0.0f, v = 0.0f;  // texture coordinates (textured scenes)
 };
 
 inline f3 interp(f3 a, f3 b, f3 c, float b1, float b2) {
@@ -1055,7 +1056,9 @@ bool load_surface_inst(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
     return !isnan3(sf.n);
 }
 
-bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
+// normal_map = false: the editor's ray-cast G-buffer (gbuffer_rtx_shader.cu:
+// 95-160), which records (u, 1 - v) but neither textures nor normal maps.
+bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf, bool normal_map = true) {
     if (s->instanced) return load_surface_inst(s, h, dir, sf);
     f3 a, b, c;
     tri_verts(s, h.prim, a, b, c);
@@ -1081,6 +1084,9 @@ bool load_surface(const orc_scene *s, const Hit &h, f3 dir, Surface &sf) {
         const float w = 1.0f - h.b1 - h.b2;
         const float u = (w * ta[0] + h.b1 * tb[0]) + h.b2 * tc[0];
         const float v = 1.0f - ((w * ta[1] + h.b1 * tb[1]) + h.b2 * tc[1]);
+        sf.u = u;
+        sf.v = v;
+        if (!normal_map) return !isnan3(sf.n);
         const int32_t *mt = &s->mat_tex[5 * (size_t)m];
         float t4[4];
         // calculate_material (:154-187)
@@ -2206,10 +2212,11 @@ struct MRes {  // Reservoir<LightInfo> (include/amadeus/reservoir.cuh:19-76)
     }
 };
 
-struct MGb {  // G-buffer texel: position, normal (w: entering), triangle, material (-1: no surface)
+struct MGb {  // G-buffer texel: position, normal (w: entering), triangle, material (-1: no surface), uv
     f3 x{0, 0, 0}, n{0, 0, 0};
     float nw = 0.0f;
     int prim = -1, mat = -1;
+    float u = 0.0f, v = 0.0f;
 };
 
 const float kMambaEps = 1e-3f;   // mamba_shader.cu:38
@@ -2302,6 +2309,12 @@ bool mamba_surface(const orc_mamba *m, const orc_scene *s, const orc_camera *cam
     sh.n = normalize(n);
     sh.entering = t.nw > 0.0f;
     sh.mat = s->mats[t.mat];
+    // convert_material (editor/optix/common.cuh:34-47): the diffuse texture only
+    if (s->textured && s->mat_tex[5 * (size_t)t.mat] >= 0) {
+        float t4[4];
+        tex_fetch(s->texture(s->mat_tex[5 * (size_t)t.mat]), t.u, t.v, t4);
+        sh.mat.diffuse = mk(t4[0], t4[1], t4[2]);
+    }
     const float sign = (sh.mat.type == eTransmission) ? -1.0f : 1.0f;
     sh.x = sh.x + (sign * sh.n) * kMambaEps;
     return true;
@@ -2426,12 +2439,14 @@ void orc_mamba_frame(orc_mamba *m, const orc_scene *s, const orc_camera *cam, in
             MGb &t = m->g[(size_t)y * W + x];
             t = MGb();
             Surface sf;
-            if (h.prim >= 0 && load_surface(s, h, d, sf)) {
+            if (h.prim >= 0 && load_surface(s, h, d, sf, false)) {
                 t.x = sf.x;
                 t.n = sf.n;
                 t.nw = sf.entering ? 1.0f : -1.0f;
                 t.prim = h.prim;
                 t.mat = (int)s->tri_mat[h.prim];
+                t.u = sf.u;
+                t.v = sf.v;
             }
         }
         c.add(cx.c);

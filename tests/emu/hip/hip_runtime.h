@@ -103,6 +103,8 @@ inline void __syncthreads() { krt_emu::syncthreads(); }
 inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline int __syncthreads_count(int pred) { return krt_emu::syncthreads_count(pred); }
 inline unsigned long long __ballot(int pred) { return krt_emu::wave_ballot(pred); }
+inline int __any(int pred) { return krt_emu::wave_ballot(pred) != 0ull; }
+inline int __all(int pred) { return krt_emu::wave_ballot(!pred) == 0ull; }
 // the wave's LDS accesses before it are visible to its lanes after it
 inline void __builtin_amdgcn_wave_barrier() { (void)krt_emu::wave_ballot(0); }
 

@@ -9,14 +9,16 @@ maps on walls and a sphere, specular, roughness and emission textures):
   each pixel's texture coordinates for the temporal and spatial passes);
 * served requests, in the batched launch and on a lane;
 * the triangle lights (textured emission: the material's constant);
-* ReSTIR GI and the editor's ReSTIR refuse textured scenes; invalid texture
-  arrays are refused at upload."""
+* ReSTIR GI (its G-buffer keeps the primary hit's (u, v)) and the editor's
+  ReSTIR (the diffuse texture only, no normal map, as its G-buffer and
+  convert_material do), reservoirs and colour;
+* invalid texture arrays are refused at upload."""
 import numpy as np
 import pytest
 
 import oracle as O
 from artemis_amd import ArmadaRTX, Mamba, PathTracer, ReSTIR, ReSTIRGI, Server, look_at, scenes
-from artemis_amd._lib import KRT_ERR_INVALID, KRT_ERR_STATE, KrtError
+from artemis_amd._lib import KRT_ERR_INVALID, KrtError
 
 pytestmark = pytest.mark.gpu
 
@@ -105,14 +107,49 @@ def test_cached_path_tracer_runs(krt_ctx, room):
     assert np.all(np.isfinite(arm.color_buffer()))
 
 
-@pytest.mark.parametrize("kind", ["gi", "mamba"])
-def test_gi_and_editor_restir_refuse_textures(krt_ctx, room, kind):
-    s, sc, _ = room
-    arm = ArmadaRTX(krt_ctx, 16, 16)
-    arm.attach(kind, ReSTIRGI(krt_ctx) if kind == "gi" else Mamba(krt_ctx))
-    with pytest.raises(KrtError) as e:
-        arm.render(sc, look_at(**s.camera))
-    assert e.value.code == KRT_ERR_STATE
+def test_restir_gi_frames(krt_ctx, room):
+    """ReSTIR GI: the textured primary surface (its G-buffer keeps (u, v)),
+    textured and normal-mapped secondary vertices; two frames, reservoirs
+    and colour bit for bit."""
+    s, sc, osc = room
+    W, H = 40, 32
+    cam = look_at(**s.camera, aspect=W / H)
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("gi", ReSTIRGI(krt_ctx))
+    arm.set_option("spatial_passes", 2)
+    arm.set_option("radius", 8)
+    arm.set_depth(3)
+    ogi = O.OracleGI(W, H)
+    print(f"[textures] ReSTIR GI {W}x{H}, 2 frames", flush=True)
+    for f in range(2):
+        arm.render(sc, cam)
+        ogi.frame(osc, cam, max_depth=3, m_cap=30, spatial_passes=2, neighbours=5, radius=8)
+        for which in (0, 1):
+            ok = exact(arm.reservoirs(which, gi=True), ogi.reservoirs(which))
+            assert ok.all(), f"frame {f} reservoirs[{which}]: {ok.mean():.4f} bit-exact"
+        ok = exact(arm.color_buffer(), ogi.color)
+        assert ok.all(), f"frame {f} colour: {ok.mean():.4f} bit-exact"
+
+
+def test_editor_restir(krt_ctx, room):
+    """The editor's ReSTIR (mamba): its ray-cast G-buffer keeps (u, 1 - v)
+    and no normal map, convert_material reads the diffuse texture only
+    (editor/optix/common.cuh:34-47); three frames of a moving camera."""
+    s, sc, osc = room
+    W, H = 40, 32
+    arm = ArmadaRTX(krt_ctx, W, H)
+    arm.attach("mamba", Mamba(krt_ctx))
+    om = O.OracleMamba(osc, W, H)
+    print(f"[textures] editor ReSTIR {W}x{H}, 3 frames", flush=True)
+    for f in range(3):
+        cam = look_at((0.2 * f - 0.2, 1.0, 3.5), (0.0, 0.9, 0.0), aspect=W / H)
+        arm.render(sc, cam)
+        om.frame(cam, temporal=True, spatial=True)
+        for which in (0, 1):
+            ok = exact(arm.reservoirs(which), om.reservoirs(which))
+            assert ok.all(), f"frame {f} reservoirs[{which}]: {ok.mean():.4f} bit-exact"
+        ok = exact(arm.color_buffer(), om.color)
+        assert ok.all(), f"frame {f} colour: {ok.mean():.4f} bit-exact"
 
 
 def test_invalid_texture_arrays_are_refused(krt_ctx):

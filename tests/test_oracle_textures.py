@@ -210,3 +210,32 @@ def test_specular_and_roughness_textures_equal_constant_materials():
     assert close.mean() > 0.97
     assert ca.mean() == pytest.approx(cb.mean(), rel=5e-3)
     assert np.abs(ca - cc).max() > 1e-2  # the textures change the image
+
+
+def test_editor_restir_reads_only_the_diffuse_texture():
+    """The editor's ReSTIR takes its material through convert_material
+    (editor/optix/common.cuh:34-47: the diffuse texture only) and its surface
+    from the ray-cast G-buffer (gbuffer_rtx_shader.cu:95-160: no normal
+    map): removing every other texture leaves its frames bit for bit, while
+    removing the diffuse textures changes them."""
+    d = scenes.textured_room()
+    W, H = 32, 24
+    cams = [look_at((0.2 * f - 0.2, 1.0, 3.5), (0.0, 0.9, 0.0), aspect=W / H) for f in range(2)]
+
+    def frames(mt):
+        e = scenes.SceneData(d.positions, d.normals, d.indices, d.tri_material, d.materials, camera=d.camera)
+        e.uvs, e.tangents, e.textures, e.material_textures = d.uvs, d.tangents, d.textures, mt
+        om = O.OracleMamba(O.OracleScene(e), W, H)
+        for c in cams:
+            om.frame(c, temporal=True, spatial=True)
+        return om.color.copy(), om.reservoirs(0).copy()
+
+    full = frames(d.material_textures)
+    only = d.material_textures.copy()
+    only[:, 1:] = -1
+    none = np.full_like(d.material_textures, -1)
+    a, b, c = frames(d.material_textures), frames(only), frames(none)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert not np.array_equal(a[0], c[0])
+    assert np.array_equal(full[0].view(np.uint32), a[0].view(np.uint32))  # deterministic
