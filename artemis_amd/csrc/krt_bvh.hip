@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <vector>
 #include <cstring>
@@ -34,7 +35,6 @@ namespace krt {
 
 namespace {
 
-constexpr int BB = 256;  // build workgroup
 constexpr int NBIN = 11;
 
 struct Seg {
@@ -106,27 +106,42 @@ __global__ void root_small_kernel(BuildOut o, int n, const float4 *bmin, const f
     else emit_pair(o, 0, -1, 0, 0, 1, bmin, bmax);
 }
 
-// One workgroup per pending node (count >= 3).
-__global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *perm_in, int *perm_out,
+// Primitive boxes in the order of a permutation array (perm[i]'s box at i):
+// each level reads its segments' boxes as contiguous runs instead of
+// gathering them through the permutation, and writes them in the order of
+// its output permutation.
+struct PermBoxes {
+    float4 *mn, *mx;
+};
+
+// One workgroup of NT lanes per pending node (count >= 3).  The result does
+// not depend on NT or on the order of the reductions: candidates carry their
+// index (ties to the lower), the bins are order-free and the partition is
+// stable.
+template <int NT>
+__global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *perm_in, int *perm_out,
                                                      const float4 *bmin, const float4 *bmax, BuildOut o, Seg *next,
-                                                     int *next_count) {
+                                                     int *next_count, PermBoxes bin_, PermBoxes bout) {
     const Seg sg = segs[blockIdx.x];
     const int tid = threadIdx.x;
     const int n = sg.count;
-    const int chunk = (n + BB - 1) / BB;
-    const int j0 = min(tid * chunk, n), j1 = min(j0 + chunk, n);
 
-    __shared__ float s_ext[BB], s_lo[BB], s_hi[BB];
-    __shared__ int s_axis[BB], s_has[BB];
-    __shared__ float s_umn[3][BB], s_umx[3][BB];
+    __shared__ float s_ext[NT], s_lo[NT], s_hi[NT];
+    __shared__ int s_axis[NT], s_jb[NT], s_has[NT];
+    __shared__ float s_umn[3][NT], s_umx[3][NT];
+    __shared__ int s_jmn[3][NT], s_jmx[3][NT];
 
-    // ---- pass 1: largest single extent (ordered) and ordered union box
+    // ---- pass 1: largest single extent and union box.  The lanes stride the
+    // segment (coalesced reads); every candidate carries its index and ties
+    // go to the lower one, which is what the sequential scan of bvh.cpp
+    // gives (a later element replaces the current one only if strictly
+    // larger, or smaller for the box minimum) -- also between +0 and -0.
     float best = 0.0f, lo = FLT_MAX, hi = -FLT_MAX;
-    int axis = 0;
+    int axis = 0, jb = INT_MAX;
     float umn[3] = {0, 0, 0}, umx[3] = {0, 0, 0};
-    for (int j = j0; j < j1; j++) {
-        int prim = perm_in[sg.start + j];
-        float4 mn = bmin[prim], mx = bmax[prim];
+    int jmn[3] = {0, 0, 0}, jmx[3] = {0, 0, 0};
+    for (int j = tid; j < n; j += NT) {
+        float4 mn = bin_.mn[sg.start + j], mx = bin_.mx[sg.start + j];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             float e = fabsf(comp(mx, a) - comp(mn, a));
@@ -135,45 +150,71 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
                 lo = comp(mn, a);
                 hi = comp(mx, a);
                 axis = a;
+                jb = j;
             }
         }
-        if (j == j0) {
-            umn[0] = mn.x; umn[1] = mn.y; umn[2] = mn.z;
-            umx[0] = mx.x; umx[1] = mx.y; umx[2] = mx.z;
+        const float pm[3] = {mn.x, mn.y, mn.z}, px[3] = {mx.x, mx.y, mx.z};
+        if (j == tid) {
+            for (int a = 0; a < 3; a++) {
+                umn[a] = pm[a];
+                umx[a] = px[a];
+                jmn[a] = jmx[a] = j;
+            }
         } else {
-            umn[0] = gmin(umn[0], mn.x); umn[1] = gmin(umn[1], mn.y); umn[2] = gmin(umn[2], mn.z);
-            umx[0] = gmax(umx[0], mx.x); umx[1] = gmax(umx[1], mx.y); umx[2] = gmax(umx[2], mx.z);
+            for (int a = 0; a < 3; a++) {
+                if (pm[a] < umn[a]) {
+                    umn[a] = pm[a];
+                    jmn[a] = j;
+                }
+                if (px[a] > umx[a]) {
+                    umx[a] = px[a];
+                    jmx[a] = j;
+                }
+            }
         }
     }
     s_ext[tid] = best;
     s_lo[tid] = lo;
     s_hi[tid] = hi;
     s_axis[tid] = axis;
-    s_has[tid] = j1 > j0;
+    s_jb[tid] = jb;
+    s_has[tid] = tid < n;
     for (int a = 0; a < 3; a++) {
         s_umn[a][tid] = umn[a];
         s_umx[a][tid] = umx[a];
+        s_jmn[a][tid] = jmn[a];
+        s_jmx[a][tid] = jmx[a];
     }
     __syncthreads();
-    for (int s = 1; s < BB; s <<= 1) {
-        if ((tid % (2 * s)) == 0 && tid + s < BB) {
+    for (int s = 1; s < NT; s <<= 1) {
+        if ((tid % (2 * s)) == 0 && tid + s < NT) {
             int r = tid + s;
-            if (s_ext[r] > s_ext[tid]) {  // later wins only if strictly larger
+            if (s_ext[r] > s_ext[tid] || (s_ext[r] == s_ext[tid] && s_jb[r] < s_jb[tid])) {
                 s_ext[tid] = s_ext[r];
                 s_lo[tid] = s_lo[r];
                 s_hi[tid] = s_hi[r];
                 s_axis[tid] = s_axis[r];
+                s_jb[tid] = s_jb[r];
             }
             if (s_has[r]) {
                 if (s_has[tid]) {
                     for (int a = 0; a < 3; a++) {
-                        s_umn[a][tid] = gmin(s_umn[a][tid], s_umn[a][r]);
-                        s_umx[a][tid] = gmax(s_umx[a][tid], s_umx[a][r]);
+                        const float vn = s_umn[a][r], vx = s_umx[a][r];
+                        if (vn < s_umn[a][tid] || (vn == s_umn[a][tid] && s_jmn[a][r] < s_jmn[a][tid])) {
+                            s_umn[a][tid] = vn;
+                            s_jmn[a][tid] = s_jmn[a][r];
+                        }
+                        if (vx > s_umx[a][tid] || (vx == s_umx[a][tid] && s_jmx[a][r] < s_jmx[a][tid])) {
+                            s_umx[a][tid] = vx;
+                            s_jmx[a][tid] = s_jmx[a][r];
+                        }
                     }
                 } else {
                     for (int a = 0; a < 3; a++) {
                         s_umn[a][tid] = s_umn[a][r];
                         s_umx[a][tid] = s_umx[a][r];
+                        s_jmn[a][tid] = s_jmn[a][r];
+                        s_jmx[a][tid] = s_jmx[a][r];
                     }
                     s_has[tid] = 1;
                 }
@@ -201,9 +242,8 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
             bmx[b][a] = -FLT_MAX;
         }
     }
-    for (int j = j0; j < j1; j++) {
-        int prim = perm_in[sg.start + j];
-        float4 mn = bmin[prim], mx = bmax[prim];
+    for (int j = tid; j < n; j += NT) {  // order-free: counts and box unions per bin
+        float4 mn = bin_.mn[sg.start + j], mx = bin_.mx[sg.start + j];
         float c = (comp(mn, axis) + comp(mx, axis)) / 2.0f;
         int bin = 0;
 #pragma unroll
@@ -218,8 +258,8 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
         }
     }
     // wave reduction, then across the 4 waves through LDS
-    __shared__ int w_c[BB / 64][NBIN];
-    __shared__ float w_mn[BB / 64][NBIN][3], w_mx[BB / 64][NBIN][3];
+    __shared__ int w_c[NT / 64][NBIN];
+    __shared__ float w_mn[NT / 64][NBIN][3], w_mx[NT / 64][NBIN][3];
     const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
     for (int b = 0; b < NBIN; b++) {
@@ -250,7 +290,7 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
                 mn[b][a] = FLT_MAX;
                 mx[b][a] = -FLT_MAX;
             }
-            for (int w = 0; w < BB / 64; w++) {
+            for (int w = 0; w < NT / 64; w++) {
                 cnt[b] += w_c[w][b];
                 for (int a = 0; a < 3; a++) {
                     mn[b][a] = fminf(mn[b][a], w_mn[w][b][a]);
@@ -302,17 +342,20 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
     const int nl = s_nl;
 
     // ---- pass 3: stable partition into perm_out
-    __shared__ int s_wl[BB / 64];
+    __shared__ int s_wl[NT / 64];
     int lbase = 0, rbase = 0;
-    for (int t0 = 0; t0 < n; t0 += BB) {
+    for (int t0 = 0; t0 < n; t0 += NT) {
         int j = t0 + tid;
         bool valid = j < n;
         int prim = valid ? perm_in[sg.start + j] : 0;
+        float4 pmn = make_float4(0, 0, 0, 0), pmx = pmn;
         bool left = false;
         if (valid) {
+            pmn = bin_.mn[sg.start + j];
+            pmx = bin_.mx[sg.start + j];
             if (fallback) left = (j % 2) == 0;
             else {
-                float c = (comp(bmin[prim], axis) + comp(bmax[prim], axis)) / 2.0f;
+                float c = (comp(pmn, axis) + comp(pmx, axis)) / 2.0f;
                 left = c < msplit;
             }
         }
@@ -321,15 +364,17 @@ __global__ __launch_bounds__(BB) void level_kernel(const Seg *segs, const int *p
         if (lane == 0) s_wl[wv] = __popcll(ball);
         __syncthreads();
         int woff = 0, tot = 0;
-        for (int w = 0; w < BB / 64; w++) {
+        for (int w = 0; w < NT / 64; w++) {
             if (w < wv) woff += s_wl[w];
             tot += s_wl[w];
         }
-        int tile_valid = min(BB, n - t0);
+        int tile_valid = min(NT, n - t0);
         if (valid) {
             int lrank = woff + lpre;  // left items before me in the tile
-            if (left) perm_out[sg.start + lbase + lrank] = prim;
-            else perm_out[sg.start + nl + rbase + (tid - lrank)] = prim;
+            const int at = sg.start + (left ? lbase + lrank : nl + rbase + (tid - lrank));
+            perm_out[at] = prim;
+            bout.mn[at] = pmn;
+            bout.mx[at] = pmx;
         }
         lbase += tot;
         rbase += tile_valid - tot;
@@ -495,8 +540,9 @@ __device__ __forceinline__ float sa_of(const float *mn, const float *mx) {
     return 2.0f * (dx * dy + dy * dz + dx * dz);
 }
 
-__global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const int *perm_in, int *perm_out,
-                                                         int *perm_final, const float4 *bmin, const float4 *bmax,
+template <int NT>
+__global__ __launch_bounds__(NT) void sah_level_kernel(const SahSeg *segs, const int *perm_in, int *perm_out,
+                                                         int *perm_final, PermBoxes bin_, PermBoxes bout,
                                                          float4 *nodes, int *node_ctr, SahSeg *next,
                                                          int *next_count, int leaf_max) {
     const SahSeg sg = segs[blockIdx.x];
@@ -509,7 +555,7 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
     __shared__ int s_nl, s_axis_split, s_median;
     __shared__ float s_lo, s_scale;
     __shared__ int s_axis;
-    __shared__ int s_wl[BB / 64];
+    __shared__ int s_wl[NT / 64];
     if (tid == 0) s_cb.clear();
     if (tid < SB) {
         s_bin[tid].clear();
@@ -520,9 +566,8 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
     // centroid bounds
     {
         float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-        for (int j = tid; j < n; j += BB) {
-            int prim = perm_in[sg.start + j];
-            float4 a = bmin[prim], b = bmax[prim];
+        for (int j = tid; j < n; j += NT) {
+            float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
             float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
             for (int k = 0; k < 3; k++) {
                 lo[k] = fminf(lo[k], c[k]);
@@ -544,9 +589,8 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
     const int axis = s_axis;
     const float clo = s_lo, scale = s_scale;
     // bins
-    for (int j = tid; j < n; j += BB) {
-        int prim = perm_in[sg.start + j];
-        float4 a = bmin[prim], b = bmax[prim];
+    for (int j = tid; j < n; j += NT) {
+        float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
         float c = 0.5f * (comp(a, axis) + comp(b, axis));
         int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
         atomicAdd(&s_cnt[bin], 1);
@@ -603,15 +647,18 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
     const int lane = tid & 63, wv = tid >> 6;
     // stable partition (median split: by position)
     int lbase = 0, rbase = 0;
-    for (int t0 = 0; t0 < n; t0 += BB) {
+    for (int t0 = 0; t0 < n; t0 += NT) {
         int j = t0 + tid;
         bool valid = j < n;
         int prim = valid ? perm_in[sg.start + j] : 0;
+        float4 pmn = make_float4(0, 0, 0, 0), pmx = pmn;
         bool left = false;
         if (valid) {
+            pmn = bin_.mn[sg.start + j];
+            pmx = bin_.mx[sg.start + j];
             if (median) left = j < nl;
             else {
-                float c = 0.5f * (comp(bmin[prim], axis) + comp(bmax[prim], axis));
+                float c = 0.5f * (comp(pmn, axis) + comp(pmx, axis));
                 int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
                 left = bin <= bsplit;
             }
@@ -621,17 +668,19 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
         if (lane == 0) s_wl[wv] = __popcll(ball);
         __syncthreads();
         int woff = 0, tot = 0;
-        for (int w = 0; w < BB / 64; w++) {
+        for (int w = 0; w < NT / 64; w++) {
             if (w < wv) woff += s_wl[w];
             tot += s_wl[w];
         }
         if (valid) {
             int lrank = woff + lpre;
-            if (left) perm_out[sg.start + lbase + lrank] = prim;
-            else perm_out[sg.start + nl + rbase + (tid - lrank)] = prim;
+            const int at = sg.start + (left ? lbase + lrank : nl + rbase + (tid - lrank));
+            perm_out[at] = prim;
+            bout.mn[at] = pmn;
+            bout.mx[at] = pmx;
         }
         lbase += tot;
-        rbase += min(BB, n - t0) - tot;
+        rbase += min(NT, n - t0) - tot;
         __syncthreads();
     }
     // child boxes
@@ -643,10 +692,10 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
                 hi[c][k] = -3.4e38f;
             }
         bool any[2] = {false, false};
-        for (int j = tid; j < n; j += BB) {
-            int prim = perm_out[sg.start + j];
+        __syncthreads();  // the partition's box copies are read back below
+        for (int j = tid; j < n; j += NT) {
             int c = j < nl ? 0 : 1;
-            float4 a = bmin[prim], b = bmax[prim];
+            float4 a = bout.mn[sg.start + j], b = bout.mx[sg.start + j];
             lo[c][0] = fminf(lo[c][0], a.x); lo[c][1] = fminf(lo[c][1], a.y); lo[c][2] = fminf(lo[c][2], a.z);
             hi[c][0] = fmaxf(hi[c][0], b.x); hi[c][1] = fmaxf(hi[c][1], b.y); hi[c][2] = fmaxf(hi[c][2], b.z);
             any[c] = true;
@@ -660,7 +709,7 @@ __global__ __launch_bounds__(BB) void sah_level_kernel(const SahSeg *segs, const
     const int st[2] = {sg.start, sg.start + nl};
     for (int c = 0; c < 2; c++)
         if (cnt[c] <= leaf_max)
-            for (int j = tid; j < cnt[c]; j += BB) perm_final[st[c] + j] = perm_out[st[c] + j];
+            for (int j = tid; j < cnt[c]; j += NT) perm_final[st[c] + j] = perm_out[st[c] + j];
     if (tid == 0) {
         int link[2];
         for (int c = 0; c < 2; c++) {
@@ -829,21 +878,47 @@ static uint32_t collapse_wide(Context &ctx, Accel &A, int nbin) {
 // preorder rank of every primitive), the traversal tree of `layout` (A.nodes)
 // and its 4-wide collapse (A.wide).  order receives the primitive at every
 // position of the traversal tree's leaf ranges.
+// Workgroup size of a level: the top levels have few nodes with many
+// primitives each, where more lanes per node hide more of the memory latency.
+constexpr int kBuildWide = 1024;
+constexpr int kBuildNarrow = 256;
+
 static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, uint32_t n, int leaf_max, int layout,
                         Accel &A, DevBuf &order_out, DevBuf &rank_of) {
     hipStream_t st = ctx.stream;
+    // the binned-SAH tree builds on the second stream, level by level beside
+    // the bvh.cpp tree: both trees' top levels occupy few CUs
+    hipStream_t st2 = ctx.side ? ctx.side : ctx.stream;
     const uint32_t nser = 2 * n - 1;
     DevBuf perm[2] = {DevBuf(sizeof(int) * n), DevBuf(sizeof(int) * n)};
     DevBuf segs[2] = {DevBuf(sizeof(Seg) * (n / 3 + 2)), DevBuf(sizeof(Seg) * (n / 3 + 2))};
-    DevBuf cnt(sizeof(int));
+    DevBuf cnt(sizeof(int)), cnt2(sizeof(int));
+    // boxes in permutation order, double-buffered per tree (level 0 reads
+    // bmin / bmax: its permutation is the identity)
+    DevBuf pbox[2][2][2];  // [tree][buffer][min / max]
+    auto boxes = [&](int tree, uint32_t level, bool out) {
+        const uint32_t b = (level + (out ? 1u : 0u)) & 1u;
+        if (!out && level == 0) return PermBoxes{const_cast<float4 *>(bmin), const_cast<float4 *>(bmax)};
+        DevBuf *d = pbox[tree][b];
+        return PermBoxes{d[0].as<float4>(), d[1].as<float4>()};
+    };
     DevBuf leafcount(sizeof(int) * nser), rank(sizeof(int) * nser);
     A.ser.alloc(sizeof(float4) * 3 * nser);
     A.leaf_order.alloc(sizeof(int) * n);
+    order_out.alloc(sizeof(int) * n);
     BuildOut o{A.ser.as<float4>(), A.leaf_order.as<int>(), leafcount.as<int>(), rank.as<int>()};
     iota_kernel<<<blocks(n, 256), 256, 0, st>>>(perm[0].as<int>(), n);
     KRT_HIP(hipGetLastError());
+    hipEvent_t boxes_ready, sah_done;
+    KRT_HIP(hipEventCreate(&boxes_ready));
+    KRT_HIP(hipEventCreate(&sah_done));
+    KRT_HIP(hipEventRecord(boxes_ready, st));
+    KRT_HIP(hipStreamWaitEvent(st2, boxes_ready, 0));
 
-    uint32_t levels = 0;
+    const bool sah = layout != KRT_LAYOUT_REFERENCE;
+    DevBuf sperm[2], ssegs[2], nctr;
+    int nseg = 0, nseg2 = 0, nnodes = 0;
+    uint32_t levels = 0, lv = 0, depth = 0;
     if (n <= 2) {
         root_small_kernel<<<1, 64, 0, st>>>(o, (int)n, bmin, bmax);
         KRT_HIP(hipGetLastError());
@@ -851,27 +926,108 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
     } else {
         Seg root{0, (int)n, 0, -1, 0};
         KRT_HIP(hipMemcpyAsync(segs[0].p, &root, sizeof(Seg), hipMemcpyHostToDevice, st));
-        int nseg = 1;
-        while (nseg > 0) {
-            KRT_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
-            level_kernel<<<nseg, BB, 0, st>>>(segs[levels & 1].as<Seg>(), perm[levels & 1].as<int>(),
-                                              perm[(levels + 1) & 1].as<int>(), bmin, bmax, o,
-                                              segs[(levels + 1) & 1].as<Seg>(), cnt.as<int>());
+        nseg = 1;
+    }
+    if (sah) {
+        A.nodes.alloc(sizeof(float4) * 4 * (size_t)(n + 1));
+        if ((int)n <= leaf_max) {
+            sah_root_leaf_kernel<<<1, 64, 0, st2>>>(bmin, bmax, (int)n, order_out.as<int>(), A.nodes.as<float4>());
             KRT_HIP(hipGetLastError());
-            KRT_HIP(hipMemcpyAsync(&nseg, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            nnodes = 1;
+            depth = 1;
+        } else {
+            for (int k = 0; k < 2; k++) {
+                sperm[k].alloc(sizeof(int) * n);
+                ssegs[k].alloc(sizeof(SahSeg) * (n / 2 + 2));
+            }
+            nctr.alloc(sizeof(int));
+            iota_kernel<<<blocks(n, 256), 256, 0, st2>>>(sperm[0].as<int>(), n);
+            KRT_HIP(hipGetLastError());
+            SahSeg root{0, (int)n, 0};
+            int one = 1;
+            KRT_HIP(hipMemcpyAsync(ssegs[0].p, &root, sizeof(SahSeg), hipMemcpyHostToDevice, st2));
+            KRT_HIP(hipMemcpyAsync(nctr.p, &one, sizeof(int), hipMemcpyHostToDevice, st2));
+            nseg2 = 1;
+        }
+    }
+    // the next level's segment counts land in pinned memory, so the copies
+    // stay asynchronous and both streams' levels are in flight together
+    struct Pinned {
+        int *p = nullptr;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    } pinned;
+    KRT_HIP(hipHostMalloc((void **)&pinned.p, 2 * sizeof(int), hipHostMallocDefault));
+    int &next = pinned.p[0], &next2 = pinned.p[1];
+    next = next2 = 0;
+    for (int tree = 0; tree < 2; tree++)
+        if (tree == 0 ? nseg > 0 : nseg2 > 0)
+            for (auto &b : pbox[tree])
+                for (auto &d : b) d.alloc(sizeof(float4) * n);
+    while (nseg > 0 || nseg2 > 0) {
+        if (nseg > 0) {
+            KRT_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+            const Seg *sin = segs[levels & 1].as<Seg>();
+            Seg *sout = segs[(levels + 1) & 1].as<Seg>();
+            const int *pin = perm[levels & 1].as<int>();
+            int *pout = perm[(levels + 1) & 1].as<int>();
+            const PermBoxes bi = boxes(0, levels, false), bo = boxes(0, levels, true);
+            if (nseg < 1024)
+                level_kernel<kBuildWide><<<nseg, kBuildWide, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout, cnt.as<int>(),
+                                                                      bi, bo);
+            else
+                level_kernel<kBuildNarrow><<<nseg, kBuildNarrow, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout,
+                                                                          cnt.as<int>(), bi, bo);
+            KRT_HIP(hipGetLastError());
+            KRT_HIP(hipMemcpyAsync(&next, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        }
+        if (nseg2 > 0) {
+            KRT_HIP(hipMemsetAsync(cnt2.p, 0, sizeof(int), st2));
+            const SahSeg *sin = ssegs[lv & 1].as<SahSeg>();
+            SahSeg *sout = ssegs[(lv + 1) & 1].as<SahSeg>();
+            const int *pin = sperm[lv & 1].as<int>();
+            int *pout = sperm[(lv + 1) & 1].as<int>();
+            const PermBoxes bi = boxes(1, lv, false), bo = boxes(1, lv, true);
+            if (nseg2 < 1024)
+                sah_level_kernel<kBuildWide><<<nseg2, kBuildWide, 0, st2>>>(sin, pin, pout, order_out.as<int>(), bi, bo,
+                                                                           A.nodes.as<float4>(), nctr.as<int>(),
+                                                                           sout, cnt2.as<int>(), leaf_max);
+            else
+                sah_level_kernel<kBuildNarrow><<<nseg2, kBuildNarrow, 0, st2>>>(
+                    sin, pin, pout, order_out.as<int>(), bi, bo, A.nodes.as<float4>(), nctr.as<int>(), sout,
+                    cnt2.as<int>(), leaf_max);
+            KRT_HIP(hipGetLastError());
+            KRT_HIP(hipMemcpyAsync(&next2, cnt2.p, sizeof(int), hipMemcpyDeviceToHost, st2));
+        }
+        if (nseg > 0) {
             KRT_HIP(hipStreamSynchronize(st));
+            nseg = next;
             levels++;
             KRT_REQUIRE(levels < 1000000, KRT_ERR_HIP, "BVH build did not converge");
         }
+        if (nseg2 > 0) {
+            KRT_HIP(hipStreamSynchronize(st2));
+            nseg2 = next2;
+            lv++;
+            KRT_REQUIRE(lv < 100000, KRT_ERR_HIP, "SAH build did not converge");
+        }
     }
+    if (sah && (int)n > leaf_max) {
+        KRT_HIP(hipMemcpyAsync(&nnodes, nctr.p, sizeof(int), hipMemcpyDeviceToHost, st2));
+        KRT_HIP(hipStreamSynchronize(st2));
+        depth = lv + 1;
+    }
+    KRT_HIP(hipEventRecord(sah_done, st2));
+    KRT_HIP(hipStreamWaitEvent(st, sah_done, 0));
+    (void)hipEventDestroy(boxes_ready);
+    (void)hipEventDestroy(sah_done);
 
     // rank of each primitive in the bvh.cpp preorder (closest-hit tie-break)
     rank_of.alloc(sizeof(int) * n);
     invert_kernel<<<blocks(n, 256), 256, 0, st>>>(A.leaf_order.as<int>(), n, rank_of.as<int>());
-    int nnodes = 0;
-    uint32_t depth = levels + 2;
-    order_out.alloc(sizeof(int) * n);
-    if (layout == KRT_LAYOUT_REFERENCE) {
+    if (!sah) {
+        depth = levels + 2;
         // collapse of the bvh.cpp tree itself
         DevBuf flag(sizeof(int) * nser), node_id(sizeof(int) * nser), total(sizeof(int));
         mark_kernel<<<blocks(nser, 256), 256, 0, st>>>(A.ser.as<float4>(), leafcount.as<int>(), nser, leaf_max,
@@ -884,41 +1040,6 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
                                                              rank.as<int>(), flag.as<int>(), node_id.as<int>(), nser,
                                                              leaf_max, A.nodes.as<float4>());
         KRT_HIP(hipMemcpyAsync(order_out.p, A.leaf_order.p, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    } else {
-        // binned-SAH traversal tree over the same primitives
-        A.nodes.alloc(sizeof(float4) * 4 * (size_t)(n + 1));
-        DevBuf sperm[2] = {DevBuf(sizeof(int) * n), DevBuf(sizeof(int) * n)};
-        DevBuf ssegs[2] = {DevBuf(sizeof(SahSeg) * (n / 2 + 2)), DevBuf(sizeof(SahSeg) * (n / 2 + 2))};
-        DevBuf nctr(sizeof(int));
-        if ((int)n <= leaf_max) {
-            sah_root_leaf_kernel<<<1, 64, 0, st>>>(bmin, bmax, (int)n, order_out.as<int>(), A.nodes.as<float4>());
-            KRT_HIP(hipGetLastError());
-            nnodes = 1;
-            depth = 1;
-        } else {
-            iota_kernel<<<blocks(n, 256), 256, 0, st>>>(sperm[0].as<int>(), n);
-            SahSeg root{0, (int)n, 0};
-            int one = 1;
-            KRT_HIP(hipMemcpyAsync(ssegs[0].p, &root, sizeof(SahSeg), hipMemcpyHostToDevice, st));
-            KRT_HIP(hipMemcpyAsync(nctr.p, &one, sizeof(int), hipMemcpyHostToDevice, st));
-            int nseg = 1;
-            uint32_t lv = 0;
-            while (nseg > 0) {
-                KRT_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
-                sah_level_kernel<<<nseg, BB, 0, st>>>(ssegs[lv & 1].as<SahSeg>(), sperm[lv & 1].as<int>(),
-                                                      sperm[(lv + 1) & 1].as<int>(), order_out.as<int>(), bmin, bmax,
-                                                      A.nodes.as<float4>(), nctr.as<int>(),
-                                                      ssegs[(lv + 1) & 1].as<SahSeg>(), cnt.as<int>(), leaf_max);
-                KRT_HIP(hipGetLastError());
-                KRT_HIP(hipMemcpyAsync(&nseg, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
-                KRT_HIP(hipStreamSynchronize(st));
-                lv++;
-                KRT_REQUIRE(lv < 100000, KRT_ERR_HIP, "SAH build did not converge");
-            }
-            KRT_HIP(hipMemcpyAsync(&nnodes, nctr.p, sizeof(int), hipMemcpyDeviceToHost, st));
-            KRT_HIP(hipStreamSynchronize(st));
-            depth = lv + 1;
-        }
     }
     // each wide level pushes at most three siblings
     const uint32_t wide_levels = collapse_wide(ctx, A, nnodes);

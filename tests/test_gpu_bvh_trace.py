@@ -21,6 +21,12 @@ def degenerate_scene(kind):
     elif kind == "signed_zero":
         s.positions[::2, 1] = -0.0
         s.positions[1::2, 1] = 0.0
+    elif kind == "signed_zero_large":  # several elements per lane, ties between +0 and -0 at every level
+        s = scenes.random_soup(6000, seed=12)
+        rng = np.random.default_rng(4)
+        for a in range(3):
+            sel = rng.random(len(s.positions)) < 0.3
+            s.positions[sel, a] = np.where(rng.random(sel.sum()) < 0.5, -0.0, 0.0).astype(np.float32)
     return s
 
 
@@ -34,6 +40,7 @@ SCENES = {
     "points": lambda: degenerate_scene("points"),
     "collinear": lambda: degenerate_scene("collinear"),
     "signed_zero": lambda: degenerate_scene("signed_zero"),
+    "signed_zero_large": lambda: degenerate_scene("signed_zero_large"),
     "cornell": scenes.cornell,
 }
 
