@@ -491,7 +491,7 @@ def _texture(rng: SplitMix64, w: int, h: int, fn) -> np.ndarray:
     return img
 
 
-def textured_room(seed: int = 6) -> SceneData:
+def textured_room(seed: int = 6, coarse: bool = False) -> SceneData:
     """The Cornell room with every texture kind of calculate_material /
     calculate_normal in use (RGBA8 images, 5 per material: diffuse, normal,
     specular, emission, roughness):
@@ -504,7 +504,9 @@ def textured_room(seed: int = 6) -> SceneData:
     * the ceiling light: an emission texture beside its constant emission
       15; a small wall quad with an emission texture and zero constant
       emission (its triangles are lights of zero intensity);
-    * right wall, ceiling and the short box untextured."""
+    * right wall, ceiling and the short box untextured.
+    coarse=True tessellates it lightly (580 triangles instead of 10,084)."""
+    gw, gb, lvl = (4, 2, 2) if coarse else (16, 10, 4)
     rng = SplitMix64(seed)
     checker = _texture(rng, 16, 16, lambda x, y, r: (0.85, 0.8, 0.7, 1.0) if (x // 4 + y // 4) % 2 else
                        (0.15, 0.25, 0.45, 1.0))
@@ -536,19 +538,19 @@ def textured_room(seed: int = 6) -> SceneData:
     for o, du, dv, nrm, tex, scale in walls:
         m = mat(material([rng.uniform(0.2, 0.8) for _ in range(3)], rng.uniform(0.2, 1.0),
                          specular=(0.2, 0.2, 0.2) if "specular" in tex else (0.0, 0.0, 0.0)), **tex)
-        uv, tan = grid_frame(du, dv, 16, scale)
-        b.add(*grid_quad(o, du, dv, 16, nrm), m, uv, tan)
+        uv, tan = grid_frame(du, dv, gw, scale)
+        b.add(*grid_quad(o, du, dv, gw, nrm), m, uv, tan)
     for center, half, tex in (((-0.4, 0.6, -0.35), (0.3, 0.6, 0.3), dict(diffuse=NOISE)),
                               ((0.45, 0.3, 0.3), (0.3, 0.3, 0.3), {})):
         m = mat(material([rng.uniform(0.2, 0.8) for _ in range(3)], rng.uniform(0.05, 1.0)), **tex)
-        for face in box_mesh(center, half, 10):
+        for face in box_mesh(center, half, gb):
             pos = face[0]
-            du = pos[11] - pos[0]  # grid_quad: vertex (a=1, b=0) minus (0, 0), times n
+            du = pos[gb + 1] - pos[0]  # grid_quad: vertex (a=1, b=0) minus (0, 0), times n
             dv = pos[1] - pos[0]
-            uv, tan = grid_frame(du * 10, dv * 10, 10, (2.0, 2.0), (-1.0, -1.5))
+            uv, tan = grid_frame(du * gb, dv * gb, gb, (2.0, 2.0), (-1.0, -1.5))
             b.add(*face, m, uv, tan)
     m = mat(material((0.7, 0.7, 0.7), 0.4), diffuse=CHECKER, nmap=NORMAL)
-    v, f = icosphere(4)
+    v, f = icosphere(lvl)
     r, c = 0.28, np.array([0.35, 0.88, 0.25])
     uv = np.stack([np.arctan2(v[:, 0], v[:, 2]) / (2 * np.pi) + 0.5, np.arccos(np.clip(v[:, 1], -1, 1)) / np.pi],
                   axis=1) * 2.0

@@ -8,9 +8,10 @@ Each `kernel<<<grid, block, shmem, stream>>>(args);` launch is rewritten into
 `krt_emu::launch("kernel", grid, block, shmem, stream,
 krt_emu::bind_args([](auto &&...a) { kernel(a...); }, args));` -- the
 arguments are evaluated and copied at the launch, as a real launch does;
-the only other change is that the two-level (instanced) path, which the GPU
-build keeps switched off, is switched on (kTwoLevelEnabled), so that the host
-run can exercise it.
+the only other changes are that the two-level (instanced) path, which the
+GPU build keeps switched off, is switched on (kTwoLevelEnabled), so that the
+host run can exercise it, and that the BVH build's top levels use 256-lane
+workgroups as its lower ones do (kBuildWide).
 """
 from __future__ import annotations
 
@@ -143,6 +144,12 @@ def prepare() -> list[Path]:
             flag = "constexpr bool kTwoLevelEnabled = false;"
             assert flag in text, "kTwoLevelEnabled not found"
             text = text.replace(flag, "constexpr bool kTwoLevelEnabled = true;")
+            # the build's result does not depend on its workgroup width (the
+            # GPU tests compare both widths' trees with the oracle); one
+            # emulated lane is a fiber, so the host run keeps the narrow one
+            wide = "constexpr int kBuildWide = 1024;"
+            assert wide in text, "kBuildWide not found"
+            text = text.replace(wide, "constexpr int kBuildWide = 256;")
         (src_dir / f.name).write_text(text)
         if f.suffix in (".hip", ".cpp"):
             units.append(src_dir / f.name)

@@ -277,11 +277,11 @@ def case_textures(ctx):
     oracle."""
     from artemis_amd import Server
 
-    s = scenes.textured_room()
+    s = scenes.textured_room(coarse=True)
     sc = ctx.upload(s)
     osc = O.OracleScene(s, sc.serialized())
     assert bits_equal(sc.lights(), osc.lights()), "lights"
-    W, H = 24, 16
+    W, H = 16, 12
     cam = look_at(**s.camera, aspect=W / H)
     arm = ArmadaRTX(ctx, W, H)
     arm.attach("pt", PathTracer(ctx))
@@ -315,7 +315,7 @@ def case_textures(ctx):
     arm.set_option("radius", 8)
     arm.set_depth(3)
     ogi = O.OracleGI(W, H)
-    for _ in range(2):
+    for _ in range(1):
         arm.render(sc, cam)
         ogi.frame(osc, cam, max_depth=3, m_cap=30, spatial_passes=2, neighbours=5, radius=8)
     for which in (0, 1):
