@@ -879,9 +879,12 @@ static uint32_t collapse_wide(Context &ctx, Accel &A, int nbin) {
 // and its 4-wide collapse (A.wide).  order receives the primitive at every
 // position of the traversal tree's leaf ranges.
 // Workgroup size of a level: the top levels have few nodes with many
-// primitives each, where more lanes per node hide more of the memory latency.
+// primitives each, where more lanes per node hide more of the memory latency;
+// the deep levels have tens of thousands of nodes of a few dozen primitives,
+// where one wavefront per node keeps the per-node reductions short.
 constexpr int kBuildWide = 1024;
 constexpr int kBuildNarrow = 256;
+constexpr int kBuildWave = 64;
 
 static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, uint32_t n, int leaf_max, int layout,
                         Accel &A, DevBuf &order_out, DevBuf &rank_of) {
@@ -976,9 +979,12 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
             if (nseg < 1024)
                 level_kernel<kBuildWide><<<nseg, kBuildWide, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout, cnt.as<int>(),
                                                                       bi, bo);
-            else
+            else if (nseg < 8192)
                 level_kernel<kBuildNarrow><<<nseg, kBuildNarrow, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout,
                                                                           cnt.as<int>(), bi, bo);
+            else
+                level_kernel<kBuildWave><<<nseg, kBuildWave, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout,
+                                                                      cnt.as<int>(), bi, bo);
             KRT_HIP(hipGetLastError());
             KRT_HIP(hipMemcpyAsync(&next, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
         }
@@ -993,8 +999,12 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
                 sah_level_kernel<kBuildWide><<<nseg2, kBuildWide, 0, st2>>>(sin, pin, pout, order_out.as<int>(), bi, bo,
                                                                            A.nodes.as<float4>(), nctr.as<int>(),
                                                                            sout, cnt2.as<int>(), leaf_max);
-            else
+            else if (nseg2 < 8192)
                 sah_level_kernel<kBuildNarrow><<<nseg2, kBuildNarrow, 0, st2>>>(
+                    sin, pin, pout, order_out.as<int>(), bi, bo, A.nodes.as<float4>(), nctr.as<int>(), sout,
+                    cnt2.as<int>(), leaf_max);
+            else
+                sah_level_kernel<kBuildWave><<<nseg2, kBuildWave, 0, st2>>>(
                     sin, pin, pout, order_out.as<int>(), bi, bo, A.nodes.as<float4>(), nctr.as<int>(), sout,
                     cnt2.as<int>(), leaf_max);
             KRT_HIP(hipGetLastError());

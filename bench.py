@@ -203,6 +203,37 @@ def rendering_setup(args, ctx, scene, sc, W, H, workload):
     return arm
 
 
+def bvh_workload(args, ctx):
+    """Acceleration-structure builds and refits: the bench scene (1,048,576
+    triangles) and the 2,097,152-triangle scene, each uploaded --steps times
+    (after --warmup uploads): the bvh.cpp tree and the binned-SAH traversal
+    tree with its 4-wide collapse, then a refit after moving one sphere."""
+    from artemis_amd import scenes
+
+    out = {}
+    for name, data in (("1M", scenes.spheres()), ("2M", scenes.spheres(instances=512, seed=3))):
+        build, refit = [], []
+        for k in range(args.warmup + args.steps):
+            sc = ctx.upload(data)
+            ms = sc.stats()["build_ms"]
+            pos = np.array(data.positions[:3 * 1024], np.float32) + np.float32(0.01)
+            sc.update_vertices(0, pos)
+            sc.refit()
+            if k >= args.warmup:
+                build.append(ms)
+                refit.append(sc.stats()["refit_ms"])
+            sc.free()
+        log(f"[bvh] {name}: build {np.median(build):.2f} ms, refit {np.median(refit):.2f} ms")
+        out[name] = {"triangles": data.triangles, "build_ms_p50": float(np.median(build)),
+                     "refit_ms_p50": float(np.median(refit))}
+    return {"metric": "BVH build / refit time", "value": out["1M"]["build_ms_p50"], "unit": "ms",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["1M"]["build_ms_p50"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded procedural scenes)",
+            "config": {"workload": "BVH build (bvh.cpp tree + binned-SAH tree + 4-wide collapse) and refit"},
+            "scenes": out}
+
+
 def mlp_workload(args, ctx):
     """The radiance-cache network alone: per step, inference on 2^21 queries
     and one training step on 2^16 samples, inputs resident in HBM."""
@@ -779,7 +810,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["restir_di", "pt", "pt_cache", "restir_cache", "restir_gi", "mlp",
-                                           "serve", "relmse"],
+                                           "serve", "relmse", "bvh"],
                     default="restir_di")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -827,6 +858,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(serve_workload(args, ctx)), flush=True)
+        return
+    if args.workload == "bvh":
+        print(json.dumps(bvh_workload(args, ctx)), flush=True)
         return
     W, H = args.width, args.height
     wl = "restir_di" if args.workload == "relmse" else args.workload
