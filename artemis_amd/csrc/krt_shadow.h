@@ -96,14 +96,32 @@ __global__ __launch_bounds__(kScanBlock) void seg_scan_kernel(RayList L) {
 
 constexpr int kCompactBlock = 256;
 
+// Direction class of a shadow ray: its octahedral projection on a
+// kDirCells x kDirCells grid, so that rays of a segment toward nearby light
+// samples share a class.
+constexpr int kDirCells = 8;
+KRT_DEV uint32_t dir_class(float x, float y, float z) {
+    const float s = fabsf(x) + fabsf(y) + fabsf(z);
+    float u = s > 0.0f ? x / s : 0.0f, v = s > 0.0f ? y / s : 0.0f;
+    if (z < 0.0f) {
+        const float uu = (1.0f - fabsf(v)) * (u < 0.0f ? -1.0f : 1.0f);
+        v = (1.0f - fabsf(u)) * (v < 0.0f ? -1.0f : 1.0f);
+        u = uu;
+    }
+    const int iu = min(kDirCells - 1, max(0, (int)((u * 0.5f + 0.5f) * (float)kDirCells)));
+    const int iv = min(kDirCells - 1, max(0, (int)((v * 0.5f + 0.5f) * (float)kDirCells)));
+    return (uint32_t)(iv * kDirCells + iu);
+}
+
 // Segment b -> dense[prefix[b] ...], one workgroup per segment.  Shadow rays
-// (`by_octant`) are grouped by direction octant with a counting sort in LDS,
-// so that a wavefront of the walk starts 64 rays of one octant (their
+// (`by_octant`) are grouped by direction class with a counting sort in LDS,
+// so that a wavefront of the walk starts rays of similar directions (their
 // targets are scattered light samples); closest rays keep their order, in
 // which a wavefront's rays come from one 8x8 tile.  Results are written by
 // slot, so the order changes no answer.
 __global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L, bool by_octant) {
-    __shared__ uint32_t bins[8], base[8];
+    constexpr int NB = kDirCells * kDirCells;
+    __shared__ uint32_t bins[NB], base[NB];
     const uint32_t b = blockIdx.x;
     const uint32_t n = L.counts[b];
     const float4 *src = L.rays + 2 * (size_t)b * L.seg;
@@ -112,17 +130,16 @@ __global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L, b
         for (uint32_t k = threadIdx.x; k < 2 * n; k += kCompactBlock) dst[k] = src[k];
         return;
     }
-    if (threadIdx.x < 8) bins[threadIdx.x] = 0;
+    if (threadIdx.x < NB) bins[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < n; k += kCompactBlock) {
         const float4 c = src[2 * k + 1];
-        const uint32_t oct = (c.x < 0.0f ? 1u : 0u) | (c.y < 0.0f ? 2u : 0u) | (c.z < 0.0f ? 4u : 0u);
-        atomicAdd(&bins[oct], 1u);
+        atomicAdd(&bins[dir_class(c.x, c.y, c.z)], 1u);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (int o = 0; o < 8; o++) {
+        for (int o = 0; o < NB; o++) {
             base[o] = run;
             run += bins[o];
         }
@@ -130,8 +147,7 @@ __global__ __launch_bounds__(kCompactBlock) void seg_compact_kernel(RayList L, b
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < n; k += kCompactBlock) {
         const float4 a = src[2 * k], c = src[2 * k + 1];
-        const uint32_t oct = (c.x < 0.0f ? 1u : 0u) | (c.y < 0.0f ? 2u : 0u) | (c.z < 0.0f ? 4u : 0u);
-        const uint32_t at = atomicAdd(&base[oct], 1u);
+        const uint32_t at = atomicAdd(&base[dir_class(c.x, c.y, c.z)], 1u);
         dst[2 * at] = a;
         dst[2 * at + 1] = c;
     }
