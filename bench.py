@@ -937,6 +937,14 @@ def main():
             log(f"[bench] mlp failed: {e!r}")
             extra["mlp"] = {"error": repr(e)}
         try:
+            ba = argparse.Namespace(**vars(args))
+            ba.steps, ba.warmup = 2, 1
+            extra["bvh"] = {k: v for k, v in bvh_workload(ba, ctx).items()
+                            if k not in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline")}
+        except Exception as e:
+            log(f"[bench] bvh failed: {e!r}")
+            extra["bvh"] = {"error": repr(e)}
+        try:
             extra["serve"] = {k: v for k, v in serve_workload(args, ctx, rates=[10.0, 50.0, 200.0],
                                                               seconds=3.0).items()
                               if k not in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline")}
