@@ -1054,6 +1054,8 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
     // each wide level pushes at most three siblings
     const uint32_t wide_levels = collapse_wide(ctx, A, nnodes);
     A.nt = n;
+    A.ser_depth_off.clear();  // a new tree: the refit schedule is made again
+    A.ser_by_depth.release();
     A.nser = nser;
     A.nnodes = (uint32_t)nnodes;
     A.levels = levels;
@@ -1230,10 +1232,12 @@ static void accel_build_instanced(Context &ctx, Scene &s, int leaf_max, int layo
 // Refit (moved geometry, same topology): Accelerator::update rebuilds the
 // acceleration structure when an entity moves (accelerator.cuh:145-196,
 // armada.cu:575-602, 716-727); here the trees keep their shape and their
-// boxes are recomputed bottom-up from the new vertex positions.  A node is
-// computed by the last of its children to finish (an arrival counter per
-// node): leaves start, each finished node counts itself at its parent, and
-// the thread whose arrival completes a parent goes on with it.  Boxes are
+// boxes are recomputed bottom-up from the new vertex positions: the bvh.cpp
+// tree one depth per launch (its million-node chains would otherwise pay a
+// device-scope fence per step), the traversal tree by arrival counters -- a
+// node is computed by the last of its children to finish: leaves start,
+// each finished node counts itself at its parent, and the thread whose
+// arrival completes a parent goes on with it.  Boxes are
 // unions (min / max) of triangle boxes, so a refit node's box is exactly the
 // one a build of the same shape would give.
 // ---------------------------------------------------------------------------
@@ -1253,24 +1257,68 @@ __global__ void ser_parent_kernel(const float4 *ser, uint32_t nser, int *parent)
 }
 
 // bvh.cpp layout: leaf box = its primitive's box, internal = union of both children
-__global__ void ser_refit_kernel(float4 *ser, uint32_t nser, const int *parent, int *arrivals, const float4 *bmin,
-                                 const float4 *bmax) {
+
+// Level-synchronous refit of the bvh.cpp tree: the depth of every node from
+// the parent links (-1 for leaves), once per built tree; then per refit the
+// leaves, and the internal nodes one depth per launch from the deepest up,
+// each the union of its two children (finished by the launch before).  No
+// device-scope fences between the steps: the launches order them.
+__global__ void ser_depth_kernel(const float4 *ser, const int *parent, uint32_t nser, int *depth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nser) return;
+    int d = 0;
+    for (int p = parent[i]; p >= 0; p = parent[p]) d++;
+    depth[i] = ser[3 * i].x != 0.0f ? -1 : d;
+}
+
+// Counting sort of the internal nodes by depth on the device: a histogram
+// (per-workgroup in LDS, then added), the offsets, and the scatter (any
+// order within a depth: its nodes are independent).
+constexpr int kMaxSerDepth = 1024;
+__global__ void depth_hist_kernel(const int *depth, uint32_t nser, uint32_t *hist) {
+    __shared__ uint32_t h[kMaxSerDepth];
+    for (int k = threadIdx.x; k < kMaxSerDepth; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nser && depth[i] >= 0) atomicAdd(&h[depth[i]], 1u);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kMaxSerDepth; k += blockDim.x)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+__global__ void depth_offsets_kernel(const uint32_t *hist, uint32_t *off, uint32_t *fill) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t run = 0;
+    for (int d = 0; d < kMaxSerDepth; d++) {
+        off[d] = fill[d] = run;
+        run += hist[d];
+    }
+    off[kMaxSerDepth] = run;
+}
+
+__global__ void depth_scatter_kernel(const int *depth, uint32_t nser, uint32_t *fill, int *list) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nser || depth[i] < 0) return;
+    list[atomicAdd(&fill[depth[i]], 1u)] = (int)i;
+}
+
+__global__ void ser_leaf_kernel(float4 *ser, uint32_t nser, const float4 *bmin, const float4 *bmax) {
     const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
     if (leaf >= nser || ser[3 * leaf].x == 0.0f) return;
     const int prim = __float_as_int(ser[3 * leaf].y);
     ser[3 * leaf + 1] = make_float4(bmin[prim].x, bmin[prim].y, bmin[prim].z, 0.0f);
     ser[3 * leaf + 2] = make_float4(bmax[prim].x, bmax[prim].y, bmax[prim].z, 0.0f);
-    int p = parent[leaf];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&arrivals[p], 1) == 0) return;  // the other child finishes this node
-        const int c0 = p + 1, c1 = __float_as_int(ser[3 * c0].w) / 3;
-        const float4 a0 = vload(ser + 3 * c0 + 1), a1 = vload(ser + 3 * c0 + 2);
-        const float4 b0 = vload(ser + 3 * c1 + 1), b1 = vload(ser + 3 * c1 + 2);
-        ser[3 * p + 1] = make_float4(gmin(a0.x, b0.x), gmin(a0.y, b0.y), gmin(a0.z, b0.z), 0.0f);
-        ser[3 * p + 2] = make_float4(gmax(a1.x, b1.x), gmax(a1.y, b1.y), gmax(a1.z, b1.z), 0.0f);
-        p = parent[p];
-    }
+}
+
+__global__ void ser_level_refit_kernel(float4 *ser, const int *nodes, uint32_t count) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int p = nodes[k];
+    const int c0 = p + 1, c1 = __float_as_int(ser[3 * c0].w) / 3;
+    const float4 a0 = ser[3 * c0 + 1], a1 = ser[3 * c0 + 2];
+    const float4 b0 = ser[3 * c1 + 1], b1 = ser[3 * c1 + 2];
+    ser[3 * p + 1] = make_float4(gmin(a0.x, b0.x), gmin(a0.y, b0.y), gmin(a0.z, b0.z), 0.0f);
+    ser[3 * p + 2] = make_float4(gmax(a1.x, b1.x), gmax(a1.y, b1.y), gmax(a1.z, b1.z), 0.0f);
 }
 
 // Binary traversal tree (either layout): links are node ids (> 0) or leaf
@@ -1382,14 +1430,42 @@ void accel_refit(Context &ctx, Scene &s) {
     tri_boxes_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), n, bmin.as<float4>(),
                                                      bmax.as<float4>());
     KRT_HIP(hipGetLastError());
-    // bvh.cpp tree
-    DevBuf parent(sizeof(int) * A.nser), arrivals(sizeof(int) * A.nser);
-    KRT_HIP(hipMemsetAsync(arrivals.p, 0, arrivals.bytes, st));
-    ser_parent_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, parent.as<int>());
+    // bvh.cpp tree: its schedule (internal nodes by depth) once per build
+    if (A.ser_depth_off.empty()) {
+        DevBuf parent(sizeof(int) * A.nser), depth(sizeof(int) * A.nser);
+        ser_parent_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, parent.as<int>());
+        KRT_HIP(hipGetLastError());
+        ser_depth_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), parent.as<int>(), A.nser,
+                                                              depth.as<int>());
+        KRT_HIP(hipGetLastError());
+        KRT_REQUIRE(A.levels + 2 < (uint32_t)kMaxSerDepth, KRT_ERR_STATE,
+                    "bvh.cpp tree deeper than 1024 levels: rebuild (krt_accel_build) instead of refit");
+        DevBuf hist(sizeof(uint32_t) * kMaxSerDepth), off(sizeof(uint32_t) * (kMaxSerDepth + 1)),
+            fill(sizeof(uint32_t) * kMaxSerDepth);
+        KRT_HIP(hipMemsetAsync(hist.p, 0, hist.bytes, st));
+        depth_hist_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(depth.as<int>(), A.nser, hist.as<uint32_t>());
+        depth_offsets_kernel<<<1, 1, 0, st>>>(hist.as<uint32_t>(), off.as<uint32_t>(), fill.as<uint32_t>());
+        A.ser_by_depth.alloc(sizeof(int) * A.nser);
+        depth_scatter_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(depth.as<int>(), A.nser, fill.as<uint32_t>(),
+                                                                  A.ser_by_depth.as<int>());
+        KRT_HIP(hipGetLastError());
+        std::vector<uint32_t> oh(kMaxSerDepth + 1);
+        KRT_HIP(hipMemcpyAsync(oh.data(), off.p, sizeof(uint32_t) * oh.size(), hipMemcpyDeviceToHost, st));
+        KRT_HIP(hipStreamSynchronize(st));
+        size_t used = 1;
+        while (used < (size_t)kMaxSerDepth && oh[used] < oh[kMaxSerDepth]) used++;
+        A.ser_depth_off.assign(oh.begin(), oh.begin() + (long)used + 1);  // depths 0 .. used-1
+    }
+    ser_leaf_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, bmin.as<float4>(),
+                                                         bmax.as<float4>());
     KRT_HIP(hipGetLastError());
-    ser_refit_kernel<<<blocks(A.nser, 256), 256, 0, st>>>(A.ser.as<float4>(), A.nser, parent.as<int>(),
-                                                          arrivals.as<int>(), bmin.as<float4>(), bmax.as<float4>());
-    KRT_HIP(hipGetLastError());
+    for (size_t d = A.ser_depth_off.size() - 1; d-- > 0;) {
+        const uint32_t first = A.ser_depth_off[d], count = A.ser_depth_off[d + 1] - first;
+        if (!count) continue;
+        ser_level_refit_kernel<<<blocks(count, 256), 256, 0, st>>>(A.ser.as<float4>(),
+                                                                   A.ser_by_depth.as<int>() + first, count);
+        KRT_HIP(hipGetLastError());
+    }
     // triangle records, then the binary traversal tree over them, then its 4-wide collapse
     tri_repack_kernel<<<blocks(n, 256), 256, 0, st>>>(s.pos.as<float>(), s.idx.as<uint32_t>(), n, A.tri.as<float4>());
     KRT_HIP(hipGetLastError());

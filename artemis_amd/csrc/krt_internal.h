@@ -92,6 +92,10 @@ struct Accel {
     DevBuf nodes;      // float4 x 4 per binary traversal node (collapse input)
     DevBuf wide;       // float4 x 8 per 4-wide traversal node (what the kernels walk)
     uint32_t nwide = 0;
+    // refit schedule of the bvh.cpp tree (made at the first refit after a
+    // build): internal nodes grouped by depth, and each depth's offset
+    DevBuf ser_by_depth;
+    std::vector<uint32_t> ser_depth_off;
 };
 
 struct Scene {
