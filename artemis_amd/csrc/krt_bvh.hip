@@ -1297,9 +1297,18 @@ __global__ void depth_offsets_kernel(const uint32_t *hist, uint32_t *off, uint32
 }
 
 __global__ void depth_scatter_kernel(const int *depth, uint32_t nser, uint32_t *fill, int *list) {
+    // a workgroup reserves one range per depth it holds, then fills it
+    __shared__ uint32_t cnt[kMaxSerDepth], base[kMaxSerDepth];
+    for (int k = threadIdx.x; k < kMaxSerDepth; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nser || depth[i] < 0) return;
-    list[atomicAdd(&fill[depth[i]], 1u)] = (int)i;
+    const int d = i < nser ? depth[i] : -1;
+    const uint32_t local = d >= 0 ? atomicAdd(&cnt[d], 1u) : 0u;
+    __syncthreads();
+    for (int k = threadIdx.x; k < kMaxSerDepth; k += blockDim.x)
+        if (cnt[k]) base[k] = atomicAdd(&fill[k], cnt[k]);
+    __syncthreads();
+    if (d >= 0) list[base[d] + local] = (int)i;
 }
 
 __global__ void ser_leaf_kernel(float4 *ser, uint32_t nser, const float4 *bmin, const float4 *bmax) {
