@@ -205,6 +205,15 @@ public:
         check(krt_accel_build(m_ctx.get(), h, leaf_max));
         return h;
     }
+    // The same with material textures and normal maps (convert_material's
+    // textures, armada.cu:333-398; Vertex::tex_coords / tangent / bitangent).
+    krt_handle update(const krt_scene_desc &scene, const krt_scene_textures &textures, int leaf_max = 4) {
+        krt_handle h = 0;
+        check(krt_scene_upload_textured(m_ctx.get(), &scene, &textures, &h));
+        m_scenes.push_back(h);
+        check(krt_accel_build(m_ctx.get(), h, leaf_max));
+        return h;
+    }
     // A moved entity: new world-space vertices for [first, first + count),
     // then the trees refit to them (same shapes) or rebuilt (after a large
     // motion, where refit boxes grow loose).
