@@ -540,11 +540,98 @@ __device__ __forceinline__ float sa_of(const float *mn, const float *mx) {
     return 2.0f * (dx * dy + dy * dz + dx * dz);
 }
 
+// The top levels' centroid bounds and bins, computed by kPreBlocks
+// workgroups per node (per-workgroup LDS results merged with global
+// atomics): min / max / counts do not depend on the order, so the level
+// kernel that takes them over gives the same split.
+constexpr int kPreSegs = 16;    // levels with at most this many nodes
+constexpr int kPreBlocks = 64;  // workgroups per node
+struct PreBins {
+    LdsBox cb[kPreSegs];
+    LdsBox bin[kPreSegs][SB];
+    int cnt[kPreSegs][SB];
+};
+
+__global__ void pre_clear_kernel(PreBins *pb) {
+    const int s = blockIdx.x, t = threadIdx.x;
+    if (t == 0) pb->cb[s].clear();
+    if (t < SB) {
+        pb->bin[s][t].clear();
+        pb->cnt[s][t] = 0;
+    }
+}
+
+__device__ void pre_axis(const LdsBox &cb, int &axis, float &clo, float &scale) {
+    float ext[3];
+    for (int k = 0; k < 3; k++) ext[k] = u2f(cb.mx[k]) - u2f(cb.mn[k]);
+    axis = (ext[0] >= ext[1] && ext[0] >= ext[2]) ? 0 : (ext[1] >= ext[2] ? 1 : 2);
+    clo = u2f(cb.mn[axis]);
+    scale = ext[axis] > 0.0f ? (float)SB / ext[axis] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void pre_bounds_kernel(const SahSeg *segs, PermBoxes bin_, PreBins *pb) {
+    const int seg = blockIdx.x / kPreBlocks, part = blockIdx.x % kPreBlocks;
+    const SahSeg sg = segs[seg];
+    __shared__ LdsBox s_cb;
+    if (threadIdx.x == 0) s_cb.clear();
+    __syncthreads();
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    bool any = false;
+    for (int j = part * 256 + threadIdx.x; j < sg.count; j += kPreBlocks * 256) {
+        float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
+        float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], c[k]);
+            hi[k] = fmaxf(hi[k], c[k]);
+        }
+        any = true;
+    }
+    if (any) s_cb.add(lo, hi);
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        atomicMin(&pb->cb[seg].mn[threadIdx.x], s_cb.mn[threadIdx.x]);
+        atomicMax(&pb->cb[seg].mx[threadIdx.x], s_cb.mx[threadIdx.x]);
+    }
+}
+
+__global__ __launch_bounds__(256) void pre_bins_kernel(const SahSeg *segs, PermBoxes bin_, PreBins *pb) {
+    const int seg = blockIdx.x / kPreBlocks, part = blockIdx.x % kPreBlocks;
+    const SahSeg sg = segs[seg];
+    __shared__ LdsBox s_bin[SB];
+    __shared__ int s_cnt[SB];
+    if (threadIdx.x < SB) {
+        s_bin[threadIdx.x].clear();
+        s_cnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    int axis;
+    float clo, scale;
+    pre_axis(pb->cb[seg], axis, clo, scale);
+    for (int j = part * 256 + threadIdx.x; j < sg.count; j += kPreBlocks * 256) {
+        float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
+        float c = 0.5f * (comp(a, axis) + comp(b, axis));
+        int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
+        atomicAdd(&s_cnt[bin], 1);
+        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+        s_bin[bin].add(lo, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x < SB && s_cnt[threadIdx.x]) {
+        const int b = threadIdx.x;
+        atomicAdd(&pb->cnt[seg][b], s_cnt[b]);
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&pb->bin[seg][b].mn[k], s_bin[b].mn[k]);
+            atomicMax(&pb->bin[seg][b].mx[k], s_bin[b].mx[k]);
+        }
+    }
+}
+
+// pb != nullptr: the level's centroid bounds and bins come from the pre-pass.
 template <int NT>
 __global__ __launch_bounds__(NT) void sah_level_kernel(const SahSeg *segs, const int *perm_in, int *perm_out,
                                                          int *perm_final, PermBoxes bin_, PermBoxes bout,
                                                          float4 *nodes, int *node_ctr, SahSeg *next,
-                                                         int *next_count, int leaf_max) {
+                                                         int *next_count, int leaf_max, const PreBins *pb) {
     const SahSeg sg = segs[blockIdx.x];
     const int tid = threadIdx.x;
     const int n = sg.count;
@@ -564,7 +651,9 @@ __global__ __launch_bounds__(NT) void sah_level_kernel(const SahSeg *segs, const
     if (tid < 2) s_child[tid].clear();
     __syncthreads();
     // centroid bounds
-    {
+    if (pb) {
+        if (tid == 0) s_cb = pb->cb[blockIdx.x];
+    } else {
         float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
         for (int j = tid; j < n; j += NT) {
             float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
@@ -589,13 +678,20 @@ __global__ __launch_bounds__(NT) void sah_level_kernel(const SahSeg *segs, const
     const int axis = s_axis;
     const float clo = s_lo, scale = s_scale;
     // bins
-    for (int j = tid; j < n; j += NT) {
-        float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
-        float c = 0.5f * (comp(a, axis) + comp(b, axis));
-        int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
-        atomicAdd(&s_cnt[bin], 1);
-        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
-        s_bin[bin].add(lo, hi);
+    if (pb) {
+        if (tid < SB) {
+            s_bin[tid] = pb->bin[blockIdx.x][tid];
+            s_cnt[tid] = pb->cnt[blockIdx.x][tid];
+        }
+    } else {
+        for (int j = tid; j < n; j += NT) {
+            float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
+            float c = 0.5f * (comp(a, axis) + comp(b, axis));
+            int bin = min(SB - 1, max(0, (int)((c - clo) * scale)));
+            atomicAdd(&s_cnt[bin], 1);
+            float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+            s_bin[bin].add(lo, hi);
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -899,6 +995,7 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
     // boxes in permutation order, double-buffered per tree (level 0 reads
     // bmin / bmax: its permutation is the identity)
     DevBuf pbox[2][2][2];  // [tree][buffer][min / max]
+    DevBuf pre_buf;        // PreBins of the binned-SAH tree's first levels
     auto boxes = [&](int tree, uint32_t level, bool out) {
         const uint32_t b = (level + (out ? 1u : 0u)) & 1u;
         if (!out && level == 0) return PermBoxes{const_cast<float4 *>(bmin), const_cast<float4 *>(bmax)};
@@ -995,18 +1092,28 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
             const int *pin = sperm[lv & 1].as<int>();
             int *pout = sperm[(lv + 1) & 1].as<int>();
             const PermBoxes bi = boxes(1, lv, false), bo = boxes(1, lv, true);
+            // the first levels of a large tree: bounds and bins by many workgroups per node
+            const PreBins *pre = nullptr;
+            if (nseg2 <= kPreSegs && n >= (1u << 16)) {
+                if (!pre_buf.p) pre_buf.alloc(sizeof(PreBins));
+                pre = pre_buf.as<PreBins>();
+                pre_clear_kernel<<<nseg2, 64, 0, st2>>>(pre_buf.as<PreBins>());
+                pre_bounds_kernel<<<nseg2 * kPreBlocks, 256, 0, st2>>>(sin, bi, pre_buf.as<PreBins>());
+                pre_bins_kernel<<<nseg2 * kPreBlocks, 256, 0, st2>>>(sin, bi, pre_buf.as<PreBins>());
+                KRT_HIP(hipGetLastError());
+            }
             if (nseg2 < 1024)
                 sah_level_kernel<kBuildWide><<<nseg2, kBuildWide, 0, st2>>>(sin, pin, pout, order_out.as<int>(), bi, bo,
                                                                            A.nodes.as<float4>(), nctr.as<int>(),
-                                                                           sout, cnt2.as<int>(), leaf_max);
+                                                                           sout, cnt2.as<int>(), leaf_max, pre);
             else if (nseg2 < 8192)
                 sah_level_kernel<kBuildNarrow><<<nseg2, kBuildNarrow, 0, st2>>>(
                     sin, pin, pout, order_out.as<int>(), bi, bo, A.nodes.as<float4>(), nctr.as<int>(), sout,
-                    cnt2.as<int>(), leaf_max);
+                    cnt2.as<int>(), leaf_max, nullptr);
             else
                 sah_level_kernel<kBuildWave><<<nseg2, kBuildWave, 0, st2>>>(
                     sin, pin, pout, order_out.as<int>(), bi, bo, A.nodes.as<float4>(), nctr.as<int>(), sout,
-                    cnt2.as<int>(), leaf_max);
+                    cnt2.as<int>(), leaf_max, nullptr);
             KRT_HIP(hipGetLastError());
             KRT_HIP(hipMemcpyAsync(&next2, cnt2.p, sizeof(int), hipMemcpyDeviceToHost, st2));
         }
