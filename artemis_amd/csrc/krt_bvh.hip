@@ -114,6 +114,149 @@ struct PermBoxes {
     float4 *mn, *mx;
 };
 
+// The bvh.cpp tree's first levels: pass 1 (largest single extent, union
+// box) and pass 2 (the bins) by kPreWide workgroups per node.  Every ordered
+// choice becomes a 64-bit key that one atomic min / max settles in any order:
+// {value, index} with ties to the lower index, +0 and -0 compared equal (the
+// winning element's own bits are read back), so the level kernel that takes
+// the results over makes the sequential scan's choices.
+constexpr int kPreWideSegs = 16;
+constexpr uint32_t kPreMinPrims = 1u << 16;  // both trees' pre-passes: builds of at least this many
+constexpr int kPreWide = 64;
+struct PreExt {
+    unsigned long long ext;    // {extent bits, ~index}: max
+    unsigned long long mn[3];  // {order key of the coordinate, index}: min
+    unsigned long long mx[3];  // {order key, ~index}: max
+    int cnt[NBIN];
+    uint32_t bmn[NBIN][3], bmx[NBIN][3];  // order keys
+};
+
+__device__ __forceinline__ uint32_t okey(float v) {  // order-preserving, +0 == -0
+    const uint32_t u = __float_as_uint(v == 0.0f ? 0.0f : v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ounkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void pre_ext_clear_kernel(PreExt *pe) {
+    PreExt &e = pe[blockIdx.x];
+    const int t = threadIdx.x;
+    if (t == 0) e.ext = 0ull;
+    if (t < 3) {
+        e.mn[t] = ~0ull;
+        e.mx[t] = 0ull;
+    }
+    if (t < NBIN) {
+        e.cnt[t] = 0;
+        for (int a = 0; a < 3; a++) {
+            e.bmn[t][a] = 0xffffffffu;
+            e.bmx[t][a] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pre_ext_kernel(const Seg *segs, PermBoxes bin_, PreExt *pe) {
+    const int seg = blockIdx.x / kPreWide, part = blockIdx.x % kPreWide;
+    const Seg sg = segs[seg];
+    __shared__ unsigned long long s_ext, s_mn[3], s_mx[3];
+    if (threadIdx.x == 0) s_ext = 0ull;
+    if (threadIdx.x < 3) {
+        s_mn[threadIdx.x] = ~0ull;
+        s_mx[threadIdx.x] = 0ull;
+    }
+    __syncthreads();
+    unsigned long long ext = 0ull, mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0ull, 0ull, 0ull};
+    for (int j = part * 256 + threadIdx.x; j < sg.count; j += kPreWide * 256) {
+        const float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
+        const float pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+        float e = 0.0f;
+        for (int k = 0; k < 3; k++) e = fmaxf(e, fabsf(pb[k] - pa[k]));
+        const unsigned long long ke = (unsigned long long)__float_as_uint(e) << 32 | (0xffffffffu - (uint32_t)j);
+        ext = ke > ext ? ke : ext;
+        for (int k = 0; k < 3; k++) {
+            const unsigned long long kn = (unsigned long long)okey(pa[k]) << 32 | (uint32_t)j;
+            const unsigned long long kx = (unsigned long long)okey(pb[k]) << 32 | (0xffffffffu - (uint32_t)j);
+            mn[k] = kn < mn[k] ? kn : mn[k];
+            mx[k] = kx > mx[k] ? kx : mx[k];
+        }
+    }
+    atomicMax(&s_ext, ext);
+    for (int k = 0; k < 3; k++) {
+        atomicMin(&s_mn[k], mn[k]);
+        atomicMax(&s_mx[k], mx[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&pe[seg].ext, s_ext);
+    if (threadIdx.x < 3) {
+        atomicMin(&pe[seg].mn[threadIdx.x], s_mn[threadIdx.x]);
+        atomicMax(&pe[seg].mx[threadIdx.x], s_mx[threadIdx.x]);
+    }
+}
+
+// The winner of pass 1 from the keys: the element of largest extent (lowest
+// index on ties) and its first axis of that extent; extent 0 everywhere
+// keeps the scan's initial values.
+__device__ void pre_ext_result(const PreExt &e, const Seg &sg, PermBoxes bin_, int &axis, float &lo, float &hi) {
+    axis = 0;
+    lo = FLT_MAX;
+    hi = -FLT_MAX;
+    if ((e.ext >> 32) == 0ull) return;
+    const int j = (int)(0xffffffffu - (uint32_t)(e.ext & 0xffffffffull));
+    const float4 a = bin_.mn[sg.start + j], b = bin_.mx[sg.start + j];
+    float best = 0.0f;
+    for (int k = 0; k < 3; k++) {
+        const float ev = fabsf(comp(b, k) - comp(a, k));
+        if (ev > best) {
+            best = ev;
+            lo = comp(a, k);
+            hi = comp(b, k);
+            axis = k;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pre_split_bins_kernel(const Seg *segs, PermBoxes bin_, PreExt *pe) {
+    const int seg = blockIdx.x / kPreWide, part = blockIdx.x % kPreWide;
+    const Seg sg = segs[seg];
+    __shared__ int s_cnt[NBIN];
+    __shared__ uint32_t s_mn[NBIN][3], s_mx[NBIN][3];
+    if (threadIdx.x < NBIN) {
+        s_cnt[threadIdx.x] = 0;
+        for (int a = 0; a < 3; a++) {
+            s_mn[threadIdx.x][a] = 0xffffffffu;
+            s_mx[threadIdx.x][a] = 0u;
+        }
+    }
+    __syncthreads();
+    int axis;
+    float lo, hi;
+    pre_ext_result(pe[seg], sg, bin_, axis, lo, hi);
+    float split[10];
+    for (int i = 0; i < 10; i++) split[i] = (hi - lo) / 10.0f * (float)i + lo;
+    for (int j = part * 256 + threadIdx.x; j < sg.count; j += kPreWide * 256) {
+        const float4 mn = bin_.mn[sg.start + j], mx = bin_.mx[sg.start + j];
+        const float c = (comp(mn, axis) + comp(mx, axis)) / 2.0f;
+        int bin = 0;
+        for (int i = 0; i < 10; i++) bin += (c < split[i]) ? 0 : 1;
+        atomicAdd(&s_cnt[bin], 1);
+        const float pa[3] = {mn.x, mn.y, mn.z}, pb[3] = {mx.x, mx.y, mx.z};
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&s_mn[bin][a], okey(pa[a]));
+            atomicMax(&s_mx[bin][a], okey(pb[a]));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NBIN && s_cnt[threadIdx.x]) {
+        const int b = threadIdx.x;
+        atomicAdd(&pe[seg].cnt[b], s_cnt[b]);
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&pe[seg].bmn[b][a], s_mn[b][a]);
+            atomicMax(&pe[seg].bmx[b][a], s_mx[b][a]);
+        }
+    }
+}
+
 // One workgroup of NT lanes per pending node (count >= 3).  The result does
 // not depend on NT or on the order of the reductions: candidates carry their
 // index (ties to the lower), the bins are order-free and the partition is
@@ -121,7 +264,8 @@ struct PermBoxes {
 template <int NT>
 __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *perm_in, int *perm_out,
                                                      const float4 *bmin, const float4 *bmax, BuildOut o, Seg *next,
-                                                     int *next_count, PermBoxes bin_, PermBoxes bout) {
+                                                     int *next_count, PermBoxes bin_, PermBoxes bout,
+                                                     const PreExt *pre) {
     const Seg sg = segs[blockIdx.x];
     const int tid = threadIdx.x;
     const int n = sg.count;
@@ -138,6 +282,19 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
     // larger, or smaller for the box minimum) -- also between +0 and -0.
     float best = 0.0f, lo = FLT_MAX, hi = -FLT_MAX;
     int axis = 0, jb = INT_MAX;
+    if (pre) {  // pass 1 was done by the pre-pass: take its winners
+        if (tid == 0) {
+            const PreExt &e = pre[blockIdx.x];
+            pre_ext_result(e, sg, bin_, s_axis[0], s_lo[0], s_hi[0]);
+            for (int a = 0; a < 3; a++) {
+                const int jn = (int)(uint32_t)(e.mn[a] & 0xffffffffull);
+                const int jx = (int)(0xffffffffu - (uint32_t)(e.mx[a] & 0xffffffffull));
+                s_umn[a][0] = comp(bin_.mn[sg.start + jn], a);  // the winner's own bits (+0 or -0)
+                s_umx[a][0] = comp(bin_.mx[sg.start + jx], a);
+            }
+        }
+        __syncthreads();
+    } else {
     float umn[3] = {0, 0, 0}, umx[3] = {0, 0, 0};
     int jmn[3] = {0, 0, 0}, jmx[3] = {0, 0, 0};
     for (int j = tid; j < n; j += NT) {
@@ -222,6 +379,7 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
         }
         __syncthreads();
     }
+    }  // pass 1
     axis = s_axis[0];
     lo = s_lo[0];
     hi = s_hi[0];
@@ -242,7 +400,7 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
             bmx[b][a] = -FLT_MAX;
         }
     }
-    for (int j = tid; j < n; j += NT) {  // order-free: counts and box unions per bin
+    for (int j = pre ? n : tid; j < n; j += NT) {  // order-free: counts and box unions per bin
         float4 mn = bin_.mn[sg.start + j], mx = bin_.mx[sg.start + j];
         float c = (comp(mn, axis) + comp(mx, axis)) / 2.0f;
         int bin = 0;
@@ -296,6 +454,14 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
                     mn[b][a] = fminf(mn[b][a], w_mn[w][b][a]);
                     mx[b][a] = fmaxf(mx[b][a], w_mx[w][b][a]);
                 }
+            }
+            if (pre) {  // the bins of the pre-pass (an empty bin keeps +-FLT_MAX)
+                cnt[b] = pre[blockIdx.x].cnt[b];
+                if (cnt[b])
+                    for (int a = 0; a < 3; a++) {
+                        mn[b][a] = ounkey(pre[blockIdx.x].bmn[b][a]);
+                        mx[b][a] = ounkey(pre[blockIdx.x].bmx[b][a]);
+                    }
             }
         }
         float sa_total = surface_area(nmn.x, nmn.y, nmn.z, nmx.x, nmx.y, nmx.z);
@@ -996,6 +1162,7 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
     // bmin / bmax: its permutation is the identity)
     DevBuf pbox[2][2][2];  // [tree][buffer][min / max]
     DevBuf pre_buf;        // PreBins of the binned-SAH tree's first levels
+    DevBuf pre_ext_buf;    // PreExt of the bvh.cpp tree's first levels
     auto boxes = [&](int tree, uint32_t level, bool out) {
         const uint32_t b = (level + (out ? 1u : 0u)) & 1u;
         if (!out && level == 0) return PermBoxes{const_cast<float4 *>(bmin), const_cast<float4 *>(bmax)};
@@ -1073,15 +1240,23 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
             const int *pin = perm[levels & 1].as<int>();
             int *pout = perm[(levels + 1) & 1].as<int>();
             const PermBoxes bi = boxes(0, levels, false), bo = boxes(0, levels, true);
+            const PreExt *pre = nullptr;
+            if (nseg <= kPreWideSegs && n >= kPreMinPrims) {
+                if (!pre_ext_buf.p) pre_ext_buf.alloc(sizeof(PreExt) * kPreWideSegs);
+                pre = pre_ext_buf.as<PreExt>();
+                pre_ext_clear_kernel<<<nseg, 64, 0, st>>>(pre_ext_buf.as<PreExt>());
+                pre_ext_kernel<<<nseg * kPreWide, 256, 0, st>>>(sin, bi, pre_ext_buf.as<PreExt>());
+                pre_split_bins_kernel<<<nseg * kPreWide, 256, 0, st>>>(sin, bi, pre_ext_buf.as<PreExt>());
+            }
             if (nseg < 1024)
                 level_kernel<kBuildWide><<<nseg, kBuildWide, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout, cnt.as<int>(),
-                                                                      bi, bo);
+                                                                      bi, bo, pre);
             else if (nseg < 8192)
                 level_kernel<kBuildNarrow><<<nseg, kBuildNarrow, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout,
-                                                                          cnt.as<int>(), bi, bo);
+                                                                          cnt.as<int>(), bi, bo, pre);
             else
                 level_kernel<kBuildWave><<<nseg, kBuildWave, 0, st>>>(sin, pin, pout, bmin, bmax, o, sout,
-                                                                      cnt.as<int>(), bi, bo);
+                                                                      cnt.as<int>(), bi, bo, pre);
             KRT_HIP(hipGetLastError());
             KRT_HIP(hipMemcpyAsync(&next, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
         }
@@ -1094,7 +1269,7 @@ static void build_trees(Context &ctx, const float4 *bmin, const float4 *bmax, ui
             const PermBoxes bi = boxes(1, lv, false), bo = boxes(1, lv, true);
             // the first levels of a large tree: bounds and bins by many workgroups per node
             const PreBins *pre = nullptr;
-            if (nseg2 <= kPreSegs && n >= (1u << 16)) {
+            if (nseg2 <= kPreSegs && n >= kPreMinPrims) {
                 if (!pre_buf.p) pre_buf.alloc(sizeof(PreBins));
                 pre = pre_buf.as<PreBins>();
                 pre_clear_kernel<<<nseg2, 64, 0, st2>>>(pre_buf.as<PreBins>());

@@ -10,8 +10,9 @@ krt_emu::bind_args([](auto &&...a) { kernel(a...); }, args));` -- the
 arguments are evaluated and copied at the launch, as a real launch does;
 the only other changes are that the two-level (instanced) path, which the
 GPU build keeps switched off, is switched on (kTwoLevelEnabled), so that the
-host run can exercise it, and that the BVH build's top levels use 256-lane
-workgroups as its lower ones do (kBuildWide).
+host run can exercise it, that the BVH build's top levels use 256-lane
+workgroups as its lower ones do (kBuildWide), and that the top levels'
+pre-passes start at 64 primitives instead of 65536 (kPreMinPrims).
 """
 from __future__ import annotations
 
@@ -150,6 +151,11 @@ def prepare() -> list[Path]:
             wide = "constexpr int kBuildWide = 1024;"
             assert wide in text, "kBuildWide not found"
             text = text.replace(wide, "constexpr int kBuildWide = 256;")
+            # the top levels' multi-workgroup pre-passes run from 64 primitives
+            # on, so that the host run's small scenes take them too
+            pre = "constexpr uint32_t kPreMinPrims = 1u << 16;"
+            assert pre in text, "kPreMinPrims not found"
+            text = text.replace(pre, "constexpr uint32_t kPreMinPrims = 1u << 6;")
         (src_dir / f.name).write_text(text)
         if f.suffix in (".hip", ".cpp"):
             units.append(src_dir / f.name)

@@ -27,6 +27,13 @@ def degenerate_scene(kind):
         for a in range(3):
             sel = rng.random(len(s.positions)) < 0.3
             s.positions[sel, a] = np.where(rng.random(sel.sum()) < 0.5, -0.0, 0.0).astype(np.float32)
+    elif kind == "ties_large":  # past the top levels' pre-pass threshold: equal extents and +-0 everywhere
+        s = scenes.random_soup(70000, seed=13)
+        rng = np.random.default_rng(5)
+        s.positions[:] = np.round(s.positions * 4.0) / 4.0
+        for a in range(3):
+            sel = rng.random(len(s.positions)) < 0.2
+            s.positions[sel, a] = np.where(rng.random(sel.sum()) < 0.5, -0.0, 0.0).astype(np.float32)
     return s
 
 
@@ -41,6 +48,7 @@ SCENES = {
     "collinear": lambda: degenerate_scene("collinear"),
     "signed_zero": lambda: degenerate_scene("signed_zero"),
     "signed_zero_large": lambda: degenerate_scene("signed_zero_large"),
+    "ties_large": lambda: degenerate_scene("ties_large"),
     "cornell": scenes.cornell,
 }
 
