@@ -390,6 +390,45 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
     float split[10];
 #pragma unroll
     for (int i = 0; i < 10; i++) split[i] = (hi - lo) / 10.0f * (float)i + lo;
+    // wave reduction, then across the waves through LDS
+    __shared__ int w_c[NT / 64][NBIN];
+    __shared__ float w_mn[NT / 64][NBIN][3], w_mx[NT / 64][NBIN][3];
+    const int lane = tid & 63, wv = tid >> 6;
+    if constexpr (NT == 64) {
+        // one wavefront per node (the deep levels: few elements each): LDS
+        // atomics on order keys keep the bins out of registers
+        __shared__ int l_c[NBIN];
+        __shared__ uint32_t l_mn[NBIN][3], l_mx[NBIN][3];
+        if (tid < NBIN) {
+            l_c[tid] = 0;
+            for (int a = 0; a < 3; a++) {
+                l_mn[tid][a] = 0xffffffffu;
+                l_mx[tid][a] = 0u;
+            }
+        }
+        __syncthreads();
+        for (int j = pre ? n : tid; j < n; j += NT) {
+            float4 mn = bin_.mn[sg.start + j], mx = bin_.mx[sg.start + j];
+            float c = (comp(mn, axis) + comp(mx, axis)) / 2.0f;
+            int bin = 0;
+#pragma unroll
+            for (int i = 0; i < 10; i++) bin += (c < split[i]) ? 0 : 1;
+            atomicAdd(&l_c[bin], 1);
+            const float pa[3] = {mn.x, mn.y, mn.z}, pb[3] = {mx.x, mx.y, mx.z};
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&l_mn[bin][a], okey(pa[a]));
+                atomicMax(&l_mx[bin][a], okey(pb[a]));
+            }
+        }
+        __syncthreads();
+        if (tid < NBIN) {  // an empty bin keeps +-FLT_MAX
+            w_c[0][tid] = l_c[tid];
+            for (int a = 0; a < 3; a++) {
+                w_mn[0][tid][a] = l_c[tid] ? ounkey(l_mn[tid][a]) : FLT_MAX;
+                w_mx[0][tid][a] = l_c[tid] ? ounkey(l_mx[tid][a]) : -FLT_MAX;
+            }
+        }
+    } else {
     int bc[NBIN];
     float bmn[NBIN][3], bmx[NBIN][3];
 #pragma unroll
@@ -415,10 +454,6 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
             }
         }
     }
-    // wave reduction, then across the 4 waves through LDS
-    __shared__ int w_c[NT / 64][NBIN];
-    __shared__ float w_mn[NT / 64][NBIN][3], w_mx[NT / 64][NBIN][3];
-    const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
     for (int b = 0; b < NBIN; b++) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -436,71 +471,89 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
             }
         }
     }
+    }  // pass 2
     __shared__ int s_nl, s_fallback;
     __shared__ float s_split;
     __syncthreads();
-    if (tid == 0) {
-        int cnt[NBIN];
-        float mn[NBIN][3], mx[NBIN][3];
-        for (int b = 0; b < NBIN; b++) {
-            cnt[b] = 0;
-            for (int a = 0; a < 3; a++) {
-                mn[b][a] = FLT_MAX;
-                mx[b][a] = -FLT_MAX;
-            }
+    if (tid < 64) {
+        // wave 0: lane b holds bin b (merged over the waves), a scan over the
+        // lanes gives every plane its two sides, lane i prices plane i.  The
+        // counts are exact; min / max in the scan's order can differ from
+        // the sequential one only in the sign of a zero, which no surface
+        // area (differences of the bounds) and no cost (1 + ...) keeps.
+        int c = 0;
+        float bm[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (lane < NBIN) {
             for (int w = 0; w < NT / 64; w++) {
-                cnt[b] += w_c[w][b];
+                c += w_c[w][lane];
                 for (int a = 0; a < 3; a++) {
-                    mn[b][a] = fminf(mn[b][a], w_mn[w][b][a]);
-                    mx[b][a] = fmaxf(mx[b][a], w_mx[w][b][a]);
+                    bm[a] = fminf(bm[a], w_mn[w][lane][a]);
+                    bx[a] = fmaxf(bx[a], w_mx[w][lane][a]);
                 }
             }
             if (pre) {  // the bins of the pre-pass (an empty bin keeps +-FLT_MAX)
-                cnt[b] = pre[blockIdx.x].cnt[b];
-                if (cnt[b])
+                c = pre[blockIdx.x].cnt[lane];
+                if (c)
                     for (int a = 0; a < 3; a++) {
-                        mn[b][a] = ounkey(pre[blockIdx.x].bmn[b][a]);
-                        mx[b][a] = ounkey(pre[blockIdx.x].bmx[b][a]);
+                        bm[a] = ounkey(pre[blockIdx.x].bmn[lane][a]);
+                        bx[a] = ounkey(pre[blockIdx.x].bmx[lane][a]);
                     }
             }
         }
-        float sa_total = surface_area(nmn.x, nmn.y, nmn.z, nmx.x, nmx.y, nmx.z);
-        float min_cost = FLT_MAX, min_split = 0.0f;
-        int best_nl = 0;
-        for (int i = 0; i < 10; i++) {
-            int nl = 0, nr = 0;
-            float lmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            float rmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, rmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            for (int b = 0; b < NBIN; b++) {
-                if (b <= i) {
-                    nl += cnt[b];
-                    for (int a = 0; a < 3; a++) {
-                        lmn[a] = fminf(lmn[a], mn[b][a]);
-                        lmx[a] = fmaxf(lmx[a], mx[b][a]);
-                    }
-                } else {
-                    nr += cnt[b];
-                    for (int a = 0; a < 3; a++) {
-                        rmn[a] = fminf(rmn[a], mn[b][a]);
-                        rmx[a] = fmaxf(rmx[a], mx[b][a]);
-                    }
+        int lc = c, rc = c;  // bins 0..lane, bins lane..NBIN-1
+        float lm[3], lx[3], rm[3], rx[3];
+        for (int a = 0; a < 3; a++) {
+            lm[a] = rm[a] = bm[a];
+            lx[a] = rx[a] = bx[a];
+        }
+        for (int d = 1; d < 16; d <<= 1) {
+            const int tl = __shfl_up(lc, d, 64), tr = __shfl_down(rc, d, 64);
+            if (lane >= d) lc += tl;
+            if (lane + d < 64) rc += tr;
+            for (int a = 0; a < 3; a++) {
+                const float ulm = __shfl_up(lm[a], d, 64), ulx = __shfl_up(lx[a], d, 64);
+                const float drm = __shfl_down(rm[a], d, 64), drx = __shfl_down(rx[a], d, 64);
+                if (lane >= d) {
+                    lm[a] = fminf(lm[a], ulm);
+                    lx[a] = fmaxf(lx[a], ulx);
+                }
+                if (lane + d < 64) {
+                    rm[a] = fminf(rm[a], drm);
+                    rx[a] = fmaxf(rx[a], drx);
                 }
             }
-            float cost;
-            if (nl == 0 || nr == 0) cost = FLT_MAX;
-            else
-                cost = 1.0f + ((float)nl * surface_area(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2]) +
-                               (float)nr * surface_area(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2])) /
-                                  sa_total;
-            if (cost < min_cost) {
-                min_cost = cost;
-                min_split = split[i];
-                best_nl = nl;
+        }
+        // the right side of plane i: bins i+1..NBIN-1
+        const int nl = lc, nr = __shfl_down(rc, 1, 64);
+        float q[6];
+        for (int a = 0; a < 3; a++) {
+            q[a] = __shfl_down(rm[a], 1, 64);
+            q[3 + a] = __shfl_down(rx[a], 1, 64);
+        }
+        const float sa_total = surface_area(nmn.x, nmn.y, nmn.z, nmx.x, nmx.y, nmx.z);
+        float cost = FLT_MAX;
+        if (lane < 10 && nl != 0 && nr != 0)
+            cost = 1.0f + ((float)nl * surface_area(lm[0], lm[1], lm[2], lx[0], lx[1], lx[2]) +
+                           (float)nr * surface_area(q[0], q[1], q[2], q[3], q[4], q[5])) /
+                              sa_total;
+        // the planes in order, a later one only if strictly cheaper
+        float min_cost = FLT_MAX, min_split = 0.0f;
+        int best_nl = 0;
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            const float ck = __shfl(cost, k, 64);
+            const int nk = __shfl(nl, k, 64);
+            if (ck < min_cost) {
+                min_cost = ck;
+                min_split = split[k];
+                best_nl = nk;
             }
         }
-        s_fallback = (min_cost == FLT_MAX);
-        s_split = min_split;
-        s_nl = s_fallback ? (n + 1) / 2 : best_nl;
+        if (tid == 0) {
+            s_fallback = (min_cost == FLT_MAX);
+            s_split = min_split;
+            s_nl = s_fallback ? (n + 1) / 2 : best_nl;
+        }
     }
     __syncthreads();
     const bool fallback = s_fallback;

@@ -135,6 +135,13 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return krt_emu_from_bits<T>(krt_emu::wave_exchange(krt_emu_to_bits(v), s));
 }
 template <class T>
+inline T __shfl_up(T v, unsigned delta, int width = 64) {
+    const int lane = (int)(threadIdx.x & 63u);
+    int s = lane - (int)delta;
+    if (s < (lane & ~(width - 1))) s = lane;
+    return krt_emu_from_bits<T>(krt_emu::wave_exchange(krt_emu_to_bits(v), s));
+}
+template <class T>
 inline T __shfl_down(T v, unsigned delta, int width = 64) {
     const int lane = (int)(threadIdx.x & 63u);
     int s = lane + (int)delta;
