@@ -611,11 +611,14 @@ __global__ __launch_bounds__(NT) void level_kernel(const Seg *segs, const int *p
         Seg L{sg.start, nl, p + 1, 3 * (p + 2 * nl), sg.rank};
         Seg R{sg.start + nl, nr, p + 2 * nl, sg.miss, sg.rank + nl};
         const Seg ch[2] = {L, R};
+        // one reservation for both pending children (the deep levels have
+        // ~10^5 nodes on this counter)
+        int slot = (L.count >= 3) + (R.count >= 3);
+        if (slot) slot = atomicAdd(next_count, slot);
         for (int k = 0; k < 2; k++) {
             const Seg &c = ch[k];
             if (c.count >= 3) {
-                int slot = atomicAdd(next_count, 1);
-                next[slot] = c;
+                next[slot++] = c;
             } else if (c.count == 2) {
                 emit_pair(o, c.pre, c.miss, c.rank, perm_out[c.start], perm_out[c.start + 1], bmin, bmax);
             } else {
@@ -1027,14 +1030,14 @@ __global__ __launch_bounds__(NT) void sah_level_kernel(const SahSeg *segs, const
             for (int j = tid; j < cnt[c]; j += NT) perm_final[st[c] + j] = perm_out[st[c] + j];
     if (tid == 0) {
         int link[2];
+        const int inner = (cnt[0] > leaf_max) + (cnt[1] > leaf_max);  // one reservation for both
+        int id = inner ? atomicAdd(node_ctr, inner) : 0, slot = inner ? atomicAdd(next_count, inner) : 0;
         for (int c = 0; c < 2; c++) {
             if (cnt[c] <= leaf_max) {
                 link[c] = leaf_link(st[c], cnt[c]);
             } else {
-                int id = atomicAdd(node_ctr, 1);
                 link[c] = id;
-                int slot = atomicAdd(next_count, 1);
-                next[slot] = SahSeg{st[c], cnt[c], id};
+                next[slot++] = SahSeg{st[c], cnt[c], id++};
             }
         }
         float mn[2][3], mx[2][3];
